@@ -1,0 +1,22 @@
+"""detray_amd: MI355X-native track propagation (detray's propagate hot path)
+behind a C ABI, with a thin Python mirror of the reference interface."""
+
+from .capi import DetrayError, load  # noqa: F401
+from .propagator import (  # noqa: F401
+    GEV,
+    MEV,
+    MM,
+    TESLA,
+    UM,
+    Detector,
+    MagneticField,
+    PropagationConfig,
+    PropagationResult,
+    Propagator,
+    generate_tracks,
+)
+
+__all__ = [
+    "DetrayError", "load", "Detector", "MagneticField", "PropagationConfig",
+    "PropagationResult", "Propagator", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
+]

@@ -1,0 +1,81 @@
+"""In-tree build of the detray_amd shared library (HIP, gfx950) and of the
+oracle used by the tests.
+
+``build_library()`` compiles ``libdetray_amd.so`` with hipcc for gfx950:
+the C ABI, the device kernels and the host-side detector emitters.
+``build_oracle()`` compiles the FP64 CPU restatement into
+``oracle/liboracle.so`` with g++ (test infrastructure).
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+LIB = PKG / "libdetray_amd.so"
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+ARCH = os.environ.get("DETRAY_AMD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # keep FP64 arithmetic in source order (no FMA contraction) so the
+    # device path rounds like the oracle
+    "-ffp-contract=off",
+    # packed FP32 instructions disabled
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+
+
+def build_library(out: Path = LIB) -> Path:
+    srcs = [CSRC / "capi.hip", CSRC / "host" / "builders.cpp"]
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [_hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, srcs), "-o", str(tmp)]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle(out: Path = ORACLE_LIB) -> Path:
+    cxx = os.environ.get("CXX", "g++")
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+           f"-I{INCLUDE}", str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp)]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_library()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print(f"built {LIB} and {ORACLE_LIB}")

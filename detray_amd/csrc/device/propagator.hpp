@@ -1,0 +1,1119 @@
+// Device implementation of detray's propagation hot path for one track per
+// work-item: rk_stepper + caching_navigator + actor chain
+//   pathlimit_aborter, parameter_updater<pointwise_material_interactor>,
+//   surface_sequencer
+// (core/include/detray/propagator/propagator.hpp:262-400).
+//
+// Layout choices for CDNA (64-wide waves):
+//  * track inputs/outputs are structure-of-arrays so that a wave's loads and
+//    stores of one parameter are contiguous;
+//  * the transport Jacobian is kept in the reduced 25-value form of
+//    codegen/transport_jacobian.hpp:18-325 (fixed identity/zero blocks are
+//    implicit) and the covariance as its 21 independent values;
+//  * the 8-slot candidate cache (caching_navigator.hpp:29) is one small array
+//    per work-item.
+// The detector tables are read-only and stay L2-resident.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "detray_amd.h"
+
+namespace dtr {
+
+#define DTR_HD __host__ __device__ inline
+
+constexpr double kMax = 1.7976931348623157e308;
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr uint16_t kEndOfWorld = 0xffffu;
+constexpr int kCache = 8;
+
+struct DevDetector {
+    const dtr_volume* volumes;
+    const dtr_surface* surfaces;
+    const dtr_transform* transforms;
+    const dtr_mask* masks;
+    const dtr_material_slab* materials;
+    const dtr_grid* grids;
+    const uint32_t* bf;
+    const uint32_t* bins;
+};
+
+struct DevField {
+    int32_t kind;
+    uint32_t n[3];
+    double b[3];
+    double min[3];
+    double pitch[3];
+    const float* values;
+};
+
+// Propagation config as doubles where the reference converts float config
+// values to the scalar type at the point of use.
+struct DevConfig {
+    double min_stepsize, rk_error_tol, step_constraint, path_limit;
+    uint32_t max_rk_updates;
+    float min_mask_tol, max_mask_tol, mask_scalor, path_tol, overstep_tol;
+    uint32_t win0, win1;
+    double accumulated_error;
+    double n_stddev;
+    uint32_t max_surfaces;
+    uint8_t do_cov, estimate_noise, do_eloss, do_ms, use_material;
+};
+
+struct DevResults {
+    double* bound_params;
+    double* bound_cov;
+    double* free_params;
+    double* path_length;
+    uint32_t* surfaces;
+    uint32_t* n_surfaces;
+    uint32_t* n_steps;
+    uint32_t* n_trials;
+    uint8_t* status;
+};
+
+// ---------------------------------------------------------------- algebra --
+struct V3 {
+    double x, y, z;
+};
+DTR_HD V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+DTR_HD V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+DTR_HD V3 mul(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+DTR_HD double dot(V3 a, V3 b) {
+    double r = 0.0;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+DTR_HD V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+DTR_HD double norm(V3 a) { return sqrt(dot(a, a)); }
+DTR_HD double perp(V3 a) { return sqrt(fma(a.x, a.x, a.y * a.y)); }
+DTR_HD double phi_of(V3 a) { return atan2(a.y, a.x); }
+DTR_HD double theta_of(V3 a) { return atan2(perp(a), a.z); }
+DTR_HD double dmin(double a, double b) { return a < b ? a : b; }
+DTR_HD double dmax(double a, double b) { return a > b ? a : b; }
+
+DTR_HD V3 trf_axis(const dtr_transform& t, int c) { return {t.r[c], t.r[3 + c], t.r[6 + c]}; }
+DTR_HD V3 trf_translation(const dtr_transform& t) { return {t.t[0], t.t[1], t.t[2]}; }
+DTR_HD V3 to_local(const dtr_transform& t, V3 p) {
+    const V3 d = sub(p, trf_translation(t));
+    return {t.r[0] * d.x + t.r[3] * d.y + t.r[6] * d.z,
+            t.r[1] * d.x + t.r[4] * d.y + t.r[7] * d.z,
+            t.r[2] * d.x + t.r[5] * d.y + t.r[8] * d.z};
+}
+DTR_HD V3 to_global(const dtr_transform& t, V3 l) {
+    return {t.r[0] * l.x + t.r[1] * l.y + t.r[2] * l.z + t.t[0],
+            t.r[3] * l.x + t.r[4] * l.y + t.r[5] * l.z + t.t[1],
+            t.r[6] * l.x + t.r[7] * l.y + t.r[8] * l.z + t.t[2]};
+}
+
+// ------------------------------------------------------------------ field --
+DTR_HD V3 field_at(const DevField& f, V3 p) {
+    if (f.kind == DTR_FIELD_CONSTANT) return {f.b[0], f.b[1], f.b[2]};
+    const double pc[3] = {p.x, p.y, p.z};
+    int i0[3];
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+        const int n = static_cast<int>(f.n[a]);
+        double u = (pc[a] - f.min[a]) / f.pitch[a];
+        u = u < 0.0 ? 0.0 : (u > n - 1 ? static_cast<double>(n - 1) : u);
+        int i = static_cast<int>(floor(u));
+        i = i > n - 2 ? n - 2 : i;
+        i0[a] = i;
+        t[a] = u - i;
+    }
+    const uint64_t ny = f.n[1], nz = f.n[2];
+    const uint64_t base = (static_cast<uint64_t>(i0[0]) * ny + i0[1]) * nz + i0[2];
+    const uint64_t sx = ny * nz, sy = nz;
+    const float* v000 = f.values + 3 * base;
+    double out[3];
+    for (int c = 0; c < 3; ++c) {
+        const double a000 = v000[c], a100 = v000[3 * sx + c];
+        const double a001 = v000[3 + c], a101 = v000[3 * sx + 3 + c];
+        const double a010 = v000[3 * sy + c], a110 = v000[3 * (sx + sy) + c];
+        const double a011 = v000[3 * sy + 3 + c], a111 = v000[3 * (sx + sy) + 3 + c];
+        const double c00 = a000 * (1.0 - t[0]) + a100 * t[0];
+        const double c01 = a001 * (1.0 - t[0]) + a101 * t[0];
+        const double c10 = a010 * (1.0 - t[0]) + a110 * t[0];
+        const double c11 = a011 * (1.0 - t[0]) + a111 * t[0];
+        const double c0 = c00 * (1.0 - t[1]) + c10 * t[1];
+        const double c1 = c01 * (1.0 - t[1]) + c11 * t[1];
+        out[c] = c0 * (1.0 - t[2]) + c1 * t[2];
+    }
+    return {out[0], out[1], out[2]};
+}
+
+// ------------------------------------------------------------ navigation --
+enum : uint8_t { kInside = 0, kEdge = 1, kOutside = 2 };
+enum : int8_t { kAbort = -3, kExit = -2, kUnknown = -1, kTowards = 0, kOnObject = 1, kOnPortal = 2 };
+enum : uint8_t { kNoTrust = 0, kFair = 1, kHigh = 3, kFull = 4 };
+
+struct Cand {
+    double path;
+    uint32_t sf;
+    uint16_t link;
+    uint8_t status;
+    uint8_t pad;
+};
+
+struct NavTol {
+    float min_mask_tol, max_mask_tol, mask_scalor, path_tol, overstep_tol;
+};
+
+struct Track {
+    // navigation state (navigation_state.hpp:636-667)
+    Cand c[kCache];
+    double ext_tol;
+    int8_t next, last, status;
+    uint8_t trust;
+    uint16_t volume;
+    bool path_limit_hit;
+    bool overflow;
+    // stepper state (base_stepper.hpp:63-292, rk_stepper.hpp:86-263)
+    V3 pos, dir;
+    double time, qop;
+    double J[25];  // reduced transport Jacobian
+    double step_size, next_step_size, path_length, abs_path_length;
+    double constraint_acc, constraint_abort;
+    V3 dtds3;
+    double mass, charge;
+    uint32_t n_trials, n_steps;
+    // parameter_updater_state
+    double bound[6];
+    double cov[21];
+    uint32_t up_sf;
+    uint32_t n_seq;
+    bool heartbeat;
+};
+
+// Reduced Jacobian index helpers: rows 0-2 and 4-6 over columns 4..7, (7,7)
+DTR_HD int jpos(int r, int c) { return r * 4 + (c - 4); }     // r in 0..2
+DTR_HD int jdir(int r, int c) { return 12 + r * 4 + (c - 4); }  // r in 0..2 (rows 4..6)
+constexpr int kJqq = 24;
+
+DTR_HD int covi(int r, int c) {
+    // upper-triangle row-major index of (r, c), r <= c
+    if (r > c) {
+        const int t = r;
+        r = c;
+        c = t;
+    }
+    return r * 6 - (r * (r - 1)) / 2 + (c - r);
+}
+
+struct Engine {
+    const DevDetector& det;
+    const DevField& field;
+    const DevConfig& cfg;
+    Track& tk;
+
+    DTR_HD bool on_surface() const { return tk.status == kOnObject || tk.status == kOnPortal; }
+    DTR_HD int n_cand() const { return tk.last - tk.next + 1; }
+    DTR_HD bool exhausted() const { return n_cand() == 0; }
+    DTR_HD int begin_idx() const { return (on_surface() && tk.next >= 1) ? tk.next - 1 : tk.next; }
+    DTR_HD double nav_dist() const { return tk.c[tk.next].path; }
+    DTR_HD void set_high_trust() { tk.trust = tk.trust < kHigh ? tk.trust : kHigh; }
+    DTR_HD void set_fair_trust() { tk.trust = tk.trust < kFair ? tk.trust : kFair; }
+    DTR_HD bool is_portal(uint32_t s) const { return det.surfaces[s].sf_id == DTR_PORTAL; }
+
+    // resolve_mask (intersector_base.hpp:171-252)
+    DTR_HD void resolve_mask(Cand& is, double path, V3 glob, double z2d, uint32_t sf_idx,
+                             const dtr_surface& sf, const dtr_mask& m, const dtr_transform& t,
+                             const NavTol& tol) const {
+        is.status = kOutside;
+        if (!(path >= static_cast<double>(tol.overstep_tol))) return;
+        double base_tol = 0.0, ext_tol = 0.0;
+        if (sf.sf_id != DTR_PORTAL || tol.min_mask_tol == 3.40282347e+38f) {
+            ext_tol = tk.ext_tol;
+            base_tol = dmax(static_cast<double>(tol.min_mask_tol),
+                            dmin(static_cast<double>(tol.max_mask_tol),
+                                 static_cast<double>(tol.mask_scalor) * fabs(path)));
+        }
+        bool inside, edge;
+        if (m.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            inside = (m.v[1] - base_tol <= z2d && z2d <= m.v[2] + base_tol);
+            edge = inside;
+        } else {
+            const V3 loc = to_local(t, glob);
+            if (m.shape == DTR_RECTANGLE2D) {
+                const double l0 = fabs(loc.x), l1 = fabs(loc.y);
+                inside = (l0 <= (m.v[0] + base_tol) && l1 <= (m.v[1] + base_tol));
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = (l0 <= (m.v[0] + ft) && l1 <= (m.v[1] + ft));
+                }
+            } else if (m.shape == DTR_TRAPEZOID2D) {
+                const double rel_y = (m.v[2] + loc.y) * m.v[3];
+                const double bound_x = m.v[0] + rel_y * (m.v[1] - m.v[0]);
+                inside = (fabs(loc.x) <= (bound_x + base_tol) && fabs(loc.y) <= (m.v[2] + base_tol));
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (m.v[2] + ft));
+                }
+            } else {
+                const double r = perp(loc);
+                inside = ((r + base_tol) >= m.v[0] && r <= (m.v[1] + base_tol));
+                edge = inside;
+            }
+        }
+        if (edge) is.status = kEdge;
+        if (inside) is.status = kInside;
+        is.path = path;
+        is.sf = sf_idx;
+        is.link = m.volume_link;
+    }
+
+    // ray_plane_intersector.hpp:60-87
+    static DTR_HD bool plane_intersect(V3 ro, V3 rd, const dtr_transform& t, double& s, V3& glob) {
+        const V3 sn = trf_axis(t, 2);
+        const V3 st = trf_translation(t);
+        const double denom = dot(rd, sn);
+        if (denom == 0.0) {
+            s = kMax;
+            glob = {0.0, 0.0, 0.0};
+            return false;
+        }
+        s = dot(sn, sub(st, ro)) / denom;
+        glob = add(ro, mul(s, rd));
+        return fabs(s) < 10.0 * 1000.0;
+    }
+
+    // ray_cylinder_portal_intersector.hpp:68-132
+    static DTR_HD bool cyl_intersect(V3 ro, V3 rd, double r, double overstep, double& path, double& z) {
+        path = kMax;
+        const double rd_perp_2 = rd.x * rd.x + rd.y * rd.y;
+        if (rd_perp_2 < 2.220446049250313e-16) {
+            z = 0.0;
+            return false;
+        }
+        const double ro_perp_2 = ro.x * ro.x + ro.y * ro.y;
+        const double rad_diff = r * r - ro_perp_2;
+        if (fabs(rad_diff) > 1.0 * 1e-3) {
+            const double inv2 = 1.0 / rd_perp_2;
+            const double k = -inv2 * (ro.x * rd.x + ro.y * rd.y);
+            const double discr = inv2 * rad_diff + k * k;
+            if (discr < 0.0) {
+                z = 0.0;
+                return false;
+            }
+            const double sq = sqrt(discr);
+            const double s1 = k + sq, s2 = k - sq;
+            path = (fabs(s1) < fabs(s2)) ? s1 : s2;
+            if (path < overstep && sq > 0.0) path = (fabs(s1) >= fabs(s2)) ? s1 : s2;
+        } else {
+            path = rad_diff;
+        }
+        if (path < overstep) path = kMax;
+        z = ro.z + path * rd.z;
+        return fabs(path) < 10.0 * 1000.0;
+    }
+
+    DTR_HD bool intersect(uint32_t s, V3 ro, V3 rd, double overstep, double& path, V3& glob,
+                          double& z) const {
+        const dtr_surface& sf = det.surfaces[s];
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
+            return cyl_intersect(ro, rd, det.masks[sf.mask].v[0], overstep, path, z);
+        return plane_intersect(ro, rd, det.transforms[sf.transform], path, glob);
+    }
+
+    // caching_navigator.hpp:149-229
+    DTR_HD void insert(int pos, const Cand& nc) {
+        if (pos == kCache) return;
+        if (n_cand() == 0) {
+            tk.c[0] = nc;
+            tk.last = static_cast<int8_t>(tk.last + 1);
+            return;
+        }
+        int idx = pos;
+        const uint64_t nbar = det.surfaces[nc.sf].barcode;
+        auto overlap = [&](int i) { return fabs(tk.c[i].path - nc.path) <= 1.0 * 1e-3; };
+        auto clash = [&](int i) {
+            const uint64_t a = tk.c[i].sf == kInvalid ? ~0ull : det.surfaces[tk.c[i].sf].barcode;
+            return a == nbar && overlap(i);
+        };
+        if (clash(idx) || (idx > 0 && clash(idx - 1))) return;
+        const int shift_begin = tk.last < kCache - 2 ? tk.last : kCache - 2;
+        if (overlap(idx) && (nc.status == kEdge || is_portal(nc.sf))) {
+            idx++;
+            idx = idx > shift_begin ? shift_begin : idx;
+        }
+        for (int i = shift_begin; i >= idx; --i) tk.c[i + 1] = tk.c[i];
+        tk.c[idx] = nc;
+        tk.last = static_cast<int8_t>(tk.last + 1 < kCache - 1 ? tk.last + 1 : kCache - 1);
+    }
+
+    // navigation_functions.hpp:44-68 + intersection_kernel.hpp:24-168
+    DTR_HD void candidate_search(uint32_t s, const NavTol& tol) {
+        const V3 rd = tk.dir;  // forward navigation
+        double path, z;
+        V3 glob;
+        if (!intersect(s, tk.pos, rd, tol.overstep_tol, path, glob, z)) return;
+        const dtr_surface& sf = det.surfaces[s];
+        const dtr_transform& t = det.transforms[sf.transform];
+        for (uint32_t k = 0; k < sf.n_masks; ++k) {
+            Cand is;
+            is.path = kMax;
+            is.sf = kInvalid;
+            is.link = 0;
+            is.pad = 0;
+            resolve_mask(is, path, glob, z, s, sf, det.masks[sf.mask + k], t, tol);
+            if (is.status <= kEdge) {
+                int p = begin_idx();
+                const int e = tk.last + 1;
+                while (p < e && !(fabs(is.path) < fabs(tk.c[p].path))) ++p;
+                insert(p, is);
+                return;
+            }
+        }
+    }
+
+    // navigation_functions.hpp:99-129 + intersection_kernel.hpp:172-272
+    DTR_HD bool update_candidate(Cand& cand) const {
+        if (cand.sf == kInvalid) return false;
+        double path, z;
+        V3 glob;
+        const NavTol tol{cfg.min_mask_tol, cfg.max_mask_tol, cfg.mask_scalor, cfg.path_tol,
+                         cfg.overstep_tol};
+        if (!intersect(cand.sf, tk.pos, tk.dir, tol.overstep_tol, path, glob, z)) return false;
+        const dtr_surface& sf = det.surfaces[cand.sf];
+        const dtr_transform& t = det.transforms[sf.transform];
+        for (uint32_t k = 0; k < sf.n_masks; ++k) {
+            resolve_mask(cand, path, glob, z, cand.sf, sf, det.masks[sf.mask + k], t, tol);
+            if (cand.status <= kEdge) return true;
+        }
+        return false;
+    }
+
+    // spatial_grid.hpp:100-202, axis_binning.hpp:80-115, axis_bounds.hpp
+    DTR_HD void grid_search(const dtr_volume& vol, const NavTol& tol) {
+        const dtr_grid& g = det.grids[vol.grid];
+        double l0, l1;
+        if (g.type == DTR_GRID_CYLINDER) {
+            double path, z;
+            cyl_intersect(tk.pos, tk.dir, g.radius, -kMax, path, z);
+            l0 = phi_of(add(tk.pos, mul(path, tk.dir)));
+            l1 = z;
+        } else {
+            const dtr_transform& vt = det.transforms[vol.transform];
+            double s;
+            V3 glob;
+            plane_intersect(tk.pos, tk.dir, vt, s, glob);
+            const V3 loc = to_local(vt, glob);
+            l0 = perp(loc);
+            l1 = phi_of(loc);
+        }
+        const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
+        const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
+        const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
+        const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
+        const int lb0 = ib0 - static_cast<int>(cfg.win0), ub0 = ib0 + static_cast<int>(cfg.win1);
+        const int lb1 = ib1 - static_cast<int>(cfg.win0), ub1 = ib1 + static_cast<int>(cfg.win1);
+        const bool cyl = (g.type == DTR_GRID_CYLINDER);
+        const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
+        int a0, e0, a1, e1;
+        if (cyl) {
+            a0 = lb0;
+            e0 = ub0 + 1;
+            int mn = lb1 > 0 ? lb1 : 0;
+            mn = mn >= n1 ? n1 - 1 : mn;
+            int mx = ub1 >= n1 ? n1 - 1 : ub1;
+            mx = mx < 0 ? 0 : mx;
+            a1 = mn;
+            e1 = mx + 1;
+        } else {
+            int mn = lb0 > 0 ? lb0 : 0;
+            mn = mn >= n0 ? n0 - 1 : mn;
+            int mx = ub0 >= n0 ? n0 - 1 : ub0;
+            mx = mx < 0 ? 0 : mx;
+            a0 = mn;
+            e0 = mx + 1;
+            a1 = lb1;
+            e1 = ub1 + 1;
+        }
+        for (int b0 = a0; b0 < e0; ++b0) {
+            const int i0 = cyl ? (n0 + (b0 % n0)) % n0 : b0;
+            for (int b1 = a1; b1 < e1; ++b1) {
+                const int i1 = cyl ? b1 : (n1 + (b1 % n1)) % n1;
+                const uint32_t s = det.bins[g.bins + static_cast<uint32_t>(i0) * g.n1 + i1];
+                if (s != kInvalid) candidate_search(s, tol);
+            }
+        }
+    }
+
+    // navigation_functions.hpp:147-177
+    DTR_HD void update_status() {
+        if (!exhausted() && fabs(tk.c[tk.next].path) < static_cast<double>(cfg.path_tol)) {
+            tk.status = is_portal(tk.c[tk.next].sf) ? kOnPortal : kOnObject;
+            tk.next = static_cast<int8_t>(tk.next + 1);
+        } else {
+            tk.status = kTowards;
+        }
+        tk.trust = (exhausted() || tk.status == kOnPortal) ? kNoTrust : kFull;
+    }
+
+    // navigation_functions.hpp:193-246
+    DTR_HD void local_navigation(float overstep_tol, bool resolve_overstepping) {
+        const dtr_volume& vol = det.volumes[tk.volume];
+        for (int i = 0; i < kCache; ++i) tk.c[i].path = kMax;
+        tk.next = 0;
+        tk.last = -1;
+        const NavTol tol{cfg.min_mask_tol, cfg.max_mask_tol, cfg.mask_scalor, cfg.path_tol,
+                         resolve_overstepping ? overstep_tol : -cfg.path_tol};
+        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf[k], tol);
+        if (vol.grid >= 0) grid_search(vol, tol);
+        update_status();
+        if (tk.trust != kFull && tk.status == kOnPortal)
+            tk.trust = (tk.c[tk.next - 1].link == kEndOfWorld) ? kFull : kNoTrust;
+    }
+
+    DTR_HD void volume_switch() {
+        const uint16_t link = tk.c[tk.next - 1].link;
+        if (link == kEndOfWorld) {
+            tk.status = kExit;
+            return;
+        }
+        if (link != tk.volume) tk.trust = kNoTrust;
+        tk.volume = link;
+        local_navigation(cfg.overstep_tol, true);
+        tk.trust = kFull;
+    }
+
+    DTR_HD void init_loose_cfg() {
+        const float a = 100.f * cfg.overstep_tol, b = -10.f * cfg.max_mask_tol;
+        local_navigation(a < b ? a : b, true);
+        if (tk.trust != kFull || exhausted()) tk.status = kAbort;
+    }
+
+    // caching_navigator.hpp:250-388
+    DTR_HD bool update_impl() {
+        if (tk.trust == kHigh) {
+            if (!update_candidate(tk.c[tk.next])) {
+                tk.status = kUnknown;
+                set_fair_trust();
+            } else {
+                update_status();
+                if (tk.status == kTowards || tk.status == kOnPortal) return false;
+                if (tk.trust == kFull && update_candidate(tk.c[tk.next])) return false;
+                set_fair_trust();
+            }
+        }
+        if (tk.trust == kFair && !exhausted()) {
+            const int b = begin_idx(), e = tk.last + 1;
+            for (int i = b; i < e; ++i) {
+                if (!update_candidate(tk.c[i])) tk.c[i].path = kMax;
+            }
+            for (int i = b; i < e; ++i) {
+                for (int j = i + 1; j < e; ++j) {
+                    if (fabs(tk.c[j].path) < fabs(tk.c[i].path)) {
+                        const Cand tmp = tk.c[i];
+                        tk.c[i] = tk.c[j];
+                        tk.c[j] = tmp;
+                    }
+                }
+            }
+            tk.next = static_cast<int8_t>(b);
+            int inv = kCache;
+            for (int i = kCache - 1; i >= 0; --i) {
+                if (tk.c[i].path == kMax) inv = i;
+            }
+            tk.last = static_cast<int8_t>(inv - 1);
+            update_status();
+            if (exhausted()) tk.trust = kNoTrust;
+        }
+        if (tk.trust == kNoTrust) {
+            local_navigation(cfg.overstep_tol, true);
+            return true;
+        }
+        return false;
+    }
+
+    // navigator_base.hpp:97-154
+    DTR_HD bool nav_update() {
+        if (tk.trust == kFull) return false;
+        bool is_init = update_impl();
+        if (tk.status == kOnPortal) {
+            volume_switch();
+            is_init = true;
+            if (!(tk.status > kUnknown)) return false;
+        }
+        if (tk.trust != kFull || exhausted()) {
+            is_init = true;
+            init_loose_cfg();
+        }
+        return is_init;
+    }
+
+    // -------------------------------------------------------------- stepper --
+    // rk_stepper.hpp:273-530 (no volume material: q/p constant in a step)
+    DTR_HD void step(double dist, bool do_reset) {
+        if (fabs(dist) < static_cast<double>(1.f * 1e-3f)) {
+            if (fabs(tk.next_step_size) < cfg.min_stepsize)
+                tk.next_step_size = copysign(cfg.min_stepsize, dist);
+            return;
+        }
+        if (do_reset) {
+            tk.step_size = dist;
+        } else if (tk.next_step_size > 0) {
+            tk.step_size = dmin(tk.next_step_size, dist);
+        } else {
+            tk.step_size = dmax(tk.next_step_size, dist);
+        }
+        const double min_step = dmin(fabs(tk.step_size), cfg.min_stepsize);
+        const V3 pos = tk.pos;
+        const V3 dir = tk.dir;
+        const double qop = tk.qop;
+        const V3 b1 = field_at(field, pos);
+        const V3 k1 = mul(qop, cross(dir, b1));
+        V3 b2, b3, t2, t3, t4, k2, k3, k4;
+        double error = 1e20;
+        for (uint32_t i = 0; i < cfg.max_rk_updates; ++i) {
+            tk.n_trials++;
+            const double h = tk.step_size;
+            const double h2 = h * h;
+            const double half_h = h * 0.5;
+            const V3 pos1 = add(add(pos, mul(half_h, dir)), mul(h2 * 0.125, k1));
+            b2 = field_at(field, pos1);
+            t2 = (half_h == 0.0) ? dir : add(dir, mul(half_h, k1));
+            k2 = mul(qop, cross(t2, b2));
+            t3 = (half_h == 0.0) ? dir : add(dir, mul(half_h, k2));
+            k3 = mul(qop, cross(t3, b2));
+            const V3 pos2 = add(add(pos, mul(h, dir)), mul(h2 * 0.5, k3));
+            b3 = field_at(field, pos2);
+            t4 = (h == 0.0) ? dir : add(dir, mul(h, k3));
+            k4 = mul(qop, cross(t4, b3));
+            const V3 d = add(sub(sub(k1, k2), k3), k4);
+            const V3 err = mul((1.0 / 6.0) * h2, d);
+            error = dmax(norm(err), 1e-20);
+            if (error <= 4.0 * cfg.rk_error_tol) break;
+            tk.step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
+        }
+        tk.dtds3 = k4;
+        // constrained_step (accuracy, aborter)
+        const double cmin = dmin(tk.constraint_acc, tk.constraint_abort);
+        const double max_step = (tk.step_size >= 0.0 ? 1.0 : -1.0) * cmin;
+        if (fabs(tk.step_size) > fabs(max_step)) tk.step_size = max_step;
+        if (fabs(tk.step_size) < min_step) tk.step_size = copysign(min_step, tk.step_size);
+        tk.next_step_size =
+            tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
+        if (fabs(tk.next_step_size) < cfg.min_stepsize)
+            tk.next_step_size = copysign(cfg.min_stepsize, tk.next_step_size);
+        // advance_track (rk_stepper.hpp:594-629)
+        const double h = tk.step_size;
+        const double h_6 = h * (1.0 / 6.0);
+        tk.pos = add(pos, mul(h, add(dir, mul(h_6, add(add(k1, k2), k3)))));
+        const V3 nd = add(dir, mul(h_6, add(add(k1, mul(2.0, add(k2, k3))), k4)));
+        const double nn = norm(nd);
+        tk.dir = {nd.x / nn, nd.y / nn, nd.z / nn};
+        tk.path_length += h;
+        tk.abs_path_length += fabs(h);
+        if (cfg.do_cov) advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+        tk.n_steps++;
+    }
+
+    static DTR_HD void cwc(const double M[9], V3 b, double out[9]) {
+        for (int j = 0; j < 3; ++j) {
+            const V3 col{M[j], M[3 + j], M[6 + j]};
+            const V3 r = cross(col, b);
+            out[j] = r.x;
+            out[3 + j] = r.y;
+            out[6 + j] = r.z;
+        }
+    }
+
+    // rk_stepper.hpp:633-956 (dqop_n/dqop1 = 1, no field gradient) and
+    // codegen/update_rk_transport_jacobian.hpp:18-199 on the reduced storage
+    DTR_HD void advance_jacobian(V3 t1, V3 t2, V3 t3, V3 t4, V3 b1, V3 b2, V3 b3, double qop) {
+        const double h = tk.step_size;
+        const double half_h = h * 0.5;
+        const double h_6 = h * static_cast<double>(1.f / 6.f);
+        V3 dk0 = mul(1.0, cross(t1, b1));
+        V3 dk1 = add(mul(1.0, cross(t2, b2)), mul(qop * half_h, cross(dk0, b2)));
+        V3 dk2 = add(mul(1.0, cross(t3, b2)), mul(qop * half_h, cross(dk1, b2)));
+        V3 dk3 = add(mul(1.0, cross(t4, b3)), mul(qop * h, cross(dk2, b3)));
+        const V3 dFdqop = mul(h * h_6, add(add(dk0, dk1), dk2));
+        const V3 dGdqop = mul(h_6, add(add(dk0, mul(2.0, add(dk1, dk2))), dk3));
+        const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        double d0[9], d1[9], d2[9], d3[9], tmp[9];
+        cwc(I, b1, tmp);
+        for (int i = 0; i < 9; ++i) d0[i] = qop * tmp[i];
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * d0[i];
+        cwc(tmp, b2, d1);
+        for (int i = 0; i < 9; ++i) d1[i] *= qop;
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * d1[i];
+        cwc(tmp, b2, d2);
+        for (int i = 0; i < 9; ++i) d2[i] *= qop;
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + h * d2[i];
+        cwc(tmp, b3, d3);
+        for (int i = 0; i < 9; ++i) d3[i] *= qop;
+        double dFdt[9], dGdt[9];
+        for (int i = 0; i < 9; ++i) {
+            dFdt[i] = h * (I[i] + h_6 * (d0[i] + d1[i] + d2[i]));
+            dGdt[i] = I[i] + h_6 * (d0[i] + 2.0 * (d1[i] + d2[i]) + d3[i]);
+        }
+        // new_J = D * J on the structural non-zeros (columns 4..7)
+        double nj[25];
+        for (int c = 4; c < 8; ++c) {
+            const double jd0 = tk.J[jdir(0, c)], jd1 = tk.J[jdir(1, c)], jd2 = tk.J[jdir(2, c)];
+            const double jq = (c == 7) ? tk.J[kJqq] : 0.0;
+            for (int r = 0; r < 3; ++r) {
+                double s = 0.0;
+                s += tk.J[jpos(r, c)];
+                s += dFdt[r * 3 + 0] * jd0;
+                s += dFdt[r * 3 + 1] * jd1;
+                s += dFdt[r * 3 + 2] * jd2;
+                nj[jpos(r, c)] = s;
+                double g = 0.0;
+                g += dGdt[r * 3 + 0] * jd0;
+                g += dGdt[r * 3 + 1] * jd1;
+                g += dGdt[r * 3 + 2] * jd2;
+                nj[jdir(r, c)] = g;
+            }
+            if (c == 7) {
+                const double dfq[3] = {dFdqop.x, dFdqop.y, dFdqop.z};
+                const double dgq[3] = {dGdqop.x, dGdqop.y, dGdqop.z};
+                for (int r = 0; r < 3; ++r) {
+                    nj[jpos(r, 7)] += dfq[r] * jq;
+                    nj[jdir(r, 7)] += dgq[r] * jq;
+                }
+            }
+        }
+        nj[kJqq] = 1.0 * tk.J[kJqq];
+        for (int i = 0; i < 25; ++i) tk.J[i] = nj[i];
+    }
+
+    // policies.hpp:99-139
+    DTR_HD void rk_policy() {
+        const double d = nav_dist();
+        if (fabs(d) <= static_cast<double>(1.f * 1e-3f)) {
+            set_high_trust();
+            return;
+        }
+        const double rel = (tk.step_size - d) / d;
+        if (rel > 0.1) tk.trust = kNoTrust;
+        else if (rel > 0.05) set_fair_trust();
+        else set_high_trust();
+    }
+
+    // ------------------------------------------------------ frames, actors --
+    DTR_HD void free_to_bound(uint32_t s, double out[6]) const {
+        const dtr_surface& sf = det.surfaces[s];
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            out[0] = phi_of(tk.pos);
+            out[1] = tk.pos.z;
+        } else if (sf.shape == DTR_RING2D) {
+            const V3 l = to_local(det.transforms[sf.transform], tk.pos);
+            out[0] = perp(l);
+            out[1] = phi_of(l);
+        } else {
+            const V3 l = to_local(det.transforms[sf.transform], tk.pos);
+            out[0] = l.x;
+            out[1] = l.y;
+        }
+        out[2] = phi_of(tk.dir);
+        out[3] = theta_of(tk.dir);
+        out[4] = tk.qop;
+        out[5] = tk.time;
+    }
+    static DTR_HD V3 bound_dir(const double b[6]) {
+        const double s = sin(b[3]);
+        return {cos(b[2]) * s, sin(b[2]) * s, cos(b[3])};
+    }
+    DTR_HD V3 bound_pos(uint32_t s, const double b[6]) const {
+        const dtr_surface& sf = det.surfaces[s];
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double r = det.masks[sf.mask].v[0];
+            return {r * cos(b[0]), r * sin(b[0]), b[1]};
+        }
+        if (sf.shape == DTR_RING2D)
+            return to_global(det.transforms[sf.transform], {b[0] * cos(b[1]), b[0] * sin(b[1]), 0.0});
+        return to_global(det.transforms[sf.transform], {b[0], b[1], 0.0});
+    }
+
+    DTR_HD V3 dtds() const {
+        if (tk.path_length == 0.0) return mul(tk.qop, cross(tk.dir, field_at(field, tk.pos)));
+        return tk.dtds3;
+    }
+
+    // parameter_updater.hpp:422-572 get_full_jacobian, then
+    // covariance_transport.hpp C' = J C J^T
+    DTR_HD void transport_covariance(uint32_t dest) {
+        const uint32_t dep = tk.up_sf;
+        const dtr_surface& dsf = det.surfaces[dep];
+        // bound-to-free at the departure surface: A (3x2 dpos/dloc), C (3x2 ddir/dangle)
+        double A[6];
+        if (dsf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const V3 dp = bound_pos(dep, tk.bound);
+            const double r = perp(dp), ph = phi_of(dp);
+            A[0] = -r * sin(ph); A[1] = 0.0;
+            A[2] = r * cos(ph);  A[3] = 0.0;
+            A[4] = 0.0;          A[5] = 1.0;
+        } else {
+            const dtr_transform& t = det.transforms[dsf.transform];
+            A[0] = t.r[0]; A[1] = t.r[1];
+            A[2] = t.r[3]; A[3] = t.r[4];
+            A[4] = t.r[6]; A[5] = t.r[7];
+        }
+        double Cd[6];
+        {
+            const double th = tk.bound[3], ph = tk.bound[2];
+            const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+            Cd[0] = -st * sp; Cd[1] = ct * cp;
+            Cd[2] = st * cp;  Cd[3] = ct * sp;
+            Cd[4] = 0.0;      Cd[5] = -st;
+        }
+        // M = J_transport * B2F  (8x6, rows 0-2 pos, 4-6 dir, 7 qop; row 3 = time)
+        // columns: 0,1 loc; 2,3 angles; 4 qop; 5 time
+        double Mp[3][6], Md[3][6];
+        for (int r = 0; r < 3; ++r) {
+            // pos rows: J_pos = [I | Jp(4..7)]: I*A for loc, Jp[4..6]*C for angles, Jp7 for qop
+            Mp[r][0] = A[r * 2 + 0];
+            Mp[r][1] = A[r * 2 + 1];
+            Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
+            Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            Mp[r][4] = tk.J[jpos(r, 7)];
+            Mp[r][5] = 0.0;
+            Md[r][0] = 0.0;
+            Md[r][1] = 0.0;
+            Md[r][2] = tk.J[jdir(r, 4)] * Cd[0] + tk.J[jdir(r, 5)] * Cd[2] + tk.J[jdir(r, 6)] * Cd[4];
+            Md[r][3] = tk.J[jdir(r, 4)] * Cd[1] + tk.J[jdir(r, 5)] * Cd[3] + tk.J[jdir(r, 6)] * Cd[5];
+            Md[r][4] = tk.J[jdir(r, 7)];
+            Md[r][5] = 0.0;
+        }
+        const double Mq4 = tk.J[kJqq];  // qop row: only column 4
+        // path correction: P = (I + p2f f2p) applied on the left;
+        // p2f = [dir, 0, dtds, 0], f2p = [n_pos, 0...]
+        const V3 k = dtds();
+        V3 n;
+        const dtr_surface& esf = det.surfaces[dest];
+        if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double ph = phi_of(tk.pos);
+            n = {cos(ph), sin(ph), 0.0};
+        } else {
+            n = trf_axis(det.transforms[esf.transform], 2);
+        }
+        const V3 f2p = mul(-1.0 / dot(n, tk.dir), n);
+        double proj[6];  // f2p . M_pos (per column)
+        for (int c = 0; c < 6; ++c) proj[c] = f2p.x * Mp[0][c] + f2p.y * Mp[1][c] + f2p.z * Mp[2][c];
+        const double p2f_pos[3] = {tk.dir.x, tk.dir.y, tk.dir.z};
+        const double p2f_dir[3] = {k.x, k.y, k.z};
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 6; ++c) {
+                Mp[r][c] += p2f_pos[r] * proj[c];
+                Md[r][c] += p2f_dir[r] * proj[c];
+            }
+        }
+        // free-to-bound at the destination
+        double Bl[6];
+        if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
+            Bl[0] = -rinv * sin(ph); Bl[1] = rinv * cos(ph); Bl[2] = 0.0;
+            Bl[3] = 0.0;             Bl[4] = 0.0;            Bl[5] = 1.0;
+        } else {
+            const dtr_transform& t = det.transforms[esf.transform];
+            Bl[0] = t.r[0]; Bl[1] = t.r[3]; Bl[2] = t.r[6];
+            Bl[3] = t.r[1]; Bl[4] = t.r[4]; Bl[5] = t.r[7];
+        }
+        double Ba[6];
+        {
+            const double th = theta_of(tk.dir), ph = phi_of(tk.dir);
+            const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+            Ba[0] = -sp / st; Ba[1] = cp / st; Ba[2] = 0.0;
+            Ba[3] = cp * ct;  Ba[4] = sp * ct; Ba[5] = -st;
+        }
+        double Jf[6][6];
+        for (int c = 0; c < 6; ++c) {
+            Jf[0][c] = Bl[0] * Mp[0][c] + Bl[1] * Mp[1][c] + Bl[2] * Mp[2][c];
+            Jf[1][c] = Bl[3] * Mp[0][c] + Bl[4] * Mp[1][c] + Bl[5] * Mp[2][c];
+            Jf[2][c] = Ba[0] * Md[0][c] + Ba[1] * Md[1][c] + Ba[2] * Md[2][c];
+            Jf[3][c] = Ba[3] * Md[0][c] + Ba[4] * Md[1][c] + Ba[5] * Md[2][c];
+            Jf[4][c] = (c == 4) ? Mq4 : 0.0;
+            Jf[5][c] = (c == 5) ? 1.0 : 0.0;
+        }
+        // C' = Jf C Jf^T
+        double Cf[6][6];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) Cf[r][c] = tk.cov[covi(r, c)];
+        double T[6][6];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 6; ++m) s += Jf[r][m] * Cf[m][c];
+                T[r][c] = s;
+            }
+        for (int r = 0; r < 6; ++r)
+            for (int c = r; c < 6; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 6; ++m) s += T[r][m] * Jf[c][m];
+                tk.cov[covi(r, c)] = s;
+            }
+    }
+
+    DTR_HD bool on_sensitive() const {
+        return tk.status == kOnObject && det.surfaces[tk.c[tk.next - 1].sf].sf_id == DTR_SENSITIVE;
+    }
+    DTR_HD bool on_material() const {
+        return on_surface() && det.surfaces[tk.c[tk.next - 1].sf].material != kInvalid;
+    }
+
+    DTR_HD void reset_J() {
+        for (int i = 0; i < 25; ++i) tk.J[i] = 0.0;
+        tk.J[jdir(0, 4)] = 1.0;
+        tk.J[jdir(1, 5)] = 1.0;
+        tk.J[jdir(2, 6)] = 1.0;
+        tk.J[kJqq] = 1.0;
+    }
+
+    // relativistic_quantities.hpp + interaction.hpp
+    DTR_HD void material_interaction(uint32_t dest) {
+        const dtr_material_slab& m = det.materials[det.surfaces[dest].material];
+        V3 nrm;
+        if (det.surfaces[dest].shape == DTR_CONCENTRIC_CYLINDER2D)
+            nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
+        else
+            nrm = trf_axis(det.transforms[det.surfaces[dest].transform], 2);
+        const double cos_inc = fabs(dot(bound_dir(tk.bound), nrm));
+        if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
+        const double qop = tk.bound[4];
+        const double path_seg = m.thickness / cos_inc;
+        const double mass = tk.mass, q = tk.charge;
+        const double q2b2 = q * q + (mass * qop) * (mass * qop);
+        const double mOverP = mass * ((q != 0.0) ? fabs(qop / q) : fabs(qop));
+        const double pOverM = 1.0 / mOverP;
+        const double beta2 = 1.0 / (1.0 + mOverP * mOverP);
+        const double bg = pOverM;
+        const double gamma = sqrt(1.0 + pOverM * pOverM);
+        const double m_e = 0.51099895069 * 1e-3;
+        const double mfrac = m_e / mass;
+        const double Wmax = (2.0 * m_e * bg * bg) / (1.0 + 2.0 * gamma * mfrac + mfrac * mfrac);
+        const double K = 0.307075 * 1e-3 * 1e2;
+        const double ned = m.Z * m.molar_rho;
+        const double eps_pl = 0.5 * K * ned * q2b2;
+        double e_loss = 0.0, sigma_qop = 0.0, theta0 = 0.0;
+        if (cfg.do_eloss) {
+            double bb = 0.0;
+            if (eps_pl > 0.0) {
+                double dhalf = 0.0;
+                if (bg >= 10.0) {
+                    const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
+                    dhalf = log(bg * plasma / m.mean_exc) - 0.5;
+                }
+                const double u = 2.0 * m_e * bg * bg;
+                const double A = 0.5 * log(u * Wmax / (m.mean_exc * m.mean_exc));
+                bb = 2.0 * eps_pl * (A - beta2 - dhalf);
+            }
+            e_loss = path_seg * bb;
+            if (cfg.do_cov) {
+                const double eps = eps_pl * path_seg;
+                const double fwhm = 4.0 * eps;
+                const double sigmaE = 0.5 * 0.70710678118654752440 * fwhm / sqrt(0.69314718055994530942);
+                const double pInv = qop / q;
+                sigma_qop = sqrt(q2b2) * pInv * pInv * sigmaE;
+            }
+        }
+        if (cfg.do_ms) {
+            const double x0 = m.thickness_in_X0 / cos_inc;
+            const double pinv = fabs(qop / q);
+            if (x0 > 0.0) {
+                const double t = sqrt(x0 * q2b2);
+                theta0 = static_cast<double>(13.6f) * 1e-3 * pinv * t *
+                         (1.0 + static_cast<double>(0.038f) * 2.0 * log(t));
+            }
+        }
+        if (cfg.do_eloss) {
+            const double p = q / tk.bound[4];
+            const double next_e = sqrt(mass * mass + p * p) - copysign(e_loss, 1.0);
+            const double next_p = (mass < next_e) ? sqrt(next_e * next_e - mass * mass) : 0.0;
+            const double next_qop = (q != 0.0) ? q / next_p : 1.0 / next_p;
+            tk.bound[4] = (next_p == 0.0) ? kMax : next_qop;
+            if (cfg.do_cov) tk.cov[covi(4, 4)] += sigma_qop * sigma_qop;
+        }
+        if (cfg.do_cov) {
+            const V3 d = bound_dir(tk.bound);
+            const double var = theta0 * theta0;
+            tk.cov[covi(2, 2)] += (d.z == 1.0) ? kMax : var / (1.0 - d.z * d.z);
+            tk.cov[covi(3, 3)] += var;
+        }
+        // parameter_setter (parameter_updater.hpp:585-627)
+        tk.pos = bound_pos(dest, tk.bound);
+        tk.dir = bound_dir(tk.bound);
+        tk.time = tk.bound[5];
+        tk.qop = tk.bound[4];
+        if (cfg.estimate_noise) estimate_noise();
+    }
+
+    // noise_estimation.hpp:23-107
+    DTR_HD void estimate_noise() {
+        const double ns = cfg.n_stddev;
+        const double v0 = tk.cov[covi(0, 0)], v1 = tk.cov[covi(1, 1)];
+        const double dphi = ns * sqrt(tk.cov[covi(2, 2)]);
+        const double dth = ns * sqrt(tk.cov[covi(3, 3)]);
+        const double pe = tk.bound[2] + dphi, te = tk.bound[3] + dth;
+        const double ste = sin(te);
+        V3 displ{cos(pe) * ste, sin(pe) * ste, cos(te)};
+        const double max_tol = 5.0;
+        const double path = exhausted() ? max_tol : fabs(tk.c[tk.next].path);
+        displ = mul(path, sub(displ, tk.dir));
+        const double p = tk.charge / tk.qop;
+        const double acc = 1.0 / p * cfg.accumulated_error * tk.path_length;
+        const double ext = sqrt(ns * ns * (v0 + v1) + dot(displ, displ) + acc * acc);
+        tk.ext_tol = ext > max_tol ? max_tol : ext;
+    }
+
+    DTR_HD void parameter_updater(const double* init_cov, uint32_t n, uint32_t gid) {
+        if (!(on_sensitive() || on_material())) return;
+        const uint32_t dest = tk.c[tk.next - 1].sf;
+        if (tk.up_sf != kInvalid) {
+            if (tk.up_sf != dest) {
+                transport_covariance(dest);
+                if (fabs(tk.path_length) > 0.0) reset_J();
+                free_to_bound(dest, tk.bound);
+                tk.up_sf = dest;
+            }
+        } else {
+            if (init_cov) {
+                for (int k = 0; k < 21; ++k) tk.cov[k] = init_cov[static_cast<uint64_t>(k) * n + gid];
+            } else {
+                for (int r = 0; r < 6; ++r)
+                    for (int c = r; c < 6; ++c) tk.cov[covi(r, c)] = (r == c) ? 1.0 : 0.0;
+            }
+            free_to_bound(dest, tk.bound);
+            tk.up_sf = dest;
+        }
+        if (on_material()) material_interaction(dest);
+    }
+};
+
+// One track: propagator::propagate (propagator.hpp:262-400)
+DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
+                            const DevConfig& cfg, const double* __restrict__ fp,
+                                const double* __restrict__ init_cov, const DevResults& out,
+                                uint32_t n, uint32_t gid) {
+    Track tk;
+    for (int i = 0; i < kCache; ++i) {
+        tk.c[i].path = kMax;
+        tk.c[i].sf = kInvalid;
+        tk.c[i].link = 0;
+        tk.c[i].status = kOutside;
+        tk.c[i].pad = 0;
+    }
+    tk.ext_tol = 0.0;
+    tk.next = 0;
+    tk.last = -1;
+    tk.status = kUnknown;
+    tk.trust = kNoTrust;
+    tk.volume = 0;
+    tk.path_limit_hit = false;
+    tk.overflow = false;
+    tk.pos = {fp[0 * static_cast<uint64_t>(n) + gid], fp[1 * static_cast<uint64_t>(n) + gid],
+              fp[2 * static_cast<uint64_t>(n) + gid]};
+    tk.time = fp[3 * static_cast<uint64_t>(n) + gid];
+    tk.dir = {fp[4 * static_cast<uint64_t>(n) + gid], fp[5 * static_cast<uint64_t>(n) + gid],
+              fp[6 * static_cast<uint64_t>(n) + gid]};
+    tk.qop = fp[7 * static_cast<uint64_t>(n) + gid];
+    tk.step_size = 0.0;
+    tk.next_step_size = 0.0;
+    tk.path_length = 0.0;
+    tk.abs_path_length = 0.0;
+    tk.constraint_acc = dmin(kMax, fabs(cfg.step_constraint));
+    tk.constraint_abort = kMax;
+    tk.dtds3 = {0.0, 0.0, 0.0};
+    // muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64)
+    tk.mass = static_cast<double>(105.6583755f * 1e-3f);
+    tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
+    tk.n_trials = 0;
+    tk.n_steps = 0;
+    for (int i = 0; i < 6; ++i) tk.bound[i] = 0.0;
+    for (int i = 0; i < 21; ++i) tk.cov[i] = 0.0;
+    tk.up_sf = kInvalid;
+    tk.n_seq = 0;
+
+    Engine e{det, field, cfg, tk};
+    e.reset_J();
+
+    // navigator_base::init -> local_navigation without overstepping resolution
+    e.local_navigation(cfg.overstep_tol, false);
+    tk.heartbeat = tk.status > kUnknown;
+    bool is_init = true;
+    double path_length = 0.0;
+    unsigned stall = 0;
+    for (unsigned i = 0; i % 2 == 0 || tk.heartbeat; ++i) {
+        if (i % 2 == 0) {
+            // pathlimit_aborter (aborters.hpp:44-75)
+            if (tk.status != kExit) {
+                const double step_limit = cfg.path_limit - fabs(tk.abs_path_length);
+                if (step_limit <= 0.0) {
+                    tk.status = kAbort;
+                    tk.path_limit_hit = true;
+                    tk.heartbeat = false;
+                }
+                tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
+            }
+            // parameter_updater<pointwise_material_interactor>
+            if (cfg.use_material) e.parameter_updater(init_cov, n, gid);
+            // surface_sequencer (surface_sequencer.hpp:78-106)
+            if (cfg.max_surfaces != 0u && (e.on_sensitive() || e.on_material())) {
+                if (tk.n_seq == cfg.max_surfaces) {
+                    tk.overflow = true;
+                    tk.status = kExit;
+                    tk.heartbeat = false;
+                } else {
+                    out.surfaces[static_cast<uint64_t>(tk.n_seq) * n + gid] = tk.c[tk.next - 1].sf;
+                    tk.n_seq++;
+                }
+            }
+            if (!tk.heartbeat) continue;
+            path_length = tk.path_length;
+        } else {
+            const bool reset = e.on_surface() || is_init;
+            e.step(e.nav_dist(), reset);
+            e.rk_policy();
+            is_init = false;
+            if (fabs(tk.path_length) <= fabs(path_length) + static_cast<double>(cfg.path_tol)) {
+                if (stall >= 10u) {
+                    tk.heartbeat = false;
+                    tk.status = kAbort;
+                }
+                stall++;
+            } else {
+                stall = 0;
+            }
+        }
+        is_init |= e.nav_update();
+        tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
+    }
+
+    uint8_t status;
+    if (tk.status == kExit) status = tk.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
+    else status = tk.path_limit_hit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
+    const uint64_t N = n;
+    if (out.status) out.status[gid] = status;
+    if (out.bound_params)
+        for (int k = 0; k < 6; ++k) out.bound_params[k * N + gid] = tk.bound[k];
+    if (out.bound_cov)
+        for (int k = 0; k < 21; ++k) out.bound_cov[k * N + gid] = tk.cov[k];
+    if (out.free_params) {
+        out.free_params[0 * N + gid] = tk.pos.x;
+        out.free_params[1 * N + gid] = tk.pos.y;
+        out.free_params[2 * N + gid] = tk.pos.z;
+        out.free_params[3 * N + gid] = tk.time;
+        out.free_params[4 * N + gid] = tk.dir.x;
+        out.free_params[5 * N + gid] = tk.dir.y;
+        out.free_params[6 * N + gid] = tk.dir.z;
+        out.free_params[7 * N + gid] = tk.qop;
+    }
+    if (out.path_length) out.path_length[gid] = tk.path_length;
+    if (out.n_surfaces) out.n_surfaces[gid] = tk.n_seq;
+    if (out.n_steps) out.n_steps[gid] = tk.n_steps;
+    if (out.n_trials) out.n_trials[gid] = tk.n_trials;
+}
+
+
+}  // namespace dtr

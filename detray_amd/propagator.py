@@ -1,0 +1,338 @@
+"""Host-side mirror of detray's propagation interface over the C ABI.
+
+The names follow the reference:
+
+* :class:`PropagationConfig` mirrors ``propagation::config``
+  (core/include/detray/propagator/propagation_config.hpp: stepping,
+  navigation and intersection configs with their defaults).
+* :class:`Detector` holds the flattened ``detector<metadata>`` tables
+  (toy or telescope geometry) and its device copy
+  (``detector::view_type``, core/include/detray/core/detector.hpp:136-202).
+* :class:`MagneticField` is the field view (constant or sampled grid).
+* :class:`Propagator` runs ``propagator<rk_stepper, caching_navigator,
+  actor_chain<pathlimit_aborter, parameter_updater<
+  pointwise_material_interactor>, surface_sequencer>>::propagate`` over a
+  batch of tracks on the GPU.
+
+PyTorch is only used for device memory and the stream; the compute is the
+hand-written HIP kernel in ``libdetray_amd.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field as dc_field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import capi
+
+# detray units (core/include/detray/definitions/units.hpp)
+MM = 1.0
+M = 1000.0
+UM = 1e-3
+GEV = 1.0
+MEV = 1e-3
+TESLA = 0.000299792458
+
+
+def _f32(x: float) -> float:
+    return float(np.float32(x))
+
+
+@dataclass
+class PropagationConfig:
+    """``propagation::config`` with the reference defaults."""
+
+    # stepping::config (stepping_config.hpp:27-48)
+    min_stepsize: float = 1e-4 * MM
+    rk_error_tol: float = 1e-4 * MM
+    step_constraint: float = float(np.finfo(np.float32).max)
+    path_limit: float = 5.0 * M
+    max_rk_updates: int = 10000
+    use_mean_loss: bool = True
+    use_eloss_gradient: bool = False
+    use_field_gradient: bool = False
+    do_covariance_transport: bool = True
+    # intersection::config (intersection_config.hpp:20-36)
+    min_mask_tolerance: float = 1e-5 * MM
+    max_mask_tolerance: float = 3.0 * MM
+    mask_tolerance_scalor: float = 5e-2
+    path_tolerance: float = 1.0 * UM
+    overstep_tolerance: float = -1000.0 * UM
+    # navigation::config (navigation_config.hpp:23-40)
+    search_window: Sequence[int] = (0, 0)
+    accumulated_error: float = 0.001
+    n_scattering_stddev: int = 3
+    estimate_scattering_noise: bool = True
+    # pointwise_material_interactor::state
+    do_energy_loss: bool = True
+    do_multiple_scattering: bool = True
+    # actor chain: run parameter_updater<pointwise_material_interactor>
+    use_material: bool = True
+    # surface_sequencer capacity
+    max_surfaces: int = 64
+
+    def to_c(self) -> capi.PropConfig:
+        c = capi.PropConfig()
+        c.min_stepsize = self.min_stepsize
+        c.rk_error_tol = self.rk_error_tol
+        c.step_constraint = self.step_constraint
+        c.path_limit = self.path_limit
+        c.max_rk_updates = int(self.max_rk_updates)
+        c.use_mean_loss = int(self.use_mean_loss)
+        c.use_eloss_gradient = int(self.use_eloss_gradient)
+        c.use_field_gradient = int(self.use_field_gradient)
+        c.do_covariance_transport = int(self.do_covariance_transport)
+        c.min_mask_tolerance = self.min_mask_tolerance
+        c.max_mask_tolerance = self.max_mask_tolerance
+        c.mask_tolerance_scalor = self.mask_tolerance_scalor
+        c.path_tolerance = self.path_tolerance
+        c.overstep_tolerance = self.overstep_tolerance
+        c.search_window[0] = int(self.search_window[0])
+        c.search_window[1] = int(self.search_window[1])
+        c.accumulated_error = self.accumulated_error
+        c.n_scattering_stddev = int(self.n_scattering_stddev)
+        c.estimate_scattering_noise = int(self.estimate_scattering_noise)
+        c.do_energy_loss = int(self.do_energy_loss)
+        c.do_multiple_scattering = int(self.do_multiple_scattering)
+        c.use_material = int(self.use_material)
+        c.max_surfaces = int(self.max_surfaces)
+        return c
+
+
+class Detector:
+    """Flattened detector tables (host) plus a lazily created device copy."""
+
+    def __init__(self, desc: capi.DetectorDesc, name: str):
+        self.desc = desc
+        self.name = name
+        self._device = None
+
+    # -- builders ----------------------------------------------------------
+    @classmethod
+    def toy(cls, n_brl_layers: int = 4, n_edc_layers: int = 3, use_material: bool = True) -> "Detector":
+        """detray's toy detector (build_toy_detector.hpp)."""
+        lib = capi.load()
+        d = capi.DetectorDesc()
+        cfg = capi.ToyConfig(n_brl_layers, n_edc_layers, int(use_material))
+        capi.check(lib.dtr_build_toy_detector(C.byref(cfg), C.byref(d)), "dtr_build_toy_detector")
+        return cls(d, f"toy_{n_brl_layers}brl_{n_edc_layers}edc")
+
+    @classmethod
+    def telescope(cls, positions: Sequence[float], half_x: float = 500.0, half_y: float = 500.0,
+                  thickness: float = 0.15, envelope: float = 60.0) -> "Detector":
+        """Telescope of rectangles along the z-axis pilot ray
+        (build_telescope_detector.hpp; tel_det_config::envelope sets the
+        portal box margin)."""
+        lib = capi.load()
+        pos = (C.c_double * len(positions))(*positions)
+        cfg = capi.TelescopeConfig(len(positions), 0, pos, half_x, half_y, thickness, envelope)
+        d = capi.DetectorDesc()
+        capi.check(lib.dtr_build_telescope_detector(C.byref(cfg), C.byref(d)),
+                   "dtr_build_telescope_detector")
+        return cls(d, f"telescope_{len(positions)}")
+
+    # -- table views -------------------------------------------------------
+    def table(self, name: str):
+        n = {
+            "volumes": self.desc.n_volumes, "surfaces": self.desc.n_surfaces,
+            "transforms": self.desc.n_transforms, "masks": self.desc.n_masks,
+            "materials": self.desc.n_materials, "grids": self.desc.n_grids,
+            "bf_surfaces": self.desc.n_bf, "grid_bins": self.desc.n_bins,
+        }[name]
+        ptr = getattr(self.desc, name)
+        return [ptr[i] for i in range(n)]
+
+    @property
+    def n_surfaces(self) -> int:
+        return int(self.desc.n_surfaces)
+
+    def device(self):
+        if self._device is None:
+            lib = capi.load()
+            h = C.c_void_p()
+            capi.check(lib.dtr_detector_create(C.byref(self.desc), C.byref(h)), "dtr_detector_create")
+            self._device = h
+        return self._device
+
+    def close(self) -> None:
+        lib = capi.load()
+        if self._device is not None:
+            lib.dtr_detector_destroy(self._device)
+            self._device = None
+        if self.desc.n_surfaces:
+            lib.dtr_host_detector_free(C.byref(self.desc))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MagneticField:
+    """Constant field or a regular xyz grid with trilinear interpolation."""
+
+    def __init__(self, desc: capi.FieldDesc, values: Optional[np.ndarray] = None, name: str = ""):
+        self.desc = desc
+        self.values = values  # keep the host samples alive
+        self.name = name
+        self._device = None
+
+    @classmethod
+    def constant(cls, bx: float, by: float, bz: float) -> "MagneticField":
+        d = capi.FieldDesc()
+        d.kind = 0
+        d.b[0], d.b[1], d.b[2] = bx, by, bz
+        return cls(d, None, f"const({bx:.3g},{by:.3g},{bz:.3g})")
+
+    @classmethod
+    def solenoid(cls, b_center: float = 2.0 * TESLA, radius: float = 1200.0,
+                 half_length: float = 3000.0, r_max: float = 1500.0, z_max: float = 3500.0,
+                 pitch: float = 50.0) -> "MagneticField":
+        """Analytic finite solenoid sampled on x, y in [-r_max, r_max],
+        z in [-z_max, z_max] with the given pitch, stored as fp32 xyz."""
+        lib = capi.load()
+        nxy = int(round(2 * r_max / pitch)) + 1
+        nz = int(round(2 * z_max / pitch)) + 1
+        n = (C.c_uint32 * 3)(nxy, nxy, nz)
+        mn = (C.c_double * 3)(-r_max, -r_max, -z_max)
+        pt = (C.c_double * 3)(pitch, pitch, pitch)
+        values = np.zeros(nxy * nxy * nz * 3, dtype=np.float32)
+        capi.check(lib.dtr_sample_solenoid(b_center, radius, half_length, n, mn, pt,
+                                           values.ctypes.data_as(C.POINTER(C.c_float))),
+                   "dtr_sample_solenoid")
+        d = capi.FieldDesc()
+        d.kind = 1
+        for i in range(3):
+            d.n[i] = n[i]
+            d.min[i] = mn[i]
+            d.pitch[i] = pt[i]
+        d.values = values.ctypes.data_as(C.POINTER(C.c_float))
+        return cls(d, values, f"solenoid({nxy}x{nxy}x{nz})")
+
+    def device(self):
+        if self._device is None:
+            lib = capi.load()
+            h = C.c_void_p()
+            capi.check(lib.dtr_field_create(C.byref(self.desc), C.byref(h)), "dtr_field_create")
+            self._device = h
+        return self._device
+
+    def close(self) -> None:
+        if self._device is not None:
+            capi.load().dtr_field_destroy(self._device)
+            self._device = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate_tracks(kind: str, n: int, seed: int, **kw) -> np.ndarray:
+    """Seeded track generation, returns free parameters [8, n] (float64).
+
+    kind="beam": origin, q = +-1, |p| ~ U[p_min, p_max], polar angle
+    |N(0, sigma_theta)|, phi ~ U[-pi, pi).
+    kind="toy": eta ~ U[eta_min, eta_max], phi ~ U, pT log-uniform
+    [pt_min, pt_max], vertex x,y ~ N(0, sigma_xy), z ~ N(0, sigma_z).
+    """
+    lib = capi.load()
+    g = capi.TrackGenConfig()
+    g.seed = seed
+    g.n = n
+    if kind == "beam":
+        g.kind = 0
+        g.p_min = kw.get("p_min", 1.0)
+        g.p_max = kw.get("p_max", 10.0)
+        g.sigma_theta = kw.get("sigma_theta", 0.1)
+    elif kind == "toy":
+        g.kind = 1
+        g.eta_min = kw.get("eta_min", -2.5)
+        g.eta_max = kw.get("eta_max", 2.5)
+        g.pt_min = kw.get("pt_min", 1.0)
+        g.pt_max = kw.get("pt_max", 100.0)
+        g.sigma_xy = kw.get("sigma_xy", 0.01)
+        g.sigma_z = kw.get("sigma_z", 50.0)
+    else:
+        raise ValueError(kind)
+    out = np.zeros((8, n), dtype=np.float64)
+    capi.check(lib.dtr_generate_tracks(C.byref(g), out.ctypes.data_as(C.POINTER(C.c_double))),
+               "dtr_generate_tracks")
+    return out
+
+
+@dataclass
+class PropagationResult:
+    """Per-track outputs (device tensors, SoA)."""
+
+    bound_params: "object"   # [6, n] float64
+    bound_cov: "object"      # [21, n] float64 (upper triangle)
+    free_params: "object"    # [8, n]
+    path_length: "object"    # [n]
+    surfaces: "object"       # [max_surfaces, n] uint32 (as int32 storage)
+    n_surfaces: "object"     # [n]
+    n_steps: "object"        # [n]
+    n_trials: "object"       # [n]
+    status: "object"         # [n] uint8
+
+
+class Propagator:
+    """``detray::propagator`` over a batch of tracks on one GPU."""
+
+    def __init__(self, config: Optional[PropagationConfig] = None):
+        self.config = config or PropagationConfig()
+        self._c_config = self.config.to_c()
+
+    def allocate(self, n: int, device="cuda") -> PropagationResult:
+        import torch
+
+        ms = max(int(self.config.max_surfaces), 1)
+        return PropagationResult(
+            bound_params=torch.zeros((6, n), dtype=torch.float64, device=device),
+            bound_cov=torch.zeros((21, n), dtype=torch.float64, device=device),
+            free_params=torch.zeros((8, n), dtype=torch.float64, device=device),
+            path_length=torch.zeros((n,), dtype=torch.float64, device=device),
+            surfaces=torch.full((ms, n), -1, dtype=torch.int32, device=device),
+            n_surfaces=torch.zeros((n,), dtype=torch.int32, device=device),
+            n_steps=torch.zeros((n,), dtype=torch.int32, device=device),
+            n_trials=torch.zeros((n,), dtype=torch.int32, device=device),
+            status=torch.full((n,), 255, dtype=torch.uint8, device=device),
+        )
+
+    def propagate(self, detector: Detector, field: MagneticField, tracks, initial_cov=None,
+                  out: Optional[PropagationResult] = None, stream=None) -> PropagationResult:
+        """Propagate ``tracks`` (device float64 tensor [8, n], free parameters)."""
+        import torch
+
+        if not (isinstance(tracks, torch.Tensor) and tracks.is_cuda):
+            raise TypeError("tracks must be a CUDA (HIP) tensor [8, n] of float64")
+        if tracks.dtype != torch.float64 or tracks.dim() != 2 or tracks.shape[0] != 8:
+            raise ValueError("tracks must be float64 with shape [8, n]")
+        tracks = tracks.contiguous()
+        n = int(tracks.shape[1])
+        if initial_cov is not None:
+            if initial_cov.shape != (21, n) or initial_cov.dtype != torch.float64 or not initial_cov.is_cuda:
+                raise ValueError("initial_cov must be a float64 CUDA tensor [21, n]")
+            initial_cov = initial_cov.contiguous()
+        if out is None:
+            out = self.allocate(n, tracks.device)
+        lib = capi.load()
+        tb = capi.TrackBatch(n, 0, tracks.data_ptr(),
+                             initial_cov.data_ptr() if initial_cov is not None else None)
+        rb = capi.ResultBatch(out.bound_params.data_ptr(), out.bound_cov.data_ptr(),
+                              out.free_params.data_ptr(), out.path_length.data_ptr(),
+                              out.surfaces.data_ptr(), out.n_surfaces.data_ptr(),
+                              out.n_steps.data_ptr(), out.n_trials.data_ptr(),
+                              out.status.data_ptr())
+        if stream is None:
+            stream = torch.cuda.current_stream(tracks.device).cuda_stream
+        capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
+                                     C.byref(tb), C.byref(rb), C.c_void_p(stream)),
+                   "dtr_propagate")
+        return out

@@ -1,0 +1,302 @@
+/*
+ * detray_amd — MI355X-native track propagation behind a C ABI.
+ *
+ * This header is the drop-in boundary for detray's propagation hot path:
+ *   detray::propagator<rk_stepper, caching_navigator, actor_chain>::propagate
+ *   (core/include/detray/propagator/propagator.hpp:260-400)
+ * run over a batch of tracks on one GPU.  Every entry point takes plain
+ * pointers and sizes, returns an int status code (never exits the process,
+ * unlike DETRAY_HIP_ERROR_CHECK, core/include/detray/definitions/detail/
+ * hip_definitions.hpp:20-31) and keeps detray's vocabulary: detector tables,
+ * field, propagation config, actor chain, bound/free track parameters.
+ *
+ * Units follow detray (core/include/detray/definitions/units.hpp):
+ * mm, GeV, e; magnetic field in GeV/(e mm) (1 T = 0.000299792458).
+ *
+ * Track parametrisation (core/include/detray/definitions/track_parametrization.hpp:18-64):
+ *   free  vector [x, y, z, t, dx, dy, dz, q/p]
+ *   bound vector [loc0, loc1, phi, theta, q/p, t]
+ */
+#ifndef DETRAY_AMD_H
+#define DETRAY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+enum dtr_status_code {
+    DTR_SUCCESS = 0,
+    DTR_ERROR_INVALID_ARGUMENT = 1,
+    DTR_ERROR_HIP = 2,
+    DTR_ERROR_OUT_OF_MEMORY = 3,
+    DTR_ERROR_NO_DEVICE = 4,
+    DTR_ERROR_UNSUPPORTED = 5
+};
+
+/* Per-track propagation outcome (navigation::status at the end of
+ * propagate(), core/include/detray/definitions/navigation.hpp:24-31). */
+enum dtr_track_status {
+    DTR_TRACK_EXIT = 0,        /* left the world: propagate() returned true */
+    DTR_TRACK_ABORT = 1,       /* navigation aborted (stall, no surfaces) */
+    DTR_TRACK_PATH_LIMIT = 2,  /* pathlimit_aborter fired */
+    DTR_TRACK_OVERFLOW = 3,    /* surface_sequencer capacity reached */
+    DTR_TRACK_NOT_RUN = 255
+};
+
+/* ------------------------------------------------------------------------ */
+/* Detector tables (flattened detray::detector<toy_metadata|telescope>)      */
+/* ------------------------------------------------------------------------ */
+
+/* Mask shapes (toy_metadata mask_id, detectors/include/detray/detectors/
+ * toy_metadata.hpp:48-53; the telescope uses rectangles only). */
+enum dtr_mask_shape {
+    DTR_RECTANGLE2D = 0,          /* v = {half_x, half_y}                      */
+    DTR_TRAPEZOID2D = 1,          /* v = {half_x_low, half_x_high, half_y, 1/(2 half_y)} */
+    DTR_CONCENTRIC_CYLINDER2D = 2,/* v = {r, lower_z, upper_z}                  */
+    DTR_RING2D = 3                /* v = {inner_r, outer_r}                     */
+};
+
+/* Surface categories (core/include/detray/definitions/geometry.hpp:40-46). */
+enum dtr_surface_id { DTR_PORTAL = 0, DTR_SENSITIVE = 1, DTR_PASSIVE = 2 };
+
+/* Placement: global = R * local + t.  r is row-major, its columns are the
+ * local x, y, z axes in global coordinates (generic_transform3.hpp:79-101). */
+typedef struct dtr_transform {
+    double r[9];
+    double t[3];
+} dtr_transform;
+
+typedef struct dtr_mask {
+    double v[4];
+    uint16_t volume_link; /* 0xffff = leaves the world */
+    uint8_t shape;        /* dtr_mask_shape */
+    uint8_t pad[5];
+} dtr_mask;
+
+/* Homogeneous material slab (core/include/detray/material/material_slab.hpp,
+ * material.hpp).  Derived quantities are filled by the builder. */
+typedef struct dtr_material_slab {
+    double X0, L0, A, Z, mass_rho;
+    double molar_rho;     /* mass_rho / (A * u) in mol/mm^3 */
+    double mean_exc;      /* I = 16 eV * Z^0.9 */
+    double thickness;     /* mm */
+    double thickness_in_X0;
+} dtr_material_slab;
+
+typedef struct dtr_surface {
+    uint64_t barcode;   /* detray::geometry::identifier (identifier.hpp:153-157) */
+    uint32_t transform; /* into transforms[] */
+    uint32_t mask;      /* first mask in masks[] */
+    uint32_t material;  /* into materials[], 0xffffffff = none */
+    uint16_t volume;
+    uint8_t n_masks;
+    uint8_t sf_id;      /* dtr_surface_id */
+    uint8_t shape;      /* dtr_mask_shape of all masks of this surface */
+    uint8_t pad[7];
+} dtr_surface;
+
+/* Grid types on the volume's sensitive surfaces (toy_metadata.hpp:119-160) */
+enum dtr_grid_type {
+    DTR_GRID_CYLINDER = 0, /* axes (phi circular [-pi,pi), z closed) */
+    DTR_GRID_RING = 1      /* axes (r closed, phi circular [-pi,pi)) */
+};
+
+typedef struct dtr_grid {
+    double min0, max0, min1, max1; /* axis spans */
+    double radius;                 /* reference cylinder radius (cylinder grids) */
+    uint32_t n0, n1;               /* bins per axis */
+    uint32_t bins;                 /* offset into grid_bins[], n0*n1 entries, bin(i0,i1) = i0*n1 + i1 */
+    uint8_t type;                  /* dtr_grid_type */
+    uint8_t pad[3];
+} dtr_grid;
+
+typedef struct dtr_volume {
+    uint32_t transform;         /* placement */
+    uint32_t bf_begin, bf_end;  /* brute-force surface list range in bf_surfaces[] */
+    uint32_t sf_begin, sf_end;  /* all surfaces of the volume */
+    int32_t grid;               /* -1: none */
+    uint32_t pad;
+} dtr_volume;
+
+typedef struct dtr_detector_desc {
+    uint32_t n_volumes, n_surfaces, n_transforms, n_masks;
+    uint32_t n_materials, n_grids, n_bf, n_bins;
+    const dtr_volume* volumes;
+    const dtr_surface* surfaces;
+    const dtr_transform* transforms;
+    const dtr_mask* masks;
+    const dtr_material_slab* materials;
+    const dtr_grid* grids;
+    const uint32_t* bf_surfaces;
+    const uint32_t* grid_bins; /* surface index or 0xffffffff */
+} dtr_detector_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Magnetic field                                                           */
+/* ------------------------------------------------------------------------ */
+enum dtr_field_kind { DTR_FIELD_CONSTANT = 0, DTR_FIELD_GRID = 1 };
+
+/* Regular xyz grid of float3 samples, trilinear interpolation with clamping
+ * at the border (the covfie affine -> linear -> clamp -> strided composition
+ * used by tests/include/detray/test/common/bfield.hpp:44-47). */
+typedef struct dtr_field_desc {
+    int32_t kind;
+    int32_t pad;
+    double b[3];               /* constant field */
+    uint32_t n[3];             /* grid points per axis */
+    uint32_t pad2;
+    double min[3];             /* position of sample (0,0,0) */
+    double pitch[3];           /* sample spacing */
+    const float* values;       /* [(ix*ny + iy)*nz + iz][3] */
+} dtr_field_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Propagation configuration (propagation::config, propagation_config.hpp;  */
+/* stepping_config.hpp:27-48, navigation_config.hpp:23-40,                  */
+/* intersection_config.hpp:20-36).  Same field types and defaults.          */
+/* ------------------------------------------------------------------------ */
+typedef struct dtr_prop_config {
+    /* stepping::config */
+    float min_stepsize;       /* 1e-4 mm */
+    float rk_error_tol;       /* 1e-4 mm */
+    float step_constraint;    /* FLT_MAX (accuracy constraint of constrained_step) */
+    float path_limit;         /* 5 m (pathlimit_aborter) */
+    uint32_t max_rk_updates;  /* 10000 */
+    uint8_t use_mean_loss;    /* 1 */
+    uint8_t use_eloss_gradient;    /* 0 */
+    uint8_t use_field_gradient;    /* 0 */
+    uint8_t do_covariance_transport; /* 1 */
+    /* intersection::config */
+    float min_mask_tolerance; /* 1e-5 mm */
+    float max_mask_tolerance; /* 3 mm */
+    float mask_tolerance_scalor; /* 5e-2 */
+    float path_tolerance;     /* 1 um */
+    float overstep_tolerance; /* -1000 um */
+    /* navigation::config */
+    uint32_t search_window[2];/* {0, 0} */
+    float accumulated_error;  /* 0.001 */
+    int32_t n_scattering_stddev; /* 3 */
+    uint8_t estimate_scattering_noise; /* 1 */
+    /* pointwise_material_interactor::state */
+    uint8_t do_energy_loss;        /* 1 */
+    uint8_t do_multiple_scattering;/* 1 */
+    uint8_t use_material;          /* 1: run parameter_updater<pointwise_material_interactor> */
+    /* surface_sequencer capacity (0 disables recording) */
+    uint32_t max_surfaces;
+} dtr_prop_config;
+
+/* ------------------------------------------------------------------------ */
+/* Track batches (structure of arrays, parameter-major: x[n], y[n], ...)     */
+/* ------------------------------------------------------------------------ */
+typedef struct dtr_track_batch {
+    uint32_t n;
+    uint32_t pad;
+    const double* free_params;  /* [8][n] */
+    const double* initial_cov;  /* NULL or [21][n] upper-triangle bound cov used at
+                                   the first surface in place of the identity
+                                   placeholder (parameter_updater.hpp:392-397) */
+} dtr_track_batch;
+
+typedef struct dtr_result_batch {
+    double* bound_params;   /* [6][n]  at the last sensitive/material surface */
+    double* bound_cov;      /* [21][n] upper triangle, row-major (0,0),(0,1)..(5,5) */
+    double* free_params;    /* [8][n]  at the end of propagation */
+    double* path_length;    /* [n] */
+    uint32_t* surfaces;     /* [max_surfaces][n] surface indices, in order */
+    uint32_t* n_surfaces;   /* [n] */
+    uint32_t* n_steps;      /* [n] RK steps taken */
+    uint32_t* n_trials;     /* [n] RK trials (stepping.n_total_trials()) */
+    uint8_t* status;        /* [n] dtr_track_status */
+} dtr_result_batch;
+
+/* ------------------------------------------------------------------------ */
+/* Handles                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct dtr_detector dtr_detector; /* device-resident, immutable */
+typedef struct dtr_field dtr_field;
+
+/* Defaults of propagation::config (and the interactor state). */
+void dtr_default_config(dtr_prop_config* cfg);
+
+/* Copy the detector tables to the current HIP device.
+ * Replaces detray::get_buffer(detector) + detector(view)
+ * (core/include/detray/core/detector.hpp:136-202). */
+int dtr_detector_create(const dtr_detector_desc* desc, dtr_detector** out);
+int dtr_detector_destroy(dtr_detector* det);
+
+/* Field view on the device: constant or sampled grid.
+ * Replaces covfie::field<...>::view_t (tests/include/detray/test/common/bfield.hpp:35-88). */
+int dtr_field_create(const dtr_field_desc* desc, dtr_field** out);
+int dtr_field_destroy(dtr_field* field);
+
+/* Propagate a batch of tracks on the device.
+ * in/out point to DEVICE memory (SoA, see above).  The actor chain is
+ *   pathlimit_aborter, parameter_updater<pointwise_material_interactor>,
+ *   surface_sequencer
+ * as in tests/include/detray/test/device/propagator_test.hpp:87-99 and
+ * core/include/detray/propagator/actors/surface_sequencer.hpp:25-106.
+ * The stepper is rk_stepper<field, algebra, constrained_step> with the
+ * caching_navigator (cache size 8).  Work is enqueued on `stream`
+ * (a hipStream_t, NULL = default stream); the call does not synchronise. */
+int dtr_propagate(const dtr_detector* det, const dtr_field* field,
+                  const dtr_prop_config* cfg, const dtr_track_batch* in,
+                  dtr_result_batch* out, void* stream);
+
+/* Host-side builders (detector emitters and inputs).  The returned tables
+ * are owned by the library; free them with dtr_host_detector_free. */
+typedef struct dtr_toy_config {
+    uint32_t n_brl_layers;    /* 4 (build_toy_detector.hpp:117) */
+    uint32_t n_edc_layers;    /* 3 (build_toy_detector.hpp:119) */
+    uint8_t use_material;     /* 1: Si slabs on modules, Be beampipe */
+    uint8_t pad[3];
+} dtr_toy_config;
+
+typedef struct dtr_telescope_config {
+    uint32_t n_planes;
+    uint32_t pad;
+    const double* positions; /* along the pilot ray (0,0,0)->(0,0,1), mm */
+    double half_x, half_y;   /* rectangle half lengths */
+    double thickness;        /* silicon_tml slab thickness, 0 = no material */
+    double envelope;         /* cuboid portal envelope (tel_det_config::envelope) */
+} dtr_telescope_config;
+
+int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out);
+int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
+                                 dtr_detector_desc* out);
+void dtr_host_detector_free(dtr_detector_desc* desc);
+
+/* Sample the analytic finite solenoid (on-axis 2 T, radius 1.2 m, half
+ * length 3 m) on a regular grid.  `values` must hold nx*ny*nz*3 floats. */
+int dtr_sample_solenoid(double b_center, double radius, double half_length,
+                        const uint32_t n[3], const double min[3],
+                        const double pitch[3], float* values);
+
+/* Seeded track generators (counter-based SplitMix64; identical bits on
+ * every host).  `free_params` is host memory [8][n]. */
+typedef struct dtr_track_gen_config {
+    uint64_t seed;
+    uint32_t n;
+    uint32_t kind;            /* 0: telescope beam, 1: toy muons */
+    /* kind 0: |p| uniform [p_min,p_max], polar angle |N(0, sigma_theta)| */
+    double p_min, p_max, sigma_theta;
+    /* kind 1: eta uniform, phi uniform, pT log-uniform, Gaussian vertex */
+    double eta_min, eta_max, pt_min, pt_max;
+    double sigma_xy, sigma_z;
+} dtr_track_gen_config;
+
+int dtr_generate_tracks(const dtr_track_gen_config* cfg, double* free_params);
+
+/* Library version string. */
+const char* dtr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DETRAY_AMD_H */

@@ -1,0 +1,1347 @@
+// Oracle: scalar FP64 CPU restatement of detray::propagator::propagate.
+//
+// TEST INFRASTRUCTURE.  This file restates the reference algorithm in plain
+// C++ so that the device path (detray_amd) can be checked against it.  Each
+// function cites the reference file:line it follows.  It is only loaded by
+// tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
+//
+// Scope (the propagation chain of tests/include/detray/test/device/
+// propagator_test.hpp:87-99 plus surface_sequencer):
+//   rk_stepper<field, algebra, constrained_step, stepper_rk_policy>
+//   caching_navigator<detector, 8>
+//   actor_chain<pathlimit_aborter,
+//               parameter_updater<pointwise_material_interactor>,
+//               surface_sequencer>
+// No volume material (toy / telescope detectors have none), so d(qop)/ds = 0
+// inside volumes (rk_stepper.hpp:148-188 returns 0 without material).
+
+#include "oracle.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+
+constexpr double kMax = std::numeric_limits<double>::max();
+constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr uint16_t kEndOfWorld = 0xffffu;
+constexpr int kCache = 8;  // navigation::default_cache_size
+
+// ----------------------------------------------------------------------------
+// Small vector algebra (generic_vector.hpp, array_vector.hpp)
+// ----------------------------------------------------------------------------
+struct V3 {
+    double v[3];
+    double& operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+};
+inline V3 add(const V3& a, const V3& b) { return {{a[0] + b[0], a[1] + b[1], a[2] + b[2]}}; }
+inline V3 sub(const V3& a, const V3& b) { return {{a[0] - b[0], a[1] - b[1], a[2] - b[2]}}; }
+inline V3 mul(double s, const V3& a) { return {{s * a[0], s * a[1], s * a[2]}}; }
+// array_vector.hpp:104-116: tmp products summed left to right from 0
+inline double dot(const V3& a, const V3& b) {
+    double r = 0.0;
+    r += a[0] * b[0];
+    r += a[1] * b[1];
+    r += a[2] * b[2];
+    return r;
+}
+inline V3 cross(const V3& a, const V3& b) {
+    return {{a[1] * b[2] - b[1] * a[2], a[2] * b[0] - b[2] * a[0], a[0] * b[1] - b[0] * a[1]}};
+}
+inline double norm(const V3& a) { return std::sqrt(dot(a, a)); }
+// generic_vector.hpp:35-55
+inline double perp(const V3& a) { return std::sqrt(std::fma(a[0], a[0], a[1] * a[1])); }
+inline double phi(const V3& a) { return std::atan2(a[1], a[0]); }
+inline double theta(const V3& a) { return std::atan2(perp(a), a[2]); }
+inline V3 normalize(const V3& a) {
+    const double n = norm(a);
+    return {{a[0] / n, a[1] / n, a[2] / n}};
+}
+
+// Transform helpers: global = R local + t
+inline V3 trf_axis(const dtr_transform& t, int c) { return {{t.r[c], t.r[3 + c], t.r[6 + c]}}; }
+inline V3 trf_translation(const dtr_transform& t) { return {{t.t[0], t.t[1], t.t[2]}}; }
+inline V3 point_to_local(const dtr_transform& t, const V3& p) {
+    const V3 d = sub(p, trf_translation(t));
+    return {{t.r[0] * d[0] + t.r[3] * d[1] + t.r[6] * d[2],
+             t.r[1] * d[0] + t.r[4] * d[1] + t.r[7] * d[2],
+             t.r[2] * d[0] + t.r[5] * d[1] + t.r[8] * d[2]}};
+}
+inline V3 point_to_global(const dtr_transform& t, const V3& l) {
+    return {{t.r[0] * l[0] + t.r[1] * l[1] + t.r[2] * l[2] + t.t[0],
+             t.r[3] * l[0] + t.r[4] * l[1] + t.r[5] * l[2] + t.t[1],
+             t.r[6] * l[0] + t.r[7] * l[1] + t.r[8] * l[2] + t.t[2]}};
+}
+
+// ----------------------------------------------------------------------------
+// Field (tests/include/detray/test/common/bfield.hpp:35-47: constant or
+// affine -> linear -> clamp -> strided xyz grid)
+// ----------------------------------------------------------------------------
+struct Field {
+    const dtr_field_desc* d;
+    V3 at(const V3& p) const {
+        if (d->kind == DTR_FIELD_CONSTANT) return {{d->b[0], d->b[1], d->b[2]}};
+        int i0[3];
+        double t[3];
+        for (int a = 0; a < 3; ++a) {
+            const int n = static_cast<int>(d->n[a]);
+            double u = (p[a] - d->min[a]) / d->pitch[a];
+            u = u < 0.0 ? 0.0 : (u > n - 1 ? static_cast<double>(n - 1) : u);
+            int i = static_cast<int>(std::floor(u));
+            i = i > n - 2 ? n - 2 : i;
+            i0[a] = i;
+            t[a] = u - i;
+        }
+        const uint64_t ny = d->n[1], nz = d->n[2];
+        V3 out{{0, 0, 0}};
+        for (int c = 0; c < 3; ++c) {
+            auto v = [&](int dx, int dy, int dz) {
+                const uint64_t idx = ((static_cast<uint64_t>(i0[0] + dx) * ny + (i0[1] + dy)) * nz +
+                                      (i0[2] + dz));
+                return static_cast<double>(d->values[3 * idx + c]);
+            };
+            const double c00 = v(0, 0, 0) * (1.0 - t[0]) + v(1, 0, 0) * t[0];
+            const double c01 = v(0, 0, 1) * (1.0 - t[0]) + v(1, 0, 1) * t[0];
+            const double c10 = v(0, 1, 0) * (1.0 - t[0]) + v(1, 1, 0) * t[0];
+            const double c11 = v(0, 1, 1) * (1.0 - t[0]) + v(1, 1, 1) * t[0];
+            const double c0 = c00 * (1.0 - t[1]) + c10 * t[1];
+            const double c1 = c01 * (1.0 - t[1]) + c11 * t[1];
+            out[c] = c0 * (1.0 - t[2]) + c1 * t[2];
+        }
+        return out;
+    }
+};
+
+// ----------------------------------------------------------------------------
+// Detector view
+// ----------------------------------------------------------------------------
+struct Det {
+    const dtr_detector_desc* d;
+    const dtr_surface& sf(uint32_t i) const { return d->surfaces[i]; }
+    const dtr_transform& trf(uint32_t s) const { return d->transforms[d->surfaces[s].transform]; }
+    const dtr_mask& mask(uint32_t m) const { return d->masks[m]; }
+};
+
+// ----------------------------------------------------------------------------
+// Particle (pdg_particle.hpp:61-91): muon / antimuon
+// ----------------------------------------------------------------------------
+struct Particle {
+    double mass;
+    double charge;
+    int pdg;
+};
+inline Particle muon_for(double qop) {
+    const double m = static_cast<double>(105.6583755f * 1e-3f);
+    Particle mu{m, -1.0, 13};
+    if (mu.charge * qop > 0.0) return mu;
+    return {m, 1.0, -13};  // charge_conjugation
+}
+
+// ----------------------------------------------------------------------------
+// Intersections (intersection.hpp:30-35, 190-430)
+// ----------------------------------------------------------------------------
+enum IStatus : uint8_t { kInside = 0, kEdge = 1, kOutside = 2 };
+
+struct Cand {
+    double path = kMax;
+    uint32_t sf = kInvalid;  // surface index, invalid for never-filled slots
+    uint16_t vol_link = 0;
+    uint8_t status = kOutside;
+    uint8_t dir = 1;
+    bool probably_inside() const { return status <= kEdge; }
+};
+
+enum NavStatus : int8_t {
+    kAbort = -3, kExit = -2, kUnknown = -1, kTowards = 0, kOnObject = 1, kOnPortal = 2
+};
+enum Trust : uint8_t { kNoTrust = 0, kFair = 1, kHigh = 3, kFull = 4 };
+
+struct NavCfg {  // intersection::config + navigation::config (floats, as reference)
+    float min_mask_tol, max_mask_tol, mask_scalor, path_tol, overstep_tol;
+    uint32_t win[2];
+};
+
+struct Nav {
+    Cand c[kCache];
+    int8_t next = 0, last = -1;
+    int8_t status = kUnknown;
+    uint8_t trust = kNoTrust;
+    int8_t direction = 1;
+    uint16_t volume = 0;
+    double ext_tol = 0.0;
+    bool exited_path_limit = false;
+
+    bool is_alive() const { return status > kUnknown; }
+    bool on_surface() const { return status == kOnObject || status == kOnPortal; }
+    bool on_portal() const { return status == kOnPortal; }
+    int n_candidates() const { return last - next + 1; }
+    bool cache_exhausted() const { return n_candidates() == 0; }
+    Cand& current() { return c[next - 1]; }
+    const Cand& current() const { return c[next - 1]; }
+    Cand& target() { return c[next]; }
+    const Cand& target() const { return c[next]; }
+    // navigation_state.hpp begin()/end()
+    int begin_idx() const { return (on_surface() && next >= 1) ? next - 1 : next; }
+    int end_idx() const { return last + 1; }
+    double operator()() const { return static_cast<double>(direction) * target().path; }
+    // caching_navigator.hpp:126-146 trust setters only lower the level
+    void set_high_trust() { trust = trust < kHigh ? trust : static_cast<uint8_t>(kHigh); }
+    void set_fair_trust() { trust = trust < kFair ? trust : static_cast<uint8_t>(kFair); }
+    void set_no_trust() { trust = kNoTrust; }
+    void set_volume(uint16_t v) {
+        if (v != volume) set_no_trust();
+        volume = v;
+    }
+    void clear_cache() {
+        for (int i = 0; i < kCache; ++i) c[i].path = kMax;
+        next = 0;
+        last = -1;
+    }
+    void exit_nav() { status = kExit; }
+    void abort_nav() { status = kAbort; }
+};
+
+// ----------------------------------------------------------------------------
+// Masks (shapes/*.hpp check_boundaries; mask.hpp:173-192)
+// ----------------------------------------------------------------------------
+// Returns {inside, inside_with_edge}
+inline void check_planar(const dtr_mask& m, const dtr_transform& t, const V3& glob,
+                         double tol, double edge_tol, bool& inside, bool& edge) {
+    const V3 loc = point_to_local(t, glob);
+    if (m.shape == DTR_RECTANGLE2D) {
+        // rectangle2D.hpp:104-125
+        const double l0 = std::fabs(loc[0]), l1 = std::fabs(loc[1]);
+        inside = (l0 <= (m.v[0] + tol) && l1 <= (m.v[1] + tol));
+        edge = false;
+        if (edge_tol > 0.0) {
+            const double ft = tol + edge_tol;
+            edge = (l0 <= (m.v[0] + ft) && l1 <= (m.v[1] + ft));
+        }
+    } else if (m.shape == DTR_TRAPEZOID2D) {
+        // trapezoid2D.hpp:121-150
+        const double rel_y = (m.v[2] + loc[1]) * m.v[3];
+        const double bound_x = m.v[0] + rel_y * (m.v[1] - m.v[0]);
+        inside = (std::fabs(loc[0]) <= (bound_x + tol) && std::fabs(loc[1]) <= (m.v[2] + tol));
+        edge = false;
+        if (edge_tol > 0.0) {
+            const double ft = tol + edge_tol;
+            edge = (std::fabs(loc[0]) <= (bound_x + ft) && std::fabs(loc[1]) <= (m.v[2] + ft));
+        }
+    } else {
+        // ring2D.hpp:85-120: radius of the point in the local cartesian frame
+        const double r = perp(loc);
+        inside = ((r + tol) >= m.v[0] && r <= (m.v[1] + tol));
+        edge = inside;
+    }
+}
+
+// intersector_base.hpp:171-252 resolve_mask (ip.point is global 3D for
+// planar intersectors and (inv, z) for the concentric cylinder)
+inline void resolve_mask(Cand& is, double path, const V3& glob, double z2d,
+                         uint32_t sf_idx, const dtr_surface& sf, const dtr_mask& m,
+                         const dtr_transform& t, const NavCfg& cfg, double ext_tol_in) {
+    is.status = kOutside;
+    if (!(path >= static_cast<double>(cfg.overstep_tol))) return;
+    double base_tol = 0.0, ext_tol = 0.0;
+    if (sf.sf_id != DTR_PORTAL || cfg.min_mask_tol == std::numeric_limits<float>::max()) {
+        ext_tol = ext_tol_in;
+        base_tol = std::max(static_cast<double>(cfg.min_mask_tol),
+                            std::min(static_cast<double>(cfg.max_mask_tol),
+                                     static_cast<double>(cfg.mask_scalor) * std::fabs(path)));
+    }
+    bool inside, edge;
+    if (m.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        // concentric_cylinder2D.hpp:110-120 (z only)
+        inside = (m.v[1] - base_tol <= z2d && z2d <= m.v[2] + base_tol);
+        edge = inside;
+    } else {
+        check_planar(m, t, glob, base_tol, ext_tol, inside, edge);
+    }
+    if (edge) is.status = kEdge;
+    if (inside) is.status = kInside;
+    is.path = path;
+    is.sf = sf_idx;
+    is.dir = !std::signbit(path);
+    is.vol_link = m.volume_link;
+}
+
+// ray_plane_intersector.hpp:60-87.  Returns validity (|s| < 10 m).
+inline bool plane_intersect(const V3& ro, const V3& rd, const dtr_transform& t,
+                            double& s, V3& glob) {
+    const V3 sn = trf_axis(t, 2);
+    const V3 st = trf_translation(t);
+    const double denom = dot(rd, sn);
+    if (denom == 0.0) {
+        s = kMax;
+        return false;
+    }
+    s = dot(sn, sub(st, ro)) / denom;
+    glob = add(ro, mul(s, rd));
+    return std::fabs(s) < 10.0 * 1000.0;
+}
+
+// ray_cylinder_portal_intersector.hpp:68-132
+inline bool cyl_portal_intersect(const V3& ro, const V3& rd, double r, double overstep,
+                                 double& path, double& z) {
+    const double inv = kMax;
+    path = inv;
+    const double rd_perp_2 = rd[0] * rd[0] + rd[1] * rd[1];
+    if (rd_perp_2 < std::numeric_limits<double>::epsilon()) {
+        z = 0.0;
+        return false;
+    }
+    const double ro_perp_2 = ro[0] * ro[0] + ro[1] * ro[1];
+    const double rad_diff = r * r - ro_perp_2;
+    if (std::fabs(rad_diff) > 1.0 * 1e-3) {
+        const double rd_perp_inv_2 = 1.0 / rd_perp_2;
+        const double k = -rd_perp_inv_2 * (ro[0] * rd[0] + ro[1] * rd[1]);
+        const double discr = rd_perp_inv_2 * rad_diff + k * k;
+        if (discr < 0.0) {
+            z = 0.0;
+            return false;
+        }
+        const double sq = std::sqrt(discr);
+        const double s1 = k + sq, s2 = k - sq;
+        path = (std::fabs(s1) < std::fabs(s2)) ? s1 : s2;
+        if (path < overstep && sq > 0.0) path = (std::fabs(s1) >= std::fabs(s2)) ? s1 : s2;
+    } else {
+        path = rad_diff;
+    }
+    if (path < overstep) path = inv;
+    z = ro[2] + path * rd[2];
+    return std::fabs(path) < 10.0 * 1000.0;
+}
+
+// Compute the intersection of a ray with a surface (the point of
+// intersection used by all of its masks).
+struct IPoint {
+    double path;
+    V3 glob;
+    double z;
+};
+inline bool intersect_surface(const Det& det, uint32_t sf_idx, const V3& ro, const V3& rd,
+                              double overstep, IPoint& ip) {
+    const dtr_surface& s = det.sf(sf_idx);
+    const dtr_transform& t = det.trf(sf_idx);
+    if (s.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        const double r = det.mask(s.mask).v[0];
+        return cyl_portal_intersect(ro, rd, r, overstep, ip.path, ip.z);
+    }
+    return plane_intersect(ro, rd, t, ip.path, ip.glob);
+}
+
+struct Engine;  // forward
+
+// ----------------------------------------------------------------------------
+// The propagation engine for one track
+// ----------------------------------------------------------------------------
+struct PropCfg {
+    // stepping
+    double min_stepsize, rk_error_tol, step_constraint, path_limit;
+    uint32_t max_rk_updates;
+    bool do_cov;
+    // navigation
+    NavCfg nav;
+    double accumulated_error;
+    int n_stddev;
+    bool estimate_noise;
+    // interactor
+    bool do_eloss, do_ms, use_material;
+    uint32_t max_surfaces;
+};
+
+struct Stepper {
+    // free track parameters
+    V3 pos, dir;
+    double time, qop;
+    // transport jacobian, dense 8x8 row-major
+    double J[64];
+    Particle ptc;
+    uint64_t n_trials = 0;
+    uint32_t n_steps = 0;
+    double step_size = 0.0, next_step_size = 0.0;
+    double path_length = 0.0, abs_path_length = 0.0;
+    double constraints[4] = {kMax, kMax, kMax, kMax};  // accuracy, actor, aborter, user
+    V3 dtds3{{0, 0, 0}};
+    double dqopds3 = 0.0;
+
+    void reset_J() {
+        for (int i = 0; i < 64; ++i) J[i] = 0.0;
+        for (int i = 0; i < 8; ++i) J[i * 8 + i] = 1.0;
+    }
+    // constrained_step.hpp:74-125
+    void set_constraint(int type, double s) { constraints[type] = std::min(constraints[type], std::fabs(s)); }
+    double constraint_size() const {
+        double m = kMax;
+        for (int i = 0; i < 4; ++i) m = std::min(m, constraints[i]);
+        const double dir_sign = step_size >= 0.0 ? 1.0 : -1.0;
+        return dir_sign * m;
+    }
+};
+
+struct Updater {
+    double bound[6];
+    double cov[36];
+    uint32_t sf = kInvalid;  // surface link (invalid: curvilinear/free start)
+};
+
+struct Engine {
+    Det det;
+    Field field;
+    PropCfg cfg;
+    Nav nav;
+    Stepper st;
+    Updater up;
+    bool heartbeat = false;
+    const double* init_cov = nullptr;  // 21 values or null
+    // sequencer
+    uint32_t* seq = nullptr;
+    uint32_t seq_stride = 0;
+    uint32_t n_seq = 0;
+    bool overflow = false;
+    // trace
+    uint32_t* tr_vol = nullptr;
+    uint32_t* tr_sf = nullptr;
+    uint32_t tr_cap = 0, tr_n = 0;
+
+    // ---------------------------------------------------------------- nav --
+    bool is_portal(uint32_t s) const { return det.sf(s).sf_id == DTR_PORTAL; }
+
+    void insert(int pos, const Cand& nc) {
+        // caching_navigator.hpp:149-229
+        if (pos == kCache) return;
+        if (nav.n_candidates() == 0) {
+            nav.c[0] = nc;
+            nav.last = static_cast<int8_t>(nav.last + 1);
+            return;
+        }
+        int idx = pos;
+        auto overlap = [&](int i) { return std::fabs(nav.c[i].path - nc.path) <= 1.0 * 1e-3; };
+        auto clash = [&](int i) {
+            const uint64_t a = nav.c[i].sf == kInvalid ? ~0ull : det.sf(nav.c[i].sf).barcode;
+            return a == det.sf(nc.sf).barcode && overlap(i);
+        };
+        if (clash(idx) || (idx > 0 && clash(idx - 1))) return;
+        const int shift_max = kCache - 2;
+        const int shift_begin = std::min(static_cast<int>(nav.last), shift_max);
+        if (overlap(idx) && (nc.status == kEdge || is_portal(nc.sf))) {
+            idx++;
+            idx = idx > shift_begin ? shift_begin : idx;
+        }
+        for (int i = shift_begin; i >= idx; --i) nav.c[i + 1] = nav.c[i];
+        nav.c[idx] = nc;
+        nav.last = static_cast<int8_t>(std::min(nav.last + 1, kCache - 1));
+    }
+
+    // intersection_kernel.hpp:24-168 intersection_initialize + insert_sorted
+    void candidate_search(uint32_t sf_idx, const NavCfg& icfg) {
+        const V3 rd = mul(static_cast<double>(nav.direction), st.dir);
+        IPoint ip;
+        if (!intersect_surface(det, sf_idx, st.pos, rd, icfg.overstep_tol, ip)) return;
+        const dtr_surface& s = det.sf(sf_idx);
+        const dtr_transform& t = det.trf(sf_idx);
+        for (uint32_t k = 0; k < s.n_masks; ++k) {
+            Cand is;
+            resolve_mask(is, ip.path, ip.glob, ip.z, sf_idx, s, det.mask(s.mask + k), t, icfg,
+                         nav.ext_tol);
+            if (is.probably_inside()) {
+                // upper_bound over [begin, end) with |path| ordering
+                int p = nav.begin_idx();
+                const int e = nav.end_idx();
+                while (p < e && !(std::fabs(is.path) < std::fabs(nav.c[p].path))) ++p;
+                insert(p, is);
+                return;
+            }
+        }
+    }
+
+    // navigation_functions.hpp:99-129 update_candidate + intersection_update
+    bool update_candidate(Cand& cand) {
+        if (cand.sf == kInvalid) return false;
+        const V3 rd = mul(static_cast<double>(nav.direction), st.dir);
+        IPoint ip;
+        if (!intersect_surface(det, cand.sf, st.pos, rd, cfg.nav.overstep_tol, ip)) return false;
+        const dtr_surface& s = det.sf(cand.sf);
+        const dtr_transform& t = det.trf(cand.sf);
+        for (uint32_t k = 0; k < s.n_masks; ++k) {
+            resolve_mask(cand, ip.path, ip.glob, ip.z, cand.sf, s, det.mask(s.mask + k), t,
+                         cfg.nav, nav.ext_tol);
+            if (cand.probably_inside()) return true;
+        }
+        return false;
+    }
+
+    // spatial_grid.hpp:100-202 + axis.hpp:417-424 + axis_binning.hpp:80-115
+    void grid_search(const dtr_volume& vol, const NavCfg& icfg) {
+        const dtr_grid& g = det.d->grids[vol.grid];
+        const dtr_transform& vt = det.d->transforms[vol.transform];
+        double l0, l1;
+        if (g.type == DTR_GRID_CYLINDER) {
+            double path, z;
+            cyl_portal_intersect(st.pos, st.dir, g.radius, -kMax, path, z);
+            const V3 p = add(st.pos, mul(path, st.dir));
+            l0 = phi(p);
+            l1 = z;
+        } else {
+            double s;
+            V3 glob{{0, 0, 0}};
+            plane_intersect(st.pos, st.dir, vt, s, glob);
+            const V3 loc = point_to_local(vt, glob);
+            l0 = perp(loc);
+            l1 = phi(loc);
+        }
+        const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
+        const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
+        const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
+        const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
+        int r0[2] = {ib0 - static_cast<int>(icfg.win[0]), ib0 + static_cast<int>(icfg.win[1])};
+        int r1[2] = {ib1 - static_cast<int>(icfg.win[0]), ib1 + static_cast<int>(icfg.win[1])};
+        auto closed = [](int lb, int ub, int bins, int out[2]) {
+            int mn = (lb > 0) ? lb : 0;
+            mn = (mn >= bins) ? bins - 1 : mn;
+            int mx = (ub >= bins) ? bins - 1 : ub;
+            mx = (mx < 0) ? 0 : mx;
+            out[0] = mn;
+            out[1] = mx + 1;
+        };
+        auto circular = [](int lb, int ub, int out[2]) {
+            out[0] = lb;
+            out[1] = ub + 1;
+        };
+        int a0[2], a1[2];
+        const bool circ0 = (g.type == DTR_GRID_CYLINDER);
+        if (circ0) {
+            circular(r0[0], r0[1], a0);
+            closed(r1[0], r1[1], static_cast<int>(g.n1), a1);
+        } else {
+            closed(r0[0], r0[1], static_cast<int>(g.n0), a0);
+            circular(r1[0], r1[1], a1);
+        }
+        auto wrap = [](int b, int n) { return (n + (b % n)) % n; };
+        for (int b0 = a0[0]; b0 < a0[1]; ++b0) {
+            for (int b1 = a1[0]; b1 < a1[1]; ++b1) {
+                const int i0 = circ0 ? wrap(b0, static_cast<int>(g.n0)) : b0;
+                const int i1 = circ0 ? b1 : wrap(b1, static_cast<int>(g.n1));
+                const uint32_t s = det.d->grid_bins[g.bins + static_cast<uint64_t>(i0) * g.n1 + i1];
+                if (s != kInvalid) candidate_search(s, icfg);
+            }
+        }
+    }
+
+    // navigation_functions.hpp:147-177
+    void update_status() {
+        if (!nav.cache_exhausted() &&
+            std::fabs(nav.target().path) < static_cast<double>(cfg.nav.path_tol)) {
+            nav.status = is_portal(nav.target().sf) ? kOnPortal : kOnObject;
+            nav.next = static_cast<int8_t>(nav.next + 1);
+        } else {
+            nav.status = kTowards;
+        }
+        nav.trust = (nav.cache_exhausted() || nav.on_portal()) ? kNoTrust : kFull;
+    }
+
+    // navigation_functions.hpp:193-246
+    void local_navigation(const NavCfg& ncfg, bool resolve_overstepping) {
+        const dtr_volume& vol = det.d->volumes[nav.volume];
+        nav.clear_cache();
+        NavCfg icfg = ncfg;
+        icfg.overstep_tol = resolve_overstepping ? ncfg.overstep_tol : -ncfg.path_tol;
+        // tracking_volume.hpp:263-278: brute force (portals/passives) first,
+        // then the sensitive-surface grid
+        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.d->bf_surfaces[k], icfg);
+        if (vol.grid >= 0) grid_search(vol, icfg);
+        update_status();
+        if (nav.trust != kFull) {
+            if (nav.on_portal()) {
+                nav.trust = (nav.current().vol_link == kEndOfWorld) ? kFull : kNoTrust;
+            }
+        }
+    }
+
+    // navigation_functions.hpp:264-288
+    void volume_switch() {
+        if (nav.current().vol_link == kEndOfWorld) {
+            nav.exit_nav();
+            return;
+        }
+        nav.set_volume(nav.current().vol_link);
+        local_navigation(cfg.nav, true);
+        nav.trust = kFull;
+    }
+
+    // navigation_functions.hpp:304-331
+    void init_loose_cfg() {
+        NavCfg loose = cfg.nav;
+        loose.overstep_tol = std::min(100.f * loose.overstep_tol, -10.f * loose.max_mask_tol);
+        local_navigation(loose, true);
+        if (nav.trust != kFull || nav.cache_exhausted()) nav.abort_nav();
+    }
+
+    // caching_navigator.hpp:250-388
+    bool update_impl() {
+        const bool is_init = true;
+        if (nav.trust == kHigh) {
+            if (!update_candidate(nav.target())) {
+                nav.status = kUnknown;
+                nav.set_fair_trust();
+            } else {
+                update_status();
+                if (nav.status == kTowards || nav.on_portal()) return !is_init;
+                if (nav.trust == kFull && update_candidate(nav.target())) return !is_init;
+                nav.set_fair_trust();
+            }
+        }
+        if (nav.trust == kFair && !nav.cache_exhausted()) {
+            const int b = nav.begin_idx(), e = nav.end_idx();
+            for (int i = b; i < e; ++i) {
+                if (!update_candidate(nav.c[i])) nav.c[i].path = kMax;
+            }
+            // sort.hpp:41-58 sequential (selection) sort over [b, e)
+            for (int i = b; i < e; ++i) {
+                for (int j = i + 1; j < e; ++j) {
+                    if (std::fabs(nav.c[j].path) < std::fabs(nav.c[i].path)) {
+                        const Cand tmp = nav.c[i];
+                        nav.c[i] = nav.c[j];
+                        nav.c[j] = tmp;
+                    }
+                }
+            }
+            nav.next = static_cast<int8_t>(b);
+            int inv = kCache;
+            for (int i = 0; i < kCache; ++i) {
+                if (nav.c[i].path == kMax) {
+                    inv = i;
+                    break;
+                }
+            }
+            nav.last = static_cast<int8_t>(inv - 1);
+            update_status();
+            if (nav.cache_exhausted()) nav.set_no_trust();
+        }
+        if (nav.trust == kNoTrust) {
+            local_navigation(cfg.nav, true);
+            return is_init;
+        }
+        return !is_init;
+    }
+
+    // navigator_base.hpp:97-154
+    bool nav_update() {
+        if (nav.trust == kFull) return false;
+        bool is_init = update_impl();
+        if (nav.on_portal()) {
+            volume_switch();
+            is_init = true;
+            if (!nav.is_alive()) return false;
+        }
+        if (nav.trust != kFull || nav.cache_exhausted()) {
+            is_init = true;
+            init_loose_cfg();
+        }
+        return is_init;
+    }
+
+    // ------------------------------------------------------------ stepper --
+    // rk_stepper.hpp:569-590
+    static void dtds_of(const V3& dir, const V3& b, double h, const V3& prev, bool first,
+                        double qop, V3& t, V3& k) {
+        t = (first || h == 0.0) ? dir : add(dir, mul(h, prev));
+        k = mul(qop, cross(t, b));
+    }
+
+    // rk_stepper.hpp:273-530 (no volume material)
+    bool step(double dist_to_next, bool do_reset) {
+        const PropCfg& c = cfg;
+        if (std::fabs(dist_to_next) < static_cast<double>(1.f * 1e-3f)) {
+            if (std::fabs(st.next_step_size) < c.min_stepsize)
+                st.next_step_size = std::copysign(c.min_stepsize, dist_to_next);
+            return true;
+        }
+        if (do_reset) {
+            st.step_size = dist_to_next;
+        } else if (st.next_step_size > 0) {
+            st.step_size = std::min(st.next_step_size, dist_to_next);
+        } else {
+            st.step_size = std::max(st.next_step_size, dist_to_next);
+        }
+        const double min_step = std::min(std::fabs(st.step_size), c.min_stepsize);
+        const V3 pos = st.pos;
+        V3 b_first = field.at(pos), b_mid{{0, 0, 0}}, b_last{{0, 0, 0}};
+        double qop[4];
+        V3 t[4], k[4];
+        qop[0] = st.qop;
+        dtds_of(st.dir, b_first, 0.0, V3{{0, 0, 0}}, true, qop[0], t[0], k[0]);
+        auto estimate_error = [&](double h) {
+            const double h2 = h * h;
+            const double half_h = h * 0.5;
+            const V3 pos1 = add(add(pos, mul(half_h, t[0])), mul(h2 * 0.125, k[0]));
+            b_mid = field.at(pos1);
+            qop[1] = st.qop;
+            dtds_of(st.dir, b_mid, half_h, k[0], false, qop[1], t[1], k[1]);
+            qop[2] = st.qop;
+            dtds_of(st.dir, b_mid, half_h, k[1], false, qop[2], t[2], k[2]);
+            const V3 pos2 = add(add(pos, mul(h, t[0])), mul(h2 * 0.5, k[2]));
+            b_last = field.at(pos2);
+            qop[3] = st.qop;
+            dtds_of(st.dir, b_last, h, k[2], false, qop[3], t[3], k[3]);
+            const double one_sixth = 1.0 / 6.0;
+            const V3 d = add(sub(sub(k[0], k[1]), k[2]), k[3]);
+            const V3 err = mul(one_sixth * h2, d);
+            return norm(err);
+        };
+        auto scaling = [&](double err) {
+            return std::min(std::max(std::sqrt(std::sqrt(c.rk_error_tol / err)), 0.25), 4.0);
+        };
+        double error = 1e20;
+        for (uint32_t i = 0; i < c.max_rk_updates; ++i) {
+            st.n_trials++;
+            error = std::max(estimate_error(st.step_size), 1e-20);
+            if (error <= 4.0 * c.rk_error_tol) break;
+            st.step_size = st.step_size * scaling(error);
+        }
+        st.dtds3 = k[3];
+        st.dqopds3 = 0.0;
+        const double max_step = st.constraint_size();
+        if (std::fabs(st.step_size) > std::fabs(max_step)) st.step_size = max_step;
+        if (std::fabs(st.step_size) < min_step) st.step_size = std::copysign(min_step, st.step_size);
+        st.next_step_size = st.step_size * scaling(error);
+        if (std::fabs(st.next_step_size) < c.min_stepsize)
+            st.next_step_size = std::copysign(c.min_stepsize, st.next_step_size);
+        advance_track(t, k);
+        if (c.do_cov) advance_jacobian(t, k, qop, b_first, b_mid, b_last);
+        st.n_steps++;
+        return true;
+    }
+
+    // rk_stepper.hpp:594-629
+    void advance_track(const V3 t[4], const V3 k[4]) {
+        const double h = st.step_size;
+        const double h_6 = h * (1.0 / 6.0);
+        st.pos = add(st.pos, mul(h, add(t[0], mul(h_6, add(add(k[0], k[1]), k[2])))));
+        V3 d = add(st.dir, mul(h_6, add(add(k[0], mul(2.0, add(k[1], k[2]))), k[3])));
+        st.dir = normalize(d);
+        st.path_length += h;
+        st.abs_path_length += std::fabs(h);
+    }
+
+    static void cwc(const double M[9], const V3& b, double out[9]) {
+        // column-wise cross product: column j of out = (column j of M) x b
+        for (int j = 0; j < 3; ++j) {
+            const V3 col{{M[0 * 3 + j], M[1 * 3 + j], M[2 * 3 + j]}};
+            const V3 r = cross(col, b);
+            out[0 * 3 + j] = r[0];
+            out[1 * 3 + j] = r[1];
+            out[2 * 3 + j] = r[2];
+        }
+    }
+
+    // rk_stepper.hpp:633-956 with dqop_n/dqop1 = 1 (no volume material) and
+    // without field gradient; J <- D(h) J with the ATL-SOFT-PUB-2009-002 blocks
+    void advance_jacobian(const V3 t[4], const V3 k[4], const double qop[4], const V3& b1,
+                          const V3& b2, const V3& b3) {
+        (void)k;
+        const double h = st.step_size;
+        const double half_h = h * 0.5;
+        // rk_stepper.hpp:664 uses the single-precision literal 1.f / 6.f here
+        const double h_6 = h * static_cast<double>(1.f / 6.f);
+        // dk_n / dqop1
+        V3 dk[4];
+        dk[0] = mul(1.0, cross(t[0], b1));
+        dk[1] = add(mul(1.0, cross(t[1], b2)), mul(qop[1] * half_h, cross(dk[0], b2)));
+        dk[2] = add(mul(1.0, cross(t[2], b2)), mul(qop[2] * half_h, cross(dk[1], b2)));
+        dk[3] = add(mul(1.0, cross(t[3], b3)), mul(qop[3] * h, cross(dk[2], b3)));
+        const V3 dFdqop = mul(h * h_6, add(add(dk[0], dk[1]), dk[2]));
+        const V3 dGdqop = mul(h_6, add(add(dk[0], mul(2.0, add(dk[1], dk[2]))), dk[3]));
+        const double dqopqop = 1.0;
+        // dk_n / dt1
+        double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        double dkdt[4][9];
+        double tmp[9];
+        cwc(I, b1, tmp);
+        for (int i = 0; i < 9; ++i) dkdt[0][i] = qop[0] * tmp[i];
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * dkdt[0][i];
+        cwc(tmp, b2, dkdt[1]);
+        for (int i = 0; i < 9; ++i) dkdt[1][i] *= qop[1];
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * dkdt[1][i];
+        cwc(tmp, b2, dkdt[2]);
+        for (int i = 0; i < 9; ++i) dkdt[2][i] *= qop[2];
+        for (int i = 0; i < 9; ++i) tmp[i] = I[i] + h * dkdt[2][i];
+        cwc(tmp, b3, dkdt[3]);
+        for (int i = 0; i < 9; ++i) dkdt[3][i] *= qop[3];
+        double dFdt[9], dGdt[9];
+        for (int i = 0; i < 9; ++i) {
+            dFdt[i] = h * (I[i] + h_6 * (dkdt[0][i] + dkdt[1][i] + dkdt[2][i]));
+            dGdt[i] = I[i] + h_6 * (dkdt[0][i] + 2.0 * (dkdt[1][i] + dkdt[2][i]) + dkdt[3][i]);
+        }
+        // D (8x8): rows/cols [pos 0-2, t 3, dir 4-6, qop 7]
+        double D[64] = {0};
+        for (int i = 0; i < 8; ++i) D[i * 8 + i] = 1.0;
+        for (int r = 0; r < 3; ++r) {
+            for (int cc = 0; cc < 3; ++cc) {
+                D[r * 8 + 4 + cc] = dFdt[r * 3 + cc];
+                D[(4 + r) * 8 + 4 + cc] = dGdt[r * 3 + cc];
+            }
+            D[r * 8 + 7] = dFdqop[r];
+            D[(4 + r) * 8 + 7] = dGdqop[r];
+        }
+        D[7 * 8 + 7] = dqopqop;
+        double NJ[64];
+        for (int r = 0; r < 8; ++r) {
+            for (int cc = 0; cc < 8; ++cc) {
+                double s = 0.0;
+                for (int m = 0; m < 8; ++m) s += D[r * 8 + m] * st.J[m * 8 + cc];
+                NJ[r * 8 + cc] = s;
+            }
+        }
+        std::memcpy(st.J, NJ, sizeof(NJ));
+    }
+
+    // stepper_rk_policy (policies.hpp:99-139)
+    void rk_policy() {
+        if (std::fabs(nav()) <= static_cast<double>(1.f * 1e-3f)) {
+            nav.set_high_trust();
+            return;
+        }
+        const double rel = (st.step_size - nav()) / nav();
+        if (rel > 0.1) nav.set_no_trust();
+        else if (rel > 0.05) nav.set_fair_trust();
+        else nav.set_high_trust();
+    }
+
+    // ------------------------------------------------- frames & transforms --
+    // free_to_bound_vector (transform_track_parameters.hpp:24-38)
+    void free_to_bound(uint32_t s, const V3& pos, const V3& dir, double time, double qop,
+                       double out[6]) const {
+        const dtr_surface& sf = det.sf(s);
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            out[0] = phi(pos);
+            out[1] = pos[2];
+        } else if (sf.shape == DTR_RING2D) {
+            const V3 l = point_to_local(det.trf(s), pos);
+            out[0] = perp(l);
+            out[1] = phi(l);
+        } else {
+            const V3 l = point_to_local(det.trf(s), pos);
+            out[0] = l[0];
+            out[1] = l[1];
+        }
+        out[2] = phi(dir);
+        out[3] = theta(dir);
+        out[4] = qop;
+        out[5] = time;
+    }
+    static V3 bound_dir(const double b[6]) {
+        const double s = std::sin(b[3]);
+        return {{std::cos(b[2]) * s, std::sin(b[2]) * s, std::cos(b[3])}};
+    }
+    // bound_to_free_vector (transform_track_parameters.hpp:40-73)
+    void bound_to_free(uint32_t s, const double b[6], V3& pos, V3& dir, double& time,
+                       double& qop) const {
+        const dtr_surface& sf = det.sf(s);
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double r = det.mask(sf.mask).v[0];
+            pos = {{r * std::cos(b[0]), r * std::sin(b[0]), b[1]}};
+        } else if (sf.shape == DTR_RING2D) {
+            pos = point_to_global(det.trf(s), V3{{b[0] * std::cos(b[1]), b[0] * std::sin(b[1]), 0.0}});
+        } else {
+            pos = point_to_global(det.trf(s), V3{{b[0], b[1], 0.0}});
+        }
+        dir = bound_dir(b);
+        time = b[5];
+        qop = b[4];
+    }
+    V3 surface_normal(uint32_t s, const double b[6]) const {
+        const dtr_surface& sf = det.sf(s);
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
+            return {{std::cos(b[0]), std::sin(b[0]), 0.0}};
+        return trf_axis(det.trf(s), 2);
+    }
+    // dpos/dloc (3x2) jacobian_cartesian.hpp:69-79 / concentric_cylindrical
+    void dpos_dloc(uint32_t s, const V3& pos, double A[6]) const {
+        const dtr_surface& sf = det.sf(s);
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double r = perp(pos), ph = phi(pos);
+            A[0] = -r * std::sin(ph); A[1] = 0.0;
+            A[2] = r * std::cos(ph);  A[3] = 0.0;
+            A[4] = 0.0;               A[5] = 1.0;
+        } else {
+            const dtr_transform& t = det.trf(s);
+            A[0] = t.r[0]; A[1] = t.r[1];
+            A[2] = t.r[3]; A[3] = t.r[4];
+            A[4] = t.r[6]; A[5] = t.r[7];
+        }
+    }
+    // dloc/dpos (2x3)
+    void dloc_dpos(uint32_t s, const V3& pos, double A[6]) const {
+        const dtr_surface& sf = det.sf(s);
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double rinv = 1.0 / perp(pos), ph = phi(pos);
+            A[0] = -rinv * std::sin(ph); A[1] = rinv * std::cos(ph); A[2] = 0.0;
+            A[3] = 0.0;                  A[4] = 0.0;                 A[5] = 1.0;
+        } else {
+            const dtr_transform& t = det.trf(s);
+            A[0] = t.r[0]; A[1] = t.r[3]; A[2] = t.r[6];
+            A[3] = t.r[1]; A[4] = t.r[4]; A[5] = t.r[7];
+        }
+    }
+    // free_to_path derivative, pos part (jacobian_cartesian.hpp:54-66 and
+    // jacobian_concentric_cylindrical.hpp:45-67)
+    V3 path_derivative(uint32_t s, const V3& pos, const V3& dir) const {
+        const dtr_surface& sf = det.sf(s);
+        V3 n;
+        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double ph = phi(pos);
+            n = {{std::cos(ph), std::sin(ph), 0.0}};
+        } else {
+            n = trf_axis(det.trf(s), 2);
+        }
+        return mul(-1.0 / dot(n, dir), n);
+    }
+
+    // stepping.dtds() (rk_stepper.hpp:121-140)
+    V3 dtds() const {
+        if (st.path_length == 0.0) return mul(st.qop, cross(st.dir, field.at(st.pos)));
+        return st.dtds3;
+    }
+
+    // parameter_transporter::get_full_jacobian (parameter_updater.hpp:422-572)
+    void full_jacobian(uint32_t dest, double Jf[36]) const {
+        const uint32_t dep = up.sf;
+        V3 dpos, ddir;
+        double dt, dq;
+        bound_to_free(dep, up.bound, dpos, ddir, dt, dq);
+        // B2F 8x6
+        double B2F[48] = {0};
+        double A[6];
+        dpos_dloc(dep, dpos, A);
+        for (int r = 0; r < 3; ++r) {
+            B2F[r * 6 + 0] = A[r * 2 + 0];
+            B2F[r * 6 + 1] = A[r * 2 + 1];
+        }
+        {
+            const double th = up.bound[3], ph = up.bound[2];
+            const double ct = std::cos(th), stt = std::sin(th), cp = std::cos(ph), sp = std::sin(ph);
+            B2F[4 * 6 + 2] = -stt * sp;
+            B2F[4 * 6 + 3] = ct * cp;
+            B2F[5 * 6 + 2] = stt * cp;
+            B2F[5 * 6 + 3] = ct * sp;
+            B2F[6 * 6 + 3] = -stt;
+        }
+        B2F[3 * 6 + 5] = 1.0;
+        B2F[7 * 6 + 4] = 1.0;
+        // path correction
+        const V3 k = dtds();
+        double p2f[8] = {st.dir[0], st.dir[1], st.dir[2], 0.0, k[0], k[1], k[2], 0.0};
+        const V3 f2p_pos = path_derivative(dest, st.pos, st.dir);
+        double f2p[8] = {f2p_pos[0], f2p_pos[1], f2p_pos[2], 0, 0, 0, 0, 0};
+        double C[64];
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) C[r * 8 + c] = (r == c ? 1.0 : 0.0) + p2f[r] * f2p[c];
+        // F2B 6x8
+        double F2B[48] = {0};
+        double Bl[6];
+        dloc_dpos(dest, st.pos, Bl);
+        for (int c = 0; c < 3; ++c) {
+            F2B[0 * 8 + c] = Bl[0 * 3 + c];
+            F2B[1 * 8 + c] = Bl[1 * 3 + c];
+        }
+        {
+            const double th = theta(st.dir), ph = phi(st.dir);
+            const double ct = std::cos(th), stt = std::sin(th), cp = std::cos(ph), sp = std::sin(ph);
+            F2B[2 * 8 + 4] = -sp / stt;
+            F2B[2 * 8 + 5] = cp / stt;
+            F2B[3 * 8 + 4] = cp * ct;
+            F2B[3 * 8 + 5] = sp * ct;
+            F2B[3 * 8 + 6] = -stt;
+        }
+        F2B[4 * 8 + 7] = 1.0;
+        F2B[5 * 8 + 3] = 1.0;
+        // J_full = F2B * C * J * B2F
+        double T1[64], T2[48], T3[36];
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 8; ++m) s += C[r * 8 + m] * st.J[m * 8 + c];
+                T1[r * 8 + c] = s;
+            }
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 8; ++m) s += T1[r * 8 + m] * B2F[m * 6 + c];
+                T2[r * 6 + c] = s;
+            }
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) {
+                double s = 0.0;
+                for (int m = 0; m < 8; ++m) s += F2B[r * 8 + m] * T2[m * 6 + c];
+                T3[r * 6 + c] = s;
+            }
+        std::memcpy(Jf, T3, sizeof(T3));
+    }
+
+    // --------------------------------------------------------------- actors --
+    bool on_sensitive() const {
+        return nav.status == kOnObject && det.sf(nav.current().sf).sf_id == DTR_SENSITIVE;
+    }
+    bool encountered_material() const {
+        return nav.on_surface() && det.sf(nav.current().sf).material != kInvalid;
+    }
+
+    // aborters.hpp:44-75
+    void pathlimit_aborter() {
+        if (nav.status == kExit) return;
+        const double step_limit = cfg.path_limit - std::fabs(st.abs_path_length);
+        if (step_limit <= 0.0) {
+            nav.abort_nav();
+            nav.exited_path_limit = true;
+            heartbeat = false;
+        }
+        st.set_constraint(2, step_limit);
+    }
+
+    // relativistic_quantities.hpp + interaction.hpp
+    struct RQ {
+        double qOverP, q2OverBeta2, beta2, betaGamma, gamma, gamma2, E, Wmax;
+    };
+    static RQ make_rq(double mass, double qop, double q) {
+        RQ r;
+        r.qOverP = qop;
+        r.q2OverBeta2 = q * q + (mass * qop) * (mass * qop);
+        const double mOverP = mass * ((q != 0.0) ? std::fabs(qop / q) : std::fabs(qop));
+        const double pOverM = 1.0 / mOverP;
+        r.beta2 = 1.0 / (1.0 + mOverP * mOverP);
+        r.betaGamma = pOverM;
+        r.gamma = std::sqrt(1.0 + pOverM * pOverM);
+        r.gamma2 = r.gamma * r.gamma;
+        r.E = r.gamma * mass;
+        const double m_e = 0.51099895069 * 1e-3;
+        const double mfrac = m_e / mass;
+        r.Wmax = (2.0 * m_e * r.betaGamma * r.betaGamma) / (1.0 + 2.0 * r.gamma * mfrac + mfrac * mfrac);
+        return r;
+    }
+    static double bethe_bloch(const dtr_material_slab& m, const RQ& r) {
+        const double K = 0.307075 * 1e-3 * 1e2;  // MeV cm^2 in GeV mm^2
+        const double eps_per_length = 0.5 * K * (m.Z * m.molar_rho) * r.q2OverBeta2;
+        if (eps_per_length <= 0.0) return 0.0;
+        double dhalf = 0.0;
+        if (r.betaGamma >= 10.0) {
+            const double plasma = (28.816 * 1e-9) * std::sqrt(1000.0 * (m.Z * m.molar_rho));
+            dhalf = std::log(r.betaGamma * plasma / m.mean_exc) - 0.5;
+        }
+        const double m_e = 0.51099895069 * 1e-3;
+        const double u = 2.0 * m_e * r.betaGamma * r.betaGamma;
+        const double A = 0.5 * std::log(u * r.Wmax / (m.mean_exc * m.mean_exc));
+        const double running = A - r.beta2 - dhalf;
+        return 2.0 * eps_per_length * running;
+    }
+    static double landau_sigma_qop(const dtr_material_slab& m, const RQ& r, double path,
+                                   double charge) {
+        const double K = 0.307075 * 1e-3 * 1e2;
+        const double eps = 0.5 * K * (m.Z * m.molar_rho) * r.q2OverBeta2 * path;
+        const double fwhm = 4.0 * eps;
+        const double sigmaE = 0.5 * M_SQRT1_2 * fwhm / std::sqrt(M_LN2);
+        const double pInv = r.qOverP / charge;
+        return std::sqrt(r.q2OverBeta2) * pInv * pInv * sigmaE;
+    }
+    static double highland(double xOverX0, double qop, double charge, double q2OverBeta2) {
+        const double momentumInv = std::fabs(qop / charge);
+        if (xOverX0 <= 0.0) return 0.0;
+        const double t = std::sqrt(xOverX0 * q2OverBeta2);
+        return static_cast<double>(13.6f) * 1e-3 * momentumInv * t *
+               (1.0 + static_cast<double>(0.038f) * 2.0 * std::log(t));
+    }
+
+    // parameter_updater (parameter_updater.hpp:314-627) + interactor
+    // (pointwise_material_interactor.hpp:147-326)
+    void parameter_updater() {
+        if (!(on_sensitive() || encountered_material())) return;
+        const uint32_t dest = nav.current().sf;
+        if (up.sf != kInvalid) {
+            if (up.sf == dest) {
+                // on the initial surface: notify observers without transport
+                material_and_setter(dest);
+                return;
+            }
+            double Jf[36];
+            full_jacobian(dest, Jf);
+            if (std::fabs(st.path_length) > 0.0) st.reset_J();
+            double T[36], NC[36];
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    double s = 0.0;
+                    for (int m = 0; m < 6; ++m) s += Jf[r * 6 + m] * up.cov[m * 6 + c];
+                    T[r * 6 + c] = s;
+                }
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    double s = 0.0;
+                    for (int m = 0; m < 6; ++m) s += T[r * 6 + m] * Jf[c * 6 + m];
+                    NC[r * 6 + c] = s;
+                }
+            std::memcpy(up.cov, NC, sizeof(NC));
+        } else {
+            // placeholder covariance at the first surface
+            if (init_cov) {
+                int k = 0;
+                for (int r = 0; r < 6; ++r)
+                    for (int c = r; c < 6; ++c, ++k) {
+                        up.cov[r * 6 + c] = init_cov[k];
+                        up.cov[c * 6 + r] = init_cov[k];
+                    }
+            } else {
+                for (int i = 0; i < 36; ++i) up.cov[i] = (i % 7 == 0) ? 1.0 : 0.0;
+            }
+        }
+        free_to_bound(dest, st.pos, st.dir, st.time, st.qop, up.bound);
+        up.sf = dest;
+        material_and_setter(dest);
+    }
+
+    void material_and_setter(uint32_t dest) {
+        bool success = false;
+        if (encountered_material()) {
+            const dtr_material_slab& m = det.d->materials[det.sf(dest).material];
+            const V3 n = surface_normal(dest, up.bound);
+            const double cos_inc = std::fabs(dot(bound_dir(up.bound), n));
+            double e_loss = 0.0, sigma_qop = 0.0, theta0 = 0.0;
+            if (m.thickness > std::numeric_limits<double>::epsilon() && up.bound[4] != 0.0) {
+                const double qop = up.bound[4];
+                const double path_seg = m.thickness / cos_inc;
+                const RQ r = make_rq(st.ptc.mass, qop, st.ptc.charge);
+                if (cfg.do_eloss) e_loss = path_seg * bethe_bloch(m, r);
+                if (cfg.do_eloss && cfg.do_cov) sigma_qop = landau_sigma_qop(m, r, path_seg, st.ptc.charge);
+                if (cfg.do_ms) theta0 = highland(m.thickness_in_X0 / cos_inc, qop, st.ptc.charge, r.q2OverBeta2);
+                success = true;
+            }
+            if (success) {
+                if (cfg.do_eloss) {
+                    // update_qop (pointwise_material_interactor.hpp:240-262)
+                    const double mass = st.ptc.mass, q = st.ptc.charge;
+                    const double p = q / up.bound[4];
+                    const double next_e = std::sqrt(mass * mass + p * p) - std::copysign(e_loss, 1.0);
+                    const double next_p = (mass < next_e) ? std::sqrt(next_e * next_e - mass * mass) : 0.0;
+                    const double next_qop = (q != 0.0) ? q / next_p : 1.0 / next_p;
+                    up.bound[4] = (next_p == 0.0) ? kMax : next_qop;
+                    if (cfg.do_cov) up.cov[4 * 6 + 4] += sigma_qop * sigma_qop;
+                }
+                if (cfg.do_cov) {
+                    const V3 d = bound_dir(up.bound);
+                    const double var = theta0 * theta0;
+                    up.cov[2 * 6 + 2] += (d[2] == 1.0) ? kMax : var / (1.0 - d[2] * d[2]);
+                    up.cov[3 * 6 + 3] += var;
+                }
+            }
+        }
+        if (!success) return;
+        // parameter_setter (parameter_updater.hpp:585-627)
+        bound_to_free(up.sf, up.bound, st.pos, st.dir, st.time, st.qop);
+        if (cfg.estimate_noise) estimate_noise();
+    }
+
+    // noise_estimation.hpp:23-107
+    void estimate_noise() {
+        const double ns = static_cast<double>(cfg.n_stddev);
+        const double v0 = up.cov[0], v1 = up.cov[7];
+        const double dphi = ns * std::sqrt(up.cov[2 * 6 + 2]);
+        const double dth = ns * std::sqrt(up.cov[3 * 6 + 3]);
+        const double pe = up.bound[2] + dphi, te = up.bound[3] + dth;
+        const double ste = std::sin(te);
+        V3 displ{{std::cos(pe) * ste, std::sin(pe) * ste, std::cos(te)}};
+        const double max_tol = static_cast<double>(5.f);
+        const double path = nav.cache_exhausted() ? max_tol : std::fabs(nav.target().path);
+        displ = mul(path, sub(displ, st.dir));
+        const double q = st.ptc.charge;
+        const double p = q / st.qop;
+        const double acc = 1.0 / p * cfg.accumulated_error * st.path_length;
+        const double ext = std::sqrt(ns * ns * (v0 + v1) + dot(displ, displ) + acc * acc);
+        nav.ext_tol = ext > max_tol ? max_tol : ext;
+    }
+
+    // surface_sequencer.hpp:78-106
+    void sequencer() {
+        if (cfg.max_surfaces == 0u) return;
+        if (!(on_sensitive() || encountered_material())) return;
+        if (n_seq == cfg.max_surfaces) {
+            overflow = true;
+            nav.exit_nav();
+            heartbeat = false;
+            return;
+        }
+        seq[static_cast<uint64_t>(n_seq) * seq_stride] = nav.current().sf;
+        n_seq++;
+    }
+
+    void run_actors() {
+        pathlimit_aborter();
+        if (cfg.use_material) parameter_updater();
+        sequencer();
+    }
+
+    void trace() {
+        if (!tr_vol || tr_n >= tr_cap) return;
+        tr_vol[tr_n] = nav.volume;
+        tr_sf[tr_n] = nav.on_surface() ? nav.current().sf : kInvalid;
+        tr_n++;
+    }
+
+    // propagator.hpp:262-400
+    bool propagate() {
+        local_navigation(cfg.nav, false);  // navigator_base::init
+        heartbeat = nav.is_alive();
+        trace();
+        bool is_init = true;
+        double path_length = 0.0;
+        unsigned stall = 0;
+        for (unsigned i = 0; i % 2 == 0 || heartbeat; ++i) {
+            if (i % 2 == 0) {
+                run_actors();
+                if (!heartbeat) continue;
+                path_length = st.path_length;
+            } else {
+                const bool reset = nav.on_surface() || is_init;
+                heartbeat = heartbeat && step(nav(), reset);
+                rk_policy();
+                if (i > 0) is_init = false;
+                if (std::fabs(st.path_length) <= std::fabs(path_length) + static_cast<double>(cfg.nav.path_tol)) {
+                    if (stall >= 10u) {
+                        heartbeat = false;
+                        nav.abort_nav();
+                    }
+                    stall++;
+                } else {
+                    stall = 0;
+                }
+            }
+            is_init |= nav_update();
+            heartbeat = heartbeat && nav.is_alive();
+            trace();
+        }
+        return nav.status == kExit;
+    }
+};
+
+}  // namespace oracle
+
+extern "C" {
+
+int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
+                     const dtr_prop_config* cfg, const dtr_track_batch* in,
+                     dtr_result_batch* out, int n_threads, oracle_trace* trace) {
+    using namespace oracle;
+    if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cfg->use_field_gradient || cfg->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    PropCfg pc;
+    pc.min_stepsize = cfg->min_stepsize;
+    pc.rk_error_tol = cfg->rk_error_tol;
+    pc.step_constraint = cfg->step_constraint;
+    pc.path_limit = cfg->path_limit;
+    pc.max_rk_updates = cfg->max_rk_updates;
+    pc.do_cov = cfg->do_covariance_transport != 0;
+    pc.nav.min_mask_tol = cfg->min_mask_tolerance;
+    pc.nav.max_mask_tol = cfg->max_mask_tolerance;
+    pc.nav.mask_scalor = cfg->mask_tolerance_scalor;
+    pc.nav.path_tol = cfg->path_tolerance;
+    pc.nav.overstep_tol = cfg->overstep_tolerance;
+    pc.nav.win[0] = cfg->search_window[0];
+    pc.nav.win[1] = cfg->search_window[1];
+    pc.accumulated_error = cfg->accumulated_error;
+    pc.n_stddev = cfg->n_scattering_stddev;
+    pc.estimate_noise = cfg->estimate_scattering_noise != 0;
+    pc.do_eloss = cfg->do_energy_loss != 0;
+    pc.do_ms = cfg->do_multiple_scattering != 0;
+    pc.use_material = cfg->use_material != 0;
+    pc.max_surfaces = out->surfaces ? cfg->max_surfaces : 0u;
+    const int64_t n = in->n;
+#ifdef _OPENMP
+    const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#else
+    (void)n_threads;
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        Engine e;
+        e.det.d = det;
+        e.field.d = field;
+        e.cfg = pc;
+        const double* fp = in->free_params;
+        e.st.pos = {{fp[0 * n + i], fp[1 * n + i], fp[2 * n + i]}};
+        e.st.time = fp[3 * n + i];
+        e.st.dir = {{fp[4 * n + i], fp[5 * n + i], fp[6 * n + i]}};
+        e.st.qop = fp[7 * n + i];
+        e.st.ptc = muon_for(e.st.qop);
+        e.st.reset_J();
+        e.st.set_constraint(0, static_cast<double>(cfg->step_constraint));
+        double ic[21];
+        if (in->initial_cov) {
+            for (int k = 0; k < 21; ++k) ic[k] = in->initial_cov[k * n + i];
+            e.init_cov = ic;
+        }
+        for (int k = 0; k < 6; ++k) e.up.bound[k] = 0.0;
+        for (int k = 0; k < 36; ++k) e.up.cov[k] = 0.0;
+        if (pc.max_surfaces) {
+            e.seq = out->surfaces + i;
+            e.seq_stride = static_cast<uint32_t>(n);
+        }
+        if (trace) {
+            e.tr_vol = trace->volume + static_cast<uint64_t>(i) * trace->capacity;
+            e.tr_sf = trace->surface + static_cast<uint64_t>(i) * trace->capacity;
+            e.tr_cap = trace->capacity;
+        }
+        e.propagate();
+        uint8_t status;
+        if (e.nav.status == kExit) status = e.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
+        else status = e.nav.exited_path_limit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
+        if (out->status) out->status[i] = status;
+        if (out->bound_params)
+            for (int k = 0; k < 6; ++k) out->bound_params[k * n + i] = e.up.bound[k];
+        if (out->bound_cov) {
+            int k = 0;
+            for (int r = 0; r < 6; ++r)
+                for (int c = r; c < 6; ++c, ++k) out->bound_cov[k * n + i] = e.up.cov[r * 6 + c];
+        }
+        if (out->free_params) {
+            const double f[8] = {e.st.pos[0], e.st.pos[1], e.st.pos[2], e.st.time,
+                                 e.st.dir[0], e.st.dir[1], e.st.dir[2], e.st.qop};
+            for (int k = 0; k < 8; ++k) out->free_params[k * n + i] = f[k];
+        }
+        if (out->path_length) out->path_length[i] = e.st.path_length;
+        if (out->n_surfaces) out->n_surfaces[i] = e.n_seq;
+        if (out->n_steps) out->n_steps[i] = e.st.n_steps;
+        if (out->n_trials) out->n_trials[i] = static_cast<uint32_t>(e.st.n_trials);
+        if (trace) trace->n[i] = e.tr_n;
+    }
+    return DTR_SUCCESS;
+}
+
+int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz, double* b) {
+    if (!field || !xyz || !b) return DTR_ERROR_INVALID_ARGUMENT;
+    oracle::Field f{field};
+    for (uint32_t i = 0; i < n; ++i) {
+        const oracle::V3 v = f.at(oracle::V3{{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}});
+        b[3 * i] = v[0];
+        b[3 * i + 1] = v[1];
+        b[3 * i + 2] = v[2];
+    }
+    return DTR_SUCCESS;
+}
+
+int oracle_material_probe(const dtr_material_slab* mat, double qop, double path,
+                          double x_over_x0, double out[3]) {
+    using oracle::Engine;
+    if (!mat || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    const oracle::Particle p = oracle::muon_for(qop);
+    const Engine::RQ r = Engine::make_rq(p.mass, qop, p.charge);
+    out[0] = Engine::bethe_bloch(*mat, r);
+    out[1] = Engine::landau_sigma_qop(*mat, r, path, p.charge);
+    out[2] = Engine::highland(x_over_x0, qop, p.charge, r.q2OverBeta2);
+    return DTR_SUCCESS;
+}
+
+}  // extern "C"

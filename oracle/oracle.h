@@ -1,0 +1,53 @@
+/*
+ * Oracle: scalar FP64 CPU restatement of detray's propagation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or as the timed CPU baseline).  The product (detray_amd) never
+ * links or calls it.
+ *
+ * It consumes the same flat detector / field / config / track tables as the
+ * device path (include/detray_amd.h) and writes results in the same SoA
+ * layout, but in HOST memory.
+ */
+#ifndef DETRAY_AMD_ORACLE_H
+#define DETRAY_AMD_ORACLE_H
+
+#include "detray_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-track debugging trace of the navigation (optional). */
+typedef struct oracle_trace {
+    uint32_t capacity;       /* entries per track */
+    uint32_t* volume;        /* [n][capacity] volume after each navigator update */
+    uint32_t* surface;       /* [n][capacity] current surface (or 0xffffffff) */
+    uint32_t* n;             /* [n] */
+} oracle_trace;
+
+/* Propagate tracks on the host with n_threads OpenMP threads (0 = all,
+ * dynamic schedule, chunk 1 as the reference CPU benchmark,
+ * tests/benchmarks/include/detray/benchmarks/cpu/propagation_benchmark.hpp:72-148).
+ * All pointers of `in`/`out` are HOST pointers.  trace may be NULL. */
+int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
+                     const dtr_prop_config* cfg, const dtr_track_batch* in,
+                     dtr_result_batch* out, int n_threads, oracle_trace* trace);
+
+/* Evaluate the field at n points (host), for field tests. */
+int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz,
+                    double* bxyz);
+
+/* Material physics probes (interaction.hpp), for KAT tests.
+ * Returns the Bethe-Bloch stopping power -dE/dx [GeV/mm] of a muon with
+ * q/p = qop in slab material `mat`, the Landau sigma(q/p) over path `path`
+ * and the Highland theta0 for path/X0 = x_over_x0. */
+int oracle_material_probe(const dtr_material_slab* mat, double qop,
+                          double path, double x_over_x0, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

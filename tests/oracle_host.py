@@ -1,0 +1,100 @@
+"""Loader for the oracle (test infrastructure).
+
+The oracle is the FP64 CPU restatement in oracle/oracle.cpp, compiled to
+oracle/liboracle.so.  Only tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline use it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from detray_amd import capi
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
+
+_lib = None
+
+
+class Trace(C.Structure):
+    _fields_ = [("capacity", C.c_uint32), ("volume", C.c_void_p), ("surface", C.c_void_p),
+                ("n", C.c_void_p)]
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} missing: run __graft_entry__.build()")
+        _lib = C.CDLL(str(ORACLE_LIB))
+        _lib.oracle_propagate.restype = C.c_int
+        _lib.oracle_field_at.restype = C.c_int
+        _lib.oracle_material_probe.restype = C.c_int
+    return _lib
+
+
+def propagate(detector, field, config, free_params: np.ndarray, initial_cov=None,
+              threads: int = 0, trace_capacity: int = 0) -> dict:
+    """Run the oracle on host arrays; returns a dict of numpy arrays."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    n = fp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+    }
+    cov = None
+    if initial_cov is not None:
+        cov = np.ascontiguousarray(initial_cov, dtype=np.float64)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data, cov.ctypes.data if cov is not None else None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    tr = None
+    if trace_capacity:
+        out["trace_volume"] = np.zeros((n, trace_capacity), dtype=np.uint32)
+        out["trace_surface"] = np.zeros((n, trace_capacity), dtype=np.uint32)
+        out["trace_n"] = np.zeros(n, dtype=np.uint32)
+        tr = Trace(trace_capacity, out["trace_volume"].ctypes.data,
+                   out["trace_surface"].ctypes.data, out["trace_n"].ctypes.data)
+    cfg = config.to_c()
+    rc = lib.oracle_propagate(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
+                              C.byref(tb), C.byref(rb), C.c_int(threads),
+                              C.byref(tr) if tr is not None else None)
+    capi.check(rc, "oracle_propagate")
+    return out
+
+
+def field_at(field, xyz: np.ndarray) -> np.ndarray:
+    lib = load()
+    pts = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(pts)
+    capi.check(lib.oracle_field_at(C.byref(field.desc), C.c_uint32(len(pts)),
+                                   pts.ctypes.data_as(C.POINTER(C.c_double)),
+                                   out.ctypes.data_as(C.POINTER(C.c_double))), "oracle_field_at")
+    return out
+
+
+def material_probe(slab, qop: float, path: float, x_over_x0: float) -> np.ndarray:
+    lib = load()
+    out = (C.c_double * 3)()
+    capi.check(lib.oracle_material_probe(C.byref(slab), C.c_double(qop), C.c_double(path),
+                                         C.c_double(x_over_x0), out), "oracle_material_probe")
+    return np.array(out[:])
+
+
+def default_threads() -> int:
+    return int(os.environ.get("ORACLE_THREADS", "0"))
