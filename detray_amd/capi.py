@@ -151,6 +151,8 @@ class TrackBatch(C.Structure):
         ("n", C.c_uint32), ("pad", C.c_uint32),
         ("free_params", C.c_void_p),
         ("initial_cov", C.c_void_p),
+        ("bound_params", C.c_void_p),
+        ("start_surface", C.c_void_p),
     ]
 
 

@@ -49,11 +49,12 @@ int upload(const T* host, uint32_t n, void** dst) {
 
 __global__ void __launch_bounds__(256) propagate_fp64_kernel(
     dtr::DevDetector det, dtr::DevField field, dtr::DevConfig cfg,
-    const double* __restrict__ fp, const double* __restrict__ init_cov, dtr::DevResults out,
-    uint32_t n) {
+    const double* __restrict__ fp, const double* __restrict__ init_cov,
+    const double* __restrict__ bound0, const uint32_t* __restrict__ start_sf,
+    dtr::DevResults out, uint32_t n) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    dtr::propagate_track(det, field, cfg, fp, init_cov, out, n, gid);
+    dtr::propagate_track(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
 }
 
 }  // namespace
@@ -185,7 +186,8 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
                   const dtr_track_batch* in, dtr_result_batch* out, void* stream) {
     if (!det || !field || !c || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
     if (in->n == 0u) return DTR_SUCCESS;
-    if (!in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
+    if (!in->free_params && !in->bound_params) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->max_surfaces != 0u && (!out->surfaces || !out->n_surfaces)) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
@@ -217,7 +219,8 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
     const uint32_t grid = (in->n + block - 1) / block;
     hipLaunchKernelGGL(propagate_fp64_kernel, dim3(grid), dim3(block), 0,
                        static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
-                       in->free_params, in->initial_cov, r, in->n);
+                       in->free_params, in->initial_cov, in->bound_params, in->start_surface,
+                       r, in->n);
     return to_status(hipGetLastError());
 }
 

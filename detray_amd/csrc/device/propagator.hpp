@@ -994,8 +994,10 @@ struct Engine {
 // One track: propagator::propagate (propagator.hpp:262-400)
 DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
                             const DevConfig& cfg, const double* __restrict__ fp,
-                                const double* __restrict__ init_cov, const DevResults& out,
-                                uint32_t n, uint32_t gid) {
+                            const double* __restrict__ init_cov,
+                            const double* __restrict__ bound0,
+                            const uint32_t* __restrict__ start_sf, const DevResults& out,
+                            uint32_t n, uint32_t gid) {
     Track tk;
     for (int i = 0; i < kCache; ++i) {
         tk.c[i].path = kMax;
@@ -1012,12 +1014,6 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     tk.volume = 0;
     tk.path_limit_hit = false;
     tk.overflow = false;
-    tk.pos = {fp[0 * static_cast<uint64_t>(n) + gid], fp[1 * static_cast<uint64_t>(n) + gid],
-              fp[2 * static_cast<uint64_t>(n) + gid]};
-    tk.time = fp[3 * static_cast<uint64_t>(n) + gid];
-    tk.dir = {fp[4 * static_cast<uint64_t>(n) + gid], fp[5 * static_cast<uint64_t>(n) + gid],
-              fp[6 * static_cast<uint64_t>(n) + gid]};
-    tk.qop = fp[7 * static_cast<uint64_t>(n) + gid];
     tk.step_size = 0.0;
     tk.next_step_size = 0.0;
     tk.path_length = 0.0;
@@ -1027,7 +1023,6 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     tk.dtds3 = {0.0, 0.0, 0.0};
     // muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64)
     tk.mass = static_cast<double>(105.6583755f * 1e-3f);
-    tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
     tk.n_trials = 0;
     tk.n_steps = 0;
     for (int i = 0; i < 6; ++i) tk.bound[i] = 0.0;
@@ -1036,6 +1031,30 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     tk.n_seq = 0;
 
     Engine e{det, field, cfg, tk};
+    const uint64_t N = n;
+    if (bound0) {
+        // bound start (propagator.hpp:121-162, base_stepper.hpp:84-108)
+        const uint32_t s0 = start_sf[gid];
+        for (int k = 0; k < 6; ++k) tk.bound[k] = bound0[k * N + gid];
+        for (int r = 0; r < 6; ++r)
+            for (int c = r; c < 6; ++c) {
+                const int k = covi(r, c);
+                tk.cov[k] = init_cov ? init_cov[k * N + gid] : (r == c ? 1.0 : 0.0);
+            }
+        tk.up_sf = s0;
+        tk.pos = e.bound_pos(s0, tk.bound);
+        tk.dir = Engine::bound_dir(tk.bound);
+        tk.time = tk.bound[5];
+        tk.qop = tk.bound[4];
+        tk.volume = det.surfaces[s0].volume;
+        init_cov = nullptr;
+    } else {
+        tk.pos = {fp[0 * N + gid], fp[1 * N + gid], fp[2 * N + gid]};
+        tk.time = fp[3 * N + gid];
+        tk.dir = {fp[4 * N + gid], fp[5 * N + gid], fp[6 * N + gid]};
+        tk.qop = fp[7 * N + gid];
+    }
+    tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
     e.reset_J();
 
     // navigator_base::init -> local_navigation without overstepping resolution
@@ -1093,7 +1112,6 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     uint8_t status;
     if (tk.status == kExit) status = tk.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
     else status = tk.path_limit_hit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
-    const uint64_t N = n;
     if (out.status) out.status[gid] = status;
     if (out.bound_params)
         for (int k = 0; k < 6; ++k) out.bound_params[k * N + gid] = tk.bound[k];

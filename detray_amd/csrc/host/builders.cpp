@@ -841,39 +841,43 @@ void build_telescope(const dtr_telescope_config& cfg, Tables& T) {
 // Generalised complete elliptic integral by the Bulirsch cel algorithm:
 // cel(kc, p, a, b) = int_0^{pi/2} (a cos^2 + b sin^2) / ((cos^2 + p sin^2)
 //                    sqrt(cos^2 + kc^2 sin^2)) dphi
-double cel(double kc, double p, double a, double b) {
+double cel(double kc, double p, double c, double s) {
     if (kc == 0.0) return std::numeric_limits<double>::infinity();
-    const double errtol = 1e-14;
-    double k = std::fabs(kc), pp = p, aa = a, bb = b, em = 1.0;
-    double qc = k;
-    if (pp > 0.0) {
-        pp = std::sqrt(pp);
-        bb = bb / pp;
+    const double errtol = 1e-12;
+    double k = std::fabs(kc), pp = p, cc = c, ss = s, em = 1.0;
+    if (p > 0.0) {
+        pp = std::sqrt(p);
+        ss = s / pp;
     } else {
         double f = kc * kc;
         double q = 1.0 - f;
-        double g = 1.0 - pp;
+        const double g = 1.0 - pp;
         f = f - pp;
-        q = q * (bb - aa * pp);
+        q = q * (ss - c * pp);
         pp = std::sqrt(f / g);
-        aa = (aa - bb) / g;
-        bb = -q / (g * g * pp) + aa * pp;
+        cc = (c - ss) / g;
+        ss = -q / (g * g * pp) + cc * pp;
     }
-    for (int it = 0; it < 100; ++it) {
-        double f = aa;
-        aa = aa + bb / pp;
-        double g = em / pp;
-        bb = bb + f * g;
-        bb = bb + bb;
+    double f = cc;
+    cc = cc + ss / pp;
+    double g = k / pp;
+    ss = 2.0 * (ss + f * g);
+    pp = g + pp;
+    g = em;
+    em = k + em;
+    double kk = k;
+    for (int it = 0; it < 200 && std::fabs(g - k) > g * errtol; ++it) {
+        k = 2.0 * std::sqrt(kk);
+        kk = k * em;
+        f = cc;
+        cc = cc + ss / pp;
+        g = kk / pp;
+        ss = 2.0 * (ss + f * g);
         pp = g + pp;
         g = em;
-        em = qc + em;
-        if (std::fabs(g - qc) <= g * errtol) break;
-        qc = std::sqrt(k);
-        qc = qc + qc;
-        k = qc * em;
+        em = k + em;
     }
-    return 0.5 * kPi * (aa * em + bb) / (em * (em + pp));
+    return 0.5 * kPi * (ss + cc * em) / (em * (em + pp));
 }
 
 // Field of a finite solenoid (unit surface current n*I*mu0 = 1) at (rho, z)

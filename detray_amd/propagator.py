@@ -305,33 +305,61 @@ class Propagator:
             status=torch.full((n,), 255, dtype=torch.uint8, device=device),
         )
 
-    def propagate(self, detector: Detector, field: MagneticField, tracks, initial_cov=None,
-                  out: Optional[PropagationResult] = None, stream=None) -> PropagationResult:
-        """Propagate ``tracks`` (device float64 tensor [8, n], free parameters)."""
+    def propagate(self, detector: Detector, field: MagneticField, tracks=None, initial_cov=None,
+                  out: Optional[PropagationResult] = None, stream=None, bound_params=None,
+                  start_surface=None) -> PropagationResult:
+        """Propagate a batch of tracks.
+
+        Free start: ``tracks`` is a device float64 tensor [8, n] of free
+        parameters (the tracks start in volume 0).  Bound start:
+        ``bound_params`` [6, n] float64 and ``start_surface`` [n] int32 give
+        bound track parameters on a detector surface
+        (propagator::state(bound_track_parameters, ...)).
+        ``initial_cov`` [21, n] is the upper-triangle bound covariance."""
         import torch
 
-        if not (isinstance(tracks, torch.Tensor) and tracks.is_cuda):
-            raise TypeError("tracks must be a CUDA (HIP) tensor [8, n] of float64")
-        if tracks.dtype != torch.float64 or tracks.dim() != 2 or tracks.shape[0] != 8:
-            raise ValueError("tracks must be float64 with shape [8, n]")
-        tracks = tracks.contiguous()
-        n = int(tracks.shape[1])
+        if bound_params is not None:
+            if start_surface is None:
+                raise ValueError("bound_params needs start_surface")
+            if (not bound_params.is_cuda or bound_params.dtype != torch.float64
+                    or bound_params.dim() != 2 or bound_params.shape[0] != 6):
+                raise ValueError("bound_params must be a float64 CUDA tensor [6, n]")
+            bound_params = bound_params.contiguous()
+            n = int(bound_params.shape[1])
+            if (not start_surface.is_cuda or start_surface.dtype != torch.int32
+                    or start_surface.shape != (n,)):
+                raise ValueError("start_surface must be an int32 CUDA tensor [n]")
+            start_surface = start_surface.contiguous()
+            if int(start_surface.max().item() if n else 0) >= detector.n_surfaces or (
+                    n and int(start_surface.min().item()) < 0):
+                raise ValueError("start_surface out of range")
+            device = bound_params.device
+        else:
+            if not (isinstance(tracks, torch.Tensor) and tracks.is_cuda):
+                raise TypeError("tracks must be a CUDA (HIP) tensor [8, n] of float64")
+            if tracks.dtype != torch.float64 or tracks.dim() != 2 or tracks.shape[0] != 8:
+                raise ValueError("tracks must be float64 with shape [8, n]")
+            tracks = tracks.contiguous()
+            n = int(tracks.shape[1])
+            device = tracks.device
         if initial_cov is not None:
             if initial_cov.shape != (21, n) or initial_cov.dtype != torch.float64 or not initial_cov.is_cuda:
                 raise ValueError("initial_cov must be a float64 CUDA tensor [21, n]")
             initial_cov = initial_cov.contiguous()
         if out is None:
-            out = self.allocate(n, tracks.device)
+            out = self.allocate(n, device)
         lib = capi.load()
-        tb = capi.TrackBatch(n, 0, tracks.data_ptr(),
-                             initial_cov.data_ptr() if initial_cov is not None else None)
+        tb = capi.TrackBatch(n, 0, tracks.data_ptr() if tracks is not None and bound_params is None else None,
+                             initial_cov.data_ptr() if initial_cov is not None else None,
+                             bound_params.data_ptr() if bound_params is not None else None,
+                             start_surface.data_ptr() if start_surface is not None else None)
         rb = capi.ResultBatch(out.bound_params.data_ptr(), out.bound_cov.data_ptr(),
                               out.free_params.data_ptr(), out.path_length.data_ptr(),
                               out.surfaces.data_ptr(), out.n_surfaces.data_ptr(),
                               out.n_steps.data_ptr(), out.n_trials.data_ptr(),
                               out.status.data_ptr())
         if stream is None:
-            stream = torch.cuda.current_stream(tracks.device).cuda_stream
+            stream = torch.cuda.current_stream(device).cuda_stream
         capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
                                      C.byref(tb), C.byref(rb), C.c_void_p(stream)),
                    "dtr_propagate")

@@ -197,10 +197,19 @@ typedef struct dtr_prop_config {
 typedef struct dtr_track_batch {
     uint32_t n;
     uint32_t pad;
-    const double* free_params;  /* [8][n] */
-    const double* initial_cov;  /* NULL or [21][n] upper-triangle bound cov used at
-                                   the first surface in place of the identity
-                                   placeholder (parameter_updater.hpp:392-397) */
+    /* Free start (propagator::state(free_track_parameters, ...),
+     * propagator.hpp:79-95): free_params [8][n], tracks start in volume 0.
+     * initial_cov: NULL or [21][n] upper-triangle bound covariance used at
+     * the first surface in place of the identity placeholder
+     * (parameter_updater.hpp:392-397). */
+    const double* free_params;
+    const double* initial_cov;
+    /* Bound start (propagator::state(bound_track_parameters, ...),
+     * propagator.hpp:121-162): when bound_params is not NULL, track i starts
+     * on surface start_surface[i] with bound parameters [6][n] and
+     * covariance initial_cov (identity if NULL); free_params is ignored. */
+    const double* bound_params;
+    const uint32_t* start_surface;
 } dtr_track_batch;
 
 typedef struct dtr_result_batch {

@@ -1237,6 +1237,8 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
     using namespace oracle;
     if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
     if (cfg->use_field_gradient || cfg->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (!in->bound_params && !in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
     PropCfg pc;
     pc.min_stepsize = cfg->min_stepsize;
     pc.rk_error_tol = cfg->rk_error_tol;
@@ -1270,21 +1272,40 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
         e.det.d = det;
         e.field.d = field;
         e.cfg = pc;
-        const double* fp = in->free_params;
-        e.st.pos = {{fp[0 * n + i], fp[1 * n + i], fp[2 * n + i]}};
-        e.st.time = fp[3 * n + i];
-        e.st.dir = {{fp[4 * n + i], fp[5 * n + i], fp[6 * n + i]}};
-        e.st.qop = fp[7 * n + i];
-        e.st.ptc = muon_for(e.st.qop);
-        e.st.reset_J();
-        e.st.set_constraint(0, static_cast<double>(cfg->step_constraint));
+        for (int k = 0; k < 6; ++k) e.up.bound[k] = 0.0;
+        for (int k = 0; k < 36; ++k) e.up.cov[k] = 0.0;
         double ic[21];
         if (in->initial_cov) {
             for (int k = 0; k < 21; ++k) ic[k] = in->initial_cov[k * n + i];
             e.init_cov = ic;
         }
-        for (int k = 0; k < 6; ++k) e.up.bound[k] = 0.0;
-        for (int k = 0; k < 36; ++k) e.up.cov[k] = 0.0;
+        if (in->bound_params) {
+            // bound start: base_stepper.hpp:84-108 (free = bound_to_free on the
+            // start surface), propagator.hpp:121-162 (volume of the surface),
+            // parameter_updater_state(cfg, bound_params)
+            const uint32_t s0 = in->start_surface[i];
+            for (int k = 0; k < 6; ++k) e.up.bound[k] = in->bound_params[k * n + i];
+            e.up.sf = s0;
+            int k = 0;
+            for (int r = 0; r < 6; ++r)
+                for (int c = r; c < 6; ++c, ++k) {
+                    const double v = in->initial_cov ? ic[k] : (r == c ? 1.0 : 0.0);
+                    e.up.cov[r * 6 + c] = v;
+                    e.up.cov[c * 6 + r] = v;
+                }
+            e.init_cov = nullptr;
+            e.bound_to_free(s0, e.up.bound, e.st.pos, e.st.dir, e.st.time, e.st.qop);
+            e.nav.set_volume(det->surfaces[s0].volume);
+        } else {
+            const double* fp = in->free_params;
+            e.st.pos = {{fp[0 * n + i], fp[1 * n + i], fp[2 * n + i]}};
+            e.st.time = fp[3 * n + i];
+            e.st.dir = {{fp[4 * n + i], fp[5 * n + i], fp[6 * n + i]}};
+            e.st.qop = fp[7 * n + i];
+        }
+        e.st.ptc = muon_for(e.st.qop);
+        e.st.reset_J();
+        e.st.set_constraint(0, static_cast<double>(cfg->step_constraint));
         if (pc.max_surfaces) {
             e.seq = out->surfaces + i;
             e.seq_stride = static_cast<uint32_t>(n);

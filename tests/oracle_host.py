@@ -38,12 +38,18 @@ def load():
     return _lib
 
 
-def propagate(detector, field, config, free_params: np.ndarray, initial_cov=None,
-              threads: int = 0, trace_capacity: int = 0) -> dict:
-    """Run the oracle on host arrays; returns a dict of numpy arrays."""
+def propagate(detector, field, config, free_params, initial_cov=None,
+              threads: int = 0, trace_capacity: int = 0, bound_params=None,
+              start_surface=None) -> dict:
+    """Run the oracle on host arrays; returns a dict of numpy arrays.
+
+    Free start: free_params [8, n].  Bound start: bound_params [6, n] and
+    start_surface [n] (free_params may be None)."""
     lib = load()
-    fp = np.ascontiguousarray(free_params, dtype=np.float64)
-    n = fp.shape[1]
+    fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
+    bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = None if start_surface is None else np.ascontiguousarray(start_surface, dtype=np.uint32)
+    n = fp.shape[1] if fp is not None else bp.shape[1]
     ms = max(int(config.max_surfaces), 1)
     out = {
         "bound_params": np.zeros((6, n)),
@@ -59,7 +65,10 @@ def propagate(detector, field, config, free_params: np.ndarray, initial_cov=None
     cov = None
     if initial_cov is not None:
         cov = np.ascontiguousarray(initial_cov, dtype=np.float64)
-    tb = capi.TrackBatch(n, 0, fp.ctypes.data, cov.ctypes.data if cov is not None else None)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data if fp is not None else None,
+                         cov.ctypes.data if cov is not None else None,
+                         bp.ctypes.data if bp is not None else None,
+                         ss.ctypes.data if ss is not None else None)
     rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
         "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
         "n_steps", "n_trials", "status")])

@@ -162,3 +162,34 @@ def test_kat_navigation_ray(toy3):
     got, ref = run_both(toy3, field, cfg, fp)
     seq = got["surfaces"][: got["n_surfaces"][0], 0].tolist()
     assert seq == [0, 475, 459, 476, 460, 829, 797, 830, 798, 1438, 1386, 2372, 2294]
+
+
+def test_bound_start_parity(telescope):
+    # bound start on plane 1 (propagator::state(bound_track_parameters, ...))
+    cfg = da.PropagationConfig(estimate_scattering_noise=False)
+    field = da.MagneticField.constant(0.0, 1.0 * da.TESLA, 0.2 * da.TESLA)
+    n = 2048
+    rng = np.random.default_rng(7)
+    bp = np.zeros((6, n))
+    bp[0] = rng.uniform(-50, 50, n)
+    bp[1] = rng.uniform(-50, 50, n)
+    bp[2] = rng.uniform(-np.pi, np.pi, n)
+    bp[3] = rng.uniform(0.0, 0.2, n)
+    bp[4] = rng.choice([-1.0, 1.0], n) / rng.uniform(1.0, 10.0, n)
+    c0 = np.zeros((21, n))
+    for k in (0, 6, 11, 15, 18, 20):
+        c0[k] = 1e-4
+    ss = np.zeros(n, dtype=np.int32)
+    prop = da.Propagator(cfg)
+    res = prop.propagate(telescope, field, bound_params=torch.from_numpy(bp).cuda(),
+                         start_surface=torch.from_numpy(ss).cuda(),
+                         initial_cov=torch.from_numpy(c0).cuda())
+    torch.cuda.synchronize()
+    got = {k: getattr(res, k).cpu().numpy() for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")}
+    for k in ("surfaces", "n_surfaces", "n_steps", "n_trials"):
+        got[k] = got[k].view(np.uint32)
+    ref = oracle_host.propagate(telescope, field, cfg, None, initial_cov=c0, bound_params=bp,
+                                start_surface=ss.astype(np.uint32))
+    assert_parity(got, ref)
