@@ -122,7 +122,8 @@ DTR_HD V3 field_at(const DevField& f, V3 p) {
     for (int a = 0; a < 3; ++a) {
         const int n = static_cast<int>(f.n[a]);
         double u = (pc[a] - f.min[a]) / f.pitch[a];
-        u = u < 0.0 ? 0.0 : (u > n - 1 ? static_cast<double>(n - 1) : u);
+        // clamp (NaN positions clamp to the first sample)
+        u = (u > 0.0) ? (u > n - 1 ? static_cast<double>(n - 1) : u) : 0.0;
         int i = static_cast<int>(floor(u));
         i = i > n - 2 ? n - 2 : i;
         i0[a] = i;
@@ -190,6 +191,7 @@ struct Track {
     uint32_t up_sf;
     uint32_t n_seq;
     bool heartbeat;
+    bool stopped;  // energy loss brought the particle to rest
 };
 
 // Reduced Jacobian index helpers: rows 0-2 and 4-6 over columns 4..7, (7,7)
@@ -412,6 +414,9 @@ struct Engine {
         }
         const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
         const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
+        // non-finite lookup point: no defined bin, no grid candidates
+        const double f0 = (l0 - g.min0) / w0 + 1.0, f1 = (l1 - g.min1) / w1 + 1.0;
+        if (!(fabs(f0) < 1e9) || !(fabs(f1) < 1e9)) return;
         const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
         const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
         const int lb0 = ib0 - static_cast<int>(cfg.win0), ub0 = ib0 + static_cast<int>(cfg.win1);
@@ -933,6 +938,7 @@ struct Engine {
             const double next_p = (mass < next_e) ? sqrt(next_e * next_e - mass * mass) : 0.0;
             const double next_qop = (q != 0.0) ? q / next_p : 1.0 / next_p;
             tk.bound[4] = (next_p == 0.0) ? kMax : next_qop;
+            if (next_p == 0.0) tk.stopped = true;
             if (cfg.do_cov) tk.cov[covi(4, 4)] += sigma_qop * sigma_qop;
         }
         if (cfg.do_cov) {
@@ -1029,6 +1035,7 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     for (int i = 0; i < 21; ++i) tk.cov[i] = 0.0;
     tk.up_sf = kInvalid;
     tk.n_seq = 0;
+    tk.stopped = false;
 
     Engine e{det, field, cfg, tk};
     const uint64_t N = n;
@@ -1077,6 +1084,10 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
             }
             // parameter_updater<pointwise_material_interactor>
             if (cfg.use_material) e.parameter_updater(init_cov, n, gid);
+            if (tk.stopped && tk.status > kUnknown) {
+                tk.status = kAbort;
+                tk.heartbeat = false;
+            }
             // surface_sequencer (surface_sequencer.hpp:78-106)
             if (cfg.max_surfaces != 0u && (e.on_sensitive() || e.on_material())) {
                 if (tk.n_seq == cfg.max_surfaces) {

@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 
@@ -93,7 +94,8 @@ struct Field {
         for (int a = 0; a < 3; ++a) {
             const int n = static_cast<int>(d->n[a]);
             double u = (p[a] - d->min[a]) / d->pitch[a];
-            u = u < 0.0 ? 0.0 : (u > n - 1 ? static_cast<double>(n - 1) : u);
+            // clamp (NaN positions clamp to the first sample)
+            u = (u > 0.0) ? (u > n - 1 ? static_cast<double>(n - 1) : u) : 0.0;
             int i = static_cast<int>(std::floor(u));
             i = i > n - 2 ? n - 2 : i;
             i0[a] = i;
@@ -400,6 +402,7 @@ struct Engine {
     Stepper st;
     Updater up;
     bool heartbeat = false;
+    bool stopped = false;  // energy loss brought the particle to rest
     const double* init_cov = nullptr;  // 21 values or null
     // sequencer
     uint32_t* seq = nullptr;
@@ -499,6 +502,10 @@ struct Engine {
         }
         const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
         const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
+        // a non-finite lookup point has no defined bin (the reference casts
+        // it to int): no sensitive candidates from the grid then
+        const double f0 = (l0 - g.min0) / w0 + 1.0, f1 = (l1 - g.min1) / w1 + 1.0;
+        if (!(std::fabs(f0) < 1e9) || !(std::fabs(f1) < 1e9)) return;
         const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
         const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
         int r0[2] = {ib0 - static_cast<int>(icfg.win[0]), ib0 + static_cast<int>(icfg.win[1])};
@@ -1129,6 +1136,8 @@ struct Engine {
                     const double next_p = (mass < next_e) ? std::sqrt(next_e * next_e - mass * mass) : 0.0;
                     const double next_qop = (q != 0.0) ? q / next_p : 1.0 / next_p;
                     up.bound[4] = (next_p == 0.0) ? kMax : next_qop;
+                    // particle at rest: nothing left to propagate
+                    if (next_p == 0.0) stopped = true;
                     if (cfg.do_cov) up.cov[4 * 6 + 4] += sigma_qop * sigma_qop;
                 }
                 if (cfg.do_cov) {
@@ -1181,6 +1190,10 @@ struct Engine {
     void run_actors() {
         pathlimit_aborter();
         if (cfg.use_material) parameter_updater();
+        if (stopped && nav.is_alive()) {
+            nav.abort_nav();
+            heartbeat = false;
+        }
         sequencer();
     }
 
@@ -1206,6 +1219,11 @@ struct Engine {
                 path_length = st.path_length;
             } else {
                 const bool reset = nav.on_surface() || is_init;
+#ifdef ORACLE_DEBUG
+                std::fprintf(stderr, "i=%u vol=%u status=%d trust=%u next=%d last=%d dist=%.6g pos=(%.6g %.6g %.6g) r=%.6g dir=(%.4g %.4g %.4g) path=%.6g\n",
+                             i, nav.volume, nav.status, nav.trust, nav.next, nav.last, nav(), st.pos[0], st.pos[1], st.pos[2],
+                             perp(st.pos), st.dir[0], st.dir[1], st.dir[2], st.path_length);
+#endif
                 heartbeat = heartbeat && step(nav(), reset);
                 rk_policy();
                 if (i > 0) is_init = false;
