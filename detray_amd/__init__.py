@@ -9,6 +9,7 @@ from .propagator import (  # noqa: F401
     TESLA,
     UM,
     Detector,
+    LaunchConfig,
     MagneticField,
     PropagationConfig,
     PropagationResult,
@@ -17,6 +18,6 @@ from .propagator import (  # noqa: F401
 )
 
 __all__ = [
-    "DetrayError", "load", "Detector", "MagneticField", "PropagationConfig",
+    "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
     "PropagationResult", "Propagator", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
 ]

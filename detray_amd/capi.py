@@ -170,6 +170,24 @@ class ResultBatch(C.Structure):
     ]
 
 
+# dtr_precision, dtr_schedule
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [
+        ("precision", C.c_uint32),
+        ("schedule", C.c_uint32),
+        ("block_size", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
+        ("tracks_per_claim", C.c_uint32),
+        ("min_idle", C.c_uint32),
+        ("order", C.c_void_p),
+        ("queue", C.c_void_p),
+    ]
+
+
 class ToyConfig(C.Structure):
     _fields_ = [
         ("n_brl_layers", C.c_uint32),
@@ -210,6 +228,10 @@ _SIGNATURES = {
     "dtr_field_destroy": (C.c_int, [C.c_void_p]),
     "dtr_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig),
                                 C.POINTER(TrackBatch), C.POINTER(ResultBatch), C.c_void_p]),
+    "dtr_default_launch": (None, [C.POINTER(LaunchConfig)]),
+    "dtr_propagate_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig),
+                                   C.POINTER(TrackBatch), C.POINTER(ResultBatch),
+                                   C.POINTER(LaunchConfig), C.c_void_p]),
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),

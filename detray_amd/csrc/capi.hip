@@ -13,13 +13,20 @@
 #include <cstring>
 #include <new>
 
+#include <vector>
+
 #include "detray_amd.h"
+#include "device/engine.hpp"
 #include "device/propagator.hpp"
 
 struct dtr_detector {
     dtr::DevDetector dev;
-    void* blocks[8];
+    dtr::FastDetector fast;
+    void* blocks[11];  // the eight tables, navigation records, bin spheres, grid tolerances
     uint32_t n_volumes;
+    uint32_t n_surfaces;
+    int device;
+    uint32_t n_cu;
 };
 
 struct dtr_field {
@@ -55,6 +62,169 @@ __global__ void __launch_bounds__(256) propagate_fp64_kernel(
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
     dtr::propagate_track(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
+}
+
+// Candidate cache of the block in LDS, slot-major: [8][blockDim] paths,
+// then [8][blockDim] surface indices and [8][blockDim] link|status words.
+__device__ inline dtr::CacheView lds_cache(unsigned char* smem) {
+    const int nb = static_cast<int>(blockDim.x);
+    double* path = reinterpret_cast<double*>(smem);
+    uint32_t* sf = reinterpret_cast<uint32_t*>(path + dtr::kCache * nb);
+    uint32_t* ls = sf + dtr::kCache * nb;
+    const int t = static_cast<int>(threadIdx.x);
+    return dtr::CacheView{path + t, sf + t, ls + t, nb};
+}
+
+size_t lds_bytes(uint32_t block) { return static_cast<size_t>(block) * dtr::kCache * 16u; }
+
+// Resident waves per SIMD the batched kernels are compiled for (the register
+// budget per lane is 512 / DTR_WAVES_PER_EU).
+#ifndef DTR_WAVES_PER_EU
+#define DTR_WAVES_PER_EU 1
+#endif
+#define DTR_BATCHED_KERNEL __global__ void __launch_bounds__(256) \
+    __attribute__((amdgpu_waves_per_eu(DTR_WAVES_PER_EU, DTR_WAVES_PER_EU)))
+
+// One work-item per track (track = order[k] when a schedule permutation is
+// given, k = global work-item index).
+template <class P>
+DTR_BATCHED_KERNEL propagate_grid_kernel(
+    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
+    dtr::DevResults out, const uint32_t* __restrict__ order) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= in.n) return;
+    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+    dtr::propagate_fast<P>(det, field, cfg, in, out, track, lds_cache(smem));
+}
+
+// Persistent waves over a device work queue: every lane runs one track; a
+// lane whose track has ended takes the next queue entry, so a wave stays
+// full while its tracks have very different step counts.  A wave claims
+// `claim` entries at a time with one atomic on *queue; the kernel ends when
+// the queue is drained and every lane's track has ended.
+template <class P>
+DTR_BATCHED_KERNEL propagate_persistent_kernel(
+    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
+    dtr::DevResults out, const uint32_t* __restrict__ order, uint32_t* __restrict__ queue,
+    uint32_t claim, uint32_t min_idle) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const dtr::CacheView c = lds_cache(smem);
+    const uint64_t n = in.n;
+    const uint32_t lane_id = __lane_id();
+    const uint64_t lt_mask = (lane_id == 0u) ? 0ull : (~0ull >> (64u - lane_id));
+    dtr::Lane<P> lane;
+    bool active = false;
+    bool drained = false;
+    uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
+    for (;;) {
+        // hand queue entries to the lanes without a track, once at least
+        // min_idle lanes (or all of them) are idle, so that new tracks start
+        // their first navigation together
+        uint64_t need = __ballot(!active);
+        uint32_t track = 0xffffffffu;
+        const uint32_t idle = static_cast<uint32_t>(__popcll(need));
+        if (idle < min_idle && idle < 64u) need = 0ull;
+        while (need != 0ull && !drained) {
+            if (pool_next >= pool_end) {
+                const int leader = __ffsll(static_cast<unsigned long long>(need)) - 1;
+                uint32_t base = 0u;
+                if (static_cast<int>(lane_id) == leader) base = atomicAdd(queue, claim);
+                base = __shfl(base, leader);
+                if (base >= n) {
+                    drained = true;
+                    break;
+                }
+                pool_next = base;
+                pool_end = (base + static_cast<uint64_t>(claim) < n) ? base + claim : n;
+            }
+            const uint64_t take = pool_end - pool_next;
+            const uint64_t rank = static_cast<uint64_t>(__popcll(need & lt_mask));
+            const bool mine = !active && ((need >> lane_id) & 1ull) && rank < take;
+            if (mine) {
+                const uint64_t q = pool_next + rank;
+                track = order ? order[q] : static_cast<uint32_t>(q);
+                active = true;
+            }
+            const uint64_t cnt = static_cast<uint64_t>(__popcll(need));
+            pool_next += cnt < take ? cnt : take;
+            need = __ballot(!active);
+        }
+        if (track != 0xffffffffu) lane.start(det, field, cfg, in, c, track);
+        if (!__any(active)) break;
+        if (active && lane.advance(det, field, cfg, in, out, c)) {
+            lane.finish(out, n);
+            active = false;
+        }
+    }
+}
+
+__global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
+
+dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
+    dtr::DevConfig dc{};
+    dc.min_stepsize = c->min_stepsize;
+    dc.rk_error_tol = c->rk_error_tol;
+    dc.step_constraint = c->step_constraint;
+    dc.path_limit = c->path_limit;
+    dc.max_rk_updates = c->max_rk_updates;
+    dc.min_mask_tol = c->min_mask_tolerance;
+    dc.max_mask_tol = c->max_mask_tolerance;
+    dc.mask_scalor = c->mask_tolerance_scalor;
+    dc.path_tol = c->path_tolerance;
+    dc.overstep_tol = c->overstep_tolerance;
+    dc.win0 = c->search_window[0];
+    dc.win1 = c->search_window[1];
+    dc.accumulated_error = c->accumulated_error;
+    dc.n_stddev = static_cast<double>(c->n_scattering_stddev);
+    dc.max_surfaces = c->max_surfaces;
+    dc.do_cov = c->do_covariance_transport;
+    dc.estimate_noise = c->estimate_scattering_noise;
+    dc.do_eloss = c->do_energy_loss;
+    dc.do_ms = c->do_multiple_scattering;
+    dc.use_material = c->use_material;
+    return dc;
+}
+
+int check_propagate_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                         const dtr_track_batch* in, const dtr_result_batch* out) {
+    if (!det || !field || !c || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->n == 0u) return DTR_SUCCESS;
+    if (!in->free_params && !in->bound_params) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
+    if (c->max_surfaces != 0u && (!out->surfaces || !out->n_surfaces)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    return DTR_SUCCESS;
+}
+
+template <class P>
+int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+                const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
+                hipStream_t stream) {
+    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov,
+                              in->bound_params, in->start_surface, in->n};
+    const uint32_t block = L.block_size;
+    const size_t shmem = lds_bytes(block);
+    if (L.schedule == DTR_SCHEDULE_GRID) {
+        const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
+        hipLaunchKernelGGL(propagate_grid_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+                           field->dev, dc, ti, r, L.order);
+        return to_status(hipGetLastError());
+    }
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, L.queue);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return to_status(e);
+    // resident waves: CUs x 4 SIMDs x waves_per_simd, never more than the
+    // tracks need
+    const uint64_t waves = static_cast<uint64_t>(det->n_cu) * 4u * L.waves_per_simd;
+    const uint64_t need = (static_cast<uint64_t>(in->n) + 63u) / 64u;
+    const uint64_t w = waves < need ? waves : need;
+    const uint32_t wpb = block / 64u;
+    const uint32_t grid = static_cast<uint32_t>((w + wpb - 1) / wpb);
+    hipLaunchKernelGGL(propagate_persistent_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+                       field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
+    return to_status(hipGetLastError());
 }
 
 }  // namespace
@@ -112,6 +282,35 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     if (!rc) rc = upload(d->grids, d->n_grids, &h->blocks[5]);
     if (!rc) rc = upload(d->bf_surfaces, d->n_bf, &h->blocks[6]);
     if (!rc) rc = upload(d->grid_bins, d->n_bins, &h->blocks[7]);
+    if (!rc) {
+        for (uint32_t k = 0; k < d->n_surfaces && !rc; ++k) {
+            const dtr_surface& sf = d->surfaces[k];
+            if (sf.transform >= d->n_transforms || sf.mask >= d->n_masks || sf.n_masks == 0u ||
+                sf.mask + sf.n_masks > d->n_masks)
+                rc = DTR_ERROR_INVALID_ARGUMENT;
+        }
+        if (!rc) {
+            std::vector<dtr::NavSurface> nav(d->n_surfaces);
+            for (uint32_t k = 0; k < d->n_surfaces; ++k) nav[k] = dtr::make_nav_surface(*d, k);
+            for (uint32_t k = 0; k < d->n_bins && !rc; ++k)
+                if (d->grid_bins[k] != 0xffffffffu && d->grid_bins[k] >= d->n_surfaces)
+                    rc = DTR_ERROR_INVALID_ARGUMENT;
+            std::vector<dtr::BinSphere> sph(d->n_bins);
+            std::vector<float> gtolk(d->n_grids);
+            if (!rc) dtr::make_bin_spheres(*d, nav.data(), sph.data(), gtolk.data());
+            if (!rc) rc = upload(nav.data(), d->n_surfaces, &h->blocks[8]);
+            if (!rc) rc = upload(sph.data(), d->n_bins, &h->blocks[9]);
+            if (!rc) rc = upload(gtolk.data(), d->n_grids, &h->blocks[10]);
+        }
+    }
+    if (!rc) {
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) rc = to_status(e);
+        h->device = dev;
+        h->n_cu = cus > 0 ? static_cast<uint32_t>(cus) : 1u;
+    }
     if (rc) {
         dtr_detector_destroy(h);
         return rc;
@@ -124,7 +323,18 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     h->dev.grids = static_cast<const dtr_grid*>(h->blocks[5]);
     h->dev.bf = static_cast<const uint32_t*>(h->blocks[6]);
     h->dev.bins = static_cast<const uint32_t*>(h->blocks[7]);
+    h->fast.sf = static_cast<const dtr::NavSurface*>(h->blocks[8]);
+    h->fast.masks = h->dev.masks;
+    h->fast.materials = h->dev.materials;
+    h->fast.volumes = h->dev.volumes;
+    h->fast.grids = h->dev.grids;
+    h->fast.transforms = h->dev.transforms;
+    h->fast.bf = h->dev.bf;
+    h->fast.bins = h->dev.bins;
+    h->fast.spheres = static_cast<const dtr::BinSphere*>(h->blocks[9]);
+    h->fast.grid_tolk = static_cast<const float*>(h->blocks[10]);
     h->n_volumes = d->n_volumes;
+    h->n_surfaces = d->n_surfaces;
     *out = h;
     return DTR_SUCCESS;
 }
@@ -184,34 +394,9 @@ int dtr_field_destroy(dtr_field* f) {
 
 int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
                   const dtr_track_batch* in, dtr_result_batch* out, void* stream) {
-    if (!det || !field || !c || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
-    if (in->n == 0u) return DTR_SUCCESS;
-    if (!in->free_params && !in->bound_params) return DTR_ERROR_INVALID_ARGUMENT;
-    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
-    if (c->max_surfaces != 0u && (!out->surfaces || !out->n_surfaces)) return DTR_ERROR_INVALID_ARGUMENT;
-    if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
-    if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
-    dtr::DevConfig dc{};
-    dc.min_stepsize = c->min_stepsize;
-    dc.rk_error_tol = c->rk_error_tol;
-    dc.step_constraint = c->step_constraint;
-    dc.path_limit = c->path_limit;
-    dc.max_rk_updates = c->max_rk_updates;
-    dc.min_mask_tol = c->min_mask_tolerance;
-    dc.max_mask_tol = c->max_mask_tolerance;
-    dc.mask_scalor = c->mask_tolerance_scalor;
-    dc.path_tol = c->path_tolerance;
-    dc.overstep_tol = c->overstep_tolerance;
-    dc.win0 = c->search_window[0];
-    dc.win1 = c->search_window[1];
-    dc.accumulated_error = c->accumulated_error;
-    dc.n_stddev = static_cast<double>(c->n_scattering_stddev);
-    dc.max_surfaces = c->max_surfaces;
-    dc.do_cov = c->do_covariance_transport;
-    dc.estimate_noise = c->estimate_scattering_noise;
-    dc.do_eloss = c->do_energy_loss;
-    dc.do_ms = c->do_multiple_scattering;
-    dc.use_material = c->use_material;
+    const int rc = check_propagate_args(det, field, c, in, out);
+    if (rc != DTR_SUCCESS || in->n == 0u) return rc;
+    const dtr::DevConfig dc = make_dev_config(c);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
@@ -222,6 +407,44 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
                        in->free_params, in->initial_cov, in->bound_params, in->start_surface,
                        r, in->n);
     return to_status(hipGetLastError());
+}
+
+void dtr_default_launch(dtr_launch_config* l) {
+    if (!l) return;
+    std::memset(l, 0, sizeof(*l));
+    l->precision = DTR_PRECISION_MIXED;
+    l->schedule = DTR_SCHEDULE_PERSISTENT;
+    l->block_size = 256u;
+    l->waves_per_simd = 1u;
+    l->tracks_per_claim = 64u;
+    l->min_idle = 1u;
+}
+
+int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                     const dtr_track_batch* in, dtr_result_batch* out, const dtr_launch_config* launch,
+                     void* stream) {
+    const int rc = check_propagate_args(det, field, c, in, out);
+    if (rc != DTR_SUCCESS) return rc;
+    if (!launch) return DTR_ERROR_INVALID_ARGUMENT;
+    dtr_launch_config L = *launch;
+    if (L.block_size == 0u) L.block_size = 256u;
+    if (L.waves_per_simd == 0u) L.waves_per_simd = 1u;
+    if (L.tracks_per_claim == 0u) L.tracks_per_claim = 64u;
+    if (L.min_idle == 0u) L.min_idle = 1u;
+    if (L.min_idle > 64u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.tracks_per_claim % 64u != 0u || L.waves_per_simd > 32u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.schedule != DTR_SCHEDULE_GRID && L.schedule != DTR_SCHEDULE_PERSISTENT) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.schedule == DTR_SCHEDULE_PERSISTENT && !L.queue) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->n == 0u) return DTR_SUCCESS;
+    const dtr::DevConfig dc = make_dev_config(c);
+    dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                      c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                      out->n_trials, out->status};
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
+    return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);
 }
 
 }  // extern "C"

@@ -1,0 +1,1370 @@
+// Batched propagation engine: the same algorithm as propagator.hpp (detray's
+// propagate loop, rk_stepper, caching_navigator, actor chain), re-laid out for
+// CDNA throughput:
+//
+//  * compact per-surface navigation records (NavSurface): translation,
+//    rotation, first mask and links in one 160-B record, so a candidate costs
+//    one dependent gather (bin -> record) instead of three
+//    (bin -> surface -> transform/mask);
+//  * the 8-slot candidate cache lives in LDS, slot-major ([slot][lane]), so a
+//    wave's accesses to one slot hit 64 consecutive banks;
+//  * grid searches gather one bin row (<= 8 bins) at a time with independent
+//    loads and run an exact, conservative prefilter (distance of the surface
+//    centre from the tangent line against the mask circumradius plus the
+//    largest tolerance) before the full intersection + mask check;
+//  * a precision policy P: FP64 everywhere, or mixed (FP32 Runge-Kutta
+//    stages, field interpolation, Jacobian and covariance transport and
+//    material noise terms; FP64 free track state, path length, step-size
+//    error estimate, mean energy loss and navigation geometry).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <limits>
+
+#include "detray_amd.h"
+#include "device/propagator.hpp"
+
+namespace dtr {
+
+// Track-state code is forced inline so that the per-track state stays in
+// registers (a call that takes its address would move it to scratch).
+#define DTR_AI __host__ __device__ inline __attribute__((always_inline))
+
+// Compact navigation record of one surface (built from the detector tables
+// when the detector is uploaded).
+struct alignas(16) NavSurface {
+    double t[3];       // translation
+    double r[9];       // rotation, row-major (columns = local axes)
+    double m[4];       // values of the first mask
+    double radius;     // circumradius of the (planar) first mask, 0 if none
+    uint64_t barcode;
+    uint32_t mask;     // first mask in the mask table
+    uint32_t material; // slab index or kInvalid
+    uint16_t link;     // volume link of the first mask
+    uint16_t volume;
+    uint8_t shape, sf_id, n_masks, pad;
+    // multiple of the largest mask tolerance by which a probably-inside
+    // intersection can lie beyond `radius` (0 disables the prefilter)
+    float tolk;
+};
+static_assert(sizeof(NavSurface) == 176, "NavSurface layout");
+
+// Prefilter sphere of the surface in one grid bin (same index as the bin
+// table): centre and an enlarged circumradius in FP32, r < 0 for an empty
+// bin, r = +inf for a surface that is always evaluated.  One 16-B load per
+// bin decides whether the surface can be probably-inside.
+struct alignas(16) BinSphere {
+    float c[3];
+    float r;
+};
+
+struct FastDetector {
+    const NavSurface* sf;
+    const dtr_mask* masks;
+    const dtr_material_slab* materials;
+    const dtr_volume* volumes;
+    const dtr_grid* grids;
+    const dtr_transform* transforms;
+    const uint32_t* bf;
+    const uint32_t* bins;
+    const BinSphere* spheres;  // [n_bins]
+    const float* grid_tolk;    // [n_grids] largest tolerance multiple of the grid's surfaces
+};
+
+struct PolicyFP64 {
+    using S = double;
+    static constexpr bool kMixed = false;
+};
+struct PolicyMixed {
+    using S = float;
+    static constexpr bool kMixed = true;
+};
+
+// Host: the navigation record of surface s of a flattened detector.  The
+// prefilter bound (radius, tolk) covers every point a mask check of the
+// first mask can accept: rectangles |x| <= hx + t, |y| <= hy + t lie within
+// hypot(hx, hy) + sqrt(2) t; trapezoids, whose x bound grows with y beyond
+// the mask edge by t |hx_high - hx_low| / (2 hy), within hypot(max hx, hy) +
+// sqrt((1 + slope)^2 + 1) t; rings within r_outer + t.
+inline NavSurface make_nav_surface(const dtr_detector_desc& d, uint32_t s) {
+    const dtr_surface& sf = d.surfaces[s];
+    const dtr_transform& t = d.transforms[sf.transform];
+    const dtr_mask& m = d.masks[sf.mask];
+    NavSurface r{};
+    for (int i = 0; i < 3; ++i) r.t[i] = t.t[i];
+    for (int i = 0; i < 9; ++i) r.r[i] = t.r[i];
+    for (int i = 0; i < 4; ++i) r.m[i] = m.v[i];
+    r.barcode = sf.barcode;
+    r.mask = sf.mask;
+    r.material = sf.material;
+    r.link = m.volume_link;
+    r.volume = sf.volume;
+    r.shape = sf.shape;
+    r.sf_id = sf.sf_id;
+    r.n_masks = sf.n_masks;
+    r.radius = 0.0;
+    r.tolk = 0.f;
+    if (sf.n_masks == 1) {
+        if (sf.shape == DTR_RECTANGLE2D) {
+            r.radius = sqrt(m.v[0] * m.v[0] + m.v[1] * m.v[1]);
+            r.tolk = 2.f;
+        } else if (sf.shape == DTR_TRAPEZOID2D) {
+            const double hx = m.v[0] > m.v[1] ? m.v[0] : m.v[1];
+            const double slope = fabs(m.v[1] - m.v[0]) * fabs(m.v[3]);
+            r.radius = sqrt(hx * hx + m.v[2] * m.v[2]);
+            r.tolk = static_cast<float>(sqrt((1.0 + slope) * (1.0 + slope) + 1.0) * 1.01 + 0.01);
+        } else if (sf.shape == DTR_RING2D) {
+            r.radius = m.v[1];
+            r.tolk = 2.f;
+        }
+    }
+    return r;
+}
+
+// Host: the prefilter spheres of all grid bins and the largest tolerance
+// multiple per grid.  FP32 rounding of the centre, of the track position and
+// of the FP32 distance computation is covered by enlarging the radius by
+// 1e-5 |c|_1 here and by 1e-5 |p|_1 + 1e-3 mm at the test.
+inline void make_bin_spheres(const dtr_detector_desc& d, const NavSurface* nav, BinSphere* sph,
+                             float* grid_tolk) {
+    for (uint32_t k = 0; k < d.n_bins; ++k) {
+        const uint32_t s = d.grid_bins[k];
+        BinSphere b{{0.f, 0.f, 0.f}, -1.f};
+        if (s != 0xffffffffu) {
+            const NavSurface& r = nav[s];
+            if (r.tolk == 0.f) {
+                b.r = std::numeric_limits<float>::infinity();
+            } else {
+                for (int i = 0; i < 3; ++i) b.c[i] = static_cast<float>(r.t[i]);
+                const double rr = r.radius + 1e-5 * (fabs(r.t[0]) + fabs(r.t[1]) + fabs(r.t[2]));
+                float rf = static_cast<float>(rr);
+                if (static_cast<double>(rf) < rr) rf = nextafterf(rf, std::numeric_limits<float>::infinity());
+                b.r = rf;
+            }
+        }
+        sph[k] = b;
+    }
+    for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
+        const dtr_grid& g = d.grids[gi];
+        float t = 0.f;
+        for (uint32_t k = 0; k < g.n0 * g.n1; ++k) {
+            const uint32_t s = d.grid_bins[g.bins + k];
+            if (s != 0xffffffffu && nav[s].tolk > t) t = nav[s].tolk;
+        }
+        grid_tolk[gi] = t;
+    }
+}
+
+template <typename S>
+struct T3 {
+    S x, y, z;
+};
+template <typename S>
+DTR_HD T3<S> tadd(T3<S> a, T3<S> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename S>
+DTR_HD T3<S> tsub(T3<S> a, T3<S> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename S>
+DTR_HD T3<S> tmul(S s, T3<S> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename S>
+DTR_HD T3<S> tcross(T3<S> a, T3<S> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <typename S>
+DTR_HD S tdot(T3<S> a, T3<S> b) {
+    S r = S(0);
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+template <typename S>
+DTR_HD T3<S> tcast(V3 v) { return {static_cast<S>(v.x), static_cast<S>(v.y), static_cast<S>(v.z)}; }
+template <typename S>
+DTR_HD V3 tdbl(T3<S> v) { return {static_cast<double>(v.x), static_cast<double>(v.y), static_cast<double>(v.z)}; }
+
+// Field lookup in the stage precision
+template <typename S>
+DTR_HD T3<S> field_at_s(const DevField& f, V3 p) {
+    if (f.kind == DTR_FIELD_CONSTANT)
+        return {static_cast<S>(f.b[0]), static_cast<S>(f.b[1]), static_cast<S>(f.b[2])};
+    const double pc[3] = {p.x, p.y, p.z};
+    int i0[3];
+    S t[3];
+    for (int a = 0; a < 3; ++a) {
+        const int n = static_cast<int>(f.n[a]);
+        double u = (pc[a] - f.min[a]) / f.pitch[a];
+        u = (u > 0.0) ? (u > n - 1 ? static_cast<double>(n - 1) : u) : 0.0;
+        int i = static_cast<int>(floor(u));
+        i = i > n - 2 ? n - 2 : i;
+        i0[a] = i;
+        t[a] = static_cast<S>(u - i);
+    }
+    const uint64_t ny = f.n[1], nz = f.n[2];
+    const uint64_t base = (static_cast<uint64_t>(i0[0]) * ny + i0[1]) * nz + i0[2];
+    const uint64_t sx = ny * nz, sy = nz;
+    const float* v000 = f.values + 3 * base;
+    S out[3];
+    for (int c = 0; c < 3; ++c) {
+        const S a000 = v000[c], a100 = v000[3 * sx + c];
+        const S a001 = v000[3 + c], a101 = v000[3 * sx + 3 + c];
+        const S a010 = v000[3 * sy + c], a110 = v000[3 * (sx + sy) + c];
+        const S a011 = v000[3 * sy + 3 + c], a111 = v000[3 * (sx + sy) + 3 + c];
+        const S one = S(1);
+        const S c00 = a000 * (one - t[0]) + a100 * t[0];
+        const S c01 = a001 * (one - t[0]) + a101 * t[0];
+        const S c10 = a010 * (one - t[0]) + a110 * t[0];
+        const S c11 = a011 * (one - t[0]) + a111 * t[0];
+        const S c0 = c00 * (one - t[1]) + c10 * t[1];
+        const S c1 = c01 * (one - t[1]) + c11 * t[1];
+        out[c] = c0 * (one - t[2]) + c1 * t[2];
+    }
+    return {out[0], out[1], out[2]};
+}
+
+// Candidate cache view: slot i of this lane at base[i * stride].
+struct CacheView {
+    double* path;
+    uint32_t* sf;
+    uint32_t* ls;  // volume link (low 16 bits) | status (bits 16-23)
+    int stride;
+    DTR_HD double& p(int i) const { return path[i * stride]; }
+    DTR_HD uint32_t& s(int i) const { return sf[i * stride]; }
+    DTR_HD uint32_t& l(int i) const { return ls[i * stride]; }
+    DTR_HD uint8_t status(int i) const { return static_cast<uint8_t>(l(i) >> 16); }
+    DTR_HD uint16_t link(int i) const { return static_cast<uint16_t>(l(i) & 0xffffu); }
+    DTR_HD void copy(int dst, int src) const {
+        p(dst) = p(src);
+        s(dst) = s(src);
+        l(dst) = l(src);
+    }
+    DTR_HD void set(int i, const Cand& c) const {
+        p(i) = c.path;
+        s(i) = c.sf;
+        l(i) = static_cast<uint32_t>(c.link) | (static_cast<uint32_t>(c.status) << 16);
+    }
+    DTR_HD Cand get(int i) const {
+        Cand c;
+        c.path = p(i);
+        c.sf = s(i);
+        c.link = link(i);
+        c.status = status(i);
+        c.pad = 0;
+        return c;
+    }
+};
+
+template <class P>
+struct FastTrack {
+    using S = typename P::S;
+    // navigation
+    double ext_tol;
+    int8_t next, last, status;
+    uint8_t trust;
+    uint16_t volume;
+    bool path_limit_hit, overflow, stopped, heartbeat;
+    // free track parameters (FP64 state)
+    V3 pos, dir;
+    double time, qop;
+    // stepping
+    double step_size, next_step_size, path_length, abs_path_length;
+    double constraint_acc, constraint_abort;
+    T3<S> dtds3;
+    double mass, charge;
+    uint32_t n_trials, n_steps;
+    // transport Jacobian (reduced) and covariance in the stage precision
+    S J[25];
+    S cov[21];
+    double bound[6];
+    uint32_t up_sf;
+    uint32_t n_seq;
+};
+
+// The navigation fields of a track: what local navigation reads and writes.
+struct NavLite {
+    double ext_tol;
+    int8_t next, last, status;
+    uint8_t trust;
+    uint16_t volume;
+    V3 pos, dir;
+};
+
+template <class P>
+DTR_HD uint32_t local_navigation_ool(const FastDetector& det, const DevConfig& cfg, V3 pos, V3 dir,
+                                     double ext_tol, uint16_t volume, float overstep, CacheView c);
+
+// TK is the track type: FastTrack<P> for the propagation loop, NavLite inside
+// the out-of-line local navigation.
+template <class P, class TK = FastTrack<P>>
+struct FastEngine {
+    using S = typename P::S;
+    const FastDetector& det;
+    const DevField& field;
+    const DevConfig& cfg;
+    TK& tk;
+    CacheView c;
+
+    DTR_AI bool on_surface() const { return tk.status == kOnObject || tk.status == kOnPortal; }
+    DTR_AI int n_cand() const { return tk.last - tk.next + 1; }
+    DTR_AI bool exhausted() const { return n_cand() == 0; }
+    DTR_AI int begin_idx() const { return (on_surface() && tk.next >= 1) ? tk.next - 1 : tk.next; }
+    DTR_AI double nav_dist() const { return c.p(tk.next); }
+    DTR_AI void set_high_trust() { tk.trust = tk.trust < kHigh ? tk.trust : kHigh; }
+    DTR_AI void set_fair_trust() { tk.trust = tk.trust < kFair ? tk.trust : kFair; }
+    DTR_AI bool is_portal(uint32_t s) const { return det.sf[s].sf_id == DTR_PORTAL; }
+    DTR_AI uint32_t cur_sf() const { return c.s(tk.next - 1); }
+
+    // -------------------------------------------------------- intersections --
+    static DTR_AI V3 rec_axis(const NavSurface& r, int k) { return {r.r[k], r.r[3 + k], r.r[6 + k]}; }
+    static DTR_AI V3 rec_to_local(const NavSurface& r, V3 p) {
+        const V3 d{p.x - r.t[0], p.y - r.t[1], p.z - r.t[2]};
+        return {r.r[0] * d.x + r.r[3] * d.y + r.r[6] * d.z, r.r[1] * d.x + r.r[4] * d.y + r.r[7] * d.z,
+                r.r[2] * d.x + r.r[5] * d.y + r.r[8] * d.z};
+    }
+    static DTR_AI V3 rec_to_global(const NavSurface& r, V3 l) {
+        return {r.r[0] * l.x + r.r[1] * l.y + r.r[2] * l.z + r.t[0],
+                r.r[3] * l.x + r.r[4] * l.y + r.r[5] * l.z + r.t[1],
+                r.r[6] * l.x + r.r[7] * l.y + r.r[8] * l.z + r.t[2]};
+    }
+
+    // ray_plane_intersector.hpp:60-87 on a record
+    static DTR_AI bool plane_rec(V3 ro, V3 rd, const NavSurface& r, double& s, V3& glob) {
+        const V3 sn = rec_axis(r, 2);
+        const V3 st{r.t[0], r.t[1], r.t[2]};
+        const double denom = dot(rd, sn);
+        if (denom == 0.0) {
+            s = kMax;
+            glob = {0.0, 0.0, 0.0};
+            return false;
+        }
+        s = dot(sn, sub(st, ro)) / denom;
+        glob = add(ro, mul(s, rd));
+        return fabs(s) < 10.0 * 1000.0;
+    }
+
+    // intersector_base.hpp:171-252 resolve_mask
+    DTR_AI void resolve(Cand& is, double path, V3 glob, double z2d, uint32_t sf_idx, uint8_t sf_id,
+                        const double* mv, uint8_t shape, uint16_t link, const NavSurface& rec,
+                        float overstep) const {
+        is.status = kOutside;
+        if (!(path >= static_cast<double>(overstep))) return;
+        double base_tol = 0.0, ext_tol = 0.0;
+        if (sf_id != DTR_PORTAL || cfg.min_mask_tol == 3.40282347e+38f) {
+            ext_tol = tk.ext_tol;
+            base_tol = dmax(static_cast<double>(cfg.min_mask_tol),
+                            dmin(static_cast<double>(cfg.max_mask_tol),
+                                 static_cast<double>(cfg.mask_scalor) * fabs(path)));
+        }
+        bool inside, edge;
+        if (shape == DTR_CONCENTRIC_CYLINDER2D) {
+            inside = (mv[1] - base_tol <= z2d && z2d <= mv[2] + base_tol);
+            edge = inside;
+        } else {
+            const V3 loc = rec_to_local(rec, glob);
+            if (shape == DTR_RECTANGLE2D) {
+                const double l0 = fabs(loc.x), l1 = fabs(loc.y);
+                inside = (l0 <= (mv[0] + base_tol) && l1 <= (mv[1] + base_tol));
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = (l0 <= (mv[0] + ft) && l1 <= (mv[1] + ft));
+                }
+            } else if (shape == DTR_TRAPEZOID2D) {
+                const double rel_y = (mv[2] + loc.y) * mv[3];
+                const double bound_x = mv[0] + rel_y * (mv[1] - mv[0]);
+                inside = (fabs(loc.x) <= (bound_x + base_tol) && fabs(loc.y) <= (mv[2] + base_tol));
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (mv[2] + ft));
+                }
+            } else {
+                const double r = perp(loc);
+                inside = ((r + base_tol) >= mv[0] && r <= (mv[1] + base_tol));
+                edge = inside;
+            }
+        }
+        if (edge) is.status = kEdge;
+        if (inside) is.status = kInside;
+        is.path = path;
+        is.sf = sf_idx;
+        is.link = link;
+    }
+
+    // Intersect surface s and resolve its masks until one is probably inside
+    // (intersection_kernel.hpp:24-168 for init, 172-272 for update).
+    // Returns true if a probably-inside result was produced in `out`.
+    DTR_AI bool evaluate(uint32_t s, float overstep, Cand& out) const {
+        const NavSurface& rec = det.sf[s];
+        double path, z = 0.0;
+        V3 glob{0.0, 0.0, 0.0};
+        bool valid;
+        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            valid = Engine::cyl_intersect(tk.pos, tk.dir, rec.m[0], overstep, path, z);
+        } else {
+            valid = plane_rec(tk.pos, tk.dir, rec, path, glob);
+        }
+        if (!valid) return false;
+        if (rec.n_masks == 1) {
+            resolve(out, path, glob, z, s, rec.sf_id, rec.m, rec.shape, rec.link, rec, overstep);
+            return out.status <= kEdge;
+        }
+        for (uint32_t k = 0; k < rec.n_masks; ++k) {
+            const dtr_mask& m = det.masks[rec.mask + k];
+            resolve(out, path, glob, z, s, rec.sf_id, m.v, rec.shape, m.volume_link, rec, overstep);
+            if (out.status <= kEdge) return true;
+        }
+        return false;
+    }
+
+    // caching_navigator.hpp:149-229 on the LDS cache
+    DTR_AI void insert(int pos, const Cand& nc) {
+        if (pos == kCache) return;
+        if (n_cand() == 0) {
+            c.set(0, nc);
+            tk.last = static_cast<int8_t>(tk.last + 1);
+            return;
+        }
+        int idx = pos;
+        const uint64_t nbar = det.sf[nc.sf].barcode;
+        auto overlap = [&](int i) { return fabs(c.p(i) - nc.path) <= 1.0 * 1e-3; };
+        auto clash = [&](int i) {
+            const uint32_t si = c.s(i);
+            const uint64_t a = si == kInvalid ? ~0ull : det.sf[si].barcode;
+            return a == nbar && overlap(i);
+        };
+        if (clash(idx) || (idx > 0 && clash(idx - 1))) return;
+        const int shift_begin = tk.last < kCache - 2 ? tk.last : kCache - 2;
+        if (overlap(idx) && (nc.status == kEdge || is_portal(nc.sf))) {
+            idx++;
+            idx = idx > shift_begin ? shift_begin : idx;
+        }
+        for (int i = shift_begin; i >= idx; --i) c.copy(i + 1, i);
+        c.set(idx, nc);
+        tk.last = static_cast<int8_t>(tk.last + 1 < kCache - 1 ? tk.last + 1 : kCache - 1);
+    }
+
+    DTR_AI void insert_sorted(const Cand& is) {
+        int p = begin_idx();
+        const int e = tk.last + 1;
+        while (p < e && !(fabs(is.path) < fabs(c.p(p)))) ++p;
+        insert(p, is);
+    }
+
+    DTR_AI void candidate_search(uint32_t s, float overstep) {
+        Cand is;
+        is.path = kMax;
+        is.sf = kInvalid;
+        is.link = 0;
+        is.status = kOutside;
+        is.pad = 0;
+        if (evaluate(s, overstep, is)) insert_sorted(is);
+    }
+
+    DTR_AI bool update_candidate(int slot) {
+        const uint32_t s = c.s(slot);
+        if (s == kInvalid) return false;
+        Cand cd = c.get(slot);
+        const NavSurface& rec = det.sf[s];
+        double path, z = 0.0;
+        V3 glob{0.0, 0.0, 0.0};
+        bool valid;
+        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            valid = Engine::cyl_intersect(tk.pos, tk.dir, rec.m[0], cfg.overstep_tol, path, z);
+        } else {
+            valid = plane_rec(tk.pos, tk.dir, rec, path, glob);
+        }
+        if (!valid) return false;
+        bool ok = false;
+        if (rec.n_masks == 1) {
+            resolve(cd, path, glob, z, s, rec.sf_id, rec.m, rec.shape, rec.link, rec, cfg.overstep_tol);
+            ok = cd.status <= kEdge;
+        } else {
+            for (uint32_t k = 0; k < rec.n_masks; ++k) {
+                const dtr_mask& m = det.masks[rec.mask + k];
+                resolve(cd, path, glob, z, s, rec.sf_id, m.v, rec.shape, m.volume_link, rec,
+                        cfg.overstep_tol);
+                if (cd.status <= kEdge) {
+                    ok = true;
+                    break;
+                }
+            }
+        }
+        c.set(slot, cd);
+        return ok;
+    }
+
+    // spatial_grid.hpp:100-202 with a row-batched gather and an exact
+    // conservative prefilter: a surface whose centre is farther from the
+    // tangent line than its mask circumradius plus twice the largest
+    // tolerance cannot be probably-inside, so it is not evaluated.
+    DTR_AI void grid_search(const dtr_volume& vol, float overstep) {
+        const dtr_grid& g = det.grids[vol.grid];
+        double l0, l1;
+        if (g.type == DTR_GRID_CYLINDER) {
+            double path, z;
+            Engine::cyl_intersect(tk.pos, tk.dir, g.radius, -kMax, path, z);
+            l0 = phi_of(add(tk.pos, mul(path, tk.dir)));
+            l1 = z;
+        } else {
+            const dtr_transform& vt = det.transforms[vol.transform];
+            double s;
+            V3 glob;
+            Engine::plane_intersect(tk.pos, tk.dir, vt, s, glob);
+            const V3 loc = to_local(vt, glob);
+            l0 = perp(loc);
+            l1 = phi_of(loc);
+        }
+        const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
+        const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
+        const double f0 = (l0 - g.min0) / w0 + 1.0, f1 = (l1 - g.min1) / w1 + 1.0;
+        if (!(fabs(f0) < 1e9) || !(fabs(f1) < 1e9)) return;
+        const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
+        const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
+        const int lb0 = ib0 - static_cast<int>(cfg.win0), ub0 = ib0 + static_cast<int>(cfg.win1);
+        const int lb1 = ib1 - static_cast<int>(cfg.win0), ub1 = ib1 + static_cast<int>(cfg.win1);
+        const bool cyl = (g.type == DTR_GRID_CYLINDER);
+        const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
+        int a0, e0, a1, e1;
+        if (cyl) {
+            a0 = lb0;
+            e0 = ub0 + 1;
+            int mn = lb1 > 0 ? lb1 : 0;
+            mn = mn >= n1 ? n1 - 1 : mn;
+            int mx = ub1 >= n1 ? n1 - 1 : ub1;
+            mx = mx < 0 ? 0 : mx;
+            a1 = mn;
+            e1 = mx + 1;
+        } else {
+            int mn = lb0 > 0 ? lb0 : 0;
+            mn = mn >= n0 ? n0 - 1 : mn;
+            int mx = ub0 >= n0 ? n0 - 1 : ub0;
+            mx = mx < 0 ? 0 : mx;
+            a0 = mn;
+            e0 = mx + 1;
+            a1 = lb1;
+            e1 = ub1 + 1;
+        }
+        // largest tolerance a module mask check can use
+        const double tol_max = dmax(static_cast<double>(cfg.min_mask_tol), static_cast<double>(cfg.max_mask_tol)) +
+                               tk.ext_tol;
+        const V3 ro = tk.pos, rd = tk.dir;
+        const float ox = static_cast<float>(ro.x), oy = static_cast<float>(ro.y), oz = static_cast<float>(ro.z);
+        const float dx = static_cast<float>(rd.x), dy = static_cast<float>(rd.y), dz = static_cast<float>(rd.z);
+        const float margin = static_cast<float>(static_cast<double>(det.grid_tolk[vol.grid]) * tol_max +
+                                                1e-5 * (fabs(ro.x) + fabs(ro.y) + fabs(ro.z)) + 1e-3);
+        constexpr int kRow = 8;
+        for (int b0 = a0; b0 < e0; ++b0) {
+            const int i0 = cyl ? wrap_bin(b0, n0) : b0;
+            const uint32_t row = g.bins + static_cast<uint32_t>(i0) * g.n1;
+            for (int b1s = a1; b1s < e1; b1s += kRow) {
+                const int cnt = (e1 - b1s) < kRow ? (e1 - b1s) : kRow;
+                uint32_t pass = 0u;
+#pragma unroll
+                for (int k = 0; k < kRow; ++k) {
+                    if (k < cnt) {
+                        const int b1 = b1s + k;
+                        const int i1 = cyl ? b1 : wrap_bin(b1, n1);
+                        const BinSphere sp = det.spheres[row + i1];
+                        // distance of the centre from the tangent line
+                        const float cx = sp.c[0] - ox, cy = sp.c[1] - oy, cz = sp.c[2] - oz;
+                        const float xx = cy * dz - dy * cz, xy = cz * dx - dz * cx, xz = cx * dy - dx * cy;
+                        const float dist2 = xx * xx + xy * xy + xz * xz;
+                        const float rb = sp.r + margin;
+                        if (sp.r >= 0.f && !(dist2 > rb * rb)) pass |= (1u << k);
+                    }
+                }
+                while (pass) {
+                    const int k = __builtin_ctz(pass);
+                    pass &= pass - 1u;
+                    const int b1 = b1s + k;
+                    const int i1 = cyl ? b1 : wrap_bin(b1, n1);
+                    candidate_search(det.bins[row + i1], overstep);
+                }
+            }
+        }
+    }
+
+    // circular bin index: b mod n in [0, n)
+    static DTR_AI int wrap_bin(int b, int n) {
+        if (b >= 0 && b < n) return b;
+        if (b < 0 && b >= -n) return b + n;
+        if (b >= n && b < 2 * n) return b - n;
+        return (n + (b % n)) % n;
+    }
+
+    // navigation_functions.hpp:147-177
+    DTR_AI void update_status() {
+        if (!exhausted() && fabs(c.p(tk.next)) < static_cast<double>(cfg.path_tol)) {
+            tk.status = is_portal(c.s(tk.next)) ? kOnPortal : kOnObject;
+            tk.next = static_cast<int8_t>(tk.next + 1);
+        } else {
+            tk.status = kTowards;
+        }
+        tk.trust = (exhausted() || tk.status == kOnPortal) ? kNoTrust : kFull;
+    }
+
+    // navigation_functions.hpp:193-246: the volume's portals and passives
+    // (brute force) and the grid neighbourhood of its sensitives
+    DTR_AI void local_navigation_body(float os) {
+        const dtr_volume& vol = det.volumes[tk.volume];
+        for (int i = 0; i < kCache; ++i) c.p(i) = kMax;
+        tk.next = 0;
+        tk.last = -1;
+        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf[k], os);
+        if (vol.grid >= 0) grid_search(vol, os);
+        update_status();
+        if (tk.trust != kFull && tk.status == kOnPortal)
+            tk.trust = (c.link(tk.next - 1) == kEndOfWorld) ? kFull : kNoTrust;
+    }
+
+    DTR_AI void local_navigation(float overstep_tol, bool resolve_overstepping) {
+        const float os = resolve_overstepping ? overstep_tol : -cfg.path_tol;
+        const uint32_t r = local_navigation_ool<P>(det, cfg, tk.pos, tk.dir, tk.ext_tol, tk.volume, os, c);
+        tk.next = static_cast<int8_t>(r & 0xffu);
+        tk.last = static_cast<int8_t>((r >> 8) & 0xffu);
+        tk.status = static_cast<int8_t>((r >> 16) & 0xffu);
+        tk.trust = static_cast<uint8_t>(r >> 24);
+    }
+
+    DTR_AI void volume_switch() {
+        const uint16_t link = c.link(tk.next - 1);
+        if (link == kEndOfWorld) {
+            tk.status = kExit;
+            return;
+        }
+        if (link != tk.volume) tk.trust = kNoTrust;
+        tk.volume = link;
+        local_navigation(cfg.overstep_tol, true);
+        tk.trust = kFull;
+    }
+
+    DTR_AI void init_loose_cfg() {
+        const float a = 100.f * cfg.overstep_tol, b = -10.f * cfg.max_mask_tol;
+        local_navigation(a < b ? a : b, true);
+        if (tk.trust != kFull || exhausted()) tk.status = kAbort;
+    }
+
+    // caching_navigator.hpp:250-388
+    DTR_AI bool update_impl() {
+        if (tk.trust == kHigh) {
+            if (!update_candidate(tk.next)) {
+                tk.status = kUnknown;
+                set_fair_trust();
+            } else {
+                update_status();
+                if (tk.status == kTowards || tk.status == kOnPortal) return false;
+                if (tk.trust == kFull && update_candidate(tk.next)) return false;
+                set_fair_trust();
+            }
+        }
+        if (tk.trust == kFair && !exhausted()) {
+            const int b = begin_idx(), e = tk.last + 1;
+            for (int i = b; i < e; ++i) {
+                if (!update_candidate(i)) c.p(i) = kMax;
+            }
+            for (int i = b; i < e; ++i) {
+                for (int j = i + 1; j < e; ++j) {
+                    if (fabs(c.p(j)) < fabs(c.p(i))) {
+                        const Cand tmp = c.get(i);
+                        c.copy(i, j);
+                        c.set(j, tmp);
+                    }
+                }
+            }
+            tk.next = static_cast<int8_t>(b);
+            int inv = kCache;
+            for (int i = kCache - 1; i >= 0; --i) {
+                if (c.p(i) == kMax) inv = i;
+            }
+            tk.last = static_cast<int8_t>(inv - 1);
+            update_status();
+            if (exhausted()) tk.trust = kNoTrust;
+        }
+        if (tk.trust == kNoTrust) {
+            local_navigation(cfg.overstep_tol, true);
+            return true;
+        }
+        return false;
+    }
+
+    // navigator_base.hpp:97-154
+    DTR_AI bool nav_update() {
+        if (tk.trust == kFull) return false;
+        bool is_init = update_impl();
+        if (tk.status == kOnPortal) {
+            volume_switch();
+            is_init = true;
+            if (!(tk.status > kUnknown)) return false;
+        }
+        if (tk.trust != kFull || exhausted()) {
+            is_init = true;
+            init_loose_cfg();
+        }
+        return is_init;
+    }
+
+    // ----------------------------------------------------------- stepping --
+    // rk_stepper.hpp:273-530: stages and field in S, free state, path length
+    // and the error estimate in FP64
+    DTR_AI void step(double dist, bool do_reset) {
+        if (fabs(dist) < static_cast<double>(1.f * 1e-3f)) {
+            if (fabs(tk.next_step_size) < cfg.min_stepsize)
+                tk.next_step_size = copysign(cfg.min_stepsize, dist);
+            return;
+        }
+        if (do_reset) {
+            tk.step_size = dist;
+        } else if (tk.next_step_size > 0) {
+            tk.step_size = dmin(tk.next_step_size, dist);
+        } else {
+            tk.step_size = dmax(tk.next_step_size, dist);
+        }
+        const double min_step = dmin(fabs(tk.step_size), cfg.min_stepsize);
+        const V3 pos = tk.pos;
+        const T3<S> dir = tcast<S>(tk.dir);
+        const S qop = static_cast<S>(tk.qop);
+        const T3<S> b1 = field_at_s<S>(field, pos);
+        const T3<S> k1 = tmul(qop, tcross(dir, b1));
+        const V3 k1d = tdbl(k1);
+        T3<S> b2, b3, t2, t3, t4, k2, k3, k4;
+        double error = 1e20;
+        for (uint32_t i = 0; i < cfg.max_rk_updates; ++i) {
+            tk.n_trials++;
+            const double h = tk.step_size;
+            const double h2 = h * h;
+            const double half_h = h * 0.5;
+            const V3 pos1 = add(add(pos, mul(half_h, tk.dir)), mul(h2 * 0.125, k1d));
+            b2 = field_at_s<S>(field, pos1);
+            const S hh = static_cast<S>(half_h), hs = static_cast<S>(h);
+            t2 = (half_h == 0.0) ? dir : tadd(dir, tmul(hh, k1));
+            k2 = tmul(qop, tcross(t2, b2));
+            t3 = (half_h == 0.0) ? dir : tadd(dir, tmul(hh, k2));
+            k3 = tmul(qop, tcross(t3, b2));
+            const V3 pos2 = add(add(pos, mul(h, tk.dir)), mul(h2 * 0.5, tdbl(k3)));
+            b3 = field_at_s<S>(field, pos2);
+            t4 = (h == 0.0) ? dir : tadd(dir, tmul(hs, k3));
+            k4 = tmul(qop, tcross(t4, b3));
+            const V3 d = add(sub(sub(k1d, tdbl(k2)), tdbl(k3)), tdbl(k4));
+            const V3 err = mul((1.0 / 6.0) * h2, d);
+            error = dmax(norm(err), 1e-20);
+            if (error <= 4.0 * cfg.rk_error_tol) break;
+            tk.step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
+        }
+        tk.dtds3 = k4;
+        const double cmin = dmin(tk.constraint_acc, tk.constraint_abort);
+        const double max_step = (tk.step_size >= 0.0 ? 1.0 : -1.0) * cmin;
+        if (fabs(tk.step_size) > fabs(max_step)) tk.step_size = max_step;
+        if (fabs(tk.step_size) < min_step) tk.step_size = copysign(min_step, tk.step_size);
+        tk.next_step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
+        if (fabs(tk.next_step_size) < cfg.min_stepsize)
+            tk.next_step_size = copysign(cfg.min_stepsize, tk.next_step_size);
+        // advance_track (rk_stepper.hpp:594-629)
+        const double h = tk.step_size;
+        const double h_6 = h * (1.0 / 6.0);
+        const V3 ksum = add(add(k1d, tdbl(k2)), tdbl(k3));
+        tk.pos = add(pos, mul(h, add(tk.dir, mul(h_6, ksum))));
+        const V3 kd = add(add(k1d, mul(2.0, add(tdbl(k2), tdbl(k3)))), tdbl(k4));
+        const V3 nd = add(tk.dir, mul(h_6, kd));
+        const double nn = norm(nd);
+        tk.dir = {nd.x / nn, nd.y / nn, nd.z / nn};
+        tk.path_length += h;
+        tk.abs_path_length += fabs(h);
+        if (cfg.do_cov) advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+        tk.n_steps++;
+    }
+
+    // rk_stepper.hpp:633-956 (dqop_n/dqop1 = 1, no field gradient), applied
+    // column by column on the reduced Jacobian
+    DTR_AI void advance_jacobian(T3<S> t1, T3<S> t2, T3<S> t3, T3<S> t4, T3<S> b1, T3<S> b2,
+                                 T3<S> b3, S qop) {
+        const S h = static_cast<S>(tk.step_size);
+        const S half_h = h * S(0.5);
+        const S h_6 = h * static_cast<S>(1.f / 6.f);
+        const T3<S> dk0 = tcross(t1, b1);
+        const T3<S> dk1 = tadd(tcross(t2, b2), tmul(qop * half_h, tcross(dk0, b2)));
+        const T3<S> dk2 = tadd(tcross(t3, b2), tmul(qop * half_h, tcross(dk1, b2)));
+        const T3<S> dk3 = tadd(tcross(t4, b3), tmul(qop * h, tcross(dk2, b3)));
+        const T3<S> dFdqop = tmul(h * h_6, tadd(tadd(dk0, dk1), dk2));
+        const T3<S> dGdqop = tmul(h_6, tadd(tadd(dk0, tmul(S(2), tadd(dk1, dk2))), dk3));
+        // dk_n/dt1 applied to the columns 4..6 of a unit vector e_j:
+        // column j of dFdt / dGdt
+        S dFdt[9], dGdt[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T3<S> e{j == 0 ? S(1) : S(0), j == 1 ? S(1) : S(0), j == 2 ? S(1) : S(0)};
+            const T3<S> d0 = tmul(qop, tcross(e, b1));
+            const T3<S> d1 = tmul(qop, tcross(tadd(e, tmul(half_h, d0)), b2));
+            const T3<S> d2 = tmul(qop, tcross(tadd(e, tmul(half_h, d1)), b2));
+            const T3<S> d3 = tmul(qop, tcross(tadd(e, tmul(h, d2)), b3));
+            const T3<S> f = tmul(h, tadd(e, tmul(h_6, tadd(tadd(d0, d1), d2))));
+            const T3<S> gg = tadd(e, tmul(h_6, tadd(tadd(d0, tmul(S(2), tadd(d1, d2))), d3)));
+            dFdt[0 * 3 + j] = f.x;
+            dFdt[1 * 3 + j] = f.y;
+            dFdt[2 * 3 + j] = f.z;
+            dGdt[0 * 3 + j] = gg.x;
+            dGdt[1 * 3 + j] = gg.y;
+            dGdt[2 * 3 + j] = gg.z;
+        }
+        const S dfq[3] = {dFdqop.x, dFdqop.y, dFdqop.z};
+        const S dgq[3] = {dGdqop.x, dGdqop.y, dGdqop.z};
+        const S jq = tk.J[kJqq];
+#pragma unroll
+        for (int cc = 4; cc < 8; ++cc) {
+            const S jd0 = tk.J[jdir(0, cc)], jd1 = tk.J[jdir(1, cc)], jd2 = tk.J[jdir(2, cc)];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                S s = S(0);
+                s += tk.J[jpos(r, cc)];
+                s += dFdt[r * 3 + 0] * jd0;
+                s += dFdt[r * 3 + 1] * jd1;
+                s += dFdt[r * 3 + 2] * jd2;
+                if (cc == 7) s += dfq[r] * jq;
+                tk.J[jpos(r, cc)] = s;
+            }
+            S g[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                S v = S(0);
+                v += dGdt[r * 3 + 0] * jd0;
+                v += dGdt[r * 3 + 1] * jd1;
+                v += dGdt[r * 3 + 2] * jd2;
+                if (cc == 7) v += dgq[r] * jq;
+                g[r] = v;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) tk.J[jdir(r, cc)] = g[r];
+        }
+    }
+
+    // policies.hpp:99-139
+    DTR_AI void rk_policy() {
+        const double d = nav_dist();
+        if (fabs(d) <= static_cast<double>(1.f * 1e-3f)) {
+            set_high_trust();
+            return;
+        }
+        const double rel = (tk.step_size - d) / d;
+        if (rel > 0.1) tk.trust = kNoTrust;
+        else if (rel > 0.05) set_fair_trust();
+        else set_high_trust();
+    }
+
+    // --------------------------------------------------------------- actors --
+    DTR_AI bool on_sensitive() const {
+        return tk.status == kOnObject && det.sf[cur_sf()].sf_id == DTR_SENSITIVE;
+    }
+    DTR_AI bool on_material() const {
+        return on_surface() && det.sf[cur_sf()].material != kInvalid;
+    }
+    DTR_AI void reset_J() {
+#pragma unroll
+        for (int i = 0; i < 25; ++i) tk.J[i] = S(0);
+        tk.J[jdir(0, 4)] = S(1);
+        tk.J[jdir(1, 5)] = S(1);
+        tk.J[jdir(2, 6)] = S(1);
+        tk.J[kJqq] = S(1);
+    }
+    DTR_AI void free_to_bound(uint32_t s, double out[6]) const {
+        const NavSurface& rec = det.sf[s];
+        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            out[0] = phi_of(tk.pos);
+            out[1] = tk.pos.z;
+        } else if (rec.shape == DTR_RING2D) {
+            const V3 l = rec_to_local(rec, tk.pos);
+            out[0] = perp(l);
+            out[1] = phi_of(l);
+        } else {
+            const V3 l = rec_to_local(rec, tk.pos);
+            out[0] = l.x;
+            out[1] = l.y;
+        }
+        out[2] = phi_of(tk.dir);
+        out[3] = theta_of(tk.dir);
+        out[4] = tk.qop;
+        out[5] = tk.time;
+    }
+    DTR_AI V3 bound_pos(uint32_t s, const double b[6]) const {
+        const NavSurface& rec = det.sf[s];
+        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double r = rec.m[0];
+            return {r * cos(b[0]), r * sin(b[0]), b[1]};
+        }
+        if (rec.shape == DTR_RING2D)
+            return rec_to_global(rec, {b[0] * cos(b[1]), b[0] * sin(b[1]), 0.0});
+        return rec_to_global(rec, {b[0], b[1], 0.0});
+    }
+    DTR_AI T3<S> dtds() const {
+        if (tk.path_length == 0.0) {
+            const T3<S> b = field_at_s<S>(field, tk.pos);
+            return tmul(static_cast<S>(tk.qop), tcross(tcast<S>(tk.dir), b));
+        }
+        return tk.dtds3;
+    }
+
+    // parameter_updater.hpp:422-572 + covariance_transport.hpp in S
+    DTR_AI void transport_covariance(uint32_t dest) {
+        const uint32_t dep = tk.up_sf;
+        const NavSurface& drec = det.sf[dep];
+        S A[6];
+        if (drec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const V3 dp = bound_pos(dep, tk.bound);
+            const double r = perp(dp), ph = phi_of(dp);
+            A[0] = static_cast<S>(-r * sin(ph)); A[1] = S(0);
+            A[2] = static_cast<S>(r * cos(ph));  A[3] = S(0);
+            A[4] = S(0);                         A[5] = S(1);
+        } else {
+            A[0] = static_cast<S>(drec.r[0]); A[1] = static_cast<S>(drec.r[1]);
+            A[2] = static_cast<S>(drec.r[3]); A[3] = static_cast<S>(drec.r[4]);
+            A[4] = static_cast<S>(drec.r[6]); A[5] = static_cast<S>(drec.r[7]);
+        }
+        S Cd[6];
+        {
+            const double th = tk.bound[3], ph = tk.bound[2];
+            const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+            Cd[0] = static_cast<S>(-st * sp); Cd[1] = static_cast<S>(ct * cp);
+            Cd[2] = static_cast<S>(st * cp);  Cd[3] = static_cast<S>(ct * sp);
+            Cd[4] = S(0);                     Cd[5] = static_cast<S>(-st);
+        }
+        S Mp[3][5], Md[3][5];  // column 5 (time) is zero for both blocks
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            Mp[r][0] = A[r * 2 + 0];
+            Mp[r][1] = A[r * 2 + 1];
+            Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
+            Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            Mp[r][4] = tk.J[jpos(r, 7)];
+            Md[r][0] = S(0);
+            Md[r][1] = S(0);
+            Md[r][2] = tk.J[jdir(r, 4)] * Cd[0] + tk.J[jdir(r, 5)] * Cd[2] + tk.J[jdir(r, 6)] * Cd[4];
+            Md[r][3] = tk.J[jdir(r, 4)] * Cd[1] + tk.J[jdir(r, 5)] * Cd[3] + tk.J[jdir(r, 6)] * Cd[5];
+            Md[r][4] = tk.J[jdir(r, 7)];
+        }
+        const S Mq4 = tk.J[kJqq];
+        const T3<S> k = dtds();
+        const NavSurface& erec = det.sf[dest];
+        V3 n;
+        if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double ph = phi_of(tk.pos);
+            n = {cos(ph), sin(ph), 0.0};
+        } else {
+            n = rec_axis(erec, 2);
+        }
+        const T3<S> f2p = tcast<S>(mul(-1.0 / dot(n, tk.dir), n));
+        const S p2f_pos[3] = {static_cast<S>(tk.dir.x), static_cast<S>(tk.dir.y), static_cast<S>(tk.dir.z)};
+        const S p2f_dir[3] = {k.x, k.y, k.z};
+#pragma unroll
+        for (int cc = 0; cc < 5; ++cc) {
+            const S proj = f2p.x * Mp[0][cc] + f2p.y * Mp[1][cc] + f2p.z * Mp[2][cc];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                Mp[r][cc] += p2f_pos[r] * proj;
+                Md[r][cc] += p2f_dir[r] * proj;
+            }
+        }
+        S Bl[6];
+        if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
+            Bl[0] = static_cast<S>(-rinv * sin(ph)); Bl[1] = static_cast<S>(rinv * cos(ph)); Bl[2] = S(0);
+            Bl[3] = S(0);                            Bl[4] = S(0);                           Bl[5] = S(1);
+        } else {
+            Bl[0] = static_cast<S>(erec.r[0]); Bl[1] = static_cast<S>(erec.r[3]); Bl[2] = static_cast<S>(erec.r[6]);
+            Bl[3] = static_cast<S>(erec.r[1]); Bl[4] = static_cast<S>(erec.r[4]); Bl[5] = static_cast<S>(erec.r[7]);
+        }
+        S Ba[6];
+        {
+            const double th = theta_of(tk.dir), ph = phi_of(tk.dir);
+            const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+            Ba[0] = static_cast<S>(-sp / st); Ba[1] = static_cast<S>(cp / st); Ba[2] = S(0);
+            Ba[3] = static_cast<S>(cp * ct);  Ba[4] = static_cast<S>(sp * ct); Ba[5] = static_cast<S>(-st);
+        }
+        // Jf rows 0..3 over columns 0..4; row 4 = Mq4 e4; row 5 = e5
+        S Jf[4][5];
+#pragma unroll
+        for (int cc = 0; cc < 5; ++cc) {
+            Jf[0][cc] = Bl[0] * Mp[0][cc] + Bl[1] * Mp[1][cc] + Bl[2] * Mp[2][cc];
+            Jf[1][cc] = Bl[3] * Mp[0][cc] + Bl[4] * Mp[1][cc] + Bl[5] * Mp[2][cc];
+            Jf[2][cc] = Ba[0] * Md[0][cc] + Ba[1] * Md[1][cc] + Ba[2] * Md[2][cc];
+            Jf[3][cc] = Ba[3] * Md[0][cc] + Ba[4] * Md[1][cc] + Ba[5] * Md[2][cc];
+        }
+        // C' = Jf C Jf^T with the block structure of Jf
+        auto C = [&](int r, int cc) { return tk.cov[covi(r, cc)]; };
+        S T[6][6];  // T = Jf C
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) {
+                S s = S(0);
+#pragma unroll
+                for (int m = 0; m < 5; ++m) s += Jf[r][m] * C(m, cc);
+                T[r][cc] = s;
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) {
+            T[4][cc] = Mq4 * C(4, cc);
+            T[5][cc] = C(5, cc);
+        }
+        S NC[21];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+#pragma unroll
+            for (int cc = r; cc < 6; ++cc) {
+                S s = S(0);
+                if (cc < 4) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) s += T[r][m] * Jf[cc][m];
+                } else if (cc == 4) {
+                    s = T[r][4] * Mq4;
+                } else {
+                    s = T[r][5];
+                }
+                NC[covi(r, cc)] = s;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 21; ++i) tk.cov[i] = NC[i];
+    }
+
+    // pointwise_material_interactor.hpp:147-326 + setter.  The mean energy
+    // loss changes the track state and is evaluated in FP64; the noise terms
+    // (Landau sigma of q/p, Highland theta0) only enter the covariance and are
+    // evaluated in S.
+    DTR_AI void material_interaction(uint32_t dest) {
+        const NavSurface& rec = det.sf[dest];
+        const dtr_material_slab& m = det.materials[rec.material];
+        V3 nrm;
+        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
+        else nrm = rec_axis(rec, 2);
+        const V3 bd = Engine::bound_dir(tk.bound);
+        const double cos_inc = fabs(dot(bd, nrm));
+        if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
+        const double qop = tk.bound[4];
+        const double path_seg = m.thickness / cos_inc;
+        const double mass = tk.mass, q = tk.charge;
+        const double q2b2 = q * q + (mass * qop) * (mass * qop);
+        const double mOverP = mass * fabs(qop / q);
+        const double pOverM = 1.0 / mOverP;
+        const double beta2 = 1.0 / (1.0 + mOverP * mOverP);
+        const double bg = pOverM;
+        const double gamma = sqrt(1.0 + pOverM * pOverM);
+        const double m_e = 0.51099895069 * 1e-3;
+        const double mfrac = m_e / mass;
+        const double Wmax = (2.0 * m_e * bg * bg) / (1.0 + 2.0 * gamma * mfrac + mfrac * mfrac);
+        const double K = 0.307075 * 1e-3 * 1e2;
+        const double ned = m.Z * m.molar_rho;
+        const double eps_pl = 0.5 * K * ned * q2b2;
+        double e_loss = 0.0;
+        S sigma_qop = S(0), theta0 = S(0);
+        if (cfg.do_eloss) {
+            double bb = 0.0;
+            if (eps_pl > 0.0) {
+                double dhalf = 0.0;
+                if (bg >= 10.0) {
+                    const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
+                    dhalf = log(bg * plasma / m.mean_exc) - 0.5;
+                }
+                const double u = 2.0 * m_e * bg * bg;
+                const double A = 0.5 * log(u * Wmax / (m.mean_exc * m.mean_exc));
+                bb = 2.0 * eps_pl * (A - beta2 - dhalf);
+            }
+            e_loss = path_seg * bb;
+            if (cfg.do_cov) {
+                const S eps = static_cast<S>(eps_pl * path_seg);
+                const S fwhm = S(4) * eps;
+                const S sigmaE = S(0.5) * static_cast<S>(0.70710678118654752440) * fwhm /
+                                 sqrt(static_cast<S>(0.69314718055994530942));
+                const S pInv = static_cast<S>(qop / q);
+                sigma_qop = sqrt(static_cast<S>(q2b2)) * pInv * pInv * sigmaE;
+            }
+        }
+        if (cfg.do_ms) {
+            const S x0 = static_cast<S>(m.thickness_in_X0 / cos_inc);
+            const S pinv = static_cast<S>(fabs(qop / q));
+            if (x0 > S(0)) {
+                const S t = sqrt(x0 * static_cast<S>(q2b2));
+                theta0 = static_cast<S>(13.6f) * static_cast<S>(1e-3) * pinv * t *
+                         (S(1) + static_cast<S>(0.038f) * S(2) * log(t));
+            }
+        }
+        if (cfg.do_eloss) {
+            const double massd = tk.mass, qd = tk.charge;
+            const double p = qd / tk.bound[4];
+            const double next_e = sqrt(massd * massd + p * p) - copysign(e_loss, 1.0);
+            const double next_p = (massd < next_e) ? sqrt(next_e * next_e - massd * massd) : 0.0;
+            const double next_qop = (qd != 0.0) ? qd / next_p : 1.0 / next_p;
+            tk.bound[4] = (next_p == 0.0) ? kMax : next_qop;
+            if (next_p == 0.0) tk.stopped = true;
+            if (cfg.do_cov) tk.cov[covi(4, 4)] += sigma_qop * sigma_qop;
+        }
+        if (cfg.do_cov) {
+            const V3 d = Engine::bound_dir(tk.bound);
+            const S var = theta0 * theta0;
+            tk.cov[covi(2, 2)] += (d.z == 1.0) ? std::numeric_limits<S>::max()
+                                               : static_cast<S>(static_cast<double>(var) / (1.0 - d.z * d.z));
+            tk.cov[covi(3, 3)] += var;
+        }
+        tk.pos = bound_pos(dest, tk.bound);
+        tk.dir = Engine::bound_dir(tk.bound);
+        tk.time = tk.bound[5];
+        tk.qop = tk.bound[4];
+        if (cfg.estimate_noise) estimate_noise();
+    }
+
+    DTR_AI void estimate_noise() {
+        const double ns = cfg.n_stddev;
+        const double v0 = tk.cov[covi(0, 0)], v1 = tk.cov[covi(1, 1)];
+        const double dphi = ns * sqrt(static_cast<double>(tk.cov[covi(2, 2)]));
+        const double dth = ns * sqrt(static_cast<double>(tk.cov[covi(3, 3)]));
+        const double pe = tk.bound[2] + dphi, te = tk.bound[3] + dth;
+        const double ste = sin(te);
+        V3 displ{cos(pe) * ste, sin(pe) * ste, cos(te)};
+        const double max_tol = 5.0;
+        const double path = exhausted() ? max_tol : fabs(c.p(tk.next));
+        displ = mul(path, sub(displ, tk.dir));
+        const double p = tk.charge / tk.qop;
+        const double acc = 1.0 / p * cfg.accumulated_error * tk.path_length;
+        const double ext = sqrt(ns * ns * (v0 + v1) + dot(displ, displ) + acc * acc);
+        tk.ext_tol = ext > max_tol ? max_tol : ext;
+    }
+
+    DTR_AI void parameter_updater(const double* init_cov, uint64_t n, uint32_t gid) {
+        if (!(on_sensitive() || on_material())) return;
+        const uint32_t dest = cur_sf();
+        if (tk.up_sf != kInvalid) {
+            if (tk.up_sf != dest) {
+                transport_covariance(dest);
+                if (fabs(tk.path_length) > 0.0) reset_J();
+                free_to_bound(dest, tk.bound);
+                tk.up_sf = dest;
+            }
+        } else {
+            if (init_cov) {
+                for (int k = 0; k < 21; ++k) tk.cov[k] = static_cast<S>(init_cov[k * n + gid]);
+            } else {
+                for (int r = 0; r < 6; ++r)
+                    for (int cc = r; cc < 6; ++cc) tk.cov[covi(r, cc)] = (r == cc) ? S(1) : S(0);
+            }
+            free_to_bound(dest, tk.bound);
+            tk.up_sf = dest;
+        }
+        if (on_material()) material_interaction(dest);
+    }
+};
+
+// Local navigation out of line (it is needed from four places of the
+// navigation cascade and is the largest function); it takes only the
+// navigation fields by value and returns next | last << 8 | status << 16 |
+// trust << 24, so the caller's track state is never addressed.
+template <class P>
+__host__ __device__ __attribute__((noinline)) uint32_t local_navigation_ool(
+    const FastDetector& det, const DevConfig& cfg, V3 pos, V3 dir, double ext_tol, uint16_t volume,
+    float overstep, CacheView c) {
+    NavLite nl;
+    nl.ext_tol = ext_tol;
+    nl.next = 0;
+    nl.last = -1;
+    nl.status = kUnknown;
+    nl.trust = kNoTrust;
+    nl.volume = volume;
+    nl.pos = pos;
+    nl.dir = dir;
+    const DevField no_field{};
+    FastEngine<P, NavLite> e{det, no_field, cfg, nl, c};
+    e.local_navigation_body(overstep);
+    return static_cast<uint32_t>(static_cast<uint8_t>(nl.next)) |
+           (static_cast<uint32_t>(static_cast<uint8_t>(nl.last)) << 8) |
+           (static_cast<uint32_t>(static_cast<uint8_t>(nl.status)) << 16) |
+           (static_cast<uint32_t>(nl.trust) << 24);
+}
+
+// Track inputs of one batch (structure of arrays, parameter-major).
+struct TrackInputs {
+    const double* fp;        // [8][n] free start, or null
+    const double* init_cov;  // [21][n] or null
+    const double* bound0;    // [6][n] bound start, or null
+    const uint32_t* start_sf;
+    uint64_t n;
+};
+
+// propagator::propagate (propagator.hpp:262-400) for one track, split into
+// start / advance / finish so that a persistent wave can swap a finished
+// lane's track for a new one.  `c` is the lane's view of the LDS candidate
+// cache.
+//
+// The loop of propagate() alternates an actor half (even i) and a step half
+// (odd i), with `i % 2 == 0 || heartbeat` as its condition, so an actor half
+// always follows a step half.  advance() runs one actor half and, if the
+// heartbeat survives it, one step half; it returns true once the loop of
+// propagate() would have ended.
+template <class P>
+struct Lane {
+    using S = typename P::S;
+    FastTrack<P> tk;
+    bool is_init;
+    bool use_init_cov;
+    double last_path;
+    uint32_t stall;
+    uint32_t gid;
+
+    DTR_AI void start(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                      const TrackInputs& in, CacheView c, uint32_t track) {
+        gid = track;
+        for (int i = 0; i < kCache; ++i) {
+            c.p(i) = kMax;
+            c.s(i) = kInvalid;
+            c.l(i) = static_cast<uint32_t>(kOutside) << 16;
+        }
+        tk.ext_tol = 0.0;
+        tk.next = 0;
+        tk.last = -1;
+        tk.status = kUnknown;
+        tk.trust = kNoTrust;
+        tk.volume = 0;
+        tk.path_limit_hit = false;
+        tk.overflow = false;
+        tk.stopped = false;
+        tk.step_size = 0.0;
+        tk.next_step_size = 0.0;
+        tk.path_length = 0.0;
+        tk.abs_path_length = 0.0;
+        tk.constraint_acc = dmin(kMax, fabs(cfg.step_constraint));
+        tk.constraint_abort = kMax;
+        tk.dtds3 = {S(0), S(0), S(0)};
+        // muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64)
+        tk.mass = static_cast<double>(105.6583755f * 1e-3f);
+        tk.n_trials = 0;
+        tk.n_steps = 0;
+        for (int i = 0; i < 6; ++i) tk.bound[i] = 0.0;
+        for (int i = 0; i < 21; ++i) tk.cov[i] = S(0);
+        tk.up_sf = kInvalid;
+        tk.n_seq = 0;
+        FastEngine<P> e{det, field, cfg, tk, c};
+        const uint64_t N = in.n;
+        use_init_cov = in.init_cov != nullptr;
+        if (in.bound0) {
+            // bound start (propagator.hpp:121-162, base_stepper.hpp:84-108)
+            const uint32_t s0 = in.start_sf[gid];
+            for (int k = 0; k < 6; ++k) tk.bound[k] = in.bound0[k * N + gid];
+            for (int r = 0; r < 6; ++r)
+                for (int cc = r; cc < 6; ++cc) {
+                    const int k = covi(r, cc);
+                    tk.cov[k] = use_init_cov ? static_cast<S>(in.init_cov[k * N + gid])
+                                             : (r == cc ? S(1) : S(0));
+                }
+            tk.up_sf = s0;
+            tk.pos = e.bound_pos(s0, tk.bound);
+            tk.dir = Engine::bound_dir(tk.bound);
+            tk.time = tk.bound[5];
+            tk.qop = tk.bound[4];
+            tk.volume = det.sf[s0].volume;
+            use_init_cov = false;
+        } else {
+            tk.pos = {in.fp[0 * N + gid], in.fp[1 * N + gid], in.fp[2 * N + gid]};
+            tk.time = in.fp[3 * N + gid];
+            tk.dir = {in.fp[4 * N + gid], in.fp[5 * N + gid], in.fp[6 * N + gid]};
+            tk.qop = in.fp[7 * N + gid];
+        }
+        tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
+        e.reset_J();
+        // navigator_base::init -> local_navigation without overstepping resolution
+        e.local_navigation(cfg.overstep_tol, false);
+        tk.heartbeat = tk.status > kUnknown;
+        is_init = true;
+        last_path = 0.0;
+        stall = 0;
+    }
+
+    DTR_AI bool advance(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                        const TrackInputs& in, const DevResults& out, CacheView c) {
+        FastEngine<P> e{det, field, cfg, tk, c};
+        // actor half: pathlimit_aborter (aborters.hpp:44-75)
+        if (tk.status != kExit) {
+            const double step_limit = cfg.path_limit - fabs(tk.abs_path_length);
+            if (step_limit <= 0.0) {
+                tk.status = kAbort;
+                tk.path_limit_hit = true;
+                tk.heartbeat = false;
+            }
+            tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
+        }
+        // parameter_updater<pointwise_material_interactor>
+        if (cfg.use_material) e.parameter_updater(use_init_cov ? in.init_cov : nullptr, in.n, gid);
+        if (tk.stopped && tk.status > kUnknown) {
+            tk.status = kAbort;
+            tk.heartbeat = false;
+        }
+        // surface_sequencer (surface_sequencer.hpp:78-106)
+        if (cfg.max_surfaces != 0u && (e.on_sensitive() || e.on_material())) {
+            if (tk.n_seq == cfg.max_surfaces) {
+                tk.overflow = true;
+                tk.status = kExit;
+                tk.heartbeat = false;
+            } else {
+                out.surfaces[static_cast<uint64_t>(tk.n_seq) * in.n + gid] = e.cur_sf();
+                tk.n_seq++;
+            }
+        }
+        if (!tk.heartbeat) return true;
+        last_path = tk.path_length;
+        is_init |= e.nav_update();
+        tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
+        if (!tk.heartbeat) return true;
+        // step half
+        const bool reset = e.on_surface() || is_init;
+        e.step(e.nav_dist(), reset);
+        e.rk_policy();
+        is_init = false;
+        if (fabs(tk.path_length) <= fabs(last_path) + static_cast<double>(cfg.path_tol)) {
+            if (stall >= 10u) {
+                tk.heartbeat = false;
+                tk.status = kAbort;
+            }
+            stall++;
+        } else {
+            stall = 0;
+        }
+        is_init |= e.nav_update();
+        tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
+        return false;
+    }
+
+    DTR_AI void finish(const DevResults& out, uint64_t N) const {
+        uint8_t status;
+        if (tk.status == kExit) status = tk.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
+        else status = tk.path_limit_hit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
+        if (out.status) out.status[gid] = status;
+        if (out.bound_params)
+            for (int k = 0; k < 6; ++k) out.bound_params[k * N + gid] = tk.bound[k];
+        if (out.bound_cov)
+            for (int k = 0; k < 21; ++k) out.bound_cov[k * N + gid] = static_cast<double>(tk.cov[k]);
+        if (out.free_params) {
+            out.free_params[0 * N + gid] = tk.pos.x;
+            out.free_params[1 * N + gid] = tk.pos.y;
+            out.free_params[2 * N + gid] = tk.pos.z;
+            out.free_params[3 * N + gid] = tk.time;
+            out.free_params[4 * N + gid] = tk.dir.x;
+            out.free_params[5 * N + gid] = tk.dir.y;
+            out.free_params[6 * N + gid] = tk.dir.z;
+            out.free_params[7 * N + gid] = tk.qop;
+        }
+        if (out.path_length) out.path_length[gid] = tk.path_length;
+        if (out.n_surfaces) out.n_surfaces[gid] = tk.n_seq;
+        if (out.n_steps) out.n_steps[gid] = tk.n_steps;
+        if (out.n_trials) out.n_trials[gid] = tk.n_trials;
+    }
+};
+
+// One track from start to finish.
+template <class P>
+DTR_HD void propagate_fast(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                           const TrackInputs& in, const DevResults& out, uint32_t track, CacheView c) {
+    Lane<P> lane;
+    lane.start(det, field, cfg, in, c, track);
+    while (!lane.advance(det, field, cfg, in, out, c)) {
+    }
+    lane.finish(out, in.n);
+}
+
+}  // namespace dtr

@@ -268,6 +268,39 @@ def generate_tracks(kind: str, n: int, seed: int, **kw) -> np.ndarray:
 
 
 @dataclass
+class LaunchConfig:
+    """Arithmetic and schedule of a batched launch (``dtr_launch_config``).
+
+    precision: "fp64" or "mixed" (FP32 Runge-Kutta stages, field, transport
+    and material noise terms; FP64 track state, path length, step-size error
+    estimate, mean energy loss and navigation).  schedule: "grid" (one work-item per track) or
+    "persistent" (resident waves over a device work queue; a wave takes new
+    tracks once ``min_idle`` of its lanes are idle).  ``order`` is an optional
+    int32 device tensor [n], a permutation giving the order in which tracks
+    are taken (results stay in track order)."""
+
+    precision: str = "mixed"
+    schedule: str = "persistent"
+    block_size: int = 256
+    waves_per_simd: int = 1
+    tracks_per_claim: int = 64
+    order: "object" = None
+    min_idle: int = 1
+
+    def to_c(self, queue_ptr: Optional[int]) -> capi.LaunchConfig:
+        c = capi.LaunchConfig()
+        c.precision = {"fp64": capi.PRECISION_FP64, "mixed": capi.PRECISION_MIXED}[self.precision]
+        c.schedule = {"grid": capi.SCHEDULE_GRID, "persistent": capi.SCHEDULE_PERSISTENT}[self.schedule]
+        c.block_size = int(self.block_size)
+        c.waves_per_simd = int(self.waves_per_simd)
+        c.tracks_per_claim = int(self.tracks_per_claim)
+        c.min_idle = int(self.min_idle)
+        c.order = self.order.data_ptr() if self.order is not None else None
+        c.queue = queue_ptr
+        return c
+
+
+@dataclass
 class PropagationResult:
     """Per-track outputs (device tensors, SoA)."""
 
@@ -285,9 +318,23 @@ class PropagationResult:
 class Propagator:
     """``detray::propagator`` over a batch of tracks on one GPU."""
 
-    def __init__(self, config: Optional[PropagationConfig] = None):
+    def __init__(self, config: Optional[PropagationConfig] = None,
+                 launch: Optional[LaunchConfig] = None):
+        """``launch`` None runs the plain FP64 kernel (``dtr_propagate``);
+        otherwise the batched kernel (``dtr_propagate_ex``) with that launch
+        configuration."""
         self.config = config or PropagationConfig()
+        self.launch = launch
         self._c_config = self.config.to_c()
+        self._queues = {}
+
+    def _queue(self, device) -> int:
+        import torch
+
+        key = str(device)
+        if key not in self._queues:
+            self._queues[key] = torch.zeros(4, dtype=torch.int32, device=device)
+        return self._queues[key].data_ptr()
 
     def allocate(self, n: int, device="cuda") -> PropagationResult:
         import torch
@@ -360,7 +407,16 @@ class Propagator:
                               out.status.data_ptr())
         if stream is None:
             stream = torch.cuda.current_stream(device).cuda_stream
-        capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
-                                     C.byref(tb), C.byref(rb), C.c_void_p(stream)),
-                   "dtr_propagate")
+        if self.launch is None:
+            capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
+                                         C.byref(tb), C.byref(rb), C.c_void_p(stream)),
+                       "dtr_propagate")
+            return out
+        order = self.launch.order
+        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)):
+            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
+        lc = self.launch.to_c(self._queue(device) if self.launch.schedule == "persistent" else None)
+        capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(self._c_config),
+                                        C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
+                   "dtr_propagate_ex")
         return out

@@ -257,6 +257,56 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field,
                   const dtr_prop_config* cfg, const dtr_track_batch* in,
                   dtr_result_batch* out, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched propagation: arithmetic and schedule of the launch               */
+/* ------------------------------------------------------------------------ */
+enum dtr_precision {
+    DTR_PRECISION_FP64 = 0,  /* FP64 throughout */
+    /* FP32 Runge-Kutta stages, field interpolation, transport Jacobian,
+     * covariance transport and material noise terms; FP64 free track state,
+     * path length, the step-size error estimate, the mean energy loss and
+     * navigation geometry */
+    DTR_PRECISION_MIXED = 1
+};
+
+enum dtr_schedule {
+    DTR_SCHEDULE_GRID = 0,       /* one work-item per track, one launch */
+    /* persistent waves over a device work queue: a lane whose track has
+     * ended takes the next queued track */
+    DTR_SCHEDULE_PERSISTENT = 1
+};
+
+typedef struct dtr_launch_config {
+    uint32_t precision;        /* dtr_precision (default MIXED) */
+    uint32_t schedule;         /* dtr_schedule (default PERSISTENT) */
+    uint32_t block_size;       /* 64, 128, 192 or 256 (default 256) */
+    uint32_t waves_per_simd;   /* persistent: resident waves per SIMD, 1..32 (default 1) */
+    uint32_t tracks_per_claim; /* persistent: queue entries a wave claims per atomic,
+                                  multiple of 64 (default 64) */
+    uint32_t min_idle;         /* persistent: idle lanes a wave waits for before it
+                                  takes new tracks, 1..64 (default 1) */
+    /* optional DEVICE permutation [n]: the k-th queue entry / work-item
+     * propagates track order[k]; results stay in track order */
+    const uint32_t* order;
+    /* persistent: DEVICE workspace of one uint32, reset on `stream` by the
+     * call (a kernel, no memset), so the launch can be graph-captured */
+    uint32_t* queue;
+} dtr_launch_config;
+
+/* Defaults of dtr_launch_config. */
+void dtr_default_launch(dtr_launch_config* launch);
+
+/* dtr_propagate with a chosen precision and schedule.  Same actor chain,
+ * navigation and results as dtr_propagate (the same
+ * propagator::propagate per track, propagator.hpp:262-400); the surface
+ * sequence of a track does not depend on the launch configuration.  Does
+ * not allocate and does not synchronise; with the persistent schedule,
+ * calls that share one `queue` must not overlap. */
+int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
+                     const dtr_prop_config* cfg, const dtr_track_batch* in,
+                     dtr_result_batch* out, const dtr_launch_config* launch,
+                     void* stream);
+
 /* Host-side builders (detector emitters and inputs).  The returned tables
  * are owned by the library; free them with dtr_host_detector_free. */
 typedef struct dtr_toy_config {

@@ -61,3 +61,24 @@ def test_invalid_arguments_are_reported():
     bad = capi.ToyConfig(2, 3, 1)  # endcaps need all four barrel layers
     assert lib.dtr_build_toy_detector(C.byref(bad), C.byref(d)) == 1
     assert lib.dtr_propagate(None, None, None, None, None, None) == 1
+
+
+def test_default_launch_matches_mirror():
+    lib = capi.load()
+    c = capi.LaunchConfig()
+    lib.dtr_default_launch(C.byref(c))
+    m = da.LaunchConfig().to_c(None)
+    for name, _ in capi.LaunchConfig._fields_:
+        assert getattr(c, name) == getattr(m, name), name
+    assert c.precision == capi.PRECISION_MIXED and c.schedule == capi.SCHEDULE_PERSISTENT
+    assert C.sizeof(capi.LaunchConfig) == 40
+
+
+def test_propagate_ex_rejects_bad_launch_before_touching_the_device():
+    lib = capi.load()
+    cfg = capi.PropConfig()
+    lib.dtr_default_config(C.byref(cfg))
+    tb = capi.TrackBatch(0, 0, None, None, None, None)
+    rb = capi.ResultBatch()
+    # null handles are rejected first
+    assert lib.dtr_propagate_ex(None, None, C.byref(cfg), C.byref(tb), C.byref(rb), None, None) == 1

@@ -1,0 +1,219 @@
+"""Batched kernels (dtr_propagate_ex) against the oracle on the same seeded
+inputs.
+
+FP64 precision: the same contract as the plain kernel (identical surface
+sequences, statuses and step counts; bound parameters within 1e-9, covariance
+within 1e-7 of sqrt(C_ii C_jj)).
+
+Mixed precision (FP32 stages, field, transport and material noise terms;
+FP64 track state, path, error estimate, mean energy loss and navigation):
+identical surface sequences and statuses; bound parameters within 1e-4 in
+the reference's own host/device convention (propagator_test.hpp:70,
+204-213: position differences relative to the path length), i.e.
+  |d loc| <= 1e-4 * max(path, 1 mm), |d angle| <= 1e-4 * max(|angle|, 1),
+  |d q/p| <= 1e-4 * |q/p|,
+and covariance within 1e-4 of sqrt(C_ii C_jj).  (FP32 stage rounding can
+flip an accept/reject decision of the adaptive step control, so a track may
+take one step more or less than in FP64 and land a truncation error,
+~1e-4 mm, away: loc differences are therefore not held to 1e-4 mm.)
+
+Every schedule (one work-item per track, or persistent waves over a work
+queue, with any permutation of the track order) must give the same result
+for a track.
+"""
+
+import numpy as np
+import pytest
+
+import detray_amd as da
+from tests import oracle_host
+from tests.test_gpu_parity import _cov_full, assert_parity
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+MIXED_BOUND_TOL = 1e-4
+MIXED_COV_TOL = 1e-4
+
+
+def bound_error_scale(ref):
+    """Per-element scale of the mixed-precision bound-parameter error."""
+    path = np.maximum(np.abs(ref["path_length"]), 1.0)
+    bp = np.abs(ref["bound_params"])
+    scale = np.empty_like(bp)
+    scale[0] = path
+    scale[1] = path
+    scale[2] = np.maximum(bp[2], 1.0)
+    scale[3] = np.maximum(bp[3], 1.0)
+    scale[4] = np.where(bp[4] > 0, bp[4], 1.0)
+    scale[5] = np.maximum(bp[5], 1.0)
+    return scale
+
+
+def run_launch(det, field, cfg, fp, launch, initial_cov=None, bound=None, start=None):
+    prop = da.Propagator(cfg, launch)
+    kw = {}
+    if bound is not None:
+        kw = dict(bound_params=torch.from_numpy(bound).cuda(),
+                  start_surface=torch.from_numpy(start.astype(np.int32)).cuda())
+        tracks = None
+    else:
+        tracks = torch.from_numpy(fp).cuda()
+    dev_cov = torch.from_numpy(initial_cov).cuda() if initial_cov is not None else None
+    res = prop.propagate(det, field, tracks, initial_cov=dev_cov, **kw)
+    torch.cuda.synchronize()
+    return {
+        "bound_params": res.bound_params.cpu().numpy(),
+        "bound_cov": res.bound_cov.cpu().numpy(),
+        "free_params": res.free_params.cpu().numpy(),
+        "path_length": res.path_length.cpu().numpy(),
+        "surfaces": res.surfaces.cpu().numpy().view(np.uint32),
+        "n_surfaces": res.n_surfaces.cpu().numpy().view(np.uint32),
+        "n_steps": res.n_steps.cpu().numpy().view(np.uint32),
+        "n_trials": res.n_trials.cpu().numpy().view(np.uint32),
+        "status": res.status.cpu().numpy(),
+    }
+
+
+def assert_mixed_parity(got, ref):
+    n = ref["status"].shape[0]
+    assert np.array_equal(got["status"], ref["status"])
+    assert np.array_equal(got["n_surfaces"], ref["n_surfaces"])
+    ms = got["surfaces"].shape[0]
+    for i in range(n):
+        k = min(int(ref["n_surfaces"][i]), ms)
+        assert np.array_equal(got["surfaces"][:k, i], ref["surfaces"][:k, i]), f"track {i}"
+    err = np.abs(got["bound_params"] - ref["bound_params"]) / bound_error_scale(ref)
+    assert np.max(err, initial=0.0) <= MIXED_BOUND_TOL, np.max(err, axis=1)
+    cg, cr = _cov_full(got["bound_cov"]), _cov_full(ref["bound_cov"])
+    d = np.sqrt(np.abs(np.einsum("iin->in", cr)))
+    norm = np.einsum("in,jn->ijn", d, d) + 1e-300
+    has = ref["n_surfaces"] > 0
+    rel = np.abs(cg - cr)[:, :, has] / norm[:, :, has]
+    assert rel.size == 0 or np.max(rel) <= MIXED_COV_TOL, np.max(rel)
+
+
+def check(got, ref, precision):
+    if precision == "fp64":
+        assert_parity(got, ref)
+    else:
+        assert_mixed_parity(got, ref)
+
+
+TOY_CFG = dict(search_window=(3, 3), estimate_scattering_noise=False)
+
+LAUNCHES = [
+    ("fp64", "grid", 256, 1, 64),
+    ("fp64", "persistent", 256, 1, 64),
+    ("mixed", "grid", 256, 1, 64),
+    ("mixed", "persistent", 256, 1, 64),
+    ("mixed", "persistent", 64, 2, 128),
+    ("mixed", "persistent", 128, 1, 256),
+]
+
+
+@pytest.mark.parametrize("precision,schedule,block,wps,claim", LAUNCHES)
+def test_toy_solenoid_launches(toy7, solenoid, precision, schedule, block, wps, claim):
+    cfg = da.PropagationConfig(**TOY_CFG)
+    fp = da.generate_tracks("toy", 8192, 44)
+    launch = da.LaunchConfig(precision, schedule, block, wps, claim)
+    print(f"launch {precision} {schedule} block={block} wps={wps} claim={claim}", flush=True)
+    got = run_launch(toy7, solenoid, cfg, fp, launch)
+    ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
+    check(got, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_loopers_persistent(toy7, solenoid, precision):
+    cfg = da.PropagationConfig(**TOY_CFG)
+    fp = da.generate_tracks("toy", 4096, 45, pt_min=0.1, pt_max=100.0, eta_min=-3.0, eta_max=3.0)
+    got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "persistent"))
+    ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
+    check(got, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_path_limit_and_noise_persistent(toy7, solenoid, precision):
+    # curling tracks reach the path limit; noise estimation widens the mask
+    # tolerances of the navigation (ext tolerance)
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=True)
+    fp = da.generate_tracks("toy", 512, 48, pt_min=0.02, pt_max=0.04, eta_min=-0.3, eta_max=0.3)
+    got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "persistent"))
+    ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
+    assert np.any(ref["status"] == 2)
+    check(got, ref, precision)
+
+
+def test_telescope_persistent_fp64(telescope):
+    cfg = da.PropagationConfig(step_constraint=50.0, estimate_scattering_noise=False)
+    field = da.MagneticField.constant(0.0, 1.0 * da.TESLA, 0.0)
+    fp = da.generate_tracks("beam", 4096, 42)
+    got = run_launch(telescope, field, cfg, fp, da.LaunchConfig("fp64", "persistent"))
+    ref = oracle_host.propagate(telescope, field, cfg, fp)
+    assert np.all(ref["n_surfaces"] == 10)
+    assert_parity(got, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_order_permutation_gives_identical_results(toy7, solenoid, precision):
+    cfg = da.PropagationConfig(**TOY_CFG)
+    fp = da.generate_tracks("toy", 3000, 44)
+    base = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "grid"))
+    perm = np.random.default_rng(7).permutation(fp.shape[1]).astype(np.int32)
+    order = torch.from_numpy(perm).cuda()
+    for schedule in ("grid", "persistent"):
+        got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, schedule, order=order))
+        for k in base:
+            assert np.array_equal(got[k], base[k]), (schedule, k)
+
+
+def test_bound_start_persistent(telescope):
+    cfg = da.PropagationConfig(step_constraint=50.0, estimate_scattering_noise=False)
+    field = da.MagneticField.constant(0.0, 1.0 * da.TESLA, 0.0)
+    n = 1024
+    rng = np.random.default_rng(3)
+    bound = np.zeros((6, n))
+    bound[0] = rng.uniform(-50, 50, n)
+    bound[1] = rng.uniform(-50, 50, n)
+    bound[2] = rng.uniform(-np.pi, np.pi, n)
+    bound[3] = rng.uniform(0.02, 0.1, n)
+    bound[4] = rng.choice([-1.0, 1.0], n) / rng.uniform(1.0, 10.0, n)
+    start = np.full(n, 1, dtype=np.uint32)  # second telescope plane
+    for precision in ("fp64", "mixed"):
+        got = run_launch(telescope, field, cfg, None, da.LaunchConfig(precision, "persistent"),
+                         bound=bound, start=start)
+        ref = oracle_host.propagate(telescope, field, cfg, None, bound_params=bound, start_surface=start)
+        check(got, ref, precision)
+
+
+def test_graph_capture_replays(toy7, solenoid):
+    """The persistent launch (queue reset kernel + kernel) captured in a HIP
+    graph and replayed several times, with eager work between the replays,
+    gives the eager result every time."""
+    cfg = da.PropagationConfig(**TOY_CFG)
+    fp = da.generate_tracks("toy", 4096, 44)
+    prop = da.Propagator(cfg, da.LaunchConfig("mixed", "persistent"))
+    tracks = torch.from_numpy(fp).cuda()
+    eager = prop.propagate(toy7, solenoid, tracks)
+    torch.cuda.synchronize()
+    ref = {k: getattr(eager, k).cpu().clone() for k in ("bound_params", "bound_cov", "surfaces", "status")}
+    out = prop.allocate(fp.shape[1])
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            prop.propagate(toy7, solenoid, tracks, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    for rep in range(3):
+        out.status.fill_(255)
+        out.bound_params.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k, v in ref.items():
+            assert torch.equal(getattr(out, k).cpu(), v), (rep, k)
+        # eager work between replays
+        prop.propagate(toy7, solenoid, tracks[:, :1000].contiguous())
+        torch.cuda.synchronize()
