@@ -35,6 +35,9 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     # packed FP32 instructions disabled
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # export the C ABI only
+    "-fvisibility=hidden",
+    "-fvisibility-inlines-hidden",
     "-Wall",
 ]
 

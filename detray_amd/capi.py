@@ -173,6 +173,7 @@ class ResultBatch(C.Structure):
 # dtr_precision, dtr_schedule
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
+NAVIGATION_WAVE, NAVIGATION_LANE = 0, 1
 
 
 class LaunchConfig(C.Structure):
@@ -183,6 +184,8 @@ class LaunchConfig(C.Structure):
         ("waves_per_simd", C.c_uint32),
         ("tracks_per_claim", C.c_uint32),
         ("min_idle", C.c_uint32),
+        ("navigation", C.c_uint32),
+        ("pad", C.c_uint32),
         ("order", C.c_void_p),
         ("queue", C.c_void_p),
     ]

@@ -16,13 +16,14 @@
 #include <vector>
 
 #include "detray_amd.h"
+#include "device/coop_nav.hpp"
 #include "device/engine.hpp"
 #include "device/propagator.hpp"
 
 struct dtr_detector {
     dtr::DevDetector dev;
     dtr::FastDetector fast;
-    void* blocks[11];  // the eight tables, navigation records, bin spheres, grid tolerances
+    void* blocks[12];  // the eight tables, surface records, bin records, brute-force records, grid tolerances
     uint32_t n_volumes;
     uint32_t n_surfaces;
     int device;
@@ -64,8 +65,9 @@ __global__ void __launch_bounds__(256) propagate_fp64_kernel(
     dtr::propagate_track(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
 }
 
-// Candidate cache of the block in LDS, slot-major: [8][blockDim] paths,
-// then [8][blockDim] surface indices and [8][blockDim] link|status words.
+// LDS of a block: the candidate cache, slot-major ([8][blockDim] paths, then
+// [8][blockDim] surface indices and [8][blockDim] link|status words), then
+// the exchange slots and FIFO of each wave for the cooperative navigation.
 __device__ inline dtr::CacheView lds_cache(unsigned char* smem) {
     const int nb = static_cast<int>(blockDim.x);
     double* path = reinterpret_cast<double*>(smem);
@@ -75,7 +77,32 @@ __device__ inline dtr::CacheView lds_cache(unsigned char* smem) {
     return dtr::CacheView{path + t, sf + t, ls + t, nb};
 }
 
-size_t lds_bytes(uint32_t block) { return static_cast<size_t>(block) * dtr::kCache * 16u; }
+__device__ inline dtr::CoopShared* lds_shared(unsigned char* smem) {
+    dtr::CoopShared* all =
+        reinterpret_cast<dtr::CoopShared*>(smem + static_cast<size_t>(blockDim.x) * dtr::kCache * 16u);
+    return all + threadIdx.x / 64u;
+}
+
+size_t lds_bytes(uint32_t block) {
+    return static_cast<size_t>(block) * dtr::kCache * 16u + (block / 64u) * sizeof(dtr::CoopShared);
+}
+
+// The wave interface of the cooperative navigation on CDNA (64 lanes).
+struct WaveHip {
+    static __device__ uint32_t lane() { return __lane_id(); }
+    template <class T>
+    static __device__ T shfl(T v, int src) {
+        return __shfl(v, src);
+    }
+    static __device__ uint64_t ballot(bool p) { return __ballot(p); }
+    static __device__ bool any(bool p) { return __any(p) != 0; }
+    static __device__ void sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    static __device__ uint32_t atomic_add(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }
+};
 
 // Resident waves per SIMD the batched kernels are compiled for (the register
 // budget per lane is 512 / DTR_WAVES_PER_EU).
@@ -98,65 +125,33 @@ DTR_BATCHED_KERNEL propagate_grid_kernel(
     dtr::propagate_fast<P>(det, field, cfg, in, out, track, lds_cache(smem));
 }
 
-// Persistent waves over a device work queue: every lane runs one track; a
-// lane whose track has ended takes the next queue entry, so a wave stays
-// full while its tracks have very different step counts.  A wave claims
-// `claim` entries at a time with one atomic on *queue; the kernel ends when
-// the queue is drained and every lane's track has ended.
+// One track per lane, navigation searches pooled across the wave.  Every
+// lane of the last wave takes part in the pooled searches, also the lanes
+// past the end of the batch.
 template <class P>
-DTR_BATCHED_KERNEL propagate_persistent_kernel(
+DTR_BATCHED_KERNEL propagate_coop_kernel(
+    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
+    dtr::DevResults out, const uint32_t* __restrict__ order) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool has = k < in.n;
+    const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
+    dtr::coop_run_grid<P, WaveHip>(det, field, cfg, in, out, has, track, lds_cache(smem), lds_shared(smem));
+}
+
+// Persistent waves over a device work queue with pooled navigation searches:
+// a lane whose track has ended takes the next queued track, so a wave stays
+// full while its tracks have very different step counts.  A wave claims
+// `claim` entries at a time with one atomic on *queue; it ends when the queue
+// is drained and its lanes are done.
+template <class P>
+DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
     dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
     dtr::DevResults out, const uint32_t* __restrict__ order, uint32_t* __restrict__ queue,
     uint32_t claim, uint32_t min_idle) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const dtr::CacheView c = lds_cache(smem);
-    const uint64_t n = in.n;
-    const uint32_t lane_id = __lane_id();
-    const uint64_t lt_mask = (lane_id == 0u) ? 0ull : (~0ull >> (64u - lane_id));
-    dtr::Lane<P> lane;
-    bool active = false;
-    bool drained = false;
-    uint64_t pool_next = 0, pool_end = 0;  // wave-uniform
-    for (;;) {
-        // hand queue entries to the lanes without a track, once at least
-        // min_idle lanes (or all of them) are idle, so that new tracks start
-        // their first navigation together
-        uint64_t need = __ballot(!active);
-        uint32_t track = 0xffffffffu;
-        const uint32_t idle = static_cast<uint32_t>(__popcll(need));
-        if (idle < min_idle && idle < 64u) need = 0ull;
-        while (need != 0ull && !drained) {
-            if (pool_next >= pool_end) {
-                const int leader = __ffsll(static_cast<unsigned long long>(need)) - 1;
-                uint32_t base = 0u;
-                if (static_cast<int>(lane_id) == leader) base = atomicAdd(queue, claim);
-                base = __shfl(base, leader);
-                if (base >= n) {
-                    drained = true;
-                    break;
-                }
-                pool_next = base;
-                pool_end = (base + static_cast<uint64_t>(claim) < n) ? base + claim : n;
-            }
-            const uint64_t take = pool_end - pool_next;
-            const uint64_t rank = static_cast<uint64_t>(__popcll(need & lt_mask));
-            const bool mine = !active && ((need >> lane_id) & 1ull) && rank < take;
-            if (mine) {
-                const uint64_t q = pool_next + rank;
-                track = order ? order[q] : static_cast<uint32_t>(q);
-                active = true;
-            }
-            const uint64_t cnt = static_cast<uint64_t>(__popcll(need));
-            pool_next += cnt < take ? cnt : take;
-            need = __ballot(!active);
-        }
-        if (track != 0xffffffffu) lane.start(det, field, cfg, in, c, track);
-        if (!__any(active)) break;
-        if (active && lane.advance(det, field, cfg, in, out, c)) {
-            lane.finish(out, n);
-            active = false;
-        }
-    }
+    dtr::coop_run_persistent<P, WaveHip>(det, field, cfg, in, out, order, queue, claim, min_idle,
+                                         lds_cache(smem), lds_shared(smem));
 }
 
 __global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
@@ -208,8 +203,12 @@ int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevC
     const size_t shmem = lds_bytes(block);
     if (L.schedule == DTR_SCHEDULE_GRID) {
         const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
-        hipLaunchKernelGGL(propagate_grid_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
-                           field->dev, dc, ti, r, L.order);
+        if (L.navigation == DTR_NAVIGATION_LANE)
+            hipLaunchKernelGGL(propagate_grid_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+                               field->dev, dc, ti, r, L.order);
+        else
+            hipLaunchKernelGGL(propagate_coop_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+                               field->dev, dc, ti, r, L.order);
         return to_status(hipGetLastError());
     }
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, L.queue);
@@ -222,8 +221,8 @@ int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevC
     const uint64_t w = waves < need ? waves : need;
     const uint32_t wpb = block / 64u;
     const uint32_t grid = static_cast<uint32_t>((w + wpb - 1) / wpb);
-    hipLaunchKernelGGL(propagate_persistent_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
-                       field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
+    hipLaunchKernelGGL(propagate_coop_persistent_kernel<P>, dim3(grid), dim3(block), shmem, stream,
+                       det->fast, field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
     return to_status(hipGetLastError());
 }
 
@@ -292,15 +291,20 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
         if (!rc) {
             std::vector<dtr::NavSurface> nav(d->n_surfaces);
             for (uint32_t k = 0; k < d->n_surfaces; ++k) nav[k] = dtr::make_nav_surface(*d, k);
+            // the cooperative search packs a bin index into 26 bits
+            if (d->n_bins >= (1u << 26)) rc = DTR_ERROR_UNSUPPORTED;
             for (uint32_t k = 0; k < d->n_bins && !rc; ++k)
                 if (d->grid_bins[k] != 0xffffffffu && d->grid_bins[k] >= d->n_surfaces)
                     rc = DTR_ERROR_INVALID_ARGUMENT;
-            std::vector<dtr::BinSphere> sph(d->n_bins);
+            for (uint32_t k = 0; k < d->n_bf && !rc; ++k)
+                if (d->bf_surfaces[k] >= d->n_surfaces) rc = DTR_ERROR_INVALID_ARGUMENT;
+            std::vector<dtr::NavRecord> bin_rec(d->n_bins), bf_rec(d->n_bf);
             std::vector<float> gtolk(d->n_grids);
-            if (!rc) dtr::make_bin_spheres(*d, nav.data(), sph.data(), gtolk.data());
+            if (!rc) dtr::make_nav_records(*d, nav.data(), bin_rec.data(), bf_rec.data(), gtolk.data());
             if (!rc) rc = upload(nav.data(), d->n_surfaces, &h->blocks[8]);
-            if (!rc) rc = upload(sph.data(), d->n_bins, &h->blocks[9]);
-            if (!rc) rc = upload(gtolk.data(), d->n_grids, &h->blocks[10]);
+            if (!rc) rc = upload(bin_rec.data(), d->n_bins, &h->blocks[9]);
+            if (!rc) rc = upload(bf_rec.data(), d->n_bf, &h->blocks[10]);
+            if (!rc) rc = upload(gtolk.data(), d->n_grids, &h->blocks[11]);
         }
     }
     if (!rc) {
@@ -331,8 +335,9 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     h->fast.transforms = h->dev.transforms;
     h->fast.bf = h->dev.bf;
     h->fast.bins = h->dev.bins;
-    h->fast.spheres = static_cast<const dtr::BinSphere*>(h->blocks[9]);
-    h->fast.grid_tolk = static_cast<const float*>(h->blocks[10]);
+    h->fast.bin_rec = static_cast<const dtr::NavRecord*>(h->blocks[9]);
+    h->fast.bf_rec = static_cast<const dtr::NavRecord*>(h->blocks[10]);
+    h->fast.grid_tolk = static_cast<const float*>(h->blocks[11]);
     h->n_volumes = d->n_volumes;
     h->n_surfaces = d->n_surfaces;
     *out = h;
@@ -418,6 +423,7 @@ void dtr_default_launch(dtr_launch_config* l) {
     l->waves_per_simd = 1u;
     l->tracks_per_claim = 64u;
     l->min_idle = 1u;
+    l->navigation = DTR_NAVIGATION_WAVE;
 }
 
 int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -437,6 +443,8 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule != DTR_SCHEDULE_GRID && L.schedule != DTR_SCHEDULE_PERSISTENT) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule == DTR_SCHEDULE_PERSISTENT && !L.queue) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.navigation != DTR_NAVIGATION_WAVE && L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.schedule == DTR_SCHEDULE_PERSISTENT && L.navigation != DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
     if (in->n == 0u) return DTR_SUCCESS;
     const dtr::DevConfig dc = make_dev_config(c);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,

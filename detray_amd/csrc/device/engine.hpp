@@ -48,17 +48,36 @@ struct alignas(16) NavSurface {
     // multiple of the largest mask tolerance by which a probably-inside
     // intersection can lie beyond `radius` (0 disables the prefilter)
     float tolk;
+    uint32_t index;  // surface index in the detector
 };
 static_assert(sizeof(NavSurface) == 176, "NavSurface layout");
 
-// Prefilter sphere of the surface in one grid bin (same index as the bin
-// table): centre and an enlarged circumradius in FP32, r < 0 for an empty
-// bin, r = +inf for a surface that is always evaluated.  One 16-B load per
-// bin decides whether the surface can be probably-inside.
-struct alignas(16) BinSphere {
+// FP32 prefilter of a surface, conservative: a surface it rejects cannot be
+// probably-inside.  Sphere: centre and enlarged circumradius (r < 0 for an
+// empty grid bin, r = +inf for a surface that is always evaluated).  Box:
+// the plane (normal, local axes) and the half lengths of the mask's
+// bounding rectangle (hx < 0: no box test); kx is the tolerance multiple of
+// the local-x bound (1 for rectangles, 1 + slope for trapezoids).
+struct alignas(16) NavBox {
     float c[3];
     float r;
+    float n[3];
+    float hx;
+    float ex[3];
+    float hy;
+    float ey[3];
+    float kx;
 };
+
+// A surface as stored in the candidate lists of the navigation (one per
+// grid bin, one per brute-force entry): the FP32 prefilter in the first
+// 64 B, then the whole navigation record, so that a candidate costs one
+// gather (list entry -> record) and the prefilter touches the same lines.
+struct alignas(16) NavRecord {
+    NavBox box;
+    NavSurface rec;
+};
+static_assert(sizeof(NavRecord) == 240, "NavRecord layout");
 
 struct FastDetector {
     const NavSurface* sf;
@@ -69,7 +88,8 @@ struct FastDetector {
     const dtr_transform* transforms;
     const uint32_t* bf;
     const uint32_t* bins;
-    const BinSphere* spheres;  // [n_bins]
+    const NavRecord* bin_rec;  // [n_bins] record of the surface in each grid bin
+    const NavRecord* bf_rec;   // [n_bf] record of each brute-force entry
     const float* grid_tolk;    // [n_grids] largest tolerance multiple of the grid's surfaces
 };
 
@@ -106,6 +126,7 @@ inline NavSurface make_nav_surface(const dtr_detector_desc& d, uint32_t s) {
     r.n_masks = sf.n_masks;
     r.radius = 0.0;
     r.tolk = 0.f;
+    r.index = s;
     if (sf.n_masks == 1) {
         if (sf.shape == DTR_RECTANGLE2D) {
             r.radius = sqrt(m.v[0] * m.v[0] + m.v[1] * m.v[1]);
@@ -123,28 +144,63 @@ inline NavSurface make_nav_surface(const dtr_detector_desc& d, uint32_t s) {
     return r;
 }
 
-// Host: the prefilter spheres of all grid bins and the largest tolerance
-// multiple per grid.  FP32 rounding of the centre, of the track position and
-// of the FP32 distance computation is covered by enlarging the radius by
-// 1e-5 |c|_1 here and by 1e-5 |p|_1 + 1e-3 mm at the test.
-inline void make_bin_spheres(const dtr_detector_desc& d, const NavSurface* nav, BinSphere* sph,
-                             float* grid_tolk) {
+// Host: the candidate-list records (grid bins and brute-force entries) and
+// the largest tolerance multiple per grid.  FP32 rounding of the sphere
+// centre, of the track position and of the FP32 distance computation is
+// covered by enlarging the radius by 1e-5 |c|_1 here and by 1e-5 |p|_1 +
+// 1e-3 mm at the test; the box test adds its own error margin.
+inline float round_up_float(double v) {
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) < v) f = nextafterf(f, std::numeric_limits<float>::infinity());
+    return f;
+}
+
+inline NavBox make_box(const NavSurface& r) {
+    NavBox b{};
+    b.r = std::numeric_limits<float>::infinity();
+    b.hx = -1.f;
+    if (r.tolk == 0.f) return b;
+    for (int i = 0; i < 3; ++i) b.c[i] = static_cast<float>(r.t[i]);
+    b.r = round_up_float(r.radius + 1e-5 * (fabs(r.t[0]) + fabs(r.t[1]) + fabs(r.t[2])));
+    if (r.shape == DTR_RECTANGLE2D || r.shape == DTR_TRAPEZOID2D) {
+        for (int i = 0; i < 3; ++i) {
+            b.ex[i] = static_cast<float>(r.r[3 * i + 0]);
+            b.ey[i] = static_cast<float>(r.r[3 * i + 1]);
+            b.n[i] = static_cast<float>(r.r[3 * i + 2]);
+        }
+        if (r.shape == DTR_RECTANGLE2D) {
+            b.hx = round_up_float(r.m[0]);
+            b.hy = round_up_float(r.m[1]);
+            b.kx = 1.f;
+        } else {
+            const double hx = r.m[0] > r.m[1] ? r.m[0] : r.m[1];
+            b.hx = round_up_float(hx);
+            b.hy = round_up_float(r.m[2]);
+            b.kx = round_up_float(1.0 + fabs(r.m[1] - r.m[0]) * fabs(r.m[3]));
+        }
+    }
+    return b;
+}
+
+inline void make_nav_records(const dtr_detector_desc& d, const NavSurface* nav, NavRecord* bin_rec,
+                             NavRecord* bf_rec, float* grid_tolk) {
     for (uint32_t k = 0; k < d.n_bins; ++k) {
         const uint32_t s = d.grid_bins[k];
-        BinSphere b{{0.f, 0.f, 0.f}, -1.f};
+        NavRecord e{};
+        e.box.r = -1.f;
+        e.box.hx = -1.f;
+        e.rec.index = 0xffffffffu;
         if (s != 0xffffffffu) {
-            const NavSurface& r = nav[s];
-            if (r.tolk == 0.f) {
-                b.r = std::numeric_limits<float>::infinity();
-            } else {
-                for (int i = 0; i < 3; ++i) b.c[i] = static_cast<float>(r.t[i]);
-                const double rr = r.radius + 1e-5 * (fabs(r.t[0]) + fabs(r.t[1]) + fabs(r.t[2]));
-                float rf = static_cast<float>(rr);
-                if (static_cast<double>(rf) < rr) rf = nextafterf(rf, std::numeric_limits<float>::infinity());
-                b.r = rf;
-            }
+            e.rec = nav[s];
+            e.box = make_box(nav[s]);
         }
-        sph[k] = b;
+        bin_rec[k] = e;
+    }
+    for (uint32_t k = 0; k < d.n_bf; ++k) {
+        NavRecord e{};
+        e.rec = nav[d.bf_surfaces[k]];
+        e.box = make_box(e.rec);
+        bf_rec[k] = e;
     }
     for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
         const dtr_grid& g = d.grids[gi];
@@ -227,13 +283,14 @@ DTR_HD T3<S> field_at_s(const DevField& f, V3 p) {
 struct CacheView {
     double* path;
     uint32_t* sf;
-    uint32_t* ls;  // volume link (low 16 bits) | status (bits 16-23)
+    uint32_t* ls;  // volume link (bits 0-15) | status (16-23) | surface id (24-31)
     int stride;
     DTR_HD double& p(int i) const { return path[i * stride]; }
     DTR_HD uint32_t& s(int i) const { return sf[i * stride]; }
     DTR_HD uint32_t& l(int i) const { return ls[i * stride]; }
     DTR_HD uint8_t status(int i) const { return static_cast<uint8_t>(l(i) >> 16); }
     DTR_HD uint16_t link(int i) const { return static_cast<uint16_t>(l(i) & 0xffffu); }
+    DTR_HD uint8_t sf_id(int i) const { return static_cast<uint8_t>(l(i) >> 24); }
     DTR_HD void copy(int dst, int src) const {
         p(dst) = p(src);
         s(dst) = s(src);
@@ -242,7 +299,8 @@ struct CacheView {
     DTR_HD void set(int i, const Cand& c) const {
         p(i) = c.path;
         s(i) = c.sf;
-        l(i) = static_cast<uint32_t>(c.link) | (static_cast<uint32_t>(c.status) << 16);
+        l(i) = static_cast<uint32_t>(c.link) | (static_cast<uint32_t>(c.status) << 16) |
+               (static_cast<uint32_t>(c.pad) << 24);
     }
     DTR_HD Cand get(int i) const {
         Cand c;
@@ -250,7 +308,7 @@ struct CacheView {
         c.sf = s(i);
         c.link = link(i);
         c.status = status(i);
-        c.pad = 0;
+        c.pad = sf_id(i);
         return c;
     }
 };
@@ -390,13 +448,14 @@ struct FastEngine {
         is.path = path;
         is.sf = sf_idx;
         is.link = link;
+        is.pad = sf_id;  // the surface id travels with the candidate
     }
 
     // Intersect surface s and resolve its masks until one is probably inside
     // (intersection_kernel.hpp:24-168 for init, 172-272 for update).
     // Returns true if a probably-inside result was produced in `out`.
-    DTR_AI bool evaluate(uint32_t s, float overstep, Cand& out) const {
-        const NavSurface& rec = det.sf[s];
+    DTR_AI bool evaluate(const NavSurface& rec, float overstep, Cand& out) const {
+        const uint32_t s = rec.index;
         double path, z = 0.0;
         V3 glob{0.0, 0.0, 0.0};
         bool valid;
@@ -427,16 +486,13 @@ struct FastEngine {
             return;
         }
         int idx = pos;
-        const uint64_t nbar = det.sf[nc.sf].barcode;
+        // barcodes are unique per surface (the surface index is part of
+        // them), so equal barcodes are equal surface indices
         auto overlap = [&](int i) { return fabs(c.p(i) - nc.path) <= 1.0 * 1e-3; };
-        auto clash = [&](int i) {
-            const uint32_t si = c.s(i);
-            const uint64_t a = si == kInvalid ? ~0ull : det.sf[si].barcode;
-            return a == nbar && overlap(i);
-        };
+        auto clash = [&](int i) { return c.s(i) == nc.sf && overlap(i); };
         if (clash(idx) || (idx > 0 && clash(idx - 1))) return;
         const int shift_begin = tk.last < kCache - 2 ? tk.last : kCache - 2;
-        if (overlap(idx) && (nc.status == kEdge || is_portal(nc.sf))) {
+        if (overlap(idx) && (nc.status == kEdge || nc.pad == DTR_PORTAL)) {
             idx++;
             idx = idx > shift_begin ? shift_begin : idx;
         }
@@ -452,14 +508,14 @@ struct FastEngine {
         insert(p, is);
     }
 
-    DTR_AI void candidate_search(uint32_t s, float overstep) {
+    DTR_AI void candidate_search(const NavSurface& rec, float overstep) {
         Cand is;
         is.path = kMax;
         is.sf = kInvalid;
         is.link = 0;
         is.status = kOutside;
         is.pad = 0;
-        if (evaluate(s, overstep, is)) insert_sorted(is);
+        if (evaluate(rec, overstep, is)) insert_sorted(is);
     }
 
     DTR_AI bool update_candidate(int slot) {
@@ -499,8 +555,12 @@ struct FastEngine {
     // conservative prefilter: a surface whose centre is farther from the
     // tangent line than its mask circumradius plus twice the largest
     // tolerance cannot be probably-inside, so it is not evaluated.
-    DTR_AI void grid_search(const dtr_volume& vol, float overstep) {
-        const dtr_grid& g = det.grids[vol.grid];
+    // The bin window of the grid lookup (spatial_grid.hpp:100-202,
+    // axis_binning.hpp:80-115, axis_bounds.hpp): bins [a0, e0) x [a1, e1)
+    // before wrapping of the circular axis.  False for a non-finite lookup
+    // point (no grid candidates).
+    DTR_AI bool grid_window(const dtr_volume& vol, const dtr_grid& g, int& a0, int& e0, int& a1,
+                            int& e1) const {
         double l0, l1;
         if (g.type == DTR_GRID_CYLINDER) {
             double path, z;
@@ -519,15 +579,13 @@ struct FastEngine {
         const double w0 = (g.max0 - g.min0) / static_cast<double>(g.n0);
         const double w1 = (g.max1 - g.min1) / static_cast<double>(g.n1);
         const double f0 = (l0 - g.min0) / w0 + 1.0, f1 = (l1 - g.min1) / w1 + 1.0;
-        if (!(fabs(f0) < 1e9) || !(fabs(f1) < 1e9)) return;
+        if (!(fabs(f0) < 1e9) || !(fabs(f1) < 1e9)) return false;
         const int ib0 = static_cast<int>((l0 - g.min0) / w0 + 1.0) - 1;
         const int ib1 = static_cast<int>((l1 - g.min1) / w1 + 1.0) - 1;
         const int lb0 = ib0 - static_cast<int>(cfg.win0), ub0 = ib0 + static_cast<int>(cfg.win1);
         const int lb1 = ib1 - static_cast<int>(cfg.win0), ub1 = ib1 + static_cast<int>(cfg.win1);
-        const bool cyl = (g.type == DTR_GRID_CYLINDER);
         const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
-        int a0, e0, a1, e1;
-        if (cyl) {
+        if (g.type == DTR_GRID_CYLINDER) {
             a0 = lb0;
             e0 = ub0 + 1;
             int mn = lb1 > 0 ? lb1 : 0;
@@ -546,14 +604,63 @@ struct FastEngine {
             a1 = lb1;
             e1 = ub1 + 1;
         }
-        // largest tolerance a module mask check can use
-        const double tol_max = dmax(static_cast<double>(cfg.min_mask_tol), static_cast<double>(cfg.max_mask_tol)) +
-                               tk.ext_tol;
-        const V3 ro = tk.pos, rd = tk.dir;
-        const float ox = static_cast<float>(ro.x), oy = static_cast<float>(ro.y), oz = static_cast<float>(ro.z);
-        const float dx = static_cast<float>(rd.x), dy = static_cast<float>(rd.y), dz = static_cast<float>(rd.z);
-        const float margin = static_cast<float>(static_cast<double>(det.grid_tolk[vol.grid]) * tol_max +
-                                                1e-5 * (fabs(ro.x) + fabs(ro.y) + fabs(ro.z)) + 1e-3);
+        return true;
+    }
+
+    // Prefilter margin of a grid's surfaces for this track: the largest
+    // mask tolerance times the grid's tolerance multiple plus the FP32
+    // rounding allowance of the track position.
+    DTR_AI float sphere_margin(int grid) const {
+        return static_cast<float>(static_cast<double>(det.grid_tolk[grid]) * tol_max() +
+                                  1e-5 * (fabs(tk.pos.x) + fabs(tk.pos.y) + fabs(tk.pos.z)) + 1e-3);
+    }
+
+    // largest tolerance a mask check can use
+    DTR_AI double tol_max() const {
+        return dmax(static_cast<double>(cfg.min_mask_tol), static_cast<double>(cfg.max_mask_tol)) + tk.ext_tol;
+    }
+
+    // FP32 prefilter of one record for the ray (o, d): false only if the
+    // surface cannot be probably-inside.  The sphere test bounds the distance
+    // of the centre from the line; the box test intersects the plane (when
+    // the incidence is not grazing, |n.d| >= 0.05) and bounds the local
+    // coordinates by the mask's rectangle plus tolerance and an FP32 error
+    // margin of 2e-4 |c - o|_1 + 1e-5 |c|_1: with unit roundoff u = 6e-8 and
+    // |n.d| >= 0.05 the local coordinates are off by at most about
+    // 1.4e3 u |c - o| + 25 u (|c| + |o|) (s = n.(c-o)/n.d, |s| <= 20 |c-o|).
+    static DTR_AI bool prefilter(const NavBox& b, float ox, float oy, float oz, float dx, float dy, float dz,
+                                 float margin, float ft) {
+        const float cx = b.c[0] - ox, cy = b.c[1] - oy, cz = b.c[2] - oz;
+        const float xx = cy * dz - dy * cz, xy = cz * dx - dz * cx, xz = cx * dy - dx * cy;
+        const float dist2 = xx * xx + xy * xy + xz * xz;
+        const float rb = b.r + margin;
+        if (!(b.r >= 0.f) || dist2 > rb * rb) return false;
+        if (b.hx < 0.f) return true;
+        const float denom = b.n[0] * dx + b.n[1] * dy + b.n[2] * dz;
+        if (!(fabsf(denom) >= 0.05f)) return true;
+        const float s = (b.n[0] * cx + b.n[1] * cy + b.n[2] * cz) / denom;
+        const float px = s * dx - cx, py = s * dy - cy, pz = s * dz - cz;
+        const float lx = px * b.ex[0] + py * b.ex[1] + pz * b.ex[2];
+        const float ly = px * b.ey[0] + py * b.ey[1] + pz * b.ey[2];
+        const float m = 2e-4f * (fabsf(cx) + fabsf(cy) + fabsf(cz)) +
+                        1e-5f * (fabsf(b.c[0]) + fabsf(b.c[1]) + fabsf(b.c[2])) + margin;
+        return !(fabsf(lx) > b.hx + b.kx * ft + m) && !(fabsf(ly) > b.hy + ft + m);
+    }
+
+    DTR_AI bool sphere_pass(uint32_t bin, float margin) const {
+        const float ft = static_cast<float>(tol_max());
+        return prefilter(det.bin_rec[bin].box, static_cast<float>(tk.pos.x), static_cast<float>(tk.pos.y),
+                         static_cast<float>(tk.pos.z), static_cast<float>(tk.dir.x),
+                         static_cast<float>(tk.dir.y), static_cast<float>(tk.dir.z), margin, ft);
+    }
+
+    DTR_AI void grid_search(const dtr_volume& vol, float overstep) {
+        const dtr_grid& g = det.grids[vol.grid];
+        int a0, e0, a1, e1;
+        if (!grid_window(vol, g, a0, e0, a1, e1)) return;
+        const bool cyl = (g.type == DTR_GRID_CYLINDER);
+        const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
+        const float margin = sphere_margin(vol.grid);
         constexpr int kRow = 8;
         for (int b0 = a0; b0 < e0; ++b0) {
             const int i0 = cyl ? wrap_bin(b0, n0) : b0;
@@ -566,21 +673,17 @@ struct FastEngine {
                     if (k < cnt) {
                         const int b1 = b1s + k;
                         const int i1 = cyl ? b1 : wrap_bin(b1, n1);
-                        const BinSphere sp = det.spheres[row + i1];
-                        // distance of the centre from the tangent line
-                        const float cx = sp.c[0] - ox, cy = sp.c[1] - oy, cz = sp.c[2] - oz;
-                        const float xx = cy * dz - dy * cz, xy = cz * dx - dz * cx, xz = cx * dy - dx * cy;
-                        const float dist2 = xx * xx + xy * xy + xz * xz;
-                        const float rb = sp.r + margin;
-                        if (sp.r >= 0.f && !(dist2 > rb * rb)) pass |= (1u << k);
+                        if (sphere_pass(row + i1, margin)) pass |= (1u << k);
                     }
                 }
                 while (pass) {
+                    // re-read the (cached) bin entry rather than index a
+                    // register array dynamically, which would go to scratch
                     const int k = __builtin_ctz(pass);
                     pass &= pass - 1u;
                     const int b1 = b1s + k;
                     const int i1 = cyl ? b1 : wrap_bin(b1, n1);
-                    candidate_search(det.bins[row + i1], overstep);
+                    candidate_search(det.bin_rec[row + i1].rec, overstep);
                 }
             }
         }
@@ -597,7 +700,7 @@ struct FastEngine {
     // navigation_functions.hpp:147-177
     DTR_AI void update_status() {
         if (!exhausted() && fabs(c.p(tk.next)) < static_cast<double>(cfg.path_tol)) {
-            tk.status = is_portal(c.s(tk.next)) ? kOnPortal : kOnObject;
+            tk.status = (c.sf_id(tk.next) == DTR_PORTAL) ? kOnPortal : kOnObject;
             tk.next = static_cast<int8_t>(tk.next + 1);
         } else {
             tk.status = kTowards;
@@ -612,42 +715,23 @@ struct FastEngine {
         for (int i = 0; i < kCache; ++i) c.p(i) = kMax;
         tk.next = 0;
         tk.last = -1;
-        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf[k], os);
+        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf_rec[k].rec, os);
         if (vol.grid >= 0) grid_search(vol, os);
         update_status();
         if (tk.trust != kFull && tk.status == kOnPortal)
             tk.trust = (c.link(tk.next - 1) == kEndOfWorld) ? kFull : kNoTrust;
     }
 
-    DTR_AI void local_navigation(float overstep_tol, bool resolve_overstepping) {
-        const float os = resolve_overstepping ? overstep_tol : -cfg.path_tol;
-        const uint32_t r = local_navigation_ool<P>(det, cfg, tk.pos, tk.dir, tk.ext_tol, tk.volume, os, c);
+    DTR_AI void apply_local_navigation(uint32_t r) {
         tk.next = static_cast<int8_t>(r & 0xffu);
         tk.last = static_cast<int8_t>((r >> 8) & 0xffu);
         tk.status = static_cast<int8_t>((r >> 16) & 0xffu);
         tk.trust = static_cast<uint8_t>(r >> 24);
     }
 
-    DTR_AI void volume_switch() {
-        const uint16_t link = c.link(tk.next - 1);
-        if (link == kEndOfWorld) {
-            tk.status = kExit;
-            return;
-        }
-        if (link != tk.volume) tk.trust = kNoTrust;
-        tk.volume = link;
-        local_navigation(cfg.overstep_tol, true);
-        tk.trust = kFull;
-    }
-
-    DTR_AI void init_loose_cfg() {
-        const float a = 100.f * cfg.overstep_tol, b = -10.f * cfg.max_mask_tol;
-        local_navigation(a < b ? a : b, true);
-        if (tk.trust != kFull || exhausted()) tk.status = kAbort;
-    }
-
-    // caching_navigator.hpp:250-388
-    DTR_AI bool update_impl() {
+    // caching_navigator.hpp:250-388 up to its local navigation: returns true
+    // when the cache is not trusted at all and the volume must be searched.
+    DTR_AI bool update_impl_head() {
         if (tk.trust == kHigh) {
             if (!update_candidate(tk.next)) {
                 tk.status = kUnknown;
@@ -682,27 +766,76 @@ struct FastEngine {
             update_status();
             if (exhausted()) tk.trust = kNoTrust;
         }
-        if (tk.trust == kNoTrust) {
-            local_navigation(cfg.overstep_tol, true);
+        return tk.trust == kNoTrust;
+    }
+
+    // navigator_base.hpp:97-154 (update) as a resumable cascade.  Each call
+    // returns true when the lane needs a local navigation of tk.volume with
+    // overstep tolerance `os`; once it has been applied
+    // (apply_local_navigation) the cascade continues with nav_resume().
+    // `ret` is the value update() returns (whether the navigation was
+    // re-initialised).  Stages: 1 after update_impl's search, 2 after the
+    // volume switch's search, 3 after init_loose_cfg's search.
+    DTR_AI bool nav_begin(uint8_t& stage, bool& ret, float& os) {
+        ret = false;
+        stage = 0;
+        if (tk.trust == kFull) return false;
+        if (update_impl_head()) {
+            ret = true;
+            stage = 1;
+            os = cfg.overstep_tol;
             return true;
         }
+        return nav_portal(stage, ret, os);
+    }
+
+    // volume_switch (navigation_functions.hpp:265-290)
+    DTR_AI bool nav_portal(uint8_t& stage, bool& ret, float& os) {
+        if (tk.status == kOnPortal) {
+            const uint16_t link = c.link(tk.next - 1);
+            if (link == kEndOfWorld) {
+                tk.status = kExit;
+                ret = false;
+                stage = 0;
+                return false;
+            }
+            if (link != tk.volume) tk.trust = kNoTrust;
+            tk.volume = link;
+            stage = 2;
+            os = cfg.overstep_tol;
+            return true;
+        }
+        return nav_loose(stage, ret, os);
+    }
+
+    // init_loose_cfg (navigation_functions.hpp:305-330)
+    DTR_AI bool nav_loose(uint8_t& stage, bool& ret, float& os) {
+        if (tk.trust != kFull || exhausted()) {
+            ret = true;
+            stage = 3;
+            const float a = 100.f * cfg.overstep_tol, b = -10.f * cfg.max_mask_tol;
+            os = a < b ? a : b;
+            return true;
+        }
+        stage = 0;
         return false;
     }
 
-    // navigator_base.hpp:97-154
-    DTR_AI bool nav_update() {
-        if (tk.trust == kFull) return false;
-        bool is_init = update_impl();
-        if (tk.status == kOnPortal) {
-            volume_switch();
-            is_init = true;
-            if (!(tk.status > kUnknown)) return false;
+    DTR_AI bool nav_resume(uint8_t& stage, bool& ret, float& os) {
+        if (stage == 1) return nav_portal(stage, ret, os);
+        if (stage == 2) {
+            tk.trust = kFull;
+            ret = true;
+            if (!(tk.status > kUnknown)) {
+                ret = false;
+                stage = 0;
+                return false;
+            }
+            return nav_loose(stage, ret, os);
         }
-        if (tk.trust != kFull || exhausted()) {
-            is_init = true;
-            init_loose_cfg();
-        }
-        return is_init;
+        if (tk.trust != kFull || exhausted()) tk.status = kAbort;
+        stage = 0;
+        return false;
     }
 
     // ----------------------------------------------------------- stepping --
@@ -1189,27 +1322,35 @@ struct TrackInputs {
 };
 
 // propagator::propagate (propagator.hpp:262-400) for one track, split into
-// start / advance / finish so that a persistent wave can swap a finished
-// lane's track for a new one.  `c` is the lane's view of the LDS candidate
-// cache.
+// phases so that the lanes of a wave can meet for the local navigations
+// (each lane's navigation cascade stops where it needs one) and a persistent
+// wave can swap a finished lane's track for a new one.  `c` is the lane's
+// view of the LDS candidate cache.
 //
 // The loop of propagate() alternates an actor half (even i) and a step half
 // (odd i), with `i % 2 == 0 || heartbeat` as its condition, so an actor half
-// always follows a step half.  advance() runs one actor half and, if the
-// heartbeat survives it, one step half; it returns true once the loop of
-// propagate() would have ended.
+// always follows a step half:
+//   actor_half()          -> ends the loop if the heartbeat is gone
+//   navigation update     -> ends the loop if the heartbeat is gone
+//   step_half()
+//   navigation update
 template <class P>
 struct Lane {
     using S = typename P::S;
     FastTrack<P> tk;
     bool is_init;
     bool use_init_cov;
+    bool nav_ret;
+    uint8_t nav_stage;
     double last_path;
     uint32_t stall;
     uint32_t gid;
 
-    DTR_AI void start(const FastDetector& det, const DevField& field, const DevConfig& cfg,
-                      const TrackInputs& in, CacheView c, uint32_t track) {
+    // Track set-up (propagator.hpp:79-162); the navigator's initialisation
+    // (navigator_base.hpp:57-66, a local navigation without overstepping
+    // resolution, overstep tolerance `os`) is then requested from the caller.
+    DTR_AI void start_begin(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                            const TrackInputs& in, CacheView c, uint32_t track, float& os) {
         gid = track;
         for (int i = 0; i < kCache; ++i) {
             c.p(i) = kMax;
@@ -1268,18 +1409,23 @@ struct Lane {
         }
         tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
         e.reset_J();
-        // navigator_base::init -> local_navigation without overstepping resolution
-        e.local_navigation(cfg.overstep_tol, false);
+        os = -cfg.path_tol;
+    }
+
+    // after the initial local navigation has been applied
+    DTR_AI void start_end() {
         tk.heartbeat = tk.status > kUnknown;
         is_init = true;
         last_path = 0.0;
         stall = 0;
     }
 
-    DTR_AI bool advance(const FastDetector& det, const DevField& field, const DevConfig& cfg,
-                        const TrackInputs& in, const DevResults& out, CacheView c) {
+    // pathlimit_aborter (aborters.hpp:44-75), parameter_updater<pointwise_
+    // material_interactor>, surface_sequencer (surface_sequencer.hpp:78-106).
+    // Returns true when propagate()'s loop ends here.
+    DTR_AI bool actor_half(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                           const TrackInputs& in, const DevResults& out, CacheView c) {
         FastEngine<P> e{det, field, cfg, tk, c};
-        // actor half: pathlimit_aborter (aborters.hpp:44-75)
         if (tk.status != kExit) {
             const double step_limit = cfg.path_limit - fabs(tk.abs_path_length);
             if (step_limit <= 0.0) {
@@ -1289,13 +1435,11 @@ struct Lane {
             }
             tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
         }
-        // parameter_updater<pointwise_material_interactor>
         if (cfg.use_material) e.parameter_updater(use_init_cov ? in.init_cov : nullptr, in.n, gid);
         if (tk.stopped && tk.status > kUnknown) {
             tk.status = kAbort;
             tk.heartbeat = false;
         }
-        // surface_sequencer (surface_sequencer.hpp:78-106)
         if (cfg.max_surfaces != 0u && (e.on_sensitive() || e.on_material())) {
             if (tk.n_seq == cfg.max_surfaces) {
                 tk.overflow = true;
@@ -1308,10 +1452,12 @@ struct Lane {
         }
         if (!tk.heartbeat) return true;
         last_path = tk.path_length;
-        is_init |= e.nav_update();
-        tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
-        if (!tk.heartbeat) return true;
-        // step half
+        return false;
+    }
+
+    // rk_stepper::step + the step-size policy + the stall guard
+    DTR_AI void step_half(const FastDetector& det, const DevField& field, const DevConfig& cfg, CacheView c) {
+        FastEngine<P> e{det, field, cfg, tk, c};
         const bool reset = e.on_surface() || is_init;
         e.step(e.nav_dist(), reset);
         e.rk_policy();
@@ -1325,9 +1471,31 @@ struct Lane {
         } else {
             stall = 0;
         }
-        is_init |= e.nav_update();
+    }
+
+    // navigation update: nav_begin, then (apply_local_navigation, nav_resume)
+    // while a local navigation is requested, then nav_end.
+    DTR_AI bool nav_begin(const FastDetector& det, const DevConfig& cfg, CacheView c, float& os) {
+        const DevField no_field{};
+        FastEngine<P> e{det, no_field, cfg, tk, c};
+        return e.nav_begin(nav_stage, nav_ret, os);
+    }
+    DTR_AI bool nav_resume(const FastDetector& det, const DevConfig& cfg, CacheView c, float& os) {
+        const DevField no_field{};
+        FastEngine<P> e{det, no_field, cfg, tk, c};
+        return e.nav_resume(nav_stage, nav_ret, os);
+    }
+    DTR_AI void apply_local_navigation(uint32_t r) {
+        tk.next = static_cast<int8_t>(r & 0xffu);
+        tk.last = static_cast<int8_t>((r >> 8) & 0xffu);
+        tk.status = static_cast<int8_t>((r >> 16) & 0xffu);
+        tk.trust = static_cast<uint8_t>(r >> 24);
+    }
+    // returns true when propagate()'s loop ends here (heartbeat gone)
+    DTR_AI bool nav_end() {
+        is_init |= nav_ret;
         tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
-        return false;
+        return !tk.heartbeat;
     }
 
     DTR_AI void finish(const DevResults& out, uint64_t N) const {
@@ -1354,9 +1522,39 @@ struct Lane {
         if (out.n_steps) out.n_steps[gid] = tk.n_steps;
         if (out.n_trials) out.n_trials[gid] = tk.n_trials;
     }
+
+    // one lane on its own, with out-of-line local navigations
+    DTR_AI void navigate_alone(const FastDetector& det, const DevConfig& cfg, CacheView c) {
+        float os;
+        bool req = nav_begin(det, cfg, c, os);
+        while (req) {
+            apply_local_navigation(local_navigation_ool<P>(det, cfg, tk.pos, tk.dir, tk.ext_tol, tk.volume, os, c));
+            req = nav_resume(det, cfg, c, os);
+        }
+    }
+
+    DTR_AI void start(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                      const TrackInputs& in, CacheView c, uint32_t track) {
+        float os;
+        start_begin(det, field, cfg, in, c, track, os);
+        apply_local_navigation(local_navigation_ool<P>(det, cfg, tk.pos, tk.dir, tk.ext_tol, tk.volume, os, c));
+        start_end();
+    }
+
+    // one actor half and one step half; true once propagate()'s loop ended
+    DTR_AI bool advance(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                        const TrackInputs& in, const DevResults& out, CacheView c) {
+        if (actor_half(det, field, cfg, in, out, c)) return true;
+        navigate_alone(det, cfg, c);
+        if (nav_end()) return true;
+        step_half(det, field, cfg, c);
+        navigate_alone(det, cfg, c);
+        nav_end();
+        return false;
+    }
 };
 
-// One track from start to finish.
+// One track from start to finish, lane on its own.
 template <class P>
 DTR_HD void propagate_fast(const FastDetector& det, const DevField& field, const DevConfig& cfg,
                            const TrackInputs& in, const DevResults& out, uint32_t track, CacheView c) {

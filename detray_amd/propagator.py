@@ -277,7 +277,9 @@ class LaunchConfig:
     "persistent" (resident waves over a device work queue; a wave takes new
     tracks once ``min_idle`` of its lanes are idle).  ``order`` is an optional
     int32 device tensor [n], a permutation giving the order in which tracks
-    are taken (results stay in track order)."""
+    are taken (results stay in track order).  navigation: "wave" (the lanes
+    that need a volume search pool their candidates and the wave evaluates
+    them together) or "lane" (each lane on its own, grid schedule only)."""
 
     precision: str = "mixed"
     schedule: str = "persistent"
@@ -286,6 +288,7 @@ class LaunchConfig:
     tracks_per_claim: int = 64
     order: "object" = None
     min_idle: int = 1
+    navigation: str = "wave"
 
     def to_c(self, queue_ptr: Optional[int]) -> capi.LaunchConfig:
         c = capi.LaunchConfig()
@@ -295,6 +298,7 @@ class LaunchConfig:
         c.waves_per_simd = int(self.waves_per_simd)
         c.tracks_per_claim = int(self.tracks_per_claim)
         c.min_idle = int(self.min_idle)
+        c.navigation = {"wave": capi.NAVIGATION_WAVE, "lane": capi.NAVIGATION_LANE}[self.navigation]
         c.order = self.order.data_ptr() if self.order is not None else None
         c.queue = queue_ptr
         return c

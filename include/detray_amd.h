@@ -23,6 +23,15 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* Only the C ABI is exported from the shared library (it is built with
+ * hidden visibility), so that its internal C++ symbols never interpose
+ * with those of other libraries in the process. */
+#if defined(__GNUC__) || defined(__clang__)
+#define DTR_API __attribute__((visibility("default")))
+#else
+#define DTR_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -231,18 +240,18 @@ typedef struct dtr_detector dtr_detector; /* device-resident, immutable */
 typedef struct dtr_field dtr_field;
 
 /* Defaults of propagation::config (and the interactor state). */
-void dtr_default_config(dtr_prop_config* cfg);
+DTR_API void dtr_default_config(dtr_prop_config* cfg);
 
 /* Copy the detector tables to the current HIP device.
  * Replaces detray::get_buffer(detector) + detector(view)
  * (core/include/detray/core/detector.hpp:136-202). */
-int dtr_detector_create(const dtr_detector_desc* desc, dtr_detector** out);
-int dtr_detector_destroy(dtr_detector* det);
+DTR_API int dtr_detector_create(const dtr_detector_desc* desc, dtr_detector** out);
+DTR_API int dtr_detector_destroy(dtr_detector* det);
 
 /* Field view on the device: constant or sampled grid.
  * Replaces covfie::field<...>::view_t (tests/include/detray/test/common/bfield.hpp:35-88). */
-int dtr_field_create(const dtr_field_desc* desc, dtr_field** out);
-int dtr_field_destroy(dtr_field* field);
+DTR_API int dtr_field_create(const dtr_field_desc* desc, dtr_field** out);
+DTR_API int dtr_field_destroy(dtr_field* field);
 
 /* Propagate a batch of tracks on the device.
  * in/out point to DEVICE memory (SoA, see above).  The actor chain is
@@ -253,7 +262,7 @@ int dtr_field_destroy(dtr_field* field);
  * The stepper is rk_stepper<field, algebra, constrained_step> with the
  * caching_navigator (cache size 8).  Work is enqueued on `stream`
  * (a hipStream_t, NULL = default stream); the call does not synchronise. */
-int dtr_propagate(const dtr_detector* det, const dtr_field* field,
+DTR_API int dtr_propagate(const dtr_detector* det, const dtr_field* field,
                   const dtr_prop_config* cfg, const dtr_track_batch* in,
                   dtr_result_batch* out, void* stream);
 
@@ -276,6 +285,14 @@ enum dtr_schedule {
     DTR_SCHEDULE_PERSISTENT = 1
 };
 
+enum dtr_navigation_mode {
+    /* lanes that need a local navigation (volume search) pool their
+     * candidate lists and the whole wave evaluates them */
+    DTR_NAVIGATION_WAVE = 0,
+    /* every lane searches on its own (grid schedule only) */
+    DTR_NAVIGATION_LANE = 1
+};
+
 typedef struct dtr_launch_config {
     uint32_t precision;        /* dtr_precision (default MIXED) */
     uint32_t schedule;         /* dtr_schedule (default PERSISTENT) */
@@ -285,6 +302,8 @@ typedef struct dtr_launch_config {
                                   multiple of 64 (default 64) */
     uint32_t min_idle;         /* persistent: idle lanes a wave waits for before it
                                   takes new tracks, 1..64 (default 1) */
+    uint32_t navigation;       /* dtr_navigation_mode (default WAVE) */
+    uint32_t pad;
     /* optional DEVICE permutation [n]: the k-th queue entry / work-item
      * propagates track order[k]; results stay in track order */
     const uint32_t* order;
@@ -294,7 +313,7 @@ typedef struct dtr_launch_config {
 } dtr_launch_config;
 
 /* Defaults of dtr_launch_config. */
-void dtr_default_launch(dtr_launch_config* launch);
+DTR_API void dtr_default_launch(dtr_launch_config* launch);
 
 /* dtr_propagate with a chosen precision and schedule.  Same actor chain,
  * navigation and results as dtr_propagate (the same
@@ -302,7 +321,7 @@ void dtr_default_launch(dtr_launch_config* launch);
  * sequence of a track does not depend on the launch configuration.  Does
  * not allocate and does not synchronise; with the persistent schedule,
  * calls that share one `queue` must not overlap. */
-int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
+DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, const dtr_launch_config* launch,
                      void* stream);
@@ -325,14 +344,14 @@ typedef struct dtr_telescope_config {
     double envelope;         /* cuboid portal envelope (tel_det_config::envelope) */
 } dtr_telescope_config;
 
-int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out);
-int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
+DTR_API int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out);
+DTR_API int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
                                  dtr_detector_desc* out);
-void dtr_host_detector_free(dtr_detector_desc* desc);
+DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);
 
 /* Sample the analytic finite solenoid (on-axis 2 T, radius 1.2 m, half
  * length 3 m) on a regular grid.  `values` must hold nx*ny*nz*3 floats. */
-int dtr_sample_solenoid(double b_center, double radius, double half_length,
+DTR_API int dtr_sample_solenoid(double b_center, double radius, double half_length,
                         const uint32_t n[3], const double min[3],
                         const double pitch[3], float* values);
 
@@ -349,10 +368,10 @@ typedef struct dtr_track_gen_config {
     double sigma_xy, sigma_z;
 } dtr_track_gen_config;
 
-int dtr_generate_tracks(const dtr_track_gen_config* cfg, double* free_params);
+DTR_API int dtr_generate_tracks(const dtr_track_gen_config* cfg, double* free_params);
 
 /* Library version string. */
-const char* dtr_version(void);
+DTR_API const char* dtr_version(void);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "detray_amd.h"
 
 def header_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(dtr_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:DTR_API\s+)?(?:int|void|const char\*)\s+(dtr_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_all_header_symbols():
@@ -71,7 +71,7 @@ def test_default_launch_matches_mirror():
     for name, _ in capi.LaunchConfig._fields_:
         assert getattr(c, name) == getattr(m, name), name
     assert c.precision == capi.PRECISION_MIXED and c.schedule == capi.SCHEDULE_PERSISTENT
-    assert C.sizeof(capi.LaunchConfig) == 40
+    assert C.sizeof(capi.LaunchConfig) == 48
 
 
 def test_propagate_ex_rejects_bad_launch_before_touching_the_device():

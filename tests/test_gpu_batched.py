@@ -104,21 +104,24 @@ def check(got, ref, precision):
 TOY_CFG = dict(search_window=(3, 3), estimate_scattering_noise=False)
 
 LAUNCHES = [
-    ("fp64", "grid", 256, 1, 64),
-    ("fp64", "persistent", 256, 1, 64),
-    ("mixed", "grid", 256, 1, 64),
-    ("mixed", "persistent", 256, 1, 64),
-    ("mixed", "persistent", 64, 2, 128),
-    ("mixed", "persistent", 128, 1, 256),
+    ("fp64", "grid", 256, 1, 64, 1, "lane"),
+    ("fp64", "grid", 256, 1, 64, 1, "wave"),
+    ("fp64", "persistent", 256, 1, 64, 1, "wave"),
+    ("mixed", "grid", 256, 1, 64, 1, "lane"),
+    ("mixed", "grid", 64, 1, 64, 1, "wave"),
+    ("mixed", "persistent", 256, 1, 64, 1, "wave"),
+    ("mixed", "persistent", 64, 2, 128, 16, "wave"),
+    ("mixed", "persistent", 128, 1, 256, 64, "wave"),
 ]
 
 
-@pytest.mark.parametrize("precision,schedule,block,wps,claim", LAUNCHES)
-def test_toy_solenoid_launches(toy7, solenoid, precision, schedule, block, wps, claim):
+@pytest.mark.parametrize("precision,schedule,block,wps,claim,idle,nav", LAUNCHES)
+def test_toy_solenoid_launches(toy7, solenoid, precision, schedule, block, wps, claim, idle, nav):
     cfg = da.PropagationConfig(**TOY_CFG)
     fp = da.generate_tracks("toy", 8192, 44)
-    launch = da.LaunchConfig(precision, schedule, block, wps, claim)
-    print(f"launch {precision} {schedule} block={block} wps={wps} claim={claim}", flush=True)
+    launch = da.LaunchConfig(precision, schedule, block, wps, claim, min_idle=idle, navigation=nav)
+    print(f"launch {precision} {schedule} block={block} wps={wps} claim={claim} idle={idle} nav={nav}",
+          flush=True)
     got = run_launch(toy7, solenoid, cfg, fp, launch)
     ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
     check(got, ref, precision)
@@ -159,13 +162,14 @@ def test_telescope_persistent_fp64(telescope):
 def test_order_permutation_gives_identical_results(toy7, solenoid, precision):
     cfg = da.PropagationConfig(**TOY_CFG)
     fp = da.generate_tracks("toy", 3000, 44)
-    base = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "grid"))
+    base = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "grid", navigation="lane"))
     perm = np.random.default_rng(7).permutation(fp.shape[1]).astype(np.int32)
     order = torch.from_numpy(perm).cuda()
-    for schedule in ("grid", "persistent"):
-        got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, schedule, order=order))
+    for schedule, nav in (("grid", "lane"), ("grid", "wave"), ("persistent", "wave")):
+        got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, schedule, order=order,
+                                                                   navigation=nav))
         for k in base:
-            assert np.array_equal(got[k], base[k]), (schedule, k)
+            assert np.array_equal(got[k], base[k]), (schedule, nav, k)
 
 
 def test_bound_start_persistent(telescope):
