@@ -191,6 +191,15 @@ class LaunchConfig(C.Structure):
     ]
 
 
+class SortConfig(C.Structure):
+    _fields_ = [
+        ("eta_bins_per_unit", C.c_uint32),
+        ("phi_bins", C.c_uint32),
+        ("eta_max", C.c_float),
+        ("pad", C.c_uint32),
+    ]
+
+
 class ToyConfig(C.Structure):
     _fields_ = [
         ("n_brl_layers", C.c_uint32),
@@ -235,6 +244,10 @@ _SIGNATURES = {
     "dtr_propagate_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig),
                                    C.POINTER(TrackBatch), C.POINTER(ResultBatch),
                                    C.POINTER(LaunchConfig), C.c_void_p]),
+    "dtr_default_sort": (None, [C.POINTER(SortConfig)]),
+    "dtr_sort_workspace_size": (C.c_size_t, [C.POINTER(SortConfig)]),
+    "dtr_sort_tracks": (C.c_int, [C.POINTER(SortConfig), C.c_void_p, C.c_uint32, C.c_void_p,
+                                  C.c_void_p, C.c_size_t, C.c_void_p]),
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),

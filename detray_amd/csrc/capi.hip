@@ -104,17 +104,13 @@ struct WaveHip {
     static __device__ uint32_t atomic_add(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }
 };
 
-// Resident waves per SIMD the batched kernels are compiled for (the register
-// budget per lane is 512 / DTR_WAVES_PER_EU).
-#ifndef DTR_WAVES_PER_EU
-#define DTR_WAVES_PER_EU 1
-#endif
+// The batched kernels are compiled for W resident waves per SIMD: the
+// register budget per lane is 512 / W (W = 2 trades a few spills for twice
+// the waves that hide memory latency).
 #define DTR_BATCHED_KERNEL __global__ void __launch_bounds__(256) \
-    __attribute__((amdgpu_waves_per_eu(DTR_WAVES_PER_EU, DTR_WAVES_PER_EU)))
+    __attribute__((amdgpu_waves_per_eu(W, W)))
 
-// One work-item per track (track = order[k] when a schedule permutation is
-// given, k = global work-item index).
-template <class P>
+template <class P, int W>
 DTR_BATCHED_KERNEL propagate_grid_kernel(
     dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
     dtr::DevResults out, const uint32_t* __restrict__ order) {
@@ -128,7 +124,7 @@ DTR_BATCHED_KERNEL propagate_grid_kernel(
 // One track per lane, navigation searches pooled across the wave.  Every
 // lane of the last wave takes part in the pooled searches, also the lanes
 // past the end of the batch.
-template <class P>
+template <class P, int W>
 DTR_BATCHED_KERNEL propagate_coop_kernel(
     dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
     dtr::DevResults out, const uint32_t* __restrict__ order) {
@@ -144,7 +140,7 @@ DTR_BATCHED_KERNEL propagate_coop_kernel(
 // full while its tracks have very different step counts.  A wave claims
 // `claim` entries at a time with one atomic on *queue; it ends when the queue
 // is drained and its lanes are done.
-template <class P>
+template <class P, int W>
 DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
     dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
     dtr::DevResults out, const uint32_t* __restrict__ order, uint32_t* __restrict__ queue,
@@ -155,6 +151,74 @@ DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
 }
 
 __global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
+
+// ---------------------------------------------------------------- schedule --
+// Counting sort of the tracks by direction cell.
+__device__ inline uint32_t direction_cell(const double* __restrict__ fp, uint64_t n, uint64_t i,
+                                          uint32_t eta_bins_per_unit, uint32_t phi_bins, float eta_max) {
+    const double dx = fp[4 * n + i], dy = fp[5 * n + i];
+    double dz = fp[6 * n + i];
+    dz = dz < -1.0 + 1e-12 ? -1.0 + 1e-12 : (dz > 1.0 - 1e-12 ? 1.0 - 1e-12 : dz);
+    float eta = static_cast<float>(0.5 * log((1.0 + dz) / (1.0 - dz)));
+    eta = eta < -eta_max ? -eta_max : (eta > eta_max ? eta_max : eta);
+    const uint32_t n_eta = static_cast<uint32_t>(2.f * eta_max * static_cast<float>(eta_bins_per_unit)) + 1u;
+    uint32_t ie = static_cast<uint32_t>((eta + eta_max) * static_cast<float>(eta_bins_per_unit));
+    ie = ie >= n_eta ? n_eta - 1u : ie;
+    const float phi = static_cast<float>(atan2(dy, dx));
+    uint32_t ip = static_cast<uint32_t>((phi + 3.14159265f) * (static_cast<float>(phi_bins) / 6.2831853f));
+    ip = ip >= phi_bins ? phi_bins - 1u : ip;
+    return ie * phi_bins + ip;
+}
+
+__global__ void sort_clear_kernel(uint32_t* __restrict__ count, uint32_t cells) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells) count[i] = 0u;
+}
+
+__global__ void sort_count_kernel(const double* __restrict__ fp, uint32_t n, uint32_t* __restrict__ count,
+                                  uint32_t eta_bins_per_unit, uint32_t phi_bins, float eta_max) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    atomicAdd(&count[direction_cell(fp, n, i, eta_bins_per_unit, phi_bins, eta_max)], 1u);
+}
+
+// exclusive scan of the cell counts in place, one block of 1024 threads
+__global__ void __launch_bounds__(1024) sort_scan_kernel(uint32_t* __restrict__ count, uint32_t cells) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (cells + 1023u) / 1024u;
+    const uint32_t b = t * per, e = (b + per < cells) ? b + per : cells;
+    uint32_t sum = 0u;
+    for (uint32_t i = b; i < e; ++i) sum += count[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1u; d < 1024u; d <<= 1) {
+        const uint32_t v = (t >= d) ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t c = count[i];
+        count[i] = run;
+        run += c;
+    }
+}
+
+__global__ void sort_scatter_kernel(const double* __restrict__ fp, uint32_t n, uint32_t* __restrict__ offset,
+                                    uint32_t* __restrict__ order, uint32_t eta_bins_per_unit, uint32_t phi_bins,
+                                    float eta_max) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t cell = direction_cell(fp, n, i, eta_bins_per_unit, phi_bins, eta_max);
+    order[atomicAdd(&offset[cell], 1u)] = i;
+}
+
+uint32_t sort_cells(const dtr_sort_config* c) {
+    const uint32_t n_eta = static_cast<uint32_t>(2.f * c->eta_max * static_cast<float>(c->eta_bins_per_unit)) + 1u;
+    return n_eta * c->phi_bins;
+}
 
 dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
     dtr::DevConfig dc{};
@@ -193,8 +257,8 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
     return DTR_SUCCESS;
 }
 
-template <class P>
-int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+template <class P, int W>
+int launch_fast_w(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
                 const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
                 hipStream_t stream) {
     const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov,
@@ -204,10 +268,10 @@ int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevC
     if (L.schedule == DTR_SCHEDULE_GRID) {
         const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
         if (L.navigation == DTR_NAVIGATION_LANE)
-            hipLaunchKernelGGL(propagate_grid_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+            hipLaunchKernelGGL((propagate_grid_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
                                field->dev, dc, ti, r, L.order);
         else
-            hipLaunchKernelGGL(propagate_coop_kernel<P>, dim3(grid), dim3(block), shmem, stream, det->fast,
+            hipLaunchKernelGGL((propagate_coop_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
                                field->dev, dc, ti, r, L.order);
         return to_status(hipGetLastError());
     }
@@ -221,9 +285,18 @@ int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevC
     const uint64_t w = waves < need ? waves : need;
     const uint32_t wpb = block / 64u;
     const uint32_t grid = static_cast<uint32_t>((w + wpb - 1) / wpb);
-    hipLaunchKernelGGL(propagate_coop_persistent_kernel<P>, dim3(grid), dim3(block), shmem, stream,
+    hipLaunchKernelGGL((propagate_coop_persistent_kernel<P, W>), dim3(grid), dim3(block), shmem, stream,
                        det->fast, field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
     return to_status(hipGetLastError());
+}
+
+// register budget: 1 or 2 resident waves per SIMD
+template <class P>
+int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+                const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
+                hipStream_t stream) {
+    if (L.waves_per_simd >= 2u) return launch_fast_w<P, 2>(det, field, dc, in, r, L, stream);
+    return launch_fast_w<P, 1>(det, field, dc, in, r, L, stream);
 }
 
 }  // namespace
@@ -420,10 +493,43 @@ void dtr_default_launch(dtr_launch_config* l) {
     l->precision = DTR_PRECISION_MIXED;
     l->schedule = DTR_SCHEDULE_PERSISTENT;
     l->block_size = 256u;
-    l->waves_per_simd = 1u;
+    l->waves_per_simd = 2u;
     l->tracks_per_claim = 64u;
     l->min_idle = 1u;
     l->navigation = DTR_NAVIGATION_WAVE;
+}
+
+void dtr_default_sort(dtr_sort_config* c) {
+    if (!c) return;
+    c->eta_bins_per_unit = 20u;
+    c->phi_bins = 128u;
+    c->eta_max = 5.f;
+    c->pad = 0u;
+}
+
+size_t dtr_sort_workspace_size(const dtr_sort_config* c) {
+    if (!c) return 0u;
+    return static_cast<size_t>(sort_cells(c)) * sizeof(uint32_t);
+}
+
+int dtr_sort_tracks(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t* order, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+    if (!c || c->eta_bins_per_unit == 0u || c->phi_bins == 0u || !(c->eta_max > 0.f) || c->eta_max > 20.f ||
+        c->eta_bins_per_unit > 1000u || c->phi_bins > 100000u)
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if (n == 0u) return DTR_SUCCESS;
+    if (!fp || !order || !workspace) return DTR_ERROR_INVALID_ARGUMENT;
+    const uint32_t cells = sort_cells(c);
+    if (workspace_bytes < static_cast<size_t>(cells) * sizeof(uint32_t)) return DTR_ERROR_INVALID_ARGUMENT;
+    uint32_t* count = static_cast<uint32_t*>(workspace);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells);
+    hipLaunchKernelGGL(sort_count_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count,
+                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, count, cells);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count, order,
+                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
+    return to_status(hipGetLastError());
 }
 
 int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -434,7 +540,7 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (!launch) return DTR_ERROR_INVALID_ARGUMENT;
     dtr_launch_config L = *launch;
     if (L.block_size == 0u) L.block_size = 256u;
-    if (L.waves_per_simd == 0u) L.waves_per_simd = 1u;
+    if (L.waves_per_simd == 0u) L.waves_per_simd = 2u;
     if (L.tracks_per_claim == 0u) L.tracks_per_claim = 64u;
     if (L.min_idle == 0u) L.min_idle = 1u;
     if (L.min_idle > 64u) return DTR_ERROR_INVALID_ARGUMENT;

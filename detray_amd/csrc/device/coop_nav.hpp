@@ -70,7 +70,7 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
     if (req) {
         for (int i = 0; i < kCache; ++i) c.p(i) = kMax;
         const dtr_volume& vol = det.volumes[volume];
-        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) me.candidate_search(det.bf_rec[k].rec, os);
+        me.bf_search(vol, os);
         if (vol.grid >= 0) {
             int e0, e1;
             if (me.grid_window(vol, det.grids[vol.grid], a0, e0, a1, e1)) {

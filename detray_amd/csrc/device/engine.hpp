@@ -654,37 +654,65 @@ struct FastEngine {
                          static_cast<float>(tk.dir.y), static_cast<float>(tk.dir.z), margin, ft);
     }
 
+    // The volume's brute-force surfaces, evaluated two at a time (independent
+    // record loads and intersections) and inserted in list order.
+    DTR_AI void bf_search(const dtr_volume& vol, float overstep) {
+        uint32_t k = vol.bf_begin;
+        for (; k + 2 <= vol.bf_end; k += 2) {
+            Cand c0, c1;
+            c0.pad = c1.pad = 0;
+            const bool a0 = evaluate(det.bf_rec[k].rec, overstep, c0);
+            const bool a1 = evaluate(det.bf_rec[k + 1].rec, overstep, c1);
+            if (a0) insert_sorted(c0);
+            if (a1) insert_sorted(c1);
+        }
+        if (k < vol.bf_end) candidate_search(det.bf_rec[k].rec, overstep);
+    }
+
+    // The window's bins in row-major order, prefiltered kChunk at a time
+    // (independent loads, so a chunk costs one memory latency), the passing
+    // ones then evaluated and inserted in order.
     DTR_AI void grid_search(const dtr_volume& vol, float overstep) {
         const dtr_grid& g = det.grids[vol.grid];
         int a0, e0, a1, e1;
         if (!grid_window(vol, g, a0, e0, a1, e1)) return;
         const bool cyl = (g.type == DTR_GRID_CYLINDER);
         const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
+        const int w = e1 - a1;
+        const int total = (e0 - a0) * w;
         const float margin = sphere_margin(vol.grid);
-        constexpr int kRow = 8;
-        for (int b0 = a0; b0 < e0; ++b0) {
+        const float ft = static_cast<float>(tol_max());
+        const float ox = static_cast<float>(tk.pos.x), oy = static_cast<float>(tk.pos.y),
+                    oz = static_cast<float>(tk.pos.z);
+        const float dx = static_cast<float>(tk.dir.x), dy = static_cast<float>(tk.dir.y),
+                    dz = static_cast<float>(tk.dir.z);
+        auto bin_of = [&](int j) {
+            const int b0 = a0 + j / w, b1 = a1 + j % w;
             const int i0 = cyl ? wrap_bin(b0, n0) : b0;
-            const uint32_t row = g.bins + static_cast<uint32_t>(i0) * g.n1;
-            for (int b1s = a1; b1s < e1; b1s += kRow) {
-                const int cnt = (e1 - b1s) < kRow ? (e1 - b1s) : kRow;
-                uint32_t pass = 0u;
+            const int i1 = cyl ? b1 : wrap_bin(b1, n1);
+            return g.bins + static_cast<uint32_t>(i0) * g.n1 + static_cast<uint32_t>(i1);
+        };
+        constexpr int kChunk = 16;
+        for (int j0 = 0; j0 < total; j0 += kChunk) {
+            uint32_t pass = 0u;
+            int b0 = a0 + j0 / w, b1 = a1 + j0 % w;
 #pragma unroll
-                for (int k = 0; k < kRow; ++k) {
-                    if (k < cnt) {
-                        const int b1 = b1s + k;
-                        const int i1 = cyl ? b1 : wrap_bin(b1, n1);
-                        if (sphere_pass(row + i1, margin)) pass |= (1u << k);
-                    }
-                }
-                while (pass) {
-                    // re-read the (cached) bin entry rather than index a
-                    // register array dynamically, which would go to scratch
-                    const int k = __builtin_ctz(pass);
-                    pass &= pass - 1u;
-                    const int b1 = b1s + k;
+            for (int k = 0; k < kChunk; ++k) {
+                if (j0 + k < total) {
+                    const int i0 = cyl ? wrap_bin(b0, n0) : b0;
                     const int i1 = cyl ? b1 : wrap_bin(b1, n1);
-                    candidate_search(det.bin_rec[row + i1].rec, overstep);
+                    const uint32_t bin = g.bins + static_cast<uint32_t>(i0) * g.n1 + static_cast<uint32_t>(i1);
+                    if (prefilter(det.bin_rec[bin].box, ox, oy, oz, dx, dy, dz, margin, ft)) pass |= (1u << k);
                 }
+                if (++b1 == e1) {
+                    b1 = a1;
+                    ++b0;
+                }
+            }
+            while (pass) {
+                const int k = __builtin_ctz(pass);
+                pass &= pass - 1u;
+                candidate_search(det.bin_rec[bin_of(j0 + k)].rec, overstep);
             }
         }
     }
@@ -715,7 +743,7 @@ struct FastEngine {
         for (int i = 0; i < kCache; ++i) c.p(i) = kMax;
         tk.next = 0;
         tk.last = -1;
-        for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf_rec[k].rec, os);
+        bf_search(vol, os);
         if (vol.grid >= 0) grid_search(vol, os);
         update_status();
         if (tk.trust != kFull && tk.status == kOnPortal)
@@ -1554,9 +1582,10 @@ struct Lane {
     }
 };
 
-// One track from start to finish, lane on its own.
+// One track from start to finish, lane on its own.  Forced inline so that
+// every kernel instantiation allocates registers for its own occupancy.
 template <class P>
-DTR_HD void propagate_fast(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+DTR_AI void propagate_fast(const FastDetector& det, const DevField& field, const DevConfig& cfg,
                            const TrackInputs& in, const DevResults& out, uint32_t track, CacheView c) {
     Lane<P> lane;
     lane.start(det, field, cfg, in, c, track);

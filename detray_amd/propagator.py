@@ -284,7 +284,7 @@ class LaunchConfig:
     precision: str = "mixed"
     schedule: str = "persistent"
     block_size: int = 256
-    waves_per_simd: int = 1
+    waves_per_simd: int = 2
     tracks_per_claim: int = 64
     order: "object" = None
     min_idle: int = 1

@@ -297,7 +297,10 @@ typedef struct dtr_launch_config {
     uint32_t precision;        /* dtr_precision (default MIXED) */
     uint32_t schedule;         /* dtr_schedule (default PERSISTENT) */
     uint32_t block_size;       /* 64, 128, 192 or 256 (default 256) */
-    uint32_t waves_per_simd;   /* persistent: resident waves per SIMD, 1..32 (default 1) */
+    uint32_t waves_per_simd;   /* resident waves per SIMD the kernel is built for: 1
+                                  (512 registers per lane) or >= 2 (256); the
+                                  persistent schedule launches this many waves
+                                  per SIMD, 1..32 (default 2) */
     uint32_t tracks_per_claim; /* persistent: queue entries a wave claims per atomic,
                                   multiple of 64 (default 64) */
     uint32_t min_idle;         /* persistent: idle lanes a wave waits for before it
@@ -325,6 +328,29 @@ DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, const dtr_launch_config* launch,
                      void* stream);
+
+/* Track schedule: a permutation of the batch that groups tracks by
+ * direction cell (pseudorapidity bins of width 1/eta_bins_per_unit over
+ * |eta| < eta_max, phi_bins slices of phi), for dtr_launch_config.order.
+ * A counting sort on the device: three kernels on `stream`, no allocation
+ * and no memset, so it can be captured in a HIP graph together with the
+ * propagation.  The order of tracks inside one cell is not specified (the
+ * results of a track do not depend on it). */
+typedef struct dtr_sort_config {
+    uint32_t eta_bins_per_unit; /* default 20 */
+    uint32_t phi_bins;          /* default 128 */
+    float eta_max;              /* default 5; |eta| beyond goes to the edge bins */
+    uint32_t pad;
+} dtr_sort_config;
+
+DTR_API void dtr_default_sort(dtr_sort_config* cfg);
+/* bytes of device workspace dtr_sort_tracks needs for this configuration */
+DTR_API size_t dtr_sort_workspace_size(const dtr_sort_config* cfg);
+/* free_params: DEVICE [8][n]; order: DEVICE [n] output; workspace: DEVICE,
+ * dtr_sort_workspace_size bytes. */
+DTR_API int dtr_sort_tracks(const dtr_sort_config* cfg, const double* free_params, uint32_t n,
+                            uint32_t* order, void* workspace, size_t workspace_bytes,
+                            void* stream);
 
 /* Host-side builders (detector emitters and inputs).  The returned tables
  * are owned by the library; free them with dtr_host_detector_free. */

@@ -16,7 +16,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "detray_amd.h"
 
 def header_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:DTR_API\s+)?(?:int|void|const char\*)\s+(dtr_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:DTR_API\s+)?(?:int|void|size_t|const char\*)\s+(dtr_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_all_header_symbols():
