@@ -1107,8 +1107,14 @@ struct FastEngine {
         const NavSurface& erec = det.sf[dest];
         V3 n;
         if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
-            const double ph = phi_of(tk.pos);
-            n = {cos(ph), sin(ph), 0.0};
+            if constexpr (P::kMixed) {
+                // (cos, sin) of phi of the position, without trigonometry
+                const double rinv = 1.0 / perp(tk.pos);
+                n = {tk.pos.x * rinv, tk.pos.y * rinv, 0.0};
+            } else {
+                const double ph = phi_of(tk.pos);
+                n = {cos(ph), sin(ph), 0.0};
+            }
         } else {
             n = rec_axis(erec, 2);
         }
@@ -1126,15 +1132,29 @@ struct FastEngine {
         }
         S Bl[6];
         if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
-            const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
-            Bl[0] = static_cast<S>(-rinv * sin(ph)); Bl[1] = static_cast<S>(rinv * cos(ph)); Bl[2] = S(0);
-            Bl[3] = S(0);                            Bl[4] = S(0);                           Bl[5] = S(1);
+            if constexpr (P::kMixed) {
+                const double rinv2 = 1.0 / (tk.pos.x * tk.pos.x + tk.pos.y * tk.pos.y);
+                Bl[0] = static_cast<S>(-tk.pos.y * rinv2); Bl[1] = static_cast<S>(tk.pos.x * rinv2); Bl[2] = S(0);
+                Bl[3] = S(0);                              Bl[4] = S(0);                             Bl[5] = S(1);
+            } else {
+                const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
+                Bl[0] = static_cast<S>(-rinv * sin(ph)); Bl[1] = static_cast<S>(rinv * cos(ph)); Bl[2] = S(0);
+                Bl[3] = S(0);                            Bl[4] = S(0);                           Bl[5] = S(1);
+            }
         } else {
             Bl[0] = static_cast<S>(erec.r[0]); Bl[1] = static_cast<S>(erec.r[3]); Bl[2] = static_cast<S>(erec.r[6]);
             Bl[3] = static_cast<S>(erec.r[1]); Bl[4] = static_cast<S>(erec.r[4]); Bl[5] = static_cast<S>(erec.r[7]);
         }
         S Ba[6];
-        {
+        if constexpr (P::kMixed) {
+            // sin(theta) = |d_T|, cos(theta) = d_z, (cos, sin)(phi) = d_T / |d_T|
+            const double st = perp(tk.dir);
+            const double inv = 1.0 / st;
+            Ba[0] = static_cast<S>(-tk.dir.y * inv * inv); Ba[1] = static_cast<S>(tk.dir.x * inv * inv);
+            Ba[2] = S(0);
+            Ba[3] = static_cast<S>(tk.dir.x * inv * tk.dir.z); Ba[4] = static_cast<S>(tk.dir.y * inv * tk.dir.z);
+            Ba[5] = static_cast<S>(-st);
+        } else {
             const double th = theta_of(tk.dir), ph = phi_of(tk.dir);
             const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
             Ba[0] = static_cast<S>(-sp / st); Ba[1] = static_cast<S>(cp / st); Ba[2] = S(0);
@@ -1195,11 +1215,25 @@ struct FastEngine {
     DTR_AI void material_interaction(uint32_t dest) {
         const NavSurface& rec = det.sf[dest];
         const dtr_material_slab& m = det.materials[rec.material];
-        V3 nrm;
-        if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
-        else nrm = rec_axis(rec, 2);
-        const V3 bd = Engine::bound_dir(tk.bound);
-        const double cos_inc = fabs(dot(bd, nrm));
+        double cos_inc;
+        if constexpr (P::kMixed) {
+            // incidence from the free direction and position (the bound
+            // angles and loc0 were just computed from them)
+            V3 nrm;
+            if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+                const double rinv = 1.0 / perp(tk.pos);
+                nrm = {tk.pos.x * rinv, tk.pos.y * rinv, 0.0};
+            } else {
+                nrm = rec_axis(rec, 2);
+            }
+            cos_inc = fabs(dot(tk.dir, nrm));
+        } else {
+            V3 nrm;
+            if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
+            else nrm = rec_axis(rec, 2);
+            const V3 bd = Engine::bound_dir(tk.bound);
+            cos_inc = fabs(dot(bd, nrm));
+        }
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];
         const double path_seg = m.thickness / cos_inc;
@@ -1260,14 +1294,32 @@ struct FastEngine {
             if (cfg.do_cov) tk.cov[covi(4, 4)] += sigma_qop * sigma_qop;
         }
         if (cfg.do_cov) {
-            const V3 d = Engine::bound_dir(tk.bound);
+            const double dz = P::kMixed ? tk.dir.z : Engine::bound_dir(tk.bound).z;
             const S var = theta0 * theta0;
-            tk.cov[covi(2, 2)] += (d.z == 1.0) ? std::numeric_limits<S>::max()
-                                               : static_cast<S>(static_cast<double>(var) / (1.0 - d.z * d.z));
+            tk.cov[covi(2, 2)] += (dz == 1.0) ? std::numeric_limits<S>::max()
+                                              : static_cast<S>(static_cast<double>(var) / (1.0 - dz * dz));
             tk.cov[covi(3, 3)] += var;
         }
-        tk.pos = bound_pos(dest, tk.bound);
-        tk.dir = Engine::bound_dir(tk.bound);
+        // parameter_setter (parameter_updater.hpp:585-627): the free state
+        // is re-derived from the bound parameters, which puts the position
+        // onto the surface.  The mixed policy does the same projection
+        // without trigonometry: onto the plane (planar masks), onto the
+        // radius (cylinders), and renormalises the direction.
+        if constexpr (P::kMixed) {
+            const NavSurface& r = det.sf[dest];
+            if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
+                const double k = r.m[0] / perp(tk.pos);
+                tk.pos = {tk.pos.x * k, tk.pos.y * k, tk.pos.z};
+            } else {
+                const V3 l = rec_to_local(r, tk.pos);
+                tk.pos = rec_to_global(r, {l.x, l.y, 0.0});
+            }
+            const double dn = 1.0 / norm(tk.dir);
+            tk.dir = {tk.dir.x * dn, tk.dir.y * dn, tk.dir.z * dn};
+        } else {
+            tk.pos = bound_pos(dest, tk.bound);
+            tk.dir = Engine::bound_dir(tk.bound);
+        }
         tk.time = tk.bound[5];
         tk.qop = tk.bound[4];
         if (cfg.estimate_noise) estimate_noise();
