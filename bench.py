@@ -48,6 +48,20 @@ def alg_bytes(n_surfaces):
     return float(np.sum(464 + 8 * n_surfaces.astype(np.int64)))
 
 
+def traffic_bytes(headline: bool):
+    """HBM bytes per launch of the propagation kernel measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by the copy-kernel
+    calibration), as recorded in profiles/traffic.json for the headline
+    workload; None if not recorded."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not headline or not path.exists():
+        return None
+    try:
+        return float(json.loads(path.read_text())["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def compare(mixed, ref):
     """Surface-sequence and bound-parameter agreement of two result sets
     (device tensors), reference convention: loc differences relative to the
@@ -95,6 +109,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
+    # launch used with --no-autotune
+    ap.add_argument("--sort", default="eta_phi")
+    ap.add_argument("--block", type=int, default=64)
+    ap.add_argument("--waves", type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,8 +143,8 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     if args.no_autotune:
-        launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
-        key = "cells"
+        launch = da.LaunchConfig("mixed", "grid", args.block, args.waves, 64, None, 1, "lane")
+        key = args.sort
     else:
         t0 = time.perf_counter()
         launch, key, table = autotune(det, field, cfg, tracks, "mixed",
@@ -301,7 +319,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_bytes(not loopers),
+            "alg_bytes_per_launch": bytes_per_step / len(subs),
         },
         "cpu_baseline": cpu,
     }
