@@ -1,6 +1,7 @@
 """detray_amd: MI355X-native track propagation (detray's propagate hot path)
 behind a C ABI, with a thin Python mirror of the reference interface."""
 
+from .adjoint import AdjointConfig, Chi2Propagator, Hits, chi2  # noqa: F401
 from .capi import DetrayError, load  # noqa: F401
 from .propagator import (  # noqa: F401
     GEV,
@@ -18,6 +19,6 @@ from .propagator import (  # noqa: F401
 )
 
 __all__ = [
-    "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
+    "AdjointConfig", "Chi2Propagator", "Hits", "chi2", "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
     "PropagationResult", "Propagator", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
 ]

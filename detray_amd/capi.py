@@ -200,6 +200,45 @@ class SortConfig(C.Structure):
     ]
 
 
+# dtr_adjoint_model, dtr_tape_status
+ADJOINT_EXACT, ADJOINT_TRANSPORT = 0, 1
+TAPE_OK, TAPE_STEP_OVERFLOW, TAPE_EVENT_OVERFLOW, TAPE_UNSUPPORTED = 0, 1, 2, 3
+
+
+class AdjointConfig(C.Structure):
+    _fields_ = [
+        ("max_steps", C.c_uint32),
+        ("max_events", C.c_uint32),
+        ("checkpoint_every", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("model", C.c_uint32),
+        ("block_size", C.c_uint32),
+        ("sigma_loc0", C.c_double),
+        ("sigma_loc1", C.c_double),
+    ]
+
+
+class Hits(C.Structure):
+    _fields_ = [
+        ("max_hits", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("surface", C.c_void_p),
+        ("loc", C.c_void_p),
+        ("n_hits", C.c_void_p),
+    ]
+
+
+class AdjointBatch(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("bound_params", C.c_void_p),
+        ("start_surface", C.c_void_p),
+        ("tape", C.c_void_p),
+        ("tape_bytes", C.c_size_t),
+    ]
+
+
 class ToyConfig(C.Structure):
     _fields_ = [
         ("n_brl_layers", C.c_uint32),
@@ -248,6 +287,13 @@ _SIGNATURES = {
     "dtr_sort_workspace_size": (C.c_size_t, [C.POINTER(SortConfig)]),
     "dtr_sort_tracks": (C.c_int, [C.POINTER(SortConfig), C.c_void_p, C.c_uint32, C.c_void_p,
                                   C.c_void_p, C.c_size_t, C.c_void_p]),
+    "dtr_default_adjoint": (None, [C.POINTER(AdjointConfig)]),
+    "dtr_adjoint_tape_size": (C.c_size_t, [C.POINTER(AdjointConfig), C.c_uint32]),
+    "dtr_adjoint_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(AdjointConfig),
+                                      C.POINTER(AdjointBatch), C.POINTER(Hits), C.POINTER(Hits), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dtr_adjoint_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(AdjointConfig),
+                                       C.POINTER(AdjointBatch), C.POINTER(Hits), C.c_void_p, C.c_void_p]),
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),

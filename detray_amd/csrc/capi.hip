@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "detray_amd.h"
+#include "device/adjoint.hpp"
 #include "device/coop_nav.hpp"
 #include "device/engine.hpp"
 #include "device/propagator.hpp"
@@ -151,6 +152,28 @@ DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
 }
 
 __global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
+
+// ------------------------------------------------------------------ adjoint --
+// Recording forward pass: one work-item per track, lane navigation with the
+// candidate cache in LDS.
+template <class P>
+__global__ void __launch_bounds__(256) adjoint_record_kernel(dtr::FastDetector det, dtr::DevField field,
+                                                             dtr::DevConfig cfg, dtr::AdjointIO io) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= io.n) return;
+    dtr::adjoint_record<P>(det, field, cfg, io, static_cast<uint32_t>(k), lds_cache(smem));
+}
+
+// Backward pass: one work-item per track.
+template <class P>
+__global__ void __launch_bounds__(256) adjoint_backward_kernel(dtr::FastDetector det, dtr::DevField field,
+                                                               dtr::DevConfig cfg, dtr::AdjointIO io, int exact,
+                                                               double* __restrict__ grad) {
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= io.n) return;
+    dtr::adjoint_backward<P>(det, field, cfg, io, exact != 0, static_cast<uint32_t>(k), grad);
+}
 
 // ---------------------------------------------------------------- schedule --
 // Counting sort of the tracks by direction cell.
@@ -288,6 +311,39 @@ int launch_fast_w(const dtr_detector* det, const dtr_field* field, const dtr::De
     hipLaunchKernelGGL((propagate_coop_persistent_kernel<P, W>), dim3(grid), dim3(block), shmem, stream,
                        det->fast, field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
     return to_status(hipGetLastError());
+}
+
+int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                       const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
+    if (!det || !field || !c || !a || !b) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->max_steps > (1u << 20) || a->max_events > 4096u || a->checkpoint_every > a->max_steps)
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->precision != DTR_PRECISION_FP64 && a->precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->model != DTR_ADJOINT_EXACT && a->model != DTR_ADJOINT_TRANSPORT) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->block_size != 0u && (a->block_size % 64u != 0u || a->block_size > 256u)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (!(a->sigma_loc0 > 0.0) || !(a->sigma_loc1 > 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if (b->n == 0u) return DTR_SUCCESS;
+    if (!b->bound_params || !b->start_surface || !b->tape) return DTR_ERROR_INVALID_ARGUMENT;
+    if (b->tape_bytes < dtr::tape_bytes(b->n, a->max_steps, a->max_events, a->checkpoint_every))
+        return DTR_ERROR_INVALID_ARGUMENT;
+    return DTR_SUCCESS;
+}
+
+dtr::AdjointIO make_adjoint_io(const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
+    dtr::AdjointIO io{};
+    io.bound0 = b->bound_params;
+    io.start_sf = b->start_surface;
+    io.n = b->n;
+    if (hits) io.hits = dtr::HitsView{hits->max_hits, hits->surface, hits->loc, hits->n_hits};
+    io.iv0 = 1.0 / (a->sigma_loc0 * a->sigma_loc0);
+    io.iv1 = 1.0 / (a->sigma_loc1 * a->sigma_loc1);
+    io.tape = dtr::tape_carve(b->tape, b->n, a->max_steps, a->max_events, a->checkpoint_every);
+    return io;
 }
 
 // register budget: 1 or 2 resident waves per SIMD
@@ -559,6 +615,73 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);
+}
+
+void dtr_default_adjoint(dtr_adjoint_config* a) {
+    if (!a) return;
+    std::memset(a, 0, sizeof(*a));
+    a->max_steps = 512u;
+    a->max_events = 64u;
+    a->checkpoint_every = 32u;
+    a->precision = DTR_PRECISION_MIXED;
+    a->model = DTR_ADJOINT_EXACT;
+    a->block_size = 128u;
+    a->sigma_loc0 = 0.010;
+    a->sigma_loc1 = 0.050;
+}
+
+size_t dtr_adjoint_tape_size(const dtr_adjoint_config* a, uint32_t n) {
+    if (!a || a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return 0u;
+    return dtr::tape_bytes(n, a->max_steps, a->max_events, a->checkpoint_every);
+}
+
+int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                        const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits,
+                        dtr_hits* record, double* chi2, double* grad_transport, uint32_t* status, void* stream) {
+    const int rc = check_adjoint_args(det, field, c, a, b, hits);
+    if (rc != DTR_SUCCESS || b->n == 0u) return rc;
+    if (record && (record->max_hits == 0u || !record->surface || !record->loc || !record->n_hits))
+        return DTR_ERROR_INVALID_ARGUMENT;
+    dtr::DevConfig dc = make_dev_config(c);
+    dc.max_surfaces = 0u;  // the event list on the tape replaces the surface sequencer
+    dtr::AdjointIO io = make_adjoint_io(a, b, hits);
+    if (record) io.record = dtr::HitsOut{record->max_hits, record->surface, record->loc, record->n_hits};
+    io.chi2 = chi2;
+    io.grad_transport = grad_transport;
+    io.status = status;
+    const uint32_t block = a->block_size ? a->block_size : 128u;
+    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
+    const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (a->precision == DTR_PRECISION_FP64)
+        hipLaunchKernelGGL(adjoint_record_kernel<dtr::PolicyFP64>, dim3(grid), dim3(block), shmem, st, det->fast,
+                           field->dev, dc, io);
+    else
+        hipLaunchKernelGGL(adjoint_record_kernel<dtr::PolicyMixed>, dim3(grid), dim3(block), shmem, st, det->fast,
+                           field->dev, dc, io);
+    return to_status(hipGetLastError());
+}
+
+int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                         const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits,
+                         double* grad, void* stream) {
+    const int rc = check_adjoint_args(det, field, c, a, b, hits);
+    if (rc != DTR_SUCCESS || b->n == 0u) return rc;
+    if (!grad) return DTR_ERROR_INVALID_ARGUMENT;
+    dtr::DevConfig dc = make_dev_config(c);
+    dc.max_surfaces = 0u;
+    const dtr::AdjointIO io = make_adjoint_io(a, b, hits);
+    const uint32_t block = a->block_size ? a->block_size : 128u;
+    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int exact = a->model == DTR_ADJOINT_EXACT ? 1 : 0;
+    if (a->precision == DTR_PRECISION_FP64)
+        hipLaunchKernelGGL(adjoint_backward_kernel<dtr::PolicyFP64>, dim3(grid), dim3(block), 0, st, det->fast,
+                           field->dev, dc, io, exact, grad);
+    else
+        hipLaunchKernelGGL(adjoint_backward_kernel<dtr::PolicyMixed>, dim3(grid), dim3(block), 0, st, det->fast,
+                           field->dev, dc, io, exact, grad);
+    return to_status(hipGetLastError());
 }
 
 }  // extern "C"

@@ -330,6 +330,7 @@ struct FastTrack {
     double constraint_acc, constraint_abort;
     T3<S> dtds3;
     double mass, charge;
+    double h_stage;  // step size the stages of the last step were evaluated at
     uint32_t n_trials, n_steps;
     // transport Jacobian (reduced) and covariance in the stage precision
     S J[25];
@@ -353,8 +354,10 @@ DTR_HD uint32_t local_navigation_ool(const FastDetector& det, const DevConfig& c
                                      double ext_tol, uint16_t volume, float overstep, CacheView c);
 
 // TK is the track type: FastTrack<P> for the propagation loop, NavLite inside
-// the out-of-line local navigation.
-template <class P, class TK = FastTrack<P>>
+// the out-of-line local navigation.  kJacOut: transport_covariance also
+// writes the full 6x6 bound-to-bound Jacobian to jf_out (recording pass of
+// the adjoint).
+template <class P, class TK = FastTrack<P>, bool kJacOut = false>
 struct FastEngine {
     using S = typename P::S;
     const FastDetector& det;
@@ -362,6 +365,7 @@ struct FastEngine {
     const DevConfig& cfg;
     TK& tk;
     CacheView c;
+    double* jf_out = nullptr;
 
     DTR_AI bool on_surface() const { return tk.status == kOnObject || tk.status == kOnPortal; }
     DTR_AI int n_cand() const { return tk.last - tk.next + 1; }
@@ -894,6 +898,7 @@ struct FastEngine {
         for (uint32_t i = 0; i < cfg.max_rk_updates; ++i) {
             tk.n_trials++;
             const double h = tk.step_size;
+            tk.h_stage = h;
             const double h2 = h * h;
             const double half_h = h * 0.5;
             const V3 pos1 = add(add(pos, mul(half_h, tk.dir)), mul(h2 * 0.125, k1d));
@@ -1168,6 +1173,16 @@ struct FastEngine {
             Jf[1][cc] = Bl[3] * Mp[0][cc] + Bl[4] * Mp[1][cc] + Bl[5] * Mp[2][cc];
             Jf[2][cc] = Ba[0] * Md[0][cc] + Ba[1] * Md[1][cc] + Ba[2] * Md[2][cc];
             Jf[3][cc] = Ba[3] * Md[0][cc] + Ba[4] * Md[1][cc] + Ba[5] * Md[2][cc];
+        }
+        if constexpr (kJacOut) {
+            for (int r = 0; r < 6; ++r)
+                for (int cc = 0; cc < 6; ++cc) {
+                    double v = 0.0;
+                    if (r < 4 && cc < 5) v = static_cast<double>(Jf[r][cc]);
+                    else if (r == 4 && cc == 4) v = static_cast<double>(Mq4);
+                    else if (r == 5 && cc == 5) v = 1.0;
+                    jf_out[r * 6 + cc] = v;
+                }
         }
         // C' = Jf C Jf^T with the block structure of Jf
         auto C = [&](int r, int cc) { return tk.cov[covi(r, cc)]; };
@@ -1447,6 +1462,7 @@ struct Lane {
         tk.overflow = false;
         tk.stopped = false;
         tk.step_size = 0.0;
+        tk.h_stage = 0.0;
         tk.next_step_size = 0.0;
         tk.path_length = 0.0;
         tk.abs_path_length = 0.0;

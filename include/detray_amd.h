@@ -352,6 +352,101 @@ DTR_API int dtr_sort_tracks(const dtr_sort_config* cfg, const double* free_param
                             uint32_t* order, void* workspace, size_t workspace_bytes,
                             void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiation: d(chi2 of hit residuals)/d(initial bound parameters)    */
+/* ------------------------------------------------------------------------ */
+/* chi2 = sum over the hits matched on the track's sensitive surfaces of
+ * (loc0 - m0)^2 / sigma_loc0^2 + (loc1 - m1)^2 / sigma_loc1^2, with loc the
+ * bound parameters at the surface before its material (the quantities the
+ * reference's parameter_updater.hpp:422-572 computes there).  A hit is
+ * matched when the track crosses its surface, in the track's hit order.
+ * The reference has no adjoint: it propagates Jacobians forward
+ * (parameter_updater.hpp:422-572, rk_stepper.hpp:633-956); the gradient the
+ * chain of those Jacobians gives is returned by dtr_adjoint_forward beside
+ * chi2 and is what DTR_ADJOINT_TRANSPORT reproduces. */
+enum dtr_adjoint_model {
+    /* derivative of the propagation itself: field gradient of the
+     * trilinear map, derivative of the mean energy loss, normalisation of
+     * the direction */
+    DTR_ADJOINT_EXACT = 0,
+    /* the derivative the reference's transport Jacobians describe
+     * (stepping::config use_field_gradient = use_eloss_gradient = false) */
+    DTR_ADJOINT_TRANSPORT = 1
+};
+
+typedef struct dtr_adjoint_config {
+    uint32_t max_steps;        /* tape capacity: Runge-Kutta steps per track (512) */
+    uint32_t max_events;       /* tape capacity: sensitive / material surfaces per track (64) */
+    uint32_t checkpoint_every; /* K: the free state is stored every K steps (32) */
+    uint32_t precision;        /* dtr_precision of both passes (MIXED) */
+    uint32_t model;            /* dtr_adjoint_model of the backward pass (EXACT) */
+    uint32_t block_size;       /* work-items per block, 64..256, multiple of 64 (128) */
+    double sigma_loc0;         /* hit resolution in loc0, mm (0.010) */
+    double sigma_loc1;         /* hit resolution in loc1, mm (0.050) */
+} dtr_adjoint_config;
+
+/* Hits of a batch, DEVICE memory: surface [max_hits][n] (in crossing
+ * order), loc [2][max_hits][n] (loc0 block, then loc1 block), n_hits [n]. */
+typedef struct dtr_hits {
+    uint32_t max_hits;
+    uint32_t pad;
+    uint32_t* surface;
+    double* loc;
+    uint32_t* n_hits;
+} dtr_hits;
+
+/* Tracks of a differentiated batch, DEVICE memory: bound start parameters
+ * [6][n] on start_surface [n] (propagator::state(bound_track_parameters),
+ * propagator.hpp:121-162), and the tape of dtr_adjoint_tape_size bytes that
+ * the forward pass writes and the backward pass reads. */
+typedef struct dtr_adjoint_batch {
+    uint32_t n;
+    uint32_t pad;
+    const double* bound_params;
+    const uint32_t* start_surface;
+    void* tape;
+    size_t tape_bytes;
+} dtr_adjoint_batch;
+
+/* Per-track tape status written by dtr_adjoint_forward. */
+enum dtr_tape_status {
+    DTR_TAPE_OK = 0,
+    DTR_TAPE_STEP_OVERFLOW = 1,   /* more than max_steps steps */
+    DTR_TAPE_EVENT_OVERFLOW = 2,  /* more than max_events surfaces */
+    /* not differentiable here: the track stopped in material, or an event
+     * reused the bound parameters of an earlier one */
+    DTR_TAPE_UNSUPPORTED = 3
+};
+
+/* Defaults of dtr_adjoint_config. */
+DTR_API void dtr_default_adjoint(dtr_adjoint_config* cfg);
+/* Bytes of tape for n tracks (about 24 KB per track at the defaults). */
+DTR_API size_t dtr_adjoint_tape_size(const dtr_adjoint_config* cfg, uint32_t n);
+
+/* Forward pass: propagator::propagate of every track from its bound start
+ * (same stepper, navigator and actor chain as dtr_propagate, without the
+ * surface sequencer), writing the tape.  Optional DEVICE outputs (NULL to
+ * skip): chi2 [n] against `hits`; grad_transport [6][n], the gradient of
+ * chi2 from the chain of the reference's bound-to-bound transport
+ * Jacobians; status [n] (dtr_tape_status).  `record`, if not NULL, receives
+ * the bound loc0/loc1 of every sensitive surface crossed (true
+ * intersections, for making hits).  `hits` may be NULL (chi2 = 0). */
+DTR_API int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field,
+                                const dtr_prop_config* cfg, const dtr_adjoint_config* acfg,
+                                const dtr_adjoint_batch* batch, const dtr_hits* hits,
+                                dtr_hits* record, double* chi2, double* grad_transport,
+                                uint32_t* status, void* stream);
+
+/* Backward pass: grad [6][n] (DEVICE) = d chi2 / d(bound start parameters)
+ * of every track, from the tape of the last dtr_adjoint_forward on this
+ * batch (same det, field, cfg, acfg precision and hits).  The segment of K
+ * steps before each checkpoint is recomputed from it.  Tracks whose tape
+ * status is not DTR_TAPE_OK get NaN. */
+DTR_API int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field,
+                                 const dtr_prop_config* cfg, const dtr_adjoint_config* acfg,
+                                 const dtr_adjoint_batch* batch, const dtr_hits* hits,
+                                 double* grad, void* stream);
+
 /* Host-side builders (detector emitters and inputs).  The returned tables
  * are owned by the library; free them with dtr_host_detector_free. */
 typedef struct dtr_toy_config {

@@ -409,6 +409,11 @@ struct Engine {
     uint32_t seq_stride = 0;
     uint32_t n_seq = 0;
     bool overflow = false;
+    // bound loc0/loc1 at the sensitive surfaces (oracle_propagate_hits)
+    uint32_t* hit_sf = nullptr;
+    double* hit_loc = nullptr;  // loc0 at [k * hit_stride], loc1 at [(hit_cap + k) * hit_stride]
+    uint64_t hit_stride = 0;
+    uint32_t hit_cap = 0, n_hit = 0;
     // trace
     uint32_t* tr_vol = nullptr;
     uint32_t* tr_sf = nullptr;
@@ -1073,6 +1078,7 @@ struct Engine {
         if (up.sf != kInvalid) {
             if (up.sf == dest) {
                 // on the initial surface: notify observers without transport
+                record_hit(dest);
                 material_and_setter(dest);
                 return;
             }
@@ -1108,7 +1114,18 @@ struct Engine {
         }
         free_to_bound(dest, st.pos, st.dir, st.time, st.qop, up.bound);
         up.sf = dest;
+        record_hit(dest);
         material_and_setter(dest);
+    }
+
+    // the bound parameters a measurement on this surface would be compared
+    // with: those of the updater before the material interaction
+    void record_hit(uint32_t dest) {
+        if (!hit_sf || !on_sensitive() || n_hit >= hit_cap) return;
+        hit_sf[n_hit * hit_stride] = dest;
+        hit_loc[n_hit * hit_stride] = up.bound[0];
+        hit_loc[(static_cast<uint64_t>(hit_cap) + n_hit) * hit_stride] = up.bound[1];
+        n_hit++;
     }
 
     void material_and_setter(uint32_t dest) {
@@ -1249,10 +1266,13 @@ struct Engine {
 
 extern "C" {
 
-int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
-                     const dtr_prop_config* cfg, const dtr_track_batch* in,
-                     dtr_result_batch* out, int n_threads, oracle_trace* trace) {
+int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* field,
+                          const dtr_prop_config* cfg, const dtr_track_batch* in,
+                          dtr_result_batch* out, oracle_hits* hits, int n_threads,
+                          oracle_trace* trace) {
     using namespace oracle;
+    if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
+        return DTR_ERROR_INVALID_ARGUMENT;
     if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
     if (cfg->use_field_gradient || cfg->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
     if (!in->bound_params && !in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
@@ -1328,6 +1348,12 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
             e.seq = out->surfaces + i;
             e.seq_stride = static_cast<uint32_t>(n);
         }
+        if (hits) {
+            e.hit_sf = hits->surface + i;
+            e.hit_loc = hits->loc + i;
+            e.hit_stride = static_cast<uint64_t>(n);
+            e.hit_cap = hits->max_hits;
+        }
         if (trace) {
             e.tr_vol = trace->volume + static_cast<uint64_t>(i) * trace->capacity;
             e.tr_sf = trace->surface + static_cast<uint64_t>(i) * trace->capacity;
@@ -1355,8 +1381,15 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
         if (out->n_steps) out->n_steps[i] = e.st.n_steps;
         if (out->n_trials) out->n_trials[i] = static_cast<uint32_t>(e.st.n_trials);
         if (trace) trace->n[i] = e.tr_n;
+        if (hits) hits->n_hits[i] = e.n_hit;
     }
     return DTR_SUCCESS;
+}
+
+int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
+                     const dtr_prop_config* cfg, const dtr_track_batch* in,
+                     dtr_result_batch* out, int n_threads, oracle_trace* trace) {
+    return oracle_propagate_hits(det, field, cfg, in, out, nullptr, n_threads, trace);
 }
 
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz, double* b) {

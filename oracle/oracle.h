@@ -35,6 +35,24 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, int n_threads, oracle_trace* trace);
 
+/* Bound loc0 / loc1 of the updater at every sensitive surface a track
+ * crosses (before the material interaction), HOST memory: surface
+ * [max_hits][n], loc [2][max_hits][n], n_hits [n]. */
+typedef struct oracle_hits {
+    uint32_t max_hits;
+    uint32_t pad;
+    uint32_t* surface;
+    double* loc;
+    uint32_t* n_hits;
+} oracle_hits;
+
+/* oracle_propagate that also records the sensitive-surface parameters
+ * (hits may be NULL). */
+int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* field,
+                          const dtr_prop_config* cfg, const dtr_track_batch* in,
+                          dtr_result_batch* out, oracle_hits* hits, int n_threads,
+                          oracle_trace* trace);
+
 /* Evaluate the field at n points (host), for field tests. */
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz,
                     double* bxyz);

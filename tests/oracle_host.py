@@ -87,6 +87,82 @@ def propagate(detector, field, config, free_params, initial_cov=None,
     return out
 
 
+class Hits(C.Structure):
+    _fields_ = [("max_hits", C.c_uint32), ("pad", C.c_uint32), ("surface", C.c_void_p), ("loc", C.c_void_p),
+                ("n_hits", C.c_void_p)]
+
+
+def propagate_hits(detector, field, config, bound_params, start_surface, max_hits: int = 32,
+                   threads: int = 0) -> dict:
+    """Bound start propagation that also returns the updater's loc0/loc1 at
+    every sensitive surface crossed: hit_surface [max_hits, n], hit_loc
+    [2, max_hits, n], n_hits [n] (the true intersections)."""
+    lib = load()
+    bp = np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = np.ascontiguousarray(start_surface, dtype=np.uint32)
+    n = bp.shape[1]
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "status": np.zeros(n, dtype=np.uint8),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "hit_surface": np.full((max_hits, n), 0xFFFFFFFF, dtype=np.uint32),
+        "hit_loc": np.zeros((2, max_hits, n)),
+        "n_hits": np.zeros(n, dtype=np.uint32),
+    }
+    tb = capi.TrackBatch(n, 0, None, None, bp.ctypes.data, ss.ctypes.data)
+    rb = capi.ResultBatch(out["bound_params"].ctypes.data, None, None, None, None, None,
+                          out["n_steps"].ctypes.data, None, out["status"].ctypes.data)
+    h = Hits(max_hits, 0, out["hit_surface"].ctypes.data, out["hit_loc"].ctypes.data, out["n_hits"].ctypes.data)
+    cfg = config.to_c()
+    cfg.max_surfaces = 0
+    lib.oracle_propagate_hits.restype = C.c_int
+    capi.check(lib.oracle_propagate_hits(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(tb),
+                                         C.byref(rb), C.byref(h), C.c_int(threads), None),
+               "oracle_propagate_hits")
+    return out
+
+
+def chi2(rec: dict, meas_surface, meas_loc, meas_n, sigma_loc0: float, sigma_loc1: float) -> np.ndarray:
+    """chi2 of the recorded sensitive-surface parameters of each track
+    against its measurements: the k-th surface crossed is compared with the
+    first not yet used measurement on the same surface (in measurement
+    order)."""
+    n = rec["n_hits"].shape[0]
+    out = np.zeros(n)
+    for i in range(n):
+        nxt = 0
+        nm = int(meas_n[i])
+        for k in range(int(rec["n_hits"][i])):
+            sf = rec["hit_surface"][k, i]
+            for j in range(nxt, nm):
+                if meas_surface[j, i] == sf:
+                    r0 = rec["hit_loc"][0, k, i] - meas_loc[0, j, i]
+                    r1 = rec["hit_loc"][1, k, i] - meas_loc[1, j, i]
+                    out[i] += r0 * r0 / sigma_loc0 ** 2 + r1 * r1 / sigma_loc1 ** 2
+                    nxt = j + 1
+                    break
+    return out
+
+
+def chi2_gradient_fd(detector, field, config, bound_params, start_surface, meas_surface, meas_loc, meas_n,
+                     sigma_loc0: float, sigma_loc1: float, rel: float = 1e-6, max_hits: int = 32,
+                     params=range(5)) -> np.ndarray:
+    """Central finite differences of the oracle's chi2 with respect to the
+    bound start parameters, step rel * max(|b_i|, 1) (rel * |q/p| for q/p)."""
+    b0 = np.asarray(bound_params, dtype=np.float64)
+    g = np.zeros_like(b0)
+    for i in params:
+        h = rel * (np.abs(b0[i]) if i == 4 else np.maximum(np.abs(b0[i]), 1.0))
+        vals = []
+        for sgn in (1.0, -1.0):
+            b = b0.copy()
+            b[i] += sgn * h
+            rec = propagate_hits(detector, field, config, b, start_surface, max_hits)
+            vals.append(chi2(rec, meas_surface, meas_loc, meas_n, sigma_loc0, sigma_loc1))
+        g[i] = (vals[0] - vals[1]) / (2.0 * h)
+    return g
+
+
 def field_at(field, xyz: np.ndarray) -> np.ndarray:
     lib = load()
     pts = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
