@@ -23,7 +23,9 @@
 //  * exact: the derivative of the propagation itself (field gradient of the
 //    trilinear map, energy-loss derivative, normalisation of the direction,
 //    the free state continuing unchanged through a surface without
-//    material);
+//    material, and the length of each step that the navigator sized to
+//    reach a surface as a function of the state its distance was computed
+//    from);
 //  * transport: the derivative the reference's Jacobians describe
 //    (rk_stepper.hpp:633-956 without field gradient, stepping_config.hpp:
 //    27-48 use_field_gradient / use_eloss_gradient off; no normalisation;
@@ -51,6 +53,7 @@ struct Tape {
     uint32_t* ev_sf;    // [max_events][n] surface
     int32_t* ev_hit;    // [max_events][n] matched hit or -1
     uint32_t* ev_flags; // [max_events][n] kEvFromFree | kEvMaterial
+    uint32_t* ev_land;  // [max_events][n] surface the step before the event was sized to reach, or kInvalid
     uint32_t* n_steps;  // [n]
     uint32_t* n_events; // [n]
     uint32_t* status;   // [n] kTape*
@@ -66,7 +69,7 @@ DTR_HD uint64_t tape_doubles(uint32_t max_steps, uint32_t max_events, uint32_t e
 
 // bytes of a tape for n tracks
 DTR_HD size_t tape_bytes(uint64_t n, uint32_t max_steps, uint32_t max_events, uint32_t every) {
-    const uint64_t words = 4ull * max_events + 3ull;
+    const uint64_t words = 5ull * max_events + 3ull;
     return static_cast<size_t>(n * (8ull * tape_doubles(max_steps, max_events, every) + 4ull * words));
 }
 
@@ -101,6 +104,8 @@ DTR_HD Tape tape_carve(void* base, uint64_t n, uint32_t max_steps, uint32_t max_
     t.ev_hit = reinterpret_cast<int32_t*>(w);
     w += me;
     t.ev_flags = w;
+    w += me;
+    t.ev_land = w;
     w += me;
     t.n_steps = w;
     w += n;
@@ -420,33 +425,44 @@ DTR_HD void b2f_transpose(const NavSurface& r, const double b[6], V3 pb, V3 db, 
     bb[5] = 0.0;
 }
 
-// (pos_bar, dir_bar, qop_bar) = J^T b_bar for the path-corrected
-// free-to-bound map at the free state (p, d) with dt/ds = k:
-// J = F (I + p2f f2p), p2f = (d, k, 0), f2p = (-n / (n.d), 0, 0)
-// (parameter_updater.hpp:463-514)
-DTR_HD void f2b_transpose(const NavSurface& r, V3 p, V3 d, V3 k, const double bb[6], V3& pb, V3& db, double& qb) {
-    V3 vp, n;
+// (pos_bar, dir_bar, qop_bar) = J^T b_bar for the free-to-bound map F onto
+// surface r at the free state (p, d).  land: the surface the last step was
+// sized to reach, or null.  With a landing surface the position is first
+// moved along the trajectory onto it (a change of the state before the step
+// changes the step length): J = F (I + p2f f2p), p2f = (d, k, 0) with
+// dt/ds = k, f2p = (-n / (n.d), 0, 0), n the normal of `land`
+// (parameter_updater.hpp:463-514 with land = r).
+DTR_HD void f2b_transpose(const NavSurface& r, V3 p, V3 d, V3 k, const NavSurface* land, const double bb[6], V3& pb,
+                          V3& db, double& qb) {
+    V3 vp;
     if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
-        const double rho2 = p.x * p.x + p.y * p.y, rho = sqrt(rho2);
+        const double rho2 = p.x * p.x + p.y * p.y;
         vp = {-p.y * bb[0] / rho2, p.x * bb[0] / rho2, bb[1]};
-        n = {p.x / rho, p.y / rho, 0.0};
     } else if (r.shape == DTR_RING2D) {
         const V3 e0 = rec_col(r, 0), e1 = rec_col(r, 1);
         const V3 dl = sub(p, V3{r.t[0], r.t[1], r.t[2]});
         const double lx = dot(e0, dl), ly = dot(e1, dl);
         const double rl2 = lx * lx + ly * ly, rl = sqrt(rl2);
         vp = add(mul(bb[0] / rl, add(mul(lx, e0), mul(ly, e1))), mul(bb[1] / rl2, sub(mul(lx, e1), mul(ly, e0))));
-        n = rec_col(r, 2);
     } else {
         vp = add(mul(bb[0], rec_col(r, 0)), mul(bb[1], rec_col(r, 1)));
-        n = rec_col(r, 2);
     }
     const double rt2 = d.x * d.x + d.y * d.y, rt = sqrt(rt2), dd = rt2 + d.z * d.z;
     const V3 vd = add(mul(bb[2], V3{-d.y / rt2, d.x / rt2, 0.0}),
                       mul(bb[3], V3{d.z * d.x / (rt * dd), d.z * d.y / (rt * dd), -rt / dd}));
-    // (I + p2f f2p)^T v: the position adjoint gains f2p (p2f . v)
-    const double s = dot(d, vp) + dot(k, vd);
-    pb = add(vp, mul(-s / dot(n, d), n));
+    pb = vp;
+    if (land) {
+        // (I + p2f f2p)^T v: the position adjoint gains f2p (p2f . v)
+        V3 n;
+        if (land->shape == DTR_CONCENTRIC_CYLINDER2D) {
+            const double rho = sqrt(p.x * p.x + p.y * p.y);
+            n = {p.x / rho, p.y / rho, 0.0};
+        } else {
+            n = rec_col(*land, 2);
+        }
+        const double s = dot(d, vp) + dot(k, vd);
+        pb = add(vp, mul(-s / dot(n, d), n));
+    }
     db = vd;
     qb = bb[4];
 }
@@ -476,7 +492,7 @@ DTR_AI void tape_get_state(const double* base, uint64_t slot, uint64_t n, uint32
 template <class P>
 DTR_AI void record_update(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
                           FastTrack<P>& tk, CacheView c, uint32_t gid, uint32_t& n_ev, uint32_t& next_hit,
-                          uint32_t& n_rec, double& chi2, uint32_t& status) {
+                          uint32_t& n_rec, double& chi2, uint32_t& status, uint32_t& land) {
     using S = typename P::S;
     FastEngine<P, FastTrack<P>, true> e{det, field, cfg, tk, c};
     const bool sens = e.on_sensitive(), mat = e.on_material();
@@ -571,8 +587,10 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
         tp.ev_sf[ev * N + gid] = dest;
         tp.ev_hit[ev * N + gid] = hit;
         tp.ev_flags[ev * N + gid] = (from_free ? kEvFromFree : 0u) | (mat ? kEvMaterial : 0u);
+        tp.ev_land[ev * N + gid] = land;
         n_ev++;
     }
+    land = kInvalid;  // later events before the next step were reached without moving
 }
 
 // propagator::propagate (propagator.hpp:262-400) for one track from its
@@ -591,7 +609,7 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
     Lane<P> lane;
     lane.start(det, field, cfg, in, c, gid);
     FastTrack<P>& tk = lane.tk;
-    uint32_t n_ev = 0u, next_hit = 0u, n_rec = 0u, status = kTapeOk;
+    uint32_t n_ev = 0u, next_hit = 0u, n_rec = 0u, status = kTapeOk, land = kInvalid;
     double chi2 = 0.0;
     while (true) {
         // actor half (Lane::actor_half)
@@ -604,7 +622,8 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
             }
             tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
         }
-        if (cfg.use_material) record_update<P>(det, field, cfg, io, tk, c, gid, n_ev, next_hit, n_rec, chi2, status);
+        if (cfg.use_material)
+            record_update<P>(det, field, cfg, io, tk, c, gid, n_ev, next_hit, n_rec, chi2, status, land);
         if (tk.stopped && tk.status > kUnknown) {
             tk.status = kAbort;
             tk.heartbeat = false;
@@ -617,8 +636,13 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
         const uint32_t s0 = tk.n_steps;
         if (s0 % tp.every == 0u && s0 / tp.every < tp.n_ckpt)
             tape_put_state(tp.ckpt, static_cast<uint64_t>(s0 / tp.every) * 7u, N, gid, tk.pos, tk.dir, tk.qop);
+        // a step whose length is the navigator's distance lands on the
+        // navigator's target surface
+        const double dist = c.p(tk.next);
+        const uint32_t target = c.s(tk.next);
         lane.step_half(det, field, cfg, c);
         if (tk.n_steps != s0) {
+            land = (tk.step_size == dist) ? target : kInvalid;
             if (s0 < tp.max_steps) {
                 tp.h[(2ull * s0 + 0u) * N + gid] = tk.h_stage;
                 tp.h[(2ull * s0 + 1u) * N + gid] = tk.step_size;
@@ -638,13 +662,42 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
 }
 
 // -------------------------------------------------------------- backward --
+// Straight-line distance from (p, d) to the plane of r, with the arithmetic
+// of FastEngine::plane_rec (the navigator's candidate path).
+DTR_HD double plane_path(const NavSurface& r, V3 p, V3 d) {
+    const V3 sn = rec_col(r, 2);
+    return dot(sn, sub(V3{r.t[0], r.t[1], r.t[2]}, p)) / dot(d, sn);
+}
+
+// The step before a surface event had the navigator's distance to the
+// plane `sf` as its length, h = n.(t - p_c) / (n.d_c), computed at an
+// earlier state (p_c, d_c): usually the step's start, but the distance is
+// not refreshed after a setter of a surface reached without moving.  hbar
+// is the adjoint of h, (td, tk) = dx_end/dh the tangent at the end of the
+// step; (p_c, d_c) receive -hbar/(n.d_c) (n, h n).
+struct Landing {
+    bool active;
+    uint32_t sf;
+    double hbar;
+    V3 td, tk;
+};
+struct Pending {
+    int32_t ev;  // event whose state is (p_c, d_c), -1: none
+    bool post;   // its state after (true) or before (false) the event
+    V3 pc, dc;   // adjoint contribution to add there
+};
+DTR_HD void landing_contrib(V3 n, double hbar, double h, V3 dc, V3& pc_out, V3& dc_out) {
+    const double nd = dot(n, dc);
+    pc_out = mul(-hbar / nd, n);
+    dc_out = mul(-hbar * h / nd, n);
+}
 // Adjoint of event ev: (pb, db, qb) enter as the adjoint of the state after
 // the event and leave as that of the state before it.  Returns false for
 // the start event (its bound parameters are the initial ones), whose bound
 // adjoint is written to grad.
 DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
                           bool exact, uint32_t gid, uint64_t ev, double mass, double charge, V3& pb, V3& db,
-                          double& qb, double grad[6]) {
+                          double& qb, double grad[6], Landing& ld) {
     const Tape& tp = io.tape;
     const uint64_t N = io.n;
     const uint32_t sfi = tp.ev_sf[ev * N + gid];
@@ -697,13 +750,60 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
         const T3<double> B = field_at_s<double>(field, p);
         k = mul(q[6 * N], cross(d, V3{B.x, B.y, B.z}));
     }
+    // exact: the step before the event was sized to reach a plane (its
+    // target, which need not be this surface where surfaces overlap): the
+    // projection is plain and the step length carries the dependence
+    // (Landing); a cylinder target gets the path correction onto it.
+    // Transport: the reference's path correction onto this surface.
+    const uint32_t land = tp.ev_land[ev * N + gid];
+    const bool by_length = exact && land != kInvalid && det.sf[land].shape != DTR_CONCENTRIC_CYLINDER2D;
+    const NavSurface* lrec = exact ? (land != kInvalid && !by_length ? &det.sf[land] : nullptr) : &rec;
     V3 fpb, fdb;
     double fqb;
-    f2b_transpose(rec, p, d, k, bb, fpb, fdb, fqb);
+    f2b_transpose(rec, p, d, k, lrec, bb, fpb, fdb, fqb);
     pb = add(cpb, fpb);
     db = add(cdb, fdb);
     qb = cqb + fqb;
+    if (by_length) {
+        // dx_end/dh = (d, dt/ds) at the end of the step; hbar is formed
+        // once the adjoint of the end state is complete (event_step_adjoint)
+        ld.active = true;
+        ld.sf = land;
+        ld.td = d;
+        ld.tk = k;
+    }
     return true;
+}
+
+// event_adjoint with a pending step-length contribution added at the state
+// it belongs to (after the event: before its adjoint; before the event:
+// after it, through the bound-to-free map for the start event).
+DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                               const AdjointIO& io, bool exact, uint32_t gid, int32_t e, double mass, double charge,
+                               V3& pb, V3& db, double& qb, double grad[6], Landing& ld, Pending& pend,
+                               const double b0[6]) {
+    if (pend.ev == e && pend.post) {
+        pb = add(pb, pend.pc);
+        db = add(db, pend.dc);
+        pend.ev = -1;
+    }
+    const bool more = event_adjoint(det, field, cfg, io, exact, gid, static_cast<uint64_t>(e), mass, charge, pb, db,
+                                    qb, grad, ld);
+    if (pend.ev == e && !pend.post) {
+        if (more) {
+            pb = add(pb, pend.pc);
+            db = add(db, pend.dc);
+        } else {
+            double cb[6];
+            b2f_transpose(det.sf[io.start_sf[gid]], b0, pend.pc, pend.dc, 0.0, cb);
+            for (int k = 0; k < 6; ++k) grad[k] += cb[k];
+        }
+        pend.ev = -1;
+    }
+    // the landing step of this event: hbar from the complete adjoint of the
+    // state it ended in
+    if (ld.active) ld.hbar = dot(ld.td, pb) + dot(ld.tk, db);
+    return more;
 }
 
 template <class P>
@@ -729,6 +829,8 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
     double qb = 0.0;
     int32_t e = n_ev - 1;
     bool open = true;  // the start event has not been reached
+    Landing ld{false, 0u, 0.0, V3{0.0, 0.0, 0.0}, V3{0.0, 0.0, 0.0}};
+    Pending pend{-1, false, V3{0.0, 0.0, 0.0}, V3{0.0, 0.0, 0.0}};
     const uint32_t K = tp.every;
     const uint32_t n_seg = (n_steps + K - 1u) / K;
     for (int32_t sgi = static_cast<int32_t>(n_seg) - 1; sgi >= 0; --sgi) {
@@ -759,8 +861,8 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
         // back through the segment: the events after step j, then step j
         for (uint32_t j = j1; j > j0; --j) {
             while (e >= 0 && tp.ev_step[static_cast<uint64_t>(e) * N + gid] == j) {
-                event_adjoint(det, field, cfg, io, exact, gid, static_cast<uint64_t>(e), mass, charge, pb, db, qb,
-                              grad);
+                event_step_adjoint(det, field, cfg, io, exact, gid, e, mass, charge, pb, db, qb, grad, ld, pend,
+                                   b0);
                 e--;
             }
             V3 sp, sd;
@@ -769,11 +871,55 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
             const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + gid];
             const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + gid];
             rk_vjp<S>(field, sp, sd, sq, hs, ha, exact, pb, db, qb);
+            if (ld.active) {
+                // the state the step length was computed from: the step's
+                // start, or a state of the events after the previous step
+                ld.active = false;
+                const NavSurface& tr = det.sf[ld.sf];
+                const V3 n = rec_col(tr, 2);
+                auto same = [&](V3 p, V3 d) {
+                    const double s = plane_path(tr, p, d);
+                    return fabs(s - ha) <= 1e-9 * fabs(ha) + 1e-15;
+                };
+                bool found = same(sp, sd);
+                if (found) {
+                    V3 cp, cd;
+                    landing_contrib(n, ld.hbar, ha, sd, cp, cd);
+                    pb = add(pb, cp);
+                    db = add(db, cd);
+                }
+                for (int32_t e2 = e; !found && e2 >= 0 && tp.ev_step[static_cast<uint64_t>(e2) * N + gid] == j - 1u;
+                     --e2) {
+                    for (int w = 0; w < 2 && !found; ++w) {
+                        V3 p, d;
+                        double qq;
+                        if (w == 0) {
+                            tape_get_state(tp.ev_post, static_cast<uint64_t>(e2) * 7u, N, gid, p, d, qq);
+                        } else {
+                            const double* q = tp.ev_pre + static_cast<uint64_t>(e2) * 10u * N + gid;
+                            p = {q[0 * N], q[1 * N], q[2 * N]};
+                            d = {q[3 * N], q[4 * N], q[5 * N]};
+                        }
+                        if (same(p, d)) {
+                            found = true;
+                            pend.ev = e2;
+                            pend.post = (w == 0);
+                            landing_contrib(n, ld.hbar, ha, d, pend.pc, pend.dc);
+                        }
+                    }
+                }
+                if (!found) {
+                    // not identified: taken from the step's start
+                    V3 cp, cd;
+                    landing_contrib(n, ld.hbar, ha, sd, cp, cd);
+                    pb = add(pb, cp);
+                    db = add(db, cd);
+                }
+            }
         }
     }
     while (e >= 0 && open) {
-        open = event_adjoint(det, field, cfg, io, exact, gid, static_cast<uint64_t>(e), mass, charge, pb, db, qb,
-                             grad);
+        open = event_step_adjoint(det, field, cfg, io, exact, gid, e, mass, charge, pb, db, qb, grad, ld, pend, b0);
         e--;
     }
     if (open) {
