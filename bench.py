@@ -17,6 +17,11 @@ same batch for the speed-up, and the two are compared track by track
 (surface sequences, bound parameters).  ``--workload loopers`` runs the
 batching workload instead: 2M tracks, pT 0.1-100 GeV, |eta| < 3, in batches
 of 262144 replayed from HIP graphs, with p50/p99 per-batch latency.
+``--workload adjoint`` runs the differentiation workload: 262144 tracks
+from the beam pipe (pT 0.5-50 GeV, |eta| < 2.5), hits = true intersections
+of 1-sigma perturbed start parameters + smear (10 um, 50 um); one step is
+the recording forward pass and the adjoint backward pass (checkpoint every
+32 steps, mixed precision) giving d chi2 / d(bound start parameters).
 
 Prints one JSON line with the metric, the roofline of the propagation
 kernel and a CPU baseline (the FP64 oracle, OpenMP on the host cores).
@@ -103,7 +108,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tracks", type=int, default=None)
-    ap.add_argument("--workload", choices=("toy_solenoid", "loopers"), default="toy_solenoid")
+    ap.add_argument("--workload", choices=("toy_solenoid", "loopers", "adjoint"), default="toy_solenoid")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=None)
@@ -120,6 +125,9 @@ def main() -> int:
               "track ranges with no exchange, multi-GPU runs are not part of this build",
               file=sys.stderr)
         return 2
+
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import numpy as np
     import torch
@@ -322,6 +330,193 @@ def main() -> int:
             "traffic": traffic_bytes(not loopers),
             "alg_bytes_per_launch": bytes_per_step / len(subs),
         },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def adjoint_alg_bytes(n_steps, n_events, n_hits, every):
+    """Algorithmic HBM bytes of one forward + backward pass: per track the
+    bound start (48 B) and start surface (4 B) read by both passes, the hits
+    (20 B each + 4 B count) read by both, chi2 (8 B), status (4 B) and the
+    gradient (48 B) written; the tape written once and read once: 16 B per
+    step (step sizes), 56 B per checkpoint, 204 B per event (free state
+    before 80 B, bound 48 B, free state after 56 B, five words), 12 B of
+    counters; the segment replay writes and reads 56 B per step."""
+    import numpy as np
+
+    n_ckpt = (n_steps + every - 1) // every
+    tape = 16 * n_steps + 56 * n_ckpt + 204 * n_events + 12
+    per = 2 * (52 + 20 * n_hits + 4) + 8 + 4 + 48 + 2 * tape + 2 * 56 * n_steps
+    return float(np.sum(per.astype(np.int64)))
+
+
+def bench_adjoint(args) -> int:
+    import numpy as np
+    import torch
+
+    import detray_amd as da
+    from detray_amd import adjoint as adj
+
+    dev = torch.device("cuda:0")
+    det = da.Detector.toy(4, 7)
+    field = da.MagneticField.solenoid()
+    cfg = da.PropagationConfig(search_window=(3, 3))
+    n = args.tracks or 262144
+    seed = args.seed if args.seed is not None else 46
+    b0, start = adj.beampipe_starts(n, seed=seed)
+    truth = adj.perturbed(b0, seed=seed + 1)
+    bound = torch.from_numpy(b0).to(dev)
+    start_d = torch.from_numpy(start).to(dev)
+    acfg = da.AdjointConfig(precision="mixed")
+    prop = da.Chi2Propagator(cfg, acfg)
+    prop64 = da.Chi2Propagator(cfg, da.AdjointConfig(precision="fp64"))
+    rec = prop64.record_hits(det, field, torch.from_numpy(truth).to(dev), start_d, 32)
+    hits = rec.smeared(generator=torch.Generator(device=dev).manual_seed(seed + 2))
+    print(f"[bench] adjoint: {n} tracks from the beam pipe, mean hits {rec.n_hits.float().mean().item():.2f}",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(p, steps, warmup):
+        for _ in range(warmup):
+            f = p.forward(det, field, bound, start_d, hits)
+            p.backward(det, field, bound, start_d, hits, f)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for s in range(steps):
+            evs[s][0].record(stream)
+            f = p.forward(det, field, bound, start_d, hits)
+            evs[s][1].record(stream)
+            g = p.backward(det, field, bound, start_d, hits, f)
+            evs[s][2].record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        return wall, fwd, bwd, f, g
+
+    wall, fwd_ms, bwd_ms, f, g = timed(prop, args.steps, args.warmup)
+    value = n / wall
+    wall64, fwd64, bwd64, f64, g64 = timed(prop64, args.fp64_steps, 1)
+    status = f.status.cpu().numpy()
+    print(f"[bench] adjoint mixed: {wall * 1e3:.2f} ms/step (forward {fwd_ms:.2f}, backward {bwd_ms:.2f}) "
+          f"{value:.3e} tracks/s; fp64 {wall64 * 1e3:.2f} ms/step; status {np.bincount(status, minlength=4).tolist()}",
+          file=sys.stderr, flush=True)
+
+    # accuracy on a sample: exact adjoint (mixed) against central finite
+    # differences of the FP64 forward chi2, h = 1e-6 relative (tracks whose
+    # finite difference changes by more than 1e-4 at h/2 are reported and
+    # left out); transport adjoint against the Jacobian-chain gradient
+    m = min(1024, n)
+    sig = adj.start_sigma(b0[:, :m])
+    sub_b = bound[:, :m].contiguous()
+    sub_s = start_d[:m].contiguous()
+    sub_h = da.Hits(hits.surface[:, :m].contiguous(), hits.loc[:, :, :m].contiguous(), hits.n_hits[:m].contiguous())
+
+    def chi2_at(b):
+        return prop64.forward(det, field, b, sub_s, sub_h).chi2
+
+    def fd(rel):
+        out = torch.zeros((6, m), dtype=torch.float64, device=dev)
+        for i in range(5):
+            hstep = rel * (sub_b[i].abs() if i == 4 else sub_b[i].abs().clamp(min=1.0))
+            bp = sub_b.clone()
+            bp[i] += hstep
+            bm = sub_b.clone()
+            bm[i] -= hstep
+            out[i] = (chi2_at(bp) - chi2_at(bm)) / (2 * hstep)
+        return out.cpu().numpy()
+
+    def scaled(x, ref):
+        d = (np.abs(x[:5] - ref[:5]) * sig[:5]).max(axis=0)
+        return d / np.maximum((np.abs(ref[:5]) * sig[:5]).max(axis=0), 1e-300)
+
+    fd1, fd2 = fd(1e-6), fd(5e-7)
+    stable = scaled(fd2, fd1) <= 1e-4
+    ok = (status[:m] == 0) & stable
+    e_mixed = scaled(g[:, :m].cpu().numpy(), fd1)[ok]
+    e_fp64 = scaled(g64[:, :m].cpu().numpy(), fd1)[ok]
+    ft = prop.forward(det, field, sub_b, sub_s, sub_h, transport_gradient=True)
+    gt = prop.backward(det, field, sub_b, sub_s, sub_h, ft, model="transport")
+    e_chain = scaled(gt.cpu().numpy(), ft.grad_transport.cpu().numpy())[ft.status.cpu().numpy() == 0]
+    e_chain_fd = scaled(ft.grad_transport.cpu().numpy(), fd1)[ok]
+    all_ok = (status == 0) & (f64.status.cpu().numpy() == 0)
+    sig_all = adj.start_sigma(b0)
+    d_mf = (np.abs(g.cpu().numpy()[:5] - g64.cpu().numpy()[:5]) * sig_all[:5]).max(axis=0)
+    e_mf = (d_mf / np.maximum((np.abs(g64.cpu().numpy()[:5]) * sig_all[:5]).max(axis=0), 1e-300))[all_ok]
+    accuracy = {
+        "metric": "max_i |g_i - r_i| sigma_i / max_i |r_i| sigma_i (sigma: 10 um, 10 um, 1 mrad, 1 mrad, 1% q/p)",
+        "fd_sample_tracks": int(m),
+        "fd_unstable_tracks": int((~stable).sum()),
+        "exact_mixed_vs_fd": {"max": float(e_mixed.max()), "median": float(np.median(e_mixed))},
+        "exact_fp64_vs_fd": {"max": float(e_fp64.max()), "median": float(np.median(e_fp64))},
+        "transport_vs_jacobian_chain": {"max": float(e_chain.max()), "median": float(np.median(e_chain))},
+        "jacobian_chain_vs_fd": {"max": float(e_chain_fd.max()), "median": float(np.median(e_chain_fd))},
+        "mixed_vs_fp64_adjoint_all_tracks": {"max": float(e_mf.max()), "median": float(np.median(e_mf)),
+                                             "p9999": float(np.quantile(e_mf, 0.9999))},
+        "tape_status": np.bincount(status, minlength=4).tolist(),
+    }
+    print(f"[bench] adjoint accuracy {accuracy}", file=sys.stderr, flush=True)
+
+    # algorithmic bytes from the tape counters
+    st_steps, st_events = prop.tape_counters(f, n)
+    n_steps = st_steps.cpu().numpy().astype(np.int64)
+    n_events = st_events.cpu().numpy().astype(np.int64)
+    n_hits = hits.n_hits.cpu().numpy().astype(np.int64)
+    alg = adjoint_alg_bytes(n_steps, n_events, n_hits, acfg.checkpoint_every)
+    achieved = alg / ((fwd_ms + bwd_ms) * 1e-3) / 1e9
+    print(f"[bench] adjoint mean steps {n_steps.mean():.1f}, events {n_events.mean():.2f}, "
+          f"{alg / n:.0f} B/track", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        from tests import oracle_host
+
+        mc = min(args.cpu_sample, n)
+        threads = os.cpu_count() or 1
+        bs = np.ascontiguousarray(b0[:, :mc])
+        ss = np.ascontiguousarray(start[:mc].astype(np.uint32))
+        oracle_host.propagate_hits(det, field, cfg, bs[:, : max(1, mc // 10)], ss[: max(1, mc // 10)], 32, threads)
+        times = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            oracle_host.propagate_hits(det, field, cfg, bs, ss, 32, threads)
+            times.append(time.perf_counter() - t1)
+        cpu = {"value": mc / float(np.median(times)), "unit": "tracks/s", "cores": threads, "kind": "port",
+               "sample": f"first {mc} tracks: FP64 oracle propagation with covariance (Jacobian) transport "
+                         f"and the sensitive-surface parameters, the reference's way to the chain "
+                         f"gradient; OpenMP dynamic/1, median of 3 runs"}
+
+    line = {
+        "metric": "tracks/s of d chi2 / d(initial bound parameters) (forward + adjoint), toy detector, "
+                  "solenoid map",
+        "value": value,
+        "unit": "tracks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded numpy generator; hits from the FP64 forward pass on perturbed parameters)",
+        "config": {
+            "workload": f"toy detector 4brl+7edc, solenoid 5cm fp32 grid, {n} muons from the beam pipe, "
+                        "|eta|<2.5, pT 0.5-50 GeV log-uniform, hits smeared 10/50 um, checkpoint every 32 steps",
+            "tracks": n,
+            "kernels": "adjoint_record_kernel<mixed> + adjoint_backward_kernel<mixed>",
+            "parallelism": "1 GPU",
+        },
+        "forward_ms": fwd_ms,
+        "backward_ms": bwd_ms,
+        "fp64": {"value": n / wall64, "unit": "tracks/s", "forward_ms": fwd64, "backward_ms": bwd64},
+        "speedup_vs_fp64": wall64 / wall,
+        "accuracy": accuracy,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

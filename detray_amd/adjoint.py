@@ -215,6 +215,19 @@ class Chi2Propagator:
             C.c_void_p(grad.data_ptr()), C.c_void_p(_stream(bound.device))), "dtr_adjoint_backward")
         return grad
 
+    def tape_counters(self, fwd: ForwardResult, n: int):
+        """(steps, surface events) per track [n] int32 from the tape of a
+        forward pass (the counters that close the tape layout of
+        dtr_adjoint_tape_size)."""
+        a = self.adjoint
+        doubles = (2 * a.max_steps + 7 * (a.max_steps // a.checkpoint_every + 1) + 23 * a.max_events
+                   + 7 * a.checkpoint_every + 36)
+        import torch
+
+        words = fwd.tape[8 * n * doubles:8 * n * doubles + 4 * n * (5 * a.max_events + 3)].view(torch.int32)
+        base = 5 * a.max_events * n
+        return words[base:base + n], words[base + n:base + 2 * n]
+
     def record_hits(self, detector: Detector, field: MagneticField, bound_params, start_surface,
                     max_hits: int = 32) -> Hits:
         """True intersections: loc0/loc1 of every sensitive surface the
