@@ -362,7 +362,7 @@ def bench_adjoint(args) -> int:
     dev = torch.device("cuda:0")
     det = da.Detector.toy(4, 7)
     field = da.MagneticField.solenoid()
-    cfg = da.PropagationConfig(search_window=(3, 3))
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
     n = args.tracks or 262144
     seed = args.seed if args.seed is not None else 46
     b0, start = adj.beampipe_starts(n, seed=seed)

@@ -49,16 +49,20 @@ SIGMA_START_QOP_REL = 0.01
 @dataclass
 class AdjointConfig:
     """``dtr_adjoint_config``: tape capacity, checkpoint interval K,
-    precision ("fp64" / "mixed") and model ("exact": derivative of the
+    precision ("fp64" / "mixed"), model ("exact": derivative of the
     propagation; "transport": the derivative the reference's Jacobians
-    describe)."""
+    describe), launch (block size, waves per SIMD the kernels are built
+    for) and the track order of both passes ("eta_phi": grouped by
+    direction, "none": as given)."""
 
     max_steps: int = 512
     max_events: int = 64
     checkpoint_every: int = 32
     precision: str = "mixed"
     model: str = "exact"
-    block_size: int = 128
+    block_size: int = 64
+    waves_per_simd: int = 2
+    sort: str = "eta_phi"
     sigma_loc0: float = SIGMA_LOC0
     sigma_loc1: float = SIGMA_LOC1
 
@@ -70,6 +74,7 @@ class AdjointConfig:
         a.precision = {"fp64": capi.PRECISION_FP64, "mixed": capi.PRECISION_MIXED}[self.precision]
         a.model = {"exact": capi.ADJOINT_EXACT, "transport": capi.ADJOINT_TRANSPORT}[model or self.model]
         a.block_size = int(self.block_size)
+        a.waves_per_simd = int(self.waves_per_simd)
         a.sigma_loc0 = float(self.sigma_loc0)
         a.sigma_loc1 = float(self.sigma_loc1)
         return a
@@ -131,6 +136,19 @@ class ForwardResult:
     grad_transport: "object"   # [6, n] float64 or None
     status: "object"           # [n] int32 (dtr_tape_status)
     tape: "object"             # uint8 tensor
+    order: "object" = None     # [n] int32 track order of the pass, or None
+
+
+def direction_order(bound_params, eta_bins_per_unit: float = 20.0):
+    """Permutation (int32, device) grouping bound tracks by direction: by
+    pseudorapidity bins of width 1/eta_bins_per_unit, then by phi."""
+    import torch
+
+    theta = bound_params[3].clamp(1e-9, math.pi - 1e-9)
+    eta = -torch.log(torch.tan(0.5 * theta))
+    eta_bin = torch.round(eta * eta_bins_per_unit).to(torch.int64) + 4096
+    phi_q = ((bound_params[2] + math.pi) * (2 ** 20 / (2 * math.pi))).to(torch.int64).clamp(0, 2 ** 20 - 1)
+    return torch.argsort((eta_bin << 21) | phi_q).to(torch.int32)
 
 
 def _stream(device):
@@ -164,8 +182,9 @@ class Chi2Propagator:
             raise ValueError("start_surface out of range")
         return bound_params.contiguous(), start_surface.contiguous(), n
 
-    def _batch(self, bound, start, n, tape) -> capi.AdjointBatch:
-        return capi.AdjointBatch(n, 0, bound.data_ptr(), start.data_ptr(), tape.data_ptr(), int(tape.numel()))
+    def _batch(self, bound, start, n, tape, order) -> capi.AdjointBatch:
+        return capi.AdjointBatch(n, 0, bound.data_ptr(), start.data_ptr(), tape.data_ptr(), int(tape.numel()),
+                                 order.data_ptr() if order is not None else None)
 
     def forward(self, detector: Detector, field: MagneticField, bound_params, start_surface,
                 hits: Optional[Hits] = None, transport_gradient: bool = False,
@@ -185,18 +204,20 @@ class Chi2Propagator:
         chi2 = torch.zeros(n, dtype=torch.float64, device=dev)
         status = torch.zeros(n, dtype=torch.int32, device=dev)
         gt = torch.zeros((6, n), dtype=torch.float64, device=dev) if transport_gradient else None
+        order = direction_order(bound) if (self.adjoint.sort == "eta_phi" and n) else None
         lib = capi.load()
         hc = hits.to_c() if hits is not None else None
         rc = record.to_c() if record is not None else None
         capi.check(lib.dtr_adjoint_forward(detector.device(), field.device(), C.byref(self._c_config),
-                                           C.byref(self.adjoint.to_c()), C.byref(self._batch(bound, start, n, tape)),
+                                           C.byref(self.adjoint.to_c()),
+                                           C.byref(self._batch(bound, start, n, tape, order)),
                                            C.byref(hc) if hc is not None else None,
                                            C.byref(rc) if rc is not None else None,
                                            C.c_void_p(chi2.data_ptr()),
                                            C.c_void_p(gt.data_ptr()) if gt is not None else None,
                                            C.c_void_p(status.data_ptr()), C.c_void_p(_stream(dev))),
                    "dtr_adjoint_forward")
-        return ForwardResult(chi2, gt, status, tape)
+        return ForwardResult(chi2, gt, status, tape, order)
 
     def backward(self, detector: Detector, field: MagneticField, bound_params, start_surface,
                  hits: Optional[Hits], fwd: ForwardResult, model: Optional[str] = None):
@@ -211,7 +232,7 @@ class Chi2Propagator:
         hc = hits.to_c() if hits is not None else None
         capi.check(capi.load().dtr_adjoint_backward(
             detector.device(), field.device(), C.byref(self._c_config), C.byref(self.adjoint.to_c(model)),
-            C.byref(self._batch(bound, start, n, fwd.tape)), C.byref(hc) if hc is not None else None,
+            C.byref(self._batch(bound, start, n, fwd.tape, fwd.order)), C.byref(hc) if hc is not None else None,
             C.c_void_p(grad.data_ptr()), C.c_void_p(_stream(bound.device))), "dtr_adjoint_backward")
         return grad
 

@@ -213,6 +213,8 @@ class AdjointConfig(C.Structure):
         ("precision", C.c_uint32),
         ("model", C.c_uint32),
         ("block_size", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
+        ("pad", C.c_uint32),
         ("sigma_loc0", C.c_double),
         ("sigma_loc1", C.c_double),
     ]
@@ -236,6 +238,7 @@ class AdjointBatch(C.Structure):
         ("start_surface", C.c_void_p),
         ("tape", C.c_void_p),
         ("tape_bytes", C.c_size_t),
+        ("order", C.c_void_p),
     ]
 
 

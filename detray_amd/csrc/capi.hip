@@ -154,25 +154,28 @@ DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
 __global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
 
 // ------------------------------------------------------------------ adjoint --
-// Recording forward pass: one work-item per track, lane navigation with the
-// candidate cache in LDS.
-template <class P>
-__global__ void __launch_bounds__(256) adjoint_record_kernel(dtr::FastDetector det, dtr::DevField field,
-                                                             dtr::DevConfig cfg, dtr::AdjointIO io) {
+// Recording forward pass: one work-item per track (work-item k runs track
+// order[k] and owns tape column k), lane navigation with the candidate
+// cache in LDS.
+template <class P, int W>
+DTR_BATCHED_KERNEL adjoint_record_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
+                                         dtr::AdjointIO io, const uint32_t* __restrict__ order) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= io.n) return;
-    dtr::adjoint_record<P>(det, field, cfg, io, static_cast<uint32_t>(k), lds_cache(smem));
+    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+    dtr::adjoint_record<P>(det, field, cfg, io, track, static_cast<uint32_t>(k), lds_cache(smem));
 }
 
-// Backward pass: one work-item per track.
-template <class P>
-__global__ void __launch_bounds__(256) adjoint_backward_kernel(dtr::FastDetector det, dtr::DevField field,
-                                                               dtr::DevConfig cfg, dtr::AdjointIO io, int exact,
-                                                               double* __restrict__ grad) {
+// Backward pass: one work-item per track, in the forward pass's order.
+template <class P, int W>
+DTR_BATCHED_KERNEL adjoint_backward_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
+                                           dtr::AdjointIO io, const uint32_t* __restrict__ order, int exact,
+                                           double* __restrict__ grad) {
     const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= io.n) return;
-    dtr::adjoint_backward<P>(det, field, cfg, io, exact != 0, static_cast<uint32_t>(k), grad);
+    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+    dtr::adjoint_backward<P>(det, field, cfg, io, exact != 0, track, static_cast<uint32_t>(k), grad);
 }
 
 // ---------------------------------------------------------------- schedule --
@@ -322,6 +325,7 @@ int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dt
     if (a->precision != DTR_PRECISION_FP64 && a->precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->model != DTR_ADJOINT_EXACT && a->model != DTR_ADJOINT_TRANSPORT) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->block_size != 0u && (a->block_size % 64u != 0u || a->block_size > 256u)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->waves_per_simd > 32u) return DTR_ERROR_INVALID_ARGUMENT;
     if (!(a->sigma_loc0 > 0.0) || !(a->sigma_loc1 > 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
@@ -625,7 +629,8 @@ void dtr_default_adjoint(dtr_adjoint_config* a) {
     a->checkpoint_every = 32u;
     a->precision = DTR_PRECISION_MIXED;
     a->model = DTR_ADJOINT_EXACT;
-    a->block_size = 128u;
+    a->block_size = 64u;
+    a->waves_per_simd = 2u;
     a->sigma_loc0 = 0.010;
     a->sigma_loc1 = 0.050;
 }
@@ -644,6 +649,10 @@ int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const d
         return DTR_ERROR_INVALID_ARGUMENT;
     dtr::DevConfig dc = make_dev_config(c);
     dc.max_surfaces = 0u;  // the event list on the tape replaces the surface sequencer
+    // chi2 and its gradient do not depend on the covariance: it is only
+    // carried when the navigation reads it (noise estimation) or the
+    // Jacobian chain is asked for
+    if (!grad_transport && !c->estimate_scattering_noise) dc.do_cov = 0u;
     dtr::AdjointIO io = make_adjoint_io(a, b, hits);
     if (record) io.record = dtr::HitsOut{record->max_hits, record->surface, record->loc, record->n_hits};
     io.chi2 = chi2;
@@ -653,12 +662,22 @@ int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const d
     const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
     const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (a->precision == DTR_PRECISION_FP64)
-        hipLaunchKernelGGL(adjoint_record_kernel<dtr::PolicyFP64>, dim3(grid), dim3(block), shmem, st, det->fast,
-                           field->dev, dc, io);
-    else
-        hipLaunchKernelGGL(adjoint_record_kernel<dtr::PolicyMixed>, dim3(grid), dim3(block), shmem, st, det->fast,
-                           field->dev, dc, io);
+    const bool w2 = a->waves_per_simd != 1u;
+    if (a->precision == DTR_PRECISION_FP64) {
+        if (w2)
+            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), shmem, st,
+                               det->fast, field->dev, dc, io, b->order);
+        else
+            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyFP64, 1>), dim3(grid), dim3(block), shmem, st,
+                               det->fast, field->dev, dc, io, b->order);
+    } else {
+        if (w2)
+            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), shmem, st,
+                               det->fast, field->dev, dc, io, b->order);
+        else
+            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyMixed, 1>), dim3(grid), dim3(block), shmem, st,
+                               det->fast, field->dev, dc, io, b->order);
+    }
     return to_status(hipGetLastError());
 }
 
@@ -675,12 +694,22 @@ int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const 
     const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int exact = a->model == DTR_ADJOINT_EXACT ? 1 : 0;
-    if (a->precision == DTR_PRECISION_FP64)
-        hipLaunchKernelGGL(adjoint_backward_kernel<dtr::PolicyFP64>, dim3(grid), dim3(block), 0, st, det->fast,
-                           field->dev, dc, io, exact, grad);
-    else
-        hipLaunchKernelGGL(adjoint_backward_kernel<dtr::PolicyMixed>, dim3(grid), dim3(block), 0, st, det->fast,
-                           field->dev, dc, io, exact, grad);
+    const bool w2 = a->waves_per_simd != 1u;
+    if (a->precision == DTR_PRECISION_FP64) {
+        if (w2)
+            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), 0, st,
+                               det->fast, field->dev, dc, io, b->order, exact, grad);
+        else
+            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyFP64, 1>), dim3(grid), dim3(block), 0, st,
+                               det->fast, field->dev, dc, io, b->order, exact, grad);
+    } else {
+        if (w2)
+            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), 0, st,
+                               det->fast, field->dev, dc, io, b->order, exact, grad);
+        else
+            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyMixed, 1>), dim3(grid), dim3(block), 0, st,
+                               det->fast, field->dev, dc, io, b->order, exact, grad);
+    }
     return to_status(hipGetLastError());
 }
 

@@ -38,7 +38,10 @@
 namespace dtr {
 
 // ------------------------------------------------------------------- tape --
-// Caller-allocated, structure of arrays over the n tracks of a batch.
+// Caller-allocated, structure of arrays over the n tracks of a batch, one
+// column per work-item: with a track order, column k holds track order[k],
+// so that the lanes of a wave, which run neighbouring tracks of the order,
+// read and write neighbouring columns.
 struct Tape {
     uint64_t n;
     uint32_t max_steps, max_events, every, n_ckpt;
@@ -491,7 +494,8 @@ DTR_AI void tape_get_state(const double* base, uint64_t slot, uint64_t n, uint32
 // tape, the Jacobian chain, chi2 and its Jacobian-chain gradient.
 template <class P>
 DTR_AI void record_update(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
-                          FastTrack<P>& tk, CacheView c, uint32_t gid, uint32_t& n_ev, uint32_t& next_hit,
+                          FastTrack<P>& tk, CacheView c, uint32_t gid, uint32_t col, uint32_t& n_ev,
+                          uint32_t& next_hit,
                           uint32_t& n_rec, double& chi2, uint32_t& status, uint32_t& land) {
     using S = typename P::S;
     FastEngine<P, FastTrack<P>, true> e{det, field, cfg, tk, c};
@@ -505,7 +509,7 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
     const uint64_t ev = keep ? n_ev : 0u;
     if (keep) {
         const T3<S> k = e.dtds();
-        double* q = tp.ev_pre + ev * 10u * N + gid;
+        double* q = tp.ev_pre + ev * 10u * N + col;
         q[0 * N] = tk.pos.x;
         q[1 * N] = tk.pos.y;
         q[2 * N] = tk.pos.z;
@@ -518,11 +522,14 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
         q[9 * N] = static_cast<double>(k.z);
     }
     double jf[36];
-    e.jf_out = jf;
+    e.jf_out = io.grad_transport ? jf : nullptr;
     bool from_free = false;
     if (tk.up_sf != kInvalid) {
         if (tk.up_sf != dest) {
-            e.transport_covariance(dest);
+            // the covariance is not an output of the adjoint; it is
+            // transported when the navigation reads it (noise estimation)
+            // or the Jacobian chain is asked for
+            if (cfg.do_cov || io.grad_transport) e.transport_covariance(dest);
             if (fabs(tk.path_length) > 0.0) e.reset_J();
             e.free_to_bound(dest, tk.bound);
             tk.up_sf = dest;
@@ -535,20 +542,20 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
     }
     if (from_free && io.grad_transport) {
         // Jcum <- Jf Jcum
-        double* J = tp.jcum + gid;
+        double* J = tp.jcum + col;
         for (int j = 0; j < 6; ++j) {
-            double col[6], nc[6];
-            for (int m = 0; m < 6; ++m) col[m] = J[(m * 6 + j) * N];
+            double jc[6], nc[6];
+            for (int m = 0; m < 6; ++m) jc[m] = J[(m * 6 + j) * N];
             for (int r = 0; r < 6; ++r) {
                 double s = 0.0;
-                for (int m = 0; m < 6; ++m) s += jf[r * 6 + m] * col[m];
+                for (int m = 0; m < 6; ++m) s += jf[r * 6 + m] * jc[m];
                 nc[r] = s;
             }
             for (int r = 0; r < 6; ++r) J[(r * 6 + j) * N] = nc[r];
         }
     }
     if (keep)
-        for (int k = 0; k < 6; ++k) tp.ev_bound[(ev * 6u + k) * N + gid] = tk.bound[k];
+        for (int k = 0; k < 6; ++k) tp.ev_bound[(ev * 6u + k) * N + col] = tk.bound[k];
     int32_t hit = -1;
     if (sens && io.hits.surface) {
         const uint32_t nh = io.hits.n_hits[gid] < io.hits.max_hits ? io.hits.n_hits[gid] : io.hits.max_hits;
@@ -568,7 +575,7 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
                 const double w0 = 2.0 * io.iv0 * r0, w1 = 2.0 * io.iv1 * r1;
                 for (int k = 0; k < 6; ++k)
                     io.grad_transport[k * N + gid] +=
-                        tp.jcum[(0 * 6 + k) * N + gid] * w0 + tp.jcum[(1 * 6 + k) * N + gid] * w1;
+                        tp.jcum[(0 * 6 + k) * N + col] * w0 + tp.jcum[(1 * 6 + k) * N + col] * w1;
             }
         }
     }
@@ -582,12 +589,12 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
     if (mat) e.material_interaction(dest);
     if (tk.stopped) status = kTapeUnsupported;
     if (keep) {
-        tape_put_state(tp.ev_post, ev * 7u, N, gid, tk.pos, tk.dir, tk.qop);
-        tp.ev_step[ev * N + gid] = tk.n_steps;
-        tp.ev_sf[ev * N + gid] = dest;
-        tp.ev_hit[ev * N + gid] = hit;
-        tp.ev_flags[ev * N + gid] = (from_free ? kEvFromFree : 0u) | (mat ? kEvMaterial : 0u);
-        tp.ev_land[ev * N + gid] = land;
+        tape_put_state(tp.ev_post, ev * 7u, N, col, tk.pos, tk.dir, tk.qop);
+        tp.ev_step[ev * N + col] = tk.n_steps;
+        tp.ev_sf[ev * N + col] = dest;
+        tp.ev_hit[ev * N + col] = hit;
+        tp.ev_flags[ev * N + col] = (from_free ? kEvFromFree : 0u) | (mat ? kEvMaterial : 0u);
+        tp.ev_land[ev * N + col] = land;
         n_ev++;
     }
     land = kInvalid;  // later events before the next step were reached without moving
@@ -598,13 +605,13 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
 // the event list.
 template <class P>
 DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const DevConfig& cfg,
-                           const AdjointIO& io, uint32_t gid, CacheView c) {
+                           const AdjointIO& io, uint32_t gid, uint32_t col, CacheView c) {
     const Tape& tp = io.tape;
     const uint64_t N = io.n;
     const TrackInputs in{nullptr, nullptr, io.bound0, io.start_sf, N};
     if (io.grad_transport) {
         for (int k = 0; k < 6; ++k) io.grad_transport[k * N + gid] = 0.0;
-        for (int k = 0; k < 36; ++k) tp.jcum[k * N + gid] = (k % 7 == 0) ? 1.0 : 0.0;
+        for (int k = 0; k < 36; ++k) tp.jcum[k * N + col] = (k % 7 == 0) ? 1.0 : 0.0;
     }
     Lane<P> lane;
     lane.start(det, field, cfg, in, c, gid);
@@ -623,7 +630,7 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
             tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
         }
         if (cfg.use_material)
-            record_update<P>(det, field, cfg, io, tk, c, gid, n_ev, next_hit, n_rec, chi2, status, land);
+            record_update<P>(det, field, cfg, io, tk, c, gid, col, n_ev, next_hit, n_rec, chi2, status, land);
         if (tk.stopped && tk.status > kUnknown) {
             tk.status = kAbort;
             tk.heartbeat = false;
@@ -635,7 +642,7 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
         // step half with the step sizes and checkpoints on the tape
         const uint32_t s0 = tk.n_steps;
         if (s0 % tp.every == 0u && s0 / tp.every < tp.n_ckpt)
-            tape_put_state(tp.ckpt, static_cast<uint64_t>(s0 / tp.every) * 7u, N, gid, tk.pos, tk.dir, tk.qop);
+            tape_put_state(tp.ckpt, static_cast<uint64_t>(s0 / tp.every) * 7u, N, col, tk.pos, tk.dir, tk.qop);
         // a step whose length is the navigator's distance lands on the
         // navigator's target surface
         const double dist = c.p(tk.next);
@@ -644,8 +651,8 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
         if (tk.n_steps != s0) {
             land = (tk.step_size == dist) ? target : kInvalid;
             if (s0 < tp.max_steps) {
-                tp.h[(2ull * s0 + 0u) * N + gid] = tk.h_stage;
-                tp.h[(2ull * s0 + 1u) * N + gid] = tk.step_size;
+                tp.h[(2ull * s0 + 0u) * N + col] = tk.h_stage;
+                tp.h[(2ull * s0 + 1u) * N + col] = tk.step_size;
             } else {
                 status = kTapeSteps;
             }
@@ -653,9 +660,9 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
         lane.navigate_alone(det, cfg, c);
         lane.nav_end();
     }
-    tp.n_steps[gid] = tk.n_steps;
-    tp.n_events[gid] = n_ev;
-    tp.status[gid] = status;
+    tp.n_steps[col] = tk.n_steps;
+    tp.n_events[col] = n_ev;
+    tp.status[col] = status;
     if (io.status) io.status[gid] = status;
     if (io.chi2) io.chi2[gid] = chi2;
     if (io.record.surface) io.record.n_hits[gid] = n_rec;
@@ -696,18 +703,18 @@ DTR_HD void landing_contrib(V3 n, double hbar, double h, V3 dc, V3& pc_out, V3& 
 // the start event (its bound parameters are the initial ones), whose bound
 // adjoint is written to grad.
 DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
-                          bool exact, uint32_t gid, uint64_t ev, double mass, double charge, V3& pb, V3& db,
-                          double& qb, double grad[6], Landing& ld) {
+                          bool exact, uint32_t gid, uint32_t col, uint64_t ev, double mass, double charge, V3& pb,
+                          V3& db, double& qb, double grad[6], Landing& ld) {
     const Tape& tp = io.tape;
     const uint64_t N = io.n;
-    const uint32_t sfi = tp.ev_sf[ev * N + gid];
-    const uint32_t flags = tp.ev_flags[ev * N + gid];
-    const int32_t hit = tp.ev_hit[ev * N + gid];
+    const uint32_t sfi = tp.ev_sf[ev * N + col];
+    const uint32_t flags = tp.ev_flags[ev * N + col];
+    const int32_t hit = tp.ev_hit[ev * N + col];
     const NavSurface& rec = det.sf[sfi];
     double b[6], bp[6], bb[6];
-    for (int k = 0; k < 6; ++k) b[k] = tp.ev_bound[(ev * 6u + k) * N + gid];
+    for (int k = 0; k < 6; ++k) b[k] = tp.ev_bound[(ev * 6u + k) * N + col];
     for (int k = 0; k < 6; ++k) bp[k] = b[k];
-    bp[4] = tp.ev_post[(ev * 7u + 6u) * N + gid];
+    bp[4] = tp.ev_post[(ev * 7u + 6u) * N + col];
     const bool mat = (flags & kEvMaterial) != 0u;
     const bool restart = mat || !exact;  // the state after the event is B2F(bound)
     V3 cpb{0.0, 0.0, 0.0}, cdb{0.0, 0.0, 0.0};
@@ -743,7 +750,7 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
         for (int k = 0; k < 6; ++k) grad[k] = bb[k] + cb[k];
         return false;
     }
-    const double* q = tp.ev_pre + ev * 10u * N + gid;
+    const double* q = tp.ev_pre + ev * 10u * N + col;
     const V3 p{q[0 * N], q[1 * N], q[2 * N]}, d{q[3 * N], q[4 * N], q[5 * N]};
     V3 k{q[7 * N], q[8 * N], q[9 * N]};
     if (exact) {
@@ -755,7 +762,7 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
     // projection is plain and the step length carries the dependence
     // (Landing); a cylinder target gets the path correction onto it.
     // Transport: the reference's path correction onto this surface.
-    const uint32_t land = tp.ev_land[ev * N + gid];
+    const uint32_t land = tp.ev_land[ev * N + col];
     const bool by_length = exact && land != kInvalid && det.sf[land].shape != DTR_CONCENTRIC_CYLINDER2D;
     const NavSurface* lrec = exact ? (land != kInvalid && !by_length ? &det.sf[land] : nullptr) : &rec;
     V3 fpb, fdb;
@@ -779,7 +786,8 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
 // it belongs to (after the event: before its adjoint; before the event:
 // after it, through the bound-to-free map for the start event).
 DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg,
-                               const AdjointIO& io, bool exact, uint32_t gid, int32_t e, double mass, double charge,
+                               const AdjointIO& io, bool exact, uint32_t gid, uint32_t col, int32_t e, double mass,
+                               double charge,
                                V3& pb, V3& db, double& qb, double grad[6], Landing& ld, Pending& pend,
                                const double b0[6]) {
     if (pend.ev == e && pend.post) {
@@ -787,8 +795,8 @@ DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, c
         db = add(db, pend.dc);
         pend.ev = -1;
     }
-    const bool more = event_adjoint(det, field, cfg, io, exact, gid, static_cast<uint64_t>(e), mass, charge, pb, db,
-                                    qb, grad, ld);
+    const bool more = event_adjoint(det, field, cfg, io, exact, gid, col, static_cast<uint64_t>(e), mass, charge, pb,
+                                    db, qb, grad, ld);
     if (pend.ev == e && !pend.post) {
         if (more) {
             pb = add(pb, pend.pc);
@@ -808,18 +816,18 @@ DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, c
 
 template <class P>
 DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, const DevConfig& cfg,
-                             const AdjointIO& io, bool exact, uint32_t gid, double* grad_out) {
+                             const AdjointIO& io, bool exact, uint32_t gid, uint32_t col, double* grad_out) {
     using S = typename P::S;
     const Tape& tp = io.tape;
     const uint64_t N = io.n;
     double grad[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (tp.status[gid] != kTapeOk) {
+    if (tp.status[col] != kTapeOk) {
         const double nan = __builtin_nan("");
         for (int k = 0; k < 6; ++k) grad_out[k * N + gid] = nan;
         return;
     }
-    const uint32_t n_steps = tp.n_steps[gid];
-    const int32_t n_ev = static_cast<int32_t>(tp.n_events[gid]);
+    const uint32_t n_steps = tp.n_steps[col];
+    const int32_t n_ev = static_cast<int32_t>(tp.n_events[col]);
     double b0[6];
     for (int k = 0; k < 6; ++k) b0[k] = io.bound0[k * N + gid];
     // muon hypothesis, charge-conjugated to the track (as Lane::start_begin)
@@ -839,37 +847,37 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
         // replay the segment from its checkpoint: the state before each step
         V3 p, d;
         double qop;
-        tape_get_state(tp.ckpt, static_cast<uint64_t>(sgi) * 7u, N, gid, p, d, qop);
+        tape_get_state(tp.ckpt, static_cast<uint64_t>(sgi) * 7u, N, col, p, d, qop);
         int32_t ef = e;
-        while (ef >= 0 && tp.ev_step[static_cast<uint64_t>(ef) * N + gid] > j0) ef--;
+        while (ef >= 0 && tp.ev_step[static_cast<uint64_t>(ef) * N + col] > j0) ef--;
         ef++;
         for (uint32_t j = j0 + 1u; j <= j1; ++j) {
-            tape_put_state(tp.seg, static_cast<uint64_t>(j - j0 - 1u) * 7u, N, gid, p, d, qop);
+            tape_put_state(tp.seg, static_cast<uint64_t>(j - j0 - 1u) * 7u, N, col, p, d, qop);
             if (j == j1) break;
-            const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + gid];
-            const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + gid];
+            const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + col];
+            const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + col];
             const Stages<S> st = rk_stages<S>(field, p, d, qop, hs);
             V3 pn, dn;
             rk_advance(st, p, d, ha, pn, dn);
             p = pn;
             d = dn;
-            while (ef < n_ev && tp.ev_step[static_cast<uint64_t>(ef) * N + gid] == j) {
-                tape_get_state(tp.ev_post, static_cast<uint64_t>(ef) * 7u, N, gid, p, d, qop);
+            while (ef < n_ev && tp.ev_step[static_cast<uint64_t>(ef) * N + col] == j) {
+                tape_get_state(tp.ev_post, static_cast<uint64_t>(ef) * 7u, N, col, p, d, qop);
                 ef++;
             }
         }
         // back through the segment: the events after step j, then step j
         for (uint32_t j = j1; j > j0; --j) {
-            while (e >= 0 && tp.ev_step[static_cast<uint64_t>(e) * N + gid] == j) {
-                event_step_adjoint(det, field, cfg, io, exact, gid, e, mass, charge, pb, db, qb, grad, ld, pend,
-                                   b0);
+            while (e >= 0 && tp.ev_step[static_cast<uint64_t>(e) * N + col] == j) {
+                event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, ld,
+                                   pend, b0);
                 e--;
             }
             V3 sp, sd;
             double sq;
-            tape_get_state(tp.seg, static_cast<uint64_t>(j - j0 - 1u) * 7u, N, gid, sp, sd, sq);
-            const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + gid];
-            const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + gid];
+            tape_get_state(tp.seg, static_cast<uint64_t>(j - j0 - 1u) * 7u, N, col, sp, sd, sq);
+            const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + col];
+            const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + col];
             rk_vjp<S>(field, sp, sd, sq, hs, ha, exact, pb, db, qb);
             if (ld.active) {
                 // the state the step length was computed from: the step's
@@ -888,15 +896,15 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
                     pb = add(pb, cp);
                     db = add(db, cd);
                 }
-                for (int32_t e2 = e; !found && e2 >= 0 && tp.ev_step[static_cast<uint64_t>(e2) * N + gid] == j - 1u;
+                for (int32_t e2 = e; !found && e2 >= 0 && tp.ev_step[static_cast<uint64_t>(e2) * N + col] == j - 1u;
                      --e2) {
                     for (int w = 0; w < 2 && !found; ++w) {
                         V3 p, d;
                         double qq;
                         if (w == 0) {
-                            tape_get_state(tp.ev_post, static_cast<uint64_t>(e2) * 7u, N, gid, p, d, qq);
+                            tape_get_state(tp.ev_post, static_cast<uint64_t>(e2) * 7u, N, col, p, d, qq);
                         } else {
-                            const double* q = tp.ev_pre + static_cast<uint64_t>(e2) * 10u * N + gid;
+                            const double* q = tp.ev_pre + static_cast<uint64_t>(e2) * 10u * N + col;
                             p = {q[0 * N], q[1 * N], q[2 * N]};
                             d = {q[3 * N], q[4 * N], q[5 * N]};
                         }
@@ -919,7 +927,8 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
         }
     }
     while (e >= 0 && open) {
-        open = event_step_adjoint(det, field, cfg, io, exact, gid, e, mass, charge, pb, db, qb, grad, ld, pend, b0);
+        open = event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, ld, pend,
+                                  b0);
         e--;
     }
     if (open) {

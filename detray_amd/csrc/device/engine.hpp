@@ -1175,6 +1175,7 @@ struct FastEngine {
             Jf[3][cc] = Ba[3] * Md[0][cc] + Ba[4] * Md[1][cc] + Ba[5] * Md[2][cc];
         }
         if constexpr (kJacOut) {
+            if (jf_out)
             for (int r = 0; r < 6; ++r)
                 for (int cc = 0; cc < 6; ++cc) {
                     double v = 0.0;

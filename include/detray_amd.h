@@ -380,7 +380,9 @@ typedef struct dtr_adjoint_config {
     uint32_t checkpoint_every; /* K: the free state is stored every K steps (32) */
     uint32_t precision;        /* dtr_precision of both passes (MIXED) */
     uint32_t model;            /* dtr_adjoint_model of the backward pass (EXACT) */
-    uint32_t block_size;       /* work-items per block, 64..256, multiple of 64 (128) */
+    uint32_t block_size;       /* work-items per block, 64..256, multiple of 64 (64) */
+    uint32_t waves_per_simd;   /* register budget the kernels are built for: 1 or 2 (2) */
+    uint32_t pad;
     double sigma_loc0;         /* hit resolution in loc0, mm (0.010) */
     double sigma_loc1;         /* hit resolution in loc1, mm (0.050) */
 } dtr_adjoint_config;
@@ -398,7 +400,9 @@ typedef struct dtr_hits {
 /* Tracks of a differentiated batch, DEVICE memory: bound start parameters
  * [6][n] on start_surface [n] (propagator::state(bound_track_parameters),
  * propagator.hpp:121-162), and the tape of dtr_adjoint_tape_size bytes that
- * the forward pass writes and the backward pass reads. */
+ * the forward pass writes and the backward pass reads.  order: optional
+ * permutation [n] (work-item k runs track order[k]); the backward pass
+ * must get the order of the forward pass.  Results stay in track order. */
 typedef struct dtr_adjoint_batch {
     uint32_t n;
     uint32_t pad;
@@ -406,6 +410,7 @@ typedef struct dtr_adjoint_batch {
     const uint32_t* start_surface;
     void* tape;
     size_t tape_bytes;
+    const uint32_t* order;
 } dtr_adjoint_batch;
 
 /* Per-track tape status written by dtr_adjoint_forward. */

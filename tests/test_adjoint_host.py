@@ -21,7 +21,7 @@ def test_default_adjoint_config():
     capi.load().dtr_default_adjoint(C.byref(a))
     assert (a.max_steps, a.max_events, a.checkpoint_every) == (512, 64, 32)
     assert a.precision == capi.PRECISION_MIXED and a.model == capi.ADJOINT_EXACT
-    assert a.block_size == 128
+    assert a.block_size == 64 and a.waves_per_simd == 2
     assert a.sigma_loc0 == 0.010 and a.sigma_loc1 == 0.050
     c = da.AdjointConfig().to_c()
     for f, _ in capi.AdjointConfig._fields_:

@@ -75,9 +75,11 @@ def oracle_fd(workload):
     return fd, stable
 
 
-def run(w, precision, model="exact", every=32, max_steps=512, max_events=64, transport=False):
+def run(w, precision, model="exact", every=32, max_steps=512, max_events=64, transport=False, sort="eta_phi",
+        waves=2, block=128):
     prop = da.Chi2Propagator(CONFIG, da.AdjointConfig(max_steps=max_steps, max_events=max_events,
-                                                      checkpoint_every=every, precision=precision, model=model))
+                                                      checkpoint_every=every, precision=precision, model=model,
+                                                      sort=sort, waves_per_simd=waves, block_size=block))
     bound = torch.from_numpy(w["b0"]).cuda()
     start = torch.from_numpy(w["start"]).cuda()
     fwd = prop.forward(w["det"], w["field"], bound, start, w["hits"], transport_gradient=transport)
@@ -159,6 +161,35 @@ def test_checkpoint_interval_does_not_change_the_gradient(workload):
         r = run(w, "mixed", every=every)
         np.testing.assert_array_equal(r["status"], ref["status"])
         np.testing.assert_array_equal(r["grad"], ref["grad"])
+
+
+def test_launch_and_order_do_not_change_the_gradient(workload):
+    """Track order (tape columns follow the order), register budget and
+    block size only change the schedule: bit-identical results."""
+    w = workload
+    ref = run(w, "mixed")
+    for kw in (dict(sort="none"), dict(waves=1), dict(block=64), dict(block=256, sort="none")):
+        r = run(w, "mixed", **kw)
+        np.testing.assert_array_equal(r["chi2"], ref["chi2"])
+        np.testing.assert_array_equal(r["grad"], ref["grad"])
+
+
+def test_covariance_is_skipped_without_noise_estimation(workload):
+    """Without noise estimation the forward pass does not carry the
+    covariance (unless the Jacobian chain is asked for); the trajectory, chi2
+    and gradient are the same either way."""
+    w = workload
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+    prop = da.Chi2Propagator(cfg, da.AdjointConfig(precision="mixed"))
+    bound = torch.from_numpy(w["b0"]).cuda()
+    start = torch.from_numpy(w["start"]).cuda()
+    f0 = prop.forward(w["det"], w["field"], bound, start, w["hits"])
+    g0 = prop.backward(w["det"], w["field"], bound, start, w["hits"], f0)
+    f1 = prop.forward(w["det"], w["field"], bound, start, w["hits"], transport_gradient=True)
+    g1 = prop.backward(w["det"], w["field"], bound, start, w["hits"], f1)
+    torch.cuda.synchronize()
+    assert torch.equal(f0.chi2, f1.chi2)
+    assert torch.equal(g0, g1)
 
 
 def test_tape_overflow_is_reported(workload):
