@@ -208,7 +208,48 @@ __global__ void sort_count_kernel(const double* __restrict__ fp, uint32_t n, uin
     atomicAdd(&count[direction_cell(fp, n, i, eta_bins_per_unit, phi_bins, eta_max)], 1u);
 }
 
-// exclusive scan of the cell counts in place, one block of 1024 threads
+// Exclusive scan of the cell counts in place, in two levels: blocks of 1024
+// threads scan chunks of kScanChunk cells and write the chunk totals, one
+// block scans the totals, and the chunk offsets are added back.
+constexpr uint32_t kScanItems = 8u;
+constexpr uint32_t kScanChunk = 1024u * kScanItems;
+
+__global__ void __launch_bounds__(1024) sort_scan_chunk_kernel(uint32_t* __restrict__ count, uint32_t cells,
+                                                               uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanChunk + static_cast<uint64_t>(t) * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < cells) ? count[base + i] : 0u;
+        sum += v[i];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1u; d < 1024u; d <<= 1) {
+        const uint32_t x = (t >= d) ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanItems; ++i) {
+        if (base + i < cells) count[base + i] = run;
+        run += v[i];
+    }
+    if (t == 1023u) total[blockIdx.x] = part[1023];
+}
+
+__global__ void sort_scan_add_kernel(uint32_t* __restrict__ count, uint32_t cells,
+                                     const uint32_t* __restrict__ offset) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cells) count[i] += offset[i / kScanChunk];
+}
+
+// exclusive scan in place of a short array, one block of 1024 threads
 __global__ void __launch_bounds__(1024) sort_scan_kernel(uint32_t* __restrict__ count, uint32_t cells) {
     __shared__ uint32_t part[1024];
     const uint32_t t = threadIdx.x;
@@ -241,9 +282,17 @@ __global__ void sort_scatter_kernel(const double* __restrict__ fp, uint32_t n, u
     order[atomicAdd(&offset[cell], 1u)] = i;
 }
 
+// cells of the sort (at most 2^26; the configuration limits keep the
+// product in range)
 uint32_t sort_cells(const dtr_sort_config* c) {
-    const uint32_t n_eta = static_cast<uint32_t>(2.f * c->eta_max * static_cast<float>(c->eta_bins_per_unit)) + 1u;
-    return n_eta * c->phi_bins;
+    const uint64_t n_eta = static_cast<uint64_t>(2.f * c->eta_max * static_cast<float>(c->eta_bins_per_unit)) + 1u;
+    const uint64_t cells = n_eta * c->phi_bins;
+    return cells > (1ull << 26) ? 0u : static_cast<uint32_t>(cells);
+}
+
+// words of workspace: the cell counts, then the chunk totals of the scan
+size_t sort_words(uint32_t cells) {
+    return static_cast<size_t>(cells) + (cells + kScanChunk - 1u) / kScanChunk;
 }
 
 dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
@@ -568,8 +617,8 @@ void dtr_default_sort(dtr_sort_config* c) {
 }
 
 size_t dtr_sort_workspace_size(const dtr_sort_config* c) {
-    if (!c) return 0u;
-    return static_cast<size_t>(sort_cells(c)) * sizeof(uint32_t);
+    if (!c || c->eta_bins_per_unit == 0u || c->phi_bins == 0u || !(c->eta_max > 0.f)) return 0u;
+    return sort_words(sort_cells(c)) * sizeof(uint32_t);
 }
 
 int dtr_sort_tracks(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t* order, void* workspace,
@@ -580,13 +629,20 @@ int dtr_sort_tracks(const dtr_sort_config* c, const double* fp, uint32_t n, uint
     if (n == 0u) return DTR_SUCCESS;
     if (!fp || !order || !workspace) return DTR_ERROR_INVALID_ARGUMENT;
     const uint32_t cells = sort_cells(c);
-    if (workspace_bytes < static_cast<size_t>(cells) * sizeof(uint32_t)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cells == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (workspace_bytes < sort_words(cells) * sizeof(uint32_t)) return DTR_ERROR_INVALID_ARGUMENT;
     uint32_t* count = static_cast<uint32_t*>(workspace);
+    uint32_t* total = count + cells;
+    const uint32_t chunks = static_cast<uint32_t>((cells + kScanChunk - 1u) / kScanChunk);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(sort_clear_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells);
     hipLaunchKernelGGL(sort_count_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count,
                        c->eta_bins_per_unit, c->phi_bins, c->eta_max);
-    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, count, cells);
+    hipLaunchKernelGGL(sort_scan_chunk_kernel, dim3(chunks), dim3(1024), 0, st, count, cells, total);
+    if (chunks > 1u) {
+        hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, total, chunks);
+        hipLaunchKernelGGL(sort_scan_add_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells, total);
+    }
     hipLaunchKernelGGL(sort_scatter_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count, order,
                        c->eta_bins_per_unit, c->phi_bins, c->eta_max);
     return to_status(hipGetLastError());

@@ -20,11 +20,12 @@ from typing import Iterable, Optional
 
 from .propagator import LaunchConfig, Propagator
 
-SORT_KEYS = ("none", "cells", "cells_coarse", "cells_fine", "eta_phi", "eta_phi_coarse", "phi_eta",
-             "eta_phi_pt")
+SORT_KEYS = ("none", "cells", "cells_coarse", "cells_fine", "cells_phi", "cells_phi_fine", "eta_phi",
+             "eta_phi_coarse", "phi_eta", "eta_phi_pt")
 
 # direction-cell granularities of the native sort: (eta bins per unit, phi bins)
-CELL_KEYS = {"cells": (20, 128), "cells_coarse": (10, 64), "cells_fine": (40, 256)}
+CELL_KEYS = {"cells": (20, 128), "cells_coarse": (10, 64), "cells_fine": (40, 256), "cells_phi": (10, 8192),
+             "cells_phi_fine": (10, 32768)}
 
 
 class TrackSorter:
@@ -109,7 +110,7 @@ def sort_order(tracks, key: str = "cells", eta_bins_per_unit: float = 20.0):
 def candidate_launches(precision: str = "mixed") -> Iterable[tuple]:
     """The autotuner's search space: (schedule, navigation, block, waves,
     claim, min_idle, sort key)."""
-    for key in ("cells", "cells_coarse", "cells_fine", "eta_phi", "none"):
+    for key in ("cells", "cells_coarse", "cells_fine", "cells_phi", "cells_phi_fine", "eta_phi", "none"):
         for block in (64, 128, 256):
             for waves in (1, 2):
                 yield ("grid", "lane", block, waves, 64, 1, key)

@@ -332,10 +332,11 @@ DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
 /* Track schedule: a permutation of the batch that groups tracks by
  * direction cell (pseudorapidity bins of width 1/eta_bins_per_unit over
  * |eta| < eta_max, phi_bins slices of phi), for dtr_launch_config.order.
- * A counting sort on the device: three kernels on `stream`, no allocation
- * and no memset, so it can be captured in a HIP graph together with the
- * propagation.  The order of tracks inside one cell is not specified (the
- * results of a track do not depend on it). */
+ * A counting sort on the device: kernels on `stream` (clear, count, a
+ * two-level scan, scatter), no allocation and no memset, so it can be
+ * captured in a HIP graph together with the propagation.  At most 2^26
+ * cells.  The order of tracks inside one cell is not specified (the results
+ * of a track do not depend on it). */
 typedef struct dtr_sort_config {
     uint32_t eta_bins_per_unit; /* default 20 */
     uint32_t phi_bins;          /* default 128 */
