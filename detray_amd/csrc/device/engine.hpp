@@ -313,6 +313,12 @@ struct CacheView {
     }
 };
 
+// Muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64):
+// the mass is a constant and the charge follows the sign of q/p, which no
+// interaction changes (a track brought to rest ends its propagation).
+constexpr double kMuonMass = static_cast<double>(105.6583755f * 1e-3f);
+DTR_AI double track_charge(double qop) { return (-1.0 * qop > 0.0) ? -1.0 : 1.0; }
+
 template <class P>
 struct FastTrack {
     using S = typename P::S;
@@ -327,9 +333,8 @@ struct FastTrack {
     double time, qop;
     // stepping
     double step_size, next_step_size, path_length, abs_path_length;
-    double constraint_acc, constraint_abort;
+    double constraint_abort;
     T3<S> dtds3;
-    double mass, charge;
     double h_stage;  // step size the stages of the last step were evaluated at
     uint32_t n_trials, n_steps;
     // transport Jacobian (reduced) and covariance in the stage precision
@@ -919,7 +924,7 @@ struct FastEngine {
             tk.step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
         }
         tk.dtds3 = k4;
-        const double cmin = dmin(tk.constraint_acc, tk.constraint_abort);
+        const double cmin = dmin(dmin(kMax, fabs(cfg.step_constraint)), tk.constraint_abort);
         const double max_step = (tk.step_size >= 0.0 ? 1.0 : -1.0) * cmin;
         if (fabs(tk.step_size) > fabs(max_step)) tk.step_size = max_step;
         if (fabs(tk.step_size) < min_step) tk.step_size = copysign(min_step, tk.step_size);
@@ -1253,7 +1258,7 @@ struct FastEngine {
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];
         const double path_seg = m.thickness / cos_inc;
-        const double mass = tk.mass, q = tk.charge;
+        const double mass = kMuonMass, q = track_charge(tk.qop);
         const double q2b2 = q * q + (mass * qop) * (mass * qop);
         const double mOverP = mass * fabs(qop / q);
         const double pOverM = 1.0 / mOverP;
@@ -1300,7 +1305,7 @@ struct FastEngine {
             }
         }
         if (cfg.do_eloss) {
-            const double massd = tk.mass, qd = tk.charge;
+            const double massd = kMuonMass, qd = track_charge(tk.qop);
             const double p = qd / tk.bound[4];
             const double next_e = sqrt(massd * massd + p * p) - copysign(e_loss, 1.0);
             const double next_p = (massd < next_e) ? sqrt(next_e * next_e - massd * massd) : 0.0;
@@ -1352,7 +1357,7 @@ struct FastEngine {
         const double max_tol = 5.0;
         const double path = exhausted() ? max_tol : fabs(c.p(tk.next));
         displ = mul(path, sub(displ, tk.dir));
-        const double p = tk.charge / tk.qop;
+        const double p = track_charge(tk.qop) / tk.qop;
         const double acc = 1.0 / p * cfg.accumulated_error * tk.path_length;
         const double ext = sqrt(ns * ns * (v0 + v1) + dot(displ, displ) + acc * acc);
         tk.ext_tol = ext > max_tol ? max_tol : ext;
@@ -1467,11 +1472,8 @@ struct Lane {
         tk.next_step_size = 0.0;
         tk.path_length = 0.0;
         tk.abs_path_length = 0.0;
-        tk.constraint_acc = dmin(kMax, fabs(cfg.step_constraint));
         tk.constraint_abort = kMax;
         tk.dtds3 = {S(0), S(0), S(0)};
-        // muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64)
-        tk.mass = static_cast<double>(105.6583755f * 1e-3f);
         tk.n_trials = 0;
         tk.n_steps = 0;
         for (int i = 0; i < 6; ++i) tk.bound[i] = 0.0;
@@ -1504,7 +1506,6 @@ struct Lane {
             tk.dir = {in.fp[4 * N + gid], in.fp[5 * N + gid], in.fp[6 * N + gid]};
             tk.qop = in.fp[7 * N + gid];
         }
-        tk.charge = (-1.0 * tk.qop > 0.0) ? -1.0 : 1.0;
         e.reset_J();
         os = -cfg.path_tol;
     }
