@@ -242,6 +242,15 @@ class AdjointBatch(C.Structure):
     ]
 
 
+class SurfaceSequence(C.Structure):
+    _fields_ = [
+        ("max_surfaces", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("surfaces", C.c_void_p),
+        ("n_surfaces", C.c_void_p),
+    ]
+
+
 class ToyConfig(C.Structure):
     _fields_ = [
         ("n_brl_layers", C.c_uint32),
@@ -290,6 +299,9 @@ _SIGNATURES = {
     "dtr_sort_workspace_size": (C.c_size_t, [C.POINTER(SortConfig)]),
     "dtr_sort_tracks": (C.c_int, [C.POINTER(SortConfig), C.c_void_p, C.c_uint32, C.c_void_p,
                                   C.c_void_p, C.c_size_t, C.c_void_p]),
+    "dtr_propagate_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(TrackBatch),
+                                       C.POINTER(SurfaceSequence), C.POINTER(ResultBatch),
+                                       C.POINTER(LaunchConfig), C.c_void_p]),
     "dtr_default_adjoint": (None, [C.POINTER(AdjointConfig)]),
     "dtr_adjoint_tape_size": (C.c_size_t, [C.POINTER(AdjointConfig), C.c_uint32]),
     "dtr_adjoint_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(AdjointConfig),

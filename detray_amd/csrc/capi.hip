@@ -18,6 +18,7 @@
 #include "detray_amd.h"
 #include "device/adjoint.hpp"
 #include "device/coop_nav.hpp"
+#include "device/direct_nav.hpp"
 #include "device/engine.hpp"
 #include "device/propagator.hpp"
 
@@ -152,6 +153,19 @@ DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
 }
 
 __global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
+
+// ----------------------------------------------------------------- direct --
+// Propagation with the direct navigator: one work-item per track.
+template <class P, int W>
+DTR_BATCHED_KERNEL propagate_direct_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
+                                           dtr::DevConfig dcfg, dtr::TrackInputs in, dtr::DevResults out,
+                                           dtr::SeqView sq, const uint32_t* __restrict__ order) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= in.n) return;
+    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+    dtr::propagate_direct<P>(det, field, cfg, dcfg, in, out, sq, track, lds_cache(smem));
+}
 
 // ------------------------------------------------------------------ adjoint --
 // Recording forward pass: one work-item per track (work-item k runs track
@@ -675,6 +689,40 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);
+}
+
+int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                         const dtr_track_batch* in, const dtr_surface_sequence* seq, dtr_result_batch* out,
+                         const dtr_launch_config* launch, void* stream) {
+    const int rc = check_propagate_args(det, field, c, in, out);
+    if (rc != DTR_SUCCESS) return rc;
+    if (!seq || !launch) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->n == 0u) return DTR_SUCCESS;
+    if (seq->max_surfaces == 0u || !seq->surfaces || !seq->n_surfaces) return DTR_ERROR_INVALID_ARGUMENT;
+    dtr_launch_config L = *launch;
+    if (L.block_size == 0u) L.block_size = 64u;
+    if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.schedule != DTR_SCHEDULE_GRID) return DTR_ERROR_UNSUPPORTED;
+    const dtr::DevConfig dc = make_dev_config(c);
+    const dtr::DevConfig dd = dtr::direct_config(dc);
+    dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                      c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                      out->n_trials, out->status};
+    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
+                              in->start_surface, in->n};
+    const dtr::SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, det->n_surfaces};
+    const uint32_t block = L.block_size;
+    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
+    const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (L.precision == DTR_PRECISION_FP64)
+        hipLaunchKernelGGL((propagate_direct_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), shmem, st,
+                           det->fast, field->dev, dc, dd, ti, r, sq, L.order);
+    else
+        hipLaunchKernelGGL((propagate_direct_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), shmem, st,
+                           det->fast, field->dev, dc, dd, ti, r, sq, L.order);
+    return to_status(hipGetLastError());
 }
 
 void dtr_default_adjoint(dtr_adjoint_config* a) {

@@ -369,6 +369,63 @@ class Propagator:
         ``initial_cov`` [21, n] is the upper-triangle bound covariance."""
         import torch
 
+        tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
+                                                      start_surface)
+        lib = capi.load()
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        if self.launch is None:
+            capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
+                                         C.byref(tb), C.byref(rb), C.c_void_p(stream)),
+                       "dtr_propagate")
+            return out
+        order = self.launch.order
+        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)):
+            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
+        lc = self.launch.to_c(self._queue(device) if self.launch.schedule == "persistent" else None)
+        capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(self._c_config),
+                                        C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
+                   "dtr_propagate_ex")
+        return out
+
+    def propagate_direct(self, detector: Detector, field: MagneticField, sequence, n_sequence, tracks=None,
+                         initial_cov=None, out: Optional[PropagationResult] = None, stream=None,
+                         bound_params=None, start_surface=None) -> PropagationResult:
+        """Propagate with the direct navigator (``dtr_propagate_direct``):
+        every track follows its surface sequence ``sequence`` [m, n] int32
+        (``n_sequence`` [n] entries each, e.g. the ``surfaces`` /
+        ``n_surfaces`` of an earlier propagation).  The launch's precision,
+        block size and order are used (grid schedule; without a launch:
+        mixed precision, block 64)."""
+        import torch
+
+        tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
+                                                      start_surface)
+        if (not isinstance(sequence, torch.Tensor) or sequence.dtype != torch.int32 or sequence.dim() != 2
+                or sequence.shape[1] != n or not sequence.is_cuda):
+            raise ValueError("sequence must be an int32 CUDA tensor [m, n]")
+        if (not isinstance(n_sequence, torch.Tensor) or n_sequence.dtype != torch.int32
+                or tuple(n_sequence.shape) != (n,) or not n_sequence.is_cuda):
+            raise ValueError("n_sequence must be an int32 CUDA tensor [n]")
+        sequence = sequence.contiguous()
+        n_sequence = n_sequence.contiguous()
+        launch = self.launch or LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
+        if launch.order is not None and (not launch.order.is_cuda or launch.order.dtype != torch.int32
+                                         or launch.order.shape != (n,)):
+            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
+        seq = capi.SurfaceSequence(int(sequence.shape[0]), 0, sequence.data_ptr(), n_sequence.data_ptr())
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        lc = launch.to_c(None)
+        capi.check(capi.load().dtr_propagate_direct(detector.device(), field.device(), C.byref(self._c_config),
+                                                    C.byref(tb), C.byref(seq), C.byref(rb), C.byref(lc),
+                                                    C.c_void_p(stream)), "dtr_propagate_direct")
+        return out
+
+    def _batches(self, detector, tracks, initial_cov, out, bound_params, start_surface):
+        """Validated C views of the inputs and (allocated) outputs."""
+        import torch
+
         if bound_params is not None:
             if start_surface is None:
                 raise ValueError("bound_params needs start_surface")
@@ -399,7 +456,6 @@ class Propagator:
             initial_cov = initial_cov.contiguous()
         if out is None:
             out = self.allocate(n, device)
-        lib = capi.load()
         tb = capi.TrackBatch(n, 0, tracks.data_ptr() if tracks is not None and bound_params is None else None,
                              initial_cov.data_ptr() if initial_cov is not None else None,
                              bound_params.data_ptr() if bound_params is not None else None,
@@ -409,18 +465,5 @@ class Propagator:
                               out.surfaces.data_ptr(), out.n_surfaces.data_ptr(),
                               out.n_steps.data_ptr(), out.n_trials.data_ptr(),
                               out.status.data_ptr())
-        if stream is None:
-            stream = torch.cuda.current_stream(device).cuda_stream
-        if self.launch is None:
-            capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
-                                         C.byref(tb), C.byref(rb), C.c_void_p(stream)),
-                       "dtr_propagate")
-            return out
-        order = self.launch.order
-        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)):
-            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
-        lc = self.launch.to_c(self._queue(device) if self.launch.schedule == "persistent" else None)
-        capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(self._c_config),
-                                        C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
-                   "dtr_propagate_ex")
-        return out
+        # the tensors behind the C views stay referenced until the launch
+        return tb, rb, out, n, device, (tracks, initial_cov, bound_params, start_surface)

@@ -354,6 +354,32 @@ DTR_API int dtr_sort_tracks(const dtr_sort_config* cfg, const double* free_param
                             void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Direct navigation: follow a given surface sequence                       */
+/* ------------------------------------------------------------------------ */
+/* Surface sequences of a batch, DEVICE memory: surfaces [max_surfaces][n]
+ * (entry k of track i at k * n + i, the surface_sequencer layout of
+ * dtr_result_batch), n_surfaces [n]. */
+typedef struct dtr_surface_sequence {
+    uint32_t max_surfaces;
+    uint32_t pad;
+    const uint32_t* surfaces;
+    const uint32_t* n_surfaces;
+} dtr_surface_sequence;
+
+/* propagator<rk_stepper, direct_navigator, actor chain>::propagate per track
+ * (direct_navigator.hpp:29-298): instead of searching the detector, each
+ * track steps to the next surface of its sequence, intersected with
+ * infinite mask tolerance (the target is always hit); the navigation exits
+ * after the last surface and aborts if a target cannot be reached or is not
+ * a surface of the detector.  Forward direction.  Same stepper, actor chain
+ * and results as dtr_propagate_ex; the launch gives the precision, block
+ * size and track order (grid schedule only). */
+DTR_API int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field,
+                                 const dtr_prop_config* cfg, const dtr_track_batch* in,
+                                 const dtr_surface_sequence* seq, dtr_result_batch* out,
+                                 const dtr_launch_config* launch, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Differentiation: d(chi2 of hit residuals)/d(initial bound parameters)    */
 /* ------------------------------------------------------------------------ */
 /* chi2 = sum over the hits matched on the track's sensitive surfaces of

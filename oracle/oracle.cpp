@@ -486,6 +486,70 @@ struct Engine {
         return false;
     }
 
+    // update_candidate with an explicit intersection config
+    bool update_candidate(Cand& cand, const NavCfg& icfg) {
+        if (cand.sf == kInvalid) return false;
+        const V3 rd = mul(static_cast<double>(nav.direction), st.dir);
+        IPoint ip;
+        if (!intersect_surface(det, cand.sf, st.pos, rd, icfg.overstep_tol, ip)) return false;
+        const dtr_surface& s = det.sf(cand.sf);
+        const dtr_transform& t = det.trf(cand.sf);
+        for (uint32_t k = 0; k < s.n_masks; ++k) {
+            resolve_mask(cand, ip.path, ip.glob, ip.z, cand.sf, s, det.mask(s.mask + k), t, icfg, nav.ext_tol);
+            if (cand.probably_inside()) return true;
+        }
+        return false;
+    }
+
+    // ------------------------------------------------------ direct navigator --
+    // direct_navigator.hpp:29-298: the target is always cache entry 1, the
+    // current surface entry 0; the targets are the given surface sequence.
+    const uint32_t* dseq = nullptr;
+    uint64_t dseq_stride = 0;
+    uint32_t dseq_len = 0, dseq_next = 0;
+
+    uint32_t dseq_at(uint32_t k) const { return dseq[static_cast<uint64_t>(k) * dseq_stride]; }
+
+    // direct_navigator::update (direct_navigator.hpp:210-298)
+    bool direct_update() {
+        if (dseq_next >= dseq_len) {
+            nav.exit_nav();
+            return false;
+        }
+        if (nav.trust == kFull) return false;
+        // infinite tolerances: the target is always hit
+        NavCfg icfg = cfg.nav;
+        icfg.min_mask_tol = std::numeric_limits<float>::max();
+        icfg.max_mask_tol = std::numeric_limits<float>::max();
+        icfg.mask_scalor = std::numeric_limits<float>::max();
+        icfg.overstep_tol = -std::numeric_limits<float>::max();
+        // a sequence entry that is not a surface of the detector ends the
+        // navigation like an unreachable target
+        if (nav.c[1].sf >= det.d->n_surfaces || !update_candidate(nav.c[1], icfg)) {
+            nav.abort_nav();
+            return false;
+        }
+        // navigation::update_status (navigation_functions.hpp:148-177) with
+        // the direct state's advance()
+        if (std::fabs(nav.c[1].path) < static_cast<double>(cfg.nav.path_tol)) {
+            nav.status = is_portal(nav.c[1].sf) ? kOnPortal : kOnObject;
+            nav.c[0] = nav.c[1];
+            dseq_next++;
+            if (dseq_next < dseq_len) nav.c[1].sf = dseq_at(dseq_next);
+        } else {
+            nav.status = kTowards;
+        }
+        nav.trust = kFull;
+        if (nav.status == kTowards) return false;
+        if (dseq_next < dseq_len && (nav.c[1].sf >= det.d->n_surfaces || !update_candidate(nav.c[1], icfg))) {
+            nav.abort_nav();
+            return false;
+        }
+        nav.set_volume(det.sf(nav.c[0].sf).volume);
+        nav.trust = kFull;
+        return true;
+    }
+
     // spatial_grid.hpp:100-202 + axis.hpp:417-424 + axis_binning.hpp:80-115
     void grid_search(const dtr_volume& vol, const NavCfg& icfg) {
         const dtr_grid& g = det.d->grids[vol.grid];
@@ -1221,9 +1285,18 @@ struct Engine {
         tr_n++;
     }
 
-    // propagator.hpp:262-400
+    // propagator.hpp:262-400; with a surface sequence the direct navigator
+    // (direct_navigator.hpp:198-206 init: clear the cache, keep the target)
     bool propagate() {
-        local_navigation(cfg.nav, false);  // navigator_base::init
+        if (dseq) {
+            nav.clear_cache();
+            nav.next = 1;
+            nav.last = 1;
+            if (dseq_len > 0) nav.c[1].sf = dseq_at(0);
+            direct_update();
+        } else {
+            local_navigation(cfg.nav, false);  // navigator_base::init
+        }
         heartbeat = nav.is_alive();
         trace();
         bool is_init = true;
@@ -1254,7 +1327,7 @@ struct Engine {
                     stall = 0;
                 }
             }
-            is_init |= nav_update();
+            is_init |= dseq ? direct_update() : nav_update();
             heartbeat = heartbeat && nav.is_alive();
             trace();
         }
@@ -1266,10 +1339,11 @@ struct Engine {
 
 extern "C" {
 
-int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* field,
+static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* field,
                           const dtr_prop_config* cfg, const dtr_track_batch* in,
                           dtr_result_batch* out, oracle_hits* hits, int n_threads,
-                          oracle_trace* trace) {
+                          oracle_trace* trace, const uint32_t* seq, const uint32_t* seq_n,
+                          uint32_t seq_max) {
     using namespace oracle;
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -1348,6 +1422,11 @@ int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* fi
             e.seq = out->surfaces + i;
             e.seq_stride = static_cast<uint32_t>(n);
         }
+        if (seq) {
+            e.dseq = seq + i;
+            e.dseq_stride = static_cast<uint64_t>(n);
+            e.dseq_len = seq_n[i] < seq_max ? seq_n[i] : seq_max;
+        }
         if (hits) {
             e.hit_sf = hits->surface + i;
             e.hit_loc = hits->loc + i;
@@ -1386,10 +1465,26 @@ int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* fi
     return DTR_SUCCESS;
 }
 
+int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* field,
+                          const dtr_prop_config* cfg, const dtr_track_batch* in,
+                          dtr_result_batch* out, oracle_hits* hits, int n_threads,
+                          oracle_trace* trace) {
+    return propagate_impl(det, field, cfg, in, out, hits, n_threads, trace, nullptr, nullptr, 0u);
+}
+
 int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, int n_threads, oracle_trace* trace) {
-    return oracle_propagate_hits(det, field, cfg, in, out, nullptr, n_threads, trace);
+    return propagate_impl(det, field, cfg, in, out, nullptr, n_threads, trace, nullptr, nullptr, 0u);
+}
+
+int oracle_propagate_direct(const dtr_detector_desc* det, const dtr_field_desc* field,
+                            const dtr_prop_config* cfg, const dtr_track_batch* in,
+                            const uint32_t* sequence, const uint32_t* n_sequence, uint32_t max_sequence,
+                            dtr_result_batch* out, int n_threads) {
+    if (in && in->n && (!sequence || !n_sequence || max_sequence == 0u)) return DTR_ERROR_INVALID_ARGUMENT;
+    return propagate_impl(det, field, cfg, in, out, nullptr, n_threads, nullptr, sequence, n_sequence,
+                          max_sequence);
 }
 
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz, double* b) {

@@ -53,6 +53,14 @@ int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* fi
                           dtr_result_batch* out, oracle_hits* hits, int n_threads,
                           oracle_trace* trace);
 
+/* Propagation with the direct navigator (direct_navigator.hpp): every track
+ * follows its surface sequence [max_sequence][n] (n_sequence[n] entries) instead
+ * of searching the detector; same stepper and actors. */
+int oracle_propagate_direct(const dtr_detector_desc* det, const dtr_field_desc* field,
+                            const dtr_prop_config* cfg, const dtr_track_batch* in,
+                            const uint32_t* sequence, const uint32_t* n_sequence, uint32_t max_sequence,
+                            dtr_result_batch* out, int n_threads);
+
 /* Evaluate the field at n points (host), for field tests. */
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz,
                     double* bxyz);

@@ -163,6 +163,39 @@ def chi2_gradient_fd(detector, field, config, bound_params, start_surface, meas_
     return g
 
 
+def propagate_direct(detector, field, config, free_params, sequence, n_sequence, threads: int = 0) -> dict:
+    """Oracle propagation with the direct navigator along the given surface
+    sequences [m, n] (n_sequence [n] entries each)."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    seq = np.ascontiguousarray(sequence, dtype=np.uint32)
+    nseq = np.ascontiguousarray(n_sequence, dtype=np.uint32)
+    n = fp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+    }
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data, None, None, None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    lib.oracle_propagate_direct.restype = C.c_int
+    cfg = config.to_c()
+    capi.check(lib.oracle_propagate_direct(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(tb),
+                                           C.c_void_p(seq.ctypes.data), C.c_void_p(nseq.ctypes.data),
+                                           C.c_uint32(seq.shape[0]), C.byref(rb), C.c_int(threads)),
+               "oracle_propagate_direct")
+    return out
+
+
 def field_at(field, xyz: np.ndarray) -> np.ndarray:
     lib = load()
     pts = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

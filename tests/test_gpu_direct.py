@@ -1,0 +1,107 @@
+"""Direct navigator on the GPU (dtr_propagate_direct) against the oracle's
+direct navigator on the same seeded inputs and sequences.
+
+FP64: identical surface sequences, statuses, step and trial counts; bound
+parameters within 1e-9 of max(|x|, 1); covariance within 1e-7 of
+sqrt(C_ii C_jj).  Mixed precision: identical sequences and statuses, bound
+parameters within 1e-4 in the reference's host/device convention.  On the
+GPU, the reference's own check (propagation_direct_navigator.cpp): a
+sequence recorded with the caching navigator is replayed surface for
+surface, with the momentum at the last surface equal to 2e-3."""
+
+import numpy as np
+import pytest
+
+import detray_amd as da
+from tests import oracle_host
+from tests.test_direct_navigator import CASES, momentum, uniform_tracks
+from tests.test_gpu_batched import MIXED_BOUND_TOL, bound_error_scale
+from tests.test_gpu_parity import assert_parity
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+def run_direct(det, field, cfg, fp, seq, n_seq, precision="fp64", order=None, block=64):
+    launch = da.LaunchConfig(precision, "grid", block, 2, 64, order, 1, "lane")
+    prop = da.Propagator(cfg, launch)
+    res = prop.propagate_direct(det, field, torch.from_numpy(np.ascontiguousarray(seq).view(np.int32)).cuda(),
+                                torch.from_numpy(np.ascontiguousarray(n_seq).view(np.int32)).cuda(),
+                                tracks=torch.from_numpy(fp).cuda())
+    torch.cuda.synchronize()
+    return {
+        "bound_params": res.bound_params.cpu().numpy(),
+        "bound_cov": res.bound_cov.cpu().numpy(),
+        "free_params": res.free_params.cpu().numpy(),
+        "path_length": res.path_length.cpu().numpy(),
+        "surfaces": res.surfaces.cpu().numpy().view(np.uint32),
+        "n_surfaces": res.n_surfaces.cpu().numpy().view(np.uint32),
+        "n_steps": res.n_steps.cpu().numpy().view(np.uint32),
+        "n_trials": res.n_trials.cpu().numpy().view(np.uint32),
+        "status": res.status.cpu().numpy(),
+    }
+
+
+@pytest.mark.parametrize("overstep,bfield", CASES[:1] + CASES[3:])
+def test_direct_fp64_matches_oracle(toy7, overstep, bfield):
+    field = da.MagneticField.constant(*(b * da.TESLA for b in bfield))
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False, overstep_tolerance=overstep)
+    fp = uniform_tracks()
+    rec = oracle_host.propagate(toy7, field, cfg, fp)
+    ref = oracle_host.propagate_direct(toy7, field, cfg, fp, rec["surfaces"], rec["n_surfaces"])
+    got = run_direct(toy7, field, cfg, fp, rec["surfaces"], rec["n_surfaces"])
+    assert_parity(got, ref)
+
+
+def test_direct_mixed_matches_oracle(toy7, solenoid):
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+    fp = da.generate_tracks("toy", 4096, 44)
+    rec = oracle_host.propagate(toy7, solenoid, cfg, fp)
+    ref = oracle_host.propagate_direct(toy7, solenoid, cfg, fp, rec["surfaces"], rec["n_surfaces"])
+    got = run_direct(toy7, solenoid, cfg, fp, rec["surfaces"], rec["n_surfaces"], "mixed")
+    np.testing.assert_array_equal(got["status"], ref["status"])
+    np.testing.assert_array_equal(got["n_surfaces"], ref["n_surfaces"])
+    np.testing.assert_array_equal(got["surfaces"], ref["surfaces"])
+    err = np.abs(got["bound_params"] - ref["bound_params"]) / bound_error_scale(ref)
+    assert err.max() <= MIXED_BOUND_TOL, err.max()
+
+
+@pytest.mark.parametrize("overstep,bfield", CASES)
+def test_direct_replays_gpu_caching_sequence(toy7, overstep, bfield):
+    """propagation_direct_navigator.cpp:46-186 on the GPU, mixed precision."""
+    field = da.MagneticField.constant(*(b * da.TESLA for b in bfield))
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False, overstep_tolerance=overstep)
+    fp = uniform_tracks()
+    launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
+    cach = da.Propagator(cfg, launch).propagate(toy7, field, torch.from_numpy(fp).cuda())
+    torch.cuda.synchronize()
+    seq = cach.surfaces.cpu().numpy().view(np.uint32)
+    n_seq = cach.n_surfaces.cpu().numpy().view(np.uint32)
+    got = run_direct(toy7, field, cfg, fp, seq, n_seq, "mixed")
+    has = n_seq > 0
+    assert (got["status"] == 0).all()
+    np.testing.assert_array_equal(got["n_surfaces"], n_seq)
+    np.testing.assert_array_equal(got["surfaces"][:, has], seq[:, has])
+    p_ref = momentum(cach.bound_params.cpu().numpy()[:, has])
+    p_dir = momentum(got["bound_params"][:, has])
+    assert np.max(np.abs(p_ref - p_dir) / p_ref) < 2e-3
+
+
+def test_direct_order_and_edge_cases(toy7):
+    field = da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA)
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+    fp = uniform_tracks(16, 16)
+    rec = oracle_host.propagate(toy7, field, cfg, fp)
+    seq, n_seq = rec["surfaces"].copy(), rec["n_surfaces"].copy()
+    n_seq[0] = 0
+    seq[1, 1] = toy7.n_surfaces + 5
+    n_seq[1] = max(n_seq[1], 2)
+    ref = oracle_host.propagate_direct(toy7, field, cfg, fp, seq, n_seq)
+    a = run_direct(toy7, field, cfg, fp, seq, n_seq, "mixed")
+    perm = torch.from_numpy(np.random.default_rng(3).permutation(fp.shape[1]).astype(np.int32)).cuda()
+    b = run_direct(toy7, field, cfg, fp, seq, n_seq, "mixed", order=perm, block=128)
+    np.testing.assert_array_equal(a["status"], ref["status"])
+    assert a["status"][0] == 0 and a["n_surfaces"][0] == 0 and a["status"][1] == 1
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
