@@ -341,7 +341,9 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
     if (!in->free_params && !in->bound_params) return DTR_ERROR_INVALID_ARGUMENT;
     if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->max_surfaces != 0u && (!out->surfaces || !out->n_surfaces)) return DTR_ERROR_INVALID_ARGUMENT;
-    if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    // use_eloss_gradient: d(dqop/ds)/dqop is zero without volume material
+    // (rk_stepper.hpp:193-200), which these detector tables do not have, so
+    // the flag changes nothing
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     return DTR_SUCCESS;
 }
@@ -390,7 +392,7 @@ int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dt
     if (a->block_size != 0u && (a->block_size % 64u != 0u || a->block_size > 256u)) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->waves_per_simd > 32u) return DTR_ERROR_INVALID_ARGUMENT;
     if (!(a->sigma_loc0 > 0.0) || !(a->sigma_loc1 > 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
-    if (c->use_field_gradient || c->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -411,6 +413,20 @@ dtr::AdjointIO make_adjoint_io(const dtr_adjoint_config* a, const dtr_adjoint_ba
     io.iv1 = 1.0 / (a->sigma_loc1 * a->sigma_loc1);
     io.tape = dtr::tape_carve(b->tape, b->n, a->max_steps, a->max_events, a->checkpoint_every);
     return io;
+}
+
+// use_field_gradient: the Jacobian with position columns, one work-item per
+// track with per-lane navigation
+template <class P>
+int launch_grad(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+                const dtr_track_batch* in, const dtr::DevResults& r, uint32_t block, const uint32_t* order,
+                hipStream_t stream) {
+    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
+                              in->start_surface, in->n};
+    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
+    hipLaunchKernelGGL((propagate_grid_kernel<P, 2>), dim3(grid), dim3(block), lds_bytes(block), stream, det->fast,
+                       field->dev, dc, ti, r, order);
+    return to_status(hipGetLastError());
 }
 
 // register budget: 1 or 2 resident waves per SIMD
@@ -601,6 +617,12 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
+    if (c->use_field_gradient) {
+        // the Jacobian with position columns runs on the batched engine's
+        // FP64 policy (same results contract as the plain kernel)
+        return launch_grad<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
+                                                                    static_cast<hipStream_t>(stream));
+    }
     const uint32_t block = 256;
     const uint32_t grid = (in->n + block - 1) / block;
     hipLaunchKernelGGL(propagate_fp64_kernel, dim3(grid), dim3(block), 0,
@@ -687,6 +709,13 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c->use_field_gradient) {
+        if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
+        if (L.precision == DTR_PRECISION_FP64)
+            return launch_grad<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size, L.order,
+                                                                        st);
+        return launch_grad<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order, st);
+    }
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);
 }
@@ -703,7 +732,7 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     if (L.block_size == 0u) L.block_size = 64u;
     if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
-    if (L.schedule != DTR_SCHEDULE_GRID) return DTR_ERROR_UNSUPPORTED;
+    if (L.schedule != DTR_SCHEDULE_GRID || c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
     const dtr::DevConfig dc = make_dev_config(c);
     const dtr::DevConfig dd = dtr::direct_config(dc);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,

@@ -96,11 +96,28 @@ struct FastDetector {
 struct PolicyFP64 {
     using S = double;
     static constexpr bool kMixed = false;
+    static constexpr bool kGrad = false;
 };
 struct PolicyMixed {
     using S = float;
     static constexpr bool kMixed = true;
+    static constexpr bool kGrad = false;
 };
+// stepping::config::use_field_gradient: the transport Jacobian also carries
+// the position columns (rk_stepper.hpp:878-937)
+template <class Base>
+struct WithFieldGradient : Base {
+    static constexpr bool kGrad = true;
+};
+
+// Position columns of the transport Jacobian (rows pos 0-2 then dir 0-2),
+// stored only when the policy transports them.
+template <typename S, bool kOn>
+struct PosColumns {
+    S v[18];
+};
+template <typename S>
+struct PosColumns<S, false> {};
 
 // Host: the navigation record of surface s of a flattened detector.  The
 // prefilter bound (radius, tolk) covers every point a mask check of the
@@ -339,6 +356,7 @@ struct FastTrack {
     uint32_t n_trials, n_steps;
     // transport Jacobian (reduced) and covariance in the stage precision
     S J[25];
+    PosColumns<S, P::kGrad> Jx;
     S cov[21];
     double bound[6];
     uint32_t up_sf;
@@ -942,7 +960,10 @@ struct FastEngine {
         tk.dir = {nd.x / nn, nd.y / nn, nd.z / nn};
         tk.path_length += h;
         tk.abs_path_length += fabs(h);
-        if (cfg.do_cov) advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+        if (cfg.do_cov) {
+            if constexpr (P::kGrad) advance_jacobian_grad(dir, k1, k3, t2, t3, t4, b1, b2, b3, qop);
+            else advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+        }
         tk.n_steps++;
     }
 
@@ -1009,6 +1030,139 @@ struct FastEngine {
         }
     }
 
+    // Field gradient by central differences over +-0.1 mm (rk_stepper.hpp:
+    // 958-998), G[r][i] = dB_r / dx_i, in the stage precision.
+    DTR_AI void field_gradient_fd(V3 p, S G[3][3]) const {
+        const double delta = static_cast<double>(1e-1f);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            V3 p1 = p, p2 = p;
+            if (i == 0) { p1.x += delta; p2.x -= delta; }
+            if (i == 1) { p1.y += delta; p2.y -= delta; }
+            if (i == 2) { p1.z += delta; p2.z -= delta; }
+            const T3<S> b1 = field_at_s<S>(field, p1), b2 = field_at_s<S>(field, p2);
+            const S inv = static_cast<S>(1.0 / (2.0 * delta));
+            G[0][i] = (b1.x - b2.x) * inv;
+            G[1][i] = (b1.y - b2.y) * inv;
+            G[2][i] = (b1.z - b2.z) * inv;
+        }
+    }
+
+    // advance_jacobian with the field gradient (rk_stepper.hpp:633-956,
+    // use_field_gradient): the full update J <- [[dF/dr, dF/dt, dF/dqop],
+    // [dG/dr, dG/dt, dG/dqop], [0, 0, 1]] J over the position, direction and
+    // q/p columns.  As in the reference, the gradient enters dk/dr only, and
+    // its points are taken around the end point of the step (the reference
+    // evaluates them after advance_track).
+    DTR_AI void advance_jacobian_grad(T3<S> t1, T3<S> k1, T3<S> k3, T3<S> t2, T3<S> t3, T3<S> t4, T3<S> b1,
+                                      T3<S> b2, T3<S> b3, S qop) {
+        const double hd = tk.step_size;
+        const S h = static_cast<S>(hd);
+        const S half_h = h * S(0.5);
+        const S h_6 = h * static_cast<S>(1.f / 6.f);
+        const T3<S> dk0 = tcross(t1, b1);
+        const T3<S> dk1 = tadd(tcross(t2, b2), tmul(qop * half_h, tcross(dk0, b2)));
+        const T3<S> dk2 = tadd(tcross(t3, b2), tmul(qop * half_h, tcross(dk1, b2)));
+        const T3<S> dk3 = tadd(tcross(t4, b3), tmul(qop * h, tcross(dk2, b3)));
+        const T3<S> dFdqop = tmul(h * h_6, tadd(tadd(dk0, dk1), dk2));
+        const T3<S> dGdqop = tmul(h_6, tadd(tadd(dk0, tmul(S(2), tadd(dk1, dk2))), dk3));
+        // the 6x7 block [dF/dr dF/dt dF/dqop; dG/dr dG/dt dG/dqop] by columns
+        S D[6][7];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T3<S> e{j == 0 ? S(1) : S(0), j == 1 ? S(1) : S(0), j == 2 ? S(1) : S(0)};
+            const T3<S> d0 = tmul(qop, tcross(e, b1));
+            const T3<S> d1 = tmul(qop, tcross(tadd(e, tmul(half_h, d0)), b2));
+            const T3<S> d2 = tmul(qop, tcross(tadd(e, tmul(half_h, d1)), b2));
+            const T3<S> d3 = tmul(qop, tcross(tadd(e, tmul(h, d2)), b3));
+            const T3<S> f = tmul(h, tadd(e, tmul(h_6, tadd(tadd(d0, d1), d2))));
+            const T3<S> g = tadd(e, tmul(h_6, tadd(tadd(d0, tmul(S(2), tadd(d1, d2))), d3)));
+            D[0][3 + j] = f.x; D[1][3 + j] = f.y; D[2][3 + j] = f.z;
+            D[3][3 + j] = g.x; D[4][3 + j] = g.y; D[5][3 + j] = g.z;
+        }
+        D[0][6] = dFdqop.x; D[1][6] = dFdqop.y; D[2][6] = dFdqop.z;
+        D[3][6] = dGdqop.x; D[4][6] = dGdqop.y; D[5][6] = dGdqop.z;
+        {
+            const V3 r_ini = tk.pos;
+            const V3 t0d = tdbl(t1), k1d = tdbl(k1), k3d = tdbl(k3);
+            const double h2 = hd * hd;
+            const V3 r_mid = add(add(r_ini, mul(hd * 0.5, t0d)), mul(h2 * 0.125, k1d));
+            const V3 r_fin = add(add(r_ini, mul(hd, t0d)), mul(h2 * 0.5, k3d));
+            S Gi[3][3], Gm[3][3], Gf[3][3];
+            field_gradient_fd(r_ini, Gi);
+            field_gradient_fd(r_mid, Gm);
+            field_gradient_fd(r_fin, Gf);
+            const S hh2 = static_cast<S>(h2);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                // column j of dk_n/dr1
+                const T3<S> gi{Gi[0][j], Gi[1][j], Gi[2][j]};
+                const T3<S> r0 = tmul(qop, tcross(t1, gi));  // -q (dB/dr)_j x t = q t x (dB/dr)_j
+                // dr_mid/dr1 column j = e_j + h^2/8 r0; dB_mid/dr1 = Gm (e_j + h^2/8 r0)
+                const T3<S> cm{(j == 0 ? S(1) : S(0)) + hh2 * S(0.125) * r0.x,
+                               (j == 1 ? S(1) : S(0)) + hh2 * S(0.125) * r0.y,
+                               (j == 2 ? S(1) : S(0)) + hh2 * S(0.125) * r0.z};
+                const T3<S> gm{Gm[0][0] * cm.x + Gm[0][1] * cm.y + Gm[0][2] * cm.z,
+                               Gm[1][0] * cm.x + Gm[1][1] * cm.y + Gm[1][2] * cm.z,
+                               Gm[2][0] * cm.x + Gm[2][1] * cm.y + Gm[2][2] * cm.z};
+                const T3<S> r1 = tadd(tmul(qop, tcross(tmul(half_h, r0), b2)), tmul(qop, tcross(t2, gm)));
+                const T3<S> r2 = tadd(tmul(qop, tcross(tmul(half_h, r1), b2)), tmul(qop, tcross(t3, gm)));
+                const T3<S> cf{(j == 0 ? S(1) : S(0)) + hh2 * S(0.5) * r2.x,
+                               (j == 1 ? S(1) : S(0)) + hh2 * S(0.5) * r2.y,
+                               (j == 2 ? S(1) : S(0)) + hh2 * S(0.5) * r2.z};
+                const T3<S> gf{Gf[0][0] * cf.x + Gf[0][1] * cf.y + Gf[0][2] * cf.z,
+                               Gf[1][0] * cf.x + Gf[1][1] * cf.y + Gf[1][2] * cf.z,
+                               Gf[2][0] * cf.x + Gf[2][1] * cf.y + Gf[2][2] * cf.z};
+                const T3<S> r3 = tadd(tmul(qop, tcross(tmul(h, r2), b3)), tmul(qop, tcross(t4, gf)));
+                const T3<S> e{j == 0 ? S(1) : S(0), j == 1 ? S(1) : S(0), j == 2 ? S(1) : S(0)};
+                const T3<S> f = tadd(e, tmul(h * h_6, tadd(tadd(r0, r1), r2)));
+                const T3<S> g = tmul(h_6, tadd(tadd(r0, tmul(S(2), tadd(r1, r2))), r3));
+                D[0][j] = f.x; D[1][j] = f.y; D[2][j] = f.z;
+                D[3][j] = g.x; D[4][j] = g.y; D[5][j] = g.z;
+            }
+        }
+        // new columns: 0-2 (position, Jx) and 4-7 (direction, q/p, J)
+        const S jq = tk.J[kJqq];
+        S nx[18];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            S old[6];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                old[r] = tk.Jx.v[r * 3 + cc];
+                old[3 + r] = tk.Jx.v[9 + r * 3 + cc];
+            }
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                S s2 = S(0);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) s2 += D[r][m] * old[m];
+                nx[(r < 3 ? 0 : 9) + (r % 3) * 3 + cc] = s2;
+            }
+        }
+#pragma unroll
+        for (int cc = 4; cc < 8; ++cc) {
+            S old[6];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                old[r] = tk.J[jpos(r, cc)];
+                old[3 + r] = tk.J[jdir(r, cc)];
+            }
+            const S q = (cc == 7) ? jq : S(0);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                S s2 = S(0);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) s2 += D[r][m] * old[m];
+                s2 += D[r][6] * q;
+                if (r < 3) tk.J[jpos(r, cc)] = s2;
+                else tk.J[jdir(r - 3, cc)] = s2;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 18; ++i) tk.Jx.v[i] = nx[i];
+    }
+
     // policies.hpp:99-139
     DTR_AI void rk_policy() {
         const double d = nav_dist();
@@ -1036,6 +1190,10 @@ struct FastEngine {
         tk.J[jdir(1, 5)] = S(1);
         tk.J[jdir(2, 6)] = S(1);
         tk.J[kJqq] = S(1);
+        if constexpr (P::kGrad) {
+#pragma unroll
+            for (int i = 0; i < 18; ++i) tk.Jx.v[i] = (i < 9 && i % 4 == 0) ? S(1) : S(0);
+        }
     }
     DTR_AI void free_to_bound(uint32_t s, double out[6]) const {
         const NavSurface& rec = det.sf[s];
@@ -1101,13 +1259,23 @@ struct FastEngine {
         S Mp[3][5], Md[3][5];  // column 5 (time) is zero for both blocks
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            Mp[r][0] = A[r * 2 + 0];
-            Mp[r][1] = A[r * 2 + 1];
+            if constexpr (P::kGrad) {
+                Mp[r][0] = tk.Jx.v[r * 3 + 0] * A[0] + tk.Jx.v[r * 3 + 1] * A[2] + tk.Jx.v[r * 3 + 2] * A[4];
+                Mp[r][1] = tk.Jx.v[r * 3 + 0] * A[1] + tk.Jx.v[r * 3 + 1] * A[3] + tk.Jx.v[r * 3 + 2] * A[5];
+            } else {
+                Mp[r][0] = A[r * 2 + 0];
+                Mp[r][1] = A[r * 2 + 1];
+            }
             Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
             Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
             Mp[r][4] = tk.J[jpos(r, 7)];
-            Md[r][0] = S(0);
-            Md[r][1] = S(0);
+            if constexpr (P::kGrad) {
+                Md[r][0] = tk.Jx.v[9 + r * 3 + 0] * A[0] + tk.Jx.v[9 + r * 3 + 1] * A[2] + tk.Jx.v[9 + r * 3 + 2] * A[4];
+                Md[r][1] = tk.Jx.v[9 + r * 3 + 0] * A[1] + tk.Jx.v[9 + r * 3 + 1] * A[3] + tk.Jx.v[9 + r * 3 + 2] * A[5];
+            } else {
+                Md[r][0] = S(0);
+                Md[r][1] = S(0);
+            }
             Md[r][2] = tk.J[jdir(r, 4)] * Cd[0] + tk.J[jdir(r, 5)] * Cd[2] + tk.J[jdir(r, 6)] * Cd[4];
             Md[r][3] = tk.J[jdir(r, 4)] * Cd[1] + tk.J[jdir(r, 5)] * Cd[3] + tk.J[jdir(r, 6)] * Cd[5];
             Md[r][4] = tk.J[jdir(r, 7)];

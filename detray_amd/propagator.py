@@ -214,6 +214,24 @@ class MagneticField:
         d.values = values.ctypes.data_as(C.POINTER(C.c_float))
         return cls(d, values, f"solenoid({nxy}x{nxy}x{nz})")
 
+    @classmethod
+    def grid(cls, values: np.ndarray, minimum: Sequence[float], pitch: Sequence[float],
+             name: str = "grid") -> "MagneticField":
+        """A sampled field: ``values`` [nx, ny, nz, 3] (converted to fp32),
+        sample (0, 0, 0) at ``minimum``, spacing ``pitch`` (trilinear
+        interpolation, clamped at the border)."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if v.ndim != 4 or v.shape[3] != 3 or min(v.shape[:3]) < 2:
+            raise ValueError("values must have shape [nx, ny, nz, 3] with at least 2 samples per axis")
+        d = capi.FieldDesc()
+        d.kind = 1
+        for i in range(3):
+            d.n[i] = v.shape[i]
+            d.min[i] = float(minimum[i])
+            d.pitch[i] = float(pitch[i])
+        d.values = v.ctypes.data_as(C.POINTER(C.c_float))
+        return cls(d, v, name)
+
     def device(self):
         if self._device is None:
             lib = capi.load()

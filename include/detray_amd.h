@@ -178,8 +178,13 @@ typedef struct dtr_prop_config {
     float path_limit;         /* 5 m (pathlimit_aborter) */
     uint32_t max_rk_updates;  /* 10000 */
     uint8_t use_mean_loss;    /* 1 */
-    uint8_t use_eloss_gradient;    /* 0 */
-    uint8_t use_field_gradient;    /* 0 */
+    uint8_t use_eloss_gradient;    /* 0: accepted; d(dqop/ds)/dqop of the
+                                      volume material, which this detector
+                                      model does not carry, so no effect */
+    uint8_t use_field_gradient;    /* 0: 1 adds dB/dr (central differences,
+                                      +-0.1 mm, at the end of each step) to
+                                      the Jacobian's position columns,
+                                      rk_stepper.hpp:878-998 */
     uint8_t do_covariance_transport; /* 1 */
     /* intersection::config */
     float min_mask_tolerance; /* 1e-5 mm */
@@ -323,7 +328,9 @@ DTR_API void dtr_default_launch(dtr_launch_config* launch);
  * propagator::propagate per track, propagator.hpp:262-400); the surface
  * sequence of a track does not depend on the launch configuration.  Does
  * not allocate and does not synchronise; with the persistent schedule,
- * calls that share one `queue` must not overlap. */
+ * calls that share one `queue` must not overlap.  use_field_gradient runs
+ * on the grid schedule with per-lane navigation only (DTR_ERROR_UNSUPPORTED
+ * otherwise). */
 DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, const dtr_launch_config* launch,
@@ -373,7 +380,7 @@ typedef struct dtr_surface_sequence {
  * after the last surface and aborts if a target cannot be reached or is not
  * a surface of the detector.  Forward direction.  Same stepper, actor chain
  * and results as dtr_propagate_ex; the launch gives the precision, block
- * size and track order (grid schedule only). */
+ * size and track order (grid schedule only, no use_field_gradient). */
 DTR_API int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field,
                                  const dtr_prop_config* cfg, const dtr_track_batch* in,
                                  const dtr_surface_sequence* seq, dtr_result_batch* out,
@@ -462,7 +469,9 @@ DTR_API size_t dtr_adjoint_tape_size(const dtr_adjoint_config* cfg, uint32_t n);
  * chi2 from the chain of the reference's bound-to-bound transport
  * Jacobians; status [n] (dtr_tape_status).  `record`, if not NULL, receives
  * the bound loc0/loc1 of every sensitive surface crossed (true
- * intersections, for making hits).  `hits` may be NULL (chi2 = 0). */
+ * intersections, for making hits).  `hits` may be NULL (chi2 = 0).
+ * cfg->use_field_gradient is rejected with DTR_ERROR_UNSUPPORTED: the
+ * exact model already carries the full field derivative. */
 DTR_API int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field,
                                 const dtr_prop_config* cfg, const dtr_adjoint_config* acfg,
                                 const dtr_adjoint_batch* batch, const dtr_hits* hits,

@@ -349,6 +349,7 @@ struct PropCfg {
     double min_stepsize, rk_error_tol, step_constraint, path_limit;
     uint32_t max_rk_updates;
     bool do_cov;
+    bool use_field_gradient;
     // navigation
     NavCfg nav;
     double accumulated_error;
@@ -807,6 +808,28 @@ struct Engine {
         st.abs_path_length += std::fabs(h);
     }
 
+    // rk_stepper.hpp:958-998 evaluate_field_gradient: central differences of
+    // the field over +-0.1 mm along each axis; G[r * 3 + i] = dB_r / dx_i
+    void field_gradient(const V3& p, double G[9]) const {
+        const double delta = static_cast<double>(1e-1f);
+        for (int i = 0; i < 3; ++i) {
+            V3 p1 = p, p2 = p;
+            p1[i] += delta;
+            p2[i] -= delta;
+            const V3 b1 = field.at(p1), b2 = field.at(p2);
+            for (int r = 0; r < 3; ++r) G[r * 3 + i] = (b1[r] - b2[r]) * (1.0 / (2.0 * delta));
+        }
+    }
+
+    static void mat3_mul(const double A[9], const double B[9], double out[9]) {
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc) {
+                double s = 0.0;
+                for (int m = 0; m < 3; ++m) s += A[r * 3 + m] * B[m * 3 + cc];
+                out[r * 3 + cc] = s;
+            }
+    }
+
     static void cwc(const double M[9], const V3& b, double out[9]) {
         // column-wise cross product: column j of out = (column j of M) x b
         for (int j = 0; j < 3; ++j) {
@@ -856,9 +879,55 @@ struct Engine {
             dFdt[i] = h * (I[i] + h_6 * (dkdt[0][i] + dkdt[1][i] + dkdt[2][i]));
             dGdt[i] = I[i] + h_6 * (dkdt[0][i] + 2.0 * (dkdt[1][i] + dkdt[2][i]) + dkdt[3][i]);
         }
+        // dk_n / dr1 with the field gradient (rk_stepper.hpp:878-937); the
+        // gradient points follow the reference, which evaluates them after
+        // advance_track, i.e. around the end point of the step
+        double dFdr[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, dGdr[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (cfg.use_field_gradient) {
+            const double h2 = h * h;
+            const V3 r_ini = st.pos;
+            const V3 r_mid = add(add(r_ini, mul(half_h, t[0])), mul(h2 * 0.125, k[0]));
+            const V3 r_fin = add(add(r_ini, mul(h, t[0])), mul(h2 * 0.5, k[2]));
+            double G_ini[9], G_mid[9], G_fin[9];
+            field_gradient(r_ini, G_ini);
+            field_gradient(r_mid, G_mid);
+            field_gradient(r_fin, G_fin);
+            double dkdr[4][9], m1[9], m2[9], tmp0[9], tmp3[9];
+            cwc(G_ini, t[0], m1);
+            for (int i = 0; i < 9; ++i) dkdr[0][i] = -qop[0] * m1[i];
+            for (int i = 0; i < 9; ++i) tmp0[i] = I[i] + h2 * 0.125 * dkdr[0][i];
+            mat3_mul(G_mid, tmp0, m2);  // dB_mid/dr1
+            for (int n = 1; n <= 2; ++n) {
+                double hd[9];
+                for (int i = 0; i < 9; ++i) hd[i] = half_h * dkdr[n - 1][i];
+                cwc(hd, b2, m1);
+                double m3[9];
+                cwc(m2, t[n], m3);
+                for (int i = 0; i < 9; ++i) dkdr[n][i] = qop[n] * m1[i] - qop[n] * m3[i];
+            }
+            {
+                double hd[9];
+                for (int i = 0; i < 9; ++i) hd[i] = h * dkdr[2][i];
+                cwc(hd, b3, m1);
+                for (int i = 0; i < 9; ++i) tmp3[i] = I[i] + h2 * 0.5 * dkdr[2][i];
+                double gb[9], m3[9];
+                mat3_mul(G_fin, tmp3, gb);
+                cwc(gb, t[3], m3);
+                for (int i = 0; i < 9; ++i) dkdr[3][i] = qop[3] * m1[i] - qop[3] * m3[i];
+            }
+            for (int i = 0; i < 9; ++i) {
+                dFdr[i] = I[i] + h * h_6 * (dkdr[0][i] + dkdr[1][i] + dkdr[2][i]);
+                dGdr[i] = h_6 * (dkdr[0][i] + 2.0 * (dkdr[1][i] + dkdr[2][i]) + dkdr[3][i]);
+            }
+        }
         // D (8x8): rows/cols [pos 0-2, t 3, dir 4-6, qop 7]
         double D[64] = {0};
         for (int i = 0; i < 8; ++i) D[i * 8 + i] = 1.0;
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc) {
+                D[r * 8 + cc] = dFdr[r * 3 + cc];
+                D[(4 + r) * 8 + cc] = dGdr[r * 3 + cc];
+            }
         for (int r = 0; r < 3; ++r) {
             for (int cc = 0; cc < 3; ++cc) {
                 D[r * 8 + 4 + cc] = dFdt[r * 3 + cc];
@@ -1348,7 +1417,8 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
     if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
-    if (cfg->use_field_gradient || cfg->use_eloss_gradient) return DTR_ERROR_UNSUPPORTED;
+    // use_eloss_gradient: d(dqop/ds)/dqop is zero without volume material
+    // (rk_stepper.hpp:193-200), which these detectors do not have
     if (!in->bound_params && !in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
     if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
     PropCfg pc;
@@ -1358,6 +1428,7 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
     pc.path_limit = cfg->path_limit;
     pc.max_rk_updates = cfg->max_rk_updates;
     pc.do_cov = cfg->do_covariance_transport != 0;
+    pc.use_field_gradient = cfg->use_field_gradient != 0;
     pc.nav.min_mask_tol = cfg->min_mask_tolerance;
     pc.nav.max_mask_tol = cfg->max_mask_tolerance;
     pc.nav.mask_scalor = cfg->mask_tolerance_scalor;
