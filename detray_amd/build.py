@@ -56,7 +56,7 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_library(out: Path = LIB) -> Path:
-    srcs = [CSRC / "capi.hip", CSRC / "host" / "builders.cpp"]
+    srcs = [CSRC / "capi.hip", CSRC / "host" / "builders.cpp", CSRC / "host" / "json_io.cpp"]
     tmp = out.with_suffix(".so.tmp")
     cmd = [_hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, srcs), "-o", str(tmp)]
     _run(cmd)

@@ -22,6 +22,7 @@ STATUS_NAMES = {
     3: "DTR_ERROR_OUT_OF_MEMORY",
     4: "DTR_ERROR_NO_DEVICE",
     5: "DTR_ERROR_UNSUPPORTED",
+    6: "DTR_ERROR_IO",
 }
 
 # dtr_track_status
@@ -312,6 +313,10 @@ _SIGNATURES = {
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),
+    "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(DetectorDesc),
+                                         C.c_char_p, C.c_size_t]),
+    "dtr_write_detector_json": (C.c_int, [C.POINTER(DetectorDesc), C.c_char_p, C.POINTER(C.c_char_p),
+                                          C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
     "dtr_sample_solenoid": (C.c_int, [C.c_double, C.c_double, C.c_double,
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_float)]),

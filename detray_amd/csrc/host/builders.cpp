@@ -12,6 +12,7 @@
 // literals are converted the same way (float -> double).
 
 #include "detray_amd.h"
+#include "host/tables.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -24,16 +25,9 @@
 
 namespace {
 
-constexpr double kPi = M_PI;
-constexpr uint16_t kEndOfWorld = 0xffffu;
-constexpr uint32_t kInvalid = 0xffffffffu;
+using namespace dtr_host;
 
-// detray unit conventions (core/include/detray/definitions/units.hpp)
-constexpr double kMM = 1.0;
-constexpr double kEV = 1e-9;
-constexpr double kGram = 1.0 / 1.782662e-24;
-constexpr double kCm3 = 1e3;
-constexpr double kMol = 1.0;
+constexpr double kPi = M_PI;
 
 inline double F(double x) { return static_cast<double>(static_cast<float>(x)); }
 
@@ -94,20 +88,7 @@ Material beryllium_tml() {
 }
 
 dtr_material_slab make_slab(const Material& m, double thickness) {
-    dtr_material_slab s{};
-    s.X0 = m.X0;
-    s.L0 = m.L0;
-    s.A = m.A;
-    s.Z = m.Z;
-    s.mass_rho = m.rho;
-    // material.hpp:226-234 mass_to_molar_density
-    const double molar_mass = m.A * kGram / kMol;
-    s.molar_rho = (m.rho == 0.0) ? 0.0 : m.rho / molar_mass;
-    // material.hpp: mean_excitation_energy without density-effect data
-    s.mean_exc = 16.0 * kEV * std::pow(m.Z, 0.9);
-    s.thickness = thickness;
-    s.thickness_in_X0 = thickness / m.X0;
-    return s;
+    return dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho, thickness);
 }
 
 // A mask as the builders create it: a shape and its boundary values
@@ -282,22 +263,6 @@ struct Extent {
 // pushes its placement transform, then the transforms of its surfaces
 // (volume_builder.hpp build/add_to_detector).
 // ---------------------------------------------------------------------------
-struct Tables {
-    std::vector<dtr_volume> volumes;
-    std::vector<dtr_surface> surfaces;
-    std::vector<dtr_transform> transforms;
-    std::vector<dtr_mask> masks;
-    std::vector<dtr_material_slab> materials;
-    std::vector<dtr_grid> grids;
-    std::vector<uint32_t> bf;
-    std::vector<uint32_t> bins;
-};
-
-uint64_t make_barcode(uint64_t vol, uint64_t id, uint64_t index, uint64_t trf) {
-    return ((vol & 0xfffull) << 52) | ((id & 0xfull) << 48) |
-           ((index & 0x1fffffull) << 27) | ((trf & 0x1fffffull) << 6);
-}
-
 void assemble(const std::vector<VolumeDef>& vols,
               const std::vector<dtr_material_slab>& mats, Tables& T) {
     T.materials = mats;
@@ -410,33 +375,6 @@ void assemble(const std::vector<VolumeDef>& vols,
     }
 }
 
-template <typename T>
-T* dup(const std::vector<T>& v) {
-    if (v.empty()) return nullptr;
-    T* p = static_cast<T*>(std::malloc(sizeof(T) * v.size()));
-    if (p) std::memcpy(p, v.data(), sizeof(T) * v.size());
-    return p;
-}
-
-void export_tables(const Tables& T, dtr_detector_desc* out) {
-    std::memset(out, 0, sizeof(*out));
-    out->n_volumes = static_cast<uint32_t>(T.volumes.size());
-    out->n_surfaces = static_cast<uint32_t>(T.surfaces.size());
-    out->n_transforms = static_cast<uint32_t>(T.transforms.size());
-    out->n_masks = static_cast<uint32_t>(T.masks.size());
-    out->n_materials = static_cast<uint32_t>(T.materials.size());
-    out->n_grids = static_cast<uint32_t>(T.grids.size());
-    out->n_bf = static_cast<uint32_t>(T.bf.size());
-    out->n_bins = static_cast<uint32_t>(T.bins.size());
-    out->volumes = dup(T.volumes);
-    out->surfaces = dup(T.surfaces);
-    out->transforms = dup(T.transforms);
-    out->masks = dup(T.masks);
-    out->materials = dup(T.materials);
-    out->grids = dup(T.grids);
-    out->bf_surfaces = dup(T.bf);
-    out->grid_bins = dup(T.bins);
-}
 
 // ---------------------------------------------------------------------------
 // Toy detector (build_toy_detector.hpp)

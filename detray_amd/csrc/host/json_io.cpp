@@ -1,0 +1,853 @@
+// Detector tables from and to detray's JSON detector files: geometry,
+// surface grids and homogeneous material.
+//
+// Format: the payloads of io/include/detray/io/frontend/payloads.hpp as
+// serialised by io/include/detray/io/json/detail/json_{common,algebra,
+// geometry,grids,material}_io.hpp; each file is {"header": ..., "data": ...}
+// (io/include/detray/io/json/json_converter.hpp).
+//
+// Reading follows what the reference's readers and builders make of a file
+// (io/include/detray/io/backend/geometry_reader.hpp, surface_grid_reader.hpp,
+// detail/grid_reader.hpp, homogeneous_material_reader.hpp with
+// core/include/detray/builders/volume_builder.hpp, surface_factory.hpp,
+// grid_builder.hpp, homogeneous_material_factory.hpp,
+// homogeneous_material_builder.hpp):
+//  * volumes are built in file order (the "index" field is not used);
+//  * a surface with "index_in_coll" is placed at that position of its
+//    volume; without it, surfaces are grouped into factories, portals first,
+//    then sensitive and passive surfaces, each group ordered by shape id
+//    (cylinder2 counts as portal_cylinder2), file order within a group;
+//  * the placement transform of a volume is followed by the transforms of
+//    its surfaces; the surface identifier is rebuilt from volume, type,
+//    index and transform;
+//  * a cylinder mask takes the z translation of its transform into its
+//    z bounds and the transform becomes the identity;
+//  * grid bins hold volume-local surface indices; a volume with a grid keeps
+//    only its portals and passive surfaces in the brute-force list;
+//  * homogeneous material: entry j of a volume's list goes to the surface
+//    at (smallest surface_idx + j), one slab per surface, placed at
+//    "index_in_coll" in the volume's slab list; volumes append their slabs in
+//    volume order.
+// What the tables cannot hold (other shapes, irregular or open grid axes,
+// several grids per volume, several surfaces per bin, material rods) is
+// rejected with DTR_ERROR_UNSUPPORTED; inconsistent files (gaps or
+// duplicates in index_in_coll, out-of-range links) with
+// DTR_ERROR_INVALID_ARGUMENT, where the reference would leave holes or
+// overwrite entries.
+//
+// Writing produces the files the reference's writers produce for the same
+// detector (geometry_writer.hpp, surface_grid_writer.hpp with
+// detail/grid_writer.hpp, homogeneous_material_writer.hpp): every surface
+// carries its index_in_coll, every material surface its own slab.
+
+#include "detray_amd.h"
+#include "host/json.hpp"
+#include "host/tables.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+using dtr_json::Value;
+using namespace dtr_host;
+
+struct IoError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void bad(const std::string& m) { throw IoError{DTR_ERROR_INVALID_ARGUMENT, m}; }
+[[noreturn]] void unsupported(const std::string& m) { throw IoError{DTR_ERROR_UNSUPPORTED, m}; }
+
+// io::shape_id (io/include/detray/io/frontend/definitions.hpp)
+constexpr unsigned kIoCylinder2 = 2u, kIoPortalCylinder2 = 4u, kIoRectangle2 = 5u, kIoRing2 = 6u,
+                   kIoTrapezoid2 = 7u, kIoShapes = 13u;
+const char* const kIoShapeNames[kIoShapes] = {
+    "annulus2", "cuboid3", "cylinder2", "cylinder3", "portal_cylinder2", "rectangle2", "ring2",
+    "trapezoid2", "drift_cell", "straw_tube", "single1", "single2", "single3"};
+// io::material_id
+constexpr unsigned kIoSlab = 6u, kIoRod = 7u;
+// io::accel_id
+constexpr unsigned kIoPolar2Grid = 3u, kIoConcCyl2Grid = 4u, kIoAccels = 7u;
+// axis::bounds, axis::binning, axis::label (core/include/detray/definitions/grid_axis.hpp)
+constexpr unsigned kClosed = 1u, kCircular = 2u, kRegular = 0u;
+// volume_id (core/include/detray/definitions/geometry.hpp)
+constexpr unsigned kVolCylinder = 0u, kVolCuboid = 4u;
+
+// ---------------------------------------------------------------------------
+// Field access
+// ---------------------------------------------------------------------------
+const Value& need(const Value& obj, const char* key, const std::string& where) {
+    const Value* v = obj.find(key);
+    if (!v) bad(where + ": missing \"" + key + "\"");
+    return *v;
+}
+
+uint64_t as_index(const Value& v, const std::string& where) {
+    if (v.kind != Value::UInt) bad(where + ": expected a non-negative integer");
+    return v.u;
+}
+
+double as_number(const Value& v, const std::string& where) {
+    if (!v.is_number()) bad(where + ": expected a number");
+    return v.d;
+}
+
+const std::vector<Value>& as_array(const Value& v, const std::string& where) {
+    if (v.kind != Value::Array) bad(where + ": expected an array");
+    return v.a;
+}
+
+void numbers(const Value& v, size_t n, double* out, const std::string& where) {
+    const auto& a = as_array(v, where);
+    if (a.size() != n) bad(where + ": expected " + std::to_string(n) + " numbers");
+    for (size_t k = 0; k < n; ++k) out[k] = as_number(a[k], where);
+}
+
+// transform_payload: translation, rotation as the x, y, z axes (column-major)
+dtr_transform read_transform(const Value& t, const std::string& where) {
+    double tr[3], rot[9];
+    numbers(need(t, "translation", where), 3, tr, where + ".translation");
+    numbers(need(t, "rotation", where), 9, rot, where + ".rotation");
+    dtr_transform out{};
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) out.r[row * 3 + col] = rot[col * 3 + row];
+    for (int k = 0; k < 3; ++k) out.t[k] = tr[k];
+    return out;
+}
+
+Value write_transform(const dtr_transform& t) {
+    Value v = Value::object();
+    Value tr = Value::array();
+    for (int k = 0; k < 3; ++k) tr.push(Value::number(t.t[k]));
+    Value rot = Value::array();
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) rot.push(Value::number(t.r[row * 3 + col]));
+    v.set("translation", std::move(tr));
+    v.set("rotation", std::move(rot));
+    return v;
+}
+
+dtr_transform identity() {
+    dtr_transform t{};
+    t.r[0] = t.r[4] = t.r[8] = 1.0;
+    return t;
+}
+
+Value load(const char* path, const char* what) {
+    std::ifstream f(path, std::ios::in | std::ios::binary);
+    if (!f) throw IoError{DTR_ERROR_IO, std::string("cannot open ") + what + " file " + path};
+    std::stringstream ss;
+    ss << f.rdbuf();
+    if (f.bad()) throw IoError{DTR_ERROR_IO, std::string("cannot read ") + what + " file " + path};
+    try {
+        return dtr_json::parse(ss.str());
+    } catch (const dtr_json::Error& e) {
+        throw IoError{DTR_ERROR_IO, std::string(what) + " file " + path + ": " + e.what()};
+    }
+}
+
+void save(const char* path, const Value& v, const char* what) {
+    std::ofstream f(path, std::ios::out | std::ios::binary | std::ios::trunc);
+    if (!f) throw IoError{DTR_ERROR_IO, std::string("cannot create ") + what + " file " + path};
+    const std::string text = dtr_json::dump(v);
+    f.write(text.data(), static_cast<std::streamsize>(text.size()));
+    if (!f) throw IoError{DTR_ERROR_IO, std::string("cannot write ") + what + " file " + path};
+}
+
+// The "data" section; null (what an empty payload serialises to) or absent
+// means no entries.
+const Value* data_of(const Value& root, const char* what) {
+    if (root.kind != Value::Object) bad(std::string(what) + " file: not a JSON object");
+    const Value* d = root.find("data");
+    if (!d || d->kind == Value::Null) return nullptr;
+    if (d->kind != Value::Object) bad(std::string(what) + " file: \"data\" is not an object");
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// Reading
+// ---------------------------------------------------------------------------
+struct SurfaceIn {
+    uint8_t sf_id = 0;
+    uint8_t shape = 0;     // dtr_mask_shape
+    unsigned io_key = 0;   // factory key: io shape id, cylinder2 -> portal_cylinder2
+    dtr_transform trf{};
+    std::vector<std::array<double, 4>> bounds;
+    std::vector<uint16_t> links;
+    bool has_index = false;
+    uint64_t index = 0;
+};
+
+struct VolumeIn {
+    dtr_transform placement{};
+    std::vector<SurfaceIn> surfaces;  // final volume-local order
+    bool has_grid = false;
+    dtr_grid grid{};
+    std::vector<uint32_t> bins;       // volume-local surface index per bin
+    std::vector<int> slab_of;         // per surface: position in `slabs`, -1 none
+    std::vector<dtr_material_slab> slabs;
+    std::vector<char> slab_set;
+};
+
+SurfaceIn read_surface(const Value& s, const std::string& where) {
+    SurfaceIn out;
+    const uint64_t type = as_index(need(s, "type", where), where + ".type");
+    if (type > 2u) bad(where + ": surface type " + std::to_string(type) + " is not portal/sensitive/passive");
+    out.sf_id = static_cast<uint8_t>(type);
+    out.trf = read_transform(need(s, "transform", where), where + ".transform");
+    std::vector<const Value*> masks;
+    if (const Value* m = s.find("mask")) {
+        masks.push_back(m);  // single-mask files
+    } else if (const Value* ms = s.find("masks")) {
+        for (const Value& m : as_array(*ms, where + ".masks")) masks.push_back(&m);
+    }
+    if (masks.empty()) bad(where + ": surface without mask");
+    if (masks.size() > 255u) unsupported(where + ": more than 255 masks on one surface");
+    // the factory is chosen by the first mask's shape; all masks take it
+    const uint64_t sid = as_index(need(*masks[0], "shape", where), where + ".shape");
+    if (sid >= kIoShapes) bad(where + ": unknown mask shape id " + std::to_string(sid));
+    size_t n_bounds = 0;
+    switch (sid) {
+        case kIoCylinder2:
+        case kIoPortalCylinder2:
+            out.shape = DTR_CONCENTRIC_CYLINDER2D;
+            out.io_key = kIoPortalCylinder2;
+            n_bounds = 3;
+            break;
+        case kIoRectangle2:
+            out.shape = DTR_RECTANGLE2D;
+            out.io_key = kIoRectangle2;
+            n_bounds = 2;
+            break;
+        case kIoRing2:
+            out.shape = DTR_RING2D;
+            out.io_key = kIoRing2;
+            n_bounds = 2;
+            break;
+        case kIoTrapezoid2:
+            out.shape = DTR_TRAPEZOID2D;
+            out.io_key = kIoTrapezoid2;
+            n_bounds = 4;
+            break;
+        default:
+            unsupported(where + ": mask shape " + kIoShapeNames[sid] +
+                        " (the tables hold rectangle2, trapezoid2, ring2 and cylinder2 masks)");
+    }
+    for (size_t k = 0; k < masks.size(); ++k) {
+        const std::string mw = where + ".masks[" + std::to_string(k) + "]";
+        std::array<double, 4> b{};
+        numbers(need(*masks[k], "boundaries", mw), n_bounds, b.data(), mw + ".boundaries");
+        out.bounds.push_back(b);
+        // the navigation link is a 16-bit value: the world exit 2^64-1 -> 0xffff
+        out.links.push_back(static_cast<uint16_t>(as_index(need(*masks[k], "volume_link", mw), mw + ".volume_link")));
+    }
+    if (out.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        const double dz = out.trf.t[2];
+        for (auto& b : out.bounds) {
+            b[1] += dz;
+            b[2] += dz;
+        }
+        out.trf = identity();
+    }
+    if (const Value* ic = s.find("index_in_coll")) {
+        out.has_index = true;
+        out.index = as_index(*ic, where + ".index_in_coll");
+    }
+    return out;
+}
+
+void read_geometry(const Value& root, std::vector<VolumeIn>& vols) {
+    const Value* data = data_of(root, "geometry");
+    const Value* jv = data ? data->find("volumes") : nullptr;
+    if (!jv || as_array(*jv, "volumes").empty()) bad("geometry file: no volumes");
+    for (size_t vi = 0; vi < jv->a.size(); ++vi) {
+        const Value& v = jv->a[vi];
+        const std::string where = "volume " + std::to_string(vi);
+        VolumeIn V;
+        V.placement = read_transform(need(v, "transform", where), where + ".transform");
+        const auto& js = as_array(need(v, "surfaces", where), where + ".surfaces");
+        if (js.empty()) bad(where + ": volume without surfaces");
+        std::vector<SurfaceIn> in;
+        in.reserve(js.size());
+        for (size_t k = 0; k < js.size(); ++k)
+            in.push_back(read_surface(js[k], where + " surface " + std::to_string(k)));
+        const size_t n = in.size();
+        size_t n_indexed = 0;
+        for (const auto& s : in) n_indexed += s.has_index ? 1u : 0u;
+        if (n_indexed == n && n > 0) {
+            V.surfaces.resize(n);
+            std::vector<char> seen(n, 0);
+            for (const auto& s : in) {
+                if (s.index >= n || seen[s.index])
+                    bad(where + ": index_in_coll values are not a permutation of 0.." + std::to_string(n - 1));
+                seen[s.index] = 1;
+                V.surfaces[s.index] = s;
+            }
+        } else if (n_indexed == 0) {
+            // factories: portals first, then the others, each by shape key
+            for (int portals = 1; portals >= 0; --portals) {
+                for (unsigned key = 0; key < kIoShapes; ++key) {
+                    for (const auto& s : in) {
+                        if ((s.sf_id == DTR_PORTAL) == (portals == 1) && s.io_key == key) V.surfaces.push_back(s);
+                    }
+                }
+            }
+        } else {
+            bad(where + ": index_in_coll given for some surfaces only");
+        }
+        V.slab_of.assign(V.surfaces.size(), -1);
+        vols.push_back(std::move(V));
+    }
+}
+
+void read_grids(const Value& root, std::vector<VolumeIn>& vols) {
+    const Value* data = data_of(root, "grids");
+    const Value* jg = data ? data->find("grids") : nullptr;
+    if (!jg) return;
+    for (const Value& entry : as_array(*jg, "grids")) {
+        const auto& list = as_array(need(entry, "grid_data", "grids entry"), "grid_data");
+        for (const Value& g : list) {
+            const uint64_t vi = as_index(need(g, "owner_link", "grid"), "grid.owner_link");
+            const std::string where = "grid of volume " + std::to_string(vi);
+            if (vi >= vols.size()) bad(where + ": the volume is not in the geometry");
+            VolumeIn& V = vols[vi];
+            const Value& link = need(g, "grid_link", where);
+            const uint64_t type = as_index(need(link, "type", where + ".grid_link"), where + ".grid_link.type");
+            if (type >= kIoAccels) bad(where + ": unknown grid type id " + std::to_string(type));
+            dtr_grid gr{};
+            unsigned want0, want1;
+            if (type == kIoConcCyl2Grid) {
+                gr.type = DTR_GRID_CYLINDER;
+                want0 = kCircular;
+                want1 = kClosed;
+            } else if (type == kIoPolar2Grid) {
+                gr.type = DTR_GRID_RING;
+                want0 = kClosed;
+                want1 = kCircular;
+            } else {
+                unsupported(where + ": grid type " + std::to_string(type) +
+                            " (the tables hold concentric_cylinder2 and polar2 grids)");
+            }
+            if (V.has_grid) unsupported(where + ": more than one surface grid in a volume");
+            const auto& axes = as_array(need(g, "axes", where), where + ".axes");
+            if (axes.size() != 2u) bad(where + ": a 2D grid needs 2 axes");
+            uint32_t nb[2];
+            double lo[2], hi[2];
+            for (int k = 0; k < 2; ++k) {
+                const std::string aw = where + ".axes[" + std::to_string(k) + "]";
+                const uint64_t binning = as_index(need(axes[k], "binning", aw), aw + ".binning");
+                const uint64_t bounds = as_index(need(axes[k], "bounds", aw), aw + ".bounds");
+                if (binning != kRegular) unsupported(aw + ": irregular binning");
+                if (bounds != (k == 0 ? want0 : want1))
+                    unsupported(aw + ": axis bounds " + std::to_string(bounds) + " (the tables hold " +
+                                (gr.type == DTR_GRID_CYLINDER ? "circular phi x closed z" : "closed r x circular phi") + ")");
+                const uint64_t bins = as_index(need(axes[k], "bins", aw), aw + ".bins");
+                if (bins == 0u || bins > (1u << 20)) bad(aw + ": bad number of bins");
+                const auto& edges = as_array(need(axes[k], "edges", aw), aw + ".edges");
+                if (edges.size() < 2u) bad(aw + ": an axis span needs 2 edges");
+                nb[k] = static_cast<uint32_t>(bins);
+                lo[k] = as_number(edges.front(), aw + ".edges");
+                hi[k] = as_number(edges.back(), aw + ".edges");
+            }
+            gr.n0 = nb[0];
+            gr.n1 = nb[1];
+            gr.min0 = lo[0];
+            gr.max0 = hi[0];
+            gr.min1 = lo[1];
+            gr.max1 = hi[1];
+            if (static_cast<uint64_t>(gr.n0) * gr.n1 > (1ull << 26)) unsupported(where + ": more than 2^26 bins");
+            V.bins.assign(static_cast<size_t>(gr.n0) * gr.n1, kInvalid);
+            for (const Value& bin : as_array(need(g, "bins", where), where + ".bins")) {
+                const auto& li = as_array(need(bin, "loc_index", where), where + ".loc_index");
+                if (li.size() != 2u) bad(where + ": a bin needs 2 local indices");
+                const uint64_t i0 = as_index(li[0], where + ".loc_index"), i1 = as_index(li[1], where + ".loc_index");
+                if (i0 >= gr.n0 || i1 >= gr.n1)
+                    bad(where + ": bin (" + std::to_string(i0) + ", " + std::to_string(i1) + ") out of bounds");
+                uint32_t& slot = V.bins[static_cast<size_t>(i0) * gr.n1 + i1];
+                for (const Value& c : as_array(need(bin, "content", where), where + ".content")) {
+                    const uint64_t sf = as_index(c, where + ".content");
+                    if (static_cast<uint32_t>(sf) == kInvalid) continue;  // invalid entries are skipped
+                    if (sf >= V.surfaces.size()) bad(where + ": surface " + std::to_string(sf) + " not in the volume");
+                    if (slot != kInvalid)
+                        unsupported(where + ": more than one surface in bin (" + std::to_string(i0) + ", " +
+                                    std::to_string(i1) + "); the tables hold one per bin");
+                    slot = static_cast<uint32_t>(sf);
+                }
+            }
+            V.has_grid = true;
+            V.grid = gr;
+        }
+    }
+}
+
+void read_material(const Value& root, std::vector<VolumeIn>& vols) {
+    const Value* data = data_of(root, "material");
+    const Value* jv = data ? data->find("volumes") : nullptr;
+    if (!jv) return;
+    for (const Value& mv : as_array(*jv, "material volumes")) {
+        const uint64_t vi = as_index(need(mv, "volume_link", "material volume"), "material volume_link");
+        const std::string where = "material of volume " + std::to_string(vi);
+        if (vi >= vols.size()) bad(where + ": the volume is not in the geometry");
+        VolumeIn& V = vols[vi];
+        const Value* list = mv.find("surface_material");
+        if (!list) list = mv.find("material_slabs");  // older files
+        if (!list) continue;
+        const auto& entries = as_array(*list, where);
+        const size_t n = entries.size();
+        if (n == 0) continue;
+        struct Entry {
+            uint64_t sf;
+            bool has_index;
+            uint64_t index;
+            dtr_material_slab slab;
+        };
+        std::vector<Entry> es;
+        for (size_t k = 0; k < n; ++k) {
+            const std::string ew = where + " entry " + std::to_string(k);
+            const Value& e = entries[k];
+            const uint64_t type = as_index(need(e, "type", ew), ew + ".type");
+            if (type == kIoRod) unsupported(ew + ": material rods");
+            if (type != kIoSlab) bad(ew + ": material type " + std::to_string(type) + " is not homogeneous");
+            double p[7];
+            numbers(need(need(e, "material", ew), "params", ew), 7, p, ew + ".material.params");
+            const double t = as_number(need(e, "thickness", ew), ew + ".thickness");
+            Entry en{as_index(need(e, "surface_idx", ew), ew + ".surface_idx"), false, 0,
+                     make_slab(p[0], p[1], p[2], p[3], p[4], t)};
+            if (const Value* ic = e.find("index_in_coll")) {
+                en.has_index = true;
+                en.index = as_index(*ic, ew + ".index_in_coll");
+            }
+            es.push_back(en);
+        }
+        // pairing by offset from the smallest surface index
+        std::vector<uint64_t> idx(n);
+        for (size_t k = 0; k < n; ++k) idx[k] = es[k].sf;
+        std::vector<uint64_t> sorted = idx;
+        std::sort(sorted.begin(), sorted.end());
+        const uint64_t first = sorted.front();
+        for (size_t k = 0; k < n; ++k)
+            if (sorted[k] != first + k) bad(where + ": surface_idx values are not a contiguous range");
+        if (first + n > V.surfaces.size()) bad(where + ": surface_idx beyond the volume's surfaces");
+        for (size_t k = 0; k < n; ++k) {
+            const uint64_t sf = idx[k];
+            if (V.slab_of[sf] >= 0) continue;  // already has material: kept
+            const Entry& en = es[sf - first];
+            size_t pos;
+            if (en.has_index) {
+                pos = static_cast<size_t>(en.index);
+                if (pos > (1u << 24)) bad(where + ": index_in_coll out of range");
+                if (V.slabs.size() <= pos) {
+                    V.slabs.resize(pos + 1);
+                    V.slab_set.resize(pos + 1, 0);
+                }
+                if (V.slab_set[pos]) bad(where + ": two slabs at index_in_coll " + std::to_string(pos));
+            } else {
+                pos = V.slabs.size();
+                V.slabs.emplace_back();
+                V.slab_set.push_back(0);
+            }
+            V.slabs[pos] = en.slab;
+            V.slab_set[pos] = 1;
+            V.slab_of[sf] = static_cast<int>(pos);
+        }
+    }
+    for (size_t vi = 0; vi < vols.size(); ++vi)
+        for (size_t k = 0; k < vols[vi].slab_set.size(); ++k)
+            if (!vols[vi].slab_set[k])
+                bad("material of volume " + std::to_string(vi) + ": no slab at index_in_coll " + std::to_string(k));
+}
+
+void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
+    if (vols.size() > 4096u) unsupported("more than 4096 volumes");
+    for (uint32_t vi = 0; vi < vols.size(); ++vi) {
+        const VolumeIn& V = vols[vi];
+        dtr_volume dv{};
+        dv.transform = static_cast<uint32_t>(T.transforms.size());
+        T.transforms.push_back(V.placement);
+        dv.sf_begin = static_cast<uint32_t>(T.surfaces.size());
+        const uint32_t mat_offset = static_cast<uint32_t>(T.materials.size());
+        for (uint32_t k = 0; k < V.surfaces.size(); ++k) {
+            const SurfaceIn& S = V.surfaces[k];
+            dtr_surface ds{};
+            const uint32_t idx = static_cast<uint32_t>(T.surfaces.size());
+            ds.transform = static_cast<uint32_t>(T.transforms.size());
+            T.transforms.push_back(S.trf);
+            ds.mask = static_cast<uint32_t>(T.masks.size());
+            ds.n_masks = static_cast<uint8_t>(S.bounds.size());
+            ds.shape = S.shape;
+            for (size_t m = 0; m < S.bounds.size(); ++m) {
+                dtr_mask dm{};
+                for (int q = 0; q < 4; ++q) dm.v[q] = S.bounds[m][q];
+                dm.volume_link = S.links[m];
+                dm.shape = S.shape;
+                T.masks.push_back(dm);
+            }
+            ds.material = V.slab_of[k] >= 0 ? mat_offset + static_cast<uint32_t>(V.slab_of[k]) : kInvalid;
+            ds.volume = static_cast<uint16_t>(vi);
+            ds.sf_id = S.sf_id;
+            ds.barcode = make_barcode(vi, S.sf_id, idx, ds.transform);
+            T.surfaces.push_back(ds);
+        }
+        if (T.surfaces.size() >= (1u << 21) || T.transforms.size() >= (1u << 21))
+            unsupported("more than 2^21 surfaces or transforms");
+        T.materials.insert(T.materials.end(), V.slabs.begin(), V.slabs.end());
+        dv.sf_end = static_cast<uint32_t>(T.surfaces.size());
+        dv.bf_begin = static_cast<uint32_t>(T.bf.size());
+        for (uint32_t s = dv.sf_begin; s < dv.sf_end; ++s)
+            if (!V.has_grid || T.surfaces[s].sf_id != DTR_SENSITIVE) T.bf.push_back(s);
+        dv.bf_end = static_cast<uint32_t>(T.bf.size());
+        dv.grid = -1;
+        if (V.has_grid) {
+            dtr_grid g = V.grid;
+            g.bins = static_cast<uint32_t>(T.bins.size());
+            for (uint32_t b : V.bins) T.bins.push_back(b == kInvalid ? kInvalid : dv.sf_begin + b);
+            // grid_builder.hpp:232-272: reference radius between the portal cylinders
+            if (g.type == DTR_GRID_CYLINDER) {
+                double rmin = 0.0, rmax = 0.0;
+                bool any = false;
+                for (uint32_t s = dv.sf_begin; s < dv.sf_end; ++s) {
+                    const dtr_surface& ds = T.surfaces[s];
+                    if (ds.sf_id != DTR_PORTAL || ds.shape != DTR_CONCENTRIC_CYLINDER2D) continue;
+                    const double r = T.masks[ds.mask].v[0];
+                    rmin = any ? std::min(rmin, r) : r;
+                    rmax = any ? std::max(rmax, r) : r;
+                    any = true;
+                }
+                g.radius = any ? 0.5 * (rmin + rmax) : 0.0;
+            }
+            dv.grid = static_cast<int32_t>(T.grids.size());
+            T.grids.push_back(g);
+        }
+        T.volumes.push_back(dv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Writing
+// ---------------------------------------------------------------------------
+void check_desc(const dtr_detector_desc& d) {
+    if (d.n_volumes == 0u || !d.volumes || !d.surfaces || !d.transforms || !d.masks)
+        bad("detector tables are empty");
+    uint32_t next = 0;
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        if (v.sf_begin != next || v.sf_end < v.sf_begin || v.sf_end > d.n_surfaces)
+            bad("volume " + std::to_string(vi) + ": surface range does not follow the previous volume");
+        if (v.transform >= d.n_transforms) bad("volume " + std::to_string(vi) + ": transform out of range");
+        for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
+            const dtr_surface& ds = d.surfaces[s];
+            if (ds.volume != vi) bad("surface " + std::to_string(s) + ": volume link does not match its range");
+            if (ds.transform >= d.n_transforms || ds.n_masks == 0u ||
+                static_cast<uint64_t>(ds.mask) + ds.n_masks > d.n_masks)
+                bad("surface " + std::to_string(s) + ": transform or mask out of range");
+            if (ds.material != kInvalid && (ds.material >= d.n_materials || !d.materials))
+                bad("surface " + std::to_string(s) + ": material out of range");
+        }
+        if (v.grid >= 0) {
+            if (static_cast<uint32_t>(v.grid) >= d.n_grids || !d.grids)
+                bad("volume " + std::to_string(vi) + ": grid out of range");
+            const dtr_grid& g = d.grids[v.grid];
+            if (static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 > d.n_bins)
+                bad("volume " + std::to_string(vi) + ": grid bins out of range");
+        }
+        next = v.sf_end;
+    }
+    if (next != d.n_surfaces) bad("surfaces not covered by the volume ranges");
+}
+
+unsigned io_shape(uint8_t shape) {
+    switch (shape) {
+        case DTR_RECTANGLE2D: return kIoRectangle2;
+        case DTR_TRAPEZOID2D: return kIoTrapezoid2;
+        case DTR_CONCENTRIC_CYLINDER2D: return kIoPortalCylinder2;
+        case DTR_RING2D: return kIoRing2;
+        default: bad("unknown mask shape " + std::to_string(shape));
+    }
+}
+
+size_t n_bounds(uint8_t shape) {
+    switch (shape) {
+        case DTR_TRAPEZOID2D: return 4;
+        case DTR_CONCENTRIC_CYLINDER2D: return 3;
+        default: return 2;
+    }
+}
+
+std::string today() {
+    const std::time_t now = std::time(nullptr);
+    std::tm tm{};
+    localtime_r(&now, &tm);
+    char buf[64];
+    std::strftime(buf, sizeof(buf), "%d/%m/%Y %H:%M", &tm);
+    return buf;
+}
+
+Value header_common(const char* name, const char* tag) {
+    Value c = Value::object();
+    c.set("version", Value::string("detray_amd"));
+    c.set("detector", Value::string(name ? name : ""));
+    c.set("date", Value::string(today()));
+    c.set("tag", Value::string(tag));
+    return c;
+}
+
+// Per-type grid ordinal (the index of a grid in the accelerator collection
+// of its type)
+std::vector<uint32_t> grid_ordinals(const dtr_detector_desc& d) {
+    std::vector<uint32_t> ord(d.n_grids, 0);
+    uint32_t count[2] = {0, 0};
+    for (uint32_t g = 0; g < d.n_grids; ++g) ord[g] = count[d.grids[g].type == DTR_GRID_RING ? 1 : 0]++;
+    return ord;
+}
+
+unsigned io_grid(uint8_t type) { return type == DTR_GRID_RING ? kIoPolar2Grid : kIoConcCyl2Grid; }
+
+Value write_geometry(const dtr_detector_desc& d, const char* name, const char* const* names) {
+    Value root = Value::object();
+    Value& h = root.set("header", Value::object());
+    h.set("common", header_common(name, "geometry"));
+    h.set("volume_count", Value::uint(d.n_volumes));
+    h.set("surface_count", Value::uint(d.n_surfaces));
+    Value& data = root.set("data", Value::object());
+    Value& jv = data.set("volumes", Value::array());
+    const std::vector<uint32_t> ord = grid_ordinals(d);
+    uint64_t slab = 0;  // one slab per material surface, in surface order
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        Value vol = Value::object();
+        vol.set("name", Value::string((names && names[vi]) ? names[vi] : ""));
+        vol.set("index", Value::uint(vi));
+        unsigned vtype = kVolCuboid;
+        for (uint32_t s = v.sf_begin; s < v.sf_end; ++s)
+            if (d.surfaces[s].sf_id == DTR_PORTAL && (d.surfaces[s].shape == DTR_CONCENTRIC_CYLINDER2D ||
+                                                      d.surfaces[s].shape == DTR_RING2D))
+                vtype = kVolCylinder;
+        vol.set("type", Value::uint(vtype));
+        vol.set("transform", write_transform(d.transforms[v.transform]));
+        Value& sfs = vol.set("surfaces", Value::array());
+        for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
+            const dtr_surface& ds = d.surfaces[s];
+            Value sf = Value::object();
+            sf.set("identifier", Value::uint(ds.barcode));
+            sf.set("type", Value::uint(ds.sf_id));
+            sf.set("source", Value::uint(0));
+            sf.set("transform", write_transform(d.transforms[ds.transform]));
+            Value& masks = sf.set("masks", Value::array());
+            for (uint32_t m = ds.mask; m < ds.mask + ds.n_masks; ++m) {
+                const dtr_mask& dm = d.masks[m];
+                Value jm = Value::object();
+                jm.set("shape", Value::uint(io_shape(ds.shape)));
+                jm.set("volume_link", Value::uint(dm.volume_link));
+                Value& b = jm.set("boundaries", Value::array());
+                for (size_t q = 0; q < n_bounds(ds.shape); ++q) b.push(Value::number(dm.v[q]));
+                masks.push(std::move(jm));
+            }
+            if (ds.material != kInvalid) {
+                Value ml = Value::object();
+                ml.set("type", Value::uint(kIoSlab));
+                ml.set("index", Value::uint(slab++));
+                sf.set("material", std::move(ml));
+            }
+            sf.set("index_in_coll", Value::uint(s - v.sf_begin));
+            sfs.push(std::move(sf));
+        }
+        if (v.grid >= 0) {
+            Value links = Value::array();
+            Value l = Value::object();
+            l.set("type", Value::uint(io_grid(d.grids[v.grid].type)));
+            l.set("index", Value::uint(ord[v.grid]));
+            links.push(std::move(l));
+            vol.set("acc_links", std::move(links));
+        }
+        jv.push(std::move(vol));
+    }
+    return root;
+}
+
+Value write_axis(unsigned label, unsigned bounds, uint32_t bins, double lo, double hi) {
+    Value a = Value::object();
+    a.set("label", Value::uint(label));
+    a.set("bounds", Value::uint(bounds));
+    a.set("binning", Value::uint(kRegular));
+    a.set("bins", Value::uint(bins));
+    Value e = Value::array();
+    e.push(Value::number(lo));
+    e.push(Value::number(hi));
+    a.set("edges", std::move(e));
+    return a;
+}
+
+Value write_grids(const dtr_detector_desc& d, const char* name) {
+    Value root = Value::object();
+    Value& h = root.set("header", Value::object());
+    h.set("common", header_common(name, "surface_grids"));
+    h.set("grid_count", Value::uint(d.n_grids));
+    const std::vector<uint32_t> ord = grid_ordinals(d);
+    Value grids = Value::array();
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        if (v.grid < 0) continue;
+        const dtr_grid& g = d.grids[v.grid];
+        Value gd = Value::object();
+        gd.set("owner_link", Value::uint(vi));
+        Value link = Value::object();
+        link.set("type", Value::uint(io_grid(g.type)));
+        link.set("index", Value::uint(ord[v.grid]));
+        gd.set("grid_link", std::move(link));
+        Value axes = Value::array();
+        const bool cyl = g.type != DTR_GRID_RING;
+        axes.push(write_axis(0u, cyl ? kCircular : kClosed, g.n0, g.min0, g.max0));
+        axes.push(write_axis(1u, cyl ? kClosed : kCircular, g.n1, g.min1, g.max1));
+        gd.set("axes", std::move(axes));
+        Value bins = Value::array();
+        // serial bin order of simple_serializer<2>: axis 0 runs fastest
+        for (uint32_t i1 = 0; i1 < g.n1; ++i1) {
+            for (uint32_t i0 = 0; i0 < g.n0; ++i0) {
+                Value b = Value::object();
+                Value li = Value::array();
+                li.push(Value::uint(i0));
+                li.push(Value::uint(i1));
+                b.set("loc_index", std::move(li));
+                Value content = Value::array();
+                const uint32_t sf = d.grid_bins[g.bins + static_cast<size_t>(i0) * g.n1 + i1];
+                if (sf != kInvalid) {
+                    if (sf < v.sf_begin || sf >= v.sf_end)
+                        bad("grid of volume " + std::to_string(vi) + ": bin holds a surface of another volume");
+                    content.push(Value::uint(sf - v.sf_begin));
+                }
+                b.set("content", std::move(content));
+                bins.push(std::move(b));
+            }
+        }
+        gd.set("bins", std::move(bins));
+        Value entry = Value::object();
+        entry.set("volume_link", Value::uint(vi));
+        Value list = Value::array();
+        list.push(std::move(gd));
+        entry.set("grid_data", std::move(list));
+        grids.push(std::move(entry));
+    }
+    if (grids.a.empty()) {
+        root.set("data", Value::null());
+    } else {
+        Value& data = root.set("data", Value::object());
+        data.set("grids", std::move(grids));
+    }
+    return root;
+}
+
+Value write_material(const dtr_detector_desc& d, const char* name) {
+    uint64_t n_slabs = 0;
+    for (uint32_t s = 0; s < d.n_surfaces; ++s) n_slabs += d.surfaces[s].material != kInvalid ? 1u : 0u;
+    Value root = Value::object();
+    Value& h = root.set("header", Value::object());
+    h.set("common", header_common(name, "homogeneous_material"));
+    h.set("slab_count", Value::uint(n_slabs));
+    h.set("rod_count", Value::uint(0));
+    h.set("slab_surface_count", Value::uint(n_slabs));
+    h.set("rod_surface_count", Value::uint(0));
+    Value& data = root.set("data", Value::object());
+    Value& jv = data.set("volumes", Value::array());
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        Value mv = Value::object();
+        mv.set("volume_link", Value::uint(vi));
+        Value list = Value::array();
+        uint64_t slab = 0;
+        for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
+            const dtr_surface& ds = d.surfaces[s];
+            if (ds.material == kInvalid) continue;
+            const dtr_material_slab& m = d.materials[ds.material];
+            Value e = Value::object();
+            e.set("type", Value::uint(kIoSlab));
+            e.set("surface_idx", Value::uint(s - v.sf_begin));
+            e.set("thickness", Value::number(m.thickness));
+            Value mat = Value::object();
+            Value p = Value::array();
+            for (double x : {m.X0, m.L0, m.A, m.Z, m.mass_rho, m.molar_rho, 0.0}) p.push(Value::number(x));
+            mat.set("params", std::move(p));
+            e.set("material", std::move(mat));
+            e.set("index_in_coll", Value::uint(slab++));
+            list.push(std::move(e));
+        }
+        if (!list.a.empty()) mv.set("surface_material", std::move(list));
+        jv.push(std::move(mv));
+    }
+    return root;
+}
+
+void message(char* out, size_t n, const std::string& m) {
+    if (!out || n == 0) return;
+    const size_t k = std::min(n - 1, m.size());
+    std::memcpy(out, m.data(), k);
+    out[k] = '\0';
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtr_read_detector_json(const char* geometry_file, const char* grids_file, const char* material_file,
+                           dtr_detector_desc* out, char* msg, size_t msg_size) {
+    message(msg, msg_size, "");
+    if (!geometry_file || !out) {
+        message(msg, msg_size, "geometry file and output tables are required");
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+    try {
+        std::vector<VolumeIn> vols;
+        read_geometry(load(geometry_file, "geometry"), vols);
+        if (grids_file) read_grids(load(grids_file, "surface grids"), vols);
+        if (material_file) read_material(load(material_file, "homogeneous material"), vols);
+        Tables T;
+        assemble(vols, T);
+        export_tables(T, out);
+        return DTR_SUCCESS;
+    } catch (const IoError& e) {
+        message(msg, msg_size, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        message(msg, msg_size, "out of host memory");
+        return DTR_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        message(msg, msg_size, e.what());
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+}
+
+int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_name,
+                            const char* const* volume_names, const char* geometry_file,
+                            const char* grids_file, const char* material_file, char* msg,
+                            size_t msg_size) {
+    message(msg, msg_size, "");
+    if (!det) {
+        message(msg, msg_size, "detector tables are required");
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+    try {
+        check_desc(*det);
+        if (geometry_file) save(geometry_file, write_geometry(*det, detector_name, volume_names), "geometry");
+        if (grids_file) save(grids_file, write_grids(*det, detector_name), "surface grids");
+        if (material_file) save(material_file, write_material(*det, detector_name), "homogeneous material");
+        return DTR_SUCCESS;
+    } catch (const IoError& e) {
+        message(msg, msg_size, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        message(msg, msg_size, "out of host memory");
+        return DTR_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        message(msg, msg_size, e.what());
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+}
+
+}  // extern "C"

@@ -21,8 +21,10 @@ hand-written HIP kernel in ``libdetray_amd.so``.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import math
 from dataclasses import dataclass, field as dc_field
+from pathlib import Path
 from typing import Optional, Sequence
 
 import numpy as np
@@ -134,6 +136,53 @@ class Detector:
         capi.check(lib.dtr_build_telescope_detector(C.byref(cfg), C.byref(d)),
                    "dtr_build_telescope_detector")
         return cls(d, f"telescope_{len(positions)}")
+
+    @classmethod
+    def from_json(cls, geometry: "os.PathLike | str", grids: "os.PathLike | str | None" = None,
+                  material: "os.PathLike | str | None" = None, name: Optional[str] = None) -> "Detector":
+        """Read detray's JSON detector files (``<name>_geometry.json`` and,
+        optionally, ``<name>_surface_grids.json`` and
+        ``<name>_homogeneous_material.json``) as the reference's
+        io::read_detector builds them (dtr_read_detector_json)."""
+        lib = capi.load()
+        d = capi.DetectorDesc()
+        msg = C.create_string_buffer(512)
+        enc = lambda p: None if p is None else os.fsencode(p)
+        code = lib.dtr_read_detector_json(enc(geometry), enc(grids), enc(material), C.byref(d), msg, len(msg))
+        if code != capi.DTR_SUCCESS:
+            raise capi.DetrayError(f"dtr_read_detector_json failed: {capi.STATUS_NAMES.get(code, code)}: "
+                                   f"{msg.value.decode(errors='replace')}")
+        if name is None:
+            stem = Path(os.fsdecode(geometry)).name
+            name = stem[:-len("_geometry.json")] if stem.endswith("_geometry.json") else Path(stem).stem
+        return cls(d, name)
+
+    def write_json(self, directory: "os.PathLike | str" = ".", name: Optional[str] = None,
+                   grids: bool = True, material: bool = True,
+                   volume_names: Optional[Sequence[str]] = None) -> dict:
+        """Write the tables as detray JSON files into ``directory``
+        (io::write_detector file names); returns {"geometry": path, ...}."""
+        lib = capi.load()
+        name = name or self.name
+        out = Path(directory)
+        files = {"geometry": out / f"{name}_geometry.json"}
+        if grids:
+            files["surface_grids"] = out / f"{name}_surface_grids.json"
+        if material:
+            files["homogeneous_material"] = out / f"{name}_homogeneous_material.json"
+        names = None
+        if volume_names is not None:
+            if len(volume_names) != self.desc.n_volumes:
+                raise ValueError("one name per volume")
+            names = (C.c_char_p * len(volume_names))(*[n.encode() for n in volume_names])
+        msg = C.create_string_buffer(512)
+        enc = lambda k: os.fsencode(files[k]) if k in files else None
+        code = lib.dtr_write_detector_json(C.byref(self.desc), name.encode(), names, enc("geometry"),
+                                           enc("surface_grids"), enc("homogeneous_material"), msg, len(msg))
+        if code != capi.DTR_SUCCESS:
+            raise capi.DetrayError(f"dtr_write_detector_json failed: {capi.STATUS_NAMES.get(code, code)}: "
+                                   f"{msg.value.decode(errors='replace')}")
+        return {k: str(v) for k, v in files.items()}
 
     # -- table views -------------------------------------------------------
     def table(self, name: str):

@@ -45,7 +45,8 @@ enum dtr_status_code {
     DTR_ERROR_HIP = 2,
     DTR_ERROR_OUT_OF_MEMORY = 3,
     DTR_ERROR_NO_DEVICE = 4,
-    DTR_ERROR_UNSUPPORTED = 5
+    DTR_ERROR_UNSUPPORTED = 5,
+    DTR_ERROR_IO = 6           /* file cannot be opened, read or written, or is not JSON */
 };
 
 /* Per-track propagation outcome (navigation::status at the end of
@@ -510,6 +511,36 @@ DTR_API int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc*
 DTR_API int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
                                  dtr_detector_desc* out);
 DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);
+
+/* Detector tables from detray's JSON detector files: the geometry file and,
+ * optionally (NULL to skip), the surface-grid and homogeneous-material
+ * files.  Replaces io::read_detector with the json geometry_reader,
+ * surface_grid_reader and homogeneous_material_reader
+ * (io/include/detray/io/frontend/detector_reader.hpp,
+ * io/include/detray/io/backend/geometry_reader.hpp:51-170,
+ * detail/grid_reader.hpp:55-420, homogeneous_material_reader.hpp:40-120):
+ * the tables are those the reference's detector builder makes of the files
+ * (volume order, surface order by index_in_coll or by portal/shape
+ * factories, cylinder z shifts folded into the masks, one slab per material
+ * surface).  Shapes, grids and materials the tables cannot hold return
+ * DTR_ERROR_UNSUPPORTED; malformed files DTR_ERROR_IO or
+ * DTR_ERROR_INVALID_ARGUMENT, with a description in `message` (may be NULL).
+ * Free the tables with dtr_host_detector_free. */
+DTR_API int dtr_read_detector_json(const char* geometry_file, const char* grids_file,
+                                   const char* material_file, dtr_detector_desc* out,
+                                   char* message, size_t message_size);
+
+/* Write detector tables as detray JSON files (geometry; surface grids and
+ * homogeneous material, NULL to skip), as io::write_detector with the json
+ * geometry_writer, surface_grid_writer and homogeneous_material_writer
+ * (io/include/detray/io/frontend/detector_writer.hpp,
+ * io/include/detray/io/backend/geometry_writer.hpp:40-150,
+ * surface_grid_writer.hpp:40-85, homogeneous_material_writer.hpp:40-190)
+ * write them.  volume_names may be NULL (empty names). */
+DTR_API int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_name,
+                                    const char* const* volume_names, const char* geometry_file,
+                                    const char* grids_file, const char* material_file,
+                                    char* message, size_t message_size);
 
 /* Sample the analytic finite solenoid (on-axis 2 T, radius 1.2 m, half
  * length 3 m) on a regular grid.  `values` must hold nx*ny*nz*3 floats. */
