@@ -28,7 +28,8 @@ STATUS_NAMES = {
 # dtr_track_status
 TRACK_EXIT, TRACK_ABORT, TRACK_PATH_LIMIT, TRACK_OVERFLOW = 0, 1, 2, 3
 # dtr_mask_shape
-RECTANGLE2D, TRAPEZOID2D, CONCENTRIC_CYLINDER2D, RING2D = 0, 1, 2, 3
+RECTANGLE2D, TRAPEZOID2D, CONCENTRIC_CYLINDER2D, RING2D, LINE_SQUARE, LINE_CIRCULAR = 0, 1, 2, 3, 4, 5
+MATERIAL_SLAB, MATERIAL_ROD = 0, 1
 # dtr_surface_id
 PORTAL, SENSITIVE, PASSIVE = 0, 1, 2
 INVALID_INDEX = 0xFFFFFFFF
@@ -53,7 +54,7 @@ class MaterialSlab(C.Structure):
         (n, C.c_double)
         for n in ("X0", "L0", "A", "Z", "mass_rho", "molar_rho", "mean_exc",
                   "thickness", "thickness_in_X0")
-    ]
+    ] + [("kind", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class Surface(C.Structure):
@@ -78,7 +79,8 @@ class Grid(C.Structure):
         ("n0", C.c_uint32), ("n1", C.c_uint32),
         ("bins", C.c_uint32),
         ("type", C.c_uint8),
-        ("pad", C.c_uint8 * 3),
+        ("capacity", C.c_uint8),
+        ("pad", C.c_uint8 * 2),
     ]
 
 
@@ -252,6 +254,18 @@ class SurfaceSequence(C.Structure):
     ]
 
 
+class WireChamberConfig(C.Structure):
+    _fields_ = [
+        ("n_layers", C.c_uint32),
+        ("square_cells", C.c_uint32),
+        ("first_layer_inner_radius", C.c_double),
+        ("half_z", C.c_double),
+        ("cell_size", C.c_double),
+        ("stereo_angle", C.c_double),
+        ("mat_radius", C.c_double),
+    ]
+
+
 class ToyConfig(C.Structure):
     _fields_ = [
         ("n_brl_layers", C.c_uint32),
@@ -311,6 +325,7 @@ _SIGNATURES = {
     "dtr_adjoint_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(AdjointConfig),
                                        C.POINTER(AdjointBatch), C.POINTER(Hits), C.c_void_p, C.c_void_p]),
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
+    "dtr_build_wire_chamber": (C.c_int, [C.POINTER(WireChamberConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),
     "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(DetectorDesc),

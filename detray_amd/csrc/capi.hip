@@ -30,6 +30,9 @@ struct dtr_detector {
     uint32_t n_surfaces;
     int device;
     uint32_t n_cu;
+    // wire surfaces, material rods or grid bins of several surfaces: only the
+    // plain FP64 kernel propagates through such a detector
+    bool wires;
 };
 
 struct dtr_field {
@@ -384,6 +387,7 @@ int launch_fast_w(const dtr_detector* det, const dtr_field* field, const dtr::De
 int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
                        const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
     if (!det || !field || !c || !a || !b) return DTR_ERROR_INVALID_ARGUMENT;
+    if (det->wires) return DTR_ERROR_UNSUPPORTED;
     if (a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->max_steps > (1u << 20) || a->max_events > 4096u || a->checkpoint_every > a->max_steps)
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -497,8 +501,22 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
         for (uint32_t k = 0; k < d->n_surfaces && !rc; ++k) {
             const dtr_surface& sf = d->surfaces[k];
             if (sf.transform >= d->n_transforms || sf.mask >= d->n_masks || sf.n_masks == 0u ||
-                sf.mask + sf.n_masks > d->n_masks)
+                sf.mask + sf.n_masks > d->n_masks || sf.shape > DTR_LINE_CIRCULAR ||
+                (sf.material != 0xffffffffu && sf.material >= d->n_materials))
                 rc = DTR_ERROR_INVALID_ARGUMENT;
+            if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR) h->wires = true;
+        }
+        for (uint32_t k = 0; k < d->n_materials && !rc; ++k) {
+            if (d->materials[k].kind > DTR_MATERIAL_ROD) rc = DTR_ERROR_INVALID_ARGUMENT;
+            if (d->materials[k].kind == DTR_MATERIAL_ROD) h->wires = true;
+        }
+        for (uint32_t k = 0; k < d->n_grids && !rc; ++k) {
+            const dtr_grid& g = d->grids[k];
+            const uint64_t cap = g.capacity ? g.capacity : 1u;
+            if (g.n0 == 0u || g.n1 == 0u || g.type > DTR_GRID_RING ||
+                static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 * cap > d->n_bins)
+                rc = DTR_ERROR_INVALID_ARGUMENT;
+            if (cap > 1u) h->wires = true;
         }
         if (!rc) {
             std::vector<dtr::NavSurface> nav(d->n_surfaces);
@@ -618,6 +636,7 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
     if (c->use_field_gradient) {
+        if (det->wires) return DTR_ERROR_UNSUPPORTED;
         // the Jacobian with position columns runs on the batched engine's
         // FP64 policy (same results contract as the plain kernel)
         return launch_grad<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
@@ -690,6 +709,7 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     const int rc = check_propagate_args(det, field, c, in, out);
     if (rc != DTR_SUCCESS) return rc;
     if (!launch) return DTR_ERROR_INVALID_ARGUMENT;
+    if (det->wires) return DTR_ERROR_UNSUPPORTED;
     dtr_launch_config L = *launch;
     if (L.block_size == 0u) L.block_size = 256u;
     if (L.waves_per_simd == 0u) L.waves_per_simd = 2u;
@@ -726,6 +746,7 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     const int rc = check_propagate_args(det, field, c, in, out);
     if (rc != DTR_SUCCESS) return rc;
     if (!seq || !launch) return DTR_ERROR_INVALID_ARGUMENT;
+    if (det->wires) return DTR_ERROR_UNSUPPORTED;
     if (in->n == 0u) return DTR_SUCCESS;
     if (seq->max_surfaces == 0u || !seq->surfaces || !seq->n_surfaces) return DTR_ERROR_INVALID_ARGUMENT;
     dtr_launch_config L = *launch;

@@ -222,7 +222,7 @@ inline void make_nav_records(const dtr_detector_desc& d, const NavSurface* nav, 
     for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
         const dtr_grid& g = d.grids[gi];
         float t = 0.f;
-        for (uint32_t k = 0; k < g.n0 * g.n1; ++k) {
+        for (uint32_t k = 0; k < g.n0 * g.n1 * (g.capacity ? g.capacity : 1u); ++k) {
             const uint32_t s = d.grid_bins[g.bins + k];
             if (s != 0xffffffffu && nav[s].tolk > t) t = nav[s].tolk;
         }

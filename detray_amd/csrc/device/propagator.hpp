@@ -93,6 +93,10 @@ DTR_HD V3 cross(V3 a, V3 b) {
     return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
 DTR_HD double norm(V3 a) { return sqrt(dot(a, a)); }
+DTR_HD V3 normalize(V3 a) {
+    const double n = norm(a);
+    return {a.x / n, a.y / n, a.z / n};
+}
 DTR_HD double perp(V3 a) { return sqrt(fma(a.x, a.x, a.y * a.y)); }
 DTR_HD double phi_of(V3 a) { return atan2(a.y, a.x); }
 DTR_HD double theta_of(V3 a) { return atan2(perp(a), a.z); }
@@ -260,6 +264,24 @@ struct Engine {
                     const double ft = base_tol + ext_tol;
                     edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (m.v[2] + ft));
                 }
+            } else if (m.shape == DTR_LINE_SQUARE || m.shape == DTR_LINE_CIRCULAR) {
+                // line.hpp:86-121: square or circular cross section, half length
+                const double c = fabs(loc.z);
+                double a, b2;
+                if (m.shape == DTR_LINE_SQUARE) {
+                    a = fabs(loc.x);
+                    b2 = fabs(loc.y);
+                    inside = (a <= (m.v[0] + base_tol) && b2 <= (m.v[0] + base_tol) && c <= (m.v[1] + base_tol));
+                } else {
+                    a = fabs(perp(loc));
+                    b2 = 0.0;
+                    inside = (a <= (m.v[0] + base_tol) && c <= (m.v[1] + base_tol));
+                }
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = (a <= (m.v[0] + ft) && b2 <= (m.v[0] + ft) && c <= (m.v[1] + ft));
+                }
             } else {
                 const double r = perp(loc);
                 inside = ((r + base_tol) >= m.v[0] && r <= (m.v[1] + base_tol));
@@ -284,6 +306,24 @@ struct Engine {
             return false;
         }
         s = dot(sn, sub(st, ro)) / denom;
+        glob = add(ro, mul(s, rd));
+        return fabs(s) < 10.0 * 1000.0;
+    }
+
+    // ray_line_intersector.hpp:44-84
+    static DTR_HD bool line_intersect(V3 ro, V3 rd, const dtr_transform& t, double& s, V3& glob) {
+        const V3 z = trf_axis(t, 2);
+        const double zd = dot(z, rd);
+        const double denom = 1.0 - (zd * zd);
+        if (denom < static_cast<double>(1e-5f)) {
+            s = kMax;
+            glob = {0.0, 0.0, 0.0};
+            return false;
+        }
+        const V3 t2l = sub(trf_translation(t), ro);
+        const double t2l_on_line = dot(t2l, z);
+        const double t2l_on_track = dot(t2l, rd);
+        s = 1.0 / denom * (t2l_on_track - t2l_on_line * zd);
         glob = add(ro, mul(s, rd));
         return fabs(s) < 10.0 * 1000.0;
     }
@@ -323,6 +363,8 @@ struct Engine {
         const dtr_surface& sf = det.surfaces[s];
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
             return cyl_intersect(ro, rd, det.masks[sf.mask].v[0], overstep, path, z);
+        if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR)
+            return line_intersect(ro, rd, det.transforms[sf.transform], path, glob);
         return plane_intersect(ro, rd, det.transforms[sf.transform], path, glob);
     }
 
@@ -447,8 +489,13 @@ struct Engine {
             const int i0 = cyl ? (n0 + (b0 % n0)) % n0 : b0;
             for (int b1 = a1; b1 < e1; ++b1) {
                 const int i1 = cyl ? b1 : (n1 + (b1 % n1)) % n1;
-                const uint32_t s = det.bins[g.bins + static_cast<uint32_t>(i0) * g.n1 + i1];
-                if (s != kInvalid) candidate_search(s, tol);
+                // every entry of the bin in fill order (capacity 0 means 1)
+                const uint32_t cap = g.capacity ? g.capacity : 1u;
+                const uint32_t first = g.bins + (static_cast<uint32_t>(i0) * g.n1 + i1) * cap;
+                for (uint32_t k = 0; k < cap; ++k) {
+                    const uint32_t s = det.bins[first + k];
+                    if (s != kInvalid) candidate_search(s, tol);
+                }
             }
         }
     }
@@ -708,9 +755,27 @@ struct Engine {
     }
 
     // ------------------------------------------------------ frames, actors --
+    static DTR_HD bool is_line(uint8_t shape) { return shape == DTR_LINE_SQUARE || shape == DTR_LINE_CIRCULAR; }
+    // line2D.hpp:29-47: signed distance of closest approach, z along the wire
+    static DTR_HD void line_local(const dtr_transform& t, V3 pos, V3 dir, double& l0, double& l1) {
+        const V3 local3 = to_local(t, pos);
+        const V3 r = cross(trf_axis(t, 2), dir);
+        const double sign = dot(r, sub(trf_translation(t), pos)) > 0.0 ? -1.0 : 1.0;
+        l0 = sign * perp(local3);
+        l1 = local3.z;
+    }
+    // jacobian_line.hpp:38-62: y along the wire, x = y x dir, z = x x y
+    static DTR_HD void line_frame(const dtr_transform& t, V3 dir, V3& nx, V3& ny, V3& nz) {
+        ny = trf_axis(t, 2);
+        nx = normalize(cross(ny, dir));
+        nz = cross(nx, ny);
+    }
+
     DTR_HD void free_to_bound(uint32_t s, double out[6]) const {
         const dtr_surface& sf = det.surfaces[s];
-        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(sf.shape)) {
+            line_local(det.transforms[sf.transform], tk.pos, tk.dir, out[0], out[1]);
+        } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi_of(tk.pos);
             out[1] = tk.pos.z;
         } else if (sf.shape == DTR_RING2D) {
@@ -733,6 +798,12 @@ struct Engine {
     }
     DTR_HD V3 bound_pos(uint32_t s, const double b[6]) const {
         const dtr_surface& sf = det.surfaces[s];
+        if (is_line(sf.shape)) {
+            // line2D.hpp:80-99 with the track direction
+            const dtr_transform& t = det.transforms[sf.transform];
+            const V3 r = cross(trf_axis(t, 2), bound_dir(b));
+            return add(to_global(t, {0.0, 0.0, b[1]}), mul(b[0], normalize(r)));
+        }
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double r = det.masks[sf.mask].v[0];
             return {r * cos(b[0]), r * sin(b[0]), b[1]};
@@ -754,7 +825,29 @@ struct Engine {
         const dtr_surface& dsf = det.surfaces[dep];
         // bound-to-free at the departure surface: A (3x2 dpos/dloc), C (3x2 ddir/dangle)
         double A[6];
-        if (dsf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        // dpos/dangle of a line (jacobian_line.hpp:102-172), zero otherwise
+        double Pa[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+        if (is_line(dsf.shape)) {
+            const dtr_transform& t = det.transforms[dsf.transform];
+            const V3 ddir = bound_dir(tk.bound);
+            const V3 dp = bound_pos(dep, tk.bound);
+            V3 nx, ny, nz;
+            line_frame(t, ddir, nx, ny, nz);
+            A[0] = nx.x; A[1] = ny.x;
+            A[2] = nx.y; A[3] = ny.y;
+            A[4] = nx.z; A[5] = ny.z;
+            double l0, l1;
+            line_local(t, dp, ddir, l0, l1);
+            const double ct = cos(tk.bound[3]), stt = sin(tk.bound[3]);
+            const double cp = cos(tk.bound[2]), sp = sin(tk.bound[2]);
+            const V3 yp = cross(ny, {-stt * sp, stt * cp, 0.0});
+            const V3 yt = cross(ny, {ct * cp, ct * sp, -stt});
+            const double C = 1.0 / dot(ddir, nz) * l0;
+            const V3 dphi = mul(C, sub(yp, mul(dot(nx, yp), nx)));
+            const V3 dtheta = mul(C, sub(yt, mul(dot(nx, yt), nx)));
+            Pa[0][0] = dphi.x; Pa[1][0] = dphi.y; Pa[2][0] = dphi.z;
+            Pa[0][1] = dtheta.x; Pa[1][1] = dtheta.y; Pa[2][1] = dtheta.z;
+        } else if (dsf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const V3 dp = bound_pos(dep, tk.bound);
             const double r = perp(dp), ph = phi_of(dp);
             A[0] = -r * sin(ph); A[1] = 0.0;
@@ -781,8 +874,8 @@ struct Engine {
             // pos rows: J_pos = [I | Jp(4..7)]: I*A for loc, Jp[4..6]*C for angles, Jp7 for qop
             Mp[r][0] = A[r * 2 + 0];
             Mp[r][1] = A[r * 2 + 1];
-            Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
-            Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            Mp[r][2] = Pa[r][0] + tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
+            Mp[r][3] = Pa[r][1] + tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
             Mp[r][4] = tk.J[jpos(r, 7)];
             Mp[r][5] = 0.0;
             Md[r][0] = 0.0;
@@ -796,17 +889,33 @@ struct Engine {
         // path correction: P = (I + p2f f2p) applied on the left;
         // p2f = [dir, 0, dtds, 0], f2p = [n_pos, 0...]
         const V3 k = dtds();
-        V3 n;
         const dtr_surface& esf = det.surfaces[dest];
-        if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
-            const double ph = phi_of(tk.pos);
-            n = {cos(ph), sin(ph), 0.0};
+        V3 f2p, f2d{0.0, 0.0, 0.0};
+        if (is_line(esf.shape)) {
+            // jacobian_line.hpp:64-92: position and direction parts with dt/ds
+            const dtr_transform& t = det.transforms[esf.transform];
+            const V3 pc = sub(tk.pos, trf_translation(t));
+            const V3 lz = trf_axis(t, 2);
+            const double pz = dot(pc, lz);
+            const double dz = dot(lz, tk.dir);
+            const V3 pc_x = sub(pc, mul(pz, lz));
+            const double nrm = -1.0 / (1.0 - dz * dz + dot(pc_x, k));
+            f2p = mul(nrm, sub(tk.dir, mul(dz, lz)));
+            f2d = mul(nrm, pc_x);
         } else {
-            n = trf_axis(det.transforms[esf.transform], 2);
+            V3 n;
+            if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+                const double ph = phi_of(tk.pos);
+                n = {cos(ph), sin(ph), 0.0};
+            } else {
+                n = trf_axis(det.transforms[esf.transform], 2);
+            }
+            f2p = mul(-1.0 / dot(n, tk.dir), n);
         }
-        const V3 f2p = mul(-1.0 / dot(n, tk.dir), n);
-        double proj[6];  // f2p . M_pos (per column)
-        for (int c = 0; c < 6; ++c) proj[c] = f2p.x * Mp[0][c] + f2p.y * Mp[1][c] + f2p.z * Mp[2][c];
+        double proj[6];  // f2p . M (per column)
+        for (int c = 0; c < 6; ++c)
+            proj[c] = f2p.x * Mp[0][c] + f2p.y * Mp[1][c] + f2p.z * Mp[2][c] +
+                      (f2d.x * Md[0][c] + f2d.y * Md[1][c] + f2d.z * Md[2][c]);
         const double p2f_pos[3] = {tk.dir.x, tk.dir.y, tk.dir.z};
         const double p2f_dir[3] = {k.x, k.y, k.z};
         for (int r = 0; r < 3; ++r) {
@@ -817,7 +926,13 @@ struct Engine {
         }
         // free-to-bound at the destination
         double Bl[6];
-        if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(esf.shape)) {
+            // jacobian_line.hpp:174-182: rows x and y of the line's frame
+            V3 nx, ny, nz;
+            line_frame(det.transforms[esf.transform], tk.dir, nx, ny, nz);
+            Bl[0] = nx.x; Bl[1] = nx.y; Bl[2] = nx.z;
+            Bl[3] = ny.x; Bl[4] = ny.y; Bl[5] = ny.z;
+        } else if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
             Bl[0] = -rinv * sin(ph); Bl[1] = rinv * cos(ph); Bl[2] = 0.0;
             Bl[3] = 0.0;             Bl[4] = 0.0;            Bl[5] = 1.0;
@@ -887,7 +1002,18 @@ struct Engine {
         const double cos_inc = fabs(dot(bound_dir(tk.bound), nrm));
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];
-        const double path_seg = m.thickness / cos_inc;
+        double path_seg, path_x0;
+        if (m.kind == DTR_MATERIAL_ROD) {
+            // material_rod.hpp:62-78, approach = bound loc0
+            const double a = tk.bound[0];
+            path_seg = 0.0;
+            if (!(fabs(a) > m.thickness))
+                path_seg = 2.0 * sqrt((m.thickness * m.thickness - a * a) / (1.0 - cos_inc * cos_inc));
+            path_x0 = path_seg / m.X0;
+        } else {
+            path_seg = m.thickness / cos_inc;
+            path_x0 = m.thickness_in_X0 / cos_inc;
+        }
         const double mass = tk.mass, q = tk.charge;
         const double q2b2 = q * q + (mass * qop) * (mass * qop);
         const double mOverP = mass * ((q != 0.0) ? fabs(qop / q) : fabs(qop));
@@ -924,7 +1050,7 @@ struct Engine {
             }
         }
         if (cfg.do_ms) {
-            const double x0 = m.thickness_in_X0 / cos_inc;
+            const double x0 = path_x0;
             const double pinv = fabs(qop / q);
             if (x0 > 0.0) {
                 const double t = sqrt(x0 * q2b2);

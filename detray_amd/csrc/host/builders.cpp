@@ -15,6 +15,8 @@
 #include "host/tables.hpp"
 
 #include <algorithm>
+#include <array>
+#include <stdexcept>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -87,8 +89,20 @@ Material beryllium_tml() {
             1.848 * kGram / kCm3};
 }
 
+// predefined_materials.hpp:159-163
+Material tungsten() {
+    return {F(3.504) * kMM, F(99.46) * kMM, F(183.84), F(74.0), 19.3 * kGram / kCm3};
+}
+
 dtr_material_slab make_slab(const Material& m, double thickness) {
     return dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho, thickness);
+}
+
+// material_rod.hpp:26-32: the radius takes the place of the thickness
+dtr_material_slab make_rod(const Material& m, double radius) {
+    dtr_material_slab r = dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho, radius);
+    r.kind = DTR_MATERIAL_ROD;
+    return r;
 }
 
 // A mask as the builders create it: a shape and its boundary values
@@ -111,6 +125,7 @@ struct GridDef {
     uint8_t type = 0;
     double min0, max0, min1, max1;
     uint32_t n0, n1;
+    uint32_t capacity = 1;  // surfaces per bin
 };
 
 struct VolumeDef {
@@ -320,7 +335,9 @@ void assemble(const std::vector<VolumeDef>& vols,
             g.n0 = V.grid.n0;
             g.n1 = V.grid.n1;
             g.bins = static_cast<uint32_t>(T.bins.size());
-            T.bins.resize(T.bins.size() + static_cast<size_t>(g.n0) * g.n1, kInvalid);
+            g.capacity = static_cast<uint8_t>(V.grid.capacity);
+            const uint32_t cap = V.grid.capacity;
+            T.bins.resize(T.bins.size() + static_cast<size_t>(g.n0) * g.n1 * cap, kInvalid);
             const dtr_transform& vt = V.placement;
             // grid_builder.hpp:232-272: radius between the portal cylinders
             if (g.type == DTR_GRID_CYLINDER) {
@@ -365,8 +382,16 @@ void assemble(const std::vector<VolumeDef>& vols,
                     b0 = clamp(b0, static_cast<int>(g.n0));
                     b1 = wrap(b1, static_cast<int>(g.n1));
                 }
-                uint32_t& slot = T.bins[g.bins + static_cast<size_t>(b0) * g.n1 + b1];
-                if (slot == kInvalid) slot = s;  // static_array<sf, 1>: first fill wins
+                uint32_t* bin = &T.bins[g.bins + (static_cast<size_t>(b0) * g.n1 + b1) * cap];
+                if (cap == 1u) {
+                    if (bin[0] == kInvalid) bin[0] = s;  // static_array<sf, 1>: first fill wins
+                } else {
+                    // dynamic bins (populators.hpp attach): appended in surface order
+                    uint32_t k = 0;
+                    while (k < cap && bin[k] != kInvalid) ++k;
+                    if (k == cap) throw std::length_error("grid bin capacity exceeded");
+                    bin[k] = s;
+                }
             }
             dv.grid = static_cast<int32_t>(T.grids.size());
             T.grids.push_back(g);
@@ -713,6 +738,99 @@ void build_toy(ToyCfg c, Tables& T) {
 }
 
 // ---------------------------------------------------------------------------
+// Wire chamber (build_wire_chamber.hpp:150-331, wire_layer_generator.hpp)
+// The wire layer generator works in the reference's default scalar (float):
+// the number of wires per layer comes from a float phi accumulation, so the
+// wire placement is computed in float and stored as double.
+// ---------------------------------------------------------------------------
+void wire_layer(VolumeDef& vol, const dtr_wire_chamber_config& cfg, float inner_layer_rad, float stereo,
+                uint16_t vol_idx, int mat) {
+    const float cell = static_cast<float>(cfg.cell_size);
+    const float half_z = static_cast<float>(cfg.half_z);
+    const float central = inner_layer_rad + cell;
+    const float two_pi = 2.f * static_cast<float>(kPi);
+    const float delta = 2.f * cell / central;
+    std::vector<std::array<float, 3>> centers;
+    float phi = 0.f;
+    while (phi <= two_pi) {
+        centers.push_back({central * std::cos(phi), central * std::sin(phi), 0.f});
+        phi += delta;
+    }
+    const float cs = std::cos(stereo), sn = std::sin(stereo);
+    for (const auto& c : centers) {
+        // r axis, then the z axis rotated about it by the stereo angle
+        // (axis_rotation.hpp:33-46 applied to (0, 0, 1))
+        const float inv = 1.f / std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        const float ux = c[0] * inv, uy = c[1] * inv, uz = c[2] * inv;
+        const float zx = cs * 0.f + sn * uy + (1.f - cs) * (ux * uz);
+        const float zy = cs * 0.f + sn * -ux + (1.f - cs) * (uy * uz);
+        const float zz = cs * 1.f + sn * 0.f + (1.f - cs) * (uz * uz);
+        // unit_vectors.hpp:28-48 make_curvilinear_unit_u
+        float xx = -zy, xy = zx;
+        const float scale = std::sqrt(xx * xx + xy * xy);
+        if (scale < 1e-6f) {
+            xx = 1.f;
+            xy = 0.f;
+        } else {
+            xx = xx * (1.f / scale);
+            xy = xy * (1.f / scale);
+        }
+        // transform3(t, z, x): y = z x x
+        const float yx = zy * 0.f - zz * xy, yy = zz * xx - zx * 0.f, yz = zx * xy - zy * xx;
+        SurfaceDef sf;
+        sf.trf = make_trf({c[0], c[1], c[2]}, {xx, xy, 0.0}, {yx, yy, yz}, {zx, zy, zz});
+        sf.sf_id = DTR_SENSITIVE;
+        sf.material = mat;
+        sf.masks.push_back({static_cast<uint8_t>(cfg.square_cells ? DTR_LINE_SQUARE : DTR_LINE_CIRCULAR),
+                            {cell, half_z}, vol_idx});
+        vol.surfaces.push_back(sf);
+    }
+}
+
+void build_wire_chamber(const dtr_wire_chamber_config& cfg, Tables& T) {
+    std::vector<VolumeDef> vols(1u + cfg.n_layers);
+    std::vector<dtr_material_slab> mats;
+    int rod = -1;
+    if (cfg.mat_radius > 0.0) {
+        mats.push_back(make_rod(tungsten(), cfg.mat_radius));
+        rod = 0;
+    }
+    const double half_z = cfg.half_z;
+    const float inner_rad = static_cast<float>(cfg.first_layer_inner_radius);
+    const float cell = static_cast<float>(cfg.cell_size);
+    // inner volume: no inner portal (build_inner(false))
+    {
+        VolumeDef& v = vols[0];
+        v.surfaces.push_back(cyl_portal(inner_rad, -half_z, half_z, 1u));
+        v.surfaces.push_back(disc_portal(0.0, inner_rad, -half_z, kEndOfWorld));
+        v.surfaces.push_back(disc_portal(0.0, inner_rad, half_z, kEndOfWorld));
+    }
+    float stereo = static_cast<float>(cfg.stereo_angle);
+    for (uint32_t i = 0; i < cfg.n_layers; ++i) {
+        const uint16_t vi = static_cast<uint16_t>(i + 1u);
+        VolumeDef& v = vols[vi];
+        const float in_r = inner_rad + static_cast<float>(i) * 2.f * cell;
+        const float out_r = inner_rad + static_cast<float>(i + 1u) * 2.f * cell;
+        const uint16_t north = (i == cfg.n_layers - 1u) ? kEndOfWorld : static_cast<uint16_t>(vi + 1u);
+        add_gap_portals(v, -half_z, half_z, in_r, out_r, north, static_cast<uint16_t>(vi - 1u), kEndOfWorld,
+                        kEndOfWorld);
+        // the sign alternates from layer to layer; the magnitude is kept
+        stereo = ((i % 2u == 0u) ? 1.f : -1.f) * std::fabs(stereo);
+        wire_layer(v, cfg, in_r, stereo, vi, rod);
+        v.grid.present = true;
+        v.grid.type = DTR_GRID_CYLINDER;
+        v.grid.min0 = -F(kPi);
+        v.grid.max0 = F(kPi);
+        v.grid.min1 = -half_z;
+        v.grid.max1 = half_z;
+        v.grid.n0 = 100u;
+        v.grid.n1 = 1u;
+        v.grid.capacity = 3u;
+    }
+    assemble(vols, mats, T);
+}
+
+// ---------------------------------------------------------------------------
 // Telescope (build_telescope_detector.hpp, cuboid_portal_generator.hpp)
 // ---------------------------------------------------------------------------
 void build_telescope(const dtr_telescope_config& cfg, Tables& T) {
@@ -875,6 +993,22 @@ int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out) {
         return DTR_ERROR_OUT_OF_MEMORY;
     } catch (...) {
         return DTR_ERROR_INVALID_ARGUMENT;
+    }
+    return DTR_SUCCESS;
+}
+
+int dtr_build_wire_chamber(const dtr_wire_chamber_config* cfg, dtr_detector_desc* out) {
+    if (!cfg || !out || cfg->n_layers == 0u || cfg->n_layers > 1000u || !(cfg->first_layer_inner_radius > 0.0) ||
+        !(cfg->half_z > 0.0) || !(cfg->cell_size > 0.0) || cfg->mat_radius < 0.0)
+        return DTR_ERROR_INVALID_ARGUMENT;
+    try {
+        Tables T;
+        build_wire_chamber(*cfg, T);
+        export_tables(T, out);
+    } catch (const std::bad_alloc&) {
+        return DTR_ERROR_OUT_OF_MEMORY;
+    } catch (const std::length_error&) {
+        return DTR_ERROR_UNSUPPORTED;
     }
     return DTR_SUCCESS;
 }

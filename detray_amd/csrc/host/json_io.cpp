@@ -24,12 +24,14 @@
 //    z bounds and the transform becomes the identity;
 //  * grid bins hold volume-local surface indices; a volume with a grid keeps
 //    only its portals and passive surfaces in the brute-force list;
+//  * grid bins keep all their entries in file order (the largest occupancy
+//    becomes the grid's bin capacity);
 //  * homogeneous material: entry j of a volume's list goes to the surface
-//    at (smallest surface_idx + j), one slab per surface, placed at
-//    "index_in_coll" in the volume's slab list; volumes append their slabs in
-//    volume order.
+//    at (smallest surface_idx + j), one slab or rod per surface, placed at
+//    "index_in_coll" in the volume's slab or rod list; volumes append their
+//    slabs, then their rods, in volume order.
 // What the tables cannot hold (other shapes, irregular or open grid axes,
-// several grids per volume, several surfaces per bin, material rods) is
+// several grids per volume) is
 // rejected with DTR_ERROR_UNSUPPORTED; inconsistent files (gaps or
 // duplicates in index_in_coll, out-of-range links) with
 // DTR_ERROR_INVALID_ARGUMENT, where the reference would leave holes or
@@ -68,7 +70,7 @@ struct IoError {
 
 // io::shape_id (io/include/detray/io/frontend/definitions.hpp)
 constexpr unsigned kIoCylinder2 = 2u, kIoPortalCylinder2 = 4u, kIoRectangle2 = 5u, kIoRing2 = 6u,
-                   kIoTrapezoid2 = 7u, kIoShapes = 13u;
+                   kIoTrapezoid2 = 7u, kIoDriftCell = 8u, kIoStrawTube = 9u, kIoShapes = 13u;
 const char* const kIoShapeNames[kIoShapes] = {
     "annulus2", "cuboid3", "cylinder2", "cylinder3", "portal_cylinder2", "rectangle2", "ring2",
     "trapezoid2", "drift_cell", "straw_tube", "single1", "single2", "single3"};
@@ -191,10 +193,11 @@ struct VolumeIn {
     std::vector<SurfaceIn> surfaces;  // final volume-local order
     bool has_grid = false;
     dtr_grid grid{};
-    std::vector<uint32_t> bins;       // volume-local surface index per bin
-    std::vector<int> slab_of;         // per surface: position in `slabs`, -1 none
-    std::vector<dtr_material_slab> slabs;
-    std::vector<char> slab_set;
+    std::vector<std::vector<uint32_t>> bins;  // volume-local surface indices per bin
+    // per surface: kind and position in that kind's list, kind -1 none
+    std::vector<std::pair<int, size_t>> mat_of;
+    std::vector<dtr_material_slab> mats[2];  // slabs, rods (separate index_in_coll spaces)
+    std::vector<char> mat_set[2];
 };
 
 SurfaceIn read_surface(const Value& s, const std::string& where) {
@@ -237,9 +240,20 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
             out.io_key = kIoTrapezoid2;
             n_bounds = 4;
             break;
+        case kIoDriftCell:
+            out.shape = DTR_LINE_SQUARE;
+            out.io_key = kIoDriftCell;
+            n_bounds = 2;
+            break;
+        case kIoStrawTube:
+            out.shape = DTR_LINE_CIRCULAR;
+            out.io_key = kIoStrawTube;
+            n_bounds = 2;
+            break;
         default:
             unsupported(where + ": mask shape " + kIoShapeNames[sid] +
-                        " (the tables hold rectangle2, trapezoid2, ring2 and cylinder2 masks)");
+                        " (the tables hold rectangle2, trapezoid2, ring2, cylinder2, drift_cell and "
+                        "straw_tube masks)");
     }
     for (size_t k = 0; k < masks.size(); ++k) {
         const std::string mw = where + ".masks[" + std::to_string(k) + "]";
@@ -303,7 +317,7 @@ void read_geometry(const Value& root, std::vector<VolumeIn>& vols) {
         } else {
             bad(where + ": index_in_coll given for some surfaces only");
         }
-        V.slab_of.assign(V.surfaces.size(), -1);
+        V.mat_of.assign(V.surfaces.size(), {-1, 0});
         vols.push_back(std::move(V));
     }
 }
@@ -364,24 +378,28 @@ void read_grids(const Value& root, std::vector<VolumeIn>& vols) {
             gr.min1 = lo[1];
             gr.max1 = hi[1];
             if (static_cast<uint64_t>(gr.n0) * gr.n1 > (1ull << 26)) unsupported(where + ": more than 2^26 bins");
-            V.bins.assign(static_cast<size_t>(gr.n0) * gr.n1, kInvalid);
+            V.bins.assign(static_cast<size_t>(gr.n0) * gr.n1, {});
+            size_t cap = 1;
             for (const Value& bin : as_array(need(g, "bins", where), where + ".bins")) {
                 const auto& li = as_array(need(bin, "loc_index", where), where + ".loc_index");
                 if (li.size() != 2u) bad(where + ": a bin needs 2 local indices");
                 const uint64_t i0 = as_index(li[0], where + ".loc_index"), i1 = as_index(li[1], where + ".loc_index");
                 if (i0 >= gr.n0 || i1 >= gr.n1)
                     bad(where + ": bin (" + std::to_string(i0) + ", " + std::to_string(i1) + ") out of bounds");
-                uint32_t& slot = V.bins[static_cast<size_t>(i0) * gr.n1 + i1];
+                std::vector<uint32_t>& entries = V.bins[static_cast<size_t>(i0) * gr.n1 + i1];
                 for (const Value& c : as_array(need(bin, "content", where), where + ".content")) {
                     const uint64_t sf = as_index(c, where + ".content");
                     if (static_cast<uint32_t>(sf) == kInvalid) continue;  // invalid entries are skipped
                     if (sf >= V.surfaces.size()) bad(where + ": surface " + std::to_string(sf) + " not in the volume");
-                    if (slot != kInvalid)
-                        unsupported(where + ": more than one surface in bin (" + std::to_string(i0) + ", " +
-                                    std::to_string(i1) + "); the tables hold one per bin");
-                    slot = static_cast<uint32_t>(sf);
+                    entries.push_back(static_cast<uint32_t>(sf));
                 }
+                cap = std::max(cap, entries.size());
             }
+            // the bins are stored with the largest occupancy as capacity
+            if (cap > 255u) unsupported(where + ": more than 255 surfaces in a bin");
+            if (static_cast<uint64_t>(gr.n0) * gr.n1 * cap > (1ull << 26))
+                unsupported(where + ": more than 2^26 bin entries");
+            gr.capacity = static_cast<uint8_t>(cap);
             V.has_grid = true;
             V.grid = gr;
         }
@@ -414,13 +432,15 @@ void read_material(const Value& root, std::vector<VolumeIn>& vols) {
             const std::string ew = where + " entry " + std::to_string(k);
             const Value& e = entries[k];
             const uint64_t type = as_index(need(e, "type", ew), ew + ".type");
-            if (type == kIoRod) unsupported(ew + ": material rods");
-            if (type != kIoSlab) bad(ew + ": material type " + std::to_string(type) + " is not homogeneous");
+            if (type != kIoSlab && type != kIoRod)
+                bad(ew + ": material type " + std::to_string(type) + " is not homogeneous");
             double p[7];
             numbers(need(need(e, "material", ew), "params", ew), 7, p, ew + ".material.params");
             const double t = as_number(need(e, "thickness", ew), ew + ".thickness");
             Entry en{as_index(need(e, "surface_idx", ew), ew + ".surface_idx"), false, 0,
                      make_slab(p[0], p[1], p[2], p[3], p[4], t)};
+            // a rod's "thickness" is its radius (material_rod.hpp:59-60)
+            if (type == kIoRod) en.slab.kind = DTR_MATERIAL_ROD;
             if (const Value* ic = e.find("index_in_coll")) {
                 en.has_index = true;
                 en.index = as_index(*ic, ew + ".index_in_coll");
@@ -438,31 +458,36 @@ void read_material(const Value& root, std::vector<VolumeIn>& vols) {
         if (first + n > V.surfaces.size()) bad(where + ": surface_idx beyond the volume's surfaces");
         for (size_t k = 0; k < n; ++k) {
             const uint64_t sf = idx[k];
-            if (V.slab_of[sf] >= 0) continue;  // already has material: kept
+            if (V.mat_of[sf].first >= 0) continue;  // already has material: kept
             const Entry& en = es[sf - first];
+            const int kind = en.slab.kind == DTR_MATERIAL_ROD ? 1 : 0;
+            auto& list = V.mats[kind];
+            auto& set = V.mat_set[kind];
             size_t pos;
             if (en.has_index) {
                 pos = static_cast<size_t>(en.index);
                 if (pos > (1u << 24)) bad(where + ": index_in_coll out of range");
-                if (V.slabs.size() <= pos) {
-                    V.slabs.resize(pos + 1);
-                    V.slab_set.resize(pos + 1, 0);
+                if (list.size() <= pos) {
+                    list.resize(pos + 1);
+                    set.resize(pos + 1, 0);
                 }
-                if (V.slab_set[pos]) bad(where + ": two slabs at index_in_coll " + std::to_string(pos));
+                if (set[pos]) bad(where + ": two materials at index_in_coll " + std::to_string(pos));
             } else {
-                pos = V.slabs.size();
-                V.slabs.emplace_back();
-                V.slab_set.push_back(0);
+                pos = list.size();
+                list.emplace_back();
+                set.push_back(0);
             }
-            V.slabs[pos] = en.slab;
-            V.slab_set[pos] = 1;
-            V.slab_of[sf] = static_cast<int>(pos);
+            list[pos] = en.slab;
+            set[pos] = 1;
+            V.mat_of[sf] = {kind, pos};
         }
     }
     for (size_t vi = 0; vi < vols.size(); ++vi)
-        for (size_t k = 0; k < vols[vi].slab_set.size(); ++k)
-            if (!vols[vi].slab_set[k])
-                bad("material of volume " + std::to_string(vi) + ": no slab at index_in_coll " + std::to_string(k));
+        for (int kind = 0; kind < 2; ++kind)
+            for (size_t k = 0; k < vols[vi].mat_set[kind].size(); ++k)
+                if (!vols[vi].mat_set[kind][k])
+                    bad("material of volume " + std::to_string(vi) + ": no " + (kind ? "rod" : "slab") +
+                        " at index_in_coll " + std::to_string(k));
 }
 
 void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
@@ -490,7 +515,11 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
                 dm.shape = S.shape;
                 T.masks.push_back(dm);
             }
-            ds.material = V.slab_of[k] >= 0 ? mat_offset + static_cast<uint32_t>(V.slab_of[k]) : kInvalid;
+            // the volume's slabs, then its rods
+            const auto mo = V.mat_of[k];
+            ds.material = mo.first < 0 ? kInvalid
+                                       : mat_offset + static_cast<uint32_t>(mo.first ? V.mats[0].size() : 0u) +
+                                             static_cast<uint32_t>(mo.second);
             ds.volume = static_cast<uint16_t>(vi);
             ds.sf_id = S.sf_id;
             ds.barcode = make_barcode(vi, S.sf_id, idx, ds.transform);
@@ -498,7 +527,8 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
         }
         if (T.surfaces.size() >= (1u << 21) || T.transforms.size() >= (1u << 21))
             unsupported("more than 2^21 surfaces or transforms");
-        T.materials.insert(T.materials.end(), V.slabs.begin(), V.slabs.end());
+        T.materials.insert(T.materials.end(), V.mats[0].begin(), V.mats[0].end());
+        T.materials.insert(T.materials.end(), V.mats[1].begin(), V.mats[1].end());
         dv.sf_end = static_cast<uint32_t>(T.surfaces.size());
         dv.bf_begin = static_cast<uint32_t>(T.bf.size());
         for (uint32_t s = dv.sf_begin; s < dv.sf_end; ++s)
@@ -508,7 +538,10 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
         if (V.has_grid) {
             dtr_grid g = V.grid;
             g.bins = static_cast<uint32_t>(T.bins.size());
-            for (uint32_t b : V.bins) T.bins.push_back(b == kInvalid ? kInvalid : dv.sf_begin + b);
+            const size_t cap = g.capacity ? g.capacity : 1u;
+            for (const auto& entries : V.bins)
+                for (size_t k = 0; k < cap; ++k)
+                    T.bins.push_back(k < entries.size() ? dv.sf_begin + entries[k] : kInvalid);
             // grid_builder.hpp:232-272: reference radius between the portal cylinders
             if (g.type == DTR_GRID_CYLINDER) {
                 double rmin = 0.0, rmax = 0.0;
@@ -555,7 +588,8 @@ void check_desc(const dtr_detector_desc& d) {
             if (static_cast<uint32_t>(v.grid) >= d.n_grids || !d.grids)
                 bad("volume " + std::to_string(vi) + ": grid out of range");
             const dtr_grid& g = d.grids[v.grid];
-            if (static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 > d.n_bins)
+            if (static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 * (g.capacity ? g.capacity : 1u) >
+                d.n_bins)
                 bad("volume " + std::to_string(vi) + ": grid bins out of range");
         }
         next = v.sf_end;
@@ -569,6 +603,8 @@ unsigned io_shape(uint8_t shape) {
         case DTR_TRAPEZOID2D: return kIoTrapezoid2;
         case DTR_CONCENTRIC_CYLINDER2D: return kIoPortalCylinder2;
         case DTR_RING2D: return kIoRing2;
+        case DTR_LINE_SQUARE: return kIoDriftCell;
+        case DTR_LINE_CIRCULAR: return kIoStrawTube;
         default: bad("unknown mask shape " + std::to_string(shape));
     }
 }
@@ -619,7 +655,7 @@ Value write_geometry(const dtr_detector_desc& d, const char* name, const char* c
     Value& data = root.set("data", Value::object());
     Value& jv = data.set("volumes", Value::array());
     const std::vector<uint32_t> ord = grid_ordinals(d);
-    uint64_t slab = 0;  // one slab per material surface, in surface order
+    uint64_t slab = 0, rods = 0;  // one slab or rod per material surface, in surface order
     for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
         const dtr_volume& v = d.volumes[vi];
         Value vol = Value::object();
@@ -651,9 +687,10 @@ Value write_geometry(const dtr_detector_desc& d, const char* name, const char* c
                 masks.push(std::move(jm));
             }
             if (ds.material != kInvalid) {
+                const bool rod = d.materials[ds.material].kind == DTR_MATERIAL_ROD;
                 Value ml = Value::object();
-                ml.set("type", Value::uint(kIoSlab));
-                ml.set("index", Value::uint(slab++));
+                ml.set("type", Value::uint(rod ? kIoRod : kIoSlab));
+                ml.set("index", Value::uint(rod ? rods++ : slab++));
                 sf.set("material", std::move(ml));
             }
             sf.set("index_in_coll", Value::uint(s - v.sf_begin));
@@ -717,8 +754,10 @@ Value write_grids(const dtr_detector_desc& d, const char* name) {
                 li.push(Value::uint(i1));
                 b.set("loc_index", std::move(li));
                 Value content = Value::array();
-                const uint32_t sf = d.grid_bins[g.bins + static_cast<size_t>(i0) * g.n1 + i1];
-                if (sf != kInvalid) {
+                const uint32_t cap = g.capacity ? g.capacity : 1u;
+                for (uint32_t k = 0; k < cap; ++k) {
+                    const uint32_t sf = d.grid_bins[g.bins + (static_cast<size_t>(i0) * g.n1 + i1) * cap + k];
+                    if (sf == kInvalid) continue;
                     if (sf < v.sf_begin || sf >= v.sf_end)
                         bad("grid of volume " + std::to_string(vi) + ": bin holds a surface of another volume");
                     content.push(Value::uint(sf - v.sf_begin));
@@ -745,15 +784,19 @@ Value write_grids(const dtr_detector_desc& d, const char* name) {
 }
 
 Value write_material(const dtr_detector_desc& d, const char* name) {
-    uint64_t n_slabs = 0;
-    for (uint32_t s = 0; s < d.n_surfaces; ++s) n_slabs += d.surfaces[s].material != kInvalid ? 1u : 0u;
+    uint64_t n_slabs = 0, n_rods = 0;
+    for (uint32_t s = 0; s < d.n_surfaces; ++s) {
+        if (d.surfaces[s].material == kInvalid) continue;
+        if (d.materials[d.surfaces[s].material].kind == DTR_MATERIAL_ROD) ++n_rods;
+        else ++n_slabs;
+    }
     Value root = Value::object();
     Value& h = root.set("header", Value::object());
     h.set("common", header_common(name, "homogeneous_material"));
     h.set("slab_count", Value::uint(n_slabs));
-    h.set("rod_count", Value::uint(0));
+    h.set("rod_count", Value::uint(n_rods));
     h.set("slab_surface_count", Value::uint(n_slabs));
-    h.set("rod_surface_count", Value::uint(0));
+    h.set("rod_surface_count", Value::uint(n_rods));
     Value& data = root.set("data", Value::object());
     Value& jv = data.set("volumes", Value::array());
     for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
@@ -761,13 +804,14 @@ Value write_material(const dtr_detector_desc& d, const char* name) {
         Value mv = Value::object();
         mv.set("volume_link", Value::uint(vi));
         Value list = Value::array();
-        uint64_t slab = 0;
+        uint64_t slab = 0, rods = 0;
         for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
             const dtr_surface& ds = d.surfaces[s];
             if (ds.material == kInvalid) continue;
             const dtr_material_slab& m = d.materials[ds.material];
+            const bool rod = m.kind == DTR_MATERIAL_ROD;
             Value e = Value::object();
-            e.set("type", Value::uint(kIoSlab));
+            e.set("type", Value::uint(rod ? kIoRod : kIoSlab));
             e.set("surface_idx", Value::uint(s - v.sf_begin));
             e.set("thickness", Value::number(m.thickness));
             Value mat = Value::object();
@@ -775,7 +819,7 @@ Value write_material(const dtr_detector_desc& d, const char* name) {
             for (double x : {m.X0, m.L0, m.A, m.Z, m.mass_rho, m.molar_rho, 0.0}) p.push(Value::number(x));
             mat.set("params", std::move(p));
             e.set("material", std::move(mat));
-            e.set("index_in_coll", Value::uint(slab++));
+            e.set("index_in_coll", Value::uint(rod ? rods++ : slab++));
             list.push(std::move(e));
         }
         if (!list.a.empty()) mv.set("surface_material", std::move(list));

@@ -124,6 +124,20 @@ class Detector:
         return cls(d, f"toy_{n_brl_layers}brl_{n_edc_layers}edc")
 
     @classmethod
+    def wire_chamber(cls, n_layers: int = 10, half_z: float = 1000.0, first_layer_inner_radius: float = 500.0,
+                     cell_size: float = 10.0, stereo_angle: float = 0.05, mat_radius: float = 0.015,
+                     square_cells: bool = True) -> "Detector":
+        """detray's wire chamber (build_wire_chamber.hpp): drift cells
+        (line_square) or straw tubes with tungsten rods, one wire layer per
+        cylinder volume, 100 x 1 phi/z grids of up to 3 wires per bin."""
+        lib = capi.load()
+        cfg = capi.WireChamberConfig(n_layers, int(square_cells), first_layer_inner_radius, half_z, cell_size,
+                                     stereo_angle, mat_radius)
+        d = capi.DetectorDesc()
+        capi.check(lib.dtr_build_wire_chamber(C.byref(cfg), C.byref(d)), "dtr_build_wire_chamber")
+        return cls(d, "wire_chamber")
+
+    @classmethod
     def telescope(cls, positions: Sequence[float], half_x: float = 500.0, half_y: float = 500.0,
                   thickness: float = 0.15, envelope: float = 60.0) -> "Detector":
         """Telescope of rectangles along the z-axis pilot ray

@@ -69,7 +69,11 @@ enum dtr_mask_shape {
     DTR_RECTANGLE2D = 0,          /* v = {half_x, half_y}                      */
     DTR_TRAPEZOID2D = 1,          /* v = {half_x_low, half_x_high, half_y, 1/(2 half_y)} */
     DTR_CONCENTRIC_CYLINDER2D = 2,/* v = {r, lower_z, upper_z}                  */
-    DTR_RING2D = 3                /* v = {inner_r, outer_r}                     */
+    DTR_RING2D = 3,               /* v = {inner_r, outer_r}                     */
+    /* wires (shapes/line.hpp, line2D frame: loc0 = signed distance of closest
+     * approach, loc1 = z along the wire; wire_chamber_metadata.hpp) */
+    DTR_LINE_SQUARE = 4,          /* drift cell, v = {half cell size, half_z}  */
+    DTR_LINE_CIRCULAR = 5         /* straw tube, v = {radius, half_z}          */
 };
 
 /* Surface categories (core/include/detray/definitions/geometry.hpp:40-46). */
@@ -91,12 +95,20 @@ typedef struct dtr_mask {
 
 /* Homogeneous material slab (core/include/detray/material/material_slab.hpp,
  * material.hpp).  Derived quantities are filled by the builder. */
+enum dtr_material_kind {
+    DTR_MATERIAL_SLAB = 0, /* material_slab.hpp: path thickness / cos(incidence) */
+    DTR_MATERIAL_ROD = 1   /* material_rod.hpp (on wires): path
+                              2 sqrt((r^2 - d^2) / (1 - cos^2)), d = loc0 */
+};
+
 typedef struct dtr_material_slab {
     double X0, L0, A, Z, mass_rho;
     double molar_rho;     /* mass_rho / (A * u) in mol/mm^3 */
     double mean_exc;      /* I = 16 eV * Z^0.9 */
-    double thickness;     /* mm */
+    double thickness;     /* mm (slab thickness, or rod radius) */
     double thickness_in_X0;
+    uint32_t kind;        /* dtr_material_kind */
+    uint32_t pad;
 } dtr_material_slab;
 
 typedef struct dtr_surface {
@@ -121,9 +133,13 @@ typedef struct dtr_grid {
     double min0, max0, min1, max1; /* axis spans */
     double radius;                 /* reference cylinder radius (cylinder grids) */
     uint32_t n0, n1;               /* bins per axis */
-    uint32_t bins;                 /* offset into grid_bins[], n0*n1 entries, bin(i0,i1) = i0*n1 + i1 */
+    uint32_t bins;                 /* offset into grid_bins[]: n0*n1*capacity entries, entry k of
+                                      bin (i0,i1) at (i0*n1 + i1)*capacity + k, unused entries
+                                      0xffffffff */
     uint8_t type;                  /* dtr_grid_type */
-    uint8_t pad[3];
+    uint8_t capacity;              /* surfaces per bin, 0 means 1 (the toy's static_array<., 1>;
+                                      the wire chamber's dynamic bins hold up to 3) */
+    uint8_t pad[2];
 } dtr_grid;
 
 typedef struct dtr_volume {
@@ -331,7 +347,8 @@ DTR_API void dtr_default_launch(dtr_launch_config* launch);
  * not allocate and does not synchronise; with the persistent schedule,
  * calls that share one `queue` must not overlap.  use_field_gradient runs
  * on the grid schedule with per-lane navigation only (DTR_ERROR_UNSUPPORTED
- * otherwise). */
+ * otherwise).  A detector with wire surfaces, material rods or grid bins of
+ * several surfaces returns DTR_ERROR_UNSUPPORTED (dtr_propagate runs it). */
 DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, const dtr_launch_config* launch,
@@ -507,7 +524,23 @@ typedef struct dtr_telescope_config {
     double envelope;         /* cuboid portal envelope (tel_det_config::envelope) */
 } dtr_telescope_config;
 
+/* build_wire_chamber.hpp:120-331 (wire_chamber_config defaults in
+ * parentheses): an empty inner volume, then n_layers cylinder layers of
+ * 2*cell_size thickness, each with one ring of wires at its central radius,
+ * alternating stereo angle, tungsten rods on the wires, a 100 x 1 phi/z grid
+ * per layer. */
+typedef struct dtr_wire_chamber_config {
+    uint32_t n_layers;           /* 10 */
+    uint32_t square_cells;       /* 1: drift cells (line_square), 0: straw tubes */
+    double first_layer_inner_radius; /* 500 mm */
+    double half_z;               /* 1000 mm */
+    double cell_size;            /* 10 mm (wire_layer_generator_config) */
+    double stereo_angle;         /* 50 mrad */
+    double mat_radius;           /* 15 um rod radius */
+} dtr_wire_chamber_config;
+
 DTR_API int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out);
+DTR_API int dtr_build_wire_chamber(const dtr_wire_chamber_config* cfg, dtr_detector_desc* out);
 DTR_API int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
                                  dtr_detector_desc* out);
 DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);

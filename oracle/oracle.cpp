@@ -244,6 +244,32 @@ inline void check_planar(const dtr_mask& m, const dtr_transform& t, const V3& gl
     }
 }
 
+// line.hpp:86-121 check_boundaries on the global point: the local cartesian
+// position inside the square cross section (drift cell) or the circle
+// (straw tube) and within the half length
+inline void check_line(const dtr_mask& m, const dtr_transform& t, const V3& glob, double tol,
+                       double edge_tol, bool& inside, bool& edge) {
+    const V3 loc = point_to_local(t, glob);
+    double a, b;
+    const double c = std::fabs(loc[2]);
+    if (m.shape == DTR_LINE_SQUARE) {
+        a = std::fabs(loc[0]);
+        b = std::fabs(loc[1]);
+        inside = (a <= (m.v[0] + tol) && b <= (m.v[0] + tol) && c <= (m.v[1] + tol));
+    } else {
+        a = std::fabs(perp(loc));
+        b = 0.0;
+        inside = (a <= (m.v[0] + tol) && c <= (m.v[1] + tol));
+    }
+    edge = false;
+    if (edge_tol > 0.0) {
+        const double ft = tol + edge_tol;
+        edge = (a <= (m.v[0] + ft) && b <= (m.v[0] + ft) && c <= (m.v[1] + ft));
+    }
+}
+
+inline bool is_line(uint8_t shape) { return shape == DTR_LINE_SQUARE || shape == DTR_LINE_CIRCULAR; }
+
 // intersector_base.hpp:171-252 resolve_mask (ip.point is global 3D for
 // planar intersectors and (inv, z) for the concentric cylinder)
 inline void resolve_mask(Cand& is, double path, const V3& glob, double z2d,
@@ -263,6 +289,8 @@ inline void resolve_mask(Cand& is, double path, const V3& glob, double z2d,
         // concentric_cylinder2D.hpp:110-120 (z only)
         inside = (m.v[1] - base_tol <= z2d && z2d <= m.v[2] + base_tol);
         edge = inside;
+    } else if (is_line(m.shape)) {
+        check_line(m, t, glob, base_tol, ext_tol, inside, edge);
     } else {
         check_planar(m, t, glob, base_tol, ext_tol, inside, edge);
     }
@@ -285,6 +313,25 @@ inline bool plane_intersect(const V3& ro, const V3& rd, const dtr_transform& t,
         return false;
     }
     s = dot(sn, sub(st, ro)) / denom;
+    glob = add(ro, mul(s, rd));
+    return std::fabs(s) < 10.0 * 1000.0;
+}
+
+// ray_line_intersector.hpp:44-84: closest approach of the ray to the wire
+// axis; no solution for a ray (nearly) parallel to the wire
+inline bool line_intersect(const V3& ro, const V3& rd, const dtr_transform& t, double& s, V3& glob) {
+    const V3 z = trf_axis(t, 2);
+    const V3 tt = trf_translation(t);
+    const double zd = dot(z, rd);
+    const double denom = 1.0 - (zd * zd);
+    if (denom < static_cast<double>(1e-5f)) {
+        s = kMax;
+        return false;
+    }
+    const V3 t2l = sub(tt, ro);
+    const double t2l_on_line = dot(t2l, z);
+    const double t2l_on_track = dot(t2l, rd);
+    s = 1.0 / denom * (t2l_on_track - t2l_on_line * zd);
     glob = add(ro, mul(s, rd));
     return std::fabs(s) < 10.0 * 1000.0;
 }
@@ -336,6 +383,7 @@ inline bool intersect_surface(const Det& det, uint32_t sf_idx, const V3& ro, con
         const double r = det.mask(s.mask).v[0];
         return cyl_portal_intersect(ro, rd, r, overstep, ip.path, ip.z);
     }
+    if (is_line(s.shape)) return line_intersect(ro, rd, t, ip.path, ip.glob);
     return plane_intersect(ro, rd, t, ip.path, ip.glob);
 }
 
@@ -606,8 +654,13 @@ struct Engine {
             for (int b1 = a1[0]; b1 < a1[1]; ++b1) {
                 const int i0 = circ0 ? wrap(b0, static_cast<int>(g.n0)) : b0;
                 const int i1 = circ0 ? b1 : wrap(b1, static_cast<int>(g.n1));
-                const uint32_t s = det.d->grid_bins[g.bins + static_cast<uint64_t>(i0) * g.n1 + i1];
-                if (s != kInvalid) candidate_search(s, icfg);
+                // every entry of the bin, in the order the bin was filled
+                const uint32_t cap = g.capacity ? g.capacity : 1u;
+                const uint64_t first = g.bins + (static_cast<uint64_t>(i0) * g.n1 + i1) * cap;
+                for (uint32_t k = 0; k < cap; ++k) {
+                    const uint32_t s = det.d->grid_bins[first + k];
+                    if (s != kInvalid) candidate_search(s, icfg);
+                }
             }
         }
     }
@@ -961,11 +1014,30 @@ struct Engine {
     }
 
     // ------------------------------------------------- frames & transforms --
+    // line2D.hpp:29-47 global_to_local_3D: signed distance of closest
+    // approach (negative on the side of cross(z, dir)) and z along the wire
+    static void line_local(const dtr_transform& t, const V3& pos, const V3& dir, double& l0, double& l1) {
+        const V3 local3 = point_to_local(t, pos);
+        const V3 z = trf_axis(t, 2);
+        const V3 r = cross(z, dir);
+        const double sign = dot(r, sub(trf_translation(t), pos)) > 0.0 ? -1.0 : 1.0;
+        l0 = sign * perp(local3);
+        l1 = local3[2];
+    }
+    // jacobian_line.hpp:38-62 reference_frame: y along the wire, x = y x dir
+    static void line_frame(const dtr_transform& t, const V3& dir, V3& nx, V3& ny, V3& nz) {
+        ny = trf_axis(t, 2);
+        nx = normalize(cross(ny, dir));
+        nz = cross(nx, ny);
+    }
+
     // free_to_bound_vector (transform_track_parameters.hpp:24-38)
     void free_to_bound(uint32_t s, const V3& pos, const V3& dir, double time, double qop,
                        double out[6]) const {
         const dtr_surface& sf = det.sf(s);
-        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(sf.shape)) {
+            line_local(det.trf(s), pos, dir, out[0], out[1]);
+        } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi(pos);
             out[1] = pos[2];
         } else if (sf.shape == DTR_RING2D) {
@@ -990,7 +1062,13 @@ struct Engine {
     void bound_to_free(uint32_t s, const double b[6], V3& pos, V3& dir, double& time,
                        double& qop) const {
         const dtr_surface& sf = det.sf(s);
-        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(sf.shape)) {
+            // line2D.hpp:80-99 local_to_global with the track direction
+            const dtr_transform& t = det.trf(s);
+            const V3 d = bound_dir(b);
+            const V3 r = cross(trf_axis(t, 2), d);
+            pos = add(point_to_global(t, V3{{0.0, 0.0, b[1]}}), mul(b[0], normalize(r)));
+        } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double r = det.mask(sf.mask).v[0];
             pos = {{r * std::cos(b[0]), r * std::sin(b[0]), b[1]}};
         } else if (sf.shape == DTR_RING2D) {
@@ -1002,16 +1080,26 @@ struct Engine {
         time = b[5];
         qop = b[4];
     }
+    // normal used for the incidence angle: the wire direction for lines
+    // (line2D.hpp:101-112)
     V3 surface_normal(uint32_t s, const double b[6]) const {
         const dtr_surface& sf = det.sf(s);
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
             return {{std::cos(b[0]), std::sin(b[0]), 0.0}};
         return trf_axis(det.trf(s), 2);
     }
-    // dpos/dloc (3x2) jacobian_cartesian.hpp:69-79 / concentric_cylindrical
-    void dpos_dloc(uint32_t s, const V3& pos, double A[6]) const {
+    // dpos/dloc (3x2) jacobian_cartesian.hpp:69-79 / concentric_cylindrical /
+    // jacobian_line.hpp:94-100 (the x and y axes of the line's frame)
+    void dpos_dloc(uint32_t s, const V3& pos, const V3& dir, double A[6]) const {
         const dtr_surface& sf = det.sf(s);
-        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(sf.shape)) {
+            V3 nx, ny, nz;
+            line_frame(det.trf(s), dir, nx, ny, nz);
+            for (int r = 0; r < 3; ++r) {
+                A[r * 2 + 0] = nx[r];
+                A[r * 2 + 1] = ny[r];
+            }
+        } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double r = perp(pos), ph = phi(pos);
             A[0] = -r * std::sin(ph); A[1] = 0.0;
             A[2] = r * std::cos(ph);  A[3] = 0.0;
@@ -1023,10 +1111,40 @@ struct Engine {
             A[4] = t.r[6]; A[5] = t.r[7];
         }
     }
+    // dpos/dangle (3x2) of a line (jacobian_line.hpp:102-172); zero for the
+    // other frames
+    void line_dpos_dangle(uint32_t s, const V3& pos, const V3& dir, const double b[6], double A[6]) const {
+        const dtr_transform& t = det.trf(s);
+        double l0, l1;
+        line_local(t, pos, dir, l0, l1);
+        V3 nx, ny, nz;
+        line_frame(t, dir, nx, ny, nz);
+        const double ipdn = 1.0 / dot(dir, nz);
+        const double ct = std::cos(b[3]), stt = std::sin(b[3]), cp = std::cos(b[2]), sp = std::sin(b[2]);
+        const V3 dn_dphi{{-stt * sp, stt * cp, 0.0}};
+        const V3 dn_dtheta{{ct * cp, ct * sp, -stt}};
+        const V3 yp = cross(ny, dn_dphi);
+        const V3 yt = cross(ny, dn_dtheta);
+        const double C = ipdn * l0;
+        const V3 dphi = mul(C, sub(yp, mul(dot(nx, yp), nx)));
+        const V3 dtheta = mul(C, sub(yt, mul(dot(nx, yt), nx)));
+        for (int r = 0; r < 3; ++r) {
+            A[r * 2 + 0] = dphi[r];
+            A[r * 2 + 1] = dtheta[r];
+        }
+    }
     // dloc/dpos (2x3)
-    void dloc_dpos(uint32_t s, const V3& pos, double A[6]) const {
+    void dloc_dpos(uint32_t s, const V3& pos, const V3& dir, double A[6]) const {
         const dtr_surface& sf = det.sf(s);
-        if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (is_line(sf.shape)) {
+            // jacobian_line.hpp:174-182: the first two rows of the frame's transpose
+            V3 nx, ny, nz;
+            line_frame(det.trf(s), dir, nx, ny, nz);
+            for (int c = 0; c < 3; ++c) {
+                A[0 * 3 + c] = nx[c];
+                A[1 * 3 + c] = ny[c];
+            }
+        } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double rinv = 1.0 / perp(pos), ph = phi(pos);
             A[0] = -rinv * std::sin(ph); A[1] = rinv * std::cos(ph); A[2] = 0.0;
             A[3] = 0.0;                  A[4] = 0.0;                 A[5] = 1.0;
@@ -1036,10 +1154,28 @@ struct Engine {
             A[3] = t.r[1]; A[4] = t.r[4]; A[5] = t.r[7];
         }
     }
-    // free_to_path derivative, pos part (jacobian_cartesian.hpp:54-66 and
-    // jacobian_concentric_cylindrical.hpp:45-67)
-    V3 path_derivative(uint32_t s, const V3& pos, const V3& dir) const {
+    // free_to_path derivative (jacobian_cartesian.hpp:54-66,
+    // jacobian_concentric_cylindrical.hpp:45-67: position part only;
+    // jacobian_line.hpp:64-92: position and direction parts, with dt/ds)
+    void path_derivative(uint32_t s, const V3& pos, const V3& dir, const V3& dtds_v, double f2p[8]) const {
         const dtr_surface& sf = det.sf(s);
+        for (int i = 0; i < 8; ++i) f2p[i] = 0.0;
+        if (is_line(sf.shape)) {
+            const dtr_transform& t = det.trf(s);
+            const V3 pc = sub(pos, trf_translation(t));
+            const V3 lz = trf_axis(t, 2);
+            const double pz = dot(pc, lz);
+            const double dz = dot(lz, dir);
+            const V3 pc_x = sub(pc, mul(pz, lz));
+            const double nrm = -1.0 / (1.0 - dz * dz + dot(pc_x, dtds_v));
+            const V3 pos_term = mul(nrm, sub(dir, mul(dz, lz)));
+            const V3 dir_term = mul(nrm, pc_x);
+            for (int i = 0; i < 3; ++i) {
+                f2p[i] = pos_term[i];
+                f2p[4 + i] = dir_term[i];
+            }
+            return;
+        }
         V3 n;
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double ph = phi(pos);
@@ -1047,7 +1183,8 @@ struct Engine {
         } else {
             n = trf_axis(det.trf(s), 2);
         }
-        return mul(-1.0 / dot(n, dir), n);
+        const V3 d = mul(-1.0 / dot(n, dir), n);
+        for (int i = 0; i < 3; ++i) f2p[i] = d[i];
     }
 
     // stepping.dtds() (rk_stepper.hpp:121-140)
@@ -1065,10 +1202,17 @@ struct Engine {
         // B2F 8x6
         double B2F[48] = {0};
         double A[6];
-        dpos_dloc(dep, dpos, A);
+        dpos_dloc(dep, dpos, ddir, A);
         for (int r = 0; r < 3; ++r) {
             B2F[r * 6 + 0] = A[r * 2 + 0];
             B2F[r * 6 + 1] = A[r * 2 + 1];
+        }
+        if (is_line(det.sf(dep).shape)) {
+            line_dpos_dangle(dep, dpos, ddir, up.bound, A);
+            for (int r = 0; r < 3; ++r) {
+                B2F[r * 6 + 2] = A[r * 2 + 0];
+                B2F[r * 6 + 3] = A[r * 2 + 1];
+            }
         }
         {
             const double th = up.bound[3], ph = up.bound[2];
@@ -1084,15 +1228,15 @@ struct Engine {
         // path correction
         const V3 k = dtds();
         double p2f[8] = {st.dir[0], st.dir[1], st.dir[2], 0.0, k[0], k[1], k[2], 0.0};
-        const V3 f2p_pos = path_derivative(dest, st.pos, st.dir);
-        double f2p[8] = {f2p_pos[0], f2p_pos[1], f2p_pos[2], 0, 0, 0, 0, 0};
+        double f2p[8];
+        path_derivative(dest, st.pos, st.dir, k, f2p);
         double C[64];
         for (int r = 0; r < 8; ++r)
             for (int c = 0; c < 8; ++c) C[r * 8 + c] = (r == c ? 1.0 : 0.0) + p2f[r] * f2p[c];
         // F2B 6x8
         double F2B[48] = {0};
         double Bl[6];
-        dloc_dpos(dest, st.pos, Bl);
+        dloc_dpos(dest, st.pos, st.dir, Bl);
         for (int c = 0; c < 3; ++c) {
             F2B[0 * 8 + c] = Bl[0 * 3 + c];
             F2B[1 * 8 + c] = Bl[1 * 3 + c];
@@ -1270,11 +1414,24 @@ struct Engine {
             double e_loss = 0.0, sigma_qop = 0.0, theta0 = 0.0;
             if (m.thickness > std::numeric_limits<double>::epsilon() && up.bound[4] != 0.0) {
                 const double qop = up.bound[4];
-                const double path_seg = m.thickness / cos_inc;
+                double path_seg, path_x0;
+                if (m.kind == DTR_MATERIAL_ROD) {
+                    // material_rod.hpp:62-78, approach = bound loc0
+                    const double a = up.bound[0];
+                    path_seg = 0.0;
+                    if (!(std::fabs(a) > m.thickness)) {
+                        const double sin2 = 1.0 - cos_inc * cos_inc;
+                        path_seg = 2.0 * std::sqrt((m.thickness * m.thickness - a * a) / sin2);
+                    }
+                    path_x0 = path_seg / m.X0;
+                } else {
+                    path_seg = m.thickness / cos_inc;
+                    path_x0 = m.thickness_in_X0 / cos_inc;
+                }
                 const RQ r = make_rq(st.ptc.mass, qop, st.ptc.charge);
                 if (cfg.do_eloss) e_loss = path_seg * bethe_bloch(m, r);
                 if (cfg.do_eloss && cfg.do_cov) sigma_qop = landau_sigma_qop(m, r, path_seg, st.ptc.charge);
-                if (cfg.do_ms) theta0 = highland(m.thickness_in_X0 / cos_inc, qop, st.ptc.charge, r.q2OverBeta2);
+                if (cfg.do_ms) theta0 = highland(path_x0, qop, st.ptc.charge, r.q2OverBeta2);
                 success = true;
             }
             if (success) {
@@ -1580,6 +1737,22 @@ int oracle_material_probe(const dtr_material_slab* mat, double qop, double path,
     out[1] = Engine::landau_sigma_qop(*mat, r, path, p.charge);
     out[2] = Engine::highland(x_over_x0, qop, p.charge, r.q2OverBeta2);
     return DTR_SUCCESS;
+}
+
+int oracle_line_intersect(const double ro[3], const double rd[3], const dtr_transform* trf,
+                          const dtr_mask* mask, double tol, double out[4]) {
+    using namespace oracle;
+    if (!ro || !rd || !trf || !mask || !out) return 0;
+    const V3 o{{ro[0], ro[1], ro[2]}}, d{{rd[0], rd[1], rd[2]}};
+    double s;
+    V3 glob{{0, 0, 0}};
+    if (!line_intersect(o, d, *trf, s, glob) || !(s >= 0.0)) return 0;
+    bool inside, edge;
+    check_line(*mask, *trf, glob, tol, 0.0, inside, edge);
+    out[0] = s;
+    Engine::line_local(*trf, glob, d, out[1], out[2]);
+    out[3] = inside ? 1.0 : 0.0;
+    return 1;
 }
 
 }  // extern "C"

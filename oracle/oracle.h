@@ -72,6 +72,12 @@ int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz,
 int oracle_material_probe(const dtr_material_slab* mat, double qop,
                           double path, double x_over_x0, double out[3]);
 
+/* Ray-wire intersection probe (ray_line_intersector.hpp with a fixed mask
+ * tolerance `tol`, overstep tolerance 0), for KAT tests:
+ * out = {path, loc0, loc1, inside}.  Returns 1 for a valid intersection. */
+int oracle_line_intersect(const double ro[3], const double rd[3], const dtr_transform* trf,
+                          const dtr_mask* mask, double tol, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -206,6 +206,15 @@ def field_at(field, xyz: np.ndarray) -> np.ndarray:
     return out
 
 
+def line_intersect(ro, rd, transform, mask, tol: float):
+    """Ray-wire intersection of the oracle: (valid, path, loc0, loc1, inside)."""
+    lib = load()
+    out = (C.c_double * 4)()
+    ok = lib.oracle_line_intersect((C.c_double * 3)(*ro), (C.c_double * 3)(*rd), C.byref(transform),
+                                   C.byref(mask), C.c_double(tol), out)
+    return bool(ok), out[0], out[1], out[2], bool(out[3])
+
+
 def material_probe(slab, qop: float, path: float, x_over_x0: float) -> np.ndarray:
     lib = load()
     out = (C.c_double * 3)()

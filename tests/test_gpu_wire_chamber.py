@@ -1,0 +1,63 @@
+"""Wire chamber on the device (drift cells and straw tubes, tungsten rods,
+grid bins of up to 3 wires): the plain FP64 kernel against the oracle at
+the FP64 contract; the batched launches, the direct navigator and the
+adjoint return DTR_ERROR_UNSUPPORTED for such a detector."""
+
+import numpy as np
+import pytest
+
+import detray_amd as da
+from detray_amd import capi
+from tests import oracle_host
+from tests.test_gpu_parity import assert_parity, run_both
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def chamber():
+    return da.Detector.wire_chamber(half_z=500.0)
+
+
+@pytest.mark.parametrize("noise", [False, True])
+@pytest.mark.parametrize("field_kind", ["const", "solenoid"])
+def test_plain_kernel_matches_oracle(chamber, solenoid, noise, field_kind):
+    field = da.MagneticField.constant(0, 0, 2 * da.TESLA) if field_kind == "const" else solenoid
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=noise)
+    fp = da.generate_tracks("toy", 4000, 44, eta_min=-1.0, eta_max=1.0)
+    got, ref = run_both(chamber, field, cfg, fp)
+    assert ref["n_surfaces"].mean() > 5
+    assert_parity(got, ref)
+
+
+def test_straw_tubes(solenoid):
+    det = da.Detector.wire_chamber(n_layers=4, half_z=500.0, square_cells=False)
+    cfg = da.PropagationConfig(search_window=(3, 3))
+    got, ref = run_both(det, solenoid, cfg, da.generate_tracks("toy", 2000, 45, eta_min=-1.0, eta_max=1.0))
+    assert_parity(got, ref)
+
+
+def test_json_read_chamber_propagates_identically(tmp_path, chamber, solenoid):
+    f = chamber.write_json(tmp_path, "wire_chamber")
+    det = da.Detector.from_json(f["geometry"], f["surface_grids"], f["homogeneous_material"])
+    cfg = da.PropagationConfig(search_window=(3, 3))
+    tracks = torch.from_numpy(da.generate_tracks("toy", 2000, 46, eta_min=-1.0, eta_max=1.0)).cuda()
+    a = da.Propagator(cfg).propagate(chamber, solenoid, tracks)
+    b = da.Propagator(cfg).propagate(det, solenoid, tracks)
+    torch.cuda.synchronize()
+    for k in ("bound_params", "bound_cov", "surfaces", "n_surfaces", "status"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_batched_paths_reject_wires(chamber, solenoid):
+    cfg = da.PropagationConfig(search_window=(3, 3))
+    tracks = torch.from_numpy(da.generate_tracks("toy", 64, 47)).cuda()
+    for launch in (da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane"),
+                   da.LaunchConfig("fp64", "persistent", 256, 1, 64, None, 1, "wave")):
+        with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
+            da.Propagator(cfg, launch).propagate(chamber, solenoid, tracks)
+    with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
+        da.Propagator(da.PropagationConfig(search_window=(3, 3), use_field_gradient=True)).propagate(
+            chamber, solenoid, tracks)

@@ -16,8 +16,11 @@ Pins:
   oracle propagates bit-identically through both;
 * reader semantics from geometry_reader.hpp / the detector builders on
   edited files: factory order without index_in_coll, the cylinder z-shift
-  fold, legacy keys, the material pairing, and the errors for what the
-  tables cannot hold.
+  fold, legacy keys, the material pairing, bins with several surfaces, and
+  the errors for what the tables cannot hold;
+* the wire chamber (drift cells, tungsten rods, bins of up to 3 wires)
+  round trips; the reference's json_wire_chamber_reader reads 11 volumes
+  (io_json_detector_roundtrip.cpp:319-342).
 """
 
 import ctypes as C
@@ -80,6 +83,12 @@ def material(det, s):
     return tuple(getattr(m, f) for f, _ in m._fields_)
 
 
+def bin_entries(det, g):
+    cap = max(g.capacity, 1)
+    raw = det.table("grid_bins")[g.bins:g.bins + g.n0 * g.n1 * cap]
+    return [[s for s in raw[b * cap:(b + 1) * cap] if s != 0xFFFFFFFF] for b in range(g.n0 * g.n1)]
+
+
 @pytest.fixture(scope="module")
 def toy3_files(tmp_path_factory):
     det = da.Detector.toy(4, 3)
@@ -105,11 +114,13 @@ def _compare_files(a, b, skip=15):
     assert not bad, (bad[:3], la[bad[0]], lb[bad[0]])
 
 
-@pytest.mark.parametrize("which", ["toy3", "toy7", "telescope"])
+@pytest.mark.parametrize("which", ["toy3", "toy7", "telescope", "wire_chamber"])
 def test_write_read_write_round_trip(tmp_path, which):
     if which == "telescope":
         positions = [1.0, 50.0, 100.0, 150.0, 200.0, 250.0, 300.0, 350.0, 400.0, 450.0, 500.0]
         det = da.Detector.telescope(positions, 100.0, 100.0)
+    elif which == "wire_chamber":
+        det = da.Detector.wire_chamber()
     else:
         det = da.Detector.toy(4, 3 if which == "toy3" else 7)
     f1 = det.write_json(tmp_path, "det")
@@ -119,11 +130,17 @@ def test_write_read_write_round_trip(tmp_path, which):
         _compare_files(f1[k], f2[k])
     if which == "telescope":
         assert d2.desc.n_volumes == 1 and d2.desc.n_materials == len(positions)
+    if which == "wire_chamber":
+        # the reference's json_wire_chamber_reader: 11 volumes
+        assert d2.desc.n_volumes == 11
+        assert all(m.kind == capi.MATERIAL_ROD for m in d2.table("materials"))
+        assert load(f1["homogeneous_material"])["header"]["rod_count"] == d2.desc.n_materials
 
 
-@pytest.mark.parametrize("which", ["toy7", "telescope"])
+@pytest.mark.parametrize("which", ["toy7", "telescope", "wire_chamber"])
 def test_read_tables_equal_emitted(tmp_path, which):
     det = (da.Detector.toy(4, 7) if which == "toy7"
+           else da.Detector.wire_chamber(half_z=500.0) if which == "wire_chamber"
            else da.Detector.telescope([100.0 * i for i in range(1, 11)], 500.0, 500.0, 0.15, 60.0))
     f = det.write_json(tmp_path, "det")
     d2 = da.Detector.from_json(f["geometry"], f["surface_grids"], f["homogeneous_material"])
@@ -140,10 +157,12 @@ def test_read_tables_equal_emitted(tmp_path, which):
             assert masks(det, sa) == masks(d2, sb)
             assert material(det, sa) == material(d2, sb)
     assert det.table("bf_surfaces") == d2.table("bf_surfaces")
-    assert det.table("grid_bins") == d2.table("grid_bins")
+    # bins: same entries in the same order (the capacity read back is the
+    # largest occupancy, the emitter's may be larger)
     for ga, gb in zip(det.table("grids"), d2.table("grids")):
-        fields = [f for f, _ in ga._fields_ if f != "pad"]
+        fields = [f for f, _ in ga._fields_ if f not in ("pad", "capacity", "bins")]
         assert [getattr(ga, f) for f in fields] == [getattr(gb, f) for f in fields]
+        assert bin_entries(det, ga) == bin_entries(d2, gb)
 
 
 def test_oracle_propagates_identically(tmp_path, toy7, solenoid):
@@ -256,6 +275,27 @@ def test_material_pairing_and_slabs(tmp_path, toy3_files):
     assert d.desc.n_materials == sum(1 for s in d.table("surfaces") if s.material != 0xFFFFFFFF)
 
 
+def test_bins_with_several_surfaces(tmp_path, toy3_files):
+    """Bins keep every entry in file order; the largest occupancy becomes the
+    grid's capacity (the wire chamber's dynamic bins hold up to 3)."""
+    _, f = toy3_files
+    grids = load(f["surface_grids"])
+    gd = grids["data"]["grids"][0]
+    g = gd["grid_data"][0]
+    full = next(b for b in g["bins"] if b["content"])
+    other = next(b["content"][0] for b in g["bins"] if b["content"] and b["content"][0] != full["content"][0])
+    full["content"].append(other)
+    d = da.Detector.from_json(f["geometry"], dump(tmp_path / "g.json", grids))
+    vol = d.desc.volumes[gd["volume_link"]]
+    grid = d.desc.grids[vol.grid]
+    assert grid.capacity == 2
+    i0, i1 = full["loc_index"]
+    first = grid.bins + (i0 * grid.n1 + i1) * 2
+    assert d.table("grid_bins")[first:first + 2] == [vol.sf_begin + c for c in full["content"]]
+    assert d.desc.n_bins == grid.n0 * grid.n1 * 2 + sum(
+        g2.n0 * g2.n1 * max(g2.capacity, 1) for k, g2 in enumerate(d.table("grids")) if k != vol.grid)
+
+
 def test_empty_grid_file(tmp_path, toy3_files):
     """A grid file whose data is null (what the reference writes for a
     detector without grids) adds no grids."""
@@ -287,12 +327,6 @@ def test_errors(tmp_path, toy3_files):
     geo["data"]["volumes"][2]["surfaces"][0]["index_in_coll"] = geo["data"]["volumes"][2]["surfaces"][1]["index_in_coll"]
     assert read_error(dump(tmp_path / "c.json", geo))[0] == INVALID_ARGUMENT
 
-    grids = load(f["surface_grids"])
-    g = grids["data"]["grids"][0]["grid_data"][0]
-    full = next(b for b in g["bins"] if b["content"])
-    full["content"].append(full["content"][0])
-    code, msg = read_error(f["geometry"], dump(tmp_path / "g1.json", grids))
-    assert code == UNSUPPORTED and "one per bin" in msg
 
     grids = load(f["surface_grids"])
     grids["data"]["grids"][0]["grid_data"][0]["axes"][1]["binning"] = 1
@@ -304,8 +338,8 @@ def test_errors(tmp_path, toy3_files):
 
     mat = load(f["homogeneous_material"])
     mv = next(mv for mv in mat["data"]["volumes"] if mv.get("surface_material"))
-    mv["surface_material"][0]["type"] = 7  # rod
-    assert read_error(f["geometry"], None, dump(tmp_path / "m1.json", mat))[0] == UNSUPPORTED
+    mv["surface_material"][0]["type"] = 3  # a material map is not homogeneous material
+    assert read_error(f["geometry"], None, dump(tmp_path / "m1.json", mat))[0] == INVALID_ARGUMENT
 
     mat = load(f["homogeneous_material"])
     mv = next(mv for mv in mat["data"]["volumes"] if len(mv.get("surface_material", [])) > 2)
