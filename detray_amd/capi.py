@@ -29,6 +29,7 @@ STATUS_NAMES = {
 TRACK_EXIT, TRACK_ABORT, TRACK_PATH_LIMIT, TRACK_OVERFLOW = 0, 1, 2, 3
 # dtr_mask_shape
 RECTANGLE2D, TRAPEZOID2D, CONCENTRIC_CYLINDER2D, RING2D, LINE_SQUARE, LINE_CIRCULAR = 0, 1, 2, 3, 4, 5
+ANNULUS2D, CYLINDER2D = 6, 7
 MATERIAL_SLAB, MATERIAL_ROD = 0, 1
 # dtr_surface_id
 PORTAL, SENSITIVE, PASSIVE = 0, 1, 2
@@ -42,7 +43,7 @@ class Transform(C.Structure):
 
 class Mask(C.Structure):
     _fields_ = [
-        ("v", C.c_double * 4),
+        ("v", C.c_double * 7),
         ("volume_link", C.c_uint16),
         ("shape", C.c_uint8),
         ("pad", C.c_uint8 * 5),

@@ -30,8 +30,9 @@ struct dtr_detector {
     uint32_t n_surfaces;
     int device;
     uint32_t n_cu;
-    // wire surfaces, material rods or grid bins of several surfaces: only the
-    // plain FP64 kernel propagates through such a detector
+    // wire, annulus or placed-cylinder surfaces, material rods or grid bins of
+    // several surfaces: only the plain FP64 kernel propagates through such a
+    // detector
     bool wires;
 };
 
@@ -501,10 +502,10 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
         for (uint32_t k = 0; k < d->n_surfaces && !rc; ++k) {
             const dtr_surface& sf = d->surfaces[k];
             if (sf.transform >= d->n_transforms || sf.mask >= d->n_masks || sf.n_masks == 0u ||
-                sf.mask + sf.n_masks > d->n_masks || sf.shape > DTR_LINE_CIRCULAR ||
+                sf.mask + sf.n_masks > d->n_masks || sf.shape > DTR_CYLINDER2D ||
                 (sf.material != 0xffffffffu && sf.material >= d->n_materials))
                 rc = DTR_ERROR_INVALID_ARGUMENT;
-            if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR) h->wires = true;
+            if (sf.shape >= DTR_LINE_SQUARE) h->wires = true;
         }
         for (uint32_t k = 0; k < d->n_materials && !rc; ++k) {
             if (d->materials[k].kind > DTR_MATERIAL_ROD) rc = DTR_ERROR_INVALID_ARGUMENT;

@@ -116,6 +116,11 @@ DTR_HD V3 to_global(const dtr_transform& t, V3 l) {
             t.r[3] * l.x + t.r[4] * l.y + t.r[5] * l.z + t.t[1],
             t.r[6] * l.x + t.r[7] * l.y + t.r[8] * l.z + t.t[2]};
 }
+DTR_HD V3 vec_to_global(const dtr_transform& t, V3 l) {
+    return {t.r[0] * l.x + t.r[1] * l.y + t.r[2] * l.z,
+            t.r[3] * l.x + t.r[4] * l.y + t.r[5] * l.z,
+            t.r[6] * l.x + t.r[7] * l.y + t.r[8] * l.z};
+}
 
 // ------------------------------------------------------------------ field --
 DTR_HD V3 field_at(const DevField& f, V3 p) {
@@ -282,6 +287,37 @@ struct Engine {
                     const double ft = base_tol + ext_tol;
                     edge = (a <= (m.v[0] + ft) && b2 <= (m.v[0] + ft) && c <= (m.v[1] + ft));
                 }
+            } else if (m.shape == DTR_ANNULUS2D) {
+                // annulus2D.hpp:140-190: beam-frame radius, strip-frame phi
+                const double nx = loc.x + m.v[5], ny = loc.y + m.v[6];
+                const double r_beam = sqrt(fma(nx, nx, ny * ny));
+                double phi_focal = 0.0;
+                bool have_phi = false;
+                inside = ((r_beam + base_tol) >= m.v[0]) && (r_beam <= (m.v[1] + base_tol));
+                if (inside) {
+                    phi_focal = phi_of(loc) - m.v[4];
+                    have_phi = true;
+                    const double pt = r_beam > 0.0 ? base_tol / r_beam : base_tol;
+                    inside = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
+                }
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = ((r_beam + ft) >= m.v[0]) && (r_beam <= (m.v[1] + ft));
+                    if (edge) {
+                        if (!have_phi) phi_focal = phi_of(loc) - m.v[4];
+                        const double pt = r_beam > 0.0 ? ft / r_beam : ft;
+                        edge = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
+                    }
+                }
+            } else if (m.shape == DTR_CYLINDER2D) {
+                // cylinder2D.hpp:63-90: local z within the bounds
+                inside = ((m.v[1] - base_tol) <= loc.z && loc.z <= (m.v[2] + base_tol));
+                edge = false;
+                if (ext_tol > 0.0) {
+                    const double ft = base_tol + ext_tol;
+                    edge = ((m.v[1] - ft) <= loc.z && loc.z <= (m.v[2] + ft));
+                }
             } else {
                 const double r = perp(loc);
                 inside = ((r + base_tol) >= m.v[0] && r <= (m.v[1] + base_tol));
@@ -328,6 +364,37 @@ struct Engine {
         return fabs(s) < 10.0 * 1000.0;
     }
 
+    // ray_cylinder_intersector.hpp:40-110, quadratic_equation.hpp:30-75:
+    // ascending solutions, n = 0, 1 or 2
+    static DTR_HD int cyl2_intersect(V3 ro, V3 rd, const dtr_transform& t, double r, double s[2]) {
+        const V3 sz = trf_axis(t, 2);
+        const V3 pc_cross_sz = cross(sub(ro, trf_translation(t)), sz);
+        const V3 rd_cross_sz = cross(rd, sz);
+        const double a = dot(rd_cross_sz, rd_cross_sz);
+        const double b = 2.0 * dot(rd_cross_sz, pc_cross_sz);
+        const double c = dot(pc_cross_sz, pc_cross_sz) - (r * r);
+        const double eps = 2.220446049250313e-16;
+        s[0] = s[1] = kMax;
+        if (fabs(a) <= eps) {
+            if (fabs(b) <= eps) return 0;
+            s[0] = -c / b;
+            return 1;
+        }
+        const double disc = b * b - 4.0 * a * c;
+        if (disc > eps) {
+            const double q = -0.5 * (b + copysign(sqrt(disc), b));
+            const double s0 = q / a, s1 = c / q;
+            s[0] = s0 > s1 ? s1 : s0;
+            s[1] = s0 > s1 ? s0 : s1;
+            return 2;
+        }
+        if (disc >= 0.0) {
+            s[0] = -0.5 * b / a;
+            return 1;
+        }
+        return 0;
+    }
+
     // ray_cylinder_portal_intersector.hpp:68-132
     static DTR_HD bool cyl_intersect(V3 ro, V3 rd, double r, double overstep, double& path, double& z) {
         path = kMax;
@@ -365,6 +432,15 @@ struct Engine {
             return cyl_intersect(ro, rd, det.masks[sf.mask].v[0], overstep, path, z);
         if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR)
             return line_intersect(ro, rd, det.transforms[sf.transform], path, glob);
+        if (sf.shape == DTR_CYLINDER2D) {
+            // intersection_update takes the smaller solution
+            double sol[2];
+            if (cyl2_intersect(ro, rd, det.transforms[sf.transform], det.masks[sf.mask].v[0], sol) == 0)
+                return false;
+            path = sol[0];
+            glob = add(ro, mul(sol[0], rd));
+            return fabs(sol[0]) < 10.0 * 1000.0;
+        }
         return plane_intersect(ro, rd, det.transforms[sf.transform], path, glob);
     }
 
@@ -397,11 +473,38 @@ struct Engine {
     // navigation_functions.hpp:44-68 + intersection_kernel.hpp:24-168
     DTR_HD void candidate_search(uint32_t s, const NavTol& tol) {
         const V3 rd = tk.dir;  // forward navigation
+        const dtr_surface& sf = det.surfaces[s];
+        const dtr_transform& t = det.transforms[sf.transform];
+        if (sf.shape == DTR_CYLINDER2D) {
+            // intersection_initialize: every valid solution a mask accepts,
+            // at most two
+            double sol[2];
+            const int n = cyl2_intersect(tk.pos, rd, t, det.masks[sf.mask].v[0], sol);
+            for (uint32_t k = 0; k < sf.n_masks; ++k) {
+                int found = 0;
+                for (int i = 0; i < n; ++i) {
+                    if (!(fabs(sol[i]) < 10.0 * 1000.0)) continue;
+                    Cand is;
+                    is.path = kMax;
+                    is.sf = kInvalid;
+                    is.link = 0;
+                    is.pad = 0;
+                    resolve_mask(is, sol[i], add(tk.pos, mul(sol[i], rd)), 0.0, s, sf, det.masks[sf.mask + k], t,
+                                 tol);
+                    if (is.status <= kEdge) {
+                        int p = begin_idx();
+                        const int e = tk.last + 1;
+                        while (p < e && !(fabs(is.path) < fabs(tk.c[p].path))) ++p;
+                        insert(p, is);
+                        if (++found == 2) return;
+                    }
+                }
+            }
+            return;
+        }
         double path, z;
         V3 glob;
         if (!intersect(s, tk.pos, rd, tol.overstep_tol, path, glob, z)) return;
-        const dtr_surface& sf = det.surfaces[s];
-        const dtr_transform& t = det.transforms[sf.transform];
         for (uint32_t k = 0; k < sf.n_masks; ++k) {
             Cand is;
             is.path = kMax;
@@ -764,6 +867,16 @@ struct Engine {
         l0 = sign * perp(local3);
         l1 = local3.z;
     }
+    // jacobian_cylindrical.hpp:24-45: y = cylinder axis, z = normal at the
+    // local point (phi from r phi / r), x = y x z
+    static DTR_HD void cyl2_frame(const dtr_transform& t, V3 pos, V3& nx, V3& ny, V3& nz) {
+        const V3 l = to_local(t, pos);
+        const double r = perp(l);
+        const double ph = (r * phi_of(l)) / r;
+        ny = trf_axis(t, 2);
+        nz = vec_to_global(t, {cos(ph), sin(ph), 0.0});
+        nx = cross(ny, nz);
+    }
     // jacobian_line.hpp:38-62: y along the wire, x = y x dir, z = x x y
     static DTR_HD void line_frame(const dtr_transform& t, V3 dir, V3& nx, V3& ny, V3& nz) {
         ny = trf_axis(t, 2);
@@ -778,7 +891,11 @@ struct Engine {
         } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi_of(tk.pos);
             out[1] = tk.pos.z;
-        } else if (sf.shape == DTR_RING2D) {
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            const V3 l = to_local(det.transforms[sf.transform], tk.pos);
+            out[0] = perp(l) * phi_of(l);
+            out[1] = l.z;
+        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
             const V3 l = to_local(det.transforms[sf.transform], tk.pos);
             out[0] = perp(l);
             out[1] = phi_of(l);
@@ -808,7 +925,12 @@ struct Engine {
             const double r = det.masks[sf.mask].v[0];
             return {r * cos(b[0]), r * sin(b[0]), b[1]};
         }
-        if (sf.shape == DTR_RING2D)
+        if (sf.shape == DTR_CYLINDER2D) {
+            const double r = det.masks[sf.mask].v[0];
+            const double ph = b[0] / r;
+            return to_global(det.transforms[sf.transform], {r * cos(ph), r * sin(ph), b[1]});
+        }
+        if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D)
             return to_global(det.transforms[sf.transform], {b[0] * cos(b[1]), b[0] * sin(b[1]), 0.0});
         return to_global(det.transforms[sf.transform], {b[0], b[1], 0.0});
     }
@@ -853,6 +975,23 @@ struct Engine {
             A[0] = -r * sin(ph); A[1] = 0.0;
             A[2] = r * cos(ph);  A[3] = 0.0;
             A[4] = 0.0;          A[5] = 1.0;
+        } else if (dsf.shape == DTR_CYLINDER2D) {
+            V3 nx, ny, nz;
+            cyl2_frame(det.transforms[dsf.transform], bound_pos(dep, tk.bound), nx, ny, nz);
+            A[0] = nx.x; A[1] = ny.x;
+            A[2] = nx.y; A[3] = ny.y;
+            A[4] = nx.z; A[5] = ny.z;
+        } else if (dsf.shape == DTR_RING2D || dsf.shape == DTR_ANNULUS2D) {
+            // jacobian_polar.hpp:44-72
+            const dtr_transform& t = det.transforms[dsf.transform];
+            const V3 l = to_local(t, bound_pos(dep, tk.bound));
+            const double lr = perp(l), lp = phi_of(l), c = cos(lp), sn = sin(lp);
+            const V3 x = trf_axis(t, 0), y = trf_axis(t, 1);
+            const V3 col0 = add(mul(c, x), mul(sn, y));
+            const V3 col1 = mul(lr, sub(mul(c, y), mul(sn, x)));
+            A[0] = col0.x; A[1] = col1.x;
+            A[2] = col0.y; A[3] = col1.y;
+            A[4] = col0.z; A[5] = col1.z;
         } else {
             const dtr_transform& t = det.transforms[dsf.transform];
             A[0] = t.r[0]; A[1] = t.r[1];
@@ -907,6 +1046,9 @@ struct Engine {
             if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
                 const double ph = phi_of(tk.pos);
                 n = {cos(ph), sin(ph), 0.0};
+            } else if (esf.shape == DTR_CYLINDER2D) {
+                V3 nx, ny;
+                cyl2_frame(det.transforms[esf.transform], tk.pos, nx, ny, n);
             } else {
                 n = trf_axis(det.transforms[esf.transform], 2);
             }
@@ -936,6 +1078,21 @@ struct Engine {
             const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
             Bl[0] = -rinv * sin(ph); Bl[1] = rinv * cos(ph); Bl[2] = 0.0;
             Bl[3] = 0.0;             Bl[4] = 0.0;            Bl[5] = 1.0;
+        } else if (esf.shape == DTR_CYLINDER2D) {
+            V3 nx, ny, nz;
+            cyl2_frame(det.transforms[esf.transform], tk.pos, nx, ny, nz);
+            Bl[0] = nx.x; Bl[1] = nx.y; Bl[2] = nx.z;
+            Bl[3] = ny.x; Bl[4] = ny.y; Bl[5] = ny.z;
+        } else if (esf.shape == DTR_RING2D || esf.shape == DTR_ANNULUS2D) {
+            // jacobian_polar.hpp:81-110
+            const dtr_transform& t = det.transforms[esf.transform];
+            const V3 l = to_local(t, tk.pos);
+            const double lr = perp(l), lp = phi_of(l), c = cos(lp), sn = sin(lp);
+            const V3 x = trf_axis(t, 0), y = trf_axis(t, 1);
+            const V3 row0 = add(mul(c, x), mul(sn, y));
+            const V3 row1 = mul(1.0 / lr, sub(mul(c, y), mul(sn, x)));
+            Bl[0] = row0.x; Bl[1] = row0.y; Bl[2] = row0.z;
+            Bl[3] = row1.x; Bl[4] = row1.y; Bl[5] = row1.z;
         } else {
             const dtr_transform& t = det.transforms[esf.transform];
             Bl[0] = t.r[0]; Bl[1] = t.r[3]; Bl[2] = t.r[6];
@@ -995,10 +1152,16 @@ struct Engine {
     DTR_HD void material_interaction(uint32_t dest) {
         const dtr_material_slab& m = det.materials[det.surfaces[dest].material];
         V3 nrm;
-        if (det.surfaces[dest].shape == DTR_CONCENTRIC_CYLINDER2D)
+        const dtr_surface& msf = det.surfaces[dest];
+        if (msf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
-        else
-            nrm = trf_axis(det.transforms[det.surfaces[dest].transform], 2);
+        } else if (msf.shape == DTR_CYLINDER2D) {
+            // cylindrical2D.hpp:62-70: phi = loc0 / r of the mask
+            const double ph = tk.bound[0] / det.masks[msf.mask].v[0];
+            nrm = vec_to_global(det.transforms[msf.transform], {cos(ph), sin(ph), 0.0});
+        } else {
+            nrm = trf_axis(det.transforms[msf.transform], 2);
+        }
         const double cos_inc = fabs(dot(bound_dir(tk.bound), nrm));
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];

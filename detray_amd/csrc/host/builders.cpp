@@ -304,7 +304,7 @@ void assemble(const std::vector<VolumeDef>& vols,
                 ds.shape = S.masks[0].shape;
                 for (const MaskDef& m : S.masks) {
                     dtr_mask dm{};
-                    for (size_t i = 0; i < m.v.size() && i < 4; ++i) dm.v[i] = m.v[i];
+                    for (size_t i = 0; i < m.v.size() && i < 7; ++i) dm.v[i] = m.v[i];
                     dm.volume_link = m.link;
                     dm.shape = m.shape;
                     T.masks.push_back(dm);

@@ -69,7 +69,7 @@ struct IoError {
 [[noreturn]] void unsupported(const std::string& m) { throw IoError{DTR_ERROR_UNSUPPORTED, m}; }
 
 // io::shape_id (io/include/detray/io/frontend/definitions.hpp)
-constexpr unsigned kIoCylinder2 = 2u, kIoPortalCylinder2 = 4u, kIoRectangle2 = 5u, kIoRing2 = 6u,
+constexpr unsigned kIoAnnulus2 = 0u, kIoCylinder2 = 2u, kIoPortalCylinder2 = 4u, kIoRectangle2 = 5u, kIoRing2 = 6u,
                    kIoTrapezoid2 = 7u, kIoDriftCell = 8u, kIoStrawTube = 9u, kIoShapes = 13u;
 const char* const kIoShapeNames[kIoShapes] = {
     "annulus2", "cuboid3", "cylinder2", "cylinder3", "portal_cylinder2", "rectangle2", "ring2",
@@ -182,7 +182,7 @@ struct SurfaceIn {
     uint8_t shape = 0;     // dtr_mask_shape
     unsigned io_key = 0;   // factory key: io shape id, cylinder2 -> portal_cylinder2
     dtr_transform trf{};
-    std::vector<std::array<double, 4>> bounds;
+    std::vector<std::array<double, 7>> bounds;
     std::vector<uint16_t> links;
     bool has_index = false;
     uint64_t index = 0;
@@ -219,6 +219,11 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
     if (sid >= kIoShapes) bad(where + ": unknown mask shape id " + std::to_string(sid));
     size_t n_bounds = 0;
     switch (sid) {
+        case kIoAnnulus2:
+            out.shape = DTR_ANNULUS2D;
+            out.io_key = kIoAnnulus2;
+            n_bounds = 7;
+            break;
         case kIoCylinder2:
         case kIoPortalCylinder2:
             out.shape = DTR_CONCENTRIC_CYLINDER2D;
@@ -252,12 +257,12 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
             break;
         default:
             unsupported(where + ": mask shape " + kIoShapeNames[sid] +
-                        " (the tables hold rectangle2, trapezoid2, ring2, cylinder2, drift_cell and "
-                        "straw_tube masks)");
+                        " (the tables hold annulus2, rectangle2, trapezoid2, ring2, cylinder2, drift_cell "
+                        "and straw_tube masks)");
     }
     for (size_t k = 0; k < masks.size(); ++k) {
         const std::string mw = where + ".masks[" + std::to_string(k) + "]";
-        std::array<double, 4> b{};
+        std::array<double, 7> b{};
         numbers(need(*masks[k], "boundaries", mw), n_bounds, b.data(), mw + ".boundaries");
         out.bounds.push_back(b);
         // the navigation link is a 16-bit value: the world exit 2^64-1 -> 0xffff
@@ -510,7 +515,7 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
             ds.shape = S.shape;
             for (size_t m = 0; m < S.bounds.size(); ++m) {
                 dtr_mask dm{};
-                for (int q = 0; q < 4; ++q) dm.v[q] = S.bounds[m][q];
+                for (int q = 0; q < 7; ++q) dm.v[q] = S.bounds[m][q];
                 dm.volume_link = S.links[m];
                 dm.shape = S.shape;
                 T.masks.push_back(dm);
@@ -605,12 +610,17 @@ unsigned io_shape(uint8_t shape) {
         case DTR_RING2D: return kIoRing2;
         case DTR_LINE_SQUARE: return kIoDriftCell;
         case DTR_LINE_CIRCULAR: return kIoStrawTube;
+        case DTR_ANNULUS2D: return kIoAnnulus2;
+        // the reference's reader takes cylinder2 as a portal cylinder
+        // (geometry_reader.hpp:110-113): a placed cylinder has no file form
+        case DTR_CYLINDER2D: unsupported("cylinder2 surfaces cannot be written: the reader makes portal cylinders of them");
         default: bad("unknown mask shape " + std::to_string(shape));
     }
 }
 
 size_t n_bounds(uint8_t shape) {
     switch (shape) {
+        case DTR_ANNULUS2D: return 7;
         case DTR_TRAPEZOID2D: return 4;
         case DTR_CONCENTRIC_CYLINDER2D: return 3;
         default: return 2;

@@ -108,10 +108,12 @@ class PropagationConfig:
 class Detector:
     """Flattened detector tables (host) plus a lazily created device copy."""
 
-    def __init__(self, desc: capi.DetectorDesc, name: str):
+    def __init__(self, desc: capi.DetectorDesc, name: str, keep=None):
         self.desc = desc
         self.name = name
         self._device = None
+        # tables owned by Python (from_tables) instead of the library
+        self._keep = keep
 
     # -- builders ----------------------------------------------------------
     @classmethod
@@ -150,6 +152,29 @@ class Detector:
         capi.check(lib.dtr_build_telescope_detector(C.byref(cfg), C.byref(d)),
                    "dtr_build_telescope_detector")
         return cls(d, f"telescope_{len(positions)}")
+
+    @classmethod
+    def from_tables(cls, volumes, surfaces, transforms, masks, materials=(), grids=(), bf_surfaces=(),
+                    grid_bins=(), name: str = "tables") -> "Detector":
+        """Detector from explicit tables (sequences of the capi structures
+        and of surface indices), e.g. for geometries no emitter builds."""
+        def arr(ctype, items):
+            items = list(items)
+            return (ctype * max(len(items), 1))(*items), len(items)
+        keep = []
+        d = capi.DetectorDesc()
+        for field, ctype, items in (("volumes", capi.Volume, volumes), ("surfaces", capi.Surface, surfaces),
+                                    ("transforms", capi.Transform, transforms), ("masks", capi.Mask, masks),
+                                    ("materials", capi.MaterialSlab, materials), ("grids", capi.Grid, grids),
+                                    ("bf_surfaces", C.c_uint32, bf_surfaces), ("grid_bins", C.c_uint32, grid_bins)):
+            a, n = arr(ctype, items)
+            keep.append(a)
+            setattr(d, field, C.cast(a, C.POINTER(ctype)))
+            count = {"volumes": "n_volumes", "surfaces": "n_surfaces", "transforms": "n_transforms",
+                     "masks": "n_masks", "materials": "n_materials", "grids": "n_grids", "bf_surfaces": "n_bf",
+                     "grid_bins": "n_bins"}[field]
+            setattr(d, count, n)
+        return cls(d, name, keep=keep)
 
     @classmethod
     def from_json(cls, geometry: "os.PathLike | str", grids: "os.PathLike | str | None" = None,
@@ -226,7 +251,7 @@ class Detector:
         if self._device is not None:
             lib.dtr_detector_destroy(self._device)
             self._device = None
-        if self.desc.n_surfaces:
+        if self.desc.n_surfaces and self._keep is None:
             lib.dtr_host_detector_free(C.byref(self.desc))
 
     def __del__(self):

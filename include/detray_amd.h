@@ -73,7 +73,13 @@ enum dtr_mask_shape {
     /* wires (shapes/line.hpp, line2D frame: loc0 = signed distance of closest
      * approach, loc1 = z along the wire; wire_chamber_metadata.hpp) */
     DTR_LINE_SQUARE = 4,          /* drift cell, v = {half cell size, half_z}  */
-    DTR_LINE_CIRCULAR = 5         /* straw tube, v = {radius, half_z}          */
+    DTR_LINE_CIRCULAR = 5,        /* straw tube, v = {radius, half_z}          */
+    /* stereo annulus (shapes/annulus2D.hpp, polar2D frame like the ring):
+     * v = {min_r, max_r, min_phi_rel, max_phi_rel, average_phi, shift_x, shift_y} */
+    DTR_ANNULUS2D = 6,
+    /* cylinder at any placement (shapes/cylinder2D.hpp, cylindrical2D frame:
+     * loc0 = r phi, loc1 = z; two intersections per ray), v = {r, lower_z, upper_z} */
+    DTR_CYLINDER2D = 7
 };
 
 /* Surface categories (core/include/detray/definitions/geometry.hpp:40-46). */
@@ -87,7 +93,7 @@ typedef struct dtr_transform {
 } dtr_transform;
 
 typedef struct dtr_mask {
-    double v[4];
+    double v[7];          /* boundary values in the shape's order (64 B per mask) */
     uint16_t volume_link; /* 0xffff = leaves the world */
     uint8_t shape;        /* dtr_mask_shape */
     uint8_t pad[5];

@@ -80,6 +80,11 @@ inline V3 point_to_global(const dtr_transform& t, const V3& l) {
              t.r[3] * l[0] + t.r[4] * l[1] + t.r[5] * l[2] + t.t[1],
              t.r[6] * l[0] + t.r[7] * l[1] + t.r[8] * l[2] + t.t[2]}};
 }
+inline V3 vector_to_global(const dtr_transform& t, const V3& l) {
+    return {{t.r[0] * l[0] + t.r[1] * l[1] + t.r[2] * l[2],
+             t.r[3] * l[0] + t.r[4] * l[1] + t.r[5] * l[2],
+             t.r[6] * l[0] + t.r[7] * l[1] + t.r[8] * l[2]}};
+}
 
 // ----------------------------------------------------------------------------
 // Field (tests/include/detray/test/common/bfield.hpp:35-47: constant or
@@ -236,6 +241,40 @@ inline void check_planar(const dtr_mask& m, const dtr_transform& t, const V3& gl
             const double ft = tol + edge_tol;
             edge = (std::fabs(loc[0]) <= (bound_x + ft) && std::fabs(loc[1]) <= (m.v[2] + ft));
         }
+    } else if (m.shape == DTR_ANNULUS2D) {
+        // annulus2D.hpp:140-190 (global point): radius in the beam frame
+        // (shifted), phi in the strip frame relative to the average phi,
+        // phi tolerance = tol / r (shape_utils.hpp:26-28)
+        const double nx = loc[0] + m.v[5], ny = loc[1] + m.v[6];
+        const double r_beam = std::sqrt(std::fma(nx, nx, ny * ny));
+        auto phi_tol = [r_beam](double t) { return r_beam > 0.0 ? t / r_beam : t; };
+        double phi_focal = 0.0;
+        bool have_phi = false;
+        inside = ((r_beam + tol) >= m.v[0]) && (r_beam <= (m.v[1] + tol));
+        if (inside) {
+            phi_focal = phi(loc) - m.v[4];
+            have_phi = true;
+            const double pt = phi_tol(tol);
+            inside = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
+        }
+        edge = false;
+        if (edge_tol > 0.0) {
+            const double ft = tol + edge_tol;
+            edge = ((r_beam + ft) >= m.v[0]) && (r_beam <= (m.v[1] + ft));
+            if (edge) {
+                if (!have_phi) phi_focal = phi(loc) - m.v[4];
+                const double pt = phi_tol(ft);
+                edge = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
+            }
+        }
+    } else if (m.shape == DTR_CYLINDER2D) {
+        // cylinder2D.hpp:63-90: the local z within [lower_z, upper_z]
+        inside = ((m.v[1] - tol) <= loc[2] && loc[2] <= (m.v[2] + tol));
+        edge = false;
+        if (edge_tol > 0.0) {
+            const double ft = tol + edge_tol;
+            edge = ((m.v[1] - ft) <= loc[2] && loc[2] <= (m.v[2] + ft));
+        }
     } else {
         // ring2D.hpp:85-120: radius of the point in the local cartesian frame
         const double r = perp(loc);
@@ -336,6 +375,38 @@ inline bool line_intersect(const V3& ro, const V3& rd, const dtr_transform& t, d
     return std::fabs(s) < 10.0 * 1000.0;
 }
 
+// ray_cylinder_intersector.hpp:40-110 with quadratic_equation.hpp:30-75:
+// the solutions of |(ro + s rd - c) x z|^2 = r^2, ascending; n = 0, 1 or 2
+inline int cyl2_intersect(const V3& ro, const V3& rd, const dtr_transform& t, double r, double s[2]) {
+    const V3 sz = trf_axis(t, 2);
+    const V3 sc = trf_translation(t);
+    const V3 pc_cross_sz = cross(sub(ro, sc), sz);
+    const V3 rd_cross_sz = cross(rd, sz);
+    const double a = dot(rd_cross_sz, rd_cross_sz);
+    const double b = 2.0 * dot(rd_cross_sz, pc_cross_sz);
+    const double c = dot(pc_cross_sz, pc_cross_sz) - (r * r);
+    const double eps = std::numeric_limits<double>::epsilon();
+    s[0] = s[1] = kMax;
+    if (std::fabs(a) <= eps) {
+        if (std::fabs(b) <= eps) return 0;
+        s[0] = -c / b;
+        return 1;
+    }
+    const double disc = b * b - 4.0 * a * c;
+    if (disc > eps) {
+        const double q = -0.5 * (b + std::copysign(std::sqrt(disc), b));
+        s[0] = q / a;
+        s[1] = c / q;
+        if (s[0] > s[1]) std::swap(s[0], s[1]);
+        return 2;
+    }
+    if (disc >= 0.0) {
+        s[0] = -0.5 * b / a;
+        return 1;
+    }
+    return 0;
+}
+
 // ray_cylinder_portal_intersector.hpp:68-132
 inline bool cyl_portal_intersect(const V3& ro, const V3& rd, double r, double overstep,
                                  double& path, double& z) {
@@ -384,6 +455,15 @@ inline bool intersect_surface(const Det& det, uint32_t sf_idx, const V3& ro, con
         return cyl_portal_intersect(ro, rd, r, overstep, ip.path, ip.z);
     }
     if (is_line(s.shape)) return line_intersect(ro, rd, t, ip.path, ip.glob);
+    if (s.shape == DTR_CYLINDER2D) {
+        // intersection_update (intersection_kernel.hpp:150-200) takes the
+        // smaller solution
+        double sol[2];
+        if (cyl2_intersect(ro, rd, t, det.mask(s.mask).v[0], sol) == 0) return false;
+        ip.path = sol[0];
+        ip.glob = add(ro, mul(sol[0], rd));
+        return std::fabs(sol[0]) < 10.0 * 1000.0;
+    }
     return plane_intersect(ro, rd, t, ip.path, ip.glob);
 }
 
@@ -500,10 +580,33 @@ struct Engine {
     // intersection_kernel.hpp:24-168 intersection_initialize + insert_sorted
     void candidate_search(uint32_t sf_idx, const NavCfg& icfg) {
         const V3 rd = mul(static_cast<double>(nav.direction), st.dir);
-        IPoint ip;
-        if (!intersect_surface(det, sf_idx, st.pos, rd, icfg.overstep_tol, ip)) return;
         const dtr_surface& s = det.sf(sf_idx);
         const dtr_transform& t = det.trf(sf_idx);
+        if (s.shape == DTR_CYLINDER2D) {
+            // intersection_initialize (intersection_kernel.hpp:24-110): every
+            // valid solution that a mask accepts is inserted, at most two
+            double sol[2];
+            const int n = cyl2_intersect(st.pos, rd, t, det.mask(s.mask).v[0], sol);
+            for (uint32_t k = 0; k < s.n_masks; ++k) {
+                int found = 0;
+                for (int i = 0; i < n; ++i) {
+                    if (!(std::fabs(sol[i]) < 10.0 * 1000.0)) continue;
+                    Cand is;
+                    resolve_mask(is, sol[i], add(st.pos, mul(sol[i], rd)), 0.0, sf_idx, s,
+                                 det.mask(s.mask + k), t, icfg, nav.ext_tol);
+                    if (is.probably_inside()) {
+                        int p = nav.begin_idx();
+                        const int e = nav.end_idx();
+                        while (p < e && !(std::fabs(is.path) < std::fabs(nav.c[p].path))) ++p;
+                        insert(p, is);
+                        if (++found == 2) return;
+                    }
+                }
+            }
+            return;
+        }
+        IPoint ip;
+        if (!intersect_surface(det, sf_idx, st.pos, rd, icfg.overstep_tol, ip)) return;
         for (uint32_t k = 0; k < s.n_masks; ++k) {
             Cand is;
             resolve_mask(is, ip.path, ip.glob, ip.z, sf_idx, s, det.mask(s.mask + k), t, icfg,
@@ -1040,7 +1143,12 @@ struct Engine {
         } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi(pos);
             out[1] = pos[2];
-        } else if (sf.shape == DTR_RING2D) {
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            // cylindrical2D.hpp:33-41: r phi and z in the local frame
+            const V3 l = point_to_local(det.trf(s), pos);
+            out[0] = perp(l) * phi(l);
+            out[1] = l[2];
+        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
             const V3 l = point_to_local(det.trf(s), pos);
             out[0] = perp(l);
             out[1] = phi(l);
@@ -1071,7 +1179,12 @@ struct Engine {
         } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double r = det.mask(sf.mask).v[0];
             pos = {{r * std::cos(b[0]), r * std::sin(b[0]), b[1]}};
-        } else if (sf.shape == DTR_RING2D) {
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            // cylindrical2D.hpp:43-60 at the mask radius
+            const double r = det.mask(sf.mask).v[0];
+            const double ph = b[0] / r;
+            pos = point_to_global(det.trf(s), V3{{r * std::cos(ph), r * std::sin(ph), b[1]}});
+        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
             pos = point_to_global(det.trf(s), V3{{b[0] * std::cos(b[1]), b[0] * std::sin(b[1]), 0.0}});
         } else {
             pos = point_to_global(det.trf(s), V3{{b[0], b[1], 0.0}});
@@ -1086,7 +1199,23 @@ struct Engine {
         const dtr_surface& sf = det.sf(s);
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
             return {{std::cos(b[0]), std::sin(b[0]), 0.0}};
+        if (sf.shape == DTR_CYLINDER2D) {
+            // cylindrical2D.hpp:62-70: phi = loc0 / r of the mask
+            const double ph = b[0] / det.mask(sf.mask).v[0];
+            return vector_to_global(det.trf(s), V3{{std::cos(ph), std::sin(ph), 0.0}});
+        }
         return trf_axis(det.trf(s), 2);
+    }
+    // jacobian_cylindrical.hpp:24-45: y = cylinder axis, z = the normal at
+    // the local point (phi from r phi / r), x = y x z
+    void cyl2_frame(uint32_t s, const V3& pos, V3& nx, V3& ny, V3& nz) const {
+        const dtr_transform& t = det.trf(s);
+        const V3 l = point_to_local(t, pos);
+        const double r = perp(l);
+        const double ph = (r * phi(l)) / r;
+        ny = trf_axis(t, 2);
+        nz = vector_to_global(t, V3{{std::cos(ph), std::sin(ph), 0.0}});
+        nx = cross(ny, nz);
     }
     // dpos/dloc (3x2) jacobian_cartesian.hpp:69-79 / concentric_cylindrical /
     // jacobian_line.hpp:94-100 (the x and y axes of the line's frame)
@@ -1104,6 +1233,25 @@ struct Engine {
             A[0] = -r * std::sin(ph); A[1] = 0.0;
             A[2] = r * std::cos(ph);  A[3] = 0.0;
             A[4] = 0.0;               A[5] = 1.0;
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            V3 nx, ny, nz;
+            cyl2_frame(s, pos, nx, ny, nz);
+            for (int r = 0; r < 3; ++r) {
+                A[r * 2 + 0] = nx[r];
+                A[r * 2 + 1] = ny[r];
+            }
+        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
+            // jacobian_polar.hpp:44-72
+            const dtr_transform& t = det.trf(s);
+            const V3 l = point_to_local(t, pos);
+            const double lr = perp(l), lp = phi(l), c = std::cos(lp), sn = std::sin(lp);
+            const V3 x = trf_axis(t, 0), y = trf_axis(t, 1);
+            const V3 col0 = add(mul(c, x), mul(sn, y));
+            const V3 col1 = mul(lr, sub(mul(c, y), mul(sn, x)));
+            for (int r = 0; r < 3; ++r) {
+                A[r * 2 + 0] = col0[r];
+                A[r * 2 + 1] = col1[r];
+            }
         } else {
             const dtr_transform& t = det.trf(s);
             A[0] = t.r[0]; A[1] = t.r[1];
@@ -1148,6 +1296,25 @@ struct Engine {
             const double rinv = 1.0 / perp(pos), ph = phi(pos);
             A[0] = -rinv * std::sin(ph); A[1] = rinv * std::cos(ph); A[2] = 0.0;
             A[3] = 0.0;                  A[4] = 0.0;                 A[5] = 1.0;
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            V3 nx, ny, nz;
+            cyl2_frame(s, pos, nx, ny, nz);
+            for (int c = 0; c < 3; ++c) {
+                A[0 * 3 + c] = nx[c];
+                A[1 * 3 + c] = ny[c];
+            }
+        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
+            // jacobian_polar.hpp:81-110
+            const dtr_transform& t = det.trf(s);
+            const V3 l = point_to_local(t, pos);
+            const double lr = perp(l), lp = phi(l), c = std::cos(lp), sn = std::sin(lp);
+            const V3 x = trf_axis(t, 0), y = trf_axis(t, 1);
+            const V3 row0 = add(mul(c, x), mul(sn, y));
+            const V3 row1 = mul(1.0 / lr, sub(mul(c, y), mul(sn, x)));
+            for (int k = 0; k < 3; ++k) {
+                A[0 * 3 + k] = row0[k];
+                A[1 * 3 + k] = row1[k];
+            }
         } else {
             const dtr_transform& t = det.trf(s);
             A[0] = t.r[0]; A[1] = t.r[3]; A[2] = t.r[6];
@@ -1180,6 +1347,10 @@ struct Engine {
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const double ph = phi(pos);
             n = {{std::cos(ph), std::sin(ph), 0.0}};
+        } else if (sf.shape == DTR_CYLINDER2D) {
+            // jacobian_cylindrical.hpp:47-62
+            V3 nx, ny;
+            cyl2_frame(s, pos, nx, ny, n);
         } else {
             n = trf_axis(det.trf(s), 2);
         }
@@ -1737,6 +1908,52 @@ int oracle_material_probe(const dtr_material_slab* mat, double qop, double path,
     out[1] = Engine::landau_sigma_qop(*mat, r, path, p.charge);
     out[2] = Engine::highland(x_over_x0, qop, p.charge, r.q2OverBeta2);
     return DTR_SUCCESS;
+}
+
+int oracle_surface_candidates(const dtr_detector_desc* det, uint32_t sf, const double ro[3],
+                              const double rd[3], double tol, double ext_tol, double overstep,
+                              double out[2][5]) {
+    using namespace oracle;
+    if (!det || !ro || !rd || !out || sf >= det->n_surfaces) return 0;
+    const Det D{det};
+    const V3 o{{ro[0], ro[1], ro[2]}}, d{{rd[0], rd[1], rd[2]}};
+    NavCfg cfg{static_cast<float>(tol), static_cast<float>(tol), 0.f, 1e-3f, static_cast<float>(overstep),
+               {0u, 0u}};
+    const dtr_surface& s = D.sf(sf);
+    const dtr_transform& t = D.trf(sf);
+    int found = 0;
+    auto record = [&](const Cand& is, const V3& glob) {
+        out[found][0] = is.path;
+        for (int i = 0; i < 3; ++i) out[found][1 + i] = glob[i];
+        out[found][4] = static_cast<double>(is.status);
+        ++found;
+    };
+    if (s.shape == DTR_CYLINDER2D) {
+        double sol[2];
+        const int n = cyl2_intersect(o, d, t, D.mask(s.mask).v[0], sol);
+        for (uint32_t k = 0; k < s.n_masks && found < 2; ++k) {
+            for (int i = 0; i < n && found < 2; ++i) {
+                if (!(std::fabs(sol[i]) < 10.0 * 1000.0)) continue;
+                Cand is;
+                const V3 glob = add(o, mul(sol[i], d));
+                resolve_mask(is, sol[i], glob, 0.0, sf, s, D.mask(s.mask + k), t, cfg, ext_tol);
+                if (is.probably_inside()) record(is, glob);
+            }
+        }
+        return found;
+    }
+    IPoint ip;
+    if (!intersect_surface(D, sf, o, d, cfg.overstep_tol, ip)) return 0;
+    const V3 glob = add(o, mul(ip.path, d));
+    for (uint32_t k = 0; k < s.n_masks; ++k) {
+        Cand is;
+        resolve_mask(is, ip.path, ip.glob, ip.z, sf, s, D.mask(s.mask + k), t, cfg, ext_tol);
+        if (is.probably_inside()) {
+            record(is, glob);
+            break;
+        }
+    }
+    return found;
 }
 
 int oracle_line_intersect(const double ro[3], const double rd[3], const dtr_transform* trf,

@@ -78,6 +78,15 @@ int oracle_material_probe(const dtr_material_slab* mat, double qop,
 int oracle_line_intersect(const double ro[3], const double rd[3], const dtr_transform* trf,
                           const dtr_mask* mask, double tol, double out[4]);
 
+/* Candidate intersections of a ray with surface `sf` of a detector as the
+ * navigator's candidate search makes them (every solution a mask accepts,
+ * fixed mask tolerance `tol`, external tolerance `ext_tol`, overstep
+ * tolerance `overstep`): out[k] = {path, x, y, z, status (0 inside, 1 edge)}
+ * for k < return value (at most 2). */
+int oracle_surface_candidates(const dtr_detector_desc* det, uint32_t sf, const double ro[3],
+                              const double rd[3], double tol, double ext_tol, double overstep,
+                              double out[2][5]);
+
 #ifdef __cplusplus
 }
 #endif

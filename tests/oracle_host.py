@@ -206,6 +206,17 @@ def field_at(field, xyz: np.ndarray) -> np.ndarray:
     return out
 
 
+def surface_candidates(detector, sf: int, ro, rd, tol: float, ext_tol: float = 0.0, overstep: float = 0.0):
+    """The navigator's candidate intersections with one surface:
+    list of (path, global point, status 0 inside / 1 edge)."""
+    lib = load()
+    out = (C.c_double * 10)()
+    n = lib.oracle_surface_candidates(C.byref(detector.desc), C.c_uint32(sf), (C.c_double * 3)(*ro),
+                                      (C.c_double * 3)(*rd), C.c_double(tol), C.c_double(ext_tol),
+                                      C.c_double(overstep), out)
+    return [(out[5 * k], np.array(out[5 * k + 1:5 * k + 4]), int(out[5 * k + 4])) for k in range(n)]
+
+
 def line_intersect(ro, rd, transform, mask, tol: float):
     """Ray-wire intersection of the oracle: (valid, path, loc0, loc1, inside)."""
     lib = load()

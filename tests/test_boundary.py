@@ -30,7 +30,7 @@ def test_library_exports_all_header_symbols():
 
 def test_struct_sizes_match_header_layout():
     assert C.sizeof(capi.Transform) == 96
-    assert C.sizeof(capi.Mask) == 40
+    assert C.sizeof(capi.Mask) == 64
     assert C.sizeof(capi.Surface) == 32
     assert C.sizeof(capi.Volume) == 28
     assert C.sizeof(capi.MaterialSlab) == 80

@@ -1,0 +1,34 @@
+"""Annulus and placed-cylinder surfaces on the device: the plain FP64 kernel
+against the oracle at the FP64 contract (telescopes of annuli and of
+cylinders across the beam, with material, constant field with and without
+noise estimation); the batched launches reject such detectors."""
+
+import pytest
+
+import detray_amd as da
+from detray_amd import capi
+from tests.test_gpu_parity import assert_parity, run_both
+from tests.test_shapes import _modified_telescope
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("noise", [False, True])
+@pytest.mark.parametrize("kind,n", [("annulus", 6), ("cylinder", 3)])
+def test_plain_kernel_matches_oracle(kind, n, noise):
+    det = _modified_telescope(kind, n=n)
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(estimate_scattering_noise=noise)
+    got, ref = run_both(det, field, cfg, da.generate_tracks("beam", 4000, 42, sigma_theta=0.1))
+    assert ref["n_surfaces"].mean() > 3
+    assert_parity(got, ref)
+
+
+def test_batched_launch_rejects_new_shapes():
+    det = _modified_telescope("annulus")
+    tracks = torch.from_numpy(da.generate_tracks("beam", 64, 42)).cuda()
+    with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
+        da.Propagator(da.PropagationConfig(), da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")).propagate(
+            det, da.MagneticField.constant(0, 0, 2 * da.TESLA), tracks)

@@ -314,9 +314,14 @@ def test_errors(tmp_path, toy3_files):
     assert code == IO and "line 1" in msg
 
     geo = load(f["geometry"])
-    geo["data"]["volumes"][3]["surfaces"][5]["masks"][0]["shape"] = 0  # annulus2
+    geo["data"]["volumes"][3]["surfaces"][5]["masks"][0]["shape"] = 1  # cuboid3
     code, msg = read_error(dump(tmp_path / "a.json", geo))
-    assert code == UNSUPPORTED and "annulus2" in msg
+    assert code == UNSUPPORTED and "cuboid3" in msg
+
+    geo = load(f["geometry"])
+    geo["data"]["volumes"][3]["surfaces"][5]["masks"][0]["shape"] = 0  # annulus2 needs 7 boundaries
+    code, msg = read_error(dump(tmp_path / "a2.json", geo))
+    assert code == INVALID_ARGUMENT and "7 numbers" in msg
 
     geo = load(f["geometry"])
     del geo["data"]["volumes"][2]["surfaces"][0]["index_in_coll"]
