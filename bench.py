@@ -17,7 +17,9 @@ same batch for the speed-up, and the two are compared track by track
 (surface sequences, bound parameters).  ``--workload loopers`` runs the
 batching workload instead: 2M tracks, pT 0.1-100 GeV, |eta| < 3, in batches
 of 262144 replayed from HIP graphs, with p50/p99 per-batch latency.
-``--workload adjoint`` runs the differentiation workload: 262144 tracks
+``--workload wire_chamber`` runs the reference's wire-chamber benchmark case
+on the plain FP64 kernel.  ``--workload adjoint`` runs the differentiation
+workload: 262144 tracks
 from the beam pipe (pT 0.5-50 GeV, |eta| < 2.5), hits = true intersections
 of 1-sigma perturbed start parameters + smear (10 um, 50 um); one step is
 the recording forward pass and the adjoint backward pass (checkpoint every
@@ -108,7 +110,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tracks", type=int, default=None)
-    ap.add_argument("--workload", choices=("toy_solenoid", "loopers", "adjoint"), default="toy_solenoid")
+    ap.add_argument("--workload", choices=("toy_solenoid", "loopers", "adjoint", "wire_chamber"),
+                    default="toy_solenoid")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=None)
@@ -128,6 +131,8 @@ def main() -> int:
 
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "wire_chamber":
+        return bench_wire_chamber(args)
 
     import numpy as np
     import torch
@@ -330,6 +335,90 @@ def main() -> int:
             "traffic": traffic_bytes(not loopers),
             "alg_bytes_per_launch": bytes_per_step / len(subs),
         },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_wire_chamber(args) -> int:
+    """The reference's WIRE_CHAMBER_W_COV_TRANSPORT benchmark case
+    (tests/benchmarks/cpu/propagation.cpp:69-73, 143-146): the wire chamber
+    with half length 500 mm, constant 2 T field, 512 x 512 tracks from the
+    origin, |eta| < 3, 1-100 GeV, covariance transport, no noise estimation;
+    the plain FP64 kernel (the batched kernels do not take wires)."""
+    import numpy as np
+    import torch
+
+    import detray_amd as da
+
+    dev = torch.device("cuda:0")
+    det = da.Detector.wire_chamber(half_z=500.0)
+    field = da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA)
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+    n = args.tracks or 512 * 512
+    seed = args.seed if args.seed is not None else 48
+    fp_host = da.generate_tracks("toy", n, seed, eta_min=-3.0, eta_max=3.0, sigma_xy=0.0, sigma_z=0.0)
+    tracks = torch.from_numpy(fp_host).to(dev)
+    prop = da.Propagator(cfg)
+    out = prop.allocate(n, dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        prop.propagate(det, field, tracks, out=out)
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        evs[s][0].record(stream)
+        prop.propagate(det, field, tracks, out=out)
+        evs[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    kern_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    n_sf = out.n_surfaces.cpu().numpy()
+    status = out.status.cpu().numpy()
+    value = n / wall
+    bytes_per_step = alg_bytes(n_sf)
+    achieved = bytes_per_step / (float(np.mean(kern_ms)) * 1e-3) / 1e9
+    print(f"[bench] wire chamber: {wall * 1e3:.2f} ms/step, {value:.3e} tracks/s, mean N_sf "
+          f"{n_sf.mean():.2f}, status {np.bincount(status, minlength=4).tolist()}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        from tests import oracle_host
+
+        m = min(args.cpu_sample, n)
+        sample = np.ascontiguousarray(fp_host[:, :m])
+        threads = os.cpu_count() or 1
+        oracle_host.propagate(det, field, cfg, np.ascontiguousarray(sample[:, :max(1, m // 10)]), threads=threads)
+        times = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            oracle_host.propagate(det, field, cfg, sample, threads=threads)
+            times.append(time.perf_counter() - t1)
+        cpu = {"value": m / float(np.median(times)), "unit": "tracks/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} tracks of the same workload, FP64 oracle, OpenMP dynamic/1 over "
+                         f"tracks, median of 3 runs"}
+    line = {
+        "metric": "tracks/s (wire chamber, constant 2 T, covariance transport)",
+        "value": value,
+        "unit": "tracks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": "wire chamber 10 layers half_z 500 mm drift cells, tungsten rods, constant "
+                               "2 T, 262144 muons from the origin |eta|<3 pT 1-100 GeV log-uniform",
+                   "tracks": n, "kernel": "propagate_fp64_kernel", "parallelism": "1 GPU"},
+        "latency_ms": {"per": "step", "p50": float(np.percentile(kern_ms, 50)),
+                       "p99": float(np.percentile(kern_ms, 99))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": bytes_per_step},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -61,6 +61,9 @@ int upload(const T* host, uint32_t n, void** dst) {
     return to_status(e);
 }
 
+// kExt: detectors with wires, annuli, placed cylinders, rods, multi-surface
+// bins or rings with material
+template <bool kExt>
 __global__ void __launch_bounds__(256) propagate_fp64_kernel(
     dtr::DevDetector det, dtr::DevField field, dtr::DevConfig cfg,
     const double* __restrict__ fp, const double* __restrict__ init_cov,
@@ -68,7 +71,7 @@ __global__ void __launch_bounds__(256) propagate_fp64_kernel(
     dtr::DevResults out, uint32_t n) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n) return;
-    dtr::propagate_track(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
+    dtr::propagate_track<kExt>(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
 }
 
 // LDS of a block: the candidate cache, slot-major ([8][blockDim] paths, then
@@ -506,6 +509,9 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
                 (sf.material != 0xffffffffu && sf.material >= d->n_materials))
                 rc = DTR_ERROR_INVALID_ARGUMENT;
             if (sf.shape >= DTR_LINE_SQUARE) h->wires = true;
+            // a ring the parameter updater stops at needs the polar Jacobians
+            if (sf.shape == DTR_RING2D && (sf.sf_id == DTR_SENSITIVE || sf.material != 0xffffffffu))
+                h->wires = true;
         }
         for (uint32_t k = 0; k < d->n_materials && !rc; ++k) {
             if (d->materials[k].kind > DTR_MATERIAL_ROD) rc = DTR_ERROR_INVALID_ARGUMENT;
@@ -645,10 +651,16 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
     }
     const uint32_t block = 256;
     const uint32_t grid = (in->n + block - 1) / block;
-    hipLaunchKernelGGL(propagate_fp64_kernel, dim3(grid), dim3(block), 0,
-                       static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
-                       in->free_params, in->initial_cov, in->bound_params, in->start_surface,
-                       r, in->n);
+    if (det->wires)
+        hipLaunchKernelGGL(propagate_fp64_kernel<true>, dim3(grid), dim3(block), 0,
+                           static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
+                           in->free_params, in->initial_cov, in->bound_params, in->start_surface,
+                           r, in->n);
+    else
+        hipLaunchKernelGGL(propagate_fp64_kernel<false>, dim3(grid), dim3(block), 0,
+                           static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
+                           in->free_params, in->initial_cov, in->bound_params, in->start_surface,
+                           r, in->n);
     return to_status(hipGetLastError());
 }
 

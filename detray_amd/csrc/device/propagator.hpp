@@ -218,7 +218,12 @@ DTR_HD int covi(int r, int c) {
     return r * 6 - (r * (r - 1)) / 2 + (c - r);
 }
 
-struct Engine {
+// kExt: the engine also handles wires, annuli, placed cylinders, rods,
+// multi-surface bins and rings with material (the extended geometry); the
+// instantiation without them is the plain FP64 kernel of detectors made of
+// rectangles, trapezoids and portal cylinders/rings
+template <bool kExt>
+struct EngineT {
     const DevDetector& det;
     const DevField& field;
     const DevConfig& cfg;
@@ -269,7 +274,7 @@ struct Engine {
                     const double ft = base_tol + ext_tol;
                     edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (m.v[2] + ft));
                 }
-            } else if (m.shape == DTR_LINE_SQUARE || m.shape == DTR_LINE_CIRCULAR) {
+            } else if (kExt && (m.shape == DTR_LINE_SQUARE || m.shape == DTR_LINE_CIRCULAR)) {
                 // line.hpp:86-121: square or circular cross section, half length
                 const double c = fabs(loc.z);
                 double a, b2;
@@ -287,7 +292,7 @@ struct Engine {
                     const double ft = base_tol + ext_tol;
                     edge = (a <= (m.v[0] + ft) && b2 <= (m.v[0] + ft) && c <= (m.v[1] + ft));
                 }
-            } else if (m.shape == DTR_ANNULUS2D) {
+            } else if (kExt && m.shape == DTR_ANNULUS2D) {
                 // annulus2D.hpp:140-190: beam-frame radius, strip-frame phi
                 const double nx = loc.x + m.v[5], ny = loc.y + m.v[6];
                 const double r_beam = sqrt(fma(nx, nx, ny * ny));
@@ -310,7 +315,7 @@ struct Engine {
                         edge = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
                     }
                 }
-            } else if (m.shape == DTR_CYLINDER2D) {
+            } else if (kExt && m.shape == DTR_CYLINDER2D) {
                 // cylinder2D.hpp:63-90: local z within the bounds
                 inside = ((m.v[1] - base_tol) <= loc.z && loc.z <= (m.v[2] + base_tol));
                 edge = false;
@@ -430,9 +435,9 @@ struct Engine {
         const dtr_surface& sf = det.surfaces[s];
         if (sf.shape == DTR_CONCENTRIC_CYLINDER2D)
             return cyl_intersect(ro, rd, det.masks[sf.mask].v[0], overstep, path, z);
-        if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR)
+        if (kExt && (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR))
             return line_intersect(ro, rd, det.transforms[sf.transform], path, glob);
-        if (sf.shape == DTR_CYLINDER2D) {
+        if (kExt && sf.shape == DTR_CYLINDER2D) {
             // intersection_update takes the smaller solution
             double sol[2];
             if (cyl2_intersect(ro, rd, det.transforms[sf.transform], det.masks[sf.mask].v[0], sol) == 0)
@@ -475,7 +480,7 @@ struct Engine {
         const V3 rd = tk.dir;  // forward navigation
         const dtr_surface& sf = det.surfaces[s];
         const dtr_transform& t = det.transforms[sf.transform];
-        if (sf.shape == DTR_CYLINDER2D) {
+        if (kExt && sf.shape == DTR_CYLINDER2D) {
             // intersection_initialize: every valid solution a mask accepts,
             // at most two
             double sol[2];
@@ -593,7 +598,7 @@ struct Engine {
             for (int b1 = a1; b1 < e1; ++b1) {
                 const int i1 = cyl ? b1 : (n1 + (b1 % n1)) % n1;
                 // every entry of the bin in fill order (capacity 0 means 1)
-                const uint32_t cap = g.capacity ? g.capacity : 1u;
+                const uint32_t cap = kExt ? (g.capacity ? g.capacity : 1u) : 1u;
                 const uint32_t first = g.bins + (static_cast<uint32_t>(i0) * g.n1 + i1) * cap;
                 for (uint32_t k = 0; k < cap; ++k) {
                     const uint32_t s = det.bins[first + k];
@@ -886,16 +891,16 @@ struct Engine {
 
     DTR_HD void free_to_bound(uint32_t s, double out[6]) const {
         const dtr_surface& sf = det.surfaces[s];
-        if (is_line(sf.shape)) {
+        if (kExt && is_line(sf.shape)) {
             line_local(det.transforms[sf.transform], tk.pos, tk.dir, out[0], out[1]);
         } else if (sf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi_of(tk.pos);
             out[1] = tk.pos.z;
-        } else if (sf.shape == DTR_CYLINDER2D) {
+        } else if (kExt && sf.shape == DTR_CYLINDER2D) {
             const V3 l = to_local(det.transforms[sf.transform], tk.pos);
             out[0] = perp(l) * phi_of(l);
             out[1] = l.z;
-        } else if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D) {
+        } else if (sf.shape == DTR_RING2D || (kExt && sf.shape == DTR_ANNULUS2D)) {
             const V3 l = to_local(det.transforms[sf.transform], tk.pos);
             out[0] = perp(l);
             out[1] = phi_of(l);
@@ -915,7 +920,7 @@ struct Engine {
     }
     DTR_HD V3 bound_pos(uint32_t s, const double b[6]) const {
         const dtr_surface& sf = det.surfaces[s];
-        if (is_line(sf.shape)) {
+        if (kExt && is_line(sf.shape)) {
             // line2D.hpp:80-99 with the track direction
             const dtr_transform& t = det.transforms[sf.transform];
             const V3 r = cross(trf_axis(t, 2), bound_dir(b));
@@ -925,12 +930,12 @@ struct Engine {
             const double r = det.masks[sf.mask].v[0];
             return {r * cos(b[0]), r * sin(b[0]), b[1]};
         }
-        if (sf.shape == DTR_CYLINDER2D) {
+        if (kExt && sf.shape == DTR_CYLINDER2D) {
             const double r = det.masks[sf.mask].v[0];
             const double ph = b[0] / r;
             return to_global(det.transforms[sf.transform], {r * cos(ph), r * sin(ph), b[1]});
         }
-        if (sf.shape == DTR_RING2D || sf.shape == DTR_ANNULUS2D)
+        if (sf.shape == DTR_RING2D || (kExt && sf.shape == DTR_ANNULUS2D))
             return to_global(det.transforms[sf.transform], {b[0] * cos(b[1]), b[0] * sin(b[1]), 0.0});
         return to_global(det.transforms[sf.transform], {b[0], b[1], 0.0});
     }
@@ -949,7 +954,7 @@ struct Engine {
         double A[6];
         // dpos/dangle of a line (jacobian_line.hpp:102-172), zero otherwise
         double Pa[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-        if (is_line(dsf.shape)) {
+        if (kExt && is_line(dsf.shape)) {
             const dtr_transform& t = det.transforms[dsf.transform];
             const V3 ddir = bound_dir(tk.bound);
             const V3 dp = bound_pos(dep, tk.bound);
@@ -975,13 +980,13 @@ struct Engine {
             A[0] = -r * sin(ph); A[1] = 0.0;
             A[2] = r * cos(ph);  A[3] = 0.0;
             A[4] = 0.0;          A[5] = 1.0;
-        } else if (dsf.shape == DTR_CYLINDER2D) {
+        } else if (kExt && dsf.shape == DTR_CYLINDER2D) {
             V3 nx, ny, nz;
             cyl2_frame(det.transforms[dsf.transform], bound_pos(dep, tk.bound), nx, ny, nz);
             A[0] = nx.x; A[1] = ny.x;
             A[2] = nx.y; A[3] = ny.y;
             A[4] = nx.z; A[5] = ny.z;
-        } else if (dsf.shape == DTR_RING2D || dsf.shape == DTR_ANNULUS2D) {
+        } else if (kExt && (dsf.shape == DTR_RING2D || dsf.shape == DTR_ANNULUS2D)) {
             // jacobian_polar.hpp:44-72
             const dtr_transform& t = det.transforms[dsf.transform];
             const V3 l = to_local(t, bound_pos(dep, tk.bound));
@@ -1013,8 +1018,12 @@ struct Engine {
             // pos rows: J_pos = [I | Jp(4..7)]: I*A for loc, Jp[4..6]*C for angles, Jp7 for qop
             Mp[r][0] = A[r * 2 + 0];
             Mp[r][1] = A[r * 2 + 1];
-            Mp[r][2] = Pa[r][0] + tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
-            Mp[r][3] = Pa[r][1] + tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
+            Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            if (kExt) {
+                Mp[r][2] += Pa[r][0];
+                Mp[r][3] += Pa[r][1];
+            }
             Mp[r][4] = tk.J[jpos(r, 7)];
             Mp[r][5] = 0.0;
             Md[r][0] = 0.0;
@@ -1030,7 +1039,7 @@ struct Engine {
         const V3 k = dtds();
         const dtr_surface& esf = det.surfaces[dest];
         V3 f2p, f2d{0.0, 0.0, 0.0};
-        if (is_line(esf.shape)) {
+        if (kExt && is_line(esf.shape)) {
             // jacobian_line.hpp:64-92: position and direction parts with dt/ds
             const dtr_transform& t = det.transforms[esf.transform];
             const V3 pc = sub(tk.pos, trf_translation(t));
@@ -1046,7 +1055,7 @@ struct Engine {
             if (esf.shape == DTR_CONCENTRIC_CYLINDER2D) {
                 const double ph = phi_of(tk.pos);
                 n = {cos(ph), sin(ph), 0.0};
-            } else if (esf.shape == DTR_CYLINDER2D) {
+            } else if (kExt && esf.shape == DTR_CYLINDER2D) {
                 V3 nx, ny;
                 cyl2_frame(det.transforms[esf.transform], tk.pos, nx, ny, n);
             } else {
@@ -1055,9 +1064,10 @@ struct Engine {
             f2p = mul(-1.0 / dot(n, tk.dir), n);
         }
         double proj[6];  // f2p . M (per column)
-        for (int c = 0; c < 6; ++c)
-            proj[c] = f2p.x * Mp[0][c] + f2p.y * Mp[1][c] + f2p.z * Mp[2][c] +
-                      (f2d.x * Md[0][c] + f2d.y * Md[1][c] + f2d.z * Md[2][c]);
+        for (int c = 0; c < 6; ++c) {
+            proj[c] = f2p.x * Mp[0][c] + f2p.y * Mp[1][c] + f2p.z * Mp[2][c];
+            if (kExt) proj[c] += f2d.x * Md[0][c] + f2d.y * Md[1][c] + f2d.z * Md[2][c];
+        }
         const double p2f_pos[3] = {tk.dir.x, tk.dir.y, tk.dir.z};
         const double p2f_dir[3] = {k.x, k.y, k.z};
         for (int r = 0; r < 3; ++r) {
@@ -1068,7 +1078,7 @@ struct Engine {
         }
         // free-to-bound at the destination
         double Bl[6];
-        if (is_line(esf.shape)) {
+        if (kExt && is_line(esf.shape)) {
             // jacobian_line.hpp:174-182: rows x and y of the line's frame
             V3 nx, ny, nz;
             line_frame(det.transforms[esf.transform], tk.dir, nx, ny, nz);
@@ -1078,12 +1088,12 @@ struct Engine {
             const double rinv = 1.0 / perp(tk.pos), ph = phi_of(tk.pos);
             Bl[0] = -rinv * sin(ph); Bl[1] = rinv * cos(ph); Bl[2] = 0.0;
             Bl[3] = 0.0;             Bl[4] = 0.0;            Bl[5] = 1.0;
-        } else if (esf.shape == DTR_CYLINDER2D) {
+        } else if (kExt && esf.shape == DTR_CYLINDER2D) {
             V3 nx, ny, nz;
             cyl2_frame(det.transforms[esf.transform], tk.pos, nx, ny, nz);
             Bl[0] = nx.x; Bl[1] = nx.y; Bl[2] = nx.z;
             Bl[3] = ny.x; Bl[4] = ny.y; Bl[5] = ny.z;
-        } else if (esf.shape == DTR_RING2D || esf.shape == DTR_ANNULUS2D) {
+        } else if (kExt && (esf.shape == DTR_RING2D || esf.shape == DTR_ANNULUS2D)) {
             // jacobian_polar.hpp:81-110
             const dtr_transform& t = det.transforms[esf.transform];
             const V3 l = to_local(t, tk.pos);
@@ -1155,7 +1165,7 @@ struct Engine {
         const dtr_surface& msf = det.surfaces[dest];
         if (msf.shape == DTR_CONCENTRIC_CYLINDER2D) {
             nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
-        } else if (msf.shape == DTR_CYLINDER2D) {
+        } else if (kExt && msf.shape == DTR_CYLINDER2D) {
             // cylindrical2D.hpp:62-70: phi = loc0 / r of the mask
             const double ph = tk.bound[0] / det.masks[msf.mask].v[0];
             nrm = vec_to_global(det.transforms[msf.transform], {cos(ph), sin(ph), 0.0});
@@ -1166,7 +1176,7 @@ struct Engine {
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];
         double path_seg, path_x0;
-        if (m.kind == DTR_MATERIAL_ROD) {
+        if (kExt && m.kind == DTR_MATERIAL_ROD) {
             // material_rod.hpp:62-78, approach = bound loc0
             const double a = tk.bound[0];
             path_seg = 0.0;
@@ -1285,8 +1295,10 @@ struct Engine {
         if (on_material()) material_interaction(dest);
     }
 };
+using Engine = EngineT<false>;
 
 // One track: propagator::propagate (propagator.hpp:262-400)
+template <bool kExt>
 DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
                             const DevConfig& cfg, const double* __restrict__ fp,
                             const double* __restrict__ init_cov,
@@ -1326,7 +1338,7 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     tk.n_seq = 0;
     tk.stopped = false;
 
-    Engine e{det, field, cfg, tk};
+    EngineT<kExt> e{det, field, cfg, tk};
     const uint64_t N = n;
     if (bound0) {
         // bound start (propagator.hpp:121-162, base_stepper.hpp:84-108)
