@@ -423,10 +423,11 @@ dtr::AdjointIO make_adjoint_io(const dtr_adjoint_config* a, const dtr_adjoint_ba
     return io;
 }
 
-// use_field_gradient: the Jacobian with position columns, one work-item per
-// track with per-lane navigation
+// One work-item per track with per-lane navigation on the grid schedule:
+// use_field_gradient (the Jacobian with position columns) and the extended
+// geometry
 template <class P>
-int launch_grad(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+int launch_lane(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
                 const dtr_track_batch* in, const dtr::DevResults& r, uint32_t block, const uint32_t* order,
                 hipStream_t stream) {
     const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
@@ -646,7 +647,7 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
         if (det->wires) return DTR_ERROR_UNSUPPORTED;
         // the Jacobian with position columns runs on the batched engine's
         // FP64 policy (same results contract as the plain kernel)
-        return launch_grad<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
+        return launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
                                                                     static_cast<hipStream_t>(stream));
     }
     const uint32_t block = 256;
@@ -722,7 +723,6 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     const int rc = check_propagate_args(det, field, c, in, out);
     if (rc != DTR_SUCCESS) return rc;
     if (!launch) return DTR_ERROR_INVALID_ARGUMENT;
-    if (det->wires) return DTR_ERROR_UNSUPPORTED;
     dtr_launch_config L = *launch;
     if (L.block_size == 0u) L.block_size = 256u;
     if (L.waves_per_simd == 0u) L.waves_per_simd = 2u;
@@ -742,12 +742,21 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (det->wires) {
+        if (c->use_field_gradient || L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE)
+            return DTR_ERROR_UNSUPPORTED;
+        if (L.precision == DTR_PRECISION_FP64)
+            return launch_lane<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size,
+                                                                           L.order, st);
+        return launch_lane<dtr::WithExtendedGeometry<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order,
+                                                                        st);
+    }
     if (c->use_field_gradient) {
         if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
         if (L.precision == DTR_PRECISION_FP64)
-            return launch_grad<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size, L.order,
+            return launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size, L.order,
                                                                         st);
-        return launch_grad<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order, st);
+        return launch_lane<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order, st);
     }
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);

@@ -97,11 +97,21 @@ struct PolicyFP64 {
     using S = double;
     static constexpr bool kMixed = false;
     static constexpr bool kGrad = false;
+    static constexpr bool kExt = false;
 };
 struct PolicyMixed {
     using S = float;
     static constexpr bool kMixed = true;
     static constexpr bool kGrad = false;
+    static constexpr bool kExt = false;
+};
+// Detectors with wires, annuli, placed cylinders, material rods, rings with
+// material and grid bins of several surfaces (the extended geometry): the
+// engine resolves these shapes, their local frames and Jacobians, and every
+// entry of a bin.
+template <class Base>
+struct WithExtendedGeometry : Base {
+    static constexpr bool kExt = true;
 };
 // stepping::config::use_field_gradient: the transport Jacobian also carries
 // the position columns (rk_stepper.hpp:878-937)
@@ -413,6 +423,26 @@ struct FastEngine {
                 r.r[6] * l.x + r.r[7] * l.y + r.r[8] * l.z + r.t[2]};
     }
 
+    static DTR_AI dtr_transform rec_trf(const NavSurface& r) {
+        dtr_transform t;
+        for (int i = 0; i < 9; ++i) t.r[i] = r.r[i];
+        for (int i = 0; i < 3; ++i) t.t[i] = r.t[i];
+        return t;
+    }
+    static DTR_AI bool is_line(uint8_t shape) { return shape == DTR_LINE_SQUARE || shape == DTR_LINE_CIRCULAR; }
+    // mask values of the first mask (annuli have seven, the record holds four)
+    DTR_AI const double* first_mask(const NavSurface& rec) const {
+        return (P::kExt && rec.shape == DTR_ANNULUS2D) ? det.masks[rec.mask].v : rec.m;
+    }
+    // intersection_update of a placed cylinder: the smaller solution
+    DTR_AI bool cyl2_update(const NavSurface& rec, double& path, V3& glob) const {
+        double sol[2];
+        if (Engine::cyl2_intersect(tk.pos, tk.dir, rec_trf(rec), rec.m[0], sol) == 0) return false;
+        path = sol[0];
+        glob = add(tk.pos, mul(sol[0], tk.dir));
+        return fabs(sol[0]) < 10.0 * 1000.0;
+    }
+
     // ray_plane_intersector.hpp:60-87 on a record
     static DTR_AI bool plane_rec(V3 ro, V3 rd, const NavSurface& r, double& s, V3& glob) {
         const V3 sn = rec_axis(r, 2);
@@ -464,6 +494,8 @@ struct FastEngine {
                     const double ft = base_tol + ext_tol;
                     edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (mv[2] + ft));
                 }
+            } else if (P::kExt && shape >= DTR_LINE_SQUARE) {
+                ext_mask_check(shape, mv, loc, base_tol, ext_tol, inside, edge);
             } else {
                 const double r = perp(loc);
                 inside = ((r + base_tol) >= mv[0] && r <= (mv[1] + base_tol));
@@ -488,12 +520,16 @@ struct FastEngine {
         bool valid;
         if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             valid = Engine::cyl_intersect(tk.pos, tk.dir, rec.m[0], overstep, path, z);
+        } else if (P::kExt && is_line(rec.shape)) {
+            valid = Engine::line_intersect(tk.pos, tk.dir, rec_trf(rec), path, glob);
+        } else if (P::kExt && rec.shape == DTR_CYLINDER2D) {
+            valid = cyl2_update(rec, path, glob);
         } else {
             valid = plane_rec(tk.pos, tk.dir, rec, path, glob);
         }
         if (!valid) return false;
         if (rec.n_masks == 1) {
-            resolve(out, path, glob, z, s, rec.sf_id, rec.m, rec.shape, rec.link, rec, overstep);
+            resolve(out, path, glob, z, s, rec.sf_id, first_mask(rec), rec.shape, rec.link, rec, overstep);
             return out.status <= kEdge;
         }
         for (uint32_t k = 0; k < rec.n_masks; ++k) {
@@ -536,6 +572,12 @@ struct FastEngine {
     }
 
     DTR_AI void candidate_search(const NavSurface& rec, float overstep) {
+        if constexpr (P::kExt) {
+            if (rec.shape == DTR_CYLINDER2D) {
+                cyl2_search(rec, overstep);
+                return;
+            }
+        }
         Cand is;
         is.path = kMax;
         is.sf = kInvalid;
@@ -543,6 +585,32 @@ struct FastEngine {
         is.status = kOutside;
         is.pad = 0;
         if (evaluate(rec, overstep, is)) insert_sorted(is);
+    }
+
+    // intersection_initialize of a placed cylinder: every valid solution a
+    // mask accepts, at most two (intersection_kernel.hpp:24-168)
+    DTR_AI void cyl2_search(const NavSurface& rec, float overstep) {
+        double sol[2];
+        const int n = Engine::cyl2_intersect(tk.pos, tk.dir, rec_trf(rec), rec.m[0], sol);
+        for (uint32_t k = 0; k < rec.n_masks; ++k) {
+            const dtr_mask& m = det.masks[rec.mask + k];
+            int found = 0;
+            for (int i = 0; i < n; ++i) {
+                if (!(fabs(sol[i]) < 10.0 * 1000.0)) continue;
+                Cand is;
+                is.path = kMax;
+                is.sf = kInvalid;
+                is.link = 0;
+                is.status = kOutside;
+                is.pad = 0;
+                resolve(is, sol[i], add(tk.pos, mul(sol[i], tk.dir)), 0.0, rec.index, rec.sf_id, m.v, rec.shape,
+                        m.volume_link, rec, overstep);
+                if (is.status <= kEdge) {
+                    insert_sorted(is);
+                    if (++found == 2) return;
+                }
+            }
+        }
     }
 
     DTR_AI bool update_candidate(int slot) {
@@ -555,13 +623,17 @@ struct FastEngine {
         bool valid;
         if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             valid = Engine::cyl_intersect(tk.pos, tk.dir, rec.m[0], cfg.overstep_tol, path, z);
+        } else if (P::kExt && is_line(rec.shape)) {
+            valid = Engine::line_intersect(tk.pos, tk.dir, rec_trf(rec), path, glob);
+        } else if (P::kExt && rec.shape == DTR_CYLINDER2D) {
+            valid = cyl2_update(rec, path, glob);
         } else {
             valid = plane_rec(tk.pos, tk.dir, rec, path, glob);
         }
         if (!valid) return false;
         bool ok = false;
         if (rec.n_masks == 1) {
-            resolve(cd, path, glob, z, s, rec.sf_id, rec.m, rec.shape, rec.link, rec, cfg.overstep_tol);
+            resolve(cd, path, glob, z, s, rec.sf_id, first_mask(rec), rec.shape, rec.link, rec, cfg.overstep_tol);
             ok = cd.status <= kEdge;
         } else {
             for (uint32_t k = 0; k < rec.n_masks; ++k) {
@@ -684,6 +756,10 @@ struct FastEngine {
     // The volume's brute-force surfaces, evaluated two at a time (independent
     // record loads and intersections) and inserted in list order.
     DTR_AI void bf_search(const dtr_volume& vol, float overstep) {
+        if constexpr (P::kExt) {
+            for (uint32_t k = vol.bf_begin; k < vol.bf_end; ++k) candidate_search(det.bf_rec[k].rec, overstep);
+            return;
+        }
         uint32_t k = vol.bf_begin;
         for (; k + 2 <= vol.bf_end; k += 2) {
             Cand c0, c1;
@@ -713,6 +789,24 @@ struct FastEngine {
                     oz = static_cast<float>(tk.pos.z);
         const float dx = static_cast<float>(tk.dir.x), dy = static_cast<float>(tk.dir.y),
                     dz = static_cast<float>(tk.dir.z);
+        if constexpr (P::kExt) {
+            // every entry of each bin in fill order (capacity 0 means 1);
+            // empty entries fail the prefilter
+            const uint32_t cap = g.capacity ? g.capacity : 1u;
+            for (int b0 = a0; b0 < e0; ++b0) {
+                const int i0 = cyl ? wrap_bin(b0, n0) : b0;
+                for (int b1 = a1; b1 < e1; ++b1) {
+                    const int i1 = cyl ? b1 : wrap_bin(b1, n1);
+                    const uint32_t first =
+                        g.bins + (static_cast<uint32_t>(i0) * g.n1 + static_cast<uint32_t>(i1)) * cap;
+                    for (uint32_t k = 0; k < cap; ++k) {
+                        const NavRecord& e = det.bin_rec[first + k];
+                        if (prefilter(e.box, ox, oy, oz, dx, dy, dz, margin, ft)) candidate_search(e.rec, overstep);
+                    }
+                }
+            }
+            return;
+        }
         auto bin_of = [&](int j) {
             const int b0 = a0 + j / w, b1 = a1 + j % w;
             const int i0 = cyl ? wrap_bin(b0, n0) : b0;
@@ -1200,7 +1294,13 @@ struct FastEngine {
         if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             out[0] = phi_of(tk.pos);
             out[1] = tk.pos.z;
-        } else if (rec.shape == DTR_RING2D) {
+        } else if (P::kExt && is_line(rec.shape)) {
+            Engine::line_local(rec_trf(rec), tk.pos, tk.dir, out[0], out[1]);
+        } else if (P::kExt && rec.shape == DTR_CYLINDER2D) {
+            const V3 l = rec_to_local(rec, tk.pos);
+            out[0] = perp(l) * phi_of(l);
+            out[1] = l.z;
+        } else if (rec.shape == DTR_RING2D || (P::kExt && rec.shape == DTR_ANNULUS2D)) {
             const V3 l = rec_to_local(rec, tk.pos);
             out[0] = perp(l);
             out[1] = phi_of(l);
@@ -1220,7 +1320,19 @@ struct FastEngine {
             const double r = rec.m[0];
             return {r * cos(b[0]), r * sin(b[0]), b[1]};
         }
-        if (rec.shape == DTR_RING2D)
+        if constexpr (P::kExt) {
+            if (is_line(rec.shape)) {
+                // line2D.hpp:80-99 with the track direction
+                const V3 r = cross(rec_axis(rec, 2), Engine::bound_dir(b));
+                return add(rec_to_global(rec, {0.0, 0.0, b[1]}), mul(b[0], normalize(r)));
+            }
+            if (rec.shape == DTR_CYLINDER2D) {
+                const double r = rec.m[0];
+                const double ph = b[0] / r;
+                return rec_to_global(rec, {r * cos(ph), r * sin(ph), b[1]});
+            }
+        }
+        if (rec.shape == DTR_RING2D || (P::kExt && rec.shape == DTR_ANNULUS2D))
             return rec_to_global(rec, {b[0] * cos(b[1]), b[0] * sin(b[1]), 0.0});
         return rec_to_global(rec, {b[0], b[1], 0.0});
     }
@@ -1232,17 +1344,103 @@ struct FastEngine {
         return tk.dtds3;
     }
 
+    // normal of a placed cylinder at the bound position (cylindrical2D.hpp:62-70:
+    // phi = loc0 / r of the mask)
+    DTR_AI V3 cyl2_normal(const NavSurface& rec) const {
+        const double ph = tk.bound[0] / rec.m[0];
+        return vec_to_global(rec_trf(rec), {cos(ph), sin(ph), 0.0});
+    }
+
+    // Bound-to-free of the extended frames at the departure surface:
+    // dpos/dloc (A, 3x2 row-major) and, for lines, dpos/dangle (Pa):
+    // jacobian_line.hpp:102-172, jacobian_cylindrical.hpp, jacobian_polar.hpp:44-72
+    DTR_AI void ext_bound_to_free(uint32_t dep, S A[6], S Pa[3][2]) const {
+        const NavSurface& rec = det.sf[dep];
+        const dtr_transform t = rec_trf(rec);
+        V3 c0, c1;
+        if (is_line(rec.shape)) {
+            const V3 ddir = Engine::bound_dir(tk.bound);
+            const V3 dp = bound_pos(dep, tk.bound);
+            V3 nx, ny, nz;
+            Engine::line_frame(t, ddir, nx, ny, nz);
+            c0 = nx;
+            c1 = ny;
+            double l0, l1;
+            Engine::line_local(t, dp, ddir, l0, l1);
+            const double ct = cos(tk.bound[3]), stt = sin(tk.bound[3]);
+            const double cp = cos(tk.bound[2]), sp = sin(tk.bound[2]);
+            const V3 yp = cross(ny, {-stt * sp, stt * cp, 0.0});
+            const V3 yt = cross(ny, {ct * cp, ct * sp, -stt});
+            const double C = 1.0 / dot(ddir, nz) * l0;
+            const V3 dphi = mul(C, sub(yp, mul(dot(nx, yp), nx)));
+            const V3 dtheta = mul(C, sub(yt, mul(dot(nx, yt), nx)));
+            Pa[0][0] = static_cast<S>(dphi.x); Pa[1][0] = static_cast<S>(dphi.y); Pa[2][0] = static_cast<S>(dphi.z);
+            Pa[0][1] = static_cast<S>(dtheta.x); Pa[1][1] = static_cast<S>(dtheta.y); Pa[2][1] = static_cast<S>(dtheta.z);
+        } else if (rec.shape == DTR_CYLINDER2D) {
+            V3 nz;
+            Engine::cyl2_frame(t, bound_pos(dep, tk.bound), c0, c1, nz);
+        } else {
+            const V3 l = rec_to_local(rec, bound_pos(dep, tk.bound));
+            const double lr = perp(l), lp = phi_of(l), c = cos(lp), sn = sin(lp);
+            const V3 x = rec_axis(rec, 0), y = rec_axis(rec, 1);
+            c0 = add(mul(c, x), mul(sn, y));
+            c1 = mul(lr, sub(mul(c, y), mul(sn, x)));
+        }
+        A[0] = static_cast<S>(c0.x); A[1] = static_cast<S>(c1.x);
+        A[2] = static_cast<S>(c0.y); A[3] = static_cast<S>(c1.y);
+        A[4] = static_cast<S>(c0.z); A[5] = static_cast<S>(c1.z);
+    }
+
+    // Free-to-bound of the extended frames at the destination: rows 0-1 of
+    // the position block (Bl), and the path correction.  Returns f2p itself
+    // for lines (with its direction part in f2d, jacobian_line.hpp:64-92 and
+    // 174-182), the surface normal otherwise (cylindrical and polar frames,
+    // jacobian_cylindrical.hpp, jacobian_polar.hpp:81-110).
+    DTR_AI V3 ext_free_to_bound(uint32_t dest, T3<S> k, S Bl[6], T3<S>& f2d) const {
+        const NavSurface& rec = det.sf[dest];
+        const dtr_transform t = rec_trf(rec);
+        V3 r0, r1, out;
+        if (is_line(rec.shape)) {
+            const V3 kd = tdbl(k);
+            const V3 pc = sub(tk.pos, {rec.t[0], rec.t[1], rec.t[2]});
+            const V3 lz = rec_axis(rec, 2);
+            const double pz = dot(pc, lz);
+            const double dz = dot(lz, tk.dir);
+            const V3 pc_x = sub(pc, mul(pz, lz));
+            const double nrm = -1.0 / (1.0 - dz * dz + dot(pc_x, kd));
+            out = mul(nrm, sub(tk.dir, mul(dz, lz)));
+            f2d = tcast<S>(mul(nrm, pc_x));
+            V3 nz;
+            Engine::line_frame(t, tk.dir, r0, r1, nz);
+        } else if (rec.shape == DTR_CYLINDER2D) {
+            Engine::cyl2_frame(t, tk.pos, r0, r1, out);
+        } else {
+            const V3 l = rec_to_local(rec, tk.pos);
+            const double lr = perp(l), lp = phi_of(l), c = cos(lp), sn = sin(lp);
+            const V3 x = rec_axis(rec, 0), y = rec_axis(rec, 1);
+            r0 = add(mul(c, x), mul(sn, y));
+            r1 = mul(1.0 / lr, sub(mul(c, y), mul(sn, x)));
+            out = rec_axis(rec, 2);
+        }
+        Bl[0] = static_cast<S>(r0.x); Bl[1] = static_cast<S>(r0.y); Bl[2] = static_cast<S>(r0.z);
+        Bl[3] = static_cast<S>(r1.x); Bl[4] = static_cast<S>(r1.y); Bl[5] = static_cast<S>(r1.z);
+        return out;
+    }
+
     // parameter_updater.hpp:422-572 + covariance_transport.hpp in S
     DTR_AI void transport_covariance(uint32_t dest) {
         const uint32_t dep = tk.up_sf;
         const NavSurface& drec = det.sf[dep];
         S A[6];
+        S Pa[3][2] = {{S(0), S(0)}, {S(0), S(0)}, {S(0), S(0)}};  // dpos/dangle of a line
         if (drec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             const V3 dp = bound_pos(dep, tk.bound);
             const double r = perp(dp), ph = phi_of(dp);
             A[0] = static_cast<S>(-r * sin(ph)); A[1] = S(0);
             A[2] = static_cast<S>(r * cos(ph));  A[3] = S(0);
             A[4] = S(0);                         A[5] = S(1);
+        } else if (P::kExt && (drec.shape >= DTR_LINE_SQUARE || drec.shape == DTR_RING2D)) {
+            ext_bound_to_free(dep, A, Pa);
         } else {
             A[0] = static_cast<S>(drec.r[0]); A[1] = static_cast<S>(drec.r[1]);
             A[2] = static_cast<S>(drec.r[3]); A[3] = static_cast<S>(drec.r[4]);
@@ -1268,6 +1466,10 @@ struct FastEngine {
             }
             Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
             Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
+            if constexpr (P::kExt) {
+                Mp[r][2] += Pa[r][0];
+                Mp[r][3] += Pa[r][1];
+            }
             Mp[r][4] = tk.J[jpos(r, 7)];
             if constexpr (P::kGrad) {
                 Md[r][0] = tk.Jx.v[9 + r * 3 + 0] * A[0] + tk.Jx.v[9 + r * 3 + 1] * A[2] + tk.Jx.v[9 + r * 3 + 2] * A[4];
@@ -1283,8 +1485,13 @@ struct FastEngine {
         const S Mq4 = tk.J[kJqq];
         const T3<S> k = dtds();
         const NavSurface& erec = det.sf[dest];
+        const bool ext_dest = P::kExt && (erec.shape >= DTR_LINE_SQUARE || erec.shape == DTR_RING2D);
+        S Bl[6];
+        T3<S> f2d{S(0), S(0), S(0)};
         V3 n;
-        if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (ext_dest) {
+            n = ext_free_to_bound(dest, k, Bl, f2d);
+        } else if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             if constexpr (P::kMixed) {
                 // (cos, sin) of phi of the position, without trigonometry
                 const double rinv = 1.0 / perp(tk.pos);
@@ -1296,20 +1503,22 @@ struct FastEngine {
         } else {
             n = rec_axis(erec, 2);
         }
-        const T3<S> f2p = tcast<S>(mul(-1.0 / dot(n, tk.dir), n));
+        const T3<S> f2p = (P::kExt && is_line(erec.shape)) ? tcast<S>(n) : tcast<S>(mul(-1.0 / dot(n, tk.dir), n));
         const S p2f_pos[3] = {static_cast<S>(tk.dir.x), static_cast<S>(tk.dir.y), static_cast<S>(tk.dir.z)};
         const S p2f_dir[3] = {k.x, k.y, k.z};
 #pragma unroll
         for (int cc = 0; cc < 5; ++cc) {
-            const S proj = f2p.x * Mp[0][cc] + f2p.y * Mp[1][cc] + f2p.z * Mp[2][cc];
+            S proj = f2p.x * Mp[0][cc] + f2p.y * Mp[1][cc] + f2p.z * Mp[2][cc];
+            if constexpr (P::kExt) proj += f2d.x * Md[0][cc] + f2d.y * Md[1][cc] + f2d.z * Md[2][cc];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 Mp[r][cc] += p2f_pos[r] * proj;
                 Md[r][cc] += p2f_dir[r] * proj;
             }
         }
-        S Bl[6];
-        if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        if (ext_dest) {
+            // rows set by ext_free_to_bound
+        } else if (erec.shape == DTR_CONCENTRIC_CYLINDER2D) {
             if constexpr (P::kMixed) {
                 const double rinv2 = 1.0 / (tk.pos.x * tk.pos.x + tk.pos.y * tk.pos.y);
                 Bl[0] = static_cast<S>(-tk.pos.y * rinv2); Bl[1] = static_cast<S>(tk.pos.x * rinv2); Bl[2] = S(0);
@@ -1412,6 +1621,8 @@ struct FastEngine {
             if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
                 const double rinv = 1.0 / perp(tk.pos);
                 nrm = {tk.pos.x * rinv, tk.pos.y * rinv, 0.0};
+            } else if (P::kExt && rec.shape == DTR_CYLINDER2D) {
+                nrm = cyl2_normal(rec);
             } else {
                 nrm = rec_axis(rec, 2);
             }
@@ -1419,13 +1630,25 @@ struct FastEngine {
         } else {
             V3 nrm;
             if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) nrm = {cos(tk.bound[0]), sin(tk.bound[0]), 0.0};
+            else if (P::kExt && rec.shape == DTR_CYLINDER2D) nrm = cyl2_normal(rec);
             else nrm = rec_axis(rec, 2);
             const V3 bd = Engine::bound_dir(tk.bound);
             cos_inc = fabs(dot(bd, nrm));
         }
         if (!(m.thickness > 2.220446049250313e-16) || tk.bound[4] == 0.0) return;
         const double qop = tk.bound[4];
-        const double path_seg = m.thickness / cos_inc;
+        double path_seg = m.thickness / cos_inc;
+        double path_x0 = m.thickness_in_X0 / cos_inc;
+        if constexpr (P::kExt) {
+            if (m.kind == DTR_MATERIAL_ROD) {
+                // material_rod.hpp:62-78, approach = bound loc0
+                const double a = tk.bound[0];
+                path_seg = 0.0;
+                if (!(fabs(a) > m.thickness))
+                    path_seg = 2.0 * sqrt((m.thickness * m.thickness - a * a) / (1.0 - cos_inc * cos_inc));
+                path_x0 = path_seg / m.X0;
+            }
+        }
         const double mass = kMuonMass, q = track_charge(tk.qop);
         const double q2b2 = q * q + (mass * qop) * (mass * qop);
         const double mOverP = mass * fabs(qop / q);
@@ -1464,7 +1687,7 @@ struct FastEngine {
             }
         }
         if (cfg.do_ms) {
-            const S x0 = static_cast<S>(m.thickness_in_X0 / cos_inc);
+            const S x0 = static_cast<S>(path_x0);
             const S pinv = static_cast<S>(fabs(qop / q));
             if (x0 > S(0)) {
                 const S t = sqrt(x0 * static_cast<S>(q2b2));
@@ -1494,7 +1717,10 @@ struct FastEngine {
         // onto the surface.  The mixed policy does the same projection
         // without trigonometry: onto the plane (planar masks), onto the
         // radius (cylinders), and renormalises the direction.
-        if constexpr (P::kMixed) {
+        if (P::kExt && (is_line(det.sf[dest].shape) || det.sf[dest].shape == DTR_CYLINDER2D)) {
+            tk.pos = bound_pos(dest, tk.bound);
+            tk.dir = Engine::bound_dir(tk.bound);
+        } else if constexpr (P::kMixed) {
             const NavSurface& r = det.sf[dest];
             if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
                 const double k = r.m[0] / perp(tk.pos);

@@ -218,6 +218,61 @@ DTR_HD int covi(int r, int c) {
     return r * 6 - (r * (r - 1)) / 2 + (c - r);
 }
 
+// Mask checks of the extended shapes on the local point `loc` (the frame of
+// the surface's placement): inside with base_tol, edge with base_tol +
+// ext_tol when ext_tol > 0.
+DTR_HD void ext_mask_check(uint8_t shape, const double* v, V3 loc, double base_tol, double ext_tol,
+                           bool& inside, bool& edge) {
+    edge = false;
+    if (shape == DTR_LINE_SQUARE || shape == DTR_LINE_CIRCULAR) {
+        // line.hpp:86-121: square or circular cross section, half length
+        const double c = fabs(loc.z);
+        double a, b2;
+        if (shape == DTR_LINE_SQUARE) {
+            a = fabs(loc.x);
+            b2 = fabs(loc.y);
+            inside = (a <= (v[0] + base_tol) && b2 <= (v[0] + base_tol) && c <= (v[1] + base_tol));
+        } else {
+            a = fabs(perp(loc));
+            b2 = 0.0;
+            inside = (a <= (v[0] + base_tol) && c <= (v[1] + base_tol));
+        }
+        if (ext_tol > 0.0) {
+            const double ft = base_tol + ext_tol;
+            edge = (a <= (v[0] + ft) && b2 <= (v[0] + ft) && c <= (v[1] + ft));
+        }
+    } else if (shape == DTR_ANNULUS2D) {
+        // annulus2D.hpp:140-190: beam-frame radius, strip-frame phi
+        const double nx = loc.x + v[5], ny = loc.y + v[6];
+        const double r_beam = sqrt(fma(nx, nx, ny * ny));
+        double phi_focal = 0.0;
+        bool have_phi = false;
+        inside = ((r_beam + base_tol) >= v[0]) && (r_beam <= (v[1] + base_tol));
+        if (inside) {
+            phi_focal = phi_of(loc) - v[4];
+            have_phi = true;
+            const double pt = r_beam > 0.0 ? base_tol / r_beam : base_tol;
+            inside = (phi_focal >= (v[2] - pt)) && (phi_focal <= (v[3] + pt));
+        }
+        if (ext_tol > 0.0) {
+            const double ft = base_tol + ext_tol;
+            edge = ((r_beam + ft) >= v[0]) && (r_beam <= (v[1] + ft));
+            if (edge) {
+                if (!have_phi) phi_focal = phi_of(loc) - v[4];
+                const double pt = r_beam > 0.0 ? ft / r_beam : ft;
+                edge = (phi_focal >= (v[2] - pt)) && (phi_focal <= (v[3] + pt));
+            }
+        }
+    } else {
+        // cylinder2D.hpp:63-90: local z within the bounds
+        inside = ((v[1] - base_tol) <= loc.z && loc.z <= (v[2] + base_tol));
+        if (ext_tol > 0.0) {
+            const double ft = base_tol + ext_tol;
+            edge = ((v[1] - ft) <= loc.z && loc.z <= (v[2] + ft));
+        }
+    }
+}
+
 // kExt: the engine also handles wires, annuli, placed cylinders, rods,
 // multi-surface bins and rings with material (the extended geometry); the
 // instantiation without them is the plain FP64 kernel of detectors made of
@@ -274,55 +329,8 @@ struct EngineT {
                     const double ft = base_tol + ext_tol;
                     edge = (fabs(loc.x) <= (bound_x + ft) && fabs(loc.y) <= (m.v[2] + ft));
                 }
-            } else if (kExt && (m.shape == DTR_LINE_SQUARE || m.shape == DTR_LINE_CIRCULAR)) {
-                // line.hpp:86-121: square or circular cross section, half length
-                const double c = fabs(loc.z);
-                double a, b2;
-                if (m.shape == DTR_LINE_SQUARE) {
-                    a = fabs(loc.x);
-                    b2 = fabs(loc.y);
-                    inside = (a <= (m.v[0] + base_tol) && b2 <= (m.v[0] + base_tol) && c <= (m.v[1] + base_tol));
-                } else {
-                    a = fabs(perp(loc));
-                    b2 = 0.0;
-                    inside = (a <= (m.v[0] + base_tol) && c <= (m.v[1] + base_tol));
-                }
-                edge = false;
-                if (ext_tol > 0.0) {
-                    const double ft = base_tol + ext_tol;
-                    edge = (a <= (m.v[0] + ft) && b2 <= (m.v[0] + ft) && c <= (m.v[1] + ft));
-                }
-            } else if (kExt && m.shape == DTR_ANNULUS2D) {
-                // annulus2D.hpp:140-190: beam-frame radius, strip-frame phi
-                const double nx = loc.x + m.v[5], ny = loc.y + m.v[6];
-                const double r_beam = sqrt(fma(nx, nx, ny * ny));
-                double phi_focal = 0.0;
-                bool have_phi = false;
-                inside = ((r_beam + base_tol) >= m.v[0]) && (r_beam <= (m.v[1] + base_tol));
-                if (inside) {
-                    phi_focal = phi_of(loc) - m.v[4];
-                    have_phi = true;
-                    const double pt = r_beam > 0.0 ? base_tol / r_beam : base_tol;
-                    inside = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
-                }
-                edge = false;
-                if (ext_tol > 0.0) {
-                    const double ft = base_tol + ext_tol;
-                    edge = ((r_beam + ft) >= m.v[0]) && (r_beam <= (m.v[1] + ft));
-                    if (edge) {
-                        if (!have_phi) phi_focal = phi_of(loc) - m.v[4];
-                        const double pt = r_beam > 0.0 ? ft / r_beam : ft;
-                        edge = (phi_focal >= (m.v[2] - pt)) && (phi_focal <= (m.v[3] + pt));
-                    }
-                }
-            } else if (kExt && m.shape == DTR_CYLINDER2D) {
-                // cylinder2D.hpp:63-90: local z within the bounds
-                inside = ((m.v[1] - base_tol) <= loc.z && loc.z <= (m.v[2] + base_tol));
-                edge = false;
-                if (ext_tol > 0.0) {
-                    const double ft = base_tol + ext_tol;
-                    edge = ((m.v[1] - ft) <= loc.z && loc.z <= (m.v[2] + ft));
-                }
+            } else if (kExt && m.shape >= DTR_LINE_SQUARE) {
+                ext_mask_check(m.shape, m.v, loc, base_tol, ext_tol, inside, edge);
             } else {
                 const double r = perp(loc);
                 inside = ((r + base_tol) >= m.v[0] && r <= (m.v[1] + base_tol));

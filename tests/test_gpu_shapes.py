@@ -1,12 +1,16 @@
 """Annulus and placed-cylinder surfaces on the device: the plain FP64 kernel
 against the oracle at the FP64 contract (telescopes of annuli and of
 cylinders across the beam, with material, constant field with and without
-noise estimation); the batched launches reject such detectors."""
+noise estimation); the batched kernels (grid schedule, per-lane navigation)
+at the FP64 and mixed-precision contracts; the pooled navigation rejects
+such detectors."""
 
 import pytest
 
 import detray_amd as da
 from detray_amd import capi
+from tests import oracle_host
+from tests.test_gpu_batched import check, run_launch
 from tests.test_gpu_parity import assert_parity, run_both
 from tests.test_shapes import _modified_telescope
 
@@ -26,9 +30,22 @@ def test_plain_kernel_matches_oracle(kind, n, noise):
     assert_parity(got, ref)
 
 
-def test_batched_launch_rejects_new_shapes():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind,n", [("annulus", 6), ("cylinder", 3)])
+def test_batched_kernel_matches_oracle(kind, n, precision):
+    det = _modified_telescope(kind, n=n)
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(estimate_scattering_noise=True)
+    fp = da.generate_tracks("beam", 4000, 42, sigma_theta=0.1)
+    ref = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
+    got = run_launch(det, field, cfg, fp, da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane"))
+    assert ref["n_surfaces"].mean() > 3
+    check(got, ref, precision)
+
+
+def test_pooled_navigation_rejects_new_shapes():
     det = _modified_telescope("annulus")
     tracks = torch.from_numpy(da.generate_tracks("beam", 64, 42)).cuda()
     with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
-        da.Propagator(da.PropagationConfig(), da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")).propagate(
+        da.Propagator(da.PropagationConfig(), da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "wave")).propagate(
             det, da.MagneticField.constant(0, 0, 2 * da.TESLA), tracks)

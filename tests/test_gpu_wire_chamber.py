@@ -1,7 +1,9 @@
 """Wire chamber on the device (drift cells and straw tubes, tungsten rods,
-grid bins of up to 3 wires): the plain FP64 kernel against the oracle at
-the FP64 contract; the batched launches, the direct navigator and the
-adjoint return DTR_ERROR_UNSUPPORTED for such a detector."""
+grid bins of up to 3 wires): the plain FP64 kernel and the batched kernels
+(grid schedule, per-lane navigation) against the oracle, at the FP64
+contract for FP64 and at the mixed-precision contract of the batched tests
+for mixed precision; the pooled (wave) navigation, the persistent schedule
+and use_field_gradient return DTR_ERROR_UNSUPPORTED for such a detector."""
 
 import numpy as np
 import pytest
@@ -9,6 +11,7 @@ import pytest
 import detray_amd as da
 from detray_amd import capi
 from tests import oracle_host
+from tests.test_gpu_batched import check, run_launch
 from tests.test_gpu_parity import assert_parity, run_both
 
 torch = pytest.importorskip("torch")
@@ -51,10 +54,35 @@ def test_json_read_chamber_propagates_identically(tmp_path, chamber, solenoid):
         assert torch.equal(getattr(a, k), getattr(b, k)), k
 
 
-def test_batched_paths_reject_wires(chamber, solenoid):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("field_kind", ["const", "solenoid"])
+def test_batched_kernel_matches_oracle(chamber, solenoid, precision, field_kind):
+    field = da.MagneticField.constant(0, 0, 2 * da.TESLA) if field_kind == "const" else solenoid
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=True)
+    fp = da.generate_tracks("toy", 4000, 48, eta_min=-1.0, eta_max=1.0)
+    ref = oracle_host.propagate(chamber, field, cfg, fp, threads=oracle_host.default_threads())
+    order = torch.from_numpy(np.random.default_rng(3).permutation(fp.shape[1]).astype(np.int32)).cuda()
+    for launch in (da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane"),
+                   da.LaunchConfig(precision, "grid", 64, 1, 64, order, 1, "lane")):
+        got = run_launch(chamber, field, cfg, fp, launch)
+        assert ref["n_surfaces"].mean() > 5
+        check(got, ref, precision)
+
+
+def test_batched_straw_tubes(solenoid):
+    det = da.Detector.wire_chamber(n_layers=4, half_z=500.0, square_cells=False)
+    cfg = da.PropagationConfig(search_window=(3, 3))
+    fp = da.generate_tracks("toy", 2000, 45, eta_min=-1.0, eta_max=1.0)
+    ref = oracle_host.propagate(det, solenoid, cfg, fp, threads=oracle_host.default_threads())
+    for precision in ("fp64", "mixed"):
+        check(run_launch(det, solenoid, cfg, fp, da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane")),
+              ref, precision)
+
+
+def test_pooled_and_persistent_paths_reject_wires(chamber, solenoid):
     cfg = da.PropagationConfig(search_window=(3, 3))
     tracks = torch.from_numpy(da.generate_tracks("toy", 64, 47)).cuda()
-    for launch in (da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane"),
+    for launch in (da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "wave"),
                    da.LaunchConfig("fp64", "persistent", 256, 1, 64, None, 1, "wave")):
         with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
             da.Propagator(cfg, launch).propagate(chamber, solenoid, tracks)
