@@ -768,7 +768,6 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     const int rc = check_propagate_args(det, field, c, in, out);
     if (rc != DTR_SUCCESS) return rc;
     if (!seq || !launch) return DTR_ERROR_INVALID_ARGUMENT;
-    if (det->wires) return DTR_ERROR_UNSUPPORTED;
     if (in->n == 0u) return DTR_SUCCESS;
     if (seq->max_surfaces == 0u || !seq->surfaces || !seq->n_surfaces) return DTR_ERROR_INVALID_ARGUMENT;
     dtr_launch_config L = *launch;
@@ -788,7 +787,13 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
     const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    if (L.precision == DTR_PRECISION_FP64)
+    if (det->wires && L.precision == DTR_PRECISION_FP64)
+        hipLaunchKernelGGL((propagate_direct_kernel<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>), dim3(grid),
+                           dim3(block), shmem, st, det->fast, field->dev, dc, dd, ti, r, sq, L.order);
+    else if (det->wires)
+        hipLaunchKernelGGL((propagate_direct_kernel<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>), dim3(grid),
+                           dim3(block), shmem, st, det->fast, field->dev, dc, dd, ti, r, sq, L.order);
+    else if (L.precision == DTR_PRECISION_FP64)
         hipLaunchKernelGGL((propagate_direct_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), shmem, st,
                            det->fast, field->dev, dc, dd, ti, r, sq, L.order);
     else

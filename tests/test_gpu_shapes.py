@@ -2,8 +2,8 @@
 against the oracle at the FP64 contract (telescopes of annuli and of
 cylinders across the beam, with material, constant field with and without
 noise estimation); the batched kernels (grid schedule, per-lane navigation)
-at the FP64 and mixed-precision contracts; the pooled navigation rejects
-such detectors."""
+at the FP64 and mixed-precision contracts, and the direct navigator against
+the oracle's; the pooled navigation rejects such detectors."""
 
 import pytest
 
@@ -11,6 +11,7 @@ import detray_amd as da
 from detray_amd import capi
 from tests import oracle_host
 from tests.test_gpu_batched import check, run_launch
+from tests.test_gpu_direct import run_direct
 from tests.test_gpu_parity import assert_parity, run_both
 from tests.test_shapes import _modified_telescope
 
@@ -40,6 +41,20 @@ def test_batched_kernel_matches_oracle(kind, n, precision):
     ref = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
     got = run_launch(det, field, cfg, fp, da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane"))
     assert ref["n_surfaces"].mean() > 3
+    check(got, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind,n", [("annulus", 6), ("cylinder", 3)])
+def test_direct_navigator_matches_oracle(kind, n, precision):
+    det = _modified_telescope(kind, n=n)
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(estimate_scattering_noise=False)
+    fp = da.generate_tracks("beam", 4000, 43, sigma_theta=0.1)
+    rec = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
+    ref = oracle_host.propagate_direct(det, field, cfg, fp, rec["surfaces"], rec["n_surfaces"],
+                                       threads=oracle_host.default_threads())
+    got = run_direct(det, field, cfg, fp, rec["surfaces"], rec["n_surfaces"], precision)
     check(got, ref, precision)
 
 

@@ -2,8 +2,10 @@
 grid bins of up to 3 wires): the plain FP64 kernel and the batched kernels
 (grid schedule, per-lane navigation) against the oracle, at the FP64
 contract for FP64 and at the mixed-precision contract of the batched tests
-for mixed precision; the pooled (wave) navigation, the persistent schedule
-and use_field_gradient return DTR_ERROR_UNSUPPORTED for such a detector."""
+for mixed precision; the direct navigator replaying the oracle's sequences
+against the oracle's direct navigator; the pooled (wave) navigation, the
+persistent schedule and use_field_gradient return DTR_ERROR_UNSUPPORTED for
+such a detector."""
 
 import numpy as np
 import pytest
@@ -12,6 +14,7 @@ import detray_amd as da
 from detray_amd import capi
 from tests import oracle_host
 from tests.test_gpu_batched import check, run_launch
+from tests.test_gpu_direct import run_direct
 from tests.test_gpu_parity import assert_parity, run_both
 
 torch = pytest.importorskip("torch")
@@ -77,6 +80,18 @@ def test_batched_straw_tubes(solenoid):
     for precision in ("fp64", "mixed"):
         check(run_launch(det, solenoid, cfg, fp, da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane")),
               ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_direct_navigator_matches_oracle(chamber, solenoid, precision):
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+    fp = da.generate_tracks("toy", 4000, 49, eta_min=-1.0, eta_max=1.0)
+    rec = oracle_host.propagate(chamber, solenoid, cfg, fp, threads=oracle_host.default_threads())
+    ref = oracle_host.propagate_direct(chamber, solenoid, cfg, fp, rec["surfaces"], rec["n_surfaces"],
+                                       threads=oracle_host.default_threads())
+    assert np.array_equal(ref["surfaces"], rec["surfaces"]) and (ref["status"] == 0).all()
+    got = run_direct(chamber, solenoid, cfg, fp, rec["surfaces"], rec["n_surfaces"], precision)
+    check(got, ref, precision)
 
 
 def test_pooled_and_persistent_paths_reject_wires(chamber, solenoid):
