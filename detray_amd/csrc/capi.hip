@@ -743,13 +743,11 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
                       out->n_trials, out->status};
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (det->wires) {
-        if (c->use_field_gradient || L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE)
-            return DTR_ERROR_UNSUPPORTED;
+        // the extended geometry is compiled for two waves per SIMD
+        if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
         if (L.precision == DTR_PRECISION_FP64)
-            return launch_lane<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size,
-                                                                           L.order, st);
-        return launch_lane<dtr::WithExtendedGeometry<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order,
-                                                                        st);
+            return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>(det, field, dc, in, r, L, st);
+        return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>(det, field, dc, in, r, L, st);
     }
     if (c->use_field_gradient) {
         if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;

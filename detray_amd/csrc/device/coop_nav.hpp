@@ -64,7 +64,7 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
     FastEngine<P, NavLite> me{det, no_field, cfg, nl, c};
 
     // brute-force surfaces, each lane on its own; then the grid window
-    uint32_t ng = 0u;
+    uint32_t ng = 0u, cap = 1u;
     int a0 = 0, a1 = 0, width = 1, grid = -1;
     float margin = 0.f, ft = 0.f;
     if (req) {
@@ -75,7 +75,9 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
             int e0, e1;
             if (me.grid_window(vol, det.grids[vol.grid], a0, e0, a1, e1)) {
                 width = e1 - a1;
-                ng = static_cast<uint32_t>((e0 - a0) * width);
+                // items are the entries of the window's bins (capacity 0 means 1)
+                if constexpr (P::kExt) cap = det.grids[vol.grid].capacity ? det.grids[vol.grid].capacity : 1u;
+                ng = static_cast<uint32_t>((e0 - a0) * width) * cap;
                 grid = vol.grid;
                 margin = me.sphere_margin(vol.grid);
                 ft = static_cast<float>(me.tol_max());
@@ -110,6 +112,7 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
             const uint32_t oexcl = W::shfl(excl, o);
             const int oa0 = W::shfl(a0, o), oa1 = W::shfl(a1, o), owidth = W::shfl(width, o);
             const int ogrid = W::shfl(grid, o);
+            const uint32_t ocap = P::kExt ? W::shfl(cap, o) : 1u;
             const float omargin = W::shfl(margin, o), oft = W::shfl(ft, o);
             const float ox = W::shfl(fpx, o), oy = W::shfl(fpy, o), oz = W::shfl(fpz, o);
             const float dx = W::shfl(fdx, o), dy = W::shfl(fdy, o), dz = W::shfl(fdz, o);
@@ -117,13 +120,14 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
             uint32_t bin = 0u;
             if (item < total) {
                 const dtr_grid& g = det.grids[ogrid];
-                const int jj = static_cast<int>(item - oexcl);
+                const int jj = static_cast<int>((item - oexcl) / ocap);
+                const uint32_t entry = (item - oexcl) % ocap;
                 const int b0 = oa0 + jj / owidth, b1 = oa1 + jj % owidth;
                 const bool cyl = g.type == DTR_GRID_CYLINDER;
                 const int n0 = static_cast<int>(g.n0), n1 = static_cast<int>(g.n1);
                 const int i0 = cyl ? FastEngine<P, NavLite>::wrap_bin(b0, n0) : b0;
                 const int i1 = cyl ? b1 : FastEngine<P, NavLite>::wrap_bin(b1, n1);
-                bin = g.bins + static_cast<uint32_t>(i0) * g.n1 + static_cast<uint32_t>(i1);
+                bin = g.bins + (static_cast<uint32_t>(i0) * g.n1 + static_cast<uint32_t>(i1)) * ocap + entry;
                 pass = FastEngine<P, NavLite>::prefilter(det.bin_rec[bin].box, ox, oy, oz, dx, dy, dz, omargin,
                                                          oft);
             }
@@ -159,7 +163,10 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
             cd.link = 0;
             cd.status = kOutside;
             cd.pad = 0;
-            if (valid) {
+            // a placed cylinder can insert two candidates: its owner searches
+            // it at its place in the order (flag bit 20, the entry in sf)
+            const bool own = P::kExt && valid && det.bin_rec[e & 0x3ffffffu].rec.shape == DTR_CYLINDER2D;
+            if (valid && !own) {
                 FastEngine<P, NavLite> we{det, no_field, cfg, wl, c};
                 acc = we.evaluate(det.bin_rec[e & 0x3ffffffu].rec, oos, cd);
             }
@@ -168,6 +175,11 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
                 sh->slot[lane].sf = cd.sf;
                 sh->slot[lane].ls = static_cast<uint32_t>(cd.link) | (static_cast<uint32_t>(cd.status) << 16) |
                                     (static_cast<uint32_t>(cd.pad) << 18) | (static_cast<uint32_t>(o) << 24);
+            } else if (own) {
+                acc = true;
+                sh->slot[lane].path = kMax;
+                sh->slot[lane].sf = e & 0x3ffffffu;
+                sh->slot[lane].ls = (1u << 20) | (static_cast<uint32_t>(o) << 24);
             }
             W::sync();
             uint64_t m = W::ballot(acc);
@@ -177,6 +189,10 @@ __host__ __device__ __attribute__((noinline)) uint32_t coop_local_navigation(
                     m &= m - 1ull;
                     const uint32_t ls = sh->slot[j].ls;
                     if ((ls >> 24) != lane) continue;
+                    if (P::kExt && ((ls >> 20) & 1u)) {
+                        me.candidate_search(det.bin_rec[sh->slot[j].sf].rec, os);
+                        continue;
+                    }
                     Cand x;
                     x.path = sh->slot[j].path;
                     x.sf = sh->slot[j].sf;

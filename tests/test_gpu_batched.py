@@ -13,7 +13,11 @@ the reference's own host/device convention (propagator_test.hpp:70,
   |d loc| <= 1e-4 * max(path, 1 mm), |d angle| <= 1e-4 * max(|angle|, 1),
   |d q/p| <= 1e-4 * |q/p|,
 and covariance within 1e-4 of sqrt(C_ii C_jj) (2e-2 for tracks brought to
-rest in material).  (FP32 stage rounding can
+rest in material; 1e-3 for tracks ending within 0.02 rad of the z axis, where
+the (phi, theta) parameters are singular: the phi variance grows as
+1/sin^2(theta) and its FP32 rounding reaches the other elements; such tracks
+only occur in the telescopes along z, at sin(theta) >= 0.02 the error there
+stays below 1e-5).  (FP32 stage rounding can
 flip an accept/reject decision of the adaptive step control, so a track may
 take one step more or less than in FP64 and land a truncation error,
 ~1e-4 mm, away: loc differences are therefore not held to 1e-4 mm.)
@@ -41,6 +45,9 @@ MIXED_COV_TOL = 1e-4
 # scattering terms of the surface where p -> 0, which change by percents
 # under last-bit perturbations of the trajectory
 MIXED_COV_TOL_STOPPED = 2e-2
+# tracks ending within 0.02 rad of the z axis (see the module docstring)
+MIXED_COV_TOL_AXIAL = 1e-3
+AXIAL_SIN_THETA = 0.02
 
 
 def bound_error_scale(ref):
@@ -96,7 +103,9 @@ def assert_mixed_parity(got, ref):
     d = np.sqrt(np.abs(np.einsum("iin->in", cr)))
     norm = np.einsum("in,jn->ijn", d, d) + 1e-300
     stopped = ref["status"] == 1
-    for sel, tol in ((~stopped, MIXED_COV_TOL), (stopped, MIXED_COV_TOL_STOPPED)):
+    axial = ~stopped & (np.abs(np.sin(ref["bound_params"][3])) < AXIAL_SIN_THETA)
+    for sel, tol in ((~stopped & ~axial, MIXED_COV_TOL), (stopped, MIXED_COV_TOL_STOPPED),
+                     (axial, MIXED_COV_TOL_AXIAL)):
         has = (ref["n_surfaces"] > 0) & sel
         rel = np.abs(cg - cr)[:, :, has] / norm[:, :, has]
         assert rel.size == 0 or np.max(rel) <= tol, (np.max(rel), tol)

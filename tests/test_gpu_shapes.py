@@ -2,13 +2,13 @@
 against the oracle at the FP64 contract (telescopes of annuli and of
 cylinders across the beam, with material, constant field with and without
 noise estimation); the batched kernels (grid schedule, per-lane navigation)
-at the FP64 and mixed-precision contracts, and the direct navigator against
-the oracle's; the pooled navigation rejects such detectors."""
+at the FP64 and mixed-precision contracts, with per-lane and pooled (wave)
+navigation and the persistent schedule, and the direct navigator against the
+oracle's."""
 
 import pytest
 
 import detray_amd as da
-from detray_amd import capi
 from tests import oracle_host
 from tests.test_gpu_batched import check, run_launch
 from tests.test_gpu_direct import run_direct
@@ -58,9 +58,14 @@ def test_direct_navigator_matches_oracle(kind, n, precision):
     check(got, ref, precision)
 
 
-def test_pooled_navigation_rejects_new_shapes():
-    det = _modified_telescope("annulus")
-    tracks = torch.from_numpy(da.generate_tracks("beam", 64, 42)).cuda()
-    with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
-        da.Propagator(da.PropagationConfig(), da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "wave")).propagate(
-            det, da.MagneticField.constant(0, 0, 2 * da.TESLA), tracks)
+@pytest.mark.parametrize("kind,n", [("annulus", 6), ("cylinder", 3)])
+def test_pooled_navigation_and_persistent_schedule(kind, n):
+    det = _modified_telescope(kind, n=n)
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(estimate_scattering_noise=True)
+    fp = da.generate_tracks("beam", 4000, 44, sigma_theta=0.1)
+    ref = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
+    for precision in ("fp64", "mixed"):
+        for launch in (da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "wave"),
+                       da.LaunchConfig(precision, "persistent", 256, 2, 64, None, 1, "wave")):
+            check(run_launch(det, field, cfg, fp, launch), ref, precision)

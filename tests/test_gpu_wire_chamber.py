@@ -2,10 +2,10 @@
 grid bins of up to 3 wires): the plain FP64 kernel and the batched kernels
 (grid schedule, per-lane navigation) against the oracle, at the FP64
 contract for FP64 and at the mixed-precision contract of the batched tests
-for mixed precision; the direct navigator replaying the oracle's sequences
-against the oracle's direct navigator; the pooled (wave) navigation, the
-persistent schedule and use_field_gradient return DTR_ERROR_UNSUPPORTED for
-such a detector."""
+for mixed precision, also with the pooled (wave) navigation and the
+persistent schedule; the direct navigator replaying the oracle's sequences
+against the oracle's direct navigator; use_field_gradient returns
+DTR_ERROR_UNSUPPORTED for such a detector."""
 
 import numpy as np
 import pytest
@@ -94,13 +94,19 @@ def test_direct_navigator_matches_oracle(chamber, solenoid, precision):
     check(got, ref, precision)
 
 
-def test_pooled_and_persistent_paths_reject_wires(chamber, solenoid):
-    cfg = da.PropagationConfig(search_window=(3, 3))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pooled_navigation_and_persistent_schedule(chamber, solenoid, precision):
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=True)
+    fp = da.generate_tracks("toy", 4000, 50, eta_min=-1.0, eta_max=1.0)
+    ref = oracle_host.propagate(chamber, solenoid, cfg, fp, threads=oracle_host.default_threads())
+    order = torch.from_numpy(np.random.default_rng(5).permutation(fp.shape[1]).astype(np.int32)).cuda()
+    for launch in (da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "wave"),
+                   da.LaunchConfig(precision, "persistent", 256, 2, 64, order, 8, "wave")):
+        check(run_launch(chamber, solenoid, cfg, fp, launch), ref, precision)
+
+
+def test_field_gradient_rejects_wires(chamber, solenoid):
     tracks = torch.from_numpy(da.generate_tracks("toy", 64, 47)).cuda()
-    for launch in (da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "wave"),
-                   da.LaunchConfig("fp64", "persistent", 256, 1, 64, None, 1, "wave")):
-        with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
-            da.Propagator(cfg, launch).propagate(chamber, solenoid, tracks)
     with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
         da.Propagator(da.PropagationConfig(search_window=(3, 3), use_field_gradient=True)).propagate(
             chamber, solenoid, tracks)
