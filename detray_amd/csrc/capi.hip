@@ -644,9 +644,11 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
     if (c->use_field_gradient) {
-        if (det->wires) return DTR_ERROR_UNSUPPORTED;
         // the Jacobian with position columns runs on the batched engine's
         // FP64 policy (same results contract as the plain kernel)
+        if (det->wires)
+            return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
+                det, field, dc, in, r, 64u, nullptr, static_cast<hipStream_t>(stream));
         return launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
                                                                     static_cast<hipStream_t>(stream));
     }
@@ -744,7 +746,14 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (det->wires) {
         // the extended geometry is compiled for two waves per SIMD
-        if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+        if (c->use_field_gradient) {
+            if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
+            if (L.precision == DTR_PRECISION_FP64)
+                return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
+                    det, field, dc, in, r, L.block_size, L.order, st);
+            return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyMixed>>>(
+                det, field, dc, in, r, L.block_size, L.order, st);
+        }
         if (L.precision == DTR_PRECISION_FP64)
             return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>(det, field, dc, in, r, L, st);
         return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>(det, field, dc, in, r, L, st);

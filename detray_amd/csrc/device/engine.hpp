@@ -1473,7 +1473,14 @@ struct FastEngine {
             }
             Mp[r][2] = tk.J[jpos(r, 4)] * Cd[0] + tk.J[jpos(r, 5)] * Cd[2] + tk.J[jpos(r, 6)] * Cd[4];
             Mp[r][3] = tk.J[jpos(r, 4)] * Cd[1] + tk.J[jpos(r, 5)] * Cd[3] + tk.J[jpos(r, 6)] * Cd[5];
-            if constexpr (P::kExt) {
+            if constexpr (P::kExt && P::kGrad) {
+                // the line's dpos/dangle through the position columns
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    Mp[r][2] += tk.Jx.v[r * 3 + m] * Pa[m][0];
+                    Mp[r][3] += tk.Jx.v[r * 3 + m] * Pa[m][1];
+                }
+            } else if constexpr (P::kExt) {
                 Mp[r][2] += Pa[r][0];
                 Mp[r][3] += Pa[r][1];
             }
@@ -1487,6 +1494,13 @@ struct FastEngine {
             }
             Md[r][2] = tk.J[jdir(r, 4)] * Cd[0] + tk.J[jdir(r, 5)] * Cd[2] + tk.J[jdir(r, 6)] * Cd[4];
             Md[r][3] = tk.J[jdir(r, 4)] * Cd[1] + tk.J[jdir(r, 5)] * Cd[3] + tk.J[jdir(r, 6)] * Cd[5];
+            if constexpr (P::kExt && P::kGrad) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    Md[r][2] += tk.Jx.v[9 + r * 3 + m] * Pa[m][0];
+                    Md[r][3] += tk.Jx.v[9 + r * 3 + m] * Pa[m][1];
+                }
+            }
             Md[r][4] = tk.J[jdir(r, 7)];
         }
         const S Mq4 = tk.J[kJqq];

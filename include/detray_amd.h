@@ -356,8 +356,8 @@ DTR_API void dtr_default_launch(dtr_launch_config* launch);
  * otherwise).  A detector with the extended geometry (wire surfaces,
  * annuli, placed cylinders, material rods, rings with material or grid bins
  * of several surfaces) runs on every schedule and navigation mode, compiled
- * for two waves per SIMD, without use_field_gradient (DTR_ERROR_UNSUPPORTED
- * otherwise). */
+ * for two waves per SIMD (use_field_gradient: per-lane navigation on the grid
+ * schedule, as for any detector). */
 DTR_API int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, const dtr_launch_config* launch,

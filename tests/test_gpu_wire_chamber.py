@@ -4,8 +4,9 @@ grid bins of up to 3 wires): the plain FP64 kernel and the batched kernels
 contract for FP64 and at the mixed-precision contract of the batched tests
 for mixed precision, also with the pooled (wave) navigation and the
 persistent schedule; the direct navigator replaying the oracle's sequences
-against the oracle's direct navigator; use_field_gradient returns
-DTR_ERROR_UNSUPPORTED for such a detector."""
+against the oracle's direct navigator; use_field_gradient (line Jacobians
+through the position columns) on the plain path and the per-lane batched
+kernels, DTR_ERROR_UNSUPPORTED with the pooled navigation."""
 
 import numpy as np
 import pytest
@@ -105,8 +106,25 @@ def test_pooled_navigation_and_persistent_schedule(chamber, solenoid, precision)
         check(run_launch(chamber, solenoid, cfg, fp, launch), ref, precision)
 
 
-def test_field_gradient_rejects_wires(chamber, solenoid):
+@pytest.mark.parametrize("precision", [None, "fp64", "mixed"])
+def test_field_gradient_matches_oracle(chamber, solenoid, precision):
+    """use_field_gradient on wires: the line's dpos/dangle block passes
+    through the position columns of the transport Jacobian (None: through
+    dtr_propagate)."""
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=True, use_field_gradient=True)
+    fp = da.generate_tracks("toy", 4000, 51, eta_min=-1.0, eta_max=1.0)
+    ref = oracle_host.propagate(chamber, solenoid, cfg, fp, threads=oracle_host.default_threads())
+    if precision is None:
+        got, _ = run_both(chamber, solenoid, cfg, fp)
+        assert_parity(got, ref)
+    else:
+        got = run_launch(chamber, solenoid, cfg, fp, da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "lane"))
+        check(got, ref, precision)
+
+
+def test_field_gradient_rejects_pooled_navigation(chamber, solenoid):
     tracks = torch.from_numpy(da.generate_tracks("toy", 64, 47)).cuda()
+    cfg = da.PropagationConfig(search_window=(3, 3), use_field_gradient=True)
     with pytest.raises(capi.DetrayError, match="UNSUPPORTED"):
-        da.Propagator(da.PropagationConfig(search_window=(3, 3), use_field_gradient=True)).propagate(
+        da.Propagator(cfg, da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "wave")).propagate(
             chamber, solenoid, tracks)
