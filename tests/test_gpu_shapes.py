@@ -69,3 +69,21 @@ def test_pooled_navigation_and_persistent_schedule(kind, n):
         for launch in (da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "wave"),
                        da.LaunchConfig(precision, "persistent", 256, 2, 64, None, 1, "wave")):
             check(run_launch(det, field, cfg, fp, launch), ref, precision)
+
+
+def test_cylinders_in_a_grid_bin():
+    """Placed cylinders in a multi-surface grid bin: the pooled navigation
+    hands them to their owner lane, which inserts both crossings in order."""
+    from tests.test_shapes import _gridded_cylinder_telescope
+
+    det = _gridded_cylinder_telescope(3)
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(search_window=(0, 0), estimate_scattering_noise=True)
+    fp = da.generate_tracks("beam", 4000, 45, sigma_theta=0.1)
+    ref = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
+    assert ref["n_surfaces"].mean() > 3
+    for precision in ("fp64", "mixed"):
+        for launch in (da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "lane"),
+                       da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "wave"),
+                       da.LaunchConfig(precision, "persistent", 256, 2, 64, None, 1, "wave")):
+            check(run_launch(det, field, cfg, fp, launch), ref, precision)

@@ -124,6 +124,40 @@ def _modified_telescope(kind, n=6, thickness=0.15):
                                    bf_surfaces=base.table("bf_surfaces"), name=f"{kind}_telescope")
 
 
+def _gridded_cylinder_telescope(n=3):
+    """The cylinder telescope with its cylinders moved from the brute-force
+    list into one ring-grid bin of capacity n (a placed cylinder in a grid
+    bin, next to the others)."""
+    base = _modified_telescope("cylinder", n=n)
+    copy = lambda items, t: [t.from_buffer_copy(x) for x in items]
+    volumes = copy(base.table("volumes"), capi.Volume)
+    surfaces = copy(base.table("surfaces"), capi.Surface)
+    cyl = [i for i, sf in enumerate(surfaces) if sf.shape == capi.CYLINDER2D]
+    bf = [i for i in base.table("bf_surfaces") if i not in cyl]
+    volumes[0].bf_begin, volumes[0].bf_end, volumes[0].grid = 0, len(bf), 0
+    g = capi.Grid()
+    g.type, g.capacity, g.n0, g.n1, g.bins = 1, n, 1, 1, 0
+    g.min0, g.max0, g.min1, g.max1 = 0.0, 1e5, -math.pi, math.pi
+    return da.Detector.from_tables(volumes, surfaces, copy(base.table("transforms"), capi.Transform),
+                                   copy(base.table("masks"), capi.Mask),
+                                   copy(base.table("materials"), capi.MaterialSlab), grids=[g], bf_surfaces=bf,
+                                   grid_bins=cyl, name="gridded_cylinder_telescope")
+
+
+def test_cylinders_in_a_grid_bin_navigate_as_in_the_list():
+    """Placed cylinders found through a multi-surface grid bin give the same
+    surface sequences and parameters as through the brute-force list."""
+    field = da.MagneticField.constant(0.0, 0.2 * da.TESLA, 1.0 * da.TESLA)
+    cfg = da.PropagationConfig(search_window=(0, 0))
+    fp = da.generate_tracks("beam", 500, 45, sigma_theta=0.1)
+    a = oracle_host.propagate(_modified_telescope("cylinder", n=3), field, cfg, fp)
+    b = oracle_host.propagate(_gridded_cylinder_telescope(3), field, cfg, fp)
+    assert a["n_surfaces"].mean() > 3
+    np.testing.assert_array_equal(a["surfaces"], b["surfaces"])
+    np.testing.assert_array_equal(a["status"], b["status"])
+    np.testing.assert_allclose(a["bound_params"], b["bound_params"], rtol=1e-12, atol=1e-12)
+
+
 @pytest.fixture(scope="module")
 def annulus_telescope():
     return _modified_telescope("annulus")
