@@ -64,7 +64,9 @@ static_assert(sizeof(NavSurface) == 176, "NavSurface layout");
 // empty grid bin, r = +inf for a surface that is always evaluated).  Box:
 // the plane (normal, local axes) and the half lengths of the mask's
 // bounding rectangle (hx < 0: no box test); kx is the tolerance multiple of
-// the local-x bound (1 for rectangles, 1 + slope for trapezoids).
+// the local-x bound (1 for rectangles, 1 + slope for trapezoids).  Wires
+// (kx < 0): n is the wire axis and hx bounds the distance of an accepted
+// point from it, sqrt(2) times the cross section.
 struct alignas(16) NavBox {
     float c[3];
     float r;
@@ -173,6 +175,12 @@ inline NavSurface make_nav_surface(const dtr_detector_desc& d, uint32_t s) {
         } else if (sf.shape == DTR_RING2D) {
             r.radius = m.v[1];
             r.tolk = 2.f;
+        } else if (sf.shape == DTR_LINE_SQUARE || sf.shape == DTR_LINE_CIRCULAR) {
+            // a wire accepts points within sqrt(2) (r + t) of its axis and
+            // hz + t along it: within hypot(hz, sqrt(2) r) + sqrt(3) t of
+            // its centre
+            r.radius = sqrt(m.v[1] * m.v[1] + 2.0 * m.v[0] * m.v[0]);
+            r.tolk = 2.f;
         }
     }
     return r;
@@ -196,7 +204,8 @@ inline NavBox make_box(const NavSurface& r) {
     if (r.tolk == 0.f) return b;
     for (int i = 0; i < 3; ++i) b.c[i] = static_cast<float>(r.t[i]);
     b.r = round_up_float(r.radius + 1e-5 * (fabs(r.t[0]) + fabs(r.t[1]) + fabs(r.t[2])));
-    if (r.shape == DTR_RECTANGLE2D || r.shape == DTR_TRAPEZOID2D) {
+    if (r.shape == DTR_RECTANGLE2D || r.shape == DTR_TRAPEZOID2D || r.shape == DTR_LINE_SQUARE ||
+        r.shape == DTR_LINE_CIRCULAR) {
         for (int i = 0; i < 3; ++i) {
             b.ex[i] = static_cast<float>(r.r[3 * i + 0]);
             b.ey[i] = static_cast<float>(r.r[3 * i + 1]);
@@ -206,6 +215,10 @@ inline NavBox make_box(const NavSurface& r) {
             b.hx = round_up_float(r.m[0]);
             b.hy = round_up_float(r.m[1]);
             b.kx = 1.f;
+        } else if (r.shape == DTR_LINE_SQUARE || r.shape == DTR_LINE_CIRCULAR) {
+            b.hx = round_up_float(1.4142135623730951 * r.m[0] * (1.0 + 1e-7));
+            b.hy = 0.f;
+            b.kx = -1.f;
         } else {
             const double hx = r.m[0] > r.m[1] ? r.m[0] : r.m[1];
             b.hx = round_up_float(hx);
@@ -742,6 +755,22 @@ struct FastEngine {
         const float rb = b.r + margin;
         if (!(b.r >= 0.f) || dist2 > rb * rb) return false;
         if (b.hx < 0.f) return true;
+        if (b.kx < 0.f) {
+            // wire: distance of its axis from the ray, |(c - o).(d x a)| /
+            // |d x a|, against sqrt(2) (r + t); near-parallel rays (|d x a|
+            // < 0.1) pass.  FP32 error margin 1e-5 (|c - o|_1 + |c|_1) / |d x a|
+            // plus the caller's (at least 1e-3 mm), over 100 times the
+            // rounding error of the FP32 evaluation.
+            const float wx = dy * b.n[2] - dz * b.n[1], wy = dz * b.n[0] - dx * b.n[2],
+                        wz = dx * b.n[1] - dy * b.n[0];
+            const float w2 = wx * wx + wy * wy + wz * wz;
+            if (!(w2 >= 0.01f)) return true;
+            const float wn = sqrtf(w2);
+            const float dist = fabsf(cx * wx + cy * wy + cz * wz) / wn;
+            const float m = 1e-5f * (fabsf(cx) + fabsf(cy) + fabsf(cz) + fabsf(b.c[0]) + fabsf(b.c[1]) +
+                                     fabsf(b.c[2])) / wn + margin;
+            return !(dist > b.hx + 1.5f * ft + m);
+        }
         const float denom = b.n[0] * dx + b.n[1] * dy + b.n[2] * dz;
         if (!(fabsf(denom) >= 0.05f)) return true;
         const float s = (b.n[0] * cx + b.n[1] * cy + b.n[2] * cz) / denom;
