@@ -755,7 +755,7 @@ struct FastEngine {
         const float rb = b.r + margin;
         if (!(b.r >= 0.f) || dist2 > rb * rb) return false;
         if (b.hx < 0.f) return true;
-        if (b.kx < 0.f) {
+        if (P::kExt && b.kx < 0.f) {
             // wire: distance of its axis from the ray, |(c - o).(d x a)| /
             // |d x a|, against sqrt(2) (r + t); near-parallel rays (|d x a|
             // < 0.1) pass.  FP32 error margin 1e-5 (|c - o|_1 + |c|_1) / |d x a|
