@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 from typing import Optional
 
 import numpy as np
@@ -137,6 +137,9 @@ class ForwardResult:
     status: "object"           # [n] int32 (dtr_tape_status)
     tape: "object"             # uint8 tensor
     order: "object" = None     # [n] int32 track order of the pass, or None
+    # the adjoint configuration the tape was written with (the backward pass
+    # reads the tape with the same layout and precision)
+    adjoint: "object" = None
 
 
 def direction_order(bound_params, eta_bins_per_unit: float = 20.0):
@@ -166,7 +169,11 @@ class Chi2Propagator:
         self._c_config = self.config.to_c()
 
     def tape_bytes(self, n: int) -> int:
-        return int(capi.load().dtr_adjoint_tape_size(C.byref(self.adjoint.to_c()), int(n)))
+        return self.tape_bytes_for(self.adjoint, n)
+
+    @staticmethod
+    def tape_bytes_for(adjoint: "AdjointConfig", n: int) -> int:
+        return int(capi.load().dtr_adjoint_tape_size(C.byref(adjoint.to_c()), int(n)))
 
     def _inputs(self, detector: Detector, bound_params, start_surface):
         import torch
@@ -217,7 +224,7 @@ class Chi2Propagator:
                                            C.c_void_p(gt.data_ptr()) if gt is not None else None,
                                            C.c_void_p(status.data_ptr()), C.c_void_p(_stream(dev))),
                    "dtr_adjoint_forward")
-        return ForwardResult(chi2, gt, status, tape, order)
+        return ForwardResult(chi2, gt, status, tape, order, replace(self.adjoint))
 
     def backward(self, detector: Detector, field: MagneticField, bound_params, start_surface,
                  hits: Optional[Hits], fwd: ForwardResult, model: Optional[str] = None):
@@ -230,8 +237,14 @@ class Chi2Propagator:
             hits.check(n)
         grad = torch.empty((6, n), dtype=torch.float64, device=bound.device)
         hc = hits.to_c() if hits is not None else None
+        # the tape layout and precision are those of the forward pass, even if
+        # self.adjoint changed since
+        acfg = fwd.adjoint if fwd.adjoint is not None else self.adjoint
+        if fwd.tape is None or int(fwd.tape.numel()) < self.tape_bytes_for(acfg, n):
+            raise RuntimeError("the forward result holds no tape for this batch")
         capi.check(capi.load().dtr_adjoint_backward(
-            detector.device(), field.device(), C.byref(self._c_config), C.byref(self.adjoint.to_c(model)),
+            detector.device(), field.device(), C.byref(self._c_config),
+            C.byref(acfg.to_c(model or self.adjoint.model)),
             C.byref(self._batch(bound, start, n, fwd.tape, fwd.order)), C.byref(hc) if hc is not None else None,
             C.c_void_p(grad.data_ptr()), C.c_void_p(_stream(bound.device))), "dtr_adjoint_backward")
         return grad
@@ -280,9 +293,10 @@ def _autograd_function():
         @staticmethod
         def backward(ctx, grad_chi2):
             bound_params, start_surface, surface, loc, n_hits = ctx.saved_tensors
+            # the tape stays with the graph (ctx), so a second backward
+            # (retain_graph=True) reads it again
             g = ctx.prop.backward(ctx.detector, ctx.field, bound_params.detach(), start_surface,
                                   Hits(surface, loc, n_hits), ctx.fwd)
-            ctx.fwd = None  # the tape is needed once
             return grad_chi2[None, :] * g, None, None, None, None, None, None, None
 
     return Chi2Function

@@ -21,6 +21,7 @@
 #include "device/direct_nav.hpp"
 #include "device/engine.hpp"
 #include "device/propagator.hpp"
+#include "host/tables.hpp"
 
 struct dtr_detector {
     dtr::DevDetector dev;
@@ -398,7 +399,7 @@ int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dt
     if (a->precision != DTR_PRECISION_FP64 && a->precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->model != DTR_ADJOINT_EXACT && a->model != DTR_ADJOINT_TRANSPORT) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->block_size != 0u && (a->block_size % 64u != 0u || a->block_size > 256u)) return DTR_ERROR_INVALID_ARGUMENT;
-    if (a->waves_per_simd > 32u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (a->waves_per_simd > 2u) return DTR_ERROR_INVALID_ARGUMENT;
     if (!(a->sigma_loc0 > 0.0) || !(a->sigma_loc1 > 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
@@ -443,7 +444,7 @@ template <class P>
 int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
                 const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
                 hipStream_t stream) {
-    if (L.waves_per_simd >= 2u) return launch_fast_w<P, 2>(det, field, dc, in, r, L, stream);
+    if (L.waves_per_simd == 2u) return launch_fast_w<P, 2>(det, field, dc, in, r, L, stream);
     return launch_fast_w<P, 1>(det, field, dc, in, r, L, stream);
 }
 
@@ -488,7 +489,9 @@ void dtr_default_config(dtr_prop_config* c) {
 int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     if (!d || !out) return DTR_ERROR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (d->n_volumes == 0u || d->n_surfaces == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    // every index the kernels follow is checked on the host first, so that
+    // inconsistent tables are an error code, never an out-of-bounds read
+    if (!dtr_host::validate_desc(*d).empty()) return DTR_ERROR_INVALID_ARGUMENT;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return DTR_ERROR_NO_DEVICE;
     dtr_detector* h = new (std::nothrow) dtr_detector{};
@@ -503,39 +506,22 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     if (!rc) rc = upload(d->bf_surfaces, d->n_bf, &h->blocks[6]);
     if (!rc) rc = upload(d->grid_bins, d->n_bins, &h->blocks[7]);
     if (!rc) {
-        for (uint32_t k = 0; k < d->n_surfaces && !rc; ++k) {
+        for (uint32_t k = 0; k < d->n_surfaces; ++k) {
             const dtr_surface& sf = d->surfaces[k];
-            if (sf.transform >= d->n_transforms || sf.mask >= d->n_masks || sf.n_masks == 0u ||
-                sf.mask + sf.n_masks > d->n_masks || sf.shape > DTR_CYLINDER2D ||
-                (sf.material != 0xffffffffu && sf.material >= d->n_materials))
-                rc = DTR_ERROR_INVALID_ARGUMENT;
             if (sf.shape >= DTR_LINE_SQUARE) h->wires = true;
             // a ring the parameter updater stops at needs the polar Jacobians
             if (sf.shape == DTR_RING2D && (sf.sf_id == DTR_SENSITIVE || sf.material != 0xffffffffu))
                 h->wires = true;
         }
-        for (uint32_t k = 0; k < d->n_materials && !rc; ++k) {
-            if (d->materials[k].kind > DTR_MATERIAL_ROD) rc = DTR_ERROR_INVALID_ARGUMENT;
+        for (uint32_t k = 0; k < d->n_materials; ++k)
             if (d->materials[k].kind == DTR_MATERIAL_ROD) h->wires = true;
-        }
-        for (uint32_t k = 0; k < d->n_grids && !rc; ++k) {
-            const dtr_grid& g = d->grids[k];
-            const uint64_t cap = g.capacity ? g.capacity : 1u;
-            if (g.n0 == 0u || g.n1 == 0u || g.type > DTR_GRID_RING ||
-                static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 * cap > d->n_bins)
-                rc = DTR_ERROR_INVALID_ARGUMENT;
-            if (cap > 1u) h->wires = true;
-        }
+        for (uint32_t k = 0; k < d->n_grids; ++k)
+            if (d->grids[k].capacity > 1u) h->wires = true;
         if (!rc) {
             std::vector<dtr::NavSurface> nav(d->n_surfaces);
             for (uint32_t k = 0; k < d->n_surfaces; ++k) nav[k] = dtr::make_nav_surface(*d, k);
             // the cooperative search packs a bin index into 26 bits
             if (d->n_bins >= (1u << 26)) rc = DTR_ERROR_UNSUPPORTED;
-            for (uint32_t k = 0; k < d->n_bins && !rc; ++k)
-                if (d->grid_bins[k] != 0xffffffffu && d->grid_bins[k] >= d->n_surfaces)
-                    rc = DTR_ERROR_INVALID_ARGUMENT;
-            for (uint32_t k = 0; k < d->n_bf && !rc; ++k)
-                if (d->bf_surfaces[k] >= d->n_surfaces) rc = DTR_ERROR_INVALID_ARGUMENT;
             std::vector<dtr::NavRecord> bin_rec(d->n_bins), bf_rec(d->n_bf);
             std::vector<float> gtolk(d->n_grids);
             if (!rc) dtr::make_nav_records(*d, nav.data(), bin_rec.data(), bf_rec.data(), gtolk.data());
@@ -732,7 +718,9 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (L.min_idle == 0u) L.min_idle = 1u;
     if (L.min_idle > 64u) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
-    if (L.tracks_per_claim % 64u != 0u || L.waves_per_simd > 32u) return DTR_ERROR_INVALID_ARGUMENT;
+    // the kernels are built for one or two resident waves per SIMD (512 or
+    // 256 registers per lane); a persistent grid is sized with the same value
+    if (L.tracks_per_claim % 64u != 0u || L.waves_per_simd > 2u) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule != DTR_SCHEDULE_GRID && L.schedule != DTR_SCHEDULE_PERSISTENT) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule == DTR_SCHEDULE_PERSISTENT && !L.queue) return DTR_ERROR_INVALID_ARGUMENT;

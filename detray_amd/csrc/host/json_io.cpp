@@ -183,7 +183,7 @@ struct SurfaceIn {
     unsigned io_key = 0;   // factory key: io shape id, cylinder2 -> portal_cylinder2
     dtr_transform trf{};
     std::vector<std::array<double, 7>> bounds;
-    std::vector<uint16_t> links;
+    std::vector<uint64_t> links;  // as in the file: a volume index or the world exit 2^64-1
     bool has_index = false;
     uint64_t index = 0;
 };
@@ -265,8 +265,8 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
         std::array<double, 7> b{};
         numbers(need(*masks[k], "boundaries", mw), n_bounds, b.data(), mw + ".boundaries");
         out.bounds.push_back(b);
-        // the navigation link is a 16-bit value: the world exit 2^64-1 -> 0xffff
-        out.links.push_back(static_cast<uint16_t>(as_index(need(*masks[k], "volume_link", mw), mw + ".volume_link")));
+        // checked against the volume count once every volume is read
+        out.links.push_back(as_index(need(*masks[k], "volume_link", mw), mw + ".volume_link"));
     }
     if (out.shape == DTR_CONCENTRIC_CYLINDER2D) {
         const double dz = out.trf.t[2];
@@ -516,7 +516,20 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
             for (size_t m = 0; m < S.bounds.size(); ++m) {
                 dtr_mask dm{};
                 for (int q = 0; q < 7; ++q) dm.v[q] = S.bounds[m][q];
-                dm.volume_link = S.links[m];
+                // the navigation link is a 16-bit value: the world exit (the
+                // invalid value of a 16-bit, 32-bit or 64-bit link,
+                // basic_converter.hpp:21-31) -> 0xffff; any other link names
+                // a volume of the file
+                const uint64_t link = S.links[m];
+                if (link == ~0ull || link == 0xffffffffull || link == kEndOfWorld) {
+                    dm.volume_link = kEndOfWorld;
+                } else if (link < vols.size()) {
+                    dm.volume_link = static_cast<uint16_t>(link);
+                } else {
+                    bad("volume " + std::to_string(vi) + " surface " + std::to_string(k) + ": mask volume_link " +
+                        std::to_string(link) + " is neither a volume of the file (" + std::to_string(vols.size()) +
+                        ") nor the world exit");
+                }
                 dm.shape = S.shape;
                 T.masks.push_back(dm);
             }
@@ -574,6 +587,8 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
 void check_desc(const dtr_detector_desc& d) {
     if (d.n_volumes == 0u || !d.volumes || !d.surfaces || !d.transforms || !d.masks)
         bad("detector tables are empty");
+    const std::string why = validate_desc(d);
+    if (!why.empty()) bad(why);
     uint32_t next = 0;
     for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
         const dtr_volume& v = d.volumes[vi];
@@ -863,6 +878,25 @@ int dtr_read_detector_json(const char* geometry_file, const char* grids_file, co
         if (material_file) read_material(load(material_file, "homogeneous material"), vols);
         Tables T;
         assemble(vols, T);
+        dtr_detector_desc view{};
+        view.n_volumes = static_cast<uint32_t>(T.volumes.size());
+        view.n_surfaces = static_cast<uint32_t>(T.surfaces.size());
+        view.n_transforms = static_cast<uint32_t>(T.transforms.size());
+        view.n_masks = static_cast<uint32_t>(T.masks.size());
+        view.n_materials = static_cast<uint32_t>(T.materials.size());
+        view.n_grids = static_cast<uint32_t>(T.grids.size());
+        view.n_bf = static_cast<uint32_t>(T.bf.size());
+        view.n_bins = static_cast<uint32_t>(T.bins.size());
+        view.volumes = T.volumes.data();
+        view.surfaces = T.surfaces.data();
+        view.transforms = T.transforms.data();
+        view.masks = T.masks.data();
+        view.materials = T.materials.data();
+        view.grids = T.grids.data();
+        view.bf_surfaces = T.bf.data();
+        view.grid_bins = T.bins.data();
+        const std::string why = validate_desc(view);
+        if (!why.empty()) bad("detector tables read from the files: " + why);
         export_tables(T, out);
         return DTR_SUCCESS;
     } catch (const IoError& e) {

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace dtr_host {
@@ -69,6 +70,58 @@ T* dup(const std::vector<T>& v) {
     T* p = static_cast<T*>(std::malloc(sizeof(T) * v.size()));
     if (p) std::memcpy(p, v.data(), sizeof(T) * v.size());
     return p;
+}
+
+// Every index the navigation follows is in range: volume transforms,
+// surface ranges, brute-force ranges and grids; surface transforms, masks,
+// materials and volumes; mask volume links (a volume, or the world exit
+// 0xffff); grid bin ranges and the surfaces in bins and brute-force lists.
+// Returns an empty string for consistent tables, otherwise what is wrong.
+inline std::string validate_desc(const dtr_detector_desc& d) {
+    auto at = [](const char* what, uint64_t i) { return std::string(what) + " " + std::to_string(i) + ": "; };
+    if (d.n_volumes == 0u || d.n_surfaces == 0u) return "no volumes or no surfaces";
+    if (d.n_volumes > kEndOfWorld) return "more volumes than a 16-bit volume link can name";
+    if (!d.volumes || !d.surfaces || (d.n_transforms && !d.transforms) || (d.n_masks && !d.masks) ||
+        (d.n_materials && !d.materials) || (d.n_grids && !d.grids) || (d.n_bf && !d.bf_surfaces) ||
+        (d.n_bins && !d.grid_bins))
+        return "a table with a non-zero count has no data";
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        if (v.transform >= d.n_transforms) return at("volume", vi) + "transform out of range";
+        if (v.sf_begin > v.sf_end || v.sf_end > d.n_surfaces) return at("volume", vi) + "surface range out of range";
+        if (v.bf_begin > v.bf_end || v.bf_end > d.n_bf) return at("volume", vi) + "brute-force range out of range";
+        if (v.grid < -1 || (v.grid >= 0 && static_cast<uint32_t>(v.grid) >= d.n_grids))
+            return at("volume", vi) + "grid out of range";
+    }
+    for (uint32_t s = 0; s < d.n_surfaces; ++s) {
+        const dtr_surface& sf = d.surfaces[s];
+        if (sf.transform >= d.n_transforms) return at("surface", s) + "transform out of range";
+        if (sf.n_masks == 0u || static_cast<uint64_t>(sf.mask) + sf.n_masks > d.n_masks)
+            return at("surface", s) + "mask range out of range";
+        if (sf.shape > DTR_CYLINDER2D) return at("surface", s) + "unknown shape";
+        if (sf.sf_id > DTR_PASSIVE) return at("surface", s) + "unknown surface id";
+        if (sf.material != kInvalid && sf.material >= d.n_materials) return at("surface", s) + "material out of range";
+        if (sf.volume >= d.n_volumes) return at("surface", s) + "volume out of range";
+    }
+    for (uint32_t m = 0; m < d.n_masks; ++m) {
+        const uint16_t l = d.masks[m].volume_link;
+        if (l != kEndOfWorld && l >= d.n_volumes) return at("mask", m) + "volume link out of range";
+        if (d.masks[m].shape > DTR_CYLINDER2D) return at("mask", m) + "unknown shape";
+    }
+    for (uint32_t k = 0; k < d.n_materials; ++k)
+        if (d.materials[k].kind > DTR_MATERIAL_ROD) return at("material", k) + "unknown kind";
+    for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
+        const dtr_grid& g = d.grids[gi];
+        const uint64_t cap = g.capacity ? g.capacity : 1u;
+        if (g.n0 == 0u || g.n1 == 0u || g.type > DTR_GRID_RING) return at("grid", gi) + "empty axis or unknown type";
+        if (static_cast<uint64_t>(g.bins) + static_cast<uint64_t>(g.n0) * g.n1 * cap > d.n_bins)
+            return at("grid", gi) + "bins out of range";
+    }
+    for (uint32_t k = 0; k < d.n_bins; ++k)
+        if (d.grid_bins[k] != kInvalid && d.grid_bins[k] >= d.n_surfaces) return at("grid bin", k) + "surface out of range";
+    for (uint32_t k = 0; k < d.n_bf; ++k)
+        if (d.bf_surfaces[k] >= d.n_surfaces) return at("brute-force entry", k) + "surface out of range";
+    return std::string();
 }
 
 inline void export_tables(const Tables& T, dtr_detector_desc* out) {

@@ -486,8 +486,9 @@ class Propagator:
                        "dtr_propagate")
             return out
         order = self.launch.order
-        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)):
-            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
+        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)
+                                  or not order.is_contiguous() or order.device != device):
+            raise ValueError("launch.order must be a contiguous int32 CUDA tensor [n] on the tracks' device")
         lc = self.launch.to_c(self._queue(device) if self.launch.schedule == "persistent" else None)
         capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(self._c_config),
                                         C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
@@ -517,8 +518,9 @@ class Propagator:
         n_sequence = n_sequence.contiguous()
         launch = self.launch or LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
         if launch.order is not None and (not launch.order.is_cuda or launch.order.dtype != torch.int32
-                                         or launch.order.shape != (n,)):
-            raise ValueError("launch.order must be an int32 CUDA tensor [n]")
+                                         or launch.order.shape != (n,) or not launch.order.is_contiguous()
+                                         or launch.order.device != device):
+            raise ValueError("launch.order must be a contiguous int32 CUDA tensor [n] on the tracks' device")
         seq = capi.SurfaceSequence(int(sequence.shape[0]), 0, sequence.data_ptr(), n_sequence.data_ptr())
         if stream is None:
             stream = torch.cuda.current_stream(device).cuda_stream
@@ -527,6 +529,27 @@ class Propagator:
                                                     C.byref(tb), C.byref(seq), C.byref(rb), C.byref(lc),
                                                     C.c_void_p(stream)), "dtr_propagate_direct")
         return out
+
+    def _check_out(self, out: PropagationResult, n: int, device) -> None:
+        """A caller-supplied result set must hold n tracks (and
+        ``max_surfaces`` sequence rows) in contiguous tensors of the right
+        type on the tracks' device: the kernels write every column of it."""
+        import torch
+
+        ms = max(int(self.config.max_surfaces), 1)
+        spec = {"bound_params": ((6, n), torch.float64), "bound_cov": ((21, n), torch.float64),
+                "free_params": ((8, n), torch.float64), "path_length": ((n,), torch.float64),
+                "n_surfaces": ((n,), torch.int32), "n_steps": ((n,), torch.int32),
+                "n_trials": ((n,), torch.int32), "status": ((n,), torch.uint8)}
+        for name, (shape, dtype) in spec.items():
+            t = getattr(out, name)
+            if (not isinstance(t, torch.Tensor) or tuple(t.shape) != shape or t.dtype != dtype
+                    or t.device != device or not t.is_contiguous()):
+                raise ValueError(f"out.{name} must be a contiguous {dtype} tensor {list(shape)} on {device}")
+        t = out.surfaces
+        if (not isinstance(t, torch.Tensor) or t.dim() != 2 or t.shape[1] != n or t.shape[0] < ms
+                or t.dtype != torch.int32 or t.device != device or not t.is_contiguous()):
+            raise ValueError(f"out.surfaces must be a contiguous int32 tensor [>= {ms}, {n}] on {device}")
 
     def _batches(self, detector, tracks, initial_cov, out, bound_params, start_surface):
         """Validated C views of the inputs and (allocated) outputs."""
@@ -562,6 +585,8 @@ class Propagator:
             initial_cov = initial_cov.contiguous()
         if out is None:
             out = self.allocate(n, device)
+        else:
+            self._check_out(out, n, device)
         tb = capi.TrackBatch(n, 0, tracks.data_ptr() if tracks is not None and bound_params is None else None,
                              initial_cov.data_ptr() if initial_cov is not None else None,
                              bound_params.data_ptr() if bound_params is not None else None,

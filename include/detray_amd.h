@@ -326,9 +326,10 @@ typedef struct dtr_launch_config {
     uint32_t schedule;         /* dtr_schedule (default PERSISTENT) */
     uint32_t block_size;       /* 64, 128, 192 or 256 (default 256) */
     uint32_t waves_per_simd;   /* resident waves per SIMD the kernel is built for: 1
-                                  (512 registers per lane) or >= 2 (256); the
+                                  (512 registers per lane) or 2 (256); the
                                   persistent schedule launches this many waves
-                                  per SIMD, 1..32 (default 2) */
+                                  per SIMD; other values are rejected with
+                                  DTR_ERROR_INVALID_ARGUMENT (default 2) */
     uint32_t tracks_per_claim; /* persistent: queue entries a wave claims per atomic,
                                   multiple of 64 (default 64) */
     uint32_t min_idle;         /* persistent: idle lanes a wave waits for before it
@@ -442,7 +443,8 @@ typedef struct dtr_adjoint_config {
     uint32_t precision;        /* dtr_precision of both passes (MIXED) */
     uint32_t model;            /* dtr_adjoint_model of the backward pass (EXACT) */
     uint32_t block_size;       /* work-items per block, 64..256, multiple of 64 (64) */
-    uint32_t waves_per_simd;   /* register budget the kernels are built for: 1 or 2 (2) */
+    uint32_t waves_per_simd;   /* register budget the kernels are built for: 1 or 2 (2);
+                                  0 means 2, other values are rejected */
     uint32_t pad;
     double sigma_loc0;         /* hit resolution in loc0, mm (0.010) */
     double sigma_loc1;         /* hit resolution in loc1, mm (0.050) */

@@ -7,6 +7,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 import detray_amd as da
 from detray_amd import capi
@@ -85,3 +86,75 @@ def test_propagate_ex_rejects_bad_launch_before_touching_the_device():
     rb = capi.ResultBatch()
     # null handles are rejected first
     assert lib.dtr_propagate_ex(None, None, C.byref(cfg), C.byref(tb), C.byref(rb), None, None) == 1
+    # waves_per_simd beyond the two register budgets the kernels are built
+    # for is rejected (it used to alias the W = 2 kernel)
+    lc = capi.LaunchConfig()
+    lib.dtr_default_launch(C.byref(lc))
+    lc.queue = C.c_void_p(8)
+    fake = C.c_void_p(8)  # never dereferenced: an empty batch returns after the launch checks
+    for w, rc in ((1, 0), (2, 0), (3, 1), (4, 1), (32, 1)):
+        lc.waves_per_simd = w
+        assert lib.dtr_propagate_ex(fake, fake, C.byref(cfg), C.byref(tb), C.byref(rb), C.byref(lc), None) == rc, w
+
+
+def _copy_tables(det):
+    """Python-owned copies of a detector's tables (for corruption tests)."""
+    return {name: [type(x).from_buffer_copy(x) if hasattr(x, "_fields_") else int(x) for x in det.table(name)]
+            for name in ("volumes", "surfaces", "transforms", "masks", "materials", "grids", "bf_surfaces",
+                         "grid_bins")}
+
+
+def _create(tables):
+    d = da.Detector.from_tables(**tables)
+    h = C.c_void_p()
+    return capi.load().dtr_detector_create(C.byref(d.desc), C.byref(h))
+
+
+def _corrupt_cases():
+    yield "mask link past the volumes", lambda t: setattr(t["masks"][5], "volume_link", len(t["volumes"]))
+    yield "volume grid out of range", lambda t: setattr(t["volumes"][7], "grid", len(t["grids"]))
+    yield "volume brute-force range", lambda t: setattr(t["volumes"][3], "bf_end", len(t["bf_surfaces"]) + 1)
+    yield "volume surface range", lambda t: setattr(t["volumes"][2], "sf_end", len(t["surfaces"]) + 5)
+    yield "volume transform", lambda t: setattr(t["volumes"][1], "transform", len(t["transforms"]))
+    yield "surface volume", lambda t: setattr(t["surfaces"][9], "volume", len(t["volumes"]))
+    yield "surface material", lambda t: setattr(t["surfaces"][9], "material", len(t["materials"]))
+    yield "grid bins", lambda t: setattr(t["grids"][0], "bins", len(t["grid_bins"]))
+    yield "bin surface", lambda t: t["grid_bins"].__setitem__(3, len(t["surfaces"]))
+    yield "brute-force surface", lambda t: t["bf_surfaces"].__setitem__(0, len(t["surfaces"]) + 1)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _corrupt_cases()])
+def test_detector_create_rejects_inconsistent_tables(toy3, case):
+    """Every index the kernels follow is validated on the host before any
+    upload: a table that points out of range is DTR_ERROR_INVALID_ARGUMENT
+    (here without a device; consistent tables reach the device check)."""
+    mutate = dict(_corrupt_cases())[case]
+    t = _copy_tables(toy3)
+    rc_good = _create(t)
+    assert rc_good in (0, 4)  # success with a GPU, DTR_ERROR_NO_DEVICE without
+    mutate(t)
+    assert _create(t) == 1, case
+
+
+def test_caller_out_is_validated():
+    """A caller-supplied result set must fit the batch: wrong shapes, types,
+    too few sequence rows or non-contiguous tensors are rejected before any
+    launch (checked here with host tensors)."""
+    torch = pytest.importorskip("torch")
+    prop = da.Propagator(da.PropagationConfig(max_surfaces=16))
+    cpu = torch.device("cpu")
+    good = prop.allocate(8, cpu)
+    prop._check_out(good, 8, cpu)
+    with pytest.raises(ValueError, match="bound_params"):
+        prop._check_out(prop.allocate(4, cpu), 8, cpu)
+    small = da.Propagator(da.PropagationConfig(max_surfaces=8)).allocate(8, cpu)
+    with pytest.raises(ValueError, match="surfaces"):
+        prop._check_out(small, 8, cpu)
+    bad = prop.allocate(8, cpu)
+    bad.status = bad.status.to(torch.int32)
+    with pytest.raises(ValueError, match="status"):
+        prop._check_out(bad, 8, cpu)
+    strided = prop.allocate(8, cpu)
+    strided.bound_cov = torch.zeros((8, 21), dtype=torch.float64).t()
+    with pytest.raises(ValueError, match="bound_cov"):
+        prop._check_out(strided, 8, cpu)

@@ -299,3 +299,31 @@ def test_graph_of_sort_and_persistent_propagation_replays(toy7, solenoid):
         da.Propagator(cfg, da.LaunchConfig("mixed", "grid")).propagate(
             toy7, solenoid, tracks[:, :1000].contiguous())
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("sort_key", ["cells_phi", "cells_phi_fine"])
+def test_headline_launch_against_oracle(toy7, solenoid, sort_key):
+    """The benchmark's own launch (mixed precision, grid schedule, block 64,
+    two waves per SIMD, per-lane navigation, tracks ordered by the native
+    direction-cell sort) on the first 131072 tracks of the benchmark batch
+    (toy muons, seed 44, solenoid map), against the oracle."""
+    from detray_amd.schedule import sort_order
+
+    cfg = da.PropagationConfig(**TOY_CFG)
+    fp = da.generate_tracks("toy", 131072, 44)
+    tracks = torch.from_numpy(fp).cuda()
+    launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
+    launch.order = sort_order(tracks, sort_key)
+    print(f"headline launch, sort {sort_key}", flush=True)
+    res = da.Propagator(cfg, launch).propagate(toy7, solenoid, tracks)
+    torch.cuda.synchronize()
+    got = {
+        "bound_params": res.bound_params.cpu().numpy(),
+        "bound_cov": res.bound_cov.cpu().numpy(),
+        "path_length": res.path_length.cpu().numpy(),
+        "surfaces": res.surfaces.cpu().numpy().view(np.uint32),
+        "n_surfaces": res.n_surfaces.cpu().numpy().view(np.uint32),
+        "status": res.status.cpu().numpy(),
+    }
+    ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
+    assert_mixed_parity(got, ref)

@@ -352,6 +352,19 @@ def test_errors(tmp_path, toy3_files):
     assert read_error(f["geometry"], None, dump(tmp_path / "m2.json", mat))[0] == INVALID_ARGUMENT
 
 
+@pytest.mark.parametrize("link", [22, 23, 65535 + 1, 2 ** 32, 2 ** 63])
+def test_out_of_range_volume_link(tmp_path, toy3_files, link):
+    """A mask link must name a volume of the file or be the world exit (the
+    invalid 16-, 32- or 64-bit link): a link to volume 22 of the 22-volume
+    toy, or one that would wrap to a valid 16-bit index (65536 -> 0), is an
+    error, not a silent link."""
+    _, f = toy3_files
+    geo = load(f["geometry"])
+    geo["data"]["volumes"][4]["surfaces"][0]["masks"][0]["volume_link"] = link
+    code, msg = read_error(dump(tmp_path / "l.json", geo))
+    assert code == INVALID_ARGUMENT and "volume_link" in msg
+
+
 def test_writer_rejects_inconsistent_tables():
     lib = capi.load()
     msg = C.create_string_buffer(256)
