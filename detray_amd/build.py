@@ -12,6 +12,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -27,9 +28,8 @@ ARCH = os.environ.get("DETRAY_AMD_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
-    "-shared",
     # keep FP64 arithmetic in source order (no FMA contraction) so the
     # device path rounds like the oracle
     "-ffp-contract=off",
@@ -55,11 +55,33 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
 
 
+def _sources() -> list[Path]:
+    """The C ABI and host code, then one translation unit per group of
+    kernel instantiations (compiled in parallel)."""
+    return [CSRC / "capi.hip", CSRC / "host" / "builders.cpp", CSRC / "host" / "json_io.cpp",
+            *sorted((CSRC / "kernels").glob("k_*.hip"))]
+
+
+def _jobs() -> int:
+    return max(1, min(int(os.environ.get("MAX_JOBS", os.cpu_count() or 1)), 16))
+
+
 def build_library(out: Path = LIB) -> Path:
-    srcs = [CSRC / "capi.hip", CSRC / "host" / "builders.cpp", CSRC / "host" / "json_io.cpp"]
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
+    hipcc = _hipcc()
+
+    def compile_one(src: Path) -> Path:
+        obj = objdir / (src.stem + ".o")
+        lang = [] if src.suffix == ".hip" else ["-x", "hip"]
+        _run([hipcc, *HIP_FLAGS, *inc, "-c", *lang, str(src), "-o", str(obj)])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=_jobs()) as pool:
+        objs = list(pool.map(compile_one, _sources()))
     tmp = out.with_suffix(".so.tmp")
-    cmd = [_hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, srcs), "-o", str(tmp)]
-    _run(cmd)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 
@@ -67,7 +89,7 @@ def build_library(out: Path = LIB) -> Path:
 def build_oracle(out: Path = ORACLE_LIB) -> Path:
     cxx = os.environ.get("CXX", "g++")
     tmp = out.with_suffix(".so.tmp")
-    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+    cmd = [cxx, "-std=c++20", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
            f"-I{INCLUDE}", str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp)]
     _run(cmd)
     os.replace(tmp, out)

@@ -16,40 +16,13 @@
 #include <vector>
 
 #include "detray_amd.h"
-#include "device/adjoint.hpp"
-#include "device/coop_nav.hpp"
-#include "device/direct_nav.hpp"
-#include "device/engine.hpp"
-#include "device/propagator.hpp"
 #include "host/tables.hpp"
-
-struct dtr_detector {
-    dtr::DevDetector dev;
-    dtr::FastDetector fast;
-    void* blocks[12];  // the eight tables, surface records, bin records, brute-force records, grid tolerances
-    uint32_t n_volumes;
-    uint32_t n_surfaces;
-    int device;
-    uint32_t n_cu;
-    // wire, annulus or placed-cylinder surfaces, material rods or grid bins of
-    // several surfaces: only the plain FP64 kernel propagates through such a
-    // detector
-    bool wires;
-};
-
-struct dtr_field {
-    dtr::DevField dev;
-    float* values;
-};
+#include "device/coop_nav.hpp"
+#include "launch.hpp"
 
 namespace {
 
-int to_status(hipError_t e) {
-    if (e == hipSuccess) return DTR_SUCCESS;
-    if (e == hipErrorOutOfMemory) return DTR_ERROR_OUT_OF_MEMORY;
-    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return DTR_ERROR_NO_DEVICE;
-    return DTR_ERROR_HIP;
-}
+using dtr::to_status;
 
 template <typename T>
 int upload(const T* host, uint32_t n, void** dst) {
@@ -60,144 +33,6 @@ int upload(const T* host, uint32_t n, void** dst) {
     if (e != hipSuccess) return to_status(e);
     e = hipMemcpy(*dst, host, sizeof(T) * static_cast<size_t>(n), hipMemcpyHostToDevice);
     return to_status(e);
-}
-
-// kExt: detectors with wires, annuli, placed cylinders, rods, multi-surface
-// bins or rings with material
-template <bool kExt>
-__global__ void __launch_bounds__(256) propagate_fp64_kernel(
-    dtr::DevDetector det, dtr::DevField field, dtr::DevConfig cfg,
-    const double* __restrict__ fp, const double* __restrict__ init_cov,
-    const double* __restrict__ bound0, const uint32_t* __restrict__ start_sf,
-    dtr::DevResults out, uint32_t n) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
-    dtr::propagate_track<kExt>(det, field, cfg, fp, init_cov, bound0, start_sf, out, n, gid);
-}
-
-// LDS of a block: the candidate cache, slot-major ([8][blockDim] paths, then
-// [8][blockDim] surface indices and [8][blockDim] link|status words), then
-// the exchange slots and FIFO of each wave for the cooperative navigation.
-__device__ inline dtr::CacheView lds_cache(unsigned char* smem) {
-    const int nb = static_cast<int>(blockDim.x);
-    double* path = reinterpret_cast<double*>(smem);
-    uint32_t* sf = reinterpret_cast<uint32_t*>(path + dtr::kCache * nb);
-    uint32_t* ls = sf + dtr::kCache * nb;
-    const int t = static_cast<int>(threadIdx.x);
-    return dtr::CacheView{path + t, sf + t, ls + t, nb};
-}
-
-__device__ inline dtr::CoopShared* lds_shared(unsigned char* smem) {
-    dtr::CoopShared* all =
-        reinterpret_cast<dtr::CoopShared*>(smem + static_cast<size_t>(blockDim.x) * dtr::kCache * 16u);
-    return all + threadIdx.x / 64u;
-}
-
-size_t lds_bytes(uint32_t block) {
-    return static_cast<size_t>(block) * dtr::kCache * 16u + (block / 64u) * sizeof(dtr::CoopShared);
-}
-
-// The wave interface of the cooperative navigation on CDNA (64 lanes).
-struct WaveHip {
-    static __device__ uint32_t lane() { return __lane_id(); }
-    template <class T>
-    static __device__ T shfl(T v, int src) {
-        return __shfl(v, src);
-    }
-    static __device__ uint64_t ballot(bool p) { return __ballot(p); }
-    static __device__ bool any(bool p) { return __any(p) != 0; }
-    static __device__ void sync() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    static __device__ uint32_t atomic_add(uint32_t* p, uint32_t v) { return atomicAdd(p, v); }
-};
-
-// The batched kernels are compiled for W resident waves per SIMD: the
-// register budget per lane is 512 / W (W = 2 trades a few spills for twice
-// the waves that hide memory latency).
-#define DTR_BATCHED_KERNEL __global__ void __launch_bounds__(256) \
-    __attribute__((amdgpu_waves_per_eu(W, W)))
-
-template <class P, int W>
-DTR_BATCHED_KERNEL propagate_grid_kernel(
-    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
-    dtr::DevResults out, const uint32_t* __restrict__ order) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= in.n) return;
-    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    dtr::propagate_fast<P>(det, field, cfg, in, out, track, lds_cache(smem));
-}
-
-// One track per lane, navigation searches pooled across the wave.  Every
-// lane of the last wave takes part in the pooled searches, also the lanes
-// past the end of the batch.
-template <class P, int W>
-DTR_BATCHED_KERNEL propagate_coop_kernel(
-    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
-    dtr::DevResults out, const uint32_t* __restrict__ order) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool has = k < in.n;
-    const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
-    dtr::coop_run_grid<P, WaveHip>(det, field, cfg, in, out, has, track, lds_cache(smem), lds_shared(smem));
-}
-
-// Persistent waves over a device work queue with pooled navigation searches:
-// a lane whose track has ended takes the next queued track, so a wave stays
-// full while its tracks have very different step counts.  A wave claims
-// `claim` entries at a time with one atomic on *queue; it ends when the queue
-// is drained and its lanes are done.
-template <class P, int W>
-DTR_BATCHED_KERNEL propagate_coop_persistent_kernel(
-    dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg, dtr::TrackInputs in,
-    dtr::DevResults out, const uint32_t* __restrict__ order, uint32_t* __restrict__ queue,
-    uint32_t claim, uint32_t min_idle) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    dtr::coop_run_persistent<P, WaveHip>(det, field, cfg, in, out, order, queue, claim, min_idle,
-                                         lds_cache(smem), lds_shared(smem));
-}
-
-__global__ void queue_reset_kernel(uint32_t* queue) { *queue = 0u; }
-
-// ----------------------------------------------------------------- direct --
-// Propagation with the direct navigator: one work-item per track.
-template <class P, int W>
-DTR_BATCHED_KERNEL propagate_direct_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
-                                           dtr::DevConfig dcfg, dtr::TrackInputs in, dtr::DevResults out,
-                                           dtr::SeqView sq, const uint32_t* __restrict__ order) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= in.n) return;
-    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    dtr::propagate_direct<P>(det, field, cfg, dcfg, in, out, sq, track, lds_cache(smem));
-}
-
-// ------------------------------------------------------------------ adjoint --
-// Recording forward pass: one work-item per track (work-item k runs track
-// order[k] and owns tape column k), lane navigation with the candidate
-// cache in LDS.
-template <class P, int W>
-DTR_BATCHED_KERNEL adjoint_record_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
-                                         dtr::AdjointIO io, const uint32_t* __restrict__ order) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= io.n) return;
-    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    dtr::adjoint_record<P>(det, field, cfg, io, track, static_cast<uint32_t>(k), lds_cache(smem));
-}
-
-// Backward pass: one work-item per track, in the forward pass's order.
-template <class P, int W>
-DTR_BATCHED_KERNEL adjoint_backward_kernel(dtr::FastDetector det, dtr::DevField field, dtr::DevConfig cfg,
-                                           dtr::AdjointIO io, const uint32_t* __restrict__ order, int exact,
-                                           double* __restrict__ grad) {
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= io.n) return;
-    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    dtr::adjoint_backward<P>(det, field, cfg, io, exact != 0, track, static_cast<uint32_t>(k), grad);
 }
 
 // ---------------------------------------------------------------- schedule --
@@ -356,39 +191,6 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
     return DTR_SUCCESS;
 }
 
-template <class P, int W>
-int launch_fast_w(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
-                const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
-                hipStream_t stream) {
-    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov,
-                              in->bound_params, in->start_surface, in->n};
-    const uint32_t block = L.block_size;
-    const size_t shmem = lds_bytes(block);
-    if (L.schedule == DTR_SCHEDULE_GRID) {
-        const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
-        if (L.navigation == DTR_NAVIGATION_LANE)
-            hipLaunchKernelGGL((propagate_grid_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
-                               field->dev, dc, ti, r, L.order);
-        else
-            hipLaunchKernelGGL((propagate_coop_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
-                               field->dev, dc, ti, r, L.order);
-        return to_status(hipGetLastError());
-    }
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, stream, L.queue);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return to_status(e);
-    // resident waves: CUs x 4 SIMDs x waves_per_simd, never more than the
-    // tracks need
-    const uint64_t waves = static_cast<uint64_t>(det->n_cu) * 4u * L.waves_per_simd;
-    const uint64_t need = (static_cast<uint64_t>(in->n) + 63u) / 64u;
-    const uint64_t w = waves < need ? waves : need;
-    const uint32_t wpb = block / 64u;
-    const uint32_t grid = static_cast<uint32_t>((w + wpb - 1) / wpb);
-    hipLaunchKernelGGL((propagate_coop_persistent_kernel<P, W>), dim3(grid), dim3(block), shmem, stream,
-                       det->fast, field->dev, dc, ti, r, L.order, L.queue, L.tracks_per_claim, L.min_idle);
-    return to_status(hipGetLastError());
-}
-
 int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
                        const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
     if (!det || !field || !c || !a || !b) return DTR_ERROR_INVALID_ARGUMENT;
@@ -424,31 +226,22 @@ dtr::AdjointIO make_adjoint_io(const dtr_adjoint_config* a, const dtr_adjoint_ba
     return io;
 }
 
-// One work-item per track with per-lane navigation on the grid schedule:
-// use_field_gradient (the Jacobian with position columns) and the extended
-// geometry
-template <class P>
-int launch_lane(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
-                const dtr_track_batch* in, const dtr::DevResults& r, uint32_t block, const uint32_t* order,
-                hipStream_t stream) {
-    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
-                              in->start_surface, in->n};
-    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
-    hipLaunchKernelGGL((propagate_grid_kernel<P, 2>), dim3(grid), dim3(block), lds_bytes(block), stream, det->fast,
-                       field->dev, dc, ti, r, order);
-    return to_status(hipGetLastError());
-}
-
 // register budget: 1 or 2 resident waves per SIMD
 template <class P>
 int launch_fast(const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
                 const dtr_track_batch* in, const dtr::DevResults& r, const dtr_launch_config& L,
                 hipStream_t stream) {
-    if (L.waves_per_simd == 2u) return launch_fast_w<P, 2>(det, field, dc, in, r, L, stream);
-    return launch_fast_w<P, 1>(det, field, dc, in, r, L, stream);
+    if (L.waves_per_simd == 2u) return dtr::launch_batched<P, 2>(det, field, dc, in, r, L, stream);
+    return dtr::launch_batched<P, 1>(det, field, dc, in, r, L, stream);
 }
 
 }  // namespace
+
+namespace dtr {
+size_t lds_bytes(uint32_t block) {
+    return static_cast<size_t>(block) * kCache * 16u + (block / 64u) * sizeof(CoopShared);
+}
+}  // namespace dtr
 
 extern "C" {
 
@@ -629,28 +422,17 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
+    const hipStream_t st = static_cast<hipStream_t>(stream);
     if (c->use_field_gradient) {
         // the Jacobian with position columns runs on the batched engine's
         // FP64 policy (same results contract as the plain kernel)
         if (det->wires)
-            return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
-                det, field, dc, in, r, 64u, nullptr, static_cast<hipStream_t>(stream));
-        return launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr,
-                                                                    static_cast<hipStream_t>(stream));
+            return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
+                det, field, dc, in, r, 64u, nullptr, st);
+        return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr, st);
     }
-    const uint32_t block = 256;
-    const uint32_t grid = (in->n + block - 1) / block;
-    if (det->wires)
-        hipLaunchKernelGGL(propagate_fp64_kernel<true>, dim3(grid), dim3(block), 0,
-                           static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
-                           in->free_params, in->initial_cov, in->bound_params, in->start_surface,
-                           r, in->n);
-    else
-        hipLaunchKernelGGL(propagate_fp64_kernel<false>, dim3(grid), dim3(block), 0,
-                           static_cast<hipStream_t>(stream), det->dev, field->dev, dc,
-                           in->free_params, in->initial_cov, in->bound_params, in->start_surface,
-                           r, in->n);
-    return to_status(hipGetLastError());
+    if (det->wires) return dtr::launch_plain<true>(det, field, dc, in, r, st);
+    return dtr::launch_plain<false>(det, field, dc, in, r, st);
 }
 
 void dtr_default_launch(dtr_launch_config* l) {
@@ -737,21 +519,22 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
         if (c->use_field_gradient) {
             if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
             if (L.precision == DTR_PRECISION_FP64)
-                return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
+                return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
                     det, field, dc, in, r, L.block_size, L.order, st);
-            return launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyMixed>>>(
+            return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyMixed>>>(
                 det, field, dc, in, r, L.block_size, L.order, st);
         }
         if (L.precision == DTR_PRECISION_FP64)
-            return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>(det, field, dc, in, r, L, st);
-        return launch_fast_w<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>(det, field, dc, in, r, L, st);
+            return dtr::launch_batched<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>(det, field, dc, in, r, L, st);
+        return dtr::launch_batched<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>(det, field, dc, in, r, L, st);
     }
     if (c->use_field_gradient) {
         if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
         if (L.precision == DTR_PRECISION_FP64)
-            return launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size, L.order,
-                                                                        st);
-        return launch_lane<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order, st);
+            return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size,
+                                                                             L.order, st);
+        return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size,
+                                                                          L.order, st);
     }
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);
@@ -775,26 +558,18 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
-    const dtr::TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
-                              in->start_surface, in->n};
     const dtr::SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, det->n_surfaces};
     const uint32_t block = L.block_size;
-    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(in->n) + block - 1) / block);
-    const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (det->wires && L.precision == DTR_PRECISION_FP64)
-        hipLaunchKernelGGL((propagate_direct_kernel<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>), dim3(grid),
-                           dim3(block), shmem, st, det->fast, field->dev, dc, dd, ti, r, sq, L.order);
-    else if (det->wires)
-        hipLaunchKernelGGL((propagate_direct_kernel<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>), dim3(grid),
-                           dim3(block), shmem, st, det->fast, field->dev, dc, dd, ti, r, sq, L.order);
-    else if (L.precision == DTR_PRECISION_FP64)
-        hipLaunchKernelGGL((propagate_direct_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), shmem, st,
-                           det->fast, field->dev, dc, dd, ti, r, sq, L.order);
-    else
-        hipLaunchKernelGGL((propagate_direct_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), shmem, st,
-                           det->fast, field->dev, dc, dd, ti, r, sq, L.order);
-    return to_status(hipGetLastError());
+        return dtr::launch_direct<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(det, field, dc, dd, in, r, sq, block,
+                                                                              L.order, st);
+    if (det->wires)
+        return dtr::launch_direct<dtr::WithExtendedGeometry<dtr::PolicyMixed>>(det, field, dc, dd, in, r, sq, block,
+                                                                               L.order, st);
+    if (L.precision == DTR_PRECISION_FP64)
+        return dtr::launch_direct<dtr::PolicyFP64>(det, field, dc, dd, in, r, sq, block, L.order, st);
+    return dtr::launch_direct<dtr::PolicyMixed>(det, field, dc, dd, in, r, sq, block, L.order, st);
 }
 
 void dtr_default_adjoint(dtr_adjoint_config* a) {
@@ -835,26 +610,13 @@ int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const d
     io.grad_transport = grad_transport;
     io.status = status;
     const uint32_t block = a->block_size ? a->block_size : 128u;
-    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
-    const size_t shmem = static_cast<size_t>(block) * dtr::kCache * 16u;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const bool w2 = a->waves_per_simd != 1u;
-    if (a->precision == DTR_PRECISION_FP64) {
-        if (w2)
-            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), shmem, st,
-                               det->fast, field->dev, dc, io, b->order);
-        else
-            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyFP64, 1>), dim3(grid), dim3(block), shmem, st,
-                               det->fast, field->dev, dc, io, b->order);
-    } else {
-        if (w2)
-            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), shmem, st,
-                               det->fast, field->dev, dc, io, b->order);
-        else
-            hipLaunchKernelGGL((adjoint_record_kernel<dtr::PolicyMixed, 1>), dim3(grid), dim3(block), shmem, st,
-                               det->fast, field->dev, dc, io, b->order);
-    }
-    return to_status(hipGetLastError());
+    if (a->precision == DTR_PRECISION_FP64)
+        return w2 ? dtr::launch_adjoint_record<dtr::PolicyFP64, 2>(det, field, dc, io, block, b->order, st)
+                  : dtr::launch_adjoint_record<dtr::PolicyFP64, 1>(det, field, dc, io, block, b->order, st);
+    return w2 ? dtr::launch_adjoint_record<dtr::PolicyMixed, 2>(det, field, dc, io, block, b->order, st)
+              : dtr::launch_adjoint_record<dtr::PolicyMixed, 1>(det, field, dc, io, block, b->order, st);
 }
 
 int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -867,26 +629,14 @@ int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const 
     dc.max_surfaces = 0u;
     const dtr::AdjointIO io = make_adjoint_io(a, b, hits);
     const uint32_t block = a->block_size ? a->block_size : 128u;
-    const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(b->n) + block - 1) / block);
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int exact = a->model == DTR_ADJOINT_EXACT ? 1 : 0;
     const bool w2 = a->waves_per_simd != 1u;
-    if (a->precision == DTR_PRECISION_FP64) {
-        if (w2)
-            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyFP64, 2>), dim3(grid), dim3(block), 0, st,
-                               det->fast, field->dev, dc, io, b->order, exact, grad);
-        else
-            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyFP64, 1>), dim3(grid), dim3(block), 0, st,
-                               det->fast, field->dev, dc, io, b->order, exact, grad);
-    } else {
-        if (w2)
-            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyMixed, 2>), dim3(grid), dim3(block), 0, st,
-                               det->fast, field->dev, dc, io, b->order, exact, grad);
-        else
-            hipLaunchKernelGGL((adjoint_backward_kernel<dtr::PolicyMixed, 1>), dim3(grid), dim3(block), 0, st,
-                               det->fast, field->dev, dc, io, b->order, exact, grad);
-    }
-    return to_status(hipGetLastError());
+    if (a->precision == DTR_PRECISION_FP64)
+        return w2 ? dtr::launch_adjoint_backward<dtr::PolicyFP64, 2>(det, field, dc, io, block, b->order, exact, grad, st)
+                  : dtr::launch_adjoint_backward<dtr::PolicyFP64, 1>(det, field, dc, io, block, b->order, exact, grad, st);
+    return w2 ? dtr::launch_adjoint_backward<dtr::PolicyMixed, 2>(det, field, dc, io, block, b->order, exact, grad, st)
+              : dtr::launch_adjoint_backward<dtr::PolicyMixed, 1>(det, field, dc, io, block, b->order, exact, grad, st);
 }
 
 }  // extern "C"

@@ -1,0 +1,14 @@
+// The adjoint's recording forward and backward kernels, mixed policy.
+#include "kernels.hpp"
+
+namespace dtr {
+#define DTR_ADJ(W)                                                                                                 \
+    template int launch_adjoint_record<PolicyMixed, W>(const dtr_detector*, const dtr_field*, const DevConfig&,     \
+                                                   const AdjointIO&, uint32_t, const uint32_t*, hipStream_t);     \
+    template int launch_adjoint_backward<PolicyMixed, W>(const dtr_detector*, const dtr_field*, const DevConfig&,   \
+                                                     const AdjointIO&, uint32_t, const uint32_t*, int, double*,   \
+                                                     hipStream_t);
+DTR_ADJ(1)
+DTR_ADJ(2)
+#undef DTR_ADJ
+}  // namespace dtr

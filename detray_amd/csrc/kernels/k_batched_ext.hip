@@ -1,0 +1,13 @@
+// The batched kernels on the extended geometry (two waves per SIMD).
+#include "kernels.hpp"
+
+namespace dtr {
+template int launch_batched<WithExtendedGeometry<PolicyFP64>, 2>(const dtr_detector*, const dtr_field*,
+                                                                 const DevConfig&, const dtr_track_batch*,
+                                                                 const DevResults&, const dtr_launch_config&,
+                                                                 hipStream_t);
+template int launch_batched<WithExtendedGeometry<PolicyMixed>, 2>(const dtr_detector*, const dtr_field*,
+                                                                  const DevConfig&, const dtr_track_batch*,
+                                                                  const DevResults&, const dtr_launch_config&,
+                                                                  hipStream_t);
+}  // namespace dtr
