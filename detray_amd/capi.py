@@ -315,6 +315,8 @@ _SIGNATURES = {
     "dtr_sort_workspace_size": (C.c_size_t, [C.POINTER(SortConfig)]),
     "dtr_sort_tracks": (C.c_int, [C.POINTER(SortConfig), C.c_void_p, C.c_uint32, C.c_void_p,
                                   C.c_void_p, C.c_size_t, C.c_void_p]),
+    "dtr_sort_tracks_static": (C.c_int, [C.POINTER(SortConfig), C.c_void_p, C.c_uint32, C.c_void_p,
+                                         C.c_void_p]),
     "dtr_propagate_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(TrackBatch),
                                        C.POINTER(SurfaceSequence), C.POINTER(ResultBatch),
                                        C.POINTER(LaunchConfig), C.c_void_p]),

@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include <vector>
@@ -150,6 +151,64 @@ uint32_t sort_cells(const dtr_sort_config* c) {
 // words of workspace: the cell counts, then the chunk totals of the scan
 size_t sort_words(uint32_t cells) {
     return static_cast<size_t>(cells) + (cells + kScanChunk - 1u) / kScanChunk;
+}
+
+// Workspace of dtr_sort_tracks_static: a file-scope device array holding the
+// cell counts and chunk totals of up to kStaticSortCells cells (4 MiB), one
+// copy per GPU for the life of the process.  In host code `scratch` is only
+// a stand-in for the symbol: its device address is taken once per device
+// with hipGetSymbolAddress and used wherever the host hands the memory to a
+// kernel.
+constexpr uint32_t kStaticSortCells = 1u << 20;
+constexpr size_t kStaticSortWords = static_cast<size_t>(kStaticSortCells) + kStaticSortCells / kScanChunk;
+static __device__ __align__(256) uint32_t scratch[kStaticSortWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+uint32_t* scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (set up on the first
+// call on that device).
+int static_scratch(uint32_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return to_status(e);
+    if (dev < 0 || dev >= kMaxDevices) return DTR_ERROR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return to_status(e);
+        if (!p) return DTR_ERROR_HIP;
+        scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return DTR_SUCCESS;
+}
+
+int sort_config_ok(const dtr_sort_config* c) {
+    return c && c->eta_bins_per_unit != 0u && c->phi_bins != 0u && c->eta_max > 0.f && c->eta_max <= 20.f &&
+           c->eta_bins_per_unit <= 1000u && c->phi_bins <= 100000u;
+}
+
+// the counting sort's kernels on `stream` with `workspace` (sort_words(cells)
+// words): clear, count, two-level exclusive scan, scatter
+int sort_launch(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t* order, uint32_t* workspace,
+                uint32_t cells, hipStream_t st) {
+    uint32_t* count = workspace;
+    uint32_t* total = count + cells;
+    const uint32_t chunks = static_cast<uint32_t>((cells + kScanChunk - 1u) / kScanChunk);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells);
+    hipLaunchKernelGGL(sort_count_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count,
+                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
+    hipLaunchKernelGGL(sort_scan_chunk_kernel, dim3(chunks), dim3(1024), 0, st, count, cells, total);
+    if (chunks > 1u) {
+        hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, total, chunks);
+        hipLaunchKernelGGL(sort_scan_add_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells, total);
+    }
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count, order,
+                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
+    return to_status(hipGetLastError());
 }
 
 dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
@@ -462,29 +521,25 @@ size_t dtr_sort_workspace_size(const dtr_sort_config* c) {
 
 int dtr_sort_tracks(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t* order, void* workspace,
                     size_t workspace_bytes, void* stream) {
-    if (!c || c->eta_bins_per_unit == 0u || c->phi_bins == 0u || !(c->eta_max > 0.f) || c->eta_max > 20.f ||
-        c->eta_bins_per_unit > 1000u || c->phi_bins > 100000u)
-        return DTR_ERROR_INVALID_ARGUMENT;
+    if (!sort_config_ok(c)) return DTR_ERROR_INVALID_ARGUMENT;
     if (n == 0u) return DTR_SUCCESS;
     if (!fp || !order || !workspace) return DTR_ERROR_INVALID_ARGUMENT;
     const uint32_t cells = sort_cells(c);
     if (cells == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (workspace_bytes < sort_words(cells) * sizeof(uint32_t)) return DTR_ERROR_INVALID_ARGUMENT;
-    uint32_t* count = static_cast<uint32_t*>(workspace);
-    uint32_t* total = count + cells;
-    const uint32_t chunks = static_cast<uint32_t>((cells + kScanChunk - 1u) / kScanChunk);
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(sort_clear_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells);
-    hipLaunchKernelGGL(sort_count_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count,
-                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
-    hipLaunchKernelGGL(sort_scan_chunk_kernel, dim3(chunks), dim3(1024), 0, st, count, cells, total);
-    if (chunks > 1u) {
-        hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, st, total, chunks);
-        hipLaunchKernelGGL(sort_scan_add_kernel, dim3((cells + 255u) / 256u), dim3(256), 0, st, count, cells, total);
-    }
-    hipLaunchKernelGGL(sort_scatter_kernel, dim3((n + 255u) / 256u), dim3(256), 0, st, fp, n, count, order,
-                       c->eta_bins_per_unit, c->phi_bins, c->eta_max);
-    return to_status(hipGetLastError());
+    return sort_launch(c, fp, n, order, static_cast<uint32_t*>(workspace), cells, static_cast<hipStream_t>(stream));
+}
+
+int dtr_sort_tracks_static(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t* order, void* stream) {
+    if (!sort_config_ok(c)) return DTR_ERROR_INVALID_ARGUMENT;
+    const uint32_t cells = sort_cells(c);
+    if (cells == 0u || cells > kStaticSortCells) return DTR_ERROR_INVALID_ARGUMENT;
+    if (n == 0u) return DTR_SUCCESS;
+    if (!fp || !order) return DTR_ERROR_INVALID_ARGUMENT;
+    uint32_t* ws = nullptr;
+    const int rc = static_scratch(&ws);
+    if (rc != DTR_SUCCESS) return rc;
+    return sort_launch(c, fp, n, order, ws, cells, static_cast<hipStream_t>(stream));
 }
 
 int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,

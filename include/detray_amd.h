@@ -388,6 +388,18 @@ DTR_API int dtr_sort_tracks(const dtr_sort_config* cfg, const double* free_param
                             uint32_t* order, void* workspace, size_t workspace_bytes,
                             void* stream);
 
+/* dtr_sort_tracks without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a library-owned
+ * device array of at most 2^20 cells (4 MiB, held on every GPU the library
+ * runs on until the process ends); configurations with more cells return
+ * DTR_ERROR_INVALID_ARGUMENT.  All calls on one GPU share that array (its
+ * counts are cleared by each call's first kernel): calls that overlap on
+ * one GPU, on different streams, race.  Same results as dtr_sort_tracks. */
+DTR_API int dtr_sort_tracks_static(const dtr_sort_config* cfg, const double* free_params, uint32_t n,
+                                   uint32_t* order, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Direct navigation: follow a given surface sequence                       */
 /* ------------------------------------------------------------------------ */
