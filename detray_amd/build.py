@@ -41,6 +41,15 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
+# Flags of the kernels' translation units.  Machine-level loop-invariant code
+# motion is off: the propagation loop body holds every phase of propagate()
+# (stepping, navigation, actors), and hoisting the constants and addresses of
+# rarely taken phases (the polynomial coefficients of atan2/log/sincos of the
+# parameter updater, say) out of the loop kept them live in registers through
+# all of it, which spilled the track state to scratch memory (the mixed
+# kernel at two waves per SIMD: 824 -> 400 bytes of scratch per lane).
+KERNEL_FLAGS = os.environ.get("DETRAY_AMD_KERNEL_FLAGS", "-mllvm -disable-machine-licm").split()
+
 
 def _hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
@@ -75,7 +84,8 @@ def build_library(out: Path = LIB) -> Path:
     def compile_one(src: Path) -> Path:
         obj = objdir / (src.stem + ".o")
         lang = [] if src.suffix == ".hip" else ["-x", "hip"]
-        _run([hipcc, *HIP_FLAGS, *inc, "-c", *lang, str(src), "-o", str(obj)])
+        extra = KERNEL_FLAGS if src.name.startswith("k_") else []
+        _run([hipcc, *HIP_FLAGS, *extra, *inc, "-c", *lang, str(src), "-o", str(obj)])
         return obj
 
     with ThreadPoolExecutor(max_workers=_jobs()) as pool:

@@ -31,7 +31,10 @@
 // that the plain FP64 kernel rounds in source order): its contract with the
 // FP64 reference is a tolerance (1e-9 relative for the FP64 policy), and
 // fused operations shorten the FP32 stage and transport arithmetic.
-#pragma clang fp contract(fast)
+// Contraction is per source expression (`on`, not `fast`), so a track
+// rounds the same way in every kernel that instantiates this code, whatever
+// the surrounding code and register allocation.
+#pragma clang fp contract(on)
 
 namespace dtr {
 
@@ -107,12 +110,14 @@ struct PolicyFP64 {
     static constexpr bool kMixed = false;
     static constexpr bool kGrad = false;
     static constexpr bool kExt = false;
+    static constexpr bool kLdsState = false;
 };
 struct PolicyMixed {
     using S = float;
     static constexpr bool kMixed = true;
     static constexpr bool kGrad = false;
     static constexpr bool kExt = false;
+    static constexpr bool kLdsState = false;
 };
 // Detectors with wires, annuli, placed cylinders, material rods, rings with
 // material and grid bins of several surfaces (the extended geometry): the
@@ -127,6 +132,31 @@ struct WithExtendedGeometry : Base {
 template <class Base>
 struct WithFieldGradient : Base {
     static constexpr bool kGrad = true;
+};
+// The reduced transport Jacobian and the covariance (46 values of the stage
+// precision) live in the lane's LDS slots instead of registers: they are
+// read and written once per step (the Jacobian) or per surface (the
+// covariance), while the registers they free keep the stepping and
+// navigation temporaries out of scratch memory.
+template <class Base>
+struct WithLdsState : Base {
+    static constexpr bool kLdsState = true;
+};
+constexpr int kStateSlots = 25 + 21;
+
+// A per-lane array of the track state: in registers, or (kLds) in LDS,
+// element i of this lane at p[i * stride].
+template <typename S, int N, bool kLds>
+struct LaneArray {
+    S v[N];
+    DTR_AI S& operator[](int i) { return v[i]; }
+    DTR_AI const S& operator[](int i) const { return v[i]; }
+};
+template <typename S, int N>
+struct LaneArray<S, N, true> {
+    S* p;
+    int stride;
+    DTR_AI S& operator[](int i) const { return p[i * stride]; }
 };
 
 // Position columns of the transport Jacobian (rows pos 0-2 then dir 0-2),
@@ -326,12 +356,15 @@ DTR_HD T3<S> field_at_s(const DevField& f, V3 p) {
     return {out[0], out[1], out[2]};
 }
 
-// Candidate cache view: slot i of this lane at base[i * stride].
+// Candidate cache view: slot i of this lane at base[i * stride].  `state`:
+// the lane's kStateSlots LDS slots of the track state (WithLdsState), element
+// i at state[i * stride], or null.
 struct CacheView {
     double* path;
     uint32_t* sf;
     uint32_t* ls;  // volume link (bits 0-15) | status (16-23) | surface id (24-31)
     int stride;
+    float* state = nullptr;
     DTR_HD double& p(int i) const { return path[i * stride]; }
     DTR_HD uint32_t& s(int i) const { return sf[i * stride]; }
     DTR_HD uint32_t& l(int i) const { return ls[i * stride]; }
@@ -385,9 +418,9 @@ struct FastTrack {
     double h_stage;  // step size the stages of the last step were evaluated at
     uint32_t n_trials, n_steps;
     // transport Jacobian (reduced) and covariance in the stage precision
-    S J[25];
+    LaneArray<S, 25, P::kLdsState> J;
     PosColumns<S, P::kGrad> Jx;
-    S cov[21];
+    LaneArray<S, 21, P::kLdsState> cov;
     double bound[6];
     uint32_t up_sf;
     uint32_t n_seq;
@@ -1897,6 +1930,13 @@ struct Lane {
     DTR_AI void start_begin(const FastDetector& det, const DevField& field, const DevConfig& cfg,
                             const TrackInputs& in, CacheView c, uint32_t track, float& os) {
         gid = track;
+        if constexpr (P::kLdsState) {
+            static_assert(sizeof(S) == sizeof(float), "LDS track state holds FP32 values");
+            tk.J.p = c.state;
+            tk.J.stride = c.stride;
+            tk.cov.p = c.state + 25 * c.stride;
+            tk.cov.stride = c.stride;
+        }
         for (int i = 0; i < kCache; ++i) {
             c.p(i) = kMax;
             c.s(i) = kInvalid;

@@ -8,6 +8,8 @@
 // surface_sequencer.
 #pragma once
 
+#include <type_traits>
+
 #include "launch.hpp"
 #include "device/coop_nav.hpp"
 #include "device/propagator.hpp"
@@ -29,14 +31,31 @@ __global__ void __launch_bounds__(256) propagate_fp64_kernel(
 // LDS of a block: the candidate cache, slot-major ([8][blockDim] paths, then
 // [8][blockDim] surface indices and [8][blockDim] link|status words), then
 // the exchange slots and FIFO of each wave for the cooperative navigation.
+// With kState the lanes' track-state slots ([kStateSlots][blockDim] floats,
+// WithLdsState policies) follow the cache.
+template <bool kState = false>
 __device__ inline CacheView lds_cache(unsigned char* smem) {
     const int nb = static_cast<int>(blockDim.x);
     double* path = reinterpret_cast<double*>(smem);
     uint32_t* sf = reinterpret_cast<uint32_t*>(path + kCache * nb);
     uint32_t* ls = sf + kCache * nb;
     const int t = static_cast<int>(threadIdx.x);
-    return CacheView{path + t, sf + t, ls + t, nb};
+    CacheView c{path + t, sf + t, ls + t, nb};
+    if constexpr (kState) c.state = reinterpret_cast<float*>(ls + kCache * nb) + t;
+    return c;
 }
+
+// LDS bytes of a one-work-item-per-track block of policy P: the cache and,
+// for WithLdsState policies, the track-state slots
+template <class P>
+inline size_t lane_lds_bytes(uint32_t block) {
+    return static_cast<size_t>(block) * (kCache * 16u + (P::kLdsState ? kStateSlots * 4u : 0u));
+}
+
+// The policy the per-lane grid kernel runs for P: FP32 stage arithmetic
+// keeps its Jacobian and covariance in LDS
+template <class P>
+using LanePolicy = std::conditional_t<(sizeof(typename P::S) == sizeof(float)) && !P::kLdsState, WithLdsState<P>, P>;
 
 __device__ inline CoopShared* lds_shared(unsigned char* smem) {
     CoopShared* all = reinterpret_cast<CoopShared*>(smem + static_cast<size_t>(blockDim.x) * kCache * 16u);
@@ -73,7 +92,7 @@ DTR_BATCHED_KERNEL propagate_grid_kernel(FastDetector det, DevField field, DevCo
     const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= in.n) return;
     const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    propagate_fast<P>(det, field, cfg, in, out, track, lds_cache(smem));
+    propagate_fast<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem));
 }
 
 // One track per lane, navigation searches pooled across the wave.  Every
@@ -167,8 +186,9 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
     if (L.schedule == DTR_SCHEDULE_GRID) {
         const uint32_t grid = grid_of(in->n, block);
         if (L.navigation == DTR_NAVIGATION_LANE)
-            hipLaunchKernelGGL((propagate_grid_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
-                               field->dev, dc, ti, r, L.order);
+            hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, W>), dim3(grid), dim3(block),
+                               lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, ti, r,
+                               L.order);
         else
             hipLaunchKernelGGL((propagate_coop_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
                                field->dev, dc, ti, r, L.order);
@@ -192,8 +212,9 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
 template <class P>
 int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
                 const DevResults& r, uint32_t block, const uint32_t* order, hipStream_t stream) {
-    hipLaunchKernelGGL((propagate_grid_kernel<P, 2>), dim3(grid_of(in->n, block)), dim3(block), lds_bytes(block),
-                       stream, det->fast, field->dev, dc, track_inputs(in), r, order);
+    hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
+                       lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in), r,
+                       order);
     return to_status(hipGetLastError());
 }
 
