@@ -177,7 +177,7 @@ class ResultBatch(C.Structure):
 # dtr_precision, dtr_schedule
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
-NAVIGATION_WAVE, NAVIGATION_LANE = 0, 1
+NAVIGATION_WAVE, NAVIGATION_LANE, NAVIGATION_GATHER = 0, 1, 2
 
 
 class LaunchConfig(C.Structure):
@@ -189,7 +189,7 @@ class LaunchConfig(C.Structure):
         ("tracks_per_claim", C.c_uint32),
         ("min_idle", C.c_uint32),
         ("navigation", C.c_uint32),
-        ("pad", C.c_uint32),
+        ("xcd_chunk", C.c_uint32),
         ("order", C.c_void_p),
         ("queue", C.c_void_p),
     ]

@@ -487,8 +487,8 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
         // FP64 policy (same results contract as the plain kernel)
         if (det->wires)
             return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
-                det, field, dc, in, r, 64u, nullptr, st);
-        return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr, st);
+                det, field, dc, in, r, 64u, nullptr, 0, st);
+        return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, 64u, nullptr, 0, st);
     }
     if (det->wires) return dtr::launch_plain<true>(det, field, dc, in, r, st);
     return dtr::launch_plain<false>(det, field, dc, in, r, st);
@@ -504,6 +504,7 @@ void dtr_default_launch(dtr_launch_config* l) {
     l->tracks_per_claim = 64u;
     l->min_idle = 1u;
     l->navigation = DTR_NAVIGATION_WAVE;
+    l->xcd_chunk = 64u;
 }
 
 void dtr_default_sort(dtr_sort_config* c) {
@@ -553,7 +554,7 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (L.waves_per_simd == 0u) L.waves_per_simd = 2u;
     if (L.tracks_per_claim == 0u) L.tracks_per_claim = 64u;
     if (L.min_idle == 0u) L.min_idle = 1u;
-    if (L.min_idle > 64u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.min_idle > 64u || L.xcd_chunk > 65536u) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
     // the kernels are built for one or two resident waves per SIMD (512 or
     // 256 registers per lane); a persistent grid is sized with the same value
@@ -561,7 +562,7 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule != DTR_SCHEDULE_GRID && L.schedule != DTR_SCHEDULE_PERSISTENT) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule == DTR_SCHEDULE_PERSISTENT && !L.queue) return DTR_ERROR_INVALID_ARGUMENT;
-    if (L.navigation != DTR_NAVIGATION_WAVE && L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.navigation > DTR_NAVIGATION_GATHER) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule == DTR_SCHEDULE_PERSISTENT && L.navigation != DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
     if (in->n == 0u) return DTR_SUCCESS;
     const dtr::DevConfig dc = make_dev_config(c);
@@ -569,27 +570,28 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int gather = L.navigation == DTR_NAVIGATION_GATHER ? 1 : 0;
     if (det->wires) {
         // the extended geometry is compiled for two waves per SIMD
         if (c->use_field_gradient) {
-            if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
+            if (L.schedule != DTR_SCHEDULE_GRID || L.navigation == DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
             if (L.precision == DTR_PRECISION_FP64)
                 return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyFP64>>>(
-                    det, field, dc, in, r, L.block_size, L.order, st);
+                    det, field, dc, in, r, L.block_size, L.order, gather, st);
             return dtr::launch_lane<dtr::WithFieldGradient<dtr::WithExtendedGeometry<dtr::PolicyMixed>>>(
-                det, field, dc, in, r, L.block_size, L.order, st);
+                det, field, dc, in, r, L.block_size, L.order, gather, st);
         }
         if (L.precision == DTR_PRECISION_FP64)
             return dtr::launch_batched<dtr::WithExtendedGeometry<dtr::PolicyFP64>, 2>(det, field, dc, in, r, L, st);
         return dtr::launch_batched<dtr::WithExtendedGeometry<dtr::PolicyMixed>, 2>(det, field, dc, in, r, L, st);
     }
     if (c->use_field_gradient) {
-        if (L.schedule != DTR_SCHEDULE_GRID || L.navigation != DTR_NAVIGATION_LANE) return DTR_ERROR_UNSUPPORTED;
+        if (L.schedule != DTR_SCHEDULE_GRID || L.navigation == DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
         if (L.precision == DTR_PRECISION_FP64)
             return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyFP64>>(det, field, dc, in, r, L.block_size,
-                                                                             L.order, st);
+                                                                             L.order, gather, st);
         return dtr::launch_lane<dtr::WithFieldGradient<dtr::PolicyMixed>>(det, field, dc, in, r, L.block_size,
-                                                                          L.order, st);
+                                                                          L.order, gather, st);
     }
     if (L.precision == DTR_PRECISION_FP64) return launch_fast<dtr::PolicyFP64>(det, field, dc, in, r, L, st);
     return launch_fast<dtr::PolicyMixed>(det, field, dc, in, r, L, st);

@@ -85,14 +85,126 @@ struct WaveHip {
 #define DTR_BATCHED_KERNEL __global__ void __launch_bounds__(256) \
     __attribute__((amdgpu_waves_per_eu(W, W)))
 
+// The block of the track order that hardware block `b` of `nb` runs.  The
+// hardware deals blocks to the 8 XCDs in turn; with chunks of G > 1, XCD x
+// runs blocks x G .. x G + G - 1 of each round of 8 G consecutive blocks, so
+// the waves resident on one XCD (and sharing its L2) work on neighbouring
+// tracks of the (sorted) order.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t G) {
+    if (G <= 1u) return b;
+    const uint32_t g = 8u;
+    // chunks of G consecutive blocks per XCD: hardware block b = 8k + x;
+    // the blocks past the last whole round of g chunks keep hardware order
+    const uint32_t full = (nb / (g * G)) * (g * G);
+    if (b >= full) return b;
+    const uint32_t x = b % g, k = b / g;
+    return (k / G) * (g * G) + x * G + (k % G);
+}
+
+// One track per lane with the volume searches (local navigations) of the
+// wave gathered: a lane whose navigation asks for a search waits while the
+// other lanes go on with their own propagate() loops, and once every lane
+// still running waits, they search together.  Between two such points the
+// lanes run their actor halves, navigation updates and steps side by side,
+// and the searches, the most expensive and most divergent part of the loop,
+// run with the whole wave active.  Each lane runs exactly the operations of
+// propagate() in its own order; only the interleaving across lanes changes,
+// so the results are those of the per-lane kernel.
+template <class P>
+__device__ __attribute__((always_inline)) inline void propagate_gathered(
+    const FastDetector& det, const DevField& field, const DevConfig& cfg, const TrackInputs& in,
+    const DevResults& out, bool has_track, uint32_t track, CacheView c) {
+    enum : uint8_t { kStart, kActor, kNav1, kNav2, kDone };
+    Lane<P> lane;
+    float os = 0.f;
+    uint8_t phase = kDone;
+    bool waiting = false;
+    if (has_track) {
+        lane.start_begin(det, field, cfg, in, c, track, os);
+        phase = kStart;
+        waiting = true;  // the navigator's initialisation is a volume search
+    }
+    for (;;) {
+        const uint64_t running = __ballot(phase != kDone);
+        if (running == 0ull) break;
+        if (__ballot(waiting) == running) {
+            // every running lane waits: the searches, then each lane's
+            // navigation cascade goes on (it may ask for another search);
+            // lanes that are done (or have no track) sit this out
+            if (!waiting) continue;
+            lane.apply_local_navigation(
+                local_navigation_ool<P>(det, cfg, lane.tk.pos, lane.tk.dir, lane.tk.ext_tol, lane.tk.volume, os, c));
+            if (phase == kStart) {
+                lane.start_end();
+                waiting = false;
+                phase = kActor;
+                continue;
+            }
+            waiting = lane.nav_resume(det, cfg, c, os);
+            if (waiting) continue;
+            // the navigation update is complete
+            const bool end = lane.nav_end();
+            if (phase == kNav2) {
+                phase = kActor;
+                continue;
+            }
+            if (end) {
+                lane.finish(out, in.n);
+                phase = kDone;
+                continue;
+            }
+            lane.step_half(det, field, cfg, c);
+            phase = kNav2;
+            waiting = lane.nav_begin(det, cfg, c, os);
+            if (!waiting) {
+                lane.nav_end();
+                phase = kActor;
+            }
+            continue;
+        }
+        if (phase == kActor && !waiting) {
+            // actor half, navigation update, step half, navigation update
+            // (Lane::advance), up to the first request for a search
+            if (lane.actor_half(det, field, cfg, in, out, c)) {
+                lane.finish(out, in.n);
+                phase = kDone;
+                continue;
+            }
+            phase = kNav1;
+            waiting = lane.nav_begin(det, cfg, c, os);
+            if (waiting) continue;
+            if (lane.nav_end()) {
+                lane.finish(out, in.n);
+                phase = kDone;
+                continue;
+            }
+            lane.step_half(det, field, cfg, c);
+            phase = kNav2;
+            waiting = lane.nav_begin(det, cfg, c, os);
+            if (waiting) continue;
+            lane.nav_end();
+            phase = kActor;
+        }
+    }
+}
+
 template <class P, int W>
 DTR_BATCHED_KERNEL propagate_grid_kernel(FastDetector det, DevField field, DevConfig cfg, TrackInputs in,
-                                         DevResults out, const uint32_t* __restrict__ order) {
+                                         DevResults out, const uint32_t* __restrict__ order, uint32_t xcd,
+                                         int gather) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= in.n) return;
-    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-    propagate_fast<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem));
+    const uint64_t k = static_cast<uint64_t>(xcd_block(blockIdx.x, gridDim.x, xcd)) * blockDim.x + threadIdx.x;
+    if (!gather) {
+        if (k >= in.n) return;
+        const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+        propagate_fast<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem));
+        return;
+    }
+    // every lane of the wave takes part in the gathering, also the lanes
+    // past the end of the batch (without a track)
+    const bool has = k < in.n;
+    const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
+    propagate_gathered<P>(det, field, cfg, in, out, has, track, lds_cache<P::kLdsState>(smem));
 }
 
 // One track per lane, navigation searches pooled across the wave.  Every
@@ -185,10 +297,10 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
     const size_t shmem = lds_bytes(block);
     if (L.schedule == DTR_SCHEDULE_GRID) {
         const uint32_t grid = grid_of(in->n, block);
-        if (L.navigation == DTR_NAVIGATION_LANE)
+        if (L.navigation == DTR_NAVIGATION_LANE || L.navigation == DTR_NAVIGATION_GATHER)
             hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, W>), dim3(grid), dim3(block),
                                lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, ti, r,
-                               L.order);
+                               L.order, L.xcd_chunk, L.navigation == DTR_NAVIGATION_GATHER ? 1 : 0);
         else
             hipLaunchKernelGGL((propagate_coop_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
                                field->dev, dc, ti, r, L.order);
@@ -211,10 +323,10 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
 
 template <class P>
 int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
-                const DevResults& r, uint32_t block, const uint32_t* order, hipStream_t stream) {
+                const DevResults& r, uint32_t block, const uint32_t* order, int gather, hipStream_t stream) {
     hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
                        lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in), r,
-                       order);
+                       order, 0u, gather);
     return to_status(hipGetLastError());
 }
 

@@ -4,7 +4,7 @@
 namespace dtr {
 #define DTR_LANE(P)                                                                                               \
     template int launch_lane<P>(const dtr_detector*, const dtr_field*, const DevConfig&, const dtr_track_batch*, \
-                                const DevResults&, uint32_t, const uint32_t*, hipStream_t);
+                                const DevResults&, uint32_t, const uint32_t*, int, hipStream_t);
 DTR_LANE(WithFieldGradient<PolicyFP64>)
 DTR_LANE(WithFieldGradient<PolicyMixed>)
 DTR_LANE(WithFieldGradient<WithExtendedGeometry<PolicyFP64>>)

@@ -62,11 +62,11 @@ template <class P, int W>
 int launch_batched(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
                    const DevResults& r, const dtr_launch_config& L, hipStream_t stream);
 
-// One work-item per track with per-lane navigation, two waves per SIMD
-// (use_field_gradient).
+// One work-item per track with per-lane (gather 0) or gathered (1)
+// navigation, two waves per SIMD (use_field_gradient).
 template <class P>
 int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
-                const DevResults& r, uint32_t block, const uint32_t* order, hipStream_t stream);
+                const DevResults& r, uint32_t block, const uint32_t* order, int gather, hipStream_t stream);
 
 // The direct navigator (dtr_propagate_direct).
 template <class P>

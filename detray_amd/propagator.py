@@ -385,7 +385,12 @@ class LaunchConfig:
     int32 device tensor [n], a permutation giving the order in which tracks
     are taken (results stay in track order).  navigation: "wave" (the lanes
     that need a volume search pool their candidates and the wave evaluates
-    them together) or "lane" (each lane on its own, grid schedule only)."""
+    them together), "lane" (each lane on its own, grid schedule only) or
+    "gather" (each lane on its own, and a lane that needs a volume search
+    waits until every running lane of its wave does, so the wave searches
+    together; grid schedule only).  ``xcd_chunk``: consecutive blocks of the
+    order run on one XCD at a time (grid schedule, lane or gather
+    navigation; 0 = hardware order)."""
 
     precision: str = "mixed"
     schedule: str = "persistent"
@@ -395,6 +400,7 @@ class LaunchConfig:
     order: "object" = None
     min_idle: int = 1
     navigation: str = "wave"
+    xcd_chunk: int = 64
 
     def to_c(self, queue_ptr: Optional[int]) -> capi.LaunchConfig:
         c = capi.LaunchConfig()
@@ -404,7 +410,9 @@ class LaunchConfig:
         c.waves_per_simd = int(self.waves_per_simd)
         c.tracks_per_claim = int(self.tracks_per_claim)
         c.min_idle = int(self.min_idle)
-        c.navigation = {"wave": capi.NAVIGATION_WAVE, "lane": capi.NAVIGATION_LANE}[self.navigation]
+        c.navigation = {"wave": capi.NAVIGATION_WAVE, "lane": capi.NAVIGATION_LANE,
+                        "gather": capi.NAVIGATION_GATHER}[self.navigation]
+        c.xcd_chunk = int(self.xcd_chunk)
         c.order = self.order.data_ptr() if self.order is not None else None
         c.queue = queue_ptr
         return c

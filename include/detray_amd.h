@@ -318,7 +318,11 @@ enum dtr_navigation_mode {
      * candidate lists and the whole wave evaluates them */
     DTR_NAVIGATION_WAVE = 0,
     /* every lane searches on its own (grid schedule only) */
-    DTR_NAVIGATION_LANE = 1
+    DTR_NAVIGATION_LANE = 1,
+    /* every lane searches on its own, and the lanes of a wave run their
+     * searches together: a lane that needs one waits until every running
+     * lane of its wave does (grid schedule only) */
+    DTR_NAVIGATION_GATHER = 2
 };
 
 typedef struct dtr_launch_config {
@@ -335,7 +339,13 @@ typedef struct dtr_launch_config {
     uint32_t min_idle;         /* persistent: idle lanes a wave waits for before it
                                   takes new tracks, 1..64 (default 1) */
     uint32_t navigation;       /* dtr_navigation_mode (default WAVE) */
-    uint32_t pad;
+    /* grid schedule, per-lane or gathered navigation: consecutive blocks of
+     * the track order that run on one accelerator complex die (XCD) at a
+     * time.  The hardware deals blocks to the 8 XCDs of an MI355X in turn;
+     * with chunks the waves that share an XCD's L2 work on neighbouring
+     * tracks of the (sorted) batch.  0 or 1: hardware order; at most 65536
+     * (default 64) */
+    uint32_t xcd_chunk;
     /* optional DEVICE permutation [n]: the k-th queue entry / work-item
      * propagates track order[k]; results stay in track order */
     const uint32_t* order;

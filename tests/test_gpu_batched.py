@@ -129,6 +129,9 @@ LAUNCHES = [
     ("mixed", "persistent", 256, 1, 64, 1, "wave"),
     ("mixed", "persistent", 64, 2, 128, 16, "wave"),
     ("mixed", "persistent", 128, 1, 256, 64, "wave"),
+    ("fp64", "grid", 64, 2, 64, 1, "gather"),
+    ("mixed", "grid", 64, 2, 64, 1, "gather"),
+    ("mixed", "grid", 256, 1, 64, 1, "gather"),
 ]
 
 
@@ -182,7 +185,7 @@ def test_order_permutation_gives_identical_results(toy7, solenoid, precision):
     base = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, "grid", navigation="lane"))
     perm = np.random.default_rng(7).permutation(fp.shape[1]).astype(np.int32)
     order = torch.from_numpy(perm).cuda()
-    for schedule, nav in (("grid", "lane"), ("grid", "wave"), ("persistent", "wave")):
+    for schedule, nav in (("grid", "lane"), ("grid", "wave"), ("persistent", "wave"), ("grid", "gather")):
         got = run_launch(toy7, solenoid, cfg, fp, da.LaunchConfig(precision, schedule, order=order,
                                                                    navigation=nav))
         for k in base:
@@ -301,20 +304,22 @@ def test_graph_of_sort_and_persistent_propagation_replays(toy7, solenoid):
         torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("sort_key", ["cells_phi", "cells_phi_fine"])
-def test_headline_launch_against_oracle(toy7, solenoid, sort_key):
+@pytest.mark.parametrize("sort_key,nav", [("cells_phi", "gather"), ("cells_phi_fine", "gather"),
+                                          ("cells_phi", "lane")])
+def test_headline_launch_against_oracle(toy7, solenoid, sort_key, nav):
     """The benchmark's own launch (mixed precision, grid schedule, block 64,
-    two waves per SIMD, per-lane navigation, tracks ordered by the native
-    direction-cell sort) on the first 131072 tracks of the benchmark batch
-    (toy muons, seed 44, solenoid map), against the oracle."""
+    two waves per SIMD, gathered or per-lane navigation, XCD chunks of 64
+    blocks, tracks ordered by the native direction-cell sort) on the first
+    131072 tracks of the benchmark batch (toy muons, seed 44, solenoid
+    map), against the oracle."""
     from detray_amd.schedule import sort_order
 
     cfg = da.PropagationConfig(**TOY_CFG)
     fp = da.generate_tracks("toy", 131072, 44)
     tracks = torch.from_numpy(fp).cuda()
-    launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
+    launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, nav, 64)
     launch.order = sort_order(tracks, sort_key)
-    print(f"headline launch, sort {sort_key}", flush=True)
+    print(f"headline launch, sort {sort_key}, navigation {nav}", flush=True)
     res = da.Propagator(cfg, launch).propagate(toy7, solenoid, tracks)
     torch.cuda.synchronize()
     got = {
