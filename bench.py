@@ -121,6 +121,8 @@ def main() -> int:
     ap.add_argument("--sort", default="eta_phi")
     ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--waves", type=int, default=2)
+    ap.add_argument("--nav", default="gather")
+    ap.add_argument("--xcd", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,7 +158,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     if args.no_autotune:
-        launch = da.LaunchConfig("mixed", "grid", args.block, args.waves, 64, None, 1, "lane")
+        launch = da.LaunchConfig("mixed", "grid", args.block, args.waves, 64, None, 1, args.nav, args.xcd)
         key = args.sort
     else:
         t0 = time.perf_counter()
@@ -164,7 +166,8 @@ def main() -> int:
                                       sample=(262144 if loopers else n), tag=args.workload)
         print(f"[bench] autotune ({time.perf_counter() - t0:.1f} s): {launch.schedule} "
               f"nav={launch.navigation} block={launch.block_size} waves={launch.waves_per_simd} "
-              f"idle={launch.min_idle} sort={key}", file=sys.stderr, flush=True)
+              f"claim={launch.tracks_per_claim} idle={launch.min_idle} xcd={launch.xcd_chunk} sort={key}",
+              file=sys.stderr, flush=True)
     prop = da.Propagator(cfg, launch)
     stream = torch.cuda.current_stream(dev)
 
@@ -311,11 +314,11 @@ def main() -> int:
         "config": {
             "workload": workload,
             "tracks": n,
-            "kernel": f"propagate_{'coop_persistent' if launch.schedule == 'persistent' else ('grid' if launch.navigation == 'lane' else 'coop')}_kernel<mixed>",
+            "kernel": f"propagate_{'coop_persistent' if launch.schedule == 'persistent' else ('coop' if launch.navigation == 'wave' else 'grid')}_kernel<mixed>",
             "launch": {"schedule": launch.schedule, "navigation": launch.navigation,
                        "block": launch.block_size, "waves_per_simd": launch.waves_per_simd,
                        "tracks_per_claim": launch.tracks_per_claim, "min_idle": launch.min_idle,
-                       "sort": key},
+                       "xcd_chunk": launch.xcd_chunk, "sort": key},
             "parallelism": "1 GPU",
         },
         "fp64_plain": {"value": value64, "unit": "tracks/s", "kernel": "propagate_fp64_kernel"},
@@ -378,7 +381,7 @@ def bench_wire_chamber(args) -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, [e[0].elapsed_time(e[1]) for e in evs], out
 
-    launch = da.LaunchConfig("mixed", "grid", args.block, 2, 64, None, 1, "lane")
+    launch = da.LaunchConfig("mixed", "grid", args.block, 2, 64, None, 1, "gather")
     wall, kern_ms, out = timed(da.Propagator(cfg, launch), args.steps)
     wall64, kern64, out64 = timed(da.Propagator(cfg), max(1, args.fp64_steps))
     n_sf = out.n_surfaces.cpu().numpy()

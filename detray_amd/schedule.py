@@ -5,7 +5,8 @@ volumes and grid bins at the same time keep the lanes on the same code path
 and the same detector records.  :func:`sort_order` builds that schedule on
 the device (a permutation of the batch, passed as ``LaunchConfig.order``),
 and :func:`autotune` picks the block size, register budget (waves per SIMD),
-schedule and sort key by timing candidates on a sample of the batch.
+schedule, navigation mode, persistent claim size, XCD chunk and sort key by
+timing candidates on a sample of the batch.
 """
 
 from __future__ import annotations
@@ -109,14 +110,18 @@ def sort_order(tracks, key: str = "cells", eta_bins_per_unit: float = 20.0):
 
 def candidate_launches(precision: str = "mixed") -> Iterable[tuple]:
     """The autotuner's search space: (schedule, navigation, block, waves,
-    claim, min_idle, sort key)."""
-    for key in ("cells", "cells_coarse", "cells_fine", "cells_phi", "cells_phi_fine", "eta_phi", "none"):
+    claim, min_idle, sort key, xcd chunk)."""
+    for key in ("cells", "cells_phi", "cells_phi_fine", "eta_phi", "none"):
         for block in (64, 128, 256):
             for waves in (1, 2):
-                yield ("grid", "lane", block, waves, 64, 1, key)
-        yield ("grid", "wave", 64, 2, 64, 1, key)
-        for idle in (16, 48):
-            yield ("persistent", "wave", 256, 2, 64, idle, key)
+                yield ("grid", "gather", block, waves, 64, 1, key, 64)
+        yield ("grid", "gather", 64, 2, 64, 1, key, 0)
+        yield ("grid", "gather", 64, 2, 64, 1, key, 256)
+        yield ("grid", "lane", 64, 2, 64, 1, key, 64)
+        yield ("grid", "wave", 64, 2, 64, 1, key, 0)
+        for claim in (64, 256):
+            for idle in (16, 48):
+                yield ("persistent", "wave", 256, 2, claim, idle, key, 0)
 
 
 def _cache_path() -> Path:
@@ -152,8 +157,9 @@ def autotune(detector, field, config, tracks, precision: str = "mixed", sample: 
     table = []
     best = None
     for cand in (candidates or candidate_launches(precision)):
-        schedule, nav, block, waves, claim, idle, key = cand
-        launch = LaunchConfig(precision, schedule, block, waves, claim, None, idle, nav)
+        schedule, nav, block, waves, claim, idle, key = cand[:7]
+        xcd = cand[7] if len(cand) > 7 else 64
+        launch = LaunchConfig(precision, schedule, block, waves, claim, None, idle, nav, xcd)
         prop = Propagator(config, launch)
         out = prop.allocate(m, sub.device)
         launch.order = sort_order(sub, key)
@@ -166,7 +172,7 @@ def autotune(detector, field, config, tracks, precision: str = "mixed", sample: 
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         row = {"schedule": schedule, "navigation": nav, "block": block, "waves_per_simd": waves,
-               "tracks_per_claim": claim, "min_idle": idle, "sort": key, "ms": dt * 1e3}
+               "tracks_per_claim": claim, "min_idle": idle, "sort": key, "xcd_chunk": xcd, "ms": dt * 1e3}
         table.append(row)
         if verbose:
             print(f"[autotune] {row}", flush=True)

@@ -44,8 +44,8 @@ def test_batched_kernels(toy7, solenoid, precision):
     cfg = da.PropagationConfig(use_field_gradient=True, **TOY)
     fp = da.generate_tracks("toy", 2000, 45)
     ref = oracle_host.propagate(toy7, solenoid, cfg, fp)
-    for block, wps in ((64, 2), (256, 1)):
-        launch = da.LaunchConfig(precision, "grid", block, wps, 64, None, 1, "lane")
+    for block, wps, nav in ((64, 2, "lane"), (256, 1, "lane"), (64, 2, "gather")):
+        launch = da.LaunchConfig(precision, "grid", block, wps, 64, None, 1, nav)
         check(run_launch(toy7, solenoid, cfg, fp, launch), ref, precision)
 
 

@@ -40,6 +40,8 @@ def test_batched_kernel_matches_oracle(kind, n, precision):
     fp = da.generate_tracks("beam", 4000, 42, sigma_theta=0.1)
     ref = oracle_host.propagate(det, field, cfg, fp, threads=oracle_host.default_threads())
     got = run_launch(det, field, cfg, fp, da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane"))
+    check(got, ref, precision)
+    got = run_launch(det, field, cfg, fp, da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "gather"))
     assert ref["n_surfaces"].mean() > 3
     check(got, ref, precision)
 

@@ -67,7 +67,8 @@ def test_batched_kernel_matches_oracle(chamber, solenoid, precision, field_kind)
     ref = oracle_host.propagate(chamber, field, cfg, fp, threads=oracle_host.default_threads())
     order = torch.from_numpy(np.random.default_rng(3).permutation(fp.shape[1]).astype(np.int32)).cuda()
     for launch in (da.LaunchConfig(precision, "grid", 256, 2, 64, None, 1, "lane"),
-                   da.LaunchConfig(precision, "grid", 64, 1, 64, order, 1, "lane")):
+                   da.LaunchConfig(precision, "grid", 64, 1, 64, order, 1, "lane"),
+                   da.LaunchConfig(precision, "grid", 64, 2, 64, order, 1, "gather")):
         got = run_launch(chamber, field, cfg, fp, launch)
         assert ref["n_surfaces"].mean() > 5
         check(got, ref, precision)
