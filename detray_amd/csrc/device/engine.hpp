@@ -1864,14 +1864,12 @@ struct FastEngine {
     }
 };
 
-// Local navigation out of line (it is needed from four places of the
-// navigation cascade and is the largest function); it takes only the
-// navigation fields by value and returns next | last << 8 | status << 16 |
+// Local navigation of a volume (navigation_functions.hpp:193-246) on the
+// navigation fields only: it returns next | last << 8 | status << 16 |
 // trust << 24, so the caller's track state is never addressed.
 template <class P>
-__host__ __device__ __attribute__((noinline)) uint32_t local_navigation_ool(
-    const FastDetector& det, const DevConfig& cfg, V3 pos, V3 dir, double ext_tol, uint16_t volume,
-    float overstep, CacheView c) {
+DTR_AI uint32_t local_navigation(const FastDetector& det, const DevConfig& cfg, V3 pos, V3 dir, double ext_tol,
+                                 uint16_t volume, float overstep, CacheView c) {
     NavLite nl;
     nl.ext_tol = ext_tol;
     nl.next = 0;
@@ -1888,6 +1886,15 @@ __host__ __device__ __attribute__((noinline)) uint32_t local_navigation_ool(
            (static_cast<uint32_t>(static_cast<uint8_t>(nl.last)) << 8) |
            (static_cast<uint32_t>(static_cast<uint8_t>(nl.status)) << 16) |
            (static_cast<uint32_t>(nl.trust) << 24);
+}
+
+// The same out of line, for the per-lane loop, which needs it from four
+// places of the navigation cascade (it is the largest function).
+template <class P>
+__host__ __device__ __attribute__((noinline)) uint32_t local_navigation_ool(
+    const FastDetector& det, const DevConfig& cfg, V3 pos, V3 dir, double ext_tol, uint16_t volume,
+    float overstep, CacheView c) {
+    return local_navigation<P>(det, cfg, pos, dir, ext_tol, volume, overstep, c);
 }
 
 // Track inputs of one batch (structure of arrays, parameter-major).

@@ -132,8 +132,9 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
             // navigation cascade goes on (it may ask for another search);
             // lanes that are done (or have no track) sit this out
             if (!waiting) continue;
+            // the one place the searches happen: inline
             lane.apply_local_navigation(
-                local_navigation_ool<P>(det, cfg, lane.tk.pos, lane.tk.dir, lane.tk.ext_tol, lane.tk.volume, os, c));
+                local_navigation<P>(det, cfg, lane.tk.pos, lane.tk.dir, lane.tk.ext_tol, lane.tk.volume, os, c));
             if (phase == kStart) {
                 lane.start_end();
                 waiting = false;
@@ -190,18 +191,22 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
 
 template <class P, int W>
 DTR_BATCHED_KERNEL propagate_grid_kernel(FastDetector det, DevField field, DevConfig cfg, TrackInputs in,
-                                         DevResults out, const uint32_t* __restrict__ order, uint32_t xcd,
-                                         int gather) {
+                                         DevResults out, const uint32_t* __restrict__ order, uint32_t xcd) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint64_t k = static_cast<uint64_t>(xcd_block(blockIdx.x, gridDim.x, xcd)) * blockDim.x + threadIdx.x;
-    if (!gather) {
-        if (k >= in.n) return;
-        const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
-        propagate_fast<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem));
-        return;
-    }
-    // every lane of the wave takes part in the gathering, also the lanes
-    // past the end of the batch (without a track)
+    if (k >= in.n) return;
+    const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+    propagate_fast<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem));
+}
+
+// One track per lane, the wave's volume searches gathered.  Every lane of
+// the wave takes part in the gathering, also the lanes past the end of the
+// batch (without a track).
+template <class P, int W>
+DTR_BATCHED_KERNEL propagate_gather_kernel(FastDetector det, DevField field, DevConfig cfg, TrackInputs in,
+                                           DevResults out, const uint32_t* __restrict__ order, uint32_t xcd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(xcd_block(blockIdx.x, gridDim.x, xcd)) * blockDim.x + threadIdx.x;
     const bool has = k < in.n;
     const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
     propagate_gathered<P>(det, field, cfg, in, out, has, track, lds_cache<P::kLdsState>(smem));
@@ -297,10 +302,14 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
     const size_t shmem = lds_bytes(block);
     if (L.schedule == DTR_SCHEDULE_GRID) {
         const uint32_t grid = grid_of(in->n, block);
-        if (L.navigation == DTR_NAVIGATION_LANE || L.navigation == DTR_NAVIGATION_GATHER)
+        if (L.navigation == DTR_NAVIGATION_LANE)
             hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, W>), dim3(grid), dim3(block),
                                lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, ti, r,
-                               L.order, L.xcd_chunk, L.navigation == DTR_NAVIGATION_GATHER ? 1 : 0);
+                               L.order, L.xcd_chunk);
+        else if (L.navigation == DTR_NAVIGATION_GATHER)
+            hipLaunchKernelGGL((propagate_gather_kernel<LanePolicy<P>, W>), dim3(grid), dim3(block),
+                               lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, ti, r,
+                               L.order, L.xcd_chunk);
         else
             hipLaunchKernelGGL((propagate_coop_kernel<P, W>), dim3(grid), dim3(block), shmem, stream, det->fast,
                                field->dev, dc, ti, r, L.order);
@@ -324,9 +333,14 @@ int launch_batched(const dtr_detector* det, const dtr_field* field, const DevCon
 template <class P>
 int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
                 const DevResults& r, uint32_t block, const uint32_t* order, int gather, hipStream_t stream) {
-    hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
-                       lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in), r,
-                       order, 0u, gather);
+    if (gather)
+        hipLaunchKernelGGL((propagate_gather_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
+                           lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in),
+                           r, order, 0u);
+    else
+        hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
+                           lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in),
+                           r, order, 0u);
     return to_status(hipGetLastError());
 }
 
