@@ -143,9 +143,13 @@ struct WithLdsState : Base {
     static constexpr bool kLdsState = true;
 };
 constexpr int kStateSlots = 25 + 21;
+// LDS words per lane of the track state: the lane's values are contiguous
+// (constant offsets from one base address); an odd stride keeps the 64
+// lanes of a wave on distinct banks when they read the same element.
+constexpr int kStateStride = kStateSlots + 1;
 
 // A per-lane array of the track state: in registers, or (kLds) in LDS,
-// element i of this lane at p[i * stride].
+// element i of this lane at p[i].
 template <typename S, int N, bool kLds>
 struct LaneArray {
     S v[N];
@@ -155,8 +159,7 @@ struct LaneArray {
 template <typename S, int N>
 struct LaneArray<S, N, true> {
     S* p;
-    int stride;
-    DTR_AI S& operator[](int i) const { return p[i * stride]; }
+    DTR_AI S& operator[](int i) const { return p[i]; }
 };
 
 // Position columns of the transport Jacobian (rows pos 0-2 then dir 0-2),
@@ -357,8 +360,8 @@ DTR_HD T3<S> field_at_s(const DevField& f, V3 p) {
 }
 
 // Candidate cache view: slot i of this lane at base[i * stride].  `state`:
-// the lane's kStateSlots LDS slots of the track state (WithLdsState), element
-// i at state[i * stride], or null.
+// the lane's kStateSlots LDS words of the track state (WithLdsState),
+// element i at state[i], or null.
 struct CacheView {
     double* path;
     uint32_t* sf;
@@ -1940,9 +1943,7 @@ struct Lane {
         if constexpr (P::kLdsState) {
             static_assert(sizeof(S) == sizeof(float), "LDS track state holds FP32 values");
             tk.J.p = c.state;
-            tk.J.stride = c.stride;
-            tk.cov.p = c.state + 25 * c.stride;
-            tk.cov.stride = c.stride;
+            tk.cov.p = c.state + 25;
         }
         for (int i = 0; i < kCache; ++i) {
             c.p(i) = kMax;

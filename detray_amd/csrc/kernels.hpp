@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) propagate_fp64_kernel(
 // LDS of a block: the candidate cache, slot-major ([8][blockDim] paths, then
 // [8][blockDim] surface indices and [8][blockDim] link|status words), then
 // the exchange slots and FIFO of each wave for the cooperative navigation.
-// With kState the lanes' track-state slots ([kStateSlots][blockDim] floats,
+// With kState the lanes' track states ([blockDim][kStateStride] floats,
 // WithLdsState policies) follow the cache.
 template <bool kState = false>
 __device__ inline CacheView lds_cache(unsigned char* smem) {
@@ -41,7 +41,7 @@ __device__ inline CacheView lds_cache(unsigned char* smem) {
     uint32_t* ls = sf + kCache * nb;
     const int t = static_cast<int>(threadIdx.x);
     CacheView c{path + t, sf + t, ls + t, nb};
-    if constexpr (kState) c.state = reinterpret_cast<float*>(ls + kCache * nb) + t;
+    if constexpr (kState) c.state = reinterpret_cast<float*>(ls + kCache * nb) + t * kStateStride;
     return c;
 }
 
@@ -49,7 +49,7 @@ __device__ inline CacheView lds_cache(unsigned char* smem) {
 // for WithLdsState policies, the track-state slots
 template <class P>
 inline size_t lane_lds_bytes(uint32_t block) {
-    return static_cast<size_t>(block) * (kCache * 16u + (P::kLdsState ? kStateSlots * 4u : 0u));
+    return static_cast<size_t>(block) * (kCache * 16u + (P::kLdsState ? kStateStride * 4u : 0u));
 }
 
 // The policy the per-lane grid kernel runs for P: FP32 stage arithmetic

@@ -446,8 +446,12 @@ struct IPoint {
     V3 glob;
     double z;
 };
+// Ray-surface intersections computed by the navigation of the current track
+// (the N_isect of the algorithmic flop count), per thread.
+thread_local uint32_t tl_isect = 0u;
 inline bool intersect_surface(const Det& det, uint32_t sf_idx, const V3& ro, const V3& rd,
                               double overstep, IPoint& ip) {
+    ++tl_isect;
     const dtr_surface& s = det.sf(sf_idx);
     const dtr_transform& t = det.trf(sf_idx);
     if (s.shape == DTR_CONCENTRIC_CYLINDER2D) {
@@ -586,6 +590,7 @@ struct Engine {
             // intersection_initialize (intersection_kernel.hpp:24-110): every
             // valid solution that a mask accepts is inserted, at most two
             double sol[2];
+            ++tl_isect;
             const int n = cyl2_intersect(st.pos, rd, t, det.mask(s.mask).v[0], sol);
             for (uint32_t k = 0; k < s.n_masks; ++k) {
                 int found = 0;
@@ -707,6 +712,7 @@ struct Engine {
         const dtr_grid& g = det.d->grids[vol.grid];
         const dtr_transform& vt = det.d->transforms[vol.transform];
         double l0, l1;
+        ++tl_isect;  // the grid lookup intersects the grid's reference surface
         if (g.type == DTR_GRID_CYLINDER) {
             double path, z;
             cyl_portal_intersect(st.pos, st.dir, g.radius, -kMax, path, z);
@@ -1740,7 +1746,7 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
                           const dtr_prop_config* cfg, const dtr_track_batch* in,
                           dtr_result_batch* out, oracle_hits* hits, int n_threads,
                           oracle_trace* trace, const uint32_t* seq, const uint32_t* seq_n,
-                          uint32_t seq_max) {
+                          uint32_t seq_max, uint32_t* n_isect = nullptr) {
     using namespace oracle;
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -1837,7 +1843,9 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
             e.tr_sf = trace->surface + static_cast<uint64_t>(i) * trace->capacity;
             e.tr_cap = trace->capacity;
         }
+        tl_isect = 0u;
         e.propagate();
+        if (n_isect) n_isect[i] = tl_isect;
         uint8_t status;
         if (e.nav.status == kExit) status = e.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
         else status = e.nav.exited_path_limit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
@@ -1875,6 +1883,12 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, int n_threads, oracle_trace* trace) {
     return propagate_impl(det, field, cfg, in, out, nullptr, n_threads, trace, nullptr, nullptr, 0u);
+}
+
+int oracle_propagate_counted(const dtr_detector_desc* det, const dtr_field_desc* field,
+                             const dtr_prop_config* cfg, const dtr_track_batch* in,
+                             dtr_result_batch* out, int n_threads, uint32_t* n_isect) {
+    return propagate_impl(det, field, cfg, in, out, nullptr, n_threads, nullptr, nullptr, nullptr, 0u, n_isect);
 }
 
 int oracle_propagate_direct(const dtr_detector_desc* det, const dtr_field_desc* field,

@@ -35,6 +35,13 @@ int oracle_propagate(const dtr_detector_desc* det, const dtr_field_desc* field,
                      const dtr_prop_config* cfg, const dtr_track_batch* in,
                      dtr_result_batch* out, int n_threads, oracle_trace* trace);
 
+/* oracle_propagate that also counts, per track, the ray-surface
+ * intersections the navigation computed (n_isect [n], HOST memory): the
+ * N_isect of the algorithmic flop model. */
+int oracle_propagate_counted(const dtr_detector_desc* det, const dtr_field_desc* field,
+                             const dtr_prop_config* cfg, const dtr_track_batch* in,
+                             dtr_result_batch* out, int n_threads, uint32_t* n_isect);
+
 /* Bound loc0 / loc1 of the updater at every sensitive surface a track
  * crosses (before the material interaction), HOST memory: surface
  * [max_hits][n], loc [2][max_hits][n], n_hits [n]. */

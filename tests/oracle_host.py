@@ -87,6 +87,37 @@ def propagate(detector, field, config, free_params, initial_cov=None,
     return out
 
 
+def propagate_counted(detector, field, config, free_params, threads: int = 0) -> dict:
+    """oracle propagate from free starts that also returns n_isect [n], the
+    ray-surface intersections each track's navigation computed."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    n = fp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+        "n_isect": np.zeros(n, dtype=np.uint32),
+    }
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data, None, None, None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    lib.oracle_propagate_counted.restype = C.c_int
+    cfg = config.to_c()
+    capi.check(lib.oracle_propagate_counted(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(tb),
+                                            C.byref(rb), C.c_int(threads), C.c_void_p(out["n_isect"].ctypes.data)),
+               "oracle_propagate_counted")
+    return out
+
+
 class Hits(C.Structure):
     _fields_ = [("max_hits", C.c_uint32), ("pad", C.c_uint32), ("surface", C.c_void_p), ("loc", C.c_void_p),
                 ("n_hits", C.c_void_p)]
