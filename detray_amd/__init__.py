@@ -4,6 +4,7 @@ behind a C ABI, with a thin Python mirror of the reference interface."""
 from .adjoint import AdjointConfig, Chi2Propagator, Hits, chi2  # noqa: F401
 from .capi import DetrayError, load  # noqa: F401
 from .propagator import (  # noqa: F401
+    Actors,
     GEV,
     MEV,
     MM,
@@ -15,10 +16,11 @@ from .propagator import (  # noqa: F401
     PropagationConfig,
     PropagationResult,
     Propagator,
+    StepTrace,
     generate_tracks,
 )
 
 __all__ = [
     "AdjointConfig", "Chi2Propagator", "Hits", "chi2", "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
-    "PropagationResult", "Propagator", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
+    "PropagationResult", "Propagator", "StepTrace", "Actors", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
 ]

@@ -26,7 +26,7 @@ STATUS_NAMES = {
 }
 
 # dtr_track_status
-TRACK_EXIT, TRACK_ABORT, TRACK_PATH_LIMIT, TRACK_OVERFLOW = 0, 1, 2, 3
+TRACK_EXIT, TRACK_ABORT, TRACK_PATH_LIMIT, TRACK_OVERFLOW, TRACK_MOMENTUM_LIMIT, TRACK_TARGET = 0, 1, 2, 3, 4, 5
 # dtr_mask_shape
 RECTANGLE2D, TRAPEZOID2D, CONCENTRIC_CYLINDER2D, RING2D, LINE_SQUARE, LINE_CIRCULAR = 0, 1, 2, 3, 4, 5
 ANNULUS2D, CYLINDER2D = 6, 7
@@ -147,6 +147,27 @@ class PropConfig(C.Structure):
         ("do_multiple_scattering", C.c_uint8),
         ("use_material", C.c_uint8),
         ("max_surfaces", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("actors", C.c_void_p),
+    ]
+
+
+ACTOR_MOMENTUM_ABORTER, ACTOR_TARGET_ABORTER, ACTOR_STEP_TRACER = 1, 2, 4
+
+
+class StepTrace(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32), ("pad", C.c_uint32),
+        ("n", C.c_void_p), ("surface", C.c_void_p), ("step_size", C.c_void_p),
+        ("path_length", C.c_void_p), ("n_trials", C.c_void_p), ("free_params", C.c_void_p),
+        ("bound_params", C.c_void_p), ("jacobian", C.c_void_p),
+    ]
+
+
+class ActorStates(C.Structure):
+    _fields_ = [
+        ("actors", C.c_uint32), ("min_p", C.c_float), ("min_pT", C.c_float), ("pad", C.c_uint32),
+        ("target_surface", C.c_void_p), ("trace", StepTrace),
     ]
 
 
@@ -308,6 +329,7 @@ _SIGNATURES = {
     "dtr_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig),
                                 C.POINTER(TrackBatch), C.POINTER(ResultBatch), C.c_void_p]),
     "dtr_default_launch": (None, [C.POINTER(LaunchConfig)]),
+    "dtr_default_actors": (None, [C.POINTER(ActorStates)]),
     "dtr_propagate_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig),
                                    C.POINTER(TrackBatch), C.POINTER(ResultBatch),
                                    C.POINTER(LaunchConfig), C.c_void_p]),

@@ -233,7 +233,28 @@ dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
     dc.do_eloss = c->do_energy_loss;
     dc.do_ms = c->do_multiple_scattering;
     dc.use_material = c->use_material;
+    if (const dtr_actor_states* a = c->actors) {
+        dc.actors = a->actors;
+        dc.min_p = static_cast<double>(a->min_p);
+        dc.min_pT = static_cast<double>(a->min_pT);
+        dc.target = a->target_surface;
+        const dtr_step_trace& t = a->trace;
+        dc.trace = dtr::TraceView{t.capacity, t.n, t.surface, t.step_size, t.path_length, t.n_trials,
+                                  t.free_params, t.bound_params, t.jacobian};
+    }
     return dc;
+}
+
+// the optional actors' states are complete for their bits
+int check_actors(const dtr_actor_states* a) {
+    if (!a) return DTR_SUCCESS;
+    constexpr uint32_t kAll = DTR_ACTOR_MOMENTUM_ABORTER | DTR_ACTOR_TARGET_ABORTER | DTR_ACTOR_STEP_TRACER;
+    if (a->actors & ~kAll) return DTR_ERROR_INVALID_ARGUMENT;
+    if ((a->actors & DTR_ACTOR_MOMENTUM_ABORTER) && !(a->min_p == a->min_p && a->min_pT == a->min_pT))
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if ((a->actors & DTR_ACTOR_TARGET_ABORTER) && !a->target_surface) return DTR_ERROR_INVALID_ARGUMENT;
+    if ((a->actors & DTR_ACTOR_STEP_TRACER) && !a->trace.n) return DTR_ERROR_INVALID_ARGUMENT;
+    return DTR_SUCCESS;
 }
 
 int check_propagate_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -247,7 +268,7 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
     // (rk_stepper.hpp:193-200), which these detector tables do not have, so
     // the flag changes nothing
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
-    return DTR_SUCCESS;
+    return check_actors(c->actors);
 }
 
 int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -263,6 +284,9 @@ int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dt
     if (a->waves_per_simd > 2u) return DTR_ERROR_INVALID_ARGUMENT;
     if (!(a->sigma_loc0 > 0.0) || !(a->sigma_loc1 > 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+    // the recording pass has its own actor chain (the tape replaces the
+    // sequencer and the tracer)
+    if (c->actors && c->actors->actors) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -492,6 +516,14 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
     }
     if (det->wires) return dtr::launch_plain<true>(det, field, dc, in, r, st);
     return dtr::launch_plain<false>(det, field, dc, in, r, st);
+}
+
+void dtr_default_actors(dtr_actor_states* a) {
+    if (!a) return;
+    std::memset(a, 0, sizeof(*a));
+    // momentum_aborter::state (aborters.hpp:99-100): 10 MeV
+    a->min_p = 10.f * 1e-3f;
+    a->min_pT = 10.f * 1e-3f;
 }
 
 void dtr_default_launch(dtr_launch_config* l) {

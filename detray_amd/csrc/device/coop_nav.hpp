@@ -239,12 +239,12 @@ DTR_AI void coop_iteration(const FastDetector& det, const DevField& field, const
                            const TrackInputs& in, const DevResults& out, Lane<P>& lane, bool& active,
                            CacheView c, CoopShared* slots) {
     if (active && lane.actor_half(det, field, cfg, in, out, c)) {
-        lane.finish(out, in.n);
+        lane.finish(out, in.n, cfg);
         active = false;
     }
     coop_navigate<P, W>(det, cfg, lane, active, c, slots);
     if (active && lane.nav_end()) {
-        lane.finish(out, in.n);
+        lane.finish(out, in.n, cfg);
         active = false;
     }
     if (active) lane.step_half(det, field, cfg, c);

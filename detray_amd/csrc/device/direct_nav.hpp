@@ -103,7 +103,7 @@ DTR_AI void propagate_direct(const FastDetector& det, const DevField& field, con
         lane.nav_ret = direct_update<P>(det, cfg, dcfg, tk, c, sq, N, track, next_ext, len);
         lane.nav_end();
     }
-    lane.finish(out, N);
+    lane.finish(out, N, cfg);
 }
 
 }  // namespace dtr

@@ -411,6 +411,7 @@ struct FastTrack {
     uint8_t trust;
     uint16_t volume;
     bool path_limit_hit, overflow, stopped, heartbeat;
+    bool momentum_hit, target_hit;  // the optional aborters fired
     // free track parameters (FP64 state)
     V3 pos, dir;
     double time, qop;
@@ -427,6 +428,7 @@ struct FastTrack {
     double bound[6];
     uint32_t up_sf;
     uint32_t n_seq;
+    uint32_t n_trace;  // step_tracer entries
 };
 
 // The navigation fields of a track: what local navigation reads and writes.
@@ -1843,9 +1845,64 @@ struct FastEngine {
         tk.ext_tol = ext > max_tol ? max_tol : ext;
     }
 
+    // step_tracer observer (step_tracer.hpp: step_data): free state on the
+    // surface, bound parameters after the material interactor, transport
+    // Jacobian expanded to the free 8x8 matrix
+    DTR_AI void trace_update(uint32_t dest, const double fp[8], uint64_t n, uint32_t gid) {
+        const TraceView& t = cfg.trace;
+        const uint32_t k = tk.n_trace++;
+        if (k >= t.cap) return;
+        const uint64_t o = static_cast<uint64_t>(k) * n + gid, M = static_cast<uint64_t>(t.cap) * n;
+        if (t.sf) t.sf[o] = dest;
+        if (t.step) t.step[o] = tk.step_size;
+        if (t.path) t.path[o] = tk.path_length;
+        if (t.trials) t.trials[o] = tk.n_trials;
+        if (t.free)
+            for (int i = 0; i < 8; ++i) t.free[i * M + o] = fp[i];
+        if (t.bound)
+            for (int i = 0; i < 6; ++i) t.bound[i * M + o] = tk.bound[i];
+        if (t.jac)
+            for (int r = 0; r < 8; ++r)
+                for (int c = 0; c < 8; ++c) {
+                    double v = free_jacobian(tk.J, r, c);
+                    if constexpr (P::kGrad) {
+                        // position columns carried by the gradient policy
+                        if (c < 3 && r < 3) v = static_cast<double>(tk.Jx.v[r * 3 + c]);
+                        if (c < 3 && r >= 4 && r < 7) v = static_cast<double>(tk.Jx.v[9 + (r - 4) * 3 + c]);
+                    }
+                    t.jac[(r * 8 + c) * M + o] = v;
+                }
+    }
+
+    // momentum_aborter (aborters.hpp:80-143): pT, then p, at or below the limits
+    DTR_AI void momentum_aborter() {
+        if (tk.status == kExit) return;
+        const double q = track_charge(tk.qop);
+        const double pT = fabs(q / tk.qop * perp(tk.dir));
+        if (pT <= cfg.min_pT || q / tk.qop <= cfg.min_p) {
+            if (tk.status != kAbort) tk.momentum_hit = true;
+            tk.status = kAbort;
+            tk.heartbeat = false;
+        }
+    }
+
+    // target_aborter (aborters.hpp:146-175)
+    DTR_AI void target_aborter(uint32_t target) {
+        if (on_surface() && cur_sf() == target && tk.path_length > 0.0) {
+            if (tk.status != kAbort) tk.target_hit = true;
+            tk.status = kAbort;
+            tk.heartbeat = false;
+        }
+    }
+
     DTR_AI void parameter_updater(const double* init_cov, uint64_t n, uint32_t gid) {
         if (!(on_sensitive() || on_material())) return;
         const uint32_t dest = cur_sf();
+        double fp[8];
+        if (cfg.actors & DTR_ACTOR_STEP_TRACER) {
+            fp[0] = tk.pos.x; fp[1] = tk.pos.y; fp[2] = tk.pos.z; fp[3] = tk.time;
+            fp[4] = tk.dir.x; fp[5] = tk.dir.y; fp[6] = tk.dir.z; fp[7] = tk.qop;
+        }
         if (tk.up_sf != kInvalid) {
             if (tk.up_sf != dest) {
                 transport_covariance(dest);
@@ -1864,6 +1921,7 @@ struct FastEngine {
             tk.up_sf = dest;
         }
         if (on_material()) material_interaction(dest);
+        if (cfg.actors & DTR_ACTOR_STEP_TRACER) trace_update(dest, fp, n, gid);
     }
 };
 
@@ -1959,6 +2017,9 @@ struct Lane {
         tk.path_limit_hit = false;
         tk.overflow = false;
         tk.stopped = false;
+        tk.momentum_hit = false;
+        tk.target_hit = false;
+        tk.n_trace = 0;
         tk.step_size = 0.0;
         tk.h_stage = 0.0;
         tk.next_step_size = 0.0;
@@ -2040,6 +2101,8 @@ struct Lane {
                 tk.n_seq++;
             }
         }
+        if (cfg.actors & DTR_ACTOR_MOMENTUM_ABORTER) e.momentum_aborter();
+        if (cfg.actors & DTR_ACTOR_TARGET_ABORTER) e.target_aborter(cfg.target[gid]);
         if (!tk.heartbeat) return true;
         last_path = tk.path_length;
         return false;
@@ -2088,11 +2151,11 @@ struct Lane {
         return !tk.heartbeat;
     }
 
-    DTR_AI void finish(const DevResults& out, uint64_t N) const {
-        uint8_t status;
-        if (tk.status == kExit) status = tk.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
-        else status = tk.path_limit_hit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
-        if (out.status) out.status[gid] = status;
+    DTR_AI void finish(const DevResults& out, uint64_t N, const DevConfig& cfg) const {
+        if (cfg.actors & DTR_ACTOR_STEP_TRACER) cfg.trace.n[gid] = tk.n_trace;
+        if (out.status)
+            out.status[gid] = track_status(tk.status == kExit, tk.overflow, tk.path_limit_hit, tk.momentum_hit,
+                                           tk.target_hit);
         if (out.bound_params)
             for (int k = 0; k < 6; ++k) out.bound_params[k * N + gid] = tk.bound[k];
         if (out.bound_cov)
@@ -2153,7 +2216,7 @@ DTR_AI void propagate_fast(const FastDetector& det, const DevField& field, const
     lane.start(det, field, cfg, in, c, track);
     while (!lane.advance(det, field, cfg, in, out, c)) {
     }
-    lane.finish(out, in.n);
+    lane.finish(out, in.n, cfg);
 }
 
 }  // namespace dtr

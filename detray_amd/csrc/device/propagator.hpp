@@ -50,6 +50,19 @@ struct DevField {
     const float* values;
 };
 
+// step_tracer outputs (dtr_step_trace), entry k of track i at k * n + i
+struct TraceView {
+    uint32_t cap;
+    uint32_t* n;
+    uint32_t* sf;
+    double* step;
+    double* path;
+    uint32_t* trials;
+    double* free;
+    double* bound;
+    double* jac;
+};
+
 // Propagation config as doubles where the reference converts float config
 // values to the scalar type at the point of use.
 struct DevConfig {
@@ -61,6 +74,11 @@ struct DevConfig {
     double n_stddev;
     uint32_t max_surfaces;
     uint8_t do_cov, estimate_noise, do_eloss, do_ms, use_material;
+    // optional actors (dtr_actor bits) and their states
+    uint32_t actors;
+    double min_p, min_pT;
+    const uint32_t* target;
+    TraceView trace;
 };
 
 struct DevResults {
@@ -199,14 +217,38 @@ struct Track {
     double cov[21];
     uint32_t up_sf;
     uint32_t n_seq;
+    uint32_t n_trace;  // step_tracer entries
     bool heartbeat;
     bool stopped;  // energy loss brought the particle to rest
+    bool momentum_hit, target_hit;  // the optional aborters fired
 };
 
 // Reduced Jacobian index helpers: rows 0-2 and 4-6 over columns 4..7, (7,7)
 DTR_HD int jpos(int r, int c) { return r * 4 + (c - 4); }     // r in 0..2
 DTR_HD int jdir(int r, int c) { return 12 + r * 4 + (c - 4); }  // r in 0..2 (rows 4..6)
 constexpr int kJqq = 24;
+
+// dtr_track_status of a finished track: exit (or sequencer overflow), else
+// the first aborter that fired
+DTR_HD uint8_t track_status(bool exit, bool overflow, bool path_limit, bool momentum, bool target) {
+    if (exit) return overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
+    if (path_limit) return DTR_TRACK_PATH_LIMIT;
+    if (momentum) return DTR_TRACK_MOMENTUM_LIMIT;
+    if (target) return DTR_TRACK_TARGET;
+    return DTR_TRACK_ABORT;
+}
+
+// Element (r, c) of the free 8x8 transport Jacobian from its reduced form:
+// identity position and time columns, rows 0-2 and 4-6 of the direction
+// and q/p columns, (7, 7) (transport_jacobian.hpp:324)
+template <typename T>
+DTR_HD double free_jacobian(const T& J, int r, int c) {
+    if (c < 4) return r == c ? 1.0 : 0.0;
+    if (r < 3) return static_cast<double>(J[r * 4 + (c - 4)]);
+    if (r >= 4 && r < 7) return static_cast<double>(J[12 + (r - 4) * 4 + (c - 4)]);
+    if (r == 7) return c == 7 ? static_cast<double>(J[24]) : 0.0;
+    return 0.0;  // time row
+}
 
 DTR_HD int covi(int r, int c) {
     // upper-triangle row-major index of (r, c), r <= c
@@ -1280,9 +1322,52 @@ struct EngineT {
         tk.ext_tol = ext > max_tol ? max_tol : ext;
     }
 
+    // step_tracer observer (step_tracer.hpp: step_data): free state on the
+    // surface, bound parameters after the material interactor, transport
+    // Jacobian expanded to the free 8x8 matrix
+    DTR_HD void trace_update(uint32_t dest, const double fp[8], uint32_t n, uint32_t gid) {
+        const TraceView& t = cfg.trace;
+        const uint32_t k = tk.n_trace++;
+        if (k >= t.cap) return;
+        const uint64_t o = static_cast<uint64_t>(k) * n + gid, M = static_cast<uint64_t>(t.cap) * n;
+        if (t.sf) t.sf[o] = dest;
+        if (t.step) t.step[o] = tk.step_size;
+        if (t.path) t.path[o] = tk.path_length;
+        if (t.trials) t.trials[o] = tk.n_trials;
+        if (t.free)
+            for (int i = 0; i < 8; ++i) t.free[i * M + o] = fp[i];
+        if (t.bound)
+            for (int i = 0; i < 6; ++i) t.bound[i * M + o] = tk.bound[i];
+        if (t.jac)
+            for (int r = 0; r < 8; ++r)
+                for (int c = 0; c < 8; ++c) t.jac[(r * 8 + c) * M + o] = free_jacobian(tk.J, r, c);
+    }
+
+    // momentum_aborter (aborters.hpp:80-143): pT, then p, at or below the limits
+    DTR_HD void momentum_aborter() {
+        if (tk.status == kExit) return;
+        const double q = tk.charge;
+        const double pT = fabs(q / tk.qop * perp(tk.dir));
+        if (pT <= cfg.min_pT || q / tk.qop <= cfg.min_p) {
+            if (tk.status != kAbort) tk.momentum_hit = true;
+            tk.status = kAbort;
+            tk.heartbeat = false;
+        }
+    }
+
+    // target_aborter (aborters.hpp:146-175)
+    DTR_HD void target_aborter(uint32_t target) {
+        if (on_surface() && tk.c[tk.next - 1].sf == target && tk.path_length > 0.0) {
+            if (tk.status != kAbort) tk.target_hit = true;
+            tk.status = kAbort;
+            tk.heartbeat = false;
+        }
+    }
+
     DTR_HD void parameter_updater(const double* init_cov, uint32_t n, uint32_t gid) {
         if (!(on_sensitive() || on_material())) return;
         const uint32_t dest = tk.c[tk.next - 1].sf;
+        const double fp[8] = {tk.pos.x, tk.pos.y, tk.pos.z, tk.time, tk.dir.x, tk.dir.y, tk.dir.z, tk.qop};
         if (tk.up_sf != kInvalid) {
             if (tk.up_sf != dest) {
                 transport_covariance(dest);
@@ -1301,6 +1386,7 @@ struct EngineT {
             tk.up_sf = dest;
         }
         if (on_material()) material_interaction(dest);
+        if (cfg.actors & DTR_ACTOR_STEP_TRACER) trace_update(dest, fp, n, gid);
     }
 };
 using Engine = EngineT<false>;
@@ -1344,6 +1430,9 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     for (int i = 0; i < 21; ++i) tk.cov[i] = 0.0;
     tk.up_sf = kInvalid;
     tk.n_seq = 0;
+    tk.n_trace = 0;
+    tk.momentum_hit = false;
+    tk.target_hit = false;
     tk.stopped = false;
 
     EngineT<kExt> e{det, field, cfg, tk};
@@ -1408,6 +1497,8 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
                     tk.n_seq++;
                 }
             }
+            if (cfg.actors & DTR_ACTOR_MOMENTUM_ABORTER) e.momentum_aborter();
+            if (cfg.actors & DTR_ACTOR_TARGET_ABORTER) e.target_aborter(cfg.target[gid]);
             if (!tk.heartbeat) continue;
             path_length = tk.path_length;
         } else {
@@ -1429,10 +1520,9 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
         tk.heartbeat = tk.heartbeat && (tk.status > kUnknown);
     }
 
-    uint8_t status;
-    if (tk.status == kExit) status = tk.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
-    else status = tk.path_limit_hit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
-    if (out.status) out.status[gid] = status;
+    if (cfg.actors & DTR_ACTOR_STEP_TRACER) cfg.trace.n[gid] = tk.n_trace;
+    if (out.status) out.status[gid] = track_status(tk.status == kExit, tk.overflow, tk.path_limit_hit,
+                                                   tk.momentum_hit, tk.target_hit);
     if (out.bound_params)
         for (int k = 0; k < 6; ++k) out.bound_params[k * N + gid] = tk.bound[k];
     if (out.bound_cov)

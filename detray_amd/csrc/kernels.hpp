@@ -150,7 +150,7 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
                 continue;
             }
             if (end) {
-                lane.finish(out, in.n);
+                lane.finish(out, in.n, cfg);
                 phase = kDone;
                 continue;
             }
@@ -167,7 +167,7 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
             // actor half, navigation update, step half, navigation update
             // (Lane::advance), up to the first request for a search
             if (lane.actor_half(det, field, cfg, in, out, c)) {
-                lane.finish(out, in.n);
+                lane.finish(out, in.n, cfg);
                 phase = kDone;
                 continue;
             }
@@ -175,7 +175,7 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
             waiting = lane.nav_begin(det, cfg, c, os);
             if (waiting) continue;
             if (lane.nav_end()) {
-                lane.finish(out, in.n);
+                lane.finish(out, in.n, cfg);
                 phase = kDone;
                 continue;
             }

@@ -431,6 +431,76 @@ class PropagationResult:
     n_steps: "object"        # [n]
     n_trials: "object"       # [n]
     status: "object"         # [n] uint8
+    trace: "object" = None   # StepTrace of the step_tracer, if it ran
+
+
+@dataclass
+class StepTrace:
+    """step_tracer output (step_tracer.hpp: step_data per parameter update):
+    entry k of track i at [..., k, i]; ``n`` [n] entries per track (may
+    exceed the capacity: later entries are counted, not stored)."""
+
+    n: "object"              # [n] int32
+    surface: "object"        # [cap, n] int32
+    step_size: "object"      # [cap, n] float64
+    path_length: "object"    # [cap, n] float64
+    n_trials: "object"       # [cap, n] int32
+    free_params: "object"    # [8, cap, n] float64, the stepping state on the surface
+    bound_params: "object"   # [6, cap, n] float64, after the material interactor
+    jacobian: "object" = None  # [64, cap, n] float64, free transport Jacobian (row-major 8x8)
+
+
+@dataclass
+class Actors:
+    """The optional actors of the chain and their states
+    (``dtr_actor_states``).  The chain is pathlimit_aborter,
+    parameter_updater<pointwise_material_interactor, step_tracer?>,
+    surface_sequencer, momentum_aborter?, target_aborter?.
+
+    momentum_aborter: stop a track once its pT <= ``min_pT`` or p <=
+    ``min_p`` (aborters.hpp:80-143).  ``target_surface``: per-track target
+    surface indices [n] (device int32 for the GPU, host for the oracle);
+    the target_aborter stops a track on reaching it (aborters.hpp:146-175).
+    ``step_tracer``: entries per track of the step tracer (0: off),
+    ``trace_jacobian`` adds the transport Jacobian to them."""
+
+    momentum_aborter: bool = False
+    min_p: float = 10.0 * MEV
+    min_pT: float = 10.0 * MEV
+    target_surface: "object" = None
+    step_tracer: int = 0
+    trace_jacobian: bool = False
+
+    def bits(self) -> int:
+        return ((capi.ACTOR_MOMENTUM_ABORTER if self.momentum_aborter else 0)
+                | (capi.ACTOR_TARGET_ABORTER if self.target_surface is not None else 0)
+                | (capi.ACTOR_STEP_TRACER if self.step_tracer > 0 else 0))
+
+    def allocate_trace(self, n: int, zeros):
+        """StepTrace arrays for n tracks made by ``zeros(shape, kind)`` (kind
+        "f8" or "i4"), or None without the step tracer."""
+        if self.step_tracer <= 0:
+            return None
+        m = int(self.step_tracer)
+        return StepTrace(zeros((n,), "i4"), zeros((m, n), "i4"), zeros((m, n), "f8"), zeros((m, n), "f8"),
+                         zeros((m, n), "i4"), zeros((8, m, n), "f8"), zeros((6, m, n), "f8"),
+                         zeros((64, m, n), "f8") if self.trace_jacobian else None)
+
+    def to_c(self, trace: "Optional[StepTrace]", ptr) -> capi.ActorStates:
+        """C states over the arrays of ``trace`` (``ptr`` gives an array's
+        address)."""
+        a = capi.ActorStates()
+        capi.load().dtr_default_actors(C.byref(a))
+        a.actors = self.bits()
+        a.min_p = float(self.min_p)
+        a.min_pT = float(self.min_pT)
+        a.target_surface = ptr(self.target_surface) if self.target_surface is not None else None
+        if trace is not None:
+            a.trace = capi.StepTrace(int(self.step_tracer), 0, ptr(trace.n), ptr(trace.surface),
+                                     ptr(trace.step_size), ptr(trace.path_length), ptr(trace.n_trials),
+                                     ptr(trace.free_params), ptr(trace.bound_params),
+                                     ptr(trace.jacobian) if trace.jacobian is not None else None)
+        return a
 
 
 class Propagator:
@@ -470,9 +540,27 @@ class Propagator:
             status=torch.full((n,), 255, dtype=torch.uint8, device=device),
         )
 
+    def _config_with(self, actors: Optional[Actors], n: int, device, out: PropagationResult):
+        """The C config for one call: with optional actors, their C states
+        (kept alive by the caller) and the trace arrays put into ``out``."""
+        import torch
+
+        if actors is None or actors.bits() == 0:
+            return self._c_config, None
+        ts = actors.target_surface
+        if ts is not None and (not isinstance(ts, torch.Tensor) or not ts.is_cuda or ts.dtype != torch.int32
+                               or tuple(ts.shape) != (n,)):
+            raise ValueError("actors.target_surface must be an int32 CUDA tensor [n]")
+        dt = {"f8": torch.float64, "i4": torch.int32}
+        out.trace = actors.allocate_trace(n, lambda shape, k: torch.zeros(shape, dtype=dt[k], device=device))
+        states = actors.to_c(out.trace, lambda t: t.contiguous().data_ptr())
+        cfg = capi.PropConfig.from_buffer_copy(self._c_config)
+        cfg.actors = C.addressof(states)
+        return cfg, states
+
     def propagate(self, detector: Detector, field: MagneticField, tracks=None, initial_cov=None,
                   out: Optional[PropagationResult] = None, stream=None, bound_params=None,
-                  start_surface=None) -> PropagationResult:
+                  start_surface=None, actors: Optional[Actors] = None) -> PropagationResult:
         """Propagate a batch of tracks.
 
         Free start: ``tracks`` is a device float64 tensor [8, n] of free
@@ -480,16 +568,19 @@ class Propagator:
         ``bound_params`` [6, n] float64 and ``start_surface`` [n] int32 give
         bound track parameters on a detector surface
         (propagator::state(bound_track_parameters, ...)).
-        ``initial_cov`` [21, n] is the upper-triangle bound covariance."""
+        ``initial_cov`` [21, n] is the upper-triangle bound covariance.
+        ``actors`` switches on optional actors (momentum and target aborters,
+        step tracer; the tracer's output is ``out.trace``)."""
         import torch
 
         tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
                                                       start_surface)
+        cfg, _states = self._config_with(actors, n, device, out)
         lib = capi.load()
         if stream is None:
             stream = torch.cuda.current_stream(device).cuda_stream
         if self.launch is None:
-            capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(self._c_config),
+            capi.check(lib.dtr_propagate(detector.device(), field.device(), C.byref(cfg),
                                          C.byref(tb), C.byref(rb), C.c_void_p(stream)),
                        "dtr_propagate")
             return out
@@ -498,14 +589,15 @@ class Propagator:
                                   or not order.is_contiguous() or order.device != device):
             raise ValueError("launch.order must be a contiguous int32 CUDA tensor [n] on the tracks' device")
         lc = self.launch.to_c(self._queue(device) if self.launch.schedule == "persistent" else None)
-        capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(self._c_config),
+        capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(cfg),
                                         C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
                    "dtr_propagate_ex")
         return out
 
     def propagate_direct(self, detector: Detector, field: MagneticField, sequence, n_sequence, tracks=None,
                          initial_cov=None, out: Optional[PropagationResult] = None, stream=None,
-                         bound_params=None, start_surface=None) -> PropagationResult:
+                         bound_params=None, start_surface=None,
+                         actors: Optional[Actors] = None) -> PropagationResult:
         """Propagate with the direct navigator (``dtr_propagate_direct``):
         every track follows its surface sequence ``sequence`` [m, n] int32
         (``n_sequence`` [n] entries each, e.g. the ``surfaces`` /
@@ -516,6 +608,7 @@ class Propagator:
 
         tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
                                                       start_surface)
+        cfg, _states = self._config_with(actors, n, device, out)
         if (not isinstance(sequence, torch.Tensor) or sequence.dtype != torch.int32 or sequence.dim() != 2
                 or sequence.shape[1] != n or not sequence.is_cuda):
             raise ValueError("sequence must be an int32 CUDA tensor [m, n]")
@@ -533,7 +626,7 @@ class Propagator:
         if stream is None:
             stream = torch.cuda.current_stream(device).cuda_stream
         lc = launch.to_c(None)
-        capi.check(capi.load().dtr_propagate_direct(detector.device(), field.device(), C.byref(self._c_config),
+        capi.check(capi.load().dtr_propagate_direct(detector.device(), field.device(), C.byref(cfg),
                                                     C.byref(tb), C.byref(seq), C.byref(rb), C.byref(lc),
                                                     C.c_void_p(stream)), "dtr_propagate_direct")
         return out

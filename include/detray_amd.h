@@ -56,6 +56,8 @@ enum dtr_track_status {
     DTR_TRACK_ABORT = 1,       /* navigation aborted (stall, no surfaces) */
     DTR_TRACK_PATH_LIMIT = 2,  /* pathlimit_aborter fired */
     DTR_TRACK_OVERFLOW = 3,    /* surface_sequencer capacity reached */
+    DTR_TRACK_MOMENTUM_LIMIT = 4, /* momentum_aborter fired (p or pT at or below its limit) */
+    DTR_TRACK_TARGET = 5,      /* target_aborter fired (the target surface was reached) */
     DTR_TRACK_NOT_RUN = 255
 };
 
@@ -189,6 +191,60 @@ typedef struct dtr_field_desc {
 } dtr_field_desc;
 
 /* ------------------------------------------------------------------------ */
+/* Optional actors (actor_chain.hpp:53-118, composite_actor.hpp:68-91)      */
+/* ------------------------------------------------------------------------ */
+/* The actor chain of every propagation is, in this order:
+ *   pathlimit_aborter                              (aborters.hpp:23-75)
+ *   parameter_updater<pointwise_material_interactor, step_tracer?>
+ *                                                  (parameter_updater.hpp:
+ *                                                   314-627, composite)
+ *   surface_sequencer                              (surface_sequencer.hpp)
+ *   momentum_aborter?                              (aborters.hpp:80-143)
+ *   target_aborter?                                (aborters.hpp:146-175)
+ * The actors marked ? are switched on by dtr_actor_states.actors; each
+ * keeps its reference semantics (an aborter does nothing once the
+ * navigation has finished; the step tracer observes the parameter
+ * transporter, after the material interactor, before the setter). */
+enum dtr_actor {
+    DTR_ACTOR_MOMENTUM_ABORTER = 1u << 0,
+    DTR_ACTOR_TARGET_ABORTER = 1u << 1,
+    DTR_ACTOR_STEP_TRACER = 1u << 2
+};
+
+/* step_tracer output (tests/include/detray/test/utils/step_tracer.hpp:
+ * step_data): one entry per parameter update (sensitive or material
+ * surface), entry k of track i at k * n + i, DEVICE memory (HOST memory for
+ * the oracle).  Any array but n may be NULL. */
+typedef struct dtr_step_trace {
+    uint32_t capacity;      /* entries per track; later ones are counted only */
+    uint32_t pad;
+    uint32_t* n;            /* [n] entries of each track (may exceed capacity) */
+    uint32_t* surface;      /* [capacity][n] surface index */
+    double* step_size;      /* [capacity][n] stepping.step_size() */
+    double* path_length;    /* [capacity][n] stepping.path_length() */
+    uint32_t* n_trials;     /* [capacity][n] stepping.n_total_trials() */
+    double* free_params;    /* [8][capacity][n] stepping state on the surface */
+    double* bound_params;   /* [6][capacity][n] after the material interactor */
+    double* jacobian;       /* [64][capacity][n] stepping.transport_jacobian(),
+                               free 8x8 row-major (already reset to the identity
+                               after a transport, as in the reference) */
+} dtr_step_trace;
+
+/* States of the optional actors (the actor_states tuple of
+ * propagator::propagate, propagator.hpp:260-264). */
+typedef struct dtr_actor_states {
+    uint32_t actors;         /* dtr_actor bits */
+    float min_p;             /* momentum_aborter::state (10 MeV) */
+    float min_pT;            /* momentum_aborter::state (10 MeV) */
+    uint32_t pad;
+    const uint32_t* target_surface; /* [n] target_aborter::state: surface index per track */
+    dtr_step_trace trace;    /* step_tracer::state */
+} dtr_actor_states;
+
+/* Defaults of dtr_actor_states (no optional actor, the reference's limits). */
+DTR_API void dtr_default_actors(dtr_actor_states* actors);
+
+/* ------------------------------------------------------------------------ */
 /* Propagation configuration (propagation::config, propagation_config.hpp;  */
 /* stepping_config.hpp:27-48, navigation_config.hpp:23-40,                  */
 /* intersection_config.hpp:20-36).  Same field types and defaults.          */
@@ -226,6 +282,10 @@ typedef struct dtr_prop_config {
     uint8_t use_material;          /* 1: run parameter_updater<pointwise_material_interactor> */
     /* surface_sequencer capacity (0 disables recording) */
     uint32_t max_surfaces;
+    uint32_t pad;
+    /* optional actors and their states, NULL for none (the adjoint passes
+     * reject them with DTR_ERROR_UNSUPPORTED) */
+    const dtr_actor_states* actors;
 } dtr_prop_config;
 
 /* ------------------------------------------------------------------------ */

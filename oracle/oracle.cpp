@@ -490,6 +490,9 @@ struct PropCfg {
     // interactor
     bool do_eloss, do_ms, use_material;
     uint32_t max_surfaces;
+    // optional actors (dtr_actor bits) and the momentum_aborter limits
+    uint32_t actors = 0u;
+    double min_p = 0.0, min_pT = 0.0;
 };
 
 struct Stepper {
@@ -551,6 +554,14 @@ struct Engine {
     uint32_t* tr_vol = nullptr;
     uint32_t* tr_sf = nullptr;
     uint32_t tr_cap = 0, tr_n = 0;
+    // optional actors: target_aborter::state, step_tracer outputs of this
+    // track (entry k at [k * st_stride], arrays of several values at
+    // [(v * cap + k) * st_stride])
+    uint32_t target = kInvalid;
+    bool momentum_hit = false, target_hit = false;
+    dtr_step_trace steps{};
+    uint64_t st_stride = 0;
+    uint32_t n_trace = 0;
 
     // ---------------------------------------------------------------- nav --
     bool is_portal(uint32_t s) const { return det.sf(s).sf_id == DTR_PORTAL; }
@@ -1529,11 +1540,12 @@ struct Engine {
     void parameter_updater() {
         if (!(on_sensitive() || encountered_material())) return;
         const uint32_t dest = nav.current().sf;
+        const double fp[8] = {st.pos[0], st.pos[1], st.pos[2], st.time, st.dir[0], st.dir[1], st.dir[2], st.qop};
         if (up.sf != kInvalid) {
             if (up.sf == dest) {
                 // on the initial surface: notify observers without transport
                 record_hit(dest);
-                material_and_setter(dest);
+                material_and_setter(dest, fp);
                 return;
             }
             double Jf[36];
@@ -1569,7 +1581,7 @@ struct Engine {
         free_to_bound(dest, st.pos, st.dir, st.time, st.qop, up.bound);
         up.sf = dest;
         record_hit(dest);
-        material_and_setter(dest);
+        material_and_setter(dest, fp);
     }
 
     // the bound parameters a measurement on this surface would be compared
@@ -1582,7 +1594,7 @@ struct Engine {
         n_hit++;
     }
 
-    void material_and_setter(uint32_t dest) {
+    void material_and_setter(uint32_t dest, const double fp[8]) {
         bool success = false;
         if (encountered_material()) {
             const dtr_material_slab& m = det.d->materials[det.sf(dest).material];
@@ -1632,6 +1644,7 @@ struct Engine {
                 }
             }
         }
+        if (cfg.actors & DTR_ACTOR_STEP_TRACER) trace_update(dest, fp);
         if (!success) return;
         // parameter_setter (parameter_updater.hpp:585-627)
         bound_to_free(up.sf, up.bound, st.pos, st.dir, st.time, st.qop);
@@ -1657,6 +1670,47 @@ struct Engine {
         nav.ext_tol = ext > max_tol ? max_tol : ext;
     }
 
+    // step_tracer (tests/include/detray/test/utils/step_tracer.hpp:
+    // collect_data as observer of the parameter transporter, after the
+    // material interactor and before the setter): fp is the stepping state
+    // on the surface
+    void trace_update(uint32_t dest, const double fp[8]) {
+        const uint32_t k = n_trace++;
+        if (k >= steps.capacity) return;
+        const uint64_t o = k * st_stride, M = static_cast<uint64_t>(steps.capacity) * st_stride;
+        if (steps.surface) steps.surface[o] = dest;
+        if (steps.step_size) steps.step_size[o] = st.step_size;
+        if (steps.path_length) steps.path_length[o] = st.path_length;
+        if (steps.n_trials) steps.n_trials[o] = static_cast<uint32_t>(st.n_trials);
+        if (steps.free_params)
+            for (int v = 0; v < 8; ++v) steps.free_params[v * M + o] = fp[v];
+        if (steps.bound_params)
+            for (int v = 0; v < 6; ++v) steps.bound_params[v * M + o] = up.bound[v];
+        if (steps.jacobian)
+            for (int v = 0; v < 64; ++v) steps.jacobian[v * M + o] = st.J[v];
+    }
+
+    // momentum_aborter (aborters.hpp:80-143)
+    void momentum_aborter() {
+        if (nav.status == kExit) return;
+        const double q = st.ptc.charge;
+        const double pT = std::fabs(q / st.qop * perp(st.dir));
+        if (pT <= cfg.min_pT || q / st.qop <= cfg.min_p) {
+            if (nav.status != kAbort) momentum_hit = true;
+            nav.abort_nav();
+            heartbeat = false;
+        }
+    }
+
+    // target_aborter (aborters.hpp:146-175)
+    void target_aborter() {
+        if (nav.on_surface() && nav.current().sf == target && st.path_length > 0.0) {
+            if (nav.status != kAbort) target_hit = true;
+            nav.abort_nav();
+            heartbeat = false;
+        }
+    }
+
     // surface_sequencer.hpp:78-106
     void sequencer() {
         if (cfg.max_surfaces == 0u) return;
@@ -1671,6 +1725,9 @@ struct Engine {
         n_seq++;
     }
 
+    // the chain pathlimit_aborter, parameter_updater<pointwise_material_
+    // interactor, step_tracer?>, surface_sequencer, momentum_aborter?,
+    // target_aborter?
     void run_actors() {
         pathlimit_aborter();
         if (cfg.use_material) parameter_updater();
@@ -1679,6 +1736,8 @@ struct Engine {
             heartbeat = false;
         }
         sequencer();
+        if (cfg.actors & DTR_ACTOR_MOMENTUM_ABORTER) momentum_aborter();
+        if (cfg.actors & DTR_ACTOR_TARGET_ABORTER) target_aborter();
     }
 
     void trace() {
@@ -1777,6 +1836,14 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
     pc.do_ms = cfg->do_multiple_scattering != 0;
     pc.use_material = cfg->use_material != 0;
     pc.max_surfaces = out->surfaces ? cfg->max_surfaces : 0u;
+    const dtr_actor_states* actors = cfg->actors;
+    if (actors) {
+        if ((actors->actors & DTR_ACTOR_TARGET_ABORTER) && !actors->target_surface) return DTR_ERROR_INVALID_ARGUMENT;
+        if ((actors->actors & DTR_ACTOR_STEP_TRACER) && !actors->trace.n) return DTR_ERROR_INVALID_ARGUMENT;
+        pc.actors = actors->actors;
+        pc.min_p = static_cast<double>(actors->min_p);
+        pc.min_pT = static_cast<double>(actors->min_pT);
+    }
     const int64_t n = in->n;
 #ifdef _OPENMP
     const int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
@@ -1838,6 +1905,15 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
             e.hit_stride = static_cast<uint64_t>(n);
             e.hit_cap = hits->max_hits;
         }
+        if (pc.actors & DTR_ACTOR_TARGET_ABORTER) e.target = actors->target_surface[i];
+        if (pc.actors & DTR_ACTOR_STEP_TRACER) {
+            // this track's column of the step_tracer arrays
+            const dtr_step_trace& t = actors->trace;
+            auto col = [&](auto* p) { return p ? p + i : p; };
+            e.steps = dtr_step_trace{t.capacity, 0u, col(t.n), col(t.surface), col(t.step_size), col(t.path_length),
+                                     col(t.n_trials), col(t.free_params), col(t.bound_params), col(t.jacobian)};
+            e.st_stride = static_cast<uint64_t>(n);
+        }
         if (trace) {
             e.tr_vol = trace->volume + static_cast<uint64_t>(i) * trace->capacity;
             e.tr_sf = trace->surface + static_cast<uint64_t>(i) * trace->capacity;
@@ -1846,9 +1922,13 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
         tl_isect = 0u;
         e.propagate();
         if (n_isect) n_isect[i] = tl_isect;
+        if (pc.actors & DTR_ACTOR_STEP_TRACER) actors->trace.n[i] = e.n_trace;
         uint8_t status;
         if (e.nav.status == kExit) status = e.overflow ? DTR_TRACK_OVERFLOW : DTR_TRACK_EXIT;
-        else status = e.nav.exited_path_limit ? DTR_TRACK_PATH_LIMIT : DTR_TRACK_ABORT;
+        else if (e.nav.exited_path_limit) status = DTR_TRACK_PATH_LIMIT;
+        else if (e.momentum_hit) status = DTR_TRACK_MOMENTUM_LIMIT;
+        else if (e.target_hit) status = DTR_TRACK_TARGET;
+        else status = DTR_TRACK_ABORT;
         if (out->status) out->status[i] = status;
         if (out->bound_params)
             for (int k = 0; k < 6; ++k) out->bound_params[k * n + i] = e.up.bound[k];

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+from dataclasses import replace as _replace
 from pathlib import Path
 
 import numpy as np
@@ -40,11 +41,13 @@ def load():
 
 def propagate(detector, field, config, free_params, initial_cov=None,
               threads: int = 0, trace_capacity: int = 0, bound_params=None,
-              start_surface=None) -> dict:
+              start_surface=None, actors=None) -> dict:
     """Run the oracle on host arrays; returns a dict of numpy arrays.
 
     Free start: free_params [8, n].  Bound start: bound_params [6, n] and
-    start_surface [n] (free_params may be None)."""
+    start_surface [n] (free_params may be None).  ``actors``: optional
+    actors (detray_amd.Actors, target_surface a host array); the step
+    tracer's output is out["steps"] (a StepTrace of numpy arrays)."""
     lib = load()
     fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
     bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
@@ -80,6 +83,14 @@ def propagate(detector, field, config, free_params, initial_cov=None,
         tr = Trace(trace_capacity, out["trace_volume"].ctypes.data,
                    out["trace_surface"].ctypes.data, out["trace_n"].ctypes.data)
     cfg = config.to_c()
+    states = None
+    if actors is not None and actors.bits():
+        if actors.target_surface is not None:
+            actors = _replace(actors, target_surface=np.ascontiguousarray(actors.target_surface, dtype=np.uint32))
+        out["steps"] = actors.allocate_trace(n, lambda shape, k: np.zeros(shape, dtype={"f8": np.float64,
+                                                                                         "i4": np.uint32}[k]))
+        states = actors.to_c(out["steps"], lambda a: a.ctypes.data)
+        cfg.actors = C.addressof(states)
     rc = lib.oracle_propagate(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
                               C.byref(tb), C.byref(rb), C.c_int(threads),
                               C.byref(tr) if tr is not None else None)
