@@ -5,6 +5,9 @@ oracle used by the tests.
 the C ABI, the device kernels and the host-side detector emitters.
 ``build_oracle()`` compiles the FP64 CPU restatement into
 ``oracle/liboracle.so`` with g++ (test infrastructure).
+``build_engine_host()`` compiles the device engine's headers for the host
+(hipcc --offload-host-only) into ``tests/native/libengine_host.so``, which
+CPU tests drive (test infrastructure).
 """
 
 from __future__ import annotations
@@ -22,6 +25,8 @@ INCLUDE = ROOT / "include"
 LIB = PKG / "libdetray_amd.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+ENGINE_HOST_SRC = ROOT / "tests" / "native" / "engine_host.cpp"
+ENGINE_HOST_LIB = ROOT / "tests" / "native" / "libengine_host.so"
 
 ARCH = os.environ.get("DETRAY_AMD_ARCH", "gfx950")
 
@@ -106,9 +111,20 @@ def build_oracle(out: Path = ORACLE_LIB) -> Path:
     return out
 
 
+def build_engine_host(out: Path = ENGINE_HOST_LIB) -> Path:
+    """The engine (device/engine.hpp) compiled for the host, with the same
+    contraction rule as the kernels' FP64 arithmetic."""
+    tmp = out.with_suffix(".so.tmp")
+    _run([_hipcc(), "-x", "hip", "--offload-host-only", "-O2", "-std=c++20", "-fPIC", "-shared",
+          "-ffp-contract=off", f"-I{INCLUDE}", f"-I{CSRC}", str(ENGINE_HOST_SRC), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_library()
     build_oracle()
+    build_engine_host()
 
 
 if __name__ == "__main__":

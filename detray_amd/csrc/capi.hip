@@ -211,40 +211,6 @@ int sort_launch(const dtr_sort_config* c, const double* fp, uint32_t n, uint32_t
     return to_status(hipGetLastError());
 }
 
-dtr::DevConfig make_dev_config(const dtr_prop_config* c) {
-    dtr::DevConfig dc{};
-    dc.min_stepsize = c->min_stepsize;
-    dc.rk_error_tol = c->rk_error_tol;
-    dc.step_constraint = c->step_constraint;
-    dc.path_limit = c->path_limit;
-    dc.max_rk_updates = c->max_rk_updates;
-    dc.min_mask_tol = c->min_mask_tolerance;
-    dc.max_mask_tol = c->max_mask_tolerance;
-    dc.mask_scalor = c->mask_tolerance_scalor;
-    dc.path_tol = c->path_tolerance;
-    dc.overstep_tol = c->overstep_tolerance;
-    dc.win0 = c->search_window[0];
-    dc.win1 = c->search_window[1];
-    dc.accumulated_error = c->accumulated_error;
-    dc.n_stddev = static_cast<double>(c->n_scattering_stddev);
-    dc.max_surfaces = c->max_surfaces;
-    dc.do_cov = c->do_covariance_transport;
-    dc.estimate_noise = c->estimate_scattering_noise;
-    dc.do_eloss = c->do_energy_loss;
-    dc.do_ms = c->do_multiple_scattering;
-    dc.use_material = c->use_material;
-    if (const dtr_actor_states* a = c->actors) {
-        dc.actors = a->actors;
-        dc.min_p = static_cast<double>(a->min_p);
-        dc.min_pT = static_cast<double>(a->min_pT);
-        dc.target = a->target_surface;
-        const dtr_step_trace& t = a->trace;
-        dc.trace = dtr::TraceView{t.capacity, t.n, t.surface, t.step_size, t.path_length, t.n_trials,
-                                  t.free_params, t.bound_params, t.jacobian};
-    }
-    return dc;
-}
-
 // the optional actors' states are complete for their bits
 int check_actors(const dtr_actor_states* a) {
     if (!a) return DTR_SUCCESS;
@@ -501,7 +467,7 @@ int dtr_propagate(const dtr_detector* det, const dtr_field* field, const dtr_pro
                   const dtr_track_batch* in, dtr_result_batch* out, void* stream) {
     const int rc = check_propagate_args(det, field, c, in, out);
     if (rc != DTR_SUCCESS || in->n == 0u) return rc;
-    const dtr::DevConfig dc = make_dev_config(c);
+    const dtr::DevConfig dc = dtr::make_dev_config(c);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
@@ -597,7 +563,7 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
     if (L.navigation > DTR_NAVIGATION_GATHER) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule == DTR_SCHEDULE_PERSISTENT && L.navigation != DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
     if (in->n == 0u) return DTR_SUCCESS;
-    const dtr::DevConfig dc = make_dev_config(c);
+    const dtr::DevConfig dc = dtr::make_dev_config(c);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
@@ -642,7 +608,7 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
     if (L.schedule != DTR_SCHEDULE_GRID || c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
-    const dtr::DevConfig dc = make_dev_config(c);
+    const dtr::DevConfig dc = dtr::make_dev_config(c);
     const dtr::DevConfig dd = dtr::direct_config(dc);
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
@@ -687,7 +653,7 @@ int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const d
     if (rc != DTR_SUCCESS || b->n == 0u) return rc;
     if (record && (record->max_hits == 0u || !record->surface || !record->loc || !record->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
-    dtr::DevConfig dc = make_dev_config(c);
+    dtr::DevConfig dc = dtr::make_dev_config(c);
     dc.max_surfaces = 0u;  // the event list on the tape replaces the surface sequencer
     // chi2 and its gradient do not depend on the covariance: it is only
     // carried when the navigation reads it (noise estimation) or the
@@ -714,7 +680,7 @@ int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const 
     const int rc = check_adjoint_args(det, field, c, a, b, hits);
     if (rc != DTR_SUCCESS || b->n == 0u) return rc;
     if (!grad) return DTR_ERROR_INVALID_ARGUMENT;
-    dtr::DevConfig dc = make_dev_config(c);
+    dtr::DevConfig dc = dtr::make_dev_config(c);
     dc.max_surfaces = 0u;
     const dtr::AdjointIO io = make_adjoint_io(a, b, hits);
     const uint32_t block = a->block_size ? a->block_size : 128u;

@@ -81,6 +81,41 @@ struct DevConfig {
     TraceView trace;
 };
 
+// The device config of a propagation config (host).
+inline DevConfig make_dev_config(const dtr_prop_config* c) {
+    DevConfig dc{};
+    dc.min_stepsize = c->min_stepsize;
+    dc.rk_error_tol = c->rk_error_tol;
+    dc.step_constraint = c->step_constraint;
+    dc.path_limit = c->path_limit;
+    dc.max_rk_updates = c->max_rk_updates;
+    dc.min_mask_tol = c->min_mask_tolerance;
+    dc.max_mask_tol = c->max_mask_tolerance;
+    dc.mask_scalor = c->mask_tolerance_scalor;
+    dc.path_tol = c->path_tolerance;
+    dc.overstep_tol = c->overstep_tolerance;
+    dc.win0 = c->search_window[0];
+    dc.win1 = c->search_window[1];
+    dc.accumulated_error = c->accumulated_error;
+    dc.n_stddev = static_cast<double>(c->n_scattering_stddev);
+    dc.max_surfaces = c->max_surfaces;
+    dc.do_cov = c->do_covariance_transport;
+    dc.estimate_noise = c->estimate_scattering_noise;
+    dc.do_eloss = c->do_energy_loss;
+    dc.do_ms = c->do_multiple_scattering;
+    dc.use_material = c->use_material;
+    if (const dtr_actor_states* a = c->actors) {
+        dc.actors = a->actors;
+        dc.min_p = static_cast<double>(a->min_p);
+        dc.min_pT = static_cast<double>(a->min_pT);
+        dc.target = a->target_surface;
+        const dtr_step_trace& t = a->trace;
+        dc.trace = TraceView{t.capacity, t.n, t.surface, t.step_size, t.path_length, t.n_trials,
+                                  t.free_params, t.bound_params, t.jacobian};
+    }
+    return dc;
+}
+
 struct DevResults {
     double* bound_params;
     double* bound_cov;

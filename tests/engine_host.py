@@ -1,0 +1,77 @@
+"""Loader for the host build of the device engine (test infrastructure).
+
+tests/native/engine_host.cpp compiles detray_amd/csrc/device/engine.hpp,
+the code the batched kernels run, for the CPU (hipcc --offload-host-only),
+so that CPU tests can check it: the FP32 grid prefilter against the exact
+mask check, and the per-lane propagation loop against the oracle."""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from detray_amd import capi
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB = ROOT / "tests" / "native" / "libengine_host.so"
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run __graft_entry__.build()")
+        _lib = C.CDLL(str(LIB))
+        _lib.dtr_test_prefilter.restype = C.c_int
+        _lib.dtr_test_engine_propagate.restype = C.c_int
+    return _lib
+
+
+def prefilter(detector, config, rays, ext_tol: float = 0.0):
+    """rays [n, 6] (origin, unit direction).  Returns (pairs evaluated,
+    pairs the exact check accepts, accepted pairs the prefilter rejects,
+    (ray, surface) of the first such miss)."""
+    lib = load()
+    r = np.ascontiguousarray(rays, dtype=np.float64)
+    out = np.zeros(3, dtype=np.uint64)
+    miss = np.zeros(2, dtype=np.uint32)
+    cfg = config.to_c()
+    capi.check(lib.dtr_test_prefilter(C.byref(detector.desc), C.byref(cfg), C.c_void_p(r.ctypes.data),
+                                      C.c_uint32(r.shape[0]), C.c_double(ext_tol),
+                                      C.c_void_p(out.ctypes.data), C.c_void_p(miss.ctypes.data)),
+               "dtr_test_prefilter")
+    return int(out[0]), int(out[1]), int(out[2]), (int(miss[0]), int(miss[1]))
+
+
+def propagate(detector, field, config, free_params, mixed: bool = False, initial_cov=None) -> dict:
+    """The engine's lane loop (propagate_fast) over host arrays, FP64 or
+    mixed policy; the result arrays of dtr_propagate_ex."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    n = fp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+    }
+    cov = None if initial_cov is None else np.ascontiguousarray(initial_cov, dtype=np.float64)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data, cov.ctypes.data if cov is not None else None, None, None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    cfg = config.to_c()
+    capi.check(lib.dtr_test_engine_propagate(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
+                                             C.byref(tb), C.byref(rb), C.c_int(1 if mixed else 0)),
+               "dtr_test_engine_propagate")
+    return out

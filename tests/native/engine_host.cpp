@@ -1,0 +1,179 @@
+// Host build of the batched engine (detray_amd/csrc/device/engine.hpp, the
+// code the GPU kernels run), for CPU tests.  TEST INFRASTRUCTURE: only
+// tests/ load this library.
+//
+//  * dtr_test_prefilter: the FP32 grid prefilter against the exact
+//    candidate evaluation for rays x every grid-bin record of a detector;
+//    the prefilter may only reject surfaces the exact mask check rejects.
+//  * dtr_test_engine_propagate: the engine's per-lane propagate loop
+//    (propagate_fast) run track by track on the host, with the same tables
+//    and results as dtr_propagate_ex, to compare with the oracle on the CPU.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "device/engine.hpp"
+
+namespace {
+
+using namespace dtr;
+
+// The device-side tables of a detector, built on the host as
+// dtr_detector_create builds them.
+struct HostDetector {
+    std::vector<NavSurface> sf;
+    std::vector<NavRecord> bin_rec, bf_rec;
+    std::vector<float> tolk;
+    FastDetector fast{};
+
+    explicit HostDetector(const dtr_detector_desc& d)
+        : sf(d.n_surfaces), bin_rec(d.n_bins), bf_rec(d.n_bf), tolk(d.n_grids) {
+        for (uint32_t k = 0; k < d.n_surfaces; ++k) sf[k] = make_nav_surface(d, k);
+        make_nav_records(d, sf.data(), bin_rec.data(), bf_rec.data(), tolk.data());
+        fast.sf = sf.data();
+        fast.masks = d.masks;
+        fast.materials = d.materials;
+        fast.volumes = d.volumes;
+        fast.grids = d.grids;
+        fast.transforms = d.transforms;
+        fast.bf = d.bf_surfaces;
+        fast.bins = d.grid_bins;
+        fast.bin_rec = bin_rec.data();
+        fast.bf_rec = bf_rec.data();
+        fast.grid_tolk = tolk.data();
+    }
+};
+
+bool has_ext(const dtr_detector_desc& d) {
+    for (uint32_t k = 0; k < d.n_surfaces; ++k) {
+        const dtr_surface& s = d.surfaces[k];
+        if (s.shape >= DTR_LINE_SQUARE) return true;
+        if (s.shape == DTR_RING2D && (s.sf_id == DTR_SENSITIVE || s.material != 0xffffffffu)) return true;
+    }
+    for (uint32_t k = 0; k < d.n_materials; ++k)
+        if (d.materials[k].kind == DTR_MATERIAL_ROD) return true;
+    for (uint32_t k = 0; k < d.n_grids; ++k)
+        if (d.grids[k].capacity > 1u) return true;
+    return false;
+}
+
+template <class P>
+void prefilter_pairs(const dtr_detector_desc& d, const HostDetector& hd, const DevConfig& cfg, const double* rays,
+                     uint32_t n_rays, double ext_tol, uint64_t out[3], uint32_t first_miss[2]) {
+    const DevField no_field{};
+    double path[kCache];
+    uint32_t sfs[kCache], ls[kCache];
+    const CacheView c{path, sfs, ls, 1};
+    for (uint32_t r = 0; r < n_rays; ++r) {
+        NavLite nl{};
+        nl.ext_tol = ext_tol;
+        nl.pos = {rays[6 * r + 0], rays[6 * r + 1], rays[6 * r + 2]};
+        nl.dir = {rays[6 * r + 3], rays[6 * r + 4], rays[6 * r + 5]};
+        FastEngine<P, NavLite> e{hd.fast, no_field, cfg, nl, c};
+        const float ox = static_cast<float>(nl.pos.x), oy = static_cast<float>(nl.pos.y),
+                    oz = static_cast<float>(nl.pos.z);
+        const float dx = static_cast<float>(nl.dir.x), dy = static_cast<float>(nl.dir.y),
+                    dz = static_cast<float>(nl.dir.z);
+        const float ft = static_cast<float>(e.tol_max());
+        for (uint32_t g = 0; g < d.n_grids; ++g) {
+            const dtr_grid& gr = d.grids[g];
+            const float margin = e.sphere_margin(static_cast<int>(g));
+            const uint32_t nb = gr.n0 * gr.n1 * (gr.capacity ? gr.capacity : 1u);
+            for (uint32_t k = 0; k < nb; ++k) {
+                const NavRecord& rec = hd.bin_rec[gr.bins + k];
+                if (rec.rec.index == kInvalid) continue;
+                out[0]++;
+                Cand cd{};
+                bool exact;
+                if (P::kExt && rec.rec.shape == DTR_CYLINDER2D) {
+                    // both solutions, as the candidate search takes them
+                    nl.next = 0;
+                    nl.last = -1;
+                    for (int i = 0; i < kCache; ++i) path[i] = kMax;
+                    e.candidate_search(rec.rec, cfg.overstep_tol);
+                    exact = nl.last >= 0;
+                } else {
+                    exact = e.evaluate(rec.rec, cfg.overstep_tol, cd);
+                }
+                if (!exact) continue;
+                out[1]++;
+                if (!FastEngine<P, NavLite>::prefilter(rec.box, ox, oy, oz, dx, dy, dz, margin, ft)) {
+                    if (out[2] == 0) {
+                        first_miss[0] = r;
+                        first_miss[1] = rec.rec.index;
+                    }
+                    out[2]++;
+                }
+            }
+        }
+    }
+}
+
+template <class P>
+void propagate_all(const HostDetector& hd, const DevField& field, const DevConfig& cfg, const TrackInputs& in,
+                   const DevResults& out) {
+    double path[kCache];
+    uint32_t sfs[kCache], ls[kCache];
+    const CacheView c{path, sfs, ls, 1};
+    for (uint64_t t = 0; t < in.n; ++t)
+        propagate_fast<P>(hd.fast, field, cfg, in, out, static_cast<uint32_t>(t), c);
+}
+
+}  // namespace
+
+extern "C" {
+
+// rays: [n_rays][6] origin and unit direction.  out = {evaluated pairs,
+// exactly accepted pairs, accepted pairs the prefilter rejected};
+// first_miss = {ray, surface} of the first miss.
+__attribute__((visibility("default"))) int dtr_test_prefilter(const dtr_detector_desc* d,
+                                                              const dtr_prop_config* cfg, const double* rays,
+                                                              uint32_t n_rays, double ext_tol, uint64_t out[3],
+                                                              uint32_t first_miss[2]) {
+    if (!d || !cfg || !rays || !out || !first_miss) return DTR_ERROR_INVALID_ARGUMENT;
+    HostDetector hd(*d);
+    const DevConfig dc = make_dev_config(cfg);
+    out[0] = out[1] = out[2] = 0u;
+    if (has_ext(*d))
+        prefilter_pairs<WithExtendedGeometry<PolicyFP64>>(*d, hd, dc, rays, n_rays, ext_tol, out, first_miss);
+    else
+        prefilter_pairs<PolicyFP64>(*d, hd, dc, rays, n_rays, ext_tol, out, first_miss);
+    return DTR_SUCCESS;
+}
+
+// precision: 0 FP64 policy, 1 mixed.  Host memory throughout.
+__attribute__((visibility("default"))) int dtr_test_engine_propagate(const dtr_detector_desc* d,
+                                                                     const dtr_field_desc* f,
+                                                                     const dtr_prop_config* cfg,
+                                                                     const dtr_track_batch* in,
+                                                                     dtr_result_batch* out, int precision) {
+    if (!d || !f || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cfg->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+    HostDetector hd(*d);
+    DevField field{};
+    field.kind = f->kind;
+    for (int i = 0; i < 3; ++i) {
+        field.b[i] = f->b[i];
+        field.n[i] = f->n[i];
+        field.min[i] = f->min[i];
+        field.pitch[i] = f->pitch[i];
+    }
+    field.values = f->values;
+    const DevConfig dc = make_dev_config(cfg);
+    const TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
+                         in->start_surface, in->n};
+    const DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                       cfg->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                       out->n_trials, out->status};
+    const bool ext = has_ext(*d);
+    if (precision == 0) {
+        if (ext) propagate_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, ti, r);
+        else propagate_all<PolicyFP64>(hd, field, dc, ti, r);
+    } else {
+        if (ext) propagate_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r);
+        else propagate_all<PolicyMixed>(hd, field, dc, ti, r);
+    }
+    return DTR_SUCCESS;
+}
+
+}  // extern "C"
