@@ -18,9 +18,10 @@ from .propagator import (  # noqa: F401
     Propagator,
     StepTrace,
     generate_tracks,
+    material,
 )
 
 __all__ = [
     "AdjointConfig", "Chi2Propagator", "Hits", "chi2", "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
-    "PropagationResult", "Propagator", "StepTrace", "Actors", "generate_tracks", "MM", "UM", "GEV", "MEV", "TESLA",
+    "PropagationResult", "Propagator", "StepTrace", "Actors", "generate_tracks", "material", "MM", "UM", "GEV", "MEV", "TESLA",
 ]

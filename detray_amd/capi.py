@@ -55,7 +55,7 @@ class MaterialSlab(C.Structure):
         (n, C.c_double)
         for n in ("X0", "L0", "A", "Z", "mass_rho", "molar_rho", "mean_exc",
                   "thickness", "thickness_in_X0")
-    ] + [("kind", C.c_uint32), ("pad", C.c_uint32)]
+    ] + [("kind", C.c_uint32), ("has_density", C.c_uint32), ("density", C.c_double * 5)]
 
 
 class Surface(C.Structure):
@@ -91,7 +91,7 @@ class Volume(C.Structure):
         ("bf_begin", C.c_uint32), ("bf_end", C.c_uint32),
         ("sf_begin", C.c_uint32), ("sf_end", C.c_uint32),
         ("grid", C.c_int32),
-        ("pad", C.c_uint32),
+        ("material", C.c_uint32),
     ]
 
 
@@ -304,6 +304,7 @@ class TelescopeConfig(C.Structure):
         ("half_x", C.c_double), ("half_y", C.c_double),
         ("thickness", C.c_double),
         ("envelope", C.c_double),
+        ("volume_material", C.POINTER(MaterialSlab)),
     ]
 
 
@@ -352,6 +353,7 @@ _SIGNATURES = {
     "dtr_build_toy_detector": (C.c_int, [C.POINTER(ToyConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_wire_chamber": (C.c_int, [C.POINTER(WireChamberConfig), C.POINTER(DetectorDesc)]),
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
+    "dtr_predefined_material": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(MaterialSlab)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),
     "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(DetectorDesc),
                                          C.c_char_p, C.c_size_t]),

@@ -230,9 +230,8 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
     if (!in->free_params && !in->bound_params) return DTR_ERROR_INVALID_ARGUMENT;
     if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
     if (c->max_surfaces != 0u && (!out->surfaces || !out->n_surfaces)) return DTR_ERROR_INVALID_ARGUMENT;
-    // use_eloss_gradient: d(dqop/ds)/dqop is zero without volume material
-    // (rk_stepper.hpp:193-200), which these detector tables do not have, so
-    // the flag changes nothing
+    // the field-gradient Jacobian has no energy-loss terms
+    if (det->volume_material && c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
     if (c->max_rk_updates == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     return check_actors(c->actors);
 }
@@ -240,7 +239,9 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
 int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
                        const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
     if (!det || !field || !c || !a || !b) return DTR_ERROR_INVALID_ARGUMENT;
-    if (det->wires) return DTR_ERROR_UNSUPPORTED;
+    // the adjoint's material derivatives follow the approximate density
+    // correction and vacuum volumes
+    if (det->wires || det->density_effect) return DTR_ERROR_UNSUPPORTED;
     if (a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->max_steps > (1u << 20) || a->max_events > 4096u || a->checkpoint_every > a->max_steps)
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -355,8 +356,14 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
             if (sf.shape == DTR_RING2D && (sf.sf_id == DTR_SENSITIVE || sf.material != 0xffffffffu))
                 h->wires = true;
         }
-        for (uint32_t k = 0; k < d->n_materials; ++k)
+        for (uint32_t k = 0; k < d->n_materials; ++k) {
             if (d->materials[k].kind == DTR_MATERIAL_ROD) h->wires = true;
+            if (d->materials[k].has_density) h->density_effect = true;
+        }
+        // the stepper's energy loss in volume material runs on the extended
+        // geometry's kernels
+        for (uint32_t k = 0; k < d->n_volumes; ++k)
+            if (d->volumes[k].material != 0u) h->volume_material = h->wires = true;
         for (uint32_t k = 0; k < d->n_grids; ++k)
             if (d->grids[k].capacity > 1u) h->wires = true;
         if (!rc) {
@@ -569,6 +576,16 @@ int dtr_propagate_ex(const dtr_detector* det, const dtr_field* field, const dtr_
                       out->n_trials, out->status};
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int gather = L.navigation == DTR_NAVIGATION_GATHER ? 1 : 0;
+    if (det->volume_material) {
+        // the energy loss in volume material runs on the per-lane kernels
+        // (lane or gathered navigation)
+        using VF = dtr::WithVolumeMaterial<dtr::WithExtendedGeometry<dtr::PolicyFP64>>;
+        using VM = dtr::WithVolumeMaterial<dtr::WithExtendedGeometry<dtr::PolicyMixed>>;
+        if (L.schedule != DTR_SCHEDULE_GRID || L.navigation == DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
+        if (L.precision == DTR_PRECISION_FP64)
+            return dtr::launch_lane<VF>(det, field, dc, in, r, L.block_size, L.order, gather, st);
+        return dtr::launch_lane<VM>(det, field, dc, in, r, L.block_size, L.order, gather, st);
+    }
     if (det->wires) {
         // the extended geometry is compiled for two waves per SIMD
         if (c->use_field_gradient) {
@@ -616,6 +633,13 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     const dtr::SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, det->n_surfaces};
     const uint32_t block = L.block_size;
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    if (det->volume_material) {
+        using VF = dtr::WithVolumeMaterial<dtr::WithExtendedGeometry<dtr::PolicyFP64>>;
+        using VM = dtr::WithVolumeMaterial<dtr::WithExtendedGeometry<dtr::PolicyMixed>>;
+        if (L.precision == DTR_PRECISION_FP64)
+            return dtr::launch_direct<VF>(det, field, dc, dd, in, r, sq, block, L.order, st);
+        return dtr::launch_direct<VM>(det, field, dc, dd, in, r, sq, block, L.order, st);
+    }
     if (det->wires && L.precision == DTR_PRECISION_FP64)
         return dtr::launch_direct<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(det, field, dc, dd, in, r, sq, block,
                                                                               L.order, st);

@@ -111,6 +111,7 @@ struct PolicyFP64 {
     static constexpr bool kGrad = false;
     static constexpr bool kExt = false;
     static constexpr bool kLdsState = false;
+    static constexpr bool kVolMat = false;
 };
 struct PolicyMixed {
     using S = float;
@@ -118,6 +119,7 @@ struct PolicyMixed {
     static constexpr bool kGrad = false;
     static constexpr bool kExt = false;
     static constexpr bool kLdsState = false;
+    static constexpr bool kVolMat = false;
 };
 // Detectors with wires, annuli, placed cylinders, material rods, rings with
 // material and grid bins of several surfaces (the extended geometry): the
@@ -132,6 +134,14 @@ struct WithExtendedGeometry : Base {
 template <class Base>
 struct WithFieldGradient : Base {
     static constexpr bool kGrad = true;
+};
+// Detectors whose volumes hold homogeneous material: the stepper follows the
+// continuous energy loss (stage q/p, d(qop)/ds in the path correction and,
+// with use_eloss_gradient, in the transport Jacobian).  Used with the
+// extended geometry only.
+template <class Base>
+struct WithVolumeMaterial : Base {
+    static constexpr bool kVolMat = true;
 };
 // The reduced transport Jacobian and the covariance (46 values of the stage
 // precision) live in the lane's LDS slots instead of registers: they are
@@ -419,6 +429,7 @@ struct FastTrack {
     double step_size, next_step_size, path_length, abs_path_length;
     double constraint_abort;
     T3<S> dtds3;
+    double dqopds3;  // d(qop)/ds of the last step's fourth stage (volume material)
     double h_stage;  // step size the stages of the last step were evaluated at
     uint32_t n_trials, n_steps;
     // transport Jacobian (reduced) and covariance in the stage precision
@@ -1062,8 +1073,16 @@ struct FastEngine {
     }
 
     // ----------------------------------------------------------- stepping --
+    // The current volume's homogeneous material (WithVolumeMaterial only)
+    DTR_AI const dtr_material_slab* vol_mat() const {
+        if constexpr (P::kVolMat) return volume_material(det.volumes, det.materials, tk.volume);
+        return nullptr;
+    }
+
     // rk_stepper.hpp:273-530: stages and field in S, free state, path length
-    // and the error estimate in FP64
+    // and the error estimate in FP64.  In volume material (WithVolumeMaterial)
+    // the stage q/p follow the energy loss, evaluated in FP64
+    // (evaluate_dqopds, rk_stepper.hpp:534-565).
     DTR_AI void step(double dist, bool do_reset) {
         if (fabs(dist) < static_cast<double>(1.f * 1e-3f)) {
             if (fabs(tk.next_step_size) < cfg.min_stepsize)
@@ -1085,6 +1104,13 @@ struct FastEngine {
         const T3<S> k1 = tmul(qop, tcross(dir, b1));
         const V3 k1d = tdbl(k1);
         T3<S> b2, b3, t2, t3, t4, k2, k3, k4;
+        // stage q/p (FP64 and S) and d(qop)/ds in volume material
+        [[maybe_unused]] const dtr_material_slab* vm = vol_mat();
+        [[maybe_unused]] double qd[4] = {tk.qop, tk.qop, tk.qop, tk.qop}, dq[4] = {0.0, 0.0, 0.0, 0.0};
+        S qs[4] = {qop, qop, qop, qop};
+        if constexpr (P::kVolMat) {
+            if (vm) dq[0] = eloss_dqopds(vm, tk.qop, kMuonMass, track_charge(tk.qop));
+        }
         double error = 1e20;
         for (uint32_t i = 0; i < cfg.max_rk_updates; ++i) {
             tk.n_trials++;
@@ -1095,14 +1121,26 @@ struct FastEngine {
             const V3 pos1 = add(add(pos, mul(half_h, tk.dir)), mul(h2 * 0.125, k1d));
             b2 = field_at_s<S>(field, pos1);
             const S hh = static_cast<S>(half_h), hs = static_cast<S>(h);
+            if constexpr (P::kVolMat) {
+                if (vm) {
+                    const double q = track_charge(tk.qop);
+                    if (cfg.use_mean_loss) qd[1] = tk.qop + half_h * dq[0];
+                    dq[1] = eloss_dqopds(vm, qd[1], kMuonMass, q);
+                    if (cfg.use_mean_loss) qd[2] = tk.qop + half_h * dq[1];
+                    dq[2] = eloss_dqopds(vm, qd[2], kMuonMass, q);
+                    if (cfg.use_mean_loss) qd[3] = tk.qop + h * dq[2];
+                    dq[3] = eloss_dqopds(vm, qd[3], kMuonMass, q);
+                    for (int n = 1; n < 4; ++n) qs[n] = static_cast<S>(qd[n]);
+                }
+            }
             t2 = (half_h == 0.0) ? dir : tadd(dir, tmul(hh, k1));
-            k2 = tmul(qop, tcross(t2, b2));
+            k2 = tmul(qs[1], tcross(t2, b2));
             t3 = (half_h == 0.0) ? dir : tadd(dir, tmul(hh, k2));
-            k3 = tmul(qop, tcross(t3, b2));
+            k3 = tmul(qs[2], tcross(t3, b2));
             const V3 pos2 = add(add(pos, mul(h, tk.dir)), mul(h2 * 0.5, tdbl(k3)));
             b3 = field_at_s<S>(field, pos2);
             t4 = (h == 0.0) ? dir : tadd(dir, tmul(hs, k3));
-            k4 = tmul(qop, tcross(t4, b3));
+            k4 = tmul(qs[3], tcross(t4, b3));
             const V3 d = add(sub(sub(k1d, tdbl(k2)), tdbl(k3)), tdbl(k4));
             const V3 err = mul((1.0 / 6.0) * h2, d);
             error = dmax(norm(err), 1e-20);
@@ -1110,6 +1148,7 @@ struct FastEngine {
             tk.step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
         }
         tk.dtds3 = k4;
+        if constexpr (P::kVolMat) tk.dqopds3 = dq[3];
         const double cmin = dmin(dmin(kMax, fabs(cfg.step_constraint)), tk.constraint_abort);
         const double max_step = (tk.step_size >= 0.0 ? 1.0 : -1.0) * cmin;
         if (fabs(tk.step_size) > fabs(max_step)) tk.step_size = max_step;
@@ -1120,32 +1159,66 @@ struct FastEngine {
         // advance_track (rk_stepper.hpp:594-629)
         const double h = tk.step_size;
         const double h_6 = h * (1.0 / 6.0);
+        [[maybe_unused]] double qn = tk.qop;
+        if constexpr (P::kVolMat) {
+            if (vm) {
+                qn = tk.qop + h_6 * (dq[0] + 2.0 * (dq[1] + dq[2]) + dq[3]);
+                // brought to rest within the step: not taken, the track stops
+                if (!(fabs(qn) <= kMax && qn * tk.qop > 0.0)) {
+                    tk.stopped = true;
+                    return;
+                }
+            }
+        }
         const V3 ksum = add(add(k1d, tdbl(k2)), tdbl(k3));
         tk.pos = add(pos, mul(h, add(tk.dir, mul(h_6, ksum))));
         const V3 kd = add(add(k1d, mul(2.0, add(tdbl(k2), tdbl(k3)))), tdbl(k4));
         const V3 nd = add(tk.dir, mul(h_6, kd));
         const double nn = norm(nd);
         tk.dir = {nd.x / nn, nd.y / nn, nd.z / nn};
+        if constexpr (P::kVolMat) tk.qop = qn;
         tk.path_length += h;
         tk.abs_path_length += fabs(h);
         if (cfg.do_cov) {
             if constexpr (P::kGrad) advance_jacobian_grad(dir, k1, k3, t2, t3, t4, b1, b2, b3, qop);
-            else advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+            else advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qs, qd, vm);
         }
         tk.n_steps++;
     }
 
-    // rk_stepper.hpp:633-956 (dqop_n/dqop1 = 1, no field gradient), applied
-    // column by column on the reduced Jacobian
+    // rk_stepper.hpp:633-956 (no field gradient), applied column by column
+    // on the reduced Jacobian.  q: stage q/p in S (qd in FP64); with
+    // use_eloss_gradient in volume material, dqop_n/dqop1 and d(qop)/d(qop1)
+    // follow d(d(qop)/ds)/dqop (rk_stepper.hpp:767-800), 1 otherwise.
     DTR_AI void advance_jacobian(T3<S> t1, T3<S> t2, T3<S> t3, T3<S> t4, T3<S> b1, T3<S> b2,
-                                 T3<S> b3, S qop) {
+                                 T3<S> b3, const S q[4], [[maybe_unused]] const double qd[4],
+                                 [[maybe_unused]] const dtr_material_slab* vm) {
         const S h = static_cast<S>(tk.step_size);
         const S half_h = h * S(0.5);
         const S h_6 = h * static_cast<S>(1.f / 6.f);
-        const T3<S> dk0 = tcross(t1, b1);
-        const T3<S> dk1 = tadd(tcross(t2, b2), tmul(qop * half_h, tcross(dk0, b2)));
-        const T3<S> dk2 = tadd(tcross(t3, b2), tmul(qop * half_h, tcross(dk1, b2)));
-        const T3<S> dk3 = tadd(tcross(t4, b3), tmul(qop * h, tcross(dk2, b3)));
+        T3<S> c0 = tcross(t1, b1), c1 = tcross(t2, b2), c2 = tcross(t3, b2), c3 = tcross(t4, b3);
+        [[maybe_unused]] double dqopqop = 1.0;
+        if constexpr (P::kVolMat) {
+            if (vm && cfg.use_eloss_gradient) {
+                const double hd = tk.step_size, hh = hd * 0.5, h6 = hd * static_cast<double>(1.f / 6.f);
+                const double q = track_charge(tk.qop);
+                const double e1 = eloss_d2qopdsdqop(vm, qd[0], kMuonMass, q);
+                const double n1 = 1.0 + hh * e1;
+                const double e2 = eloss_d2qopdsdqop(vm, qd[1], kMuonMass, q) * n1;
+                const double n2 = 1.0 + hh * e2;
+                const double e3 = eloss_d2qopdsdqop(vm, qd[2], kMuonMass, q) * n2;
+                const double n3 = 1.0 + hd * e3;
+                const double e4 = eloss_d2qopdsdqop(vm, qd[3], kMuonMass, q) * n3;
+                dqopqop = 1.0 + h6 * (e1 + 2.0 * (e2 + e3) + e4);
+                c1 = tmul(static_cast<S>(n1), c1);
+                c2 = tmul(static_cast<S>(n2), c2);
+                c3 = tmul(static_cast<S>(n3), c3);
+            }
+        }
+        const T3<S> dk0 = c0;
+        const T3<S> dk1 = tadd(c1, tmul(q[1] * half_h, tcross(dk0, b2)));
+        const T3<S> dk2 = tadd(c2, tmul(q[2] * half_h, tcross(dk1, b2)));
+        const T3<S> dk3 = tadd(c3, tmul(q[3] * h, tcross(dk2, b3)));
         const T3<S> dFdqop = tmul(h * h_6, tadd(tadd(dk0, dk1), dk2));
         const T3<S> dGdqop = tmul(h_6, tadd(tadd(dk0, tmul(S(2), tadd(dk1, dk2))), dk3));
         // dk_n/dt1 applied to the columns 4..6 of a unit vector e_j:
@@ -1154,10 +1227,10 @@ struct FastEngine {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const T3<S> e{j == 0 ? S(1) : S(0), j == 1 ? S(1) : S(0), j == 2 ? S(1) : S(0)};
-            const T3<S> d0 = tmul(qop, tcross(e, b1));
-            const T3<S> d1 = tmul(qop, tcross(tadd(e, tmul(half_h, d0)), b2));
-            const T3<S> d2 = tmul(qop, tcross(tadd(e, tmul(half_h, d1)), b2));
-            const T3<S> d3 = tmul(qop, tcross(tadd(e, tmul(h, d2)), b3));
+            const T3<S> d0 = tmul(q[0], tcross(e, b1));
+            const T3<S> d1 = tmul(q[1], tcross(tadd(e, tmul(half_h, d0)), b2));
+            const T3<S> d2 = tmul(q[2], tcross(tadd(e, tmul(half_h, d1)), b2));
+            const T3<S> d3 = tmul(q[3], tcross(tadd(e, tmul(h, d2)), b3));
             const T3<S> f = tmul(h, tadd(e, tmul(h_6, tadd(tadd(d0, d1), d2))));
             const T3<S> gg = tadd(e, tmul(h_6, tadd(tadd(d0, tmul(S(2), tadd(d1, d2))), d3)));
             dFdt[0 * 3 + j] = f.x;
@@ -1196,6 +1269,7 @@ struct FastEngine {
 #pragma unroll
             for (int r = 0; r < 3; ++r) tk.J[jdir(r, cc)] = g[r];
         }
+        if constexpr (P::kVolMat) tk.J[kJqq] = static_cast<S>(dqopqop) * jq;
     }
 
     // Field gradient by central differences over +-0.1 mm (rk_stepper.hpp:
@@ -1571,6 +1645,14 @@ struct FastEngine {
             Md[r][4] = tk.J[jdir(r, 7)];
         }
         const S Mq4 = tk.J[kJqq];
+        // the q/p row of J B2F and of the path correction with the volume's
+        // d(qop)/ds (parameter_updater.hpp:518-526, rk_stepper.hpp:147-155)
+        [[maybe_unused]] S Jq[5] = {S(0), S(0), S(0), S(0), Mq4};
+        [[maybe_unused]] S dqs = S(0);
+        if constexpr (P::kVolMat)
+            dqs = static_cast<S>(tk.path_length == 0.0
+                                     ? eloss_dqopds(vol_mat(), tk.qop, kMuonMass, track_charge(tk.qop))
+                                     : tk.dqopds3);
         const T3<S> k = dtds();
         const NavSurface& erec = det.sf[dest];
         const bool ext_dest = P::kExt && (erec.shape >= DTR_LINE_SQUARE || erec.shape == DTR_RING2D);
@@ -1598,6 +1680,7 @@ struct FastEngine {
         for (int cc = 0; cc < 5; ++cc) {
             S proj = f2p.x * Mp[0][cc] + f2p.y * Mp[1][cc] + f2p.z * Mp[2][cc];
             if constexpr (P::kExt) proj += f2d.x * Md[0][cc] + f2d.y * Md[1][cc] + f2d.z * Md[2][cc];
+            if constexpr (P::kVolMat) Jq[cc] += dqs * proj;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
                 Mp[r][cc] += p2f_pos[r] * proj;
@@ -1650,7 +1733,7 @@ struct FastEngine {
                 for (int cc = 0; cc < 6; ++cc) {
                     double v = 0.0;
                     if (r < 4 && cc < 5) v = static_cast<double>(Jf[r][cc]);
-                    else if (r == 4 && cc == 4) v = static_cast<double>(Mq4);
+                    else if (r == 4 && cc < 5) v = static_cast<double>(P::kVolMat ? Jq[cc] : (cc == 4 ? Mq4 : S(0)));
                     else if (r == 5 && cc == 5) v = 1.0;
                     jf_out[r * 6 + cc] = v;
                 }
@@ -1670,7 +1753,14 @@ struct FastEngine {
         }
 #pragma unroll
         for (int cc = 0; cc < 6; ++cc) {
-            T[4][cc] = Mq4 * C(4, cc);
+            if constexpr (P::kVolMat) {
+                S s = S(0);
+#pragma unroll
+                for (int m = 0; m < 5; ++m) s += Jq[m] * C(m, cc);
+                T[4][cc] = s;
+            } else {
+                T[4][cc] = Mq4 * C(4, cc);
+            }
             T[5][cc] = C(5, cc);
         }
         S NC[21];
@@ -1683,7 +1773,12 @@ struct FastEngine {
 #pragma unroll
                     for (int m = 0; m < 5; ++m) s += T[r][m] * Jf[cc][m];
                 } else if (cc == 4) {
-                    s = T[r][4] * Mq4;
+                    if constexpr (P::kVolMat) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) s += T[r][m] * Jq[m];
+                    } else {
+                        s = T[r][4] * Mq4;
+                    }
                 } else {
                     s = T[r][5];
                 }
@@ -1755,11 +1850,7 @@ struct FastEngine {
         if (cfg.do_eloss) {
             double bb = 0.0;
             if (eps_pl > 0.0) {
-                double dhalf = 0.0;
-                if (bg >= 10.0) {
-                    const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
-                    dhalf = log(bg * plasma / m.mean_exc) - 0.5;
-                }
+                const double dhalf = delta_half(m, bg, ned);
                 const double u = 2.0 * m_e * bg * bg;
                 const double A = 0.5 * log(u * Wmax / (m.mean_exc * m.mean_exc));
                 bb = 2.0 * eps_pl * (A - beta2 - dhalf);
@@ -2027,6 +2118,7 @@ struct Lane {
         tk.abs_path_length = 0.0;
         tk.constraint_abort = kMax;
         tk.dtds3 = {S(0), S(0), S(0)};
+        tk.dqopds3 = 0.0;
         tk.n_trials = 0;
         tk.n_steps = 0;
         for (int i = 0; i < 6; ++i) tk.bound[i] = 0.0;

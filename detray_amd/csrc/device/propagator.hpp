@@ -74,6 +74,7 @@ struct DevConfig {
     double n_stddev;
     uint32_t max_surfaces;
     uint8_t do_cov, estimate_noise, do_eloss, do_ms, use_material;
+    uint8_t use_mean_loss, use_eloss_gradient;
     // optional actors (dtr_actor bits) and their states
     uint32_t actors;
     double min_p, min_pT;
@@ -104,6 +105,8 @@ inline DevConfig make_dev_config(const dtr_prop_config* c) {
     dc.do_eloss = c->do_energy_loss;
     dc.do_ms = c->do_multiple_scattering;
     dc.use_material = c->use_material;
+    dc.use_mean_loss = c->use_mean_loss;
+    dc.use_eloss_gradient = c->use_eloss_gradient;
     if (const dtr_actor_states* a = c->actors) {
         dc.actors = a->actors;
         dc.min_p = static_cast<double>(a->min_p);
@@ -245,6 +248,7 @@ struct Track {
     double step_size, next_step_size, path_length, abs_path_length;
     double constraint_acc, constraint_abort;
     V3 dtds3;
+    double dqopds3;  // d(qop)/ds of the last step's fourth stage
     double mass, charge;
     uint32_t n_trials, n_steps;
     // parameter_updater_state
@@ -348,6 +352,109 @@ DTR_HD void ext_mask_check(uint8_t shape, const double* v, V3 loc, double base_t
             edge = ((v[1] - ft) <= loc.z && loc.z <= (v[2] + ft));
         }
     }
+}
+
+// ------------------------------------------------------------ energy loss --
+// relativistic_quantities.hpp:49-84 for a particle of mass m and charge q
+struct RelQ {
+    double qop, q2b2, beta2, bg, gamma, gamma2, E, Wmax;
+};
+DTR_HD RelQ rel_quantities(double mass, double qop, double q) {
+    RelQ r;
+    r.qop = qop;
+    r.q2b2 = q * q + (mass * qop) * (mass * qop);
+    const double mOverP = mass * ((q != 0.0) ? fabs(qop / q) : fabs(qop));
+    const double pOverM = 1.0 / mOverP;
+    r.beta2 = 1.0 / (1.0 + mOverP * mOverP);
+    r.bg = pOverM;
+    r.gamma = sqrt(1.0 + pOverM * pOverM);
+    r.gamma2 = r.gamma * r.gamma;
+    r.E = r.gamma * mass;
+    const double m_e = 0.51099895069 * 1e-3;
+    const double mfrac = m_e / mass;
+    r.Wmax = (2.0 * m_e * r.bg * r.bg) / (1.0 + 2.0 * r.gamma * mfrac + mfrac * mfrac);
+    return r;
+}
+// compute_delta_half (relativistic_quantities.hpp:137-200): the approximate
+// correction above beta gamma = 10 without density-effect data (ned: molar
+// electron density), Sternheimer's parameterisation with it
+DTR_HD double delta_half(const dtr_material_slab& m, double bg, double ned) {
+    if (!m.has_density) {
+        if (!(bg >= 10.0)) return 0.0;
+        const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
+        return log(bg * plasma / m.mean_exc) - 0.5;
+    }
+    const double x = log10(bg);
+    double delta = 0.0;
+    if (x >= m.density[2]) {
+        delta = 2.0 * 2.302585092994045684 * x - m.density[4];
+        if (x < m.density[3]) delta += m.density[0] * pow(m.density[3] - x, m.density[1]);
+    }
+    return 0.5 * delta;
+}
+// derive_delta_half (relativistic_quantities.hpp:202-240); -1/qop without
+// density-effect data at any beta gamma, as there
+DTR_HD double derive_delta_half(const dtr_material_slab& m, const RelQ& r) {
+    if (!m.has_density) return -1.0 / r.qop;
+    const double x = log10(r.bg);
+    double delta = 0.0;
+    if (x >= m.density[2]) {
+        delta = -2.0 / r.qop;
+        if (x < m.density[3])
+            delta += m.density[0] * m.density[1] / (r.qop * 2.302585092994045684) *
+                     pow(m.density[3] - x, m.density[1] - 1.0);
+    }
+    return 0.5 * delta;
+}
+// compute_bethe_bloch (interaction.hpp:44-58): the muon's stopping power
+DTR_HD double stopping_power(const dtr_material_slab& m, const RelQ& r) {
+    const double K = 0.307075 * 1e-3 * 1e2;
+    const double ned = m.Z * m.molar_rho;
+    const double eps_pl = 0.5 * K * ned * r.q2b2;
+    if (!(eps_pl > 0.0)) return 0.0;
+    const double dhalf = delta_half(m, r.bg, ned);
+    const double m_e = 0.51099895069 * 1e-3;
+    const double u = 2.0 * m_e * r.bg * r.bg;
+    const double A = 0.5 * log(u * r.Wmax / (m.mean_exc * m.mean_exc));
+    return 2.0 * eps_pl * (A - r.beta2 - dhalf);
+}
+// derive_bethe_bloch (interaction.hpp:110-154)
+DTR_HD double derive_stopping_power(const dtr_material_slab& m, const RelQ& r) {
+    const double K = 0.307075 * 1e-3 * 1e2;
+    const double eps_pl = 0.5 * K * (m.Z * m.molar_rho) * r.q2b2;
+    if (!(eps_pl > 0.0)) return 0.0;
+    const double first = 2.0 / (r.qop * r.gamma2) * stopping_power(m, r);
+    const double dA = -1.0 / (2.0 * r.qop) * (4.0 - r.Wmax / r.E);
+    const double dB = -2.0 * r.beta2 / (r.qop * r.gamma2);
+    const double dC = derive_delta_half(m, r);
+    return first + 2.0 * eps_pl * (dA - dB - dC);
+}
+// rk_stepper.hpp:157-189: d(qop)/ds = qop^3 E (-dE/ds) / q^2, zero in vacuum
+DTR_HD double eloss_dqopds(const dtr_material_slab* vm, double qop, double mass, double q) {
+    if (!vm) return 0.0;
+    const double p = q / qop;
+    const double E = sqrt(p * p + mass * mass);
+    const double sp = stopping_power(*vm, rel_quantities(mass, qop, q));
+    return qop * qop * qop * E * sp / (q * q);
+}
+// rk_stepper.hpp:194-231: d(d(qop)/ds)/dqop (JINST 4 P04016, Eq. 3.12)
+DTR_HD double eloss_d2qopdsdqop(const dtr_material_slab* vm, double qop, double mass, double q) {
+    if (!vm) return 0.0;
+    const double p = q / qop;
+    const double p2 = p * p;
+    const double E2 = p2 + mass * mass;
+    const RelQ r = rel_quantities(mass, qop, q);
+    const double g = -1.0 * stopping_power(*vm, r);
+    const double dgdqop = -1.0 * derive_stopping_power(*vm, r);
+    const double dq = eloss_dqopds(vm, qop, mass, q);
+    return dq * (1.0 / qop * (3.0 - p2 / E2) + 1.0 / g * dgdqop);
+}
+// The homogeneous material of volume v (0xffff: none), propagator.hpp:320-325
+DTR_HD const dtr_material_slab* volume_material(const dtr_volume* vols, const dtr_material_slab* mats,
+                                                uint16_t v) {
+    if (v == kEndOfWorld) return nullptr;
+    const uint32_t m = vols[v].material;
+    return m ? &mats[m - 1u] : nullptr;
 }
 
 // kExt: the engine also handles wires, annuli, placed cylinders, rods,
@@ -797,7 +904,19 @@ struct EngineT {
     }
 
     // -------------------------------------------------------------- stepper --
-    // rk_stepper.hpp:273-530 (no volume material: q/p constant in a step)
+    DTR_HD const dtr_material_slab* vol_mat() const {
+        return kExt ? volume_material(det.volumes, det.materials, tk.volume) : nullptr;
+    }
+    // rk_stepper.hpp:147-155: the last step's fourth-stage d(qop)/ds, or
+    // its value at the current q/p before the first step
+    DTR_HD double state_dqopds() const {
+        if (tk.path_length == 0.0) return eloss_dqopds(vol_mat(), tk.qop, tk.mass, tk.charge);
+        return tk.dqopds3;
+    }
+
+    // rk_stepper.hpp:273-530.  In volume material (extended geometry only)
+    // the stage q/p follow the energy loss (evaluate_dqopds,
+    // rk_stepper.hpp:534-565); in vacuum every stage sees the track's q/p.
     DTR_HD void step(double dist, bool do_reset) {
         if (fabs(dist) < static_cast<double>(1.f * 1e-3f)) {
             if (fabs(tk.next_step_size) < cfg.min_stepsize)
@@ -815,6 +934,10 @@ struct EngineT {
         const V3 pos = tk.pos;
         const V3 dir = tk.dir;
         const double qop = tk.qop;
+        const dtr_material_slab* vm = vol_mat();
+        // stage q/p and d(qop)/ds
+        double q[4] = {qop, qop, qop, qop}, dq[4] = {0.0, 0.0, 0.0, 0.0};
+        if (kExt && vm) dq[0] = eloss_dqopds(vm, qop, tk.mass, tk.charge);
         const V3 b1 = field_at(field, pos);
         const V3 k1 = mul(qop, cross(dir, b1));
         V3 b2, b3, t2, t3, t4, k2, k3, k4;
@@ -826,14 +949,26 @@ struct EngineT {
             const double half_h = h * 0.5;
             const V3 pos1 = add(add(pos, mul(half_h, dir)), mul(h2 * 0.125, k1));
             b2 = field_at(field, pos1);
+            if (kExt && vm) {
+                if (cfg.use_mean_loss) q[1] = qop + half_h * dq[0];
+                dq[1] = eloss_dqopds(vm, q[1], tk.mass, tk.charge);
+            }
             t2 = (half_h == 0.0) ? dir : add(dir, mul(half_h, k1));
-            k2 = mul(qop, cross(t2, b2));
+            k2 = mul(q[1], cross(t2, b2));
+            if (kExt && vm) {
+                if (cfg.use_mean_loss) q[2] = qop + half_h * dq[1];
+                dq[2] = eloss_dqopds(vm, q[2], tk.mass, tk.charge);
+            }
             t3 = (half_h == 0.0) ? dir : add(dir, mul(half_h, k2));
-            k3 = mul(qop, cross(t3, b2));
+            k3 = mul(q[2], cross(t3, b2));
             const V3 pos2 = add(add(pos, mul(h, dir)), mul(h2 * 0.5, k3));
             b3 = field_at(field, pos2);
+            if (kExt && vm) {
+                if (cfg.use_mean_loss) q[3] = qop + h * dq[2];
+                dq[3] = eloss_dqopds(vm, q[3], tk.mass, tk.charge);
+            }
             t4 = (h == 0.0) ? dir : add(dir, mul(h, k3));
-            k4 = mul(qop, cross(t4, b3));
+            k4 = mul(q[3], cross(t4, b3));
             const V3 d = add(sub(sub(k1, k2), k3), k4);
             const V3 err = mul((1.0 / 6.0) * h2, d);
             error = dmax(norm(err), 1e-20);
@@ -841,6 +976,7 @@ struct EngineT {
             tk.step_size = tk.step_size * dmin(dmax(sqrt(sqrt(cfg.rk_error_tol / error)), 0.25), 4.0);
         }
         tk.dtds3 = k4;
+        tk.dqopds3 = dq[3];
         // constrained_step (accuracy, aborter)
         const double cmin = dmin(tk.constraint_acc, tk.constraint_abort);
         const double max_step = (tk.step_size >= 0.0 ? 1.0 : -1.0) * cmin;
@@ -853,13 +989,23 @@ struct EngineT {
         // advance_track (rk_stepper.hpp:594-629)
         const double h = tk.step_size;
         const double h_6 = h * (1.0 / 6.0);
+        double qn = tk.qop;
+        if (kExt && vm) {
+            qn = tk.qop + h_6 * (dq[0] + 2.0 * (dq[1] + dq[2]) + dq[3]);
+            // brought to rest within the step: not taken, the track stops
+            if (!(fabs(qn) <= kMax && qn * tk.qop > 0.0)) {
+                tk.stopped = true;
+                return;
+            }
+        }
         tk.pos = add(pos, mul(h, add(dir, mul(h_6, add(add(k1, k2), k3)))));
         const V3 nd = add(dir, mul(h_6, add(add(k1, mul(2.0, add(k2, k3))), k4)));
         const double nn = norm(nd);
         tk.dir = {nd.x / nn, nd.y / nn, nd.z / nn};
+        tk.qop = qn;
         tk.path_length += h;
         tk.abs_path_length += fabs(h);
-        if (cfg.do_cov) advance_jacobian(dir, t2, t3, t4, b1, b2, b3, qop);
+        if (cfg.do_cov) advance_jacobian(dir, t2, t3, t4, b1, b2, b3, q, vm);
         tk.n_steps++;
     }
 
@@ -873,31 +1019,47 @@ struct EngineT {
         }
     }
 
-    // rk_stepper.hpp:633-956 (dqop_n/dqop1 = 1, no field gradient) and
-    // codegen/update_rk_transport_jacobian.hpp:18-199 on the reduced storage
-    DTR_HD void advance_jacobian(V3 t1, V3 t2, V3 t3, V3 t4, V3 b1, V3 b2, V3 b3, double qop) {
+    // rk_stepper.hpp:633-956 (no field gradient) and
+    // codegen/update_rk_transport_jacobian.hpp:18-199 on the reduced storage.
+    // q: stage q/p; with use_eloss_gradient dqop_n/dqop1 and d(qop)/d(qop1)
+    // follow d(d(qop)/ds)/dqop of the volume material (rk_stepper.hpp:
+    // 767-800), 1 otherwise and in vacuum.
+    DTR_HD void advance_jacobian(V3 t1, V3 t2, V3 t3, V3 t4, V3 b1, V3 b2, V3 b3, const double q[4],
+                                 const dtr_material_slab* vm) {
         const double h = tk.step_size;
         const double half_h = h * 0.5;
         const double h_6 = h * static_cast<double>(1.f / 6.f);
-        V3 dk0 = mul(1.0, cross(t1, b1));
-        V3 dk1 = add(mul(1.0, cross(t2, b2)), mul(qop * half_h, cross(dk0, b2)));
-        V3 dk2 = add(mul(1.0, cross(t3, b2)), mul(qop * half_h, cross(dk1, b2)));
-        V3 dk3 = add(mul(1.0, cross(t4, b3)), mul(qop * h, cross(dk2, b3)));
+        double dqn[4] = {1.0, 1.0, 1.0, 1.0};
+        double dqopqop = 1.0;
+        if (kExt && vm && cfg.use_eloss_gradient) {
+            const double e1 = eloss_d2qopdsdqop(vm, q[0], tk.mass, tk.charge);
+            dqn[1] = 1.0 + half_h * e1;
+            const double e2 = eloss_d2qopdsdqop(vm, q[1], tk.mass, tk.charge) * dqn[1];
+            dqn[2] = 1.0 + half_h * e2;
+            const double e3 = eloss_d2qopdsdqop(vm, q[2], tk.mass, tk.charge) * dqn[2];
+            dqn[3] = 1.0 + h * e3;
+            const double e4 = eloss_d2qopdsdqop(vm, q[3], tk.mass, tk.charge) * dqn[3];
+            dqopqop = 1.0 + h_6 * (e1 + 2.0 * (e2 + e3) + e4);
+        }
+        V3 dk0 = mul(dqn[0], cross(t1, b1));
+        V3 dk1 = add(mul(dqn[1], cross(t2, b2)), mul(q[1] * half_h, cross(dk0, b2)));
+        V3 dk2 = add(mul(dqn[2], cross(t3, b2)), mul(q[2] * half_h, cross(dk1, b2)));
+        V3 dk3 = add(mul(dqn[3], cross(t4, b3)), mul(q[3] * h, cross(dk2, b3)));
         const V3 dFdqop = mul(h * h_6, add(add(dk0, dk1), dk2));
         const V3 dGdqop = mul(h_6, add(add(dk0, mul(2.0, add(dk1, dk2))), dk3));
         const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         double d0[9], d1[9], d2[9], d3[9], tmp[9];
         cwc(I, b1, tmp);
-        for (int i = 0; i < 9; ++i) d0[i] = qop * tmp[i];
+        for (int i = 0; i < 9; ++i) d0[i] = q[0] * tmp[i];
         for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * d0[i];
         cwc(tmp, b2, d1);
-        for (int i = 0; i < 9; ++i) d1[i] *= qop;
+        for (int i = 0; i < 9; ++i) d1[i] *= q[1];
         for (int i = 0; i < 9; ++i) tmp[i] = I[i] + half_h * d1[i];
         cwc(tmp, b2, d2);
-        for (int i = 0; i < 9; ++i) d2[i] *= qop;
+        for (int i = 0; i < 9; ++i) d2[i] *= q[2];
         for (int i = 0; i < 9; ++i) tmp[i] = I[i] + h * d2[i];
         cwc(tmp, b3, d3);
-        for (int i = 0; i < 9; ++i) d3[i] *= qop;
+        for (int i = 0; i < 9; ++i) d3[i] *= q[3];
         double dFdt[9], dGdt[9];
         for (int i = 0; i < 9; ++i) {
             dFdt[i] = h * (I[i] + h_6 * (d0[i] + d1[i] + d2[i]));
@@ -930,7 +1092,7 @@ struct EngineT {
                 }
             }
         }
-        nj[kJqq] = 1.0 * tk.J[kJqq];
+        nj[kJqq] = dqopqop * tk.J[kJqq];
         for (int i = 0; i < 25; ++i) tk.J[i] = nj[i];
     }
 
@@ -1119,6 +1281,9 @@ struct EngineT {
             Md[r][5] = 0.0;
         }
         const double Mq4 = tk.J[kJqq];  // qop row: only column 4
+        // d(qop)/ds of the path correction, with the volume at the surface
+        // (parameter_updater.hpp:518-526)
+        const double dqs = kExt ? state_dqopds() : 0.0;
         // path correction: P = (I + p2f f2p) applied on the left;
         // p2f = [dir, 0, dtds, 0], f2p = [n_pos, 0...]
         const V3 k = dtds();
@@ -1207,6 +1372,8 @@ struct EngineT {
             Jf[2][c] = Ba[0] * Md[0][c] + Ba[1] * Md[1][c] + Ba[2] * Md[2][c];
             Jf[3][c] = Ba[3] * Md[0][c] + Ba[4] * Md[1][c] + Ba[5] * Md[2][c];
             Jf[4][c] = (c == 4) ? Mq4 : 0.0;
+            // the q/p row of the path correction (jacobian_engine.hpp:260-273)
+            if (kExt) Jf[4][c] += dqs * proj[c];
             Jf[5][c] = (c == 5) ? 1.0 : 0.0;
         }
         // C' = Jf C Jf^T
@@ -1289,11 +1456,7 @@ struct EngineT {
         if (cfg.do_eloss) {
             double bb = 0.0;
             if (eps_pl > 0.0) {
-                double dhalf = 0.0;
-                if (bg >= 10.0) {
-                    const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
-                    dhalf = log(bg * plasma / m.mean_exc) - 0.5;
-                }
+                const double dhalf = delta_half(m, bg, ned);
                 const double u = 2.0 * m_e * bg * bg;
                 const double A = 0.5 * log(u * Wmax / (m.mean_exc * m.mean_exc));
                 bb = 2.0 * eps_pl * (A - beta2 - dhalf);
@@ -1457,6 +1620,7 @@ DTR_HD void propagate_track(const DevDetector& det, const DevField& field,
     tk.constraint_acc = dmin(kMax, fabs(cfg.step_constraint));
     tk.constraint_abort = kMax;
     tk.dtds3 = {0.0, 0.0, 0.0};
+    tk.dqopds3 = 0.0;
     // muon hypothesis, charge-conjugated to the track (pdg_particle.hpp:61-64)
     tk.mass = static_cast<double>(105.6583755f * 1e-3f);
     tk.n_trials = 0;

@@ -98,6 +98,34 @@ dtr_material_slab make_slab(const Material& m, double thickness) {
     return dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho, thickness);
 }
 
+// The materials of predefined_materials.hpp that dtr_predefined_material
+// names: X0, L0, A, Z, mass density, and for the *_with_ded entries the
+// density-effect parameters (a, k, x0, x1, I [eV], C-bar); float literals
+// there are converted float -> double as in the reference.
+struct NamedMaterial {
+    const char* name;
+    double X0, L0, A, Z, rho_g_cm3;
+    bool ded;
+    double a, k, x0, x1, I_eV, cbar;
+};
+const NamedMaterial kMaterials[] = {
+    {"silicon", F(93.7), F(465.2), F(28.0855), F(14.0), 2.329, false, 0, 0, 0, 0, 0, 0},
+    {"silicon_with_ded", F(93.7), F(465.2), F(28.0855), F(14.0), 2.329, true,
+     F(0.1492), F(3.2546), F(0.2015), F(2.8716), F(173.0), F(4.4355)},
+    {"silicon_tml", F(95.7), F(465.2), F(28.03), F(14.0), 2.32, false, 0, 0, 0, 0, 0, 0},
+    {"beryllium_tml", F(352.8), F(407.0), F(9.012), F(4.0), 1.848, false, 0, 0, 0, 0, 0, 0},
+    {"tungsten", F(3.504), F(99.46), F(183.84), F(74.0), 19.3, false, 0, 0, 0, 0, 0, 0},
+    {"isobutane", F(1693e2), F(288.3), F(58.124), F(34.0), 2.67, false, 0, 0, 0, 0, 0, 0},
+    {"argon_liquid", F(14.0) * 10.0, F(85.77) * 10.0, F(39.948), F(18.0), 1.396, false, 0, 0, 0, 0, 0, 0},
+    {"iron_with_ded", F(1.757) * 10.0, F(16.77) * 10.0, F(55.845), F(26.0), 7.874, true,
+     F(0.14680), F(2.9632), F(-0.0012), F(3.1531), F(286.0), F(4.2911)},
+    {"copper_with_ded", F(1.436) * 10.0, F(15.32) * 10.0, F(63.546), F(29.0), 8.960, true,
+     F(0.14339), F(2.9044), F(-0.0254), F(3.2792), F(322.0), F(4.4190)},
+    {"cesium_iodide", F(1.86) * 10.0, F(38.04) * 10.0, F(259.81), F(108.0), F(4.510), false, 0, 0, 0, 0, 0, 0},
+    {"cesium_iodide_with_ded", F(1.86) * 10.0, F(38.04) * 10.0, F(259.81), F(108.0), F(4.510), true,
+     F(0.25381), F(2.6657), F(0.0395), F(3.3353), F(553.1), F(6.2807)},
+};
+
 // material_rod.hpp:26-32: the radius takes the place of the thickness
 dtr_material_slab make_rod(const Material& m, double radius) {
     dtr_material_slab r = dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho, radius);
@@ -132,6 +160,7 @@ struct VolumeDef {
     dtr_transform placement = make_translation({0, 0, 0});
     std::vector<SurfaceDef> surfaces;
     GridDef grid;
+    int material = -1;  // homogeneous volume material, index into the material table
 };
 
 // Axis-aligned bounding box, utils/bounding_volume.hpp
@@ -284,6 +313,7 @@ void assemble(const std::vector<VolumeDef>& vols,
     for (uint32_t vi = 0; vi < vols.size(); ++vi) {
         const VolumeDef& V = vols[vi];
         dtr_volume dv{};
+        dv.material = V.material >= 0 ? static_cast<uint32_t>(V.material) + 1u : 0u;
         dv.transform = static_cast<uint32_t>(T.transforms.size());
         T.transforms.push_back(V.placement);
         dv.sf_begin = static_cast<uint32_t>(T.surfaces.size());
@@ -842,6 +872,16 @@ void build_telescope(const dtr_telescope_config& cfg, Tables& T) {
         mat = 0;
     }
     VolumeDef& vol = vols[0];
+    // homogeneous_volume_material_builder decoration
+    // (build_telescope_detector.hpp:262-267)
+    if (cfg.volume_material) {
+        dtr_material_slab vm = *cfg.volume_material;
+        vm.kind = DTR_MATERIAL_SLAB;
+        vm.thickness = 0.0;
+        vm.thickness_in_X0 = 0.0;
+        vol.material = static_cast<int>(mats.size());
+        mats.push_back(vm);
+    }
     for (uint32_t i = 0; i < cfg.n_planes; ++i) {
         const double s = cfg.positions[i];
         SurfaceDef sf;
@@ -1024,6 +1064,25 @@ int dtr_build_telescope_detector(const dtr_telescope_config* cfg, dtr_detector_d
         return DTR_ERROR_OUT_OF_MEMORY;
     }
     return DTR_SUCCESS;
+}
+
+int dtr_predefined_material(const char* name, double thickness, dtr_material_slab* out) {
+    if (!name || !out || !(thickness >= 0.0)) return DTR_ERROR_INVALID_ARGUMENT;
+    for (const NamedMaterial& m : kMaterials) {
+        if (std::strcmp(m.name, name) != 0) continue;
+        *out = dtr_host::make_slab(m.X0, m.L0, m.A, m.Z, m.rho_g_cm3 * kGram / kCm3, thickness);
+        if (m.ded) {
+            out->has_density = 1u;
+            out->density[0] = m.a;
+            out->density[1] = m.k;
+            out->density[2] = m.x0;
+            out->density[3] = m.x1;
+            out->density[4] = m.cbar;
+            out->mean_exc = m.I_eV * kEV;
+        }
+        return DTR_SUCCESS;
+    }
+    return DTR_ERROR_INVALID_ARGUMENT;
 }
 
 void dtr_host_detector_free(dtr_detector_desc* d) {

@@ -922,6 +922,8 @@ int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_n
     }
     try {
         check_desc(*det);
+        for (uint32_t vi = 0; vi < det->n_volumes; ++vi)
+            if (det->volumes[vi].material != 0u) unsupported("volume material is not written");
         if (geometry_file) save(geometry_file, write_geometry(*det, detector_name, volume_names), "geometry");
         if (grids_file) save(grids_file, write_grids(*det, detector_name), "surface grids");
         if (material_file) save(material_file, write_material(*det, detector_name), "homogeneous material");

@@ -92,6 +92,9 @@ inline std::string validate_desc(const dtr_detector_desc& d) {
         if (v.bf_begin > v.bf_end || v.bf_end > d.n_bf) return at("volume", vi) + "brute-force range out of range";
         if (v.grid < -1 || (v.grid >= 0 && static_cast<uint32_t>(v.grid) >= d.n_grids))
             return at("volume", vi) + "grid out of range";
+        if (v.material != 0u &&
+            (v.material > d.n_materials || d.materials[v.material - 1u].kind != DTR_MATERIAL_SLAB))
+            return at("volume", vi) + "material out of range or not homogeneous";
     }
     for (uint32_t s = 0; s < d.n_surfaces; ++s) {
         const dtr_surface& sf = d.surfaces[s];
@@ -108,8 +111,10 @@ inline std::string validate_desc(const dtr_detector_desc& d) {
         if (l != kEndOfWorld && l >= d.n_volumes) return at("mask", m) + "volume link out of range";
         if (d.masks[m].shape > DTR_CYLINDER2D) return at("mask", m) + "unknown shape";
     }
-    for (uint32_t k = 0; k < d.n_materials; ++k)
+    for (uint32_t k = 0; k < d.n_materials; ++k) {
         if (d.materials[k].kind > DTR_MATERIAL_ROD) return at("material", k) + "unknown kind";
+        if (d.materials[k].has_density > 1u) return at("material", k) + "has_density is not 0 or 1";
+    }
     for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
         const dtr_grid& g = d.grids[gi];
         const uint64_t cap = g.capacity ? g.capacity : 1u;

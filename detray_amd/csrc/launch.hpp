@@ -23,9 +23,11 @@ struct dtr_detector {
     int device;
     uint32_t n_cu;
     // wire, annulus or placed-cylinder surfaces, material rods, rings with
-    // material or grid bins of several surfaces: the kernels are instantiated
-    // with the extended geometry
+    // material, grid bins of several surfaces or volume material: the kernels
+    // are instantiated with the extended geometry
     bool wires;
+    bool volume_material;  // a volume holds homogeneous material
+    bool density_effect;   // a material carries density-effect data
 };
 
 struct dtr_field {

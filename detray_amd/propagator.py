@@ -141,13 +141,19 @@ class Detector:
 
     @classmethod
     def telescope(cls, positions: Sequence[float], half_x: float = 500.0, half_y: float = 500.0,
-                  thickness: float = 0.15, envelope: float = 60.0) -> "Detector":
+                  thickness: float = 0.15, envelope: float = 60.0,
+                  volume_material: "str | capi.MaterialSlab | None" = None) -> "Detector":
         """Telescope of rectangles along the z-axis pilot ray
         (build_telescope_detector.hpp; tel_det_config::envelope sets the
-        portal box margin)."""
+        portal box margin; tel_det_config::volume_material, by name of
+        predefined_materials.hpp or as a slab record, fills the volume)."""
         lib = capi.load()
         pos = (C.c_double * len(positions))(*positions)
-        cfg = capi.TelescopeConfig(len(positions), 0, pos, half_x, half_y, thickness, envelope)
+        vm = None
+        if volume_material is not None:
+            vm = material(volume_material) if isinstance(volume_material, str) else volume_material
+        cfg = capi.TelescopeConfig(len(positions), 0, pos, half_x, half_y, thickness, envelope,
+                                   C.pointer(vm) if vm is not None else None)
         d = capi.DetectorDesc()
         capi.check(lib.dtr_build_telescope_detector(C.byref(cfg), C.byref(d)),
                    "dtr_build_telescope_detector")
@@ -338,6 +344,16 @@ class MagneticField:
             self.close()
         except Exception:
             pass
+
+
+def material(name: str, thickness: float = 0.0) -> capi.MaterialSlab:
+    """A material of predefined_materials.hpp by name (e.g. "silicon",
+    "cesium_iodide_with_ded", "tungsten", "isobutane"), as a slab of the
+    given thickness (0 for a volume material)."""
+    out = capi.MaterialSlab()
+    capi.check(capi.load().dtr_predefined_material(name.encode(), float(thickness), C.byref(out)),
+               f"dtr_predefined_material({name!r})")
+    return out
 
 
 def generate_tracks(kind: str, n: int, seed: int, **kw) -> np.ndarray:

@@ -102,7 +102,9 @@ typedef struct dtr_mask {
 } dtr_mask;
 
 /* Homogeneous material slab (core/include/detray/material/material_slab.hpp,
- * material.hpp).  Derived quantities are filled by the builder. */
+ * material.hpp).  Derived quantities are filled by the builder.  The same
+ * record describes a volume's homogeneous material (dtr_volume::material;
+ * the thickness is then unused). */
 enum dtr_material_kind {
     DTR_MATERIAL_SLAB = 0, /* material_slab.hpp: path thickness / cos(incidence) */
     DTR_MATERIAL_ROD = 1   /* material_rod.hpp (on wires): path
@@ -112,11 +114,15 @@ enum dtr_material_kind {
 typedef struct dtr_material_slab {
     double X0, L0, A, Z, mass_rho;
     double molar_rho;     /* mass_rho / (A * u) in mol/mm^3 */
-    double mean_exc;      /* I = 16 eV * Z^0.9 */
+    double mean_exc;      /* I = 16 eV * Z^0.9, or the density-effect table's I */
     double thickness;     /* mm (slab thickness, or rod radius) */
     double thickness_in_X0;
     uint32_t kind;        /* dtr_material_kind */
-    uint32_t pad;
+    uint32_t has_density; /* 1: density[] holds the Sternheimer density-effect
+                             parameters (density_effect_data.hpp), which then
+                             replace the approximate density correction of the
+                             Bethe-Bloch formula (relativistic_quantities.hpp) */
+    double density[5];    /* a, k (the exponent m), x0, x1, C-bar */
 } dtr_material_slab;
 
 typedef struct dtr_surface {
@@ -155,7 +161,11 @@ typedef struct dtr_volume {
     uint32_t bf_begin, bf_end;  /* brute-force surface list range in bf_surfaces[] */
     uint32_t sf_begin, sf_end;  /* all surfaces of the volume */
     int32_t grid;               /* -1: none */
-    uint32_t pad;
+    uint32_t material;          /* homogeneous volume material (tracking_volume.hpp:
+                                   118, 285; homogeneous_volume_material_builder):
+                                   0 = vacuum, k > 0: materials[k - 1] (kind
+                                   DTR_MATERIAL_SLAB, thickness unused).  One-based
+                                   so that zero-initialised records are vacuum. */
 } dtr_volume;
 
 typedef struct dtr_detector_desc {
@@ -256,10 +266,13 @@ typedef struct dtr_prop_config {
     float step_constraint;    /* FLT_MAX (accuracy constraint of constrained_step) */
     float path_limit;         /* 5 m (pathlimit_aborter) */
     uint32_t max_rk_updates;  /* 10000 */
-    uint8_t use_mean_loss;    /* 1 */
-    uint8_t use_eloss_gradient;    /* 0: accepted; d(dqop/ds)/dqop of the
-                                      volume material, which this detector
-                                      model does not carry, so no effect */
+    uint8_t use_mean_loss;    /* 1: in volume material the q/p of the RK
+                                 stages 2-4 follows the energy loss of the
+                                 previous stage (rk_stepper.hpp:534-565) */
+    uint8_t use_eloss_gradient;    /* 0: 1 carries d(dqop/ds)/dqop of the
+                                      volume material into the transport
+                                      Jacobian (rk_stepper.hpp:195-231,
+                                      767-800); no effect in vacuum */
     uint8_t use_field_gradient;    /* 0: 1 adds dB/dr (central differences,
                                       +-0.1 mm, at the end of each step) to
                                       the Jacobian's position columns,
@@ -615,6 +628,9 @@ typedef struct dtr_telescope_config {
     double half_x, half_y;   /* rectangle half lengths */
     double thickness;        /* silicon_tml slab thickness, 0 = no material */
     double envelope;         /* cuboid portal envelope (tel_det_config::envelope) */
+    /* tel_det_config::volume_material (build_telescope_detector.hpp:82,
+     * 262-267): homogeneous material of the telescope volume, NULL = vacuum */
+    const dtr_material_slab* volume_material;
 } dtr_telescope_config;
 
 /* build_wire_chamber.hpp:120-331 (wire_chamber_config defaults in
@@ -637,6 +653,14 @@ DTR_API int dtr_build_wire_chamber(const dtr_wire_chamber_config* cfg, dtr_detec
 DTR_API int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
                                  dtr_detector_desc* out);
 DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);
+
+/* A material of predefined_materials.hpp by its name there ("silicon",
+ * "silicon_with_ded", "silicon_tml", "beryllium_tml", "tungsten",
+ * "isobutane", "argon_liquid", "iron_with_ded", "copper_with_ded",
+ * "cesium_iodide", "cesium_iodide_with_ded"), as a slab of the given
+ * thickness (0 for a volume material).  DTR_ERROR_INVALID_ARGUMENT for an
+ * unknown name. */
+DTR_API int dtr_predefined_material(const char* name, double thickness, dtr_material_slab* out);
 
 /* Detector tables from detray's JSON detector files: the geometry file and,
  * optionally (NULL to skip), the surface-grid and homogeneous-material

@@ -482,6 +482,7 @@ struct PropCfg {
     uint32_t max_rk_updates;
     bool do_cov;
     bool use_field_gradient;
+    bool use_mean_loss = true, use_eloss_gradient = false;
     // navigation
     NavCfg nav;
     double accumulated_error;
@@ -906,7 +907,59 @@ struct Engine {
         k = mul(qop, cross(t, b));
     }
 
-    // rk_stepper.hpp:273-530 (no volume material)
+    // The current volume's homogeneous material, or null (propagator.hpp:
+    // 320-325: vol.has_material() ? vol.material_parameters(pos) : nullptr)
+    const dtr_material_slab* volume_material() const {
+        if (nav.volume >= det.d->n_volumes) return nullptr;
+        const uint32_t m = det.d->volumes[nav.volume].material;
+        return m ? &det.d->materials[m - 1u] : nullptr;
+    }
+
+    // rk_stepper.hpp:157-189: d(qop)/ds = qop^3 E (-dE/ds) / q^2, zero in vacuum
+    double dqopds(double qop, const dtr_material_slab* vm) const {
+        if (!vm) return 0.0;
+        const double q = st.ptc.charge;
+        const double p = q / qop;
+        const double mass = st.ptc.mass;
+        const double E = std::sqrt(p * p + mass * mass);
+        const double stopping_power = bethe_bloch(*vm, make_rq(mass, qop, q));
+        return qop * qop * qop * E * stopping_power / (q * q);
+    }
+    // rk_stepper.hpp:194-231: d(dqop/ds)/dqop (Eq. 3.12 of JINST 4 P04016)
+    double d2qopdsdqop(double qop, const dtr_material_slab* vm) const {
+        if (!vm) return 0.0;
+        const double q = st.ptc.charge;
+        const double p = q / qop;
+        const double p2 = p * p;
+        const double mass = st.ptc.mass;
+        const double E2 = p2 + mass * mass;
+        const RQ r = make_rq(mass, qop, q);
+        const double g = -1.0 * bethe_bloch(*vm, r);
+        const double dgdqop = -1.0 * derive_bethe_bloch(*vm, r);
+        const double dq = dqopds(qop, vm);
+        return dq * (1.0 / qop * (3.0 - p2 / E2) + 1.0 / g * dgdqop);
+    }
+    // rk_stepper.hpp:147-155: the last stage's value after a step
+    double state_dqopds(const dtr_material_slab* vm) const {
+        if (st.path_length == 0.0) return dqopds(st.qop, vm);
+        return st.dqopds3;
+    }
+    // rk_stepper.hpp:534-565: (dqop/ds, qop) of stage i
+    void evaluate_dqopds(int i, double h, double dq_prev, const dtr_material_slab* vm, double& dq,
+                         double& q) const {
+        if (!vm) {
+            q = st.qop;
+            dq = 0.0;
+        } else if (cfg.use_mean_loss && i != 0) {
+            q = st.qop + h * dq_prev;
+            dq = dqopds(q, vm);
+        } else {
+            q = st.qop;
+            dq = dqopds(q, vm);
+        }
+    }
+
+    // rk_stepper.hpp:273-530
     bool step(double dist_to_next, bool do_reset) {
         const PropCfg& c = cfg;
         if (std::fabs(dist_to_next) < static_cast<double>(1.f * 1e-3f)) {
@@ -923,23 +976,24 @@ struct Engine {
         }
         const double min_step = std::min(std::fabs(st.step_size), c.min_stepsize);
         const V3 pos = st.pos;
+        const dtr_material_slab* vm = volume_material();
         V3 b_first = field.at(pos), b_mid{{0, 0, 0}}, b_last{{0, 0, 0}};
-        double qop[4];
+        double qop[4], dq[4];
         V3 t[4], k[4];
-        qop[0] = st.qop;
+        evaluate_dqopds(0, 0.0, 0.0, vm, dq[0], qop[0]);
         dtds_of(st.dir, b_first, 0.0, V3{{0, 0, 0}}, true, qop[0], t[0], k[0]);
         auto estimate_error = [&](double h) {
             const double h2 = h * h;
             const double half_h = h * 0.5;
             const V3 pos1 = add(add(pos, mul(half_h, t[0])), mul(h2 * 0.125, k[0]));
             b_mid = field.at(pos1);
-            qop[1] = st.qop;
+            evaluate_dqopds(1, half_h, dq[0], vm, dq[1], qop[1]);
             dtds_of(st.dir, b_mid, half_h, k[0], false, qop[1], t[1], k[1]);
-            qop[2] = st.qop;
+            evaluate_dqopds(2, half_h, dq[1], vm, dq[2], qop[2]);
             dtds_of(st.dir, b_mid, half_h, k[1], false, qop[2], t[2], k[2]);
             const V3 pos2 = add(add(pos, mul(h, t[0])), mul(h2 * 0.5, k[2]));
             b_last = field.at(pos2);
-            qop[3] = st.qop;
+            evaluate_dqopds(3, h, dq[2], vm, dq[3], qop[3]);
             dtds_of(st.dir, b_last, h, k[2], false, qop[3], t[3], k[3]);
             const double one_sixth = 1.0 / 6.0;
             const V3 d = add(sub(sub(k[0], k[1]), k[2]), k[3]);
@@ -957,26 +1011,39 @@ struct Engine {
             st.step_size = st.step_size * scaling(error);
         }
         st.dtds3 = k[3];
-        st.dqopds3 = 0.0;
+        st.dqopds3 = dq[3];
         const double max_step = st.constraint_size();
         if (std::fabs(st.step_size) > std::fabs(max_step)) st.step_size = max_step;
         if (std::fabs(st.step_size) < min_step) st.step_size = std::copysign(min_step, st.step_size);
         st.next_step_size = st.step_size * scaling(error);
         if (std::fabs(st.next_step_size) < c.min_stepsize)
             st.next_step_size = std::copysign(c.min_stepsize, st.next_step_size);
-        advance_track(t, k);
-        if (c.do_cov) advance_jacobian(t, k, qop, b_first, b_mid, b_last);
+        // the energy loss would bring the particle to rest within this step
+        // (its q/p diverges or changes sign): the reference integrates on into
+        // non-finite parameters; here the step is not taken and the track
+        // stops as at a surface that absorbs it
+        if (vm) {
+            const double h_6 = st.step_size * (1.0 / 6.0);
+            const double qn = st.qop + h_6 * (dq[0] + 2.0 * (dq[1] + dq[2]) + dq[3]);
+            if (!(std::isfinite(qn) && qn * st.qop > 0.0)) {
+                stopped = true;
+                return true;
+            }
+        }
+        advance_track(t, k, dq, vm);
+        if (c.do_cov) advance_jacobian(t, k, qop, b_first, b_mid, b_last, vm);
         st.n_steps++;
         return true;
     }
 
     // rk_stepper.hpp:594-629
-    void advance_track(const V3 t[4], const V3 k[4]) {
+    void advance_track(const V3 t[4], const V3 k[4], const double dq[4], const dtr_material_slab* vm) {
         const double h = st.step_size;
         const double h_6 = h * (1.0 / 6.0);
         st.pos = add(st.pos, mul(h, add(t[0], mul(h_6, add(add(k[0], k[1]), k[2])))));
         V3 d = add(st.dir, mul(h_6, add(add(k[0], mul(2.0, add(k[1], k[2]))), k[3])));
         st.dir = normalize(d);
+        if (vm) st.qop = st.qop + h_6 * (dq[0] + 2.0 * (dq[1] + dq[2]) + dq[3]);
         st.path_length += h;
         st.abs_path_length += std::fabs(h);
     }
@@ -1014,24 +1081,37 @@ struct Engine {
         }
     }
 
-    // rk_stepper.hpp:633-956 with dqop_n/dqop1 = 1 (no volume material) and
-    // without field gradient; J <- D(h) J with the ATL-SOFT-PUB-2009-002 blocks
+    // rk_stepper.hpp:633-956; J <- D(h) J with the ATL-SOFT-PUB-2009-002
+    // blocks.  dqop_n/dqop1 and d(qop)/d(qop1) (rk_stepper.hpp:767-800) are
+    // 1 unless use_eloss_gradient is set; the stage q/p values qop[n] carry
+    // the volume material's energy loss either way.
     void advance_jacobian(const V3 t[4], const V3 k[4], const double qop[4], const V3& b1,
-                          const V3& b2, const V3& b3) {
+                          const V3& b2, const V3& b3, const dtr_material_slab* vm) {
         (void)k;
         const double h = st.step_size;
         const double half_h = h * 0.5;
         // rk_stepper.hpp:664 uses the single-precision literal 1.f / 6.f here
         const double h_6 = h * static_cast<double>(1.f / 6.f);
+        double dqn[4] = {1.0, 1.0, 1.0, 1.0};
+        double dqopqop = 1.0;
+        if (cfg.use_eloss_gradient) {
+            const double d1 = d2qopdsdqop(qop[0], vm);
+            dqn[1] = 1.0 + half_h * d1;
+            const double d2 = d2qopdsdqop(qop[1], vm) * dqn[1];
+            dqn[2] = 1.0 + half_h * d2;
+            const double d3 = d2qopdsdqop(qop[2], vm) * dqn[2];
+            dqn[3] = 1.0 + h * d3;
+            const double d4 = d2qopdsdqop(qop[3], vm) * dqn[3];
+            dqopqop = 1.0 + h_6 * (d1 + 2.0 * (d2 + d3) + d4);
+        }
         // dk_n / dqop1
         V3 dk[4];
-        dk[0] = mul(1.0, cross(t[0], b1));
-        dk[1] = add(mul(1.0, cross(t[1], b2)), mul(qop[1] * half_h, cross(dk[0], b2)));
-        dk[2] = add(mul(1.0, cross(t[2], b2)), mul(qop[2] * half_h, cross(dk[1], b2)));
-        dk[3] = add(mul(1.0, cross(t[3], b3)), mul(qop[3] * h, cross(dk[2], b3)));
+        dk[0] = mul(dqn[0], cross(t[0], b1));
+        dk[1] = add(mul(dqn[1], cross(t[1], b2)), mul(qop[1] * half_h, cross(dk[0], b2)));
+        dk[2] = add(mul(dqn[2], cross(t[2], b2)), mul(qop[2] * half_h, cross(dk[1], b2)));
+        dk[3] = add(mul(dqn[3], cross(t[3], b3)), mul(qop[3] * h, cross(dk[2], b3)));
         const V3 dFdqop = mul(h * h_6, add(add(dk[0], dk[1]), dk[2]));
         const V3 dGdqop = mul(h_6, add(add(dk[0], mul(2.0, add(dk[1], dk[2]))), dk[3]));
-        const double dqopqop = 1.0;
         // dk_n / dt1
         double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         double dkdt[4][9];
@@ -1415,7 +1495,10 @@ struct Engine {
         B2F[7 * 6 + 4] = 1.0;
         // path correction
         const V3 k = dtds();
-        double p2f[8] = {st.dir[0], st.dir[1], st.dir[2], 0.0, k[0], k[1], k[2], 0.0};
+        // path_to_free_derivative with the volume's d(qop)/ds
+        // (parameter_updater.hpp:518-526, jacobian_engine.hpp:260-273)
+        double p2f[8] = {st.dir[0], st.dir[1], st.dir[2], 0.0, k[0], k[1], k[2],
+                         state_dqopds(volume_material())};
         double f2p[8];
         path_derivative(dest, st.pos, st.dir, k, f2p);
         double C[64];
@@ -1503,15 +1586,54 @@ struct Engine {
         r.Wmax = (2.0 * m_e * r.betaGamma * r.betaGamma) / (1.0 + 2.0 * r.gamma * mfrac + mfrac * mfrac);
         return r;
     }
+    // relativistic_quantities.hpp compute_delta_half: the approximate
+    // correction above beta gamma = 10 without density-effect data, the
+    // Sternheimer parameterisation with it
+    static double delta_half(const dtr_material_slab& m, const RQ& r) {
+        if (!m.has_density) {
+            if (r.betaGamma < 10.0) return 0.0;
+            const double plasma = (28.816 * 1e-9) * std::sqrt(1000.0 * (m.Z * m.molar_rho));
+            return std::log(r.betaGamma * plasma / m.mean_exc) - 0.5;
+        }
+        const double a = m.density[0], k = m.density[1], x0 = m.density[2], x1 = m.density[3],
+                     cbar = m.density[4];
+        const double x = std::log10(r.betaGamma);
+        double delta = 0.0;
+        if (x >= x0) {
+            delta = 2.0 * M_LN10 * x - cbar;
+            if (x < x1) delta += a * std::pow(x1 - x, k);
+        }
+        return 0.5 * delta;
+    }
+    // relativistic_quantities.hpp derive_delta_half: d(delta/2)/dqop
+    // (-1/qop without density-effect data at any beta gamma, as there)
+    static double derive_delta_half(const dtr_material_slab& m, const RQ& r) {
+        if (!m.has_density) return -1.0 / r.qOverP;
+        const double a = m.density[0], k = m.density[1], x0 = m.density[2], x1 = m.density[3];
+        const double x = std::log10(r.betaGamma);
+        double delta = 0.0;
+        if (x >= x0) {
+            delta = -2.0 / r.qOverP;
+            if (x < x1) delta += a * k / (r.qOverP * M_LN10) * std::pow(x1 - x, k - 1.0);
+        }
+        return 0.5 * delta;
+    }
+    // interaction.hpp:110-154 derive_bethe_bloch: d(-dE/dx)/dqop
+    static double derive_bethe_bloch(const dtr_material_slab& m, const RQ& r) {
+        const double K = 0.307075 * 1e-3 * 1e2;
+        const double eps_per_length = 0.5 * K * (m.Z * m.molar_rho) * r.q2OverBeta2;
+        if (eps_per_length <= 0.0) return 0.0;
+        const double first = 2.0 / (r.qOverP * r.gamma2) * bethe_bloch(m, r);
+        const double dA = -1.0 / (2.0 * r.qOverP) * (4.0 - r.Wmax / r.E);
+        const double dB = -2.0 * r.beta2 / (r.qOverP * r.gamma2);
+        const double dC = derive_delta_half(m, r);
+        return first + 2.0 * eps_per_length * (dA - dB - dC);
+    }
     static double bethe_bloch(const dtr_material_slab& m, const RQ& r) {
         const double K = 0.307075 * 1e-3 * 1e2;  // MeV cm^2 in GeV mm^2
         const double eps_per_length = 0.5 * K * (m.Z * m.molar_rho) * r.q2OverBeta2;
         if (eps_per_length <= 0.0) return 0.0;
-        double dhalf = 0.0;
-        if (r.betaGamma >= 10.0) {
-            const double plasma = (28.816 * 1e-9) * std::sqrt(1000.0 * (m.Z * m.molar_rho));
-            dhalf = std::log(r.betaGamma * plasma / m.mean_exc) - 0.5;
-        }
+        const double dhalf = delta_half(m, r);
         const double m_e = 0.51099895069 * 1e-3;
         const double u = 2.0 * m_e * r.betaGamma * r.betaGamma;
         const double A = 0.5 * std::log(u * r.Wmax / (m.mean_exc * m.mean_exc));
@@ -1801,20 +1923,8 @@ struct Engine {
 
 extern "C" {
 
-static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* field,
-                          const dtr_prop_config* cfg, const dtr_track_batch* in,
-                          dtr_result_batch* out, oracle_hits* hits, int n_threads,
-                          oracle_trace* trace, const uint32_t* seq, const uint32_t* seq_n,
-                          uint32_t seq_max, uint32_t* n_isect = nullptr) {
-    using namespace oracle;
-    if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
-        return DTR_ERROR_INVALID_ARGUMENT;
-    if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
-    // use_eloss_gradient: d(dqop/ds)/dqop is zero without volume material
-    // (rk_stepper.hpp:193-200), which these detectors do not have
-    if (!in->bound_params && !in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
-    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
-    PropCfg pc;
+static oracle::PropCfg make_prop_cfg(const dtr_prop_config* cfg) {
+    oracle::PropCfg pc;
     pc.min_stepsize = cfg->min_stepsize;
     pc.rk_error_tol = cfg->rk_error_tol;
     pc.step_constraint = cfg->step_constraint;
@@ -1822,6 +1932,8 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
     pc.max_rk_updates = cfg->max_rk_updates;
     pc.do_cov = cfg->do_covariance_transport != 0;
     pc.use_field_gradient = cfg->use_field_gradient != 0;
+    pc.use_mean_loss = cfg->use_mean_loss != 0;
+    pc.use_eloss_gradient = cfg->use_eloss_gradient != 0;
     pc.nav.min_mask_tol = cfg->min_mask_tolerance;
     pc.nav.max_mask_tol = cfg->max_mask_tolerance;
     pc.nav.mask_scalor = cfg->mask_tolerance_scalor;
@@ -1835,6 +1947,22 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
     pc.do_eloss = cfg->do_energy_loss != 0;
     pc.do_ms = cfg->do_multiple_scattering != 0;
     pc.use_material = cfg->use_material != 0;
+    pc.max_surfaces = cfg->max_surfaces;
+    return pc;
+}
+
+static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* field,
+                          const dtr_prop_config* cfg, const dtr_track_batch* in,
+                          dtr_result_batch* out, oracle_hits* hits, int n_threads,
+                          oracle_trace* trace, const uint32_t* seq, const uint32_t* seq_n,
+                          uint32_t seq_max, uint32_t* n_isect = nullptr) {
+    using namespace oracle;
+    if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if (!det || !field || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    if (!in->bound_params && !in->free_params) return DTR_ERROR_INVALID_ARGUMENT;
+    if (in->bound_params && !in->start_surface) return DTR_ERROR_INVALID_ARGUMENT;
+    PropCfg pc = make_prop_cfg(cfg);
     pc.max_surfaces = out->surfaces ? cfg->max_surfaces : 0u;
     const dtr_actor_states* actors = cfg->actors;
     if (actors) {
@@ -2001,6 +2129,67 @@ int oracle_material_probe(const dtr_material_slab* mat, double qop, double path,
     out[0] = Engine::bethe_bloch(*mat, r);
     out[1] = Engine::landau_sigma_qop(*mat, r, path, p.charge);
     out[2] = Engine::highland(x_over_x0, qop, p.charge, r.q2OverBeta2);
+    return DTR_SUCCESS;
+}
+
+int oracle_eloss_probe(const dtr_material_slab* mat, double qop, double out[3]) {
+    using oracle::Engine;
+    if (!mat || !out || qop == 0.0) return DTR_ERROR_INVALID_ARGUMENT;
+    Engine e;
+    e.st.ptc = oracle::muon_for(qop);
+    out[0] = Engine::bethe_bloch(*mat, Engine::make_rq(e.st.ptc.mass, qop, e.st.ptc.charge));
+    out[1] = e.dqopds(qop, mat);
+    out[2] = e.d2qopdsdqop(qop, mat);
+    return DTR_SUCCESS;
+}
+
+int oracle_rk_steps(const dtr_field_desc* field, const dtr_material_slab* vol_mat, const dtr_prop_config* cfg,
+                    const double fp[8], double ds, uint32_t n_steps, double* out, double* jac) {
+    using namespace oracle;
+    if (!field || !cfg || !fp || !out || ds == 0.0) return DTR_ERROR_INVALID_ARGUMENT;
+    // a one-volume detector that holds the volume material
+    dtr_volume vol{};
+    vol.material = vol_mat ? 1u : 0u;
+    dtr_detector_desc d{};
+    d.n_volumes = 1;
+    d.volumes = &vol;
+    d.n_materials = vol_mat ? 1u : 0u;
+    d.materials = vol_mat;
+    Engine e;
+    e.det.d = &d;
+    e.field.d = field;
+    e.cfg = make_prop_cfg(cfg);
+    e.nav.set_volume(0);
+    e.st.pos = {{fp[0], fp[1], fp[2]}};
+    e.st.time = fp[3];
+    e.st.dir = {{fp[4], fp[5], fp[6]}};
+    e.st.qop = fp[7];
+    e.st.ptc = muon_for(e.st.qop);
+    e.st.reset_J();
+    auto record = [&](uint32_t row) {
+        double* o = out + 5u * row;
+        o[0] = e.st.qop;
+        o[1] = e.dqopds(e.st.qop, vol_mat);
+        o[2] = e.d2qopdsdqop(e.st.qop, vol_mat);
+        o[3] = e.st.path_length;
+        o[4] = e.st.J[7 * 8 + 7];
+    };
+    record(0);
+    for (uint32_t i = 0; i < n_steps; ++i) {
+        e.step(ds, true);
+        record(i + 1u);
+    }
+    if (jac) {
+        for (int k = 0; k < 64; ++k) jac[k] = e.st.J[k];
+        jac[64 + 0] = e.st.pos[0];
+        jac[64 + 1] = e.st.pos[1];
+        jac[64 + 2] = e.st.pos[2];
+        jac[64 + 3] = e.st.time;
+        jac[64 + 4] = e.st.dir[0];
+        jac[64 + 5] = e.st.dir[1];
+        jac[64 + 6] = e.st.dir[2];
+        jac[64 + 7] = e.st.qop;
+    }
     return DTR_SUCCESS;
 }
 

@@ -82,6 +82,21 @@ int oracle_material_probe(const dtr_material_slab* mat, double qop,
 /* Ray-wire intersection probe (ray_line_intersector.hpp with a fixed mask
  * tolerance `tol`, overstep tolerance 0), for KAT tests:
  * out = {path, loc0, loc1, inside}.  Returns 1 for a valid intersection. */
+/* Energy loss of the muon hypothesis in a homogeneous material:
+ * out = {-dE/dx (Bethe-Bloch), d(qop)/ds, d(d(qop)/ds)/dqop}
+ * (interaction.hpp, rk_stepper.hpp:157-231). */
+int oracle_eloss_probe(const dtr_material_slab* mat, double qop, double out[3]);
+
+/* The oracle's stepper alone, as rk_stepper.cpp:220-263 drives it: from the
+ * free parameters fp[8], n_steps Runge-Kutta steps of ds (step size reset
+ * each time) in a homogeneous volume material (NULL: vacuum).  out[5 * (n_steps
+ * + 1)]: per row (start, then after each step) qop, d(qop)/ds, d2(qop)/ds dqop,
+ * path length and the transport Jacobian element J(qop, qop).  jac (NULL or
+ * 72 values): the final 8x8 free transport Jacobian, then the final free
+ * parameters. */
+int oracle_rk_steps(const dtr_field_desc* field, const dtr_material_slab* vol_mat, const dtr_prop_config* cfg,
+                    const double fp[8], double ds, uint32_t n_steps, double* out, double* jac);
+
 int oracle_line_intersect(const double ro[3], const double rd[3], const dtr_transform* trf,
                           const dtr_mask* mask, double tol, double out[4]);
 

@@ -47,12 +47,15 @@ def prefilter(detector, config, rays, ext_tol: float = 0.0):
     return int(out[0]), int(out[1]), int(out[2]), (int(miss[0]), int(miss[1]))
 
 
-def propagate(detector, field, config, free_params, mixed: bool = False, initial_cov=None) -> dict:
+def propagate(detector, field, config, free_params, mixed: bool = False, initial_cov=None, bound_params=None,
+              start_surface=None) -> dict:
     """The engine's lane loop (propagate_fast) over host arrays, FP64 or
     mixed policy; the result arrays of dtr_propagate_ex."""
     lib = load()
-    fp = np.ascontiguousarray(free_params, dtype=np.float64)
-    n = fp.shape[1]
+    fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
+    bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = None if start_surface is None else np.ascontiguousarray(start_surface, dtype=np.uint32)
+    n = fp.shape[1] if fp is not None else bp.shape[1]
     ms = max(int(config.max_surfaces), 1)
     out = {
         "bound_params": np.zeros((6, n)),
@@ -66,7 +69,9 @@ def propagate(detector, field, config, free_params, mixed: bool = False, initial
         "status": np.zeros(n, dtype=np.uint8),
     }
     cov = None if initial_cov is None else np.ascontiguousarray(initial_cov, dtype=np.float64)
-    tb = capi.TrackBatch(n, 0, fp.ctypes.data, cov.ctypes.data if cov is not None else None, None, None)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data if fp is not None else None,
+                         cov.ctypes.data if cov is not None else None,
+                         bp.ctypes.data if bp is not None else None, ss.ctypes.data if ss is not None else None)
     rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
         "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
         "n_steps", "n_trials", "status")])

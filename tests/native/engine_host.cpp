@@ -44,7 +44,14 @@ struct HostDetector {
     }
 };
 
+bool has_volume_material(const dtr_detector_desc& d) {
+    for (uint32_t k = 0; k < d.n_volumes; ++k)
+        if (d.volumes[k].material != 0u) return true;
+    return false;
+}
+
 bool has_ext(const dtr_detector_desc& d) {
+    if (has_volume_material(d)) return true;
     for (uint32_t k = 0; k < d.n_surfaces; ++k) {
         const dtr_surface& s = d.surfaces[k];
         if (s.shape >= DTR_LINE_SQUARE) return true;
@@ -165,12 +172,14 @@ __attribute__((visibility("default"))) int dtr_test_engine_propagate(const dtr_d
     const DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                        cfg->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                        out->n_trials, out->status};
-    const bool ext = has_ext(*d);
+    const bool ext = has_ext(*d), vol = has_volume_material(*d);
     if (precision == 0) {
-        if (ext) propagate_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, ti, r);
+        if (vol) propagate_all<WithVolumeMaterial<WithExtendedGeometry<PolicyFP64>>>(hd, field, dc, ti, r);
+        else if (ext) propagate_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, ti, r);
         else propagate_all<PolicyFP64>(hd, field, dc, ti, r);
     } else {
-        if (ext) propagate_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r);
+        if (vol) propagate_all<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(hd, field, dc, ti, r);
+        else if (ext) propagate_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r);
         else propagate_all<PolicyMixed>(hd, field, dc, ti, r);
     }
     return DTR_SUCCESS;

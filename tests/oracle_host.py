@@ -276,5 +276,30 @@ def material_probe(slab, qop: float, path: float, x_over_x0: float) -> np.ndarra
     return np.array(out[:])
 
 
+def eloss_probe(mat, qop: float) -> np.ndarray:
+    """{-dE/dx, d(qop)/ds, d(d(qop)/ds)/dqop} of the muon hypothesis."""
+    lib = load()
+    out = (C.c_double * 3)()
+    capi.check(lib.oracle_eloss_probe(C.byref(mat), C.c_double(qop), out), "oracle_eloss_probe")
+    return np.array(out[:])
+
+
+def rk_steps(field, vol_mat, config, free_params, ds: float, n_steps: int):
+    """The oracle's stepper alone: n_steps steps of ds (reset each time)
+    in the volume material vol_mat (None: vacuum).  Returns (rows
+    [n_steps + 1, 5] of qop, dqopds, d2qopdsdqop, path length, J(qop, qop);
+    the final free Jacobian [8, 8]; the final free parameters [8])."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    out = np.zeros((n_steps + 1, 5))
+    jac = np.zeros(72)
+    cfg = config.to_c()
+    capi.check(lib.oracle_rk_steps(C.byref(field.desc), C.byref(vol_mat) if vol_mat is not None else None,
+                                   C.byref(cfg), C.c_void_p(fp.ctypes.data), C.c_double(ds),
+                                   C.c_uint32(n_steps), C.c_void_p(out.ctypes.data),
+                                   C.c_void_p(jac.ctypes.data)), "oracle_rk_steps")
+    return out, jac[:64].reshape(8, 8), jac[64:]
+
+
 def default_threads() -> int:
     return int(os.environ.get("ORACLE_THREADS", "0"))

@@ -34,7 +34,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(capi.Mask) == 64
     assert C.sizeof(capi.Surface) == 32
     assert C.sizeof(capi.Volume) == 28
-    assert C.sizeof(capi.MaterialSlab) == 80
+    assert C.sizeof(capi.MaterialSlab) == 120
     assert C.sizeof(capi.Grid) == 56
     assert C.sizeof(capi.WireChamberConfig) == 48
 

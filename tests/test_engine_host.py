@@ -213,3 +213,34 @@ def test_host_lane_loop_wire_chamber_matches_the_oracle():
     got = engine_host.propagate(det, field, cfg, fp)
     _same_sequences(got, ref)
     assert _rel(got["bound_params"], ref["bound_params"]) < 1e-9
+
+
+# ------------------------------------------- volume material, host lane loop --
+def volume_material_case(name="cesium_iodide_with_ded", n=64, seed=3, **cfg_kw):
+    det = da.Detector.telescope([100.0 * i for i in range(1, 11)], 500.0, 500.0, 0.15, 60.0, volume_material=name)
+    field = da.MagneticField.constant(1.0 * da.TESLA, 0.0, 0.5 * da.TESLA)
+    cfg = da.PropagationConfig(**cfg_kw)
+    fp = da.generate_tracks("beam", n, seed, p_min=1.0, p_max=10.0, sigma_theta=0.1)
+    cov = np.zeros((21, n))
+    for k, v in {0: 1e-3, 6: 1e-3, 11: 1e-6, 15: 1e-6, 18: 1e-6, 20: 1.0}.items():
+        cov[k] = v
+    return det, field, cfg, fp, cov
+
+
+@pytest.mark.parametrize("kw", [{}, {"use_eloss_gradient": True}, {"use_mean_loss": False},
+                                {"use_eloss_gradient": True, "estimate_scattering_noise": False}],
+                         ids=["default", "eloss_gradient", "no_mean_loss", "gradient_no_noise"])
+def test_host_lane_loop_volume_material_matches_the_oracle(kw):
+    det, field, cfg, fp, cov = volume_material_case(**kw)
+    ref = oracle_host.propagate(det, field, cfg, fp, initial_cov=cov)
+    assert np.all(ref["n_surfaces"] == 10)
+    got = engine_host.propagate(det, field, cfg, fp, initial_cov=cov)
+    _same_sequences(got, ref)
+    assert np.array_equal(got["n_steps"], ref["n_steps"])
+    assert _rel(got["bound_params"], ref["bound_params"]) < 1e-9
+    scale = np.maximum(np.abs(ref["bound_cov"]).max(axis=0), 1e-30)
+    assert float(np.max(np.abs(got["bound_cov"] - ref["bound_cov"]) / scale)) < 1e-8
+    mixed = engine_host.propagate(det, field, cfg, fp, mixed=True, initial_cov=cov)
+    _same_sequences(mixed, ref)
+    assert _rel(mixed["bound_params"], ref["bound_params"]) < 1e-4
+    assert float(np.max(np.abs(mixed["bound_cov"] - ref["bound_cov"]) / scale)) < 1e-3

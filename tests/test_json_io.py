@@ -80,7 +80,7 @@ def material(det, s):
     if s.material == 0xFFFFFFFF:
         return None
     m = det.desc.materials[s.material]
-    return tuple(getattr(m, f) for f, _ in m._fields_)
+    return tuple(tuple(v) if hasattr(v, "_length_") else v for v in (getattr(m, f) for f, _ in m._fields_))
 
 
 def bin_entries(det, g):
