@@ -27,6 +27,8 @@ ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 ENGINE_HOST_SRC = ROOT / "tests" / "native" / "engine_host.cpp"
 ENGINE_HOST_LIB = ROOT / "tests" / "native" / "libengine_host.so"
+ADAPTER_TEST_SRC = ROOT / "tests" / "native" / "adapter_test.cpp"
+ADAPTER_TEST_BIN = ROOT / "tests" / "native" / "adapter_test"
 
 ARCH = os.environ.get("DETRAY_AMD_ARCH", "gfx950")
 
@@ -121,10 +123,27 @@ def build_engine_host(out: Path = ENGINE_HOST_LIB) -> Path:
     return out
 
 
+def build_adapter_test(out: Path = ADAPTER_TEST_BIN) -> Path:
+    """The compiled test of the C++20 adapter (include/detray_amd.hpp): a
+    host program (g++ -std=c++20) linked against libdetray_amd.so, the
+    oracle and the HIP runtime (device buffers of the GPU checks)."""
+    cxx = os.environ.get("CXX", "g++")
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    tmp = out.with_suffix(".tmp")
+    _run([cxx, "-std=c++20", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", f"-I{CSRC}",
+          f"-I{ORACLE_DIR}", f"-I{rocm / 'include'}", str(ADAPTER_TEST_SRC), "-o", str(tmp),
+          f"-L{PKG}", "-ldetray_amd", f"-L{ORACLE_DIR}", "-loracle", f"-L{rocm / 'lib'}", "-lamdhip64",
+          f"-Wl,-rpath,{PKG}", f"-Wl,-rpath,{ORACLE_DIR}", f"-Wl,-rpath,{rocm / 'lib'}",
+          "-Wl,-rpath,$ORIGIN/../../detray_amd", "-Wl,-rpath,$ORIGIN/../../oracle"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_library()
     build_oracle()
     build_engine_host()
+    build_adapter_test()
 
 
 if __name__ == "__main__":

@@ -175,8 +175,260 @@ const Value* data_of(const Value& root, const char* what) {
 }
 
 // ---------------------------------------------------------------------------
-// Reading
+// Reading: files and in-memory views become payloads, payloads tables
 // ---------------------------------------------------------------------------
+// Payloads (io/include/detray/io/frontend/payloads.hpp:86-330), the fields
+// the tables are built from.  Files are parsed into them (parse_*), in-memory
+// views copied into them (from_view); ingest_* then does what the reference's
+// readers and builders make of them.
+struct MaskP {
+    uint64_t shape = 0;
+    uint64_t volume_link = 0;
+    std::vector<double> boundaries;
+};
+struct SurfaceP {
+    uint64_t type = 0;
+    dtr_transform trf{};
+    std::vector<MaskP> masks;
+    bool has_index = false;
+    uint64_t index = 0;
+};
+struct VolumeP {
+    dtr_transform placement{};
+    std::vector<SurfaceP> surfaces;
+};
+struct AxisP {
+    uint64_t binning = 0, bounds = 0, bins = 0;
+    std::vector<double> edges;
+};
+struct BinP {
+    std::vector<uint64_t> loc_index;
+    std::vector<uint64_t> content;
+};
+struct GridP {
+    uint64_t owner = 0, type = 0;
+    std::vector<AxisP> axes;
+    std::vector<BinP> bins;
+};
+struct MaterialP {
+    uint64_t type = 0, surface = 0;
+    bool has_index = false;
+    uint64_t index = 0;
+    double thickness = 0.0;
+    double params[7] = {};
+};
+struct MaterialVolumeP {
+    uint64_t volume = 0;
+    std::vector<MaterialP> entries;
+};
+struct DetectorP {
+    std::vector<VolumeP> volumes;
+    std::vector<GridP> grids;
+    std::vector<MaterialVolumeP> materials;
+};
+
+// --- files -> payloads --------------------------------------------------------
+SurfaceP parse_surface(const Value& s, const std::string& where) {
+    SurfaceP out;
+    out.type = as_index(need(s, "type", where), where + ".type");
+    out.trf = read_transform(need(s, "transform", where), where + ".transform");
+    std::vector<const Value*> masks;
+    if (const Value* m = s.find("mask")) {
+        masks.push_back(m);  // single-mask files
+    } else if (const Value* ms = s.find("masks")) {
+        for (const Value& m : as_array(*ms, where + ".masks")) masks.push_back(&m);
+    }
+    for (size_t k = 0; k < masks.size(); ++k) {
+        const std::string mw = where + ".masks[" + std::to_string(k) + "]";
+        MaskP m;
+        m.shape = as_index(need(*masks[k], "shape", mw), mw + ".shape");
+        for (const Value& b : as_array(need(*masks[k], "boundaries", mw), mw + ".boundaries"))
+            m.boundaries.push_back(as_number(b, mw + ".boundaries"));
+        m.volume_link = as_index(need(*masks[k], "volume_link", mw), mw + ".volume_link");
+        out.masks.push_back(std::move(m));
+    }
+    if (const Value* ic = s.find("index_in_coll")) {
+        out.has_index = true;
+        out.index = as_index(*ic, where + ".index_in_coll");
+    }
+    return out;
+}
+
+void parse_geometry(const Value& root, DetectorP& P) {
+    const Value* data = data_of(root, "geometry");
+    const Value* jv = data ? data->find("volumes") : nullptr;
+    if (!jv) return;
+    for (size_t vi = 0; vi < as_array(*jv, "volumes").size(); ++vi) {
+        const Value& v = jv->a[vi];
+        const std::string where = "volume " + std::to_string(vi);
+        VolumeP V;
+        V.placement = read_transform(need(v, "transform", where), where + ".transform");
+        const auto& js = as_array(need(v, "surfaces", where), where + ".surfaces");
+        for (size_t k = 0; k < js.size(); ++k)
+            V.surfaces.push_back(parse_surface(js[k], where + " surface " + std::to_string(k)));
+        P.volumes.push_back(std::move(V));
+    }
+}
+
+void parse_grids(const Value& root, DetectorP& P) {
+    const Value* data = data_of(root, "grids");
+    const Value* jg = data ? data->find("grids") : nullptr;
+    if (!jg) return;
+    for (const Value& entry : as_array(*jg, "grids")) {
+        const auto& list = as_array(need(entry, "grid_data", "grids entry"), "grid_data");
+        for (const Value& g : list) {
+            GridP G;
+            G.owner = as_index(need(g, "owner_link", "grid"), "grid.owner_link");
+            const std::string where = "grid of volume " + std::to_string(G.owner);
+            const Value& link = need(g, "grid_link", where);
+            G.type = as_index(need(link, "type", where + ".grid_link"), where + ".grid_link.type");
+            const auto& axes = as_array(need(g, "axes", where), where + ".axes");
+            for (size_t k = 0; k < axes.size(); ++k) {
+                const std::string aw = where + ".axes[" + std::to_string(k) + "]";
+                AxisP A;
+                A.binning = as_index(need(axes[k], "binning", aw), aw + ".binning");
+                A.bounds = as_index(need(axes[k], "bounds", aw), aw + ".bounds");
+                A.bins = as_index(need(axes[k], "bins", aw), aw + ".bins");
+                for (const Value& e : as_array(need(axes[k], "edges", aw), aw + ".edges"))
+                    A.edges.push_back(as_number(e, aw + ".edges"));
+                G.axes.push_back(std::move(A));
+            }
+            for (const Value& bin : as_array(need(g, "bins", where), where + ".bins")) {
+                BinP B;
+                for (const Value& i : as_array(need(bin, "loc_index", where), where + ".loc_index"))
+                    B.loc_index.push_back(as_index(i, where + ".loc_index"));
+                for (const Value& c : as_array(need(bin, "content", where), where + ".content"))
+                    B.content.push_back(as_index(c, where + ".content"));
+                G.bins.push_back(std::move(B));
+            }
+            P.grids.push_back(std::move(G));
+        }
+    }
+}
+
+void parse_material(const Value& root, DetectorP& P) {
+    const Value* data = data_of(root, "material");
+    const Value* jv = data ? data->find("volumes") : nullptr;
+    if (!jv) return;
+    for (const Value& mv : as_array(*jv, "material volumes")) {
+        MaterialVolumeP M;
+        M.volume = as_index(need(mv, "volume_link", "material volume"), "material volume_link");
+        const std::string where = "material of volume " + std::to_string(M.volume);
+        const Value* list = mv.find("surface_material");
+        if (!list) list = mv.find("material_slabs");  // older files
+        if (list) {
+            const auto& entries = as_array(*list, where);
+            for (size_t k = 0; k < entries.size(); ++k) {
+                const std::string ew = where + " entry " + std::to_string(k);
+                const Value& e = entries[k];
+                MaterialP m;
+                m.type = as_index(need(e, "type", ew), ew + ".type");
+                numbers(need(need(e, "material", ew), "params", ew), 7, m.params, ew + ".material.params");
+                m.thickness = as_number(need(e, "thickness", ew), ew + ".thickness");
+                m.surface = as_index(need(e, "surface_idx", ew), ew + ".surface_idx");
+                if (const Value* ic = e.find("index_in_coll")) {
+                    m.has_index = true;
+                    m.index = as_index(*ic, ew + ".index_in_coll");
+                }
+                M.entries.push_back(m);
+            }
+        }
+        P.materials.push_back(std::move(M));
+    }
+}
+
+// --- in-memory view -> payloads -------------------------------------------------
+constexpr uint64_t kNotGiven = ~0ull;
+
+dtr_transform view_transform(const dtr_view_transform& t) {
+    dtr_transform out{};
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) out.r[row * 3 + col] = t.rot[col * 3 + row];
+    for (int k = 0; k < 3; ++k) out.t[k] = t.tr[k];
+    return out;
+}
+
+void from_view(const dtr_detector_view& v, DetectorP& P) {
+    if (v.n_volumes && !v.volumes) bad("view: volumes missing");
+    if (v.n_grids && !v.grids) bad("view: grids missing");
+    if (v.n_materials && !v.materials) bad("view: materials missing");
+    for (uint32_t vi = 0; vi < v.n_volumes; ++vi) {
+        const dtr_view_volume& vv = v.volumes[vi];
+        const std::string where = "volume " + std::to_string(vi);
+        if (vv.n_surfaces && !vv.surfaces) bad(where + ": surfaces missing");
+        VolumeP V;
+        V.placement = view_transform(vv.transform);
+        for (uint32_t k = 0; k < vv.n_surfaces; ++k) {
+            const dtr_view_surface& vs = vv.surfaces[k];
+            const std::string sw = where + " surface " + std::to_string(k);
+            if (vs.n_masks && !vs.masks) bad(sw + ": masks missing");
+            SurfaceP S;
+            S.type = vs.type;
+            S.trf = view_transform(vs.transform);
+            for (uint32_t m = 0; m < vs.n_masks; ++m) {
+                const dtr_view_mask& vm = vs.masks[m];
+                if (vm.n_boundaries > 8u) bad(sw + ": more than 8 mask boundaries");
+                MaskP M;
+                M.shape = vm.shape;
+                M.volume_link = vm.volume_link;
+                M.boundaries.assign(vm.boundaries, vm.boundaries + vm.n_boundaries);
+                S.masks.push_back(std::move(M));
+            }
+            S.has_index = vs.index_in_coll != kNotGiven;
+            S.index = vs.index_in_coll;
+            V.surfaces.push_back(std::move(S));
+        }
+        P.volumes.push_back(std::move(V));
+    }
+    for (uint32_t gi = 0; gi < v.n_grids; ++gi) {
+        const dtr_view_grid& vg = v.grids[gi];
+        const std::string where = "grid of volume " + std::to_string(vg.owner_link);
+        if (vg.n_axes > 2u) bad(where + ": more than 2 axes");
+        if (vg.n_bins && (!vg.loc_index || !vg.content_offset)) bad(where + ": bins missing");
+        GridP G;
+        G.owner = vg.owner_link;
+        G.type = vg.type;
+        for (uint32_t k = 0; k < vg.n_axes; ++k) {
+            AxisP A;
+            A.binning = vg.axes[k].binning;
+            A.bounds = vg.axes[k].bounds;
+            A.bins = vg.axes[k].bins;
+            A.edges = {vg.axes[k].edges[0], vg.axes[k].edges[1]};
+            G.axes.push_back(std::move(A));
+        }
+        for (uint32_t b = 0; b < vg.n_bins; ++b) {
+            BinP B;
+            B.loc_index = {vg.loc_index[2u * b], vg.loc_index[2u * b + 1u]};
+            const uint64_t lo = vg.content_offset[b], hi = vg.content_offset[b + 1u];
+            if (hi < lo || (hi > lo && !vg.content)) bad(where + ": bad content offsets");
+            B.content.assign(vg.content + lo, vg.content + hi);
+            G.bins.push_back(std::move(B));
+        }
+        P.grids.push_back(std::move(G));
+    }
+    // material entries grouped by volume in first-seen order
+    for (uint32_t k = 0; k < v.n_materials; ++k) {
+        const dtr_view_material& vm = v.materials[k];
+        MaterialVolumeP* M = nullptr;
+        for (auto& mv : P.materials)
+            if (mv.volume == vm.volume_link) M = &mv;
+        if (!M) {
+            P.materials.emplace_back();
+            M = &P.materials.back();
+            M->volume = vm.volume_link;
+        }
+        MaterialP m;
+        m.type = vm.type;
+        m.surface = vm.surface;
+        m.has_index = vm.index_in_coll != kNotGiven;
+        m.index = vm.index_in_coll;
+        m.thickness = vm.thickness;
+        for (int i = 0; i < 7; ++i) m.params[i] = vm.params[i];
+        M->entries.push_back(m);
+    }
+}
+
+// --- payloads -> volumes ----------------------------------------------------------
 struct SurfaceIn {
     uint8_t sf_id = 0;
     uint8_t shape = 0;     // dtr_mask_shape
@@ -200,22 +452,15 @@ struct VolumeIn {
     std::vector<char> mat_set[2];
 };
 
-SurfaceIn read_surface(const Value& s, const std::string& where) {
+SurfaceIn ingest_surface(const SurfaceP& s, const std::string& where) {
     SurfaceIn out;
-    const uint64_t type = as_index(need(s, "type", where), where + ".type");
-    if (type > 2u) bad(where + ": surface type " + std::to_string(type) + " is not portal/sensitive/passive");
-    out.sf_id = static_cast<uint8_t>(type);
-    out.trf = read_transform(need(s, "transform", where), where + ".transform");
-    std::vector<const Value*> masks;
-    if (const Value* m = s.find("mask")) {
-        masks.push_back(m);  // single-mask files
-    } else if (const Value* ms = s.find("masks")) {
-        for (const Value& m : as_array(*ms, where + ".masks")) masks.push_back(&m);
-    }
-    if (masks.empty()) bad(where + ": surface without mask");
-    if (masks.size() > 255u) unsupported(where + ": more than 255 masks on one surface");
+    if (s.type > 2u) bad(where + ": surface type " + std::to_string(s.type) + " is not portal/sensitive/passive");
+    out.sf_id = static_cast<uint8_t>(s.type);
+    out.trf = s.trf;
+    if (s.masks.empty()) bad(where + ": surface without mask");
+    if (s.masks.size() > 255u) unsupported(where + ": more than 255 masks on one surface");
     // the factory is chosen by the first mask's shape; all masks take it
-    const uint64_t sid = as_index(need(*masks[0], "shape", where), where + ".shape");
+    const uint64_t sid = s.masks[0].shape;
     if (sid >= kIoShapes) bad(where + ": unknown mask shape id " + std::to_string(sid));
     size_t n_bounds = 0;
     switch (sid) {
@@ -260,13 +505,16 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
                         " (the tables hold annulus2, rectangle2, trapezoid2, ring2, cylinder2, drift_cell "
                         "and straw_tube masks)");
     }
-    for (size_t k = 0; k < masks.size(); ++k) {
+    for (size_t k = 0; k < s.masks.size(); ++k) {
         const std::string mw = where + ".masks[" + std::to_string(k) + "]";
+        const MaskP& m = s.masks[k];
+        if (m.boundaries.size() != n_bounds)
+            bad(mw + ".boundaries: expected " + std::to_string(n_bounds) + " numbers");
         std::array<double, 7> b{};
-        numbers(need(*masks[k], "boundaries", mw), n_bounds, b.data(), mw + ".boundaries");
+        for (size_t i = 0; i < n_bounds; ++i) b[i] = m.boundaries[i];
         out.bounds.push_back(b);
         // checked against the volume count once every volume is read
-        out.links.push_back(as_index(need(*masks[k], "volume_link", mw), mw + ".volume_link"));
+        out.links.push_back(m.volume_link);
     }
     if (out.shape == DTR_CONCENTRIC_CYLINDER2D) {
         const double dz = out.trf.t[2];
@@ -276,28 +524,23 @@ SurfaceIn read_surface(const Value& s, const std::string& where) {
         }
         out.trf = identity();
     }
-    if (const Value* ic = s.find("index_in_coll")) {
-        out.has_index = true;
-        out.index = as_index(*ic, where + ".index_in_coll");
-    }
+    out.has_index = s.has_index;
+    out.index = s.index;
     return out;
 }
 
-void read_geometry(const Value& root, std::vector<VolumeIn>& vols) {
-    const Value* data = data_of(root, "geometry");
-    const Value* jv = data ? data->find("volumes") : nullptr;
-    if (!jv || as_array(*jv, "volumes").empty()) bad("geometry file: no volumes");
-    for (size_t vi = 0; vi < jv->a.size(); ++vi) {
-        const Value& v = jv->a[vi];
+void ingest_geometry(const DetectorP& P, std::vector<VolumeIn>& vols) {
+    if (P.volumes.empty()) bad("geometry file: no volumes");
+    for (size_t vi = 0; vi < P.volumes.size(); ++vi) {
+        const VolumeP& v = P.volumes[vi];
         const std::string where = "volume " + std::to_string(vi);
         VolumeIn V;
-        V.placement = read_transform(need(v, "transform", where), where + ".transform");
-        const auto& js = as_array(need(v, "surfaces", where), where + ".surfaces");
-        if (js.empty()) bad(where + ": volume without surfaces");
+        V.placement = v.placement;
+        if (v.surfaces.empty()) bad(where + ": volume without surfaces");
         std::vector<SurfaceIn> in;
-        in.reserve(js.size());
-        for (size_t k = 0; k < js.size(); ++k)
-            in.push_back(read_surface(js[k], where + " surface " + std::to_string(k)));
+        in.reserve(v.surfaces.size());
+        for (size_t k = 0; k < v.surfaces.size(); ++k)
+            in.push_back(ingest_surface(v.surfaces[k], where + " surface " + std::to_string(k)));
         const size_t n = in.size();
         size_t n_indexed = 0;
         for (const auto& s : in) n_indexed += s.has_index ? 1u : 0u;
@@ -327,104 +570,83 @@ void read_geometry(const Value& root, std::vector<VolumeIn>& vols) {
     }
 }
 
-void read_grids(const Value& root, std::vector<VolumeIn>& vols) {
-    const Value* data = data_of(root, "grids");
-    const Value* jg = data ? data->find("grids") : nullptr;
-    if (!jg) return;
-    for (const Value& entry : as_array(*jg, "grids")) {
-        const auto& list = as_array(need(entry, "grid_data", "grids entry"), "grid_data");
-        for (const Value& g : list) {
-            const uint64_t vi = as_index(need(g, "owner_link", "grid"), "grid.owner_link");
-            const std::string where = "grid of volume " + std::to_string(vi);
-            if (vi >= vols.size()) bad(where + ": the volume is not in the geometry");
-            VolumeIn& V = vols[vi];
-            const Value& link = need(g, "grid_link", where);
-            const uint64_t type = as_index(need(link, "type", where + ".grid_link"), where + ".grid_link.type");
-            if (type >= kIoAccels) bad(where + ": unknown grid type id " + std::to_string(type));
-            dtr_grid gr{};
-            unsigned want0, want1;
-            if (type == kIoConcCyl2Grid) {
-                gr.type = DTR_GRID_CYLINDER;
-                want0 = kCircular;
-                want1 = kClosed;
-            } else if (type == kIoPolar2Grid) {
-                gr.type = DTR_GRID_RING;
-                want0 = kClosed;
-                want1 = kCircular;
-            } else {
-                unsupported(where + ": grid type " + std::to_string(type) +
-                            " (the tables hold concentric_cylinder2 and polar2 grids)");
-            }
-            if (V.has_grid) unsupported(where + ": more than one surface grid in a volume");
-            const auto& axes = as_array(need(g, "axes", where), where + ".axes");
-            if (axes.size() != 2u) bad(where + ": a 2D grid needs 2 axes");
-            uint32_t nb[2];
-            double lo[2], hi[2];
-            for (int k = 0; k < 2; ++k) {
-                const std::string aw = where + ".axes[" + std::to_string(k) + "]";
-                const uint64_t binning = as_index(need(axes[k], "binning", aw), aw + ".binning");
-                const uint64_t bounds = as_index(need(axes[k], "bounds", aw), aw + ".bounds");
-                if (binning != kRegular) unsupported(aw + ": irregular binning");
-                if (bounds != (k == 0 ? want0 : want1))
-                    unsupported(aw + ": axis bounds " + std::to_string(bounds) + " (the tables hold " +
-                                (gr.type == DTR_GRID_CYLINDER ? "circular phi x closed z" : "closed r x circular phi") + ")");
-                const uint64_t bins = as_index(need(axes[k], "bins", aw), aw + ".bins");
-                if (bins == 0u || bins > (1u << 20)) bad(aw + ": bad number of bins");
-                const auto& edges = as_array(need(axes[k], "edges", aw), aw + ".edges");
-                if (edges.size() < 2u) bad(aw + ": an axis span needs 2 edges");
-                nb[k] = static_cast<uint32_t>(bins);
-                lo[k] = as_number(edges.front(), aw + ".edges");
-                hi[k] = as_number(edges.back(), aw + ".edges");
-            }
-            gr.n0 = nb[0];
-            gr.n1 = nb[1];
-            gr.min0 = lo[0];
-            gr.max0 = hi[0];
-            gr.min1 = lo[1];
-            gr.max1 = hi[1];
-            if (static_cast<uint64_t>(gr.n0) * gr.n1 > (1ull << 26)) unsupported(where + ": more than 2^26 bins");
-            V.bins.assign(static_cast<size_t>(gr.n0) * gr.n1, {});
-            size_t cap = 1;
-            for (const Value& bin : as_array(need(g, "bins", where), where + ".bins")) {
-                const auto& li = as_array(need(bin, "loc_index", where), where + ".loc_index");
-                if (li.size() != 2u) bad(where + ": a bin needs 2 local indices");
-                const uint64_t i0 = as_index(li[0], where + ".loc_index"), i1 = as_index(li[1], where + ".loc_index");
-                if (i0 >= gr.n0 || i1 >= gr.n1)
-                    bad(where + ": bin (" + std::to_string(i0) + ", " + std::to_string(i1) + ") out of bounds");
-                std::vector<uint32_t>& entries = V.bins[static_cast<size_t>(i0) * gr.n1 + i1];
-                for (const Value& c : as_array(need(bin, "content", where), where + ".content")) {
-                    const uint64_t sf = as_index(c, where + ".content");
-                    if (static_cast<uint32_t>(sf) == kInvalid) continue;  // invalid entries are skipped
-                    if (sf >= V.surfaces.size()) bad(where + ": surface " + std::to_string(sf) + " not in the volume");
-                    entries.push_back(static_cast<uint32_t>(sf));
-                }
-                cap = std::max(cap, entries.size());
-            }
-            // the bins are stored with the largest occupancy as capacity
-            if (cap > 255u) unsupported(where + ": more than 255 surfaces in a bin");
-            if (static_cast<uint64_t>(gr.n0) * gr.n1 * cap > (1ull << 26))
-                unsupported(where + ": more than 2^26 bin entries");
-            gr.capacity = static_cast<uint8_t>(cap);
-            V.has_grid = true;
-            V.grid = gr;
+void ingest_grids(const DetectorP& P, std::vector<VolumeIn>& vols) {
+    for (const GridP& g : P.grids) {
+        const uint64_t vi = g.owner;
+        const std::string where = "grid of volume " + std::to_string(vi);
+        if (vi >= vols.size()) bad(where + ": the volume is not in the geometry");
+        VolumeIn& V = vols[vi];
+        if (g.type >= kIoAccels) bad(where + ": unknown grid type id " + std::to_string(g.type));
+        dtr_grid gr{};
+        unsigned want0, want1;
+        if (g.type == kIoConcCyl2Grid) {
+            gr.type = DTR_GRID_CYLINDER;
+            want0 = kCircular;
+            want1 = kClosed;
+        } else if (g.type == kIoPolar2Grid) {
+            gr.type = DTR_GRID_RING;
+            want0 = kClosed;
+            want1 = kCircular;
+        } else {
+            unsupported(where + ": grid type " + std::to_string(g.type) +
+                        " (the tables hold concentric_cylinder2 and polar2 grids)");
         }
+        if (V.has_grid) unsupported(where + ": more than one surface grid in a volume");
+        if (g.axes.size() != 2u) bad(where + ": a 2D grid needs 2 axes");
+        uint32_t nb[2];
+        double lo[2], hi[2];
+        for (int k = 0; k < 2; ++k) {
+            const std::string aw = where + ".axes[" + std::to_string(k) + "]";
+            const AxisP& a = g.axes[k];
+            if (a.binning != kRegular) unsupported(aw + ": irregular binning");
+            if (a.bounds != (k == 0 ? want0 : want1))
+                unsupported(aw + ": axis bounds " + std::to_string(a.bounds) + " (the tables hold " +
+                            (gr.type == DTR_GRID_CYLINDER ? "circular phi x closed z" : "closed r x circular phi") + ")");
+            if (a.bins == 0u || a.bins > (1u << 20)) bad(aw + ": bad number of bins");
+            if (a.edges.size() < 2u) bad(aw + ": an axis span needs 2 edges");
+            nb[k] = static_cast<uint32_t>(a.bins);
+            lo[k] = a.edges.front();
+            hi[k] = a.edges.back();
+        }
+        gr.n0 = nb[0];
+        gr.n1 = nb[1];
+        gr.min0 = lo[0];
+        gr.max0 = hi[0];
+        gr.min1 = lo[1];
+        gr.max1 = hi[1];
+        if (static_cast<uint64_t>(gr.n0) * gr.n1 > (1ull << 26)) unsupported(where + ": more than 2^26 bins");
+        V.bins.assign(static_cast<size_t>(gr.n0) * gr.n1, {});
+        size_t cap = 1;
+        for (const BinP& bin : g.bins) {
+            if (bin.loc_index.size() != 2u) bad(where + ": a bin needs 2 local indices");
+            const uint64_t i0 = bin.loc_index[0], i1 = bin.loc_index[1];
+            if (i0 >= gr.n0 || i1 >= gr.n1)
+                bad(where + ": bin (" + std::to_string(i0) + ", " + std::to_string(i1) + ") out of bounds");
+            std::vector<uint32_t>& entries = V.bins[static_cast<size_t>(i0) * gr.n1 + i1];
+            for (const uint64_t sf : bin.content) {
+                if (static_cast<uint32_t>(sf) == kInvalid) continue;  // invalid entries are skipped
+                if (sf >= V.surfaces.size()) bad(where + ": surface " + std::to_string(sf) + " not in the volume");
+                entries.push_back(static_cast<uint32_t>(sf));
+            }
+            cap = std::max(cap, entries.size());
+        }
+        // the bins are stored with the largest occupancy as capacity
+        if (cap > 255u) unsupported(where + ": more than 255 surfaces in a bin");
+        if (static_cast<uint64_t>(gr.n0) * gr.n1 * cap > (1ull << 26))
+            unsupported(where + ": more than 2^26 bin entries");
+        gr.capacity = static_cast<uint8_t>(cap);
+        V.has_grid = true;
+        V.grid = gr;
     }
 }
 
-void read_material(const Value& root, std::vector<VolumeIn>& vols) {
-    const Value* data = data_of(root, "material");
-    const Value* jv = data ? data->find("volumes") : nullptr;
-    if (!jv) return;
-    for (const Value& mv : as_array(*jv, "material volumes")) {
-        const uint64_t vi = as_index(need(mv, "volume_link", "material volume"), "material volume_link");
+void ingest_material(const DetectorP& P, std::vector<VolumeIn>& vols) {
+    for (const MaterialVolumeP& mv : P.materials) {
+        const uint64_t vi = mv.volume;
         const std::string where = "material of volume " + std::to_string(vi);
         if (vi >= vols.size()) bad(where + ": the volume is not in the geometry");
         VolumeIn& V = vols[vi];
-        const Value* list = mv.find("surface_material");
-        if (!list) list = mv.find("material_slabs");  // older files
-        if (!list) continue;
-        const auto& entries = as_array(*list, where);
-        const size_t n = entries.size();
+        const size_t n = mv.entries.size();
         if (n == 0) continue;
         struct Entry {
             uint64_t sf;
@@ -435,21 +657,13 @@ void read_material(const Value& root, std::vector<VolumeIn>& vols) {
         std::vector<Entry> es;
         for (size_t k = 0; k < n; ++k) {
             const std::string ew = where + " entry " + std::to_string(k);
-            const Value& e = entries[k];
-            const uint64_t type = as_index(need(e, "type", ew), ew + ".type");
-            if (type != kIoSlab && type != kIoRod)
-                bad(ew + ": material type " + std::to_string(type) + " is not homogeneous");
-            double p[7];
-            numbers(need(need(e, "material", ew), "params", ew), 7, p, ew + ".material.params");
-            const double t = as_number(need(e, "thickness", ew), ew + ".thickness");
-            Entry en{as_index(need(e, "surface_idx", ew), ew + ".surface_idx"), false, 0,
-                     make_slab(p[0], p[1], p[2], p[3], p[4], t)};
+            const MaterialP& e = mv.entries[k];
+            if (e.type != kIoSlab && e.type != kIoRod)
+                bad(ew + ": material type " + std::to_string(e.type) + " is not homogeneous");
+            const double* p = e.params;
+            Entry en{e.surface, e.has_index, e.index, make_slab(p[0], p[1], p[2], p[3], p[4], e.thickness)};
             // a rod's "thickness" is its radius (material_rod.hpp:59-60)
-            if (type == kIoRod) en.slab.kind = DTR_MATERIAL_ROD;
-            if (const Value* ic = e.find("index_in_coll")) {
-                en.has_index = true;
-                en.index = as_index(*ic, ew + ".index_in_coll");
-            }
+            if (e.type == kIoRod) en.slab.kind = DTR_MATERIAL_ROD;
             es.push_back(en);
         }
         // pairing by offset from the smallest surface index
@@ -853,6 +1067,36 @@ Value write_material(const dtr_detector_desc& d, const char* name) {
     return root;
 }
 
+// payloads -> validated tables, exported into `out`
+void build_tables(const DetectorP& P, dtr_detector_desc* out) {
+    std::vector<VolumeIn> vols;
+    ingest_geometry(P, vols);
+    ingest_grids(P, vols);
+    ingest_material(P, vols);
+    Tables T;
+    assemble(vols, T);
+    dtr_detector_desc view{};
+    view.n_volumes = static_cast<uint32_t>(T.volumes.size());
+    view.n_surfaces = static_cast<uint32_t>(T.surfaces.size());
+    view.n_transforms = static_cast<uint32_t>(T.transforms.size());
+    view.n_masks = static_cast<uint32_t>(T.masks.size());
+    view.n_materials = static_cast<uint32_t>(T.materials.size());
+    view.n_grids = static_cast<uint32_t>(T.grids.size());
+    view.n_bf = static_cast<uint32_t>(T.bf.size());
+    view.n_bins = static_cast<uint32_t>(T.bins.size());
+    view.volumes = T.volumes.data();
+    view.surfaces = T.surfaces.data();
+    view.transforms = T.transforms.data();
+    view.masks = T.masks.data();
+    view.materials = T.materials.data();
+    view.grids = T.grids.data();
+    view.bf_surfaces = T.bf.data();
+    view.grid_bins = T.bins.data();
+    const std::string why = validate_desc(view);
+    if (!why.empty()) bad("detector tables read from the files: " + why);
+    export_tables(T, out);
+}
+
 void message(char* out, size_t n, const std::string& m) {
     if (!out || n == 0) return;
     const size_t k = std::min(n - 1, m.size());
@@ -872,32 +1116,34 @@ int dtr_read_detector_json(const char* geometry_file, const char* grids_file, co
         return DTR_ERROR_INVALID_ARGUMENT;
     }
     try {
-        std::vector<VolumeIn> vols;
-        read_geometry(load(geometry_file, "geometry"), vols);
-        if (grids_file) read_grids(load(grids_file, "surface grids"), vols);
-        if (material_file) read_material(load(material_file, "homogeneous material"), vols);
-        Tables T;
-        assemble(vols, T);
-        dtr_detector_desc view{};
-        view.n_volumes = static_cast<uint32_t>(T.volumes.size());
-        view.n_surfaces = static_cast<uint32_t>(T.surfaces.size());
-        view.n_transforms = static_cast<uint32_t>(T.transforms.size());
-        view.n_masks = static_cast<uint32_t>(T.masks.size());
-        view.n_materials = static_cast<uint32_t>(T.materials.size());
-        view.n_grids = static_cast<uint32_t>(T.grids.size());
-        view.n_bf = static_cast<uint32_t>(T.bf.size());
-        view.n_bins = static_cast<uint32_t>(T.bins.size());
-        view.volumes = T.volumes.data();
-        view.surfaces = T.surfaces.data();
-        view.transforms = T.transforms.data();
-        view.masks = T.masks.data();
-        view.materials = T.materials.data();
-        view.grids = T.grids.data();
-        view.bf_surfaces = T.bf.data();
-        view.grid_bins = T.bins.data();
-        const std::string why = validate_desc(view);
-        if (!why.empty()) bad("detector tables read from the files: " + why);
-        export_tables(T, out);
+        DetectorP P;
+        parse_geometry(load(geometry_file, "geometry"), P);
+        if (grids_file) parse_grids(load(grids_file, "surface grids"), P);
+        if (material_file) parse_material(load(material_file, "homogeneous material"), P);
+        build_tables(P, out);
+        return DTR_SUCCESS;
+    } catch (const IoError& e) {
+        message(msg, msg_size, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        message(msg, msg_size, "out of host memory");
+        return DTR_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        message(msg, msg_size, e.what());
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+}
+
+int dtr_flatten_detector(const dtr_detector_view* view, dtr_detector_desc* out, char* msg, size_t msg_size) {
+    message(msg, msg_size, "");
+    if (!view || !out) {
+        message(msg, msg_size, "detector view and output tables are required");
+        return DTR_ERROR_INVALID_ARGUMENT;
+    }
+    try {
+        DetectorP P;
+        from_view(*view, P);
+        build_tables(P, out);
         return DTR_SUCCESS;
     } catch (const IoError& e) {
         message(msg, msg_size, e.msg);

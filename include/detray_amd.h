@@ -680,6 +680,84 @@ DTR_API int dtr_read_detector_json(const char* geometry_file, const char* grids_
                                    const char* material_file, dtr_detector_desc* out,
                                    char* message, size_t message_size);
 
+/* In-memory detector view: the reference's IO payloads
+ * (io/include/detray/io/frontend/payloads.hpp:86-330) as plain arrays, so a
+ * detector held by the reference (detray::detector<metadata>) reaches the
+ * tables without files: io::geometry_writer::convert, the surface-grid
+ * writer's convert and homogeneous_material_writer::convert make the payloads
+ * of a detector (geometry_writer.hpp, surface_grid_writer.hpp,
+ * homogeneous_material_writer.hpp), the C++ adapter (detray_amd.hpp,
+ * flatten()) copies them into these records and dtr_flatten_detector builds
+ * the tables with the semantics of dtr_read_detector_json.  Enumerations keep
+ * the reference's numbering (io::shape_id, io::accel_id, io::material_id,
+ * axis::binning/bounds/label, surface_id). */
+typedef struct dtr_view_mask {      /* mask_payload */
+    uint32_t shape;                 /* io::shape_id */
+    uint32_t n_boundaries;          /* entries of boundaries[] in use */
+    uint64_t volume_link;           /* single_link_payload: volume index, 2^64-1 = leaves the world */
+    double boundaries[8];
+} dtr_view_mask;
+
+typedef struct dtr_view_transform { /* transform_payload */
+    double tr[3];
+    double rot[9];                  /* column major */
+} dtr_view_transform;
+
+typedef struct dtr_view_surface {   /* surface_payload */
+    uint32_t type;                  /* surface_id: 0 portal, 1 sensitive, 2 passive */
+    uint32_t n_masks;
+    uint64_t index_in_coll;         /* 2^64-1: not given */
+    dtr_view_transform transform;
+    const dtr_view_mask* masks;
+} dtr_view_surface;
+
+typedef struct dtr_view_volume {    /* volume_payload */
+    dtr_view_transform transform;
+    uint32_t n_surfaces;
+    uint32_t pad;
+    const dtr_view_surface* surfaces;
+} dtr_view_volume;
+
+typedef struct dtr_view_axis {      /* axis_payload (regular axes: the first and last edge) */
+    uint32_t binning, bounds, label, bins;
+    double edges[2];
+} dtr_view_axis;
+
+typedef struct dtr_view_grid {      /* grid_payload<std::size_t, io::accel_id> */
+    uint64_t owner_link;            /* volume index */
+    uint32_t type;                  /* grid_link.type: io::accel_id */
+    uint32_t n_axes;
+    dtr_view_axis axes[2];
+    uint32_t n_bins;
+    uint32_t pad;
+    const uint32_t* loc_index;      /* [n_bins][2] grid_bin_payload::loc_index */
+    const uint64_t* content_offset; /* [n_bins + 1]: bin k holds content[offset[k] .. offset[k+1]) */
+    const uint64_t* content;        /* volume-local surface indices */
+} dtr_view_grid;
+
+typedef struct dtr_view_material {  /* surface_material_payload of material_volume_payload */
+    uint64_t volume_link;           /* material_volume_payload::volume_link */
+    uint64_t surface;               /* volume-local surface index (surface_material_payload::surface) */
+    uint64_t index_in_coll;         /* 2^64-1: not given */
+    uint32_t type;                  /* io::material_id: slab or rod */
+    uint32_t pad;
+    double thickness;               /* a rod's radius */
+    double params[7];               /* material_param_payload: X0, L0, Ar, Z, mass density, molar density, state */
+} dtr_view_material;
+
+typedef struct dtr_detector_view {
+    uint32_t n_volumes, n_grids, n_materials, pad;
+    const dtr_view_volume* volumes;     /* detector_payload::volumes */
+    const dtr_view_grid* grids;         /* detector_grids_payload::grids, any order */
+    const dtr_view_material* materials; /* detector_homogeneous_material_payload, any order */
+} dtr_detector_view;
+
+/* Tables of an in-memory detector view (above), as dtr_read_detector_json
+ * makes them of the same payloads read from files (same errors and
+ * messages).  Free the tables with dtr_host_detector_free. */
+DTR_API int dtr_flatten_detector(const dtr_detector_view* view, dtr_detector_desc* out, char* message,
+                                 size_t message_size);
+
 /* Write detector tables as detray JSON files (geometry; surface grids and
  * homogeneous material, NULL to skip), as io::write_detector with the json
  * geometry_writer, surface_grid_writer and homogeneous_material_writer
