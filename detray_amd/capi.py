@@ -357,6 +357,8 @@ _SIGNATURES = {
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),
     "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(DetectorDesc),
                                          C.c_char_p, C.c_size_t]),
+    # the view records are filled by the C++ adapter (include/detray_amd.hpp)
+    "dtr_flatten_detector": (C.c_int, [C.c_void_p, C.POINTER(DetectorDesc), C.c_char_p, C.c_size_t]),
     "dtr_write_detector_json": (C.c_int, [C.POINTER(DetectorDesc), C.c_char_p, C.POINTER(C.c_char_p),
                                           C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
     "dtr_sample_solenoid": (C.c_int, [C.c_double, C.c_double, C.c_double,
