@@ -17,9 +17,12 @@
 namespace dtr {
 
 // kExt: detectors with wires, annuli, placed cylinders, rods, multi-surface
-// bins or rings with material
+// bins or rings with material.  Two waves per SIMD (256 registers per lane,
+// VGPRs and AGPRs together): left to itself the compiler allocates past 256
+// registers for this kernel and runs one wave per SIMD, which hides less of
+// its memory latency.
 template <bool kExt>
-__global__ void __launch_bounds__(256) propagate_fp64_kernel(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) propagate_fp64_kernel(
     DevDetector det, DevField field, DevConfig cfg, const double* __restrict__ fp,
     const double* __restrict__ init_cov, const double* __restrict__ bound0, const uint32_t* __restrict__ start_sf,
     DevResults out, uint32_t n) {
