@@ -199,6 +199,8 @@ class ResultBatch(C.Structure):
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 SCHEDULE_GRID, SCHEDULE_PERSISTENT = 0, 1
 NAVIGATION_WAVE, NAVIGATION_LANE, NAVIGATION_GATHER = 0, 1, 2
+# dtr_nav_direction
+DTR_DIRECTION_FORWARD, DTR_DIRECTION_BACKWARD = 0, 1
 
 
 class LaunchConfig(C.Structure):
@@ -270,7 +272,7 @@ class AdjointBatch(C.Structure):
 class SurfaceSequence(C.Structure):
     _fields_ = [
         ("max_surfaces", C.c_uint32),
-        ("pad", C.c_uint32),
+        ("direction", C.c_uint32),
         ("surfaces", C.c_void_p),
         ("n_surfaces", C.c_void_p),
     ]

@@ -620,6 +620,7 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     if (!seq || !launch) return DTR_ERROR_INVALID_ARGUMENT;
     if (in->n == 0u) return DTR_SUCCESS;
     if (seq->max_surfaces == 0u || !seq->surfaces || !seq->n_surfaces) return DTR_ERROR_INVALID_ARGUMENT;
+    if (seq->direction > DTR_DIRECTION_BACKWARD) return DTR_ERROR_INVALID_ARGUMENT;
     dtr_launch_config L = *launch;
     if (L.block_size == 0u) L.block_size = 64u;
     if (L.block_size % 64u != 0u || L.block_size > 256u) return DTR_ERROR_INVALID_ARGUMENT;
@@ -630,7 +631,8 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
                       c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
                       out->n_trials, out->status};
-    const dtr::SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, det->n_surfaces};
+    const dtr::SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, det->n_surfaces,
+                          seq->direction == DTR_DIRECTION_BACKWARD ? 1u : 0u};
     const uint32_t block = L.block_size;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     if (det->volume_material) {

@@ -10,13 +10,22 @@
 namespace dtr {
 
 // Surface sequences of a batch: surfaces [max][n] (index k of track i at
-// k * n + i), counts [n].
+// k * n + i), counts [n].  backward: the targets are taken from the last
+// entry to the first (direct_navigator.hpp:79-94, 125-151 with
+// navigation::direction::e_backward).
 struct SeqView {
     const uint32_t* sf;
     const uint32_t* count;
     uint32_t max;
     uint32_t n_surfaces;  // surfaces in the detector: larger indices abort
+    uint32_t backward;
 };
+
+// The k-th target of track i's sequence of len entries.
+DTR_AI uint32_t seq_target(const SeqView& sq, uint64_t N, uint32_t track, uint32_t k, uint32_t len) {
+    const uint32_t e = sq.backward ? len - 1u - k : k;
+    return sq.sf[static_cast<uint64_t>(e) * N + track];
+}
 
 // The intersection config of direct_navigator::update (direct_navigator.hpp:
 // 233-236): {inf, inf, inf, path_tolerance, -inf}.
@@ -56,7 +65,7 @@ DTR_AI bool direct_update(const FastDetector& det, const DevConfig& cfg, const D
         tk.status = (c.sf_id(1) == DTR_PORTAL) ? kOnPortal : kOnObject;
         c.copy(0, 1);
         next_ext++;
-        if (next_ext < len) c.s(1) = sq.sf[static_cast<uint64_t>(next_ext) * N + track];
+        if (next_ext < len) c.s(1) = seq_target(sq, N, track, next_ext, len);
     } else {
         tk.status = kTowards;
     }
@@ -88,11 +97,12 @@ DTR_AI void propagate_direct(const FastDetector& det, const DevField& field, con
     const uint32_t cnt = sq.count[track];
     const uint32_t len = cnt < sq.max ? cnt : sq.max;
     uint32_t next_ext = 0u;
+    if (sq.backward) tk.nav_dir = -1.0;
     // direct_navigator::init (direct_navigator.hpp:196-206): target = first
-    // surface of the sequence, next = last = 1
+    // surface of the sequence in the navigation direction, next = last = 1
     tk.next = 1;
     tk.last = 1;
-    if (len > 0u) c.s(1) = sq.sf[track];
+    if (len > 0u) c.s(1) = seq_target(sq, N, track, 0u, len);
     direct_update<P>(det, cfg, dcfg, tk, c, sq, N, track, next_ext, len);
     lane.start_end();
     for (;;) {

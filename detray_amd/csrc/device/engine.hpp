@@ -422,6 +422,11 @@ struct FastTrack {
     uint16_t volume;
     bool path_limit_hit, overflow, stopped, heartbeat;
     bool momentum_hit, target_hit;  // the optional aborters fired
+    // navigation::direction as a sign (navigation_state.hpp:178): +1, or -1
+    // for a backward direct navigation (intersections along -dir, negative
+    // steps, energy gained in material); +1 in every other kernel, where it
+    // folds away
+    double nav_dir;
     // free track parameters (FP64 state)
     V3 pos, dir;
     double time, qop;
@@ -473,7 +478,15 @@ struct FastEngine {
     DTR_AI int n_cand() const { return tk.last - tk.next + 1; }
     DTR_AI bool exhausted() const { return n_cand() == 0; }
     DTR_AI int begin_idx() const { return (on_surface() && tk.next >= 1) ? tk.next - 1 : tk.next; }
-    DTR_AI double nav_dist() const { return c.p(tk.next); }
+    // the track's navigation direction sign; NavLite (the searches of the
+    // caching navigator) always looks forward
+    DTR_AI double dir_sign() const {
+        if constexpr (requires { tk.nav_dir; }) return tk.nav_dir;
+        else return 1.0;
+    }
+    // navigation_state::operator() (navigation_state.hpp:355-359): the signed
+    // distance to the target
+    DTR_AI double nav_dist() const { return dir_sign() * c.p(tk.next); }
     DTR_AI void set_high_trust() { tk.trust = tk.trust < kHigh ? tk.trust : kHigh; }
     DTR_AI void set_fair_trust() { tk.trust = tk.trust < kFair ? tk.trust : kFair; }
     DTR_AI bool is_portal(uint32_t s) const { return det.sf[s].sf_id == DTR_PORTAL; }
@@ -506,9 +519,10 @@ struct FastEngine {
     // intersection_update of a placed cylinder: the smaller solution
     DTR_AI bool cyl2_update(const NavSurface& rec, double& path, V3& glob) const {
         double sol[2];
-        if (Engine::cyl2_intersect(tk.pos, tk.dir, rec_trf(rec), rec.m[0], sol) == 0) return false;
+        const V3 rd = mul(dir_sign(), tk.dir);
+        if (Engine::cyl2_intersect(tk.pos, rd, rec_trf(rec), rec.m[0], sol) == 0) return false;
         path = sol[0];
-        glob = add(tk.pos, mul(sol[0], tk.dir));
+        glob = add(tk.pos, mul(sol[0], rd));
         return fabs(sol[0]) < 10.0 * 1000.0;
     }
 
@@ -587,14 +601,17 @@ struct FastEngine {
         double path, z = 0.0;
         V3 glob{0.0, 0.0, 0.0};
         bool valid;
+        // the ray along the navigation direction (navigation_functions.hpp:
+        // 119-122)
+        const V3 rd = mul(dir_sign(), tk.dir);
         if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
-            valid = Engine::cyl_intersect(tk.pos, tk.dir, rec.m[0], overstep, path, z);
+            valid = Engine::cyl_intersect(tk.pos, rd, rec.m[0], overstep, path, z);
         } else if (P::kExt && is_line(rec.shape)) {
-            valid = Engine::line_intersect(tk.pos, tk.dir, rec_trf(rec), path, glob);
+            valid = Engine::line_intersect(tk.pos, rd, rec_trf(rec), path, glob);
         } else if (P::kExt && rec.shape == DTR_CYLINDER2D) {
             valid = cyl2_update(rec, path, glob);
         } else {
-            valid = plane_rec(tk.pos, tk.dir, rec, path, glob);
+            valid = plane_rec(tk.pos, rd, rec, path, glob);
         }
         if (!valid) return false;
         if (rec.n_masks == 1) {
@@ -1877,7 +1894,9 @@ struct FastEngine {
         if (cfg.do_eloss) {
             const double massd = kMuonMass, qd = track_charge(tk.qop);
             const double p = qd / tk.bound[4];
-            const double next_e = sqrt(massd * massd + p * p) - copysign(e_loss, 1.0);
+            // update_qop (pointwise_material_interactor.hpp:240-262): an
+            // energy gain going backward
+            const double next_e = sqrt(massd * massd + p * p) - copysign(e_loss, tk.nav_dir);
             const double next_p = (massd < next_e) ? sqrt(next_e * next_e - massd * massd) : 0.0;
             const double next_qop = (qd != 0.0) ? qd / next_p : 1.0 / next_p;
             tk.bound[4] = (next_p == 0.0) ? kMax : next_qop;
@@ -2110,6 +2129,7 @@ struct Lane {
         tk.stopped = false;
         tk.momentum_hit = false;
         tk.target_hit = false;
+        tk.nav_dir = 1.0;
         tk.n_trace = 0;
         tk.step_size = 0.0;
         tk.h_stage = 0.0;

@@ -613,13 +613,15 @@ class Propagator:
     def propagate_direct(self, detector: Detector, field: MagneticField, sequence, n_sequence, tracks=None,
                          initial_cov=None, out: Optional[PropagationResult] = None, stream=None,
                          bound_params=None, start_surface=None,
-                         actors: Optional[Actors] = None) -> PropagationResult:
+                         actors: Optional[Actors] = None, backward: bool = False) -> PropagationResult:
         """Propagate with the direct navigator (``dtr_propagate_direct``):
         every track follows its surface sequence ``sequence`` [m, n] int32
         (``n_sequence`` [n] entries each, e.g. the ``surfaces`` /
-        ``n_surfaces`` of an earlier propagation).  The launch's precision,
-        block size and order are used (grid schedule; without a launch:
-        mixed precision, block 64)."""
+        ``n_surfaces`` of an earlier propagation), from the first entry to
+        the last, or with ``backward`` from the last to the first against
+        the track direction (the reference's navigation direction
+        e_backward).  The launch's precision, block size and order are used
+        (grid schedule; without a launch: mixed precision, block 64)."""
         import torch
 
         tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
@@ -638,7 +640,9 @@ class Propagator:
                                          or launch.order.shape != (n,) or not launch.order.is_contiguous()
                                          or launch.order.device != device):
             raise ValueError("launch.order must be a contiguous int32 CUDA tensor [n] on the tracks' device")
-        seq = capi.SurfaceSequence(int(sequence.shape[0]), 0, sequence.data_ptr(), n_sequence.data_ptr())
+        seq = capi.SurfaceSequence(int(sequence.shape[0]),
+                                   capi.DTR_DIRECTION_BACKWARD if backward else capi.DTR_DIRECTION_FORWARD,
+                                   sequence.data_ptr(), n_sequence.data_ptr())
         if stream is None:
             stream = torch.cuda.current_stream(device).cuda_stream
         lc = launch.to_c(None)

@@ -489,9 +489,14 @@ DTR_API int dtr_sort_tracks_static(const dtr_sort_config* cfg, const double* fre
 /* Surface sequences of a batch, DEVICE memory: surfaces [max_surfaces][n]
  * (entry k of track i at k * n + i, the surface_sequencer layout of
  * dtr_result_batch), n_surfaces [n]. */
+enum dtr_nav_direction {            /* navigation::direction (navigation_state.hpp:178) */
+    DTR_DIRECTION_FORWARD = 0,      /* e_forward */
+    DTR_DIRECTION_BACKWARD = 1      /* e_backward */
+};
+
 typedef struct dtr_surface_sequence {
     uint32_t max_surfaces;
-    uint32_t pad;
+    uint32_t direction;             /* dtr_nav_direction */
     const uint32_t* surfaces;
     const uint32_t* n_surfaces;
 } dtr_surface_sequence;
@@ -501,9 +506,16 @@ typedef struct dtr_surface_sequence {
  * track steps to the next surface of its sequence, intersected with
  * infinite mask tolerance (the target is always hit); the navigation exits
  * after the last surface and aborts if a target cannot be reached or is not
- * a surface of the detector.  Forward direction.  Same stepper, actor chain
- * and results as dtr_propagate_ex; the launch gives the precision, block
- * size and track order (grid schedule only, no use_field_gradient). */
+ * a surface of the detector.  seq->direction DTR_DIRECTION_BACKWARD is the
+ * reference's navigation().set_direction(e_backward) + reset()
+ * (tests/integration_tests/cpu/propagator/propagation_direct_navigator.cpp:
+ * 184-190): the targets are the sequence's entries from the last to the
+ * first, intersected against the track direction, the stepper takes
+ * negative steps and the material interactor gives the energy back
+ * (pointwise_material_interactor.hpp:240-262); a track usually starts bound
+ * on the sequence's last surface.  Same stepper, actor chain and results as
+ * dtr_propagate_ex; the launch gives the precision, block size and track
+ * order (grid schedule only, no use_field_gradient). */
 DTR_API int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field,
                                  const dtr_prop_config* cfg, const dtr_track_batch* in,
                                  const dtr_surface_sequence* seq, dtr_result_batch* out,

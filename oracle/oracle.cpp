@@ -677,7 +677,14 @@ struct Engine {
     uint64_t dseq_stride = 0;
     uint32_t dseq_len = 0, dseq_next = 0;
 
-    uint32_t dseq_at(uint32_t k) const { return dseq[static_cast<uint64_t>(k) * dseq_stride]; }
+    // the k-th target: entry k of the sequence going forward, entry
+    // len - 1 - k going backward (direct_navigator.hpp:79-94 reset() starts
+    // at the first or the last entry and advance() moves in the navigation
+    // direction, :125-151)
+    uint32_t dseq_at(uint32_t k) const {
+        const uint32_t e = nav.direction > 0 ? k : dseq_len - 1u - k;
+        return dseq[static_cast<uint64_t>(e) * dseq_stride];
+    }
 
     // direct_navigator::update (direct_navigator.hpp:210-298)
     bool direct_update() {
@@ -1750,7 +1757,8 @@ struct Engine {
                     // update_qop (pointwise_material_interactor.hpp:240-262)
                     const double mass = st.ptc.mass, q = st.ptc.charge;
                     const double p = q / up.bound[4];
-                    const double next_e = std::sqrt(mass * mass + p * p) - std::copysign(e_loss, 1.0);
+                    const double next_e =
+                        std::sqrt(mass * mass + p * p) - std::copysign(e_loss, static_cast<double>(nav.direction));
                     const double next_p = (mass < next_e) ? std::sqrt(next_e * next_e - mass * mass) : 0.0;
                     const double next_qop = (q != 0.0) ? q / next_p : 1.0 / next_p;
                     up.bound[4] = (next_p == 0.0) ? kMax : next_qop;
@@ -1955,7 +1963,7 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
                           const dtr_prop_config* cfg, const dtr_track_batch* in,
                           dtr_result_batch* out, oracle_hits* hits, int n_threads,
                           oracle_trace* trace, const uint32_t* seq, const uint32_t* seq_n,
-                          uint32_t seq_max, uint32_t* n_isect = nullptr) {
+                          uint32_t seq_max, uint32_t* n_isect = nullptr, int8_t direction = 1) {
     using namespace oracle;
     if (hits && (hits->max_hits == 0u || !hits->surface || !hits->loc || !hits->n_hits))
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -2026,6 +2034,10 @@ static int propagate_impl(const dtr_detector_desc* det, const dtr_field_desc* fi
             e.dseq = seq + i;
             e.dseq_stride = static_cast<uint64_t>(n);
             e.dseq_len = seq_n[i] < seq_max ? seq_n[i] : seq_max;
+            // navigation::direction (navigation_state.hpp:178): intersections
+            // along direction * dir, signed step sizes, and the material
+            // interactor's energy loss turned into a gain going backward
+            e.nav.direction = direction;
         }
         if (hits) {
             e.hit_sf = hits->surface + i;
@@ -2102,10 +2114,11 @@ int oracle_propagate_counted(const dtr_detector_desc* det, const dtr_field_desc*
 int oracle_propagate_direct(const dtr_detector_desc* det, const dtr_field_desc* field,
                             const dtr_prop_config* cfg, const dtr_track_batch* in,
                             const uint32_t* sequence, const uint32_t* n_sequence, uint32_t max_sequence,
-                            dtr_result_batch* out, int n_threads) {
+                            uint32_t direction, dtr_result_batch* out, int n_threads) {
     if (in && in->n && (!sequence || !n_sequence || max_sequence == 0u)) return DTR_ERROR_INVALID_ARGUMENT;
+    if (direction > DTR_DIRECTION_BACKWARD) return DTR_ERROR_INVALID_ARGUMENT;
     return propagate_impl(det, field, cfg, in, out, nullptr, n_threads, nullptr, sequence, n_sequence,
-                          max_sequence);
+                          max_sequence, nullptr, direction == DTR_DIRECTION_BACKWARD ? int8_t{-1} : int8_t{1});
 }
 
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz, double* b) {

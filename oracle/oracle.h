@@ -62,11 +62,13 @@ int oracle_propagate_hits(const dtr_detector_desc* det, const dtr_field_desc* fi
 
 /* Propagation with the direct navigator (direct_navigator.hpp): every track
  * follows its surface sequence [max_sequence][n] (n_sequence[n] entries) instead
- * of searching the detector; same stepper and actors. */
+ * of searching the detector; same stepper and actors.  direction
+ * DTR_DIRECTION_BACKWARD visits the sequence from its last entry to its first,
+ * stepping against the track direction. */
 int oracle_propagate_direct(const dtr_detector_desc* det, const dtr_field_desc* field,
                             const dtr_prop_config* cfg, const dtr_track_batch* in,
                             const uint32_t* sequence, const uint32_t* n_sequence, uint32_t max_sequence,
-                            dtr_result_batch* out, int n_threads);
+                            uint32_t direction, dtr_result_batch* out, int n_threads);
 
 /* Evaluate the field at n points (host), for field tests. */
 int oracle_field_at(const dtr_field_desc* field, uint32_t n, const double* xyz,

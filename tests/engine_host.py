@@ -80,3 +80,43 @@ def propagate(detector, field, config, free_params, mixed: bool = False, initial
                                              C.byref(tb), C.byref(rb), C.c_int(1 if mixed else 0)),
                "dtr_test_engine_propagate")
     return out
+
+
+def propagate_direct(detector, field, config, free_params, sequence, n_sequence, mixed: bool = False,
+                     backward: bool = False, bound_params=None, start_surface=None, initial_cov=None) -> dict:
+    """The engine's direct-navigator loop (propagate_direct) over host
+    arrays, forward or backward, FP64 or mixed policy."""
+    lib = load()
+    fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
+    bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = None if start_surface is None else np.ascontiguousarray(start_surface, dtype=np.uint32)
+    cov = None if initial_cov is None else np.ascontiguousarray(initial_cov, dtype=np.float64)
+    seq = np.ascontiguousarray(sequence, dtype=np.uint32)
+    nseq = np.ascontiguousarray(n_sequence, dtype=np.uint32)
+    n = fp.shape[1] if fp is not None else bp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+    }
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data if fp is not None else None,
+                         cov.ctypes.data if cov is not None else None,
+                         bp.ctypes.data if bp is not None else None, ss.ctypes.data if ss is not None else None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    direction = capi.DTR_DIRECTION_BACKWARD if backward else capi.DTR_DIRECTION_FORWARD
+    sq = capi.SurfaceSequence(int(seq.shape[0]), direction, seq.ctypes.data, nseq.ctypes.data)
+    cfg = config.to_c()
+    capi.check(lib.dtr_test_engine_propagate_direct(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
+                                                    C.byref(tb), C.byref(sq), C.byref(rb),
+                                                    C.c_int(1 if mixed else 0)),
+               "dtr_test_engine_propagate_direct")
+    return out

@@ -579,7 +579,8 @@ int run_gpu() {
         const Results got = out.fetch();
         Results dref(n, ms);
         dtr_result_batch db = dref.host_batch();
-        check(oracle_propagate_direct(&toy.desc(), &fd, &cfg, &host_in, ref.sf.data(), ref.n_sf.data(), ms, &db, 0),
+        check(oracle_propagate_direct(&toy.desc(), &fd, &cfg, &host_in, ref.sf.data(), ref.n_sf.data(), ms,
+                                      DTR_DIRECTION_FORWARD, &db, 0),
               "oracle_propagate_direct");
         expect(same_sequences(got, dref, n), "direct navigator: sequences equal the oracle's");
         const double e = max_rel(got.bound, dref.bound);

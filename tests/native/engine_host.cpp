@@ -8,10 +8,13 @@
 //  * dtr_test_engine_propagate: the engine's per-lane propagate loop
 //    (propagate_fast) run track by track on the host, with the same tables
 //    and results as dtr_propagate_ex, to compare with the oracle on the CPU.
+//  * dtr_test_engine_propagate_direct: the direct navigator's loop
+//    (propagate_direct) the same way, forward or backward.
 #include <cstdint>
 #include <cstring>
 #include <vector>
 
+#include "device/direct_nav.hpp"
 #include "device/engine.hpp"
 
 namespace {
@@ -126,6 +129,30 @@ void propagate_all(const HostDetector& hd, const DevField& field, const DevConfi
         propagate_fast<P>(hd.fast, field, cfg, in, out, static_cast<uint32_t>(t), c);
 }
 
+template <class P>
+void propagate_direct_all(const HostDetector& hd, const DevField& field, const DevConfig& cfg, const TrackInputs& in,
+                          const DevResults& out, const SeqView& sq) {
+    double path[kCache];
+    uint32_t sfs[kCache], ls[kCache];
+    const CacheView c{path, sfs, ls, 1};
+    const DevConfig dd = direct_config(cfg);
+    for (uint64_t t = 0; t < in.n; ++t)
+        propagate_direct<P>(hd.fast, field, cfg, dd, in, out, sq, static_cast<uint32_t>(t), c);
+}
+
+DevField host_field(const dtr_field_desc* f) {
+    DevField field{};
+    field.kind = f->kind;
+    for (int i = 0; i < 3; ++i) {
+        field.b[i] = f->b[i];
+        field.n[i] = f->n[i];
+        field.min[i] = f->min[i];
+        field.pitch[i] = f->pitch[i];
+    }
+    field.values = f->values;
+    return field;
+}
+
 }  // namespace
 
 extern "C" {
@@ -181,6 +208,36 @@ __attribute__((visibility("default"))) int dtr_test_engine_propagate(const dtr_d
         if (vol) propagate_all<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(hd, field, dc, ti, r);
         else if (ext) propagate_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r);
         else propagate_all<PolicyMixed>(hd, field, dc, ti, r);
+    }
+    return DTR_SUCCESS;
+}
+
+// The direct navigator: sequence [max_sequence][n] surfaces, n_sequence [n],
+// direction DTR_DIRECTION_FORWARD | BACKWARD.  Host memory throughout.
+__attribute__((visibility("default"))) int dtr_test_engine_propagate_direct(
+    const dtr_detector_desc* d, const dtr_field_desc* f, const dtr_prop_config* cfg, const dtr_track_batch* in,
+    const dtr_surface_sequence* seq, dtr_result_batch* out, int precision) {
+    if (!d || !f || !cfg || !in || !out || !seq) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cfg->use_field_gradient || seq->direction > DTR_DIRECTION_BACKWARD) return DTR_ERROR_UNSUPPORTED;
+    HostDetector hd(*d);
+    const DevField field = host_field(f);
+    const DevConfig dc = make_dev_config(cfg);
+    const TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
+                         in->start_surface, in->n};
+    const DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                       cfg->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                       out->n_trials, out->status};
+    const SeqView sq{seq->surfaces, seq->n_surfaces, seq->max_surfaces, d->n_surfaces,
+                     seq->direction == DTR_DIRECTION_BACKWARD ? 1u : 0u};
+    const bool ext = has_ext(*d), vol = has_volume_material(*d);
+    if (precision == 0) {
+        if (vol) propagate_direct_all<WithVolumeMaterial<WithExtendedGeometry<PolicyFP64>>>(hd, field, dc, ti, r, sq);
+        else if (ext) propagate_direct_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, ti, r, sq);
+        else propagate_direct_all<PolicyFP64>(hd, field, dc, ti, r, sq);
+    } else {
+        if (vol) propagate_direct_all<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(hd, field, dc, ti, r, sq);
+        else if (ext) propagate_direct_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r, sq);
+        else propagate_direct_all<PolicyMixed>(hd, field, dc, ti, r, sq);
     }
     return DTR_SUCCESS;
 }

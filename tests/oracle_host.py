@@ -205,14 +205,21 @@ def chi2_gradient_fd(detector, field, config, bound_params, start_surface, meas_
     return g
 
 
-def propagate_direct(detector, field, config, free_params, sequence, n_sequence, threads: int = 0) -> dict:
+def propagate_direct(detector, field, config, free_params, sequence, n_sequence, threads: int = 0,
+                     backward: bool = False, bound_params=None, start_surface=None, initial_cov=None) -> dict:
     """Oracle propagation with the direct navigator along the given surface
-    sequences [m, n] (n_sequence [n] entries each)."""
+    sequences [m, n] (n_sequence [n] entries each), forward or backward
+    (last entry first).  Free start: free_params [8, n]; bound start:
+    bound_params [6, n], start_surface [n] and optionally initial_cov
+    [21, n] (free_params may be None)."""
     lib = load()
-    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
+    bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = None if start_surface is None else np.ascontiguousarray(start_surface, dtype=np.uint32)
+    cov = None if initial_cov is None else np.ascontiguousarray(initial_cov, dtype=np.float64)
     seq = np.ascontiguousarray(sequence, dtype=np.uint32)
     nseq = np.ascontiguousarray(n_sequence, dtype=np.uint32)
-    n = fp.shape[1]
+    n = fp.shape[1] if fp is not None else bp.shape[1]
     ms = max(int(config.max_surfaces), 1)
     out = {
         "bound_params": np.zeros((6, n)),
@@ -225,15 +232,20 @@ def propagate_direct(detector, field, config, free_params, sequence, n_sequence,
         "n_trials": np.zeros(n, dtype=np.uint32),
         "status": np.zeros(n, dtype=np.uint8),
     }
-    tb = capi.TrackBatch(n, 0, fp.ctypes.data, None, None, None)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data if fp is not None else None,
+                         cov.ctypes.data if cov is not None else None,
+                         bp.ctypes.data if bp is not None else None,
+                         ss.ctypes.data if ss is not None else None)
     rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
         "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
         "n_steps", "n_trials", "status")])
     lib.oracle_propagate_direct.restype = C.c_int
     cfg = config.to_c()
+    direction = capi.DTR_DIRECTION_BACKWARD if backward else capi.DTR_DIRECTION_FORWARD
     capi.check(lib.oracle_propagate_direct(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(tb),
                                            C.c_void_p(seq.ctypes.data), C.c_void_p(nseq.ctypes.data),
-                                           C.c_uint32(seq.shape[0]), C.byref(rb), C.c_int(threads)),
+                                           C.c_uint32(seq.shape[0]), C.c_uint32(direction), C.byref(rb),
+                                           C.c_int(threads)),
                "oracle_propagate_direct")
     return out
 

@@ -244,3 +244,57 @@ def test_host_lane_loop_volume_material_matches_the_oracle(kw):
     _same_sequences(mixed, ref)
     assert _rel(mixed["bound_params"], ref["bound_params"]) < 1e-4
     assert float(np.max(np.abs(mixed["bound_cov"] - ref["bound_cov"]) / scale)) < 1e-3
+
+
+# ------------------------------------- direct navigator, forward and backward --
+def _backward_case(det, field, cfg, fp, cov):
+    """Oracle caching run, then the backward replay's inputs: bound start on
+    the last recorded surface with the forward direct pass's parameters."""
+    ref = oracle_host.propagate(det, field, cfg, fp, initial_cov=cov)
+    idx = np.nonzero(ref["n_surfaces"] > 0)[0]
+    seq, nseq = ref["surfaces"][:, idx], ref["n_surfaces"][idx]
+    fw = oracle_host.propagate_direct(det, field, cfg, fp[:, idx], seq, nseq)
+    last = seq[nseq - 1, np.arange(len(idx))]
+    return seq, nseq, fw, last
+
+
+@pytest.mark.parametrize("which", ["toy", "wire_chamber", "volume_material"])
+def test_host_direct_backward_matches_the_oracle(which):
+    if which == "toy":
+        det = da.Detector.toy(4, 7)
+        field = da.MagneticField.constant(0.0, 1.0 * da.TESLA, 1.0 * da.TESLA)
+        cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+        fp = da.generate_tracks("toy", 300, 21, pt_min=0.5, pt_max=50.0)
+    elif which == "wire_chamber":
+        det = da.Detector.wire_chamber()
+        field = da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA)
+        cfg = da.PropagationConfig(search_window=(5, 5), estimate_scattering_noise=False)
+        fp = da.generate_tracks("toy", 200, 22, eta_min=-1.0, eta_max=1.0, pt_min=1.0, pt_max=20.0)
+    else:
+        det, field, cfg, fp, _ = volume_material_case()
+    seq, nseq, fw, last = _backward_case(det, field, cfg, fp, None)
+    ref = oracle_host.propagate_direct(det, field, cfg, None, seq, nseq, backward=True,
+                                       bound_params=fw["bound_params"], start_surface=last,
+                                       initial_cov=fw["bound_cov"])
+    got = engine_host.propagate_direct(det, field, cfg, None, seq, nseq, backward=True,
+                                       bound_params=fw["bound_params"], start_surface=last,
+                                       initial_cov=fw["bound_cov"])
+    assert (ref["status"] == 0).all()
+    _same_sequences(got, ref)
+    assert np.array_equal(got["n_steps"], ref["n_steps"])
+    assert _rel(got["bound_params"], ref["bound_params"]) < 1e-9
+    assert _rel(got["path_length"], ref["path_length"]) < 1e-9
+    scale = np.maximum(np.abs(ref["bound_cov"]).max(axis=0), 1e-30)
+    assert float(np.max(np.abs(got["bound_cov"] - ref["bound_cov"]) / scale)) < 1e-8
+    mixed = engine_host.propagate_direct(det, field, cfg, None, seq, nseq, mixed=True, backward=True,
+                                         bound_params=fw["bound_params"], start_surface=last,
+                                         initial_cov=fw["bound_cov"])
+    _same_sequences(mixed, ref)
+    assert _rel(mixed["bound_params"], ref["bound_params"]) < 1e-4
+    # forward through the engine's direct loop as well
+    fwd = engine_host.propagate_direct(det, field, cfg, None, seq, nseq, bound_params=ref["bound_params"],
+                                       start_surface=seq[0], initial_cov=ref["bound_cov"])
+    fref = oracle_host.propagate_direct(det, field, cfg, None, seq, nseq, bound_params=ref["bound_params"],
+                                        start_surface=seq[0], initial_cov=ref["bound_cov"])
+    _same_sequences(fwd, fref)
+    assert _rel(fwd["bound_params"], fref["bound_params"]) < 1e-9

@@ -7,7 +7,9 @@ sqrt(C_ii C_jj).  Mixed precision: identical sequences and statuses, bound
 parameters within 1e-4 in the reference's host/device convention.  On the
 GPU, the reference's own check (propagation_direct_navigator.cpp): a
 sequence recorded with the caching navigator is replayed surface for
-surface, with the momentum at the last surface equal to 2e-3."""
+surface, with the momentum at the last surface equal to 2e-3, and then
+backward from the last surface to the first, visiting the surfaces in
+reverse order and returning the initial momentum to 2e-3."""
 
 import numpy as np
 import pytest
@@ -23,12 +25,20 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-def run_direct(det, field, cfg, fp, seq, n_seq, precision="fp64", order=None, block=64):
+def run_direct(det, field, cfg, fp, seq, n_seq, precision="fp64", order=None, block=64, backward=False,
+               bound_params=None, start_surface=None, initial_cov=None):
     launch = da.LaunchConfig(precision, "grid", block, 2, 64, order, 1, "lane")
     prop = da.Propagator(cfg, launch)
-    res = prop.propagate_direct(det, field, torch.from_numpy(np.ascontiguousarray(seq).view(np.int32)).cuda(),
-                                torch.from_numpy(np.ascontiguousarray(n_seq).view(np.int32)).cuda(),
-                                tracks=torch.from_numpy(fp).cuda())
+
+    def dev(a, dtype=None):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a)
+        return torch.from_numpy(a.view(dtype) if dtype is not None else a).cuda()
+
+    res = prop.propagate_direct(det, field, dev(seq, np.int32), dev(n_seq, np.int32), tracks=dev(fp),
+                                backward=backward, bound_params=dev(bound_params),
+                                start_surface=dev(start_surface, np.int32), initial_cov=dev(initial_cov))
     torch.cuda.synchronize()
     return {
         "bound_params": res.bound_params.cpu().numpy(),
@@ -105,3 +115,78 @@ def test_direct_order_and_edge_cases(toy7):
     assert a["status"][0] == 0 and a["n_surfaces"][0] == 0 and a["status"][1] == 1
     for k in a:
         np.testing.assert_array_equal(a[k], b[k])
+
+
+def _backward_inputs(det, field, cfg, fp):
+    rec = oracle_host.propagate(det, field, cfg, fp)
+    idx = np.nonzero(rec["n_surfaces"] > 0)[0]
+    seq, nseq = np.ascontiguousarray(rec["surfaces"][:, idx]), np.ascontiguousarray(rec["n_surfaces"][idx])
+    fw = oracle_host.propagate_direct(det, field, cfg, fp[:, idx], seq, nseq)
+    last = np.ascontiguousarray(seq[nseq - 1, np.arange(len(idx))])
+    return idx, seq, nseq, fw, last
+
+
+@pytest.mark.parametrize("overstep,bfield", CASES[:1] + CASES[3:])
+def test_direct_backward_fp64_matches_oracle(toy7, overstep, bfield):
+    field = da.MagneticField.constant(*(b * da.TESLA for b in bfield))
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False, overstep_tolerance=overstep)
+    idx, seq, nseq, fw, last = _backward_inputs(toy7, field, cfg, uniform_tracks())
+    kw = dict(backward=True, bound_params=fw["bound_params"], start_surface=last, initial_cov=fw["bound_cov"])
+    ref = oracle_host.propagate_direct(toy7, field, cfg, None, seq, nseq, **kw)
+    got = run_direct(toy7, field, cfg, None, seq, nseq, **kw)
+    assert (ref["status"] == 0).all()
+    assert_parity(got, ref)
+
+
+@pytest.mark.parametrize("which", ["toy_solenoid", "wire_chamber"])
+def test_direct_backward_mixed_matches_oracle(toy7, solenoid, which):
+    if which == "toy_solenoid":
+        det, field = toy7, solenoid
+        cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False)
+        fp = da.generate_tracks("toy", 4096, 48)
+    else:
+        det = da.Detector.wire_chamber(half_z=500.0)
+        field = da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA)
+        cfg = da.PropagationConfig(search_window=(5, 5), estimate_scattering_noise=False)
+        fp = uniform_tracks(40, 40)
+    idx, seq, nseq, fw, last = _backward_inputs(det, field, cfg, fp)
+    kw = dict(backward=True, bound_params=fw["bound_params"], start_surface=last, initial_cov=fw["bound_cov"])
+    ref = oracle_host.propagate_direct(det, field, cfg, None, seq, nseq, **kw)
+    got = run_direct(det, field, cfg, None, seq, nseq, "mixed", **kw)
+    np.testing.assert_array_equal(got["status"], ref["status"])
+    np.testing.assert_array_equal(got["n_surfaces"], ref["n_surfaces"])
+    np.testing.assert_array_equal(got["surfaces"], ref["surfaces"])
+    err = np.abs(got["bound_params"] - ref["bound_params"]) / bound_error_scale(ref)
+    assert err.max() <= MIXED_BOUND_TOL, err.max()
+    # the FP64 policy at the FP64 contract on the same inputs
+    assert_parity(run_direct(det, field, cfg, None, seq, nseq, "fp64", **kw), ref)
+
+
+@pytest.mark.parametrize("overstep,bfield", CASES)
+def test_direct_backward_replays_gpu_sequences(toy7, overstep, bfield):
+    """propagation_direct_navigator.cpp:46-213 entirely on the GPU, mixed
+    precision: caching navigator, forward replay, backward replay from the
+    forward replay's parameters at the last surface."""
+    field = da.MagneticField.constant(*(b * da.TESLA for b in bfield))
+    cfg = da.PropagationConfig(search_window=(3, 3), estimate_scattering_noise=False, overstep_tolerance=overstep)
+    fp = uniform_tracks()
+    launch = da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "lane")
+    cach = da.Propagator(cfg, launch).propagate(toy7, field, torch.from_numpy(fp).cuda())
+    torch.cuda.synchronize()
+    n_all = cach.n_surfaces.cpu().numpy().view(np.uint32)
+    idx = np.nonzero(n_all > 0)[0]
+    seq = np.ascontiguousarray(cach.surfaces.cpu().numpy().view(np.uint32)[:, idx])
+    nseq = np.ascontiguousarray(n_all[idx])
+    fw = run_direct(toy7, field, cfg, np.ascontiguousarray(fp[:, idx]), seq, nseq, "mixed")
+    last = np.ascontiguousarray(seq[nseq - 1, np.arange(len(idx))])
+    bw = run_direct(toy7, field, cfg, None, seq, nseq, "mixed", backward=True, bound_params=fw["bound_params"],
+                    start_surface=last, initial_cov=fw["bound_cov"])
+    assert (bw["status"] == 0).all()
+    np.testing.assert_array_equal(bw["n_surfaces"], nseq)
+    for j in range(len(idx)):
+        k = nseq[j]
+        np.testing.assert_array_equal(bw["surfaces"][:k, j], seq[:k, j][::-1])
+    p0 = 1.0 / np.abs(fp[7, idx])
+    p_fw, p_bw = momentum(fw["bound_params"]), momentum(bw["bound_params"])
+    assert np.max(np.abs(p_bw - p0) / p0) < 2e-3
+    assert (p_bw >= p_fw).all()
