@@ -239,9 +239,9 @@ int check_propagate_args(const dtr_detector* det, const dtr_field* field, const 
 int check_adjoint_args(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
                        const dtr_adjoint_config* a, const dtr_adjoint_batch* b, const dtr_hits* hits) {
     if (!det || !field || !c || !a || !b) return DTR_ERROR_INVALID_ARGUMENT;
-    // the adjoint's material derivatives follow the approximate density
-    // correction and vacuum volumes
-    if (det->wires || det->density_effect) return DTR_ERROR_UNSUPPORTED;
+    // the adjoint's steps follow vacuum volumes (no continuous energy loss
+    // in the Runge-Kutta stages)
+    if (det->volume_material) return DTR_ERROR_UNSUPPORTED;
     if (a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return DTR_ERROR_INVALID_ARGUMENT;
     if (a->max_steps > (1u << 20) || a->max_events > 4096u || a->checkpoint_every > a->max_steps)
         return DTR_ERROR_INVALID_ARGUMENT;
@@ -274,6 +274,21 @@ dtr::AdjointIO make_adjoint_io(const dtr_adjoint_config* a, const dtr_adjoint_ba
     io.iv1 = 1.0 / (a->sigma_loc1 * a->sigma_loc1);
     io.tape = dtr::tape_carve(b->tape, b->n, a->max_steps, a->max_events, a->checkpoint_every);
     return io;
+}
+
+// the adjoint's passes at one or two resident waves per SIMD
+template <class P>
+int adjoint_record_w(bool w2, const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+                     const dtr::AdjointIO& io, uint32_t block, const uint32_t* order, hipStream_t st) {
+    return w2 ? dtr::launch_adjoint_record<P, 2>(det, field, dc, io, block, order, st)
+              : dtr::launch_adjoint_record<P, 1>(det, field, dc, io, block, order, st);
+}
+template <class P>
+int adjoint_backward_w(bool w2, const dtr_detector* det, const dtr_field* field, const dtr::DevConfig& dc,
+                       const dtr::AdjointIO& io, uint32_t block, const uint32_t* order, int exact, double* grad,
+                       hipStream_t st) {
+    return w2 ? dtr::launch_adjoint_backward<P, 2>(det, field, dc, io, block, order, exact, grad, st)
+              : dtr::launch_adjoint_backward<P, 1>(det, field, dc, io, block, order, exact, grad, st);
 }
 
 // register budget: 1 or 2 resident waves per SIMD
@@ -693,11 +708,16 @@ int dtr_adjoint_forward(const dtr_detector* det, const dtr_field* field, const d
     const uint32_t block = a->block_size ? a->block_size : 128u;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const bool w2 = a->waves_per_simd != 1u;
+    if (det->wires) {
+        if (a->precision == DTR_PRECISION_FP64)
+            return adjoint_record_w<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(w2, det, field, dc, io, block,
+                                                                                b->order, st);
+        return adjoint_record_w<dtr::WithExtendedGeometry<dtr::PolicyMixed>>(w2, det, field, dc, io, block, b->order,
+                                                                             st);
+    }
     if (a->precision == DTR_PRECISION_FP64)
-        return w2 ? dtr::launch_adjoint_record<dtr::PolicyFP64, 2>(det, field, dc, io, block, b->order, st)
-                  : dtr::launch_adjoint_record<dtr::PolicyFP64, 1>(det, field, dc, io, block, b->order, st);
-    return w2 ? dtr::launch_adjoint_record<dtr::PolicyMixed, 2>(det, field, dc, io, block, b->order, st)
-              : dtr::launch_adjoint_record<dtr::PolicyMixed, 1>(det, field, dc, io, block, b->order, st);
+        return adjoint_record_w<dtr::PolicyFP64>(w2, det, field, dc, io, block, b->order, st);
+    return adjoint_record_w<dtr::PolicyMixed>(w2, det, field, dc, io, block, b->order, st);
 }
 
 int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
@@ -713,11 +733,16 @@ int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field, const 
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int exact = a->model == DTR_ADJOINT_EXACT ? 1 : 0;
     const bool w2 = a->waves_per_simd != 1u;
+    if (det->wires) {
+        if (a->precision == DTR_PRECISION_FP64)
+            return adjoint_backward_w<dtr::WithExtendedGeometry<dtr::PolicyFP64>>(w2, det, field, dc, io, block,
+                                                                                  b->order, exact, grad, st);
+        return adjoint_backward_w<dtr::WithExtendedGeometry<dtr::PolicyMixed>>(w2, det, field, dc, io, block,
+                                                                               b->order, exact, grad, st);
+    }
     if (a->precision == DTR_PRECISION_FP64)
-        return w2 ? dtr::launch_adjoint_backward<dtr::PolicyFP64, 2>(det, field, dc, io, block, b->order, exact, grad, st)
-                  : dtr::launch_adjoint_backward<dtr::PolicyFP64, 1>(det, field, dc, io, block, b->order, exact, grad, st);
-    return w2 ? dtr::launch_adjoint_backward<dtr::PolicyMixed, 2>(det, field, dc, io, block, b->order, exact, grad, st)
-              : dtr::launch_adjoint_backward<dtr::PolicyMixed, 1>(det, field, dc, io, block, b->order, exact, grad, st);
+        return adjoint_backward_w<dtr::PolicyFP64>(w2, det, field, dc, io, block, b->order, exact, grad, st);
+    return adjoint_backward_w<dtr::PolicyMixed>(w2, det, field, dc, io, block, b->order, exact, grad, st);
 }
 
 }  // extern "C"

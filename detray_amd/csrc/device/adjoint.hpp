@@ -23,9 +23,9 @@
 //  * exact: the derivative of the propagation itself (field gradient of the
 //    trilinear map, energy-loss derivative, normalisation of the direction,
 //    the free state continuing unchanged through a surface without
-//    material, and the length of each step that the navigator sized to
-//    reach a surface as a function of the state its distance was computed
-//    from);
+//    material, and the length of every step that the navigator sized to
+//    reach a surface, event or not, as a function of the state its distance
+//    was computed from);
 //  * transport: the derivative the reference's Jacobians describe
 //    (rk_stepper.hpp:633-956 without field gradient, stepping_config.hpp:
 //    27-48 use_field_gradient / use_eloss_gradient off; no normalisation;
@@ -55,15 +55,20 @@ struct Tape {
     uint32_t* ev_step;  // [max_events][n] steps taken before the event
     uint32_t* ev_sf;    // [max_events][n] surface
     int32_t* ev_hit;    // [max_events][n] matched hit or -1
-    uint32_t* ev_flags; // [max_events][n] kEvFromFree | kEvMaterial
-    uint32_t* ev_land;  // [max_events][n] surface the step before the event was sized to reach, or kInvalid
+    uint32_t* ev_flags; // [max_events][n] kEvFromFree | kEvMaterial | kEvReused
+    uint32_t* step_land;  // [max_steps][n] surface step j+1 was sized to reach (its length is the
+                          // navigator's distance to it), or kInvalid
     uint32_t* n_steps;  // [n]
     uint32_t* n_events; // [n]
     uint32_t* status;   // [n] kTape*
 };
 
 enum : uint32_t { kTapeOk = 0, kTapeSteps = 1, kTapeEvents = 2, kTapeUnsupported = 3 };
-enum : uint32_t { kEvFromFree = 1u, kEvMaterial = 2u };
+// kEvReused: the track reached the surface its bound parameters are on
+// (a placed cylinder crossed twice, a return to the start surface): the
+// parameter transporter keeps them (parameter_updater.hpp:314-419), so the
+// event works on the bound parameters of the last bound event
+enum : uint32_t { kEvFromFree = 1u, kEvMaterial = 2u, kEvReused = 4u };
 
 DTR_HD uint64_t tape_doubles(uint32_t max_steps, uint32_t max_events, uint32_t every) {
     const uint64_t n_ckpt = max_steps / every + 1;
@@ -72,7 +77,7 @@ DTR_HD uint64_t tape_doubles(uint32_t max_steps, uint32_t max_events, uint32_t e
 
 // bytes of a tape for n tracks
 DTR_HD size_t tape_bytes(uint64_t n, uint32_t max_steps, uint32_t max_events, uint32_t every) {
-    const uint64_t words = 5ull * max_events + 3ull;
+    const uint64_t words = 4ull * max_events + max_steps + 3ull;
     return static_cast<size_t>(n * (8ull * tape_doubles(max_steps, max_events, every) + 4ull * words));
 }
 
@@ -108,8 +113,8 @@ DTR_HD Tape tape_carve(void* base, uint64_t n, uint32_t max_steps, uint32_t max_
     w += me;
     t.ev_flags = w;
     w += me;
-    t.ev_land = w;
-    w += me;
+    t.step_land = w;
+    w += static_cast<uint64_t>(max_steps) * n;
     t.n_steps = w;
     w += n;
     t.n_events = w;
@@ -188,18 +193,58 @@ DTR_HD D4 dfabs(D4 a) { return a.v < 0.0 ? dscale(a, -1.0) : a; }
 DTR_HD D4 dcos(D4 a) { return dchain(cos(a.v), -sin(a.v), a); }
 DTR_HD D4 dsin(D4 a) { return dchain(sin(a.v), cos(a.v), a); }
 
+DTR_HD D4 dpow(D4 a, double k) { return dchain(pow(a.v, k), k * pow(a.v, k - 1.0), a); }
+
+// d/dx of half the density correction (relativistic_quantities.hpp:
+// 176-200): the approximate high-energy form without density-effect data,
+// Sternheimer's parametrisation (density_effect_data.hpp) with it.
+DTR_HD D4 delta_half_d(const dtr_material_slab& m, D4 bg, double ned) {
+    if (!m.has_density) {
+        if (!(bg.v >= 10.0)) return dconst(0.0);
+        const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
+        return dlog(dscale(bg, plasma / m.mean_exc)) - dconst(0.5);
+    }
+    const double ln10 = 2.302585092994045684;
+    const D4 x = dscale(dlog(bg), 1.0 / ln10);
+    if (!(x.v >= m.density[2])) return dconst(0.0);
+    D4 delta = dscale(x, 2.0 * ln10) - dconst(m.density[4]);
+    if (x.v < m.density[3]) delta = delta + dscale(dpow(dconst(m.density[3]) - x, m.density[1]), m.density[0]);
+    return dscale(delta, 0.5);
+}
+
 // q/p after the mean energy loss (pointwise_material_interactor.hpp:147-326,
 // the expressions of FastEngine::material_interaction) as a function of the
-// bound parameters before it: loc0 (the normal of a concentric cylinder),
-// phi, theta and q/p.
+// bound parameters before it: loc0 (the normal of a concentric or placed
+// cylinder, the closest approach to a rod), phi, theta and q/p.
 DTR_HD D4 eloss_qop(const NavSurface& rec, const dtr_material_slab& m, const double b[6], double mass, double q) {
     const D4 l0 = dvar(b[0], 0), ph = dvar(b[2], 1), th = dvar(b[3], 2), qop = dvar(b[4], 3);
     const D4 st = dsin(th);
     const D4 dx = dcos(ph) * st, dy = dsin(ph) * st, dz = dcos(th);
     D4 cos_inc;
-    if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) cos_inc = dfabs(dx * dcos(l0) + dy * dsin(l0));
-    else cos_inc = dfabs(dscale(dx, rec.r[2]) + dscale(dy, rec.r[5]) + dscale(dz, rec.r[8]));
-    const D4 path_seg = dconst(m.thickness) / cos_inc;
+    if (rec.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        cos_inc = dfabs(dx * dcos(l0) + dy * dsin(l0));
+    } else if (rec.shape == DTR_CYLINDER2D) {
+        // normal at phi = loc0 / r (cylindrical2D.hpp:62-70)
+        const D4 a = dscale(l0, 1.0 / rec.m[0]);
+        const D4 ca = dcos(a), sa = dsin(a);
+        const D4 nx = dscale(ca, rec.r[0]) + dscale(sa, rec.r[1]);
+        const D4 ny = dscale(ca, rec.r[3]) + dscale(sa, rec.r[4]);
+        const D4 nz = dscale(ca, rec.r[6]) + dscale(sa, rec.r[7]);
+        cos_inc = dfabs(dx * nx + dy * ny + dz * nz);
+    } else {
+        // plane normal, or the wire axis of a line
+        cos_inc = dfabs(dscale(dx, rec.r[2]) + dscale(dy, rec.r[5]) + dscale(dz, rec.r[8]));
+    }
+    D4 path_seg;
+    if (m.kind == DTR_MATERIAL_ROD) {
+        // material_rod.hpp:62-78 with the closest approach loc0
+        path_seg = dconst(0.0);
+        if (!(fabs(b[0]) > m.thickness))
+            path_seg = dscale(dsqrt((dconst(m.thickness * m.thickness) - l0 * l0) / (dconst(1.0) - cos_inc * cos_inc)),
+                              2.0);
+    } else {
+        path_seg = dconst(m.thickness) / cos_inc;
+    }
     const D4 mq = dscale(qop, mass);
     const D4 q2b2 = dconst(q * q) + mq * mq;
     const D4 mOverP = dscale(dfabs(qop), mass / fabs(q));
@@ -215,11 +260,7 @@ DTR_HD D4 eloss_qop(const NavSurface& rec, const dtr_material_slab& m, const dou
     const D4 eps_pl = dscale(q2b2, 0.5 * K * ned);
     D4 bb = dconst(0.0);
     if (eps_pl.v > 0.0) {
-        D4 dhalf = dconst(0.0);
-        if (bg.v >= 10.0) {
-            const double plasma = (28.816 * 1e-9) * sqrt(1000.0 * ned);
-            dhalf = dlog(dscale(bg, plasma / m.mean_exc)) - dconst(0.5);
-        }
+        const D4 dhalf = delta_half_d(m, bg, ned);
         const D4 u = dscale(bg * bg, 2.0 * m_e);
         const D4 A = dscale(dlog(dscale(u * Wmax, 1.0 / (m.mean_exc * m.mean_exc))), 0.5);
         bb = dscale(eps_pl, 2.0) * (A - beta2 - dhalf);
@@ -402,51 +443,110 @@ DTR_HD void rk_vjp(const DevField& field, V3 pos, V3 dirv, double qopd, double h
 // ------------------------------------------------- surface-event adjoints --
 DTR_HD V3 rec_col(const NavSurface& r, int k) { return {r.r[k], r.r[3 + k], r.r[6 + k]}; }
 
+DTR_HD bool is_line_shape(uint8_t shape) { return shape == DTR_LINE_SQUARE || shape == DTR_LINE_CIRCULAR; }
+
 // b_bar = J_B2F(b)^T (pos_bar, dir_bar, qop_bar): the adjoint of the setter
-// (position and direction of the bound parameters b on surface r)
+// (position and direction of the bound parameters b on surface r; the
+// frames of bound_pos: cartesian, polar for rings and annuli, concentric
+// and placed cylindrical, and line2D, whose position also depends on the
+// direction, jacobian_line.hpp:102-172)
 DTR_HD void b2f_transpose(const NavSurface& r, const double b[6], V3 pb, V3 db, double qb, double bb[6]) {
+    const double cp = cos(b[2]), sp = sin(b[2]), ct = cos(b[3]), st = sin(b[3]);
+    const V3 dphi{-sp * st, cp * st, 0.0}, dtheta{cp * ct, sp * ct, -st};
     V3 c0, c1;
+    double ang[2] = {0.0, 0.0};
     if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
         const double R = r.m[0];
         c0 = {-R * sin(b[0]), R * cos(b[0]), 0.0};
         c1 = {0.0, 0.0, 1.0};
-    } else if (r.shape == DTR_RING2D) {
+    } else if (r.shape == DTR_RING2D || r.shape == DTR_ANNULUS2D) {
         const V3 e0 = rec_col(r, 0), e1 = rec_col(r, 1);
         const double c = cos(b[1]), s = sin(b[1]);
         c0 = add(mul(c, e0), mul(s, e1));
         c1 = mul(b[0], sub(mul(c, e1), mul(s, e0)));
+    } else if (r.shape == DTR_CYLINDER2D) {
+        // pos = T (R cos(loc0/R), R sin(loc0/R), loc1)
+        const double ph = b[0] / r.m[0];
+        c0 = add(mul(-sin(ph), rec_col(r, 0)), mul(cos(ph), rec_col(r, 1)));
+        c1 = rec_col(r, 2);
+    } else if (is_line_shape(r.shape)) {
+        // pos = t + loc1 a + loc0 x, x = (a x dir) / |a x dir|; dx/dangle =
+        // (I - x x^T)(a x ddir/dangle) / |a x dir|
+        const V3 a = rec_col(r, 2);
+        const V3 w = cross(a, V3{cp * st, sp * st, ct});
+        const double rho = sqrt(dot(w, w));
+        const V3 x = mul(1.0 / rho, w);
+        c0 = x;
+        c1 = a;
+        const V3 dd[2] = {dphi, dtheta};
+        for (int i = 0; i < 2; ++i) {
+            const V3 u = cross(a, dd[i]);
+            ang[i] = (b[0] / rho) * dot(sub(u, mul(dot(x, u), x)), pb);
+        }
     } else {
         c0 = rec_col(r, 0);
         c1 = rec_col(r, 1);
     }
-    const double cp = cos(b[2]), sp = sin(b[2]), ct = cos(b[3]), st = sin(b[3]);
     bb[0] = dot(c0, pb);
     bb[1] = dot(c1, pb);
-    bb[2] = dot(V3{-sp * st, cp * st, 0.0}, db);
-    bb[3] = dot(V3{cp * ct, sp * ct, -st}, db);
+    bb[2] = dot(dphi, db) + ang[0];
+    bb[3] = dot(dtheta, db) + ang[1];
     bb[4] = qb;
     bb[5] = 0.0;
 }
 
 // (pos_bar, dir_bar, qop_bar) = J^T b_bar for the free-to-bound map F onto
-// surface r at the free state (p, d).  land: the surface the last step was
-// sized to reach, or null.  With a landing surface the position is first
-// moved along the trajectory onto it (a change of the state before the step
-// changes the step length): J = F (I + p2f f2p), p2f = (d, k, 0) with
-// dt/ds = k, f2p = (-n / (n.d), 0, 0), n the normal of `land`
+// surface r at the free state (p, d).  exact: the derivative of the frame's
+// global-to-local map as the engine evaluates it (a line's loc0 is the
+// signed distance from the wire axis, a placed cylinder's loc0 is r phi of
+// the point); otherwise the reference's free-to-bound rows (line and
+// cylindrical frames of jacobian_line.hpp:38-62, jacobian_cylindrical.hpp:
+// 24-45).  land: the surface the last step was sized to reach, or null.
+// With a landing surface the position is first moved along the trajectory
+// onto it (a change of the state before the step changes the step length):
+// J = F (I + p2f f2p), p2f = (d, k, 0) with dt/ds = k, f2p = (-n / (n.d), 0,
+// 0) with the surface normal n, for a line the path derivative of
+// jacobian_line.hpp:64-92 (position and direction parts)
 // (parameter_updater.hpp:463-514 with land = r).
-DTR_HD void f2b_transpose(const NavSurface& r, V3 p, V3 d, V3 k, const NavSurface* land, const double bb[6], V3& pb,
-                          V3& db, double& qb) {
+DTR_HD void f2b_transpose(const NavSurface& r, V3 p, V3 d, V3 k, const NavSurface* land, bool exact,
+                          const double bb[6], V3& pb, V3& db, double& qb) {
     V3 vp;
+    const V3 tr{r.t[0], r.t[1], r.t[2]};
     if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
         const double rho2 = p.x * p.x + p.y * p.y;
         vp = {-p.y * bb[0] / rho2, p.x * bb[0] / rho2, bb[1]};
-    } else if (r.shape == DTR_RING2D) {
+    } else if (r.shape == DTR_RING2D || r.shape == DTR_ANNULUS2D) {
         const V3 e0 = rec_col(r, 0), e1 = rec_col(r, 1);
-        const V3 dl = sub(p, V3{r.t[0], r.t[1], r.t[2]});
+        const V3 dl = sub(p, tr);
         const double lx = dot(e0, dl), ly = dot(e1, dl);
         const double rl2 = lx * lx + ly * ly, rl = sqrt(rl2);
         vp = add(mul(bb[0] / rl, add(mul(lx, e0), mul(ly, e1))), mul(bb[1] / rl2, sub(mul(lx, e1), mul(ly, e0))));
+    } else if (r.shape == DTR_CYLINDER2D) {
+        const V3 e0 = rec_col(r, 0), e1 = rec_col(r, 1), e2 = rec_col(r, 2);
+        const V3 dl = sub(p, tr);
+        const double lx = dot(e0, dl), ly = dot(e1, dl);
+        const double rl = sqrt(lx * lx + ly * ly), ph = atan2(ly, lx);
+        const V3 rad = add(mul(lx / rl, e0), mul(ly / rl, e1));  // normal at the point
+        const V3 tan = add(mul(-ly / rl, e0), mul(lx / rl, e1));
+        // exact: d(rl phi) = phi drl + rl dphi; the reference: the tangent
+        vp = add(mul(bb[0], exact ? add(mul(ph, rad), tan) : tan), mul(bb[1], e2));
+    } else if (is_line_shape(r.shape)) {
+        const V3 a = rec_col(r, 2);
+        V3 x;
+        if (exact) {
+            // loc0 = sign |(lx, ly)| of the local point, the sign of line2D's
+            // global_to_local; the local axes are taken as stored (a rotation
+            // built in single precision is orthonormal to 1e-7 only, which
+            // the drift distance of a point 100 mm along the wire feels)
+            const V3 e0 = rec_col(r, 0), e1 = rec_col(r, 1);
+            const V3 dl = sub(p, tr);
+            const double lx = dot(e0, dl), ly = dot(e1, dl);
+            const double sign = dot(cross(a, d), sub(tr, p)) > 0.0 ? -1.0 : 1.0;
+            x = mul(sign / sqrt(lx * lx + ly * ly), add(mul(lx, e0), mul(ly, e1)));
+        } else {
+            x = normalize(cross(a, d));
+        }
+        vp = add(mul(bb[0], x), mul(bb[1], a));
     } else {
         vp = add(mul(bb[0], rec_col(r, 0)), mul(bb[1], rec_col(r, 1)));
     }
@@ -454,19 +554,40 @@ DTR_HD void f2b_transpose(const NavSurface& r, V3 p, V3 d, V3 k, const NavSurfac
     const V3 vd = add(mul(bb[2], V3{-d.y / rt2, d.x / rt2, 0.0}),
                       mul(bb[3], V3{d.z * d.x / (rt * dd), d.z * d.y / (rt * dd), -rt / dd}));
     pb = vp;
+    db = vd;
     if (land) {
-        // (I + p2f f2p)^T v: the position adjoint gains f2p (p2f . v)
-        V3 n;
-        if (land->shape == DTR_CONCENTRIC_CYLINDER2D) {
-            const double rho = sqrt(p.x * p.x + p.y * p.y);
-            n = {p.x / rho, p.y / rho, 0.0};
+        // (I + p2f f2p)^T v: position and direction adjoints gain
+        // f2p (p2f . v)
+        V3 fp, fd{0.0, 0.0, 0.0};
+        const V3 lt{land->t[0], land->t[1], land->t[2]};
+        if (is_line_shape(land->shape)) {
+            const V3 a = rec_col(*land, 2);
+            const V3 dl = sub(p, lt);
+            const V3 pcx = sub(dl, mul(dot(dl, a), a));
+            const double dz = dot(a, d);
+            const double nrm = -1.0 / (1.0 - dz * dz + dot(pcx, k));
+            fp = mul(nrm, sub(d, mul(dz, a)));
+            fd = mul(nrm, pcx);
         } else {
-            n = rec_col(*land, 2);
+            V3 n;
+            if (land->shape == DTR_CONCENTRIC_CYLINDER2D) {
+                const double rho = sqrt(p.x * p.x + p.y * p.y);
+                n = {p.x / rho, p.y / rho, 0.0};
+            } else if (land->shape == DTR_CYLINDER2D) {
+                const V3 e0 = rec_col(*land, 0), e1 = rec_col(*land, 1);
+                const V3 dl = sub(p, lt);
+                const double lx = dot(e0, dl), ly = dot(e1, dl);
+                const double ph = atan2(ly, lx);
+                n = add(mul(cos(ph), e0), mul(sin(ph), e1));
+            } else {
+                n = rec_col(*land, 2);
+            }
+            fp = mul(-1.0 / dot(n, d), n);
         }
         const double s = dot(d, vp) + dot(k, vd);
-        pb = add(vp, mul(-s / dot(n, d), n));
+        pb = add(vp, mul(s, fp));
+        db = add(vd, mul(s, fd));
     }
-    db = vd;
     qb = bb[4];
 }
 
@@ -496,7 +617,7 @@ template <class P>
 DTR_AI void record_update(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
                           FastTrack<P>& tk, CacheView c, uint32_t gid, uint32_t col, uint32_t& n_ev,
                           uint32_t& next_hit,
-                          uint32_t& n_rec, double& chi2, uint32_t& status, uint32_t& land) {
+                          uint32_t& n_rec, double& chi2, uint32_t& status) {
     using S = typename P::S;
     FastEngine<P, FastTrack<P>, true> e{det, field, cfg, tk, c};
     const bool sens = e.on_sensitive(), mat = e.on_material();
@@ -523,7 +644,7 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
     }
     double jf[36];
     e.jf_out = io.grad_transport ? jf : nullptr;
-    bool from_free = false;
+    bool from_free = false, reused = false;
     if (tk.up_sf != kInvalid) {
         if (tk.up_sf != dest) {
             // the covariance is not an output of the adjoint; it is
@@ -535,7 +656,7 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
             tk.up_sf = dest;
             from_free = true;
         } else if (n_ev > 0u) {
-            status = kTapeUnsupported;  // bound parameters of an earlier event reused
+            reused = true;
         }
     } else {
         status = kTapeUnsupported;  // free start: no bound parameters to differentiate
@@ -593,11 +714,10 @@ DTR_AI void record_update(const FastDetector& det, const DevField& field, const 
         tp.ev_step[ev * N + col] = tk.n_steps;
         tp.ev_sf[ev * N + col] = dest;
         tp.ev_hit[ev * N + col] = hit;
-        tp.ev_flags[ev * N + col] = (from_free ? kEvFromFree : 0u) | (mat ? kEvMaterial : 0u);
-        tp.ev_land[ev * N + col] = land;
+        tp.ev_flags[ev * N + col] =
+            (from_free ? kEvFromFree : 0u) | (mat ? kEvMaterial : 0u) | (reused ? kEvReused : 0u);
         n_ev++;
     }
-    land = kInvalid;  // later events before the next step were reached without moving
 }
 
 // propagator::propagate (propagator.hpp:262-400) for one track from its
@@ -616,7 +736,7 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
     Lane<P> lane;
     lane.start(det, field, cfg, in, c, gid);
     FastTrack<P>& tk = lane.tk;
-    uint32_t n_ev = 0u, next_hit = 0u, n_rec = 0u, status = kTapeOk, land = kInvalid;
+    uint32_t n_ev = 0u, next_hit = 0u, n_rec = 0u, status = kTapeOk;
     double chi2 = 0.0;
     while (true) {
         // actor half (Lane::actor_half)
@@ -630,7 +750,7 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
             tk.constraint_abort = dmin(tk.constraint_abort, fabs(step_limit));
         }
         if (cfg.use_material)
-            record_update<P>(det, field, cfg, io, tk, c, gid, col, n_ev, next_hit, n_rec, chi2, status, land);
+            record_update<P>(det, field, cfg, io, tk, c, gid, col, n_ev, next_hit, n_rec, chi2, status);
         if (tk.stopped && tk.status > kUnknown) {
             tk.status = kAbort;
             tk.heartbeat = false;
@@ -649,10 +769,10 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
         const uint32_t target = c.s(tk.next);
         lane.step_half(det, field, cfg, c);
         if (tk.n_steps != s0) {
-            land = (tk.step_size == dist) ? target : kInvalid;
             if (s0 < tp.max_steps) {
                 tp.h[(2ull * s0 + 0u) * N + col] = tk.h_stage;
                 tp.h[(2ull * s0 + 1u) * N + col] = tk.step_size;
+                tp.step_land[static_cast<uint64_t>(s0) * N + col] = (tk.step_size == dist) ? target : kInvalid;
             } else {
                 status = kTapeSteps;
             }
@@ -669,42 +789,127 @@ DTR_AI void adjoint_record(const FastDetector& det, const DevField& field, const
 }
 
 // -------------------------------------------------------------- backward --
-// Straight-line distance from (p, d) to the plane of r, with the arithmetic
-// of FastEngine::plane_rec (the navigator's candidate path).
-DTR_HD double plane_path(const NavSurface& r, V3 p, V3 d) {
-    const V3 sn = rec_col(r, 2);
-    return dot(sn, sub(V3{r.t[0], r.t[1], r.t[2]}, p)) / dot(d, sn);
+// Straight-line distance s from (p, d) to the target surface r with the
+// arithmetic of the navigator's candidate path (FastEngine::plane_rec,
+// line_intersect, cyl2_intersect, cyl_intersect), and its gradient (gp, gd)
+// with respect to p and d.  Cylinders have two solutions: the one nearest to
+// `near`.
+// Planes: s = n.(t - p) / (n.d).  Lines (closest approach to the wire axis
+// a): s = ((t - p).d - ((t - p).a)(a.d)) / (1 - (a.d)^2).  Cylinders: the
+// root of |P (p - t + s d)|^2 = R^2, P the projection across the axis, with
+// the gradient of the implicit function, -P u / (P u . d) (1, s),
+// u = p - t + s d.
+DTR_HD bool target_path(const NavSurface& r, V3 p, V3 d, double near, double& s, V3& gp, V3& gd) {
+    const V3 tr{r.t[0], r.t[1], r.t[2]};
+    if (is_line_shape(r.shape)) {
+        const V3 a = rec_col(r, 2);
+        const double c = dot(a, d);
+        const double den = 1.0 - c * c;
+        if (den < static_cast<double>(1e-5f)) return false;
+        const V3 t2l = sub(tr, p);
+        const double on_line = dot(t2l, a);
+        s = 1.0 / den * (dot(t2l, d) - on_line * c);
+        gp = mul(-1.0 / den, sub(d, mul(c, a)));
+        gd = mul(1.0 / den, add(sub(t2l, mul(on_line, a)), mul(2.0 * c * s, a)));
+        return true;
+    }
+    if (r.shape == DTR_CONCENTRIC_CYLINDER2D) {
+        // the portal intersector (ray_cylinder_portal_intersector.hpp:68-132):
+        // within 1e-3 of R^2 its path is R^2 - rho^2 itself
+        const double R = r.m[0];
+        const double rd2 = d.x * d.x + d.y * d.y;
+        if (rd2 < 2.220446049250313e-16) return false;
+        const double rad_diff = R * R - (p.x * p.x + p.y * p.y);
+        if (!(fabs(rad_diff) > 1.0 * 1e-3)) {
+            s = rad_diff;
+            gp = {-2.0 * p.x, -2.0 * p.y, 0.0};
+            gd = {0.0, 0.0, 0.0};
+            return true;
+        }
+        const double inv2 = 1.0 / rd2;
+        const double kk = -inv2 * (p.x * d.x + p.y * d.y);
+        const double discr = inv2 * rad_diff + kk * kk;
+        if (discr < 0.0) return false;
+        const double sq = sqrt(discr);
+        s = fabs(kk + sq - near) <= fabs(kk - sq - near) ? kk + sq : kk - sq;
+        const V3 pu{p.x + s * d.x, p.y + s * d.y, 0.0};
+        const double den = dot(pu, d);
+        gp = mul(-1.0 / den, pu);
+        gd = mul(-s / den, pu);
+        return true;
+    }
+    if (r.shape == DTR_CYLINDER2D) {
+        const V3 a = rec_col(r, 2);
+        double sol[2];
+        dtr_transform t;
+        for (int i = 0; i < 9; ++i) t.r[i] = r.r[i];
+        for (int i = 0; i < 3; ++i) t.t[i] = r.t[i];
+        const int ns = Engine::cyl2_intersect(p, d, t, r.m[0], sol);
+        if (ns == 0) return false;
+        s = (ns == 2 && fabs(sol[1] - near) < fabs(sol[0] - near)) ? sol[1] : sol[0];
+        const V3 u = add(sub(p, tr), mul(s, d));
+        const V3 pu = sub(u, mul(dot(u, a), a));
+        const double den = dot(pu, d);
+        gp = mul(-1.0 / den, pu);
+        gd = mul(-s / den, pu);
+        return true;
+    }
+    const V3 n = rec_col(r, 2);
+    const double nd = dot(n, d);
+    if (nd == 0.0) return false;
+    s = dot(n, sub(tr, p)) / nd;
+    gp = mul(-1.0 / nd, n);
+    gd = mul(-s / nd, n);
+    return true;
 }
 
-// The step before a surface event had the navigator's distance to the
-// plane `sf` as its length, h = n.(t - p_c) / (n.d_c), computed at an
-// earlier state (p_c, d_c): usually the step's start, but the distance is
-// not refreshed after a setter of a surface reached without moving.  hbar
-// is the adjoint of h, (td, tk) = dx_end/dh the tangent at the end of the
-// step; (p_c, d_c) receive -hbar/(n.d_c) (n, h n).
-struct Landing {
-    bool active;
-    uint32_t sf;
-    double hbar;
-    V3 td, tk;
-};
+// A step sized by the navigator has the distance to its target surface as
+// its length, h = s(p_c, d_c) (target_path), computed at an earlier state
+// (p_c, d_c): usually the step's start, but the distance is not refreshed
+// after a setter of a surface reached without moving.  hbar, the adjoint of
+// h, is dx_end/dh . lambda_end with lambda_end the adjoint of the state the
+// step ended in; (p_c, d_c) receive hbar (ds/dp_c, ds/dd_c).  Every such
+// step counts, also one that ends on a portal or short of its target: the
+// next step starts where it ended, and a wire's closest approach from a
+// point of the trajectory moves with the direction there (by loc0 times
+// the change of angle), so the dependence does not cancel.
+//
+// dx_end/dh: the tangent of the trajectory at the end of the step, (d,
+// q/p d x B), when the stages were taken at h (the Runge-Kutta map follows
+// the trajectory to the integration tolerance); the derivative of the
+// advance at fixed stages when the step was clipped after its stages were
+// taken at hs != h (rk_stepper.hpp:474-511).
+template <typename S>
+DTR_HD void step_tangent(const DevField& field, V3 pos, V3 dirv, double qopd, double hs, double ha, V3& td, V3& tk) {
+    const Stages<S> st = rk_stages<S>(field, pos, dirv, qopd, hs);
+    V3 pe, de;
+    const double nn = rk_advance(st, pos, dirv, ha, pe, de);
+    if (hs == ha) {
+        const T3<double> B = field_at_s<double>(field, pe);
+        td = de;
+        tk = mul(qopd, cross(de, V3{B.x, B.y, B.z}));
+        return;
+    }
+    const V3 k1 = tdbl(st.k1), k2 = tdbl(st.k2), k3 = tdbl(st.k3), k4 = tdbl(st.k4);
+    td = add(dirv, mul(ha * (1.0 / 3.0), add(add(k1, k2), k3)));
+    const V3 dnd = mul(1.0 / 6.0, add(add(k1, mul(2.0, add(k2, k3))), k4));
+    tk = mul(1.0 / nn, sub(dnd, mul(dot(de, dnd), de)));
+}
+
 struct Pending {
     int32_t ev;  // event whose state is (p_c, d_c), -1: none
     bool post;   // its state after (true) or before (false) the event
     V3 pc, dc;   // adjoint contribution to add there
 };
-DTR_HD void landing_contrib(V3 n, double hbar, double h, V3 dc, V3& pc_out, V3& dc_out) {
-    const double nd = dot(n, dc);
-    pc_out = mul(-hbar / nd, n);
-    dc_out = mul(-hbar * h / nd, n);
-}
 // Adjoint of event ev: (pb, db, qb) enter as the adjoint of the state after
 // the event and leave as that of the state before it.  Returns false for
 // the start event (its bound parameters are the initial ones), whose bound
-// adjoint is written to grad.
+// adjoint is written to grad.  pbb: the adjoint of the bound parameters
+// (after material) of the last bound event, collected from later events
+// that reused them (kEvReused) and handed to that event.
 DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
                           bool exact, uint32_t gid, uint32_t col, uint64_t ev, double mass, double charge, V3& pb,
-                          V3& db, double& qb, double grad[6], Landing& ld) {
+                          V3& db, double& qb, double grad[6], double pbb[6]) {
     const Tape& tp = io.tape;
     const uint64_t N = io.n;
     const uint32_t sfi = tp.ev_sf[ev * N + col];
@@ -716,7 +921,11 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
     for (int k = 0; k < 6; ++k) bp[k] = b[k];
     bp[4] = tp.ev_post[(ev * 7u + 6u) * N + col];
     const bool mat = (flags & kEvMaterial) != 0u;
-    const bool restart = mat || !exact;  // the state after the event is B2F(bound)
+    const bool reused = (flags & kEvReused) != 0u;
+    // the state after the event is B2F(bound): after the setter of a
+    // material event, and in the transport model after every event that
+    // binds anew (a reused event continues the reference's Jacobian chain)
+    const bool restart = reused ? (mat && exact) : (mat || !exact);
     V3 cpb{0.0, 0.0, 0.0}, cdb{0.0, 0.0, 0.0};
     double cqb = 0.0;
     for (int k = 0; k < 6; ++k) bb[k] = 0.0;
@@ -726,6 +935,13 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
         cpb = pb;
         cdb = db;
         cqb = qb;
+    }
+    if (!reused) {
+        // later events that reused these bound parameters (after material)
+        for (int k = 0; k < 6; ++k) {
+            bb[k] += pbb[k];
+            pbb[k] = 0.0;
+        }
     }
     if (mat && exact && cfg.do_eloss) {
         const dtr_material_slab& m = det.materials[rec.material];
@@ -743,6 +959,15 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
         bb[0] += 2.0 * io.iv0 * (b[0] - io.hits.loc[(0u * mh + hit) * N + gid]);
         bb[1] += 2.0 * io.iv1 * (b[1] - io.hits.loc[(1u * mh + hit) * N + gid]);
     }
+    if (reused) {
+        // the free state before the event is discarded by a setter (or
+        // passes through); the bound adjoint belongs to the last bound event
+        for (int k = 0; k < 6; ++k) pbb[k] += bb[k];
+        pb = cpb;
+        db = cdb;
+        qb = cqb;
+        return true;
+    }
     if (!(flags & kEvFromFree)) {
         // start event: b is the initial bound vector
         double cb[6];
@@ -752,33 +977,16 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
     }
     const double* q = tp.ev_pre + ev * 10u * N + col;
     const V3 p{q[0 * N], q[1 * N], q[2 * N]}, d{q[3 * N], q[4 * N], q[5 * N]};
-    V3 k{q[7 * N], q[8 * N], q[9 * N]};
-    if (exact) {
-        const T3<double> B = field_at_s<double>(field, p);
-        k = mul(q[6 * N], cross(d, V3{B.x, B.y, B.z}));
-    }
-    // exact: the step before the event was sized to reach a plane (its
-    // target, which need not be this surface where surfaces overlap): the
-    // projection is plain and the step length carries the dependence
-    // (Landing); a cylinder target gets the path correction onto it.
-    // Transport: the reference's path correction onto this surface.
-    const uint32_t land = tp.ev_land[ev * N + col];
-    const bool by_length = exact && land != kInvalid && det.sf[land].shape != DTR_CONCENTRIC_CYLINDER2D;
-    const NavSurface* lrec = exact ? (land != kInvalid && !by_length ? &det.sf[land] : nullptr) : &rec;
+    const V3 k{q[7 * N], q[8 * N], q[9 * N]};
+    // exact: a plain projection (the length of the step that reached the
+    // surface carries the dependence on the earlier state); transport: the
+    // reference's path correction onto this surface
     V3 fpb, fdb;
     double fqb;
-    f2b_transpose(rec, p, d, k, lrec, bb, fpb, fdb, fqb);
+    f2b_transpose(rec, p, d, k, exact ? nullptr : &rec, exact, bb, fpb, fdb, fqb);
     pb = add(cpb, fpb);
     db = add(cdb, fdb);
     qb = cqb + fqb;
-    if (by_length) {
-        // dx_end/dh = (d, dt/ds) at the end of the step; hbar is formed
-        // once the adjoint of the end state is complete (event_step_adjoint)
-        ld.active = true;
-        ld.sf = land;
-        ld.td = d;
-        ld.tk = k;
-    }
     return true;
 }
 
@@ -788,7 +996,7 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
 DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, const DevConfig& cfg,
                                const AdjointIO& io, bool exact, uint32_t gid, uint32_t col, int32_t e, double mass,
                                double charge,
-                               V3& pb, V3& db, double& qb, double grad[6], Landing& ld, Pending& pend,
+                               V3& pb, V3& db, double& qb, double grad[6], double pbb[6], Pending& pend,
                                const double b0[6]) {
     if (pend.ev == e && pend.post) {
         pb = add(pb, pend.pc);
@@ -796,7 +1004,7 @@ DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, c
         pend.ev = -1;
     }
     const bool more = event_adjoint(det, field, cfg, io, exact, gid, col, static_cast<uint64_t>(e), mass, charge, pb,
-                                    db, qb, grad, ld);
+                                    db, qb, grad, pbb);
     if (pend.ev == e && !pend.post) {
         if (more) {
             pb = add(pb, pend.pc);
@@ -808,9 +1016,6 @@ DTR_AI bool event_step_adjoint(const FastDetector& det, const DevField& field, c
         }
         pend.ev = -1;
     }
-    // the landing step of this event: hbar from the complete adjoint of the
-    // state it ended in
-    if (ld.active) ld.hbar = dot(ld.td, pb) + dot(ld.tk, db);
     return more;
 }
 
@@ -837,8 +1042,8 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
     double qb = 0.0;
     int32_t e = n_ev - 1;
     bool open = true;  // the start event has not been reached
-    Landing ld{false, 0u, 0.0, V3{0.0, 0.0, 0.0}, V3{0.0, 0.0, 0.0}};
     Pending pend{-1, false, V3{0.0, 0.0, 0.0}, V3{0.0, 0.0, 0.0}};
+    double pbb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t K = tp.every;
     const uint32_t n_seg = (n_steps + K - 1u) / K;
     for (int32_t sgi = static_cast<int32_t>(n_seg) - 1; sgi >= 0; --sgi) {
@@ -869,7 +1074,7 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
         // back through the segment: the events after step j, then step j
         for (uint32_t j = j1; j > j0; --j) {
             while (e >= 0 && tp.ev_step[static_cast<uint64_t>(e) * N + col] == j) {
-                event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, ld,
+                event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, pbb,
                                    pend, b0);
                 e--;
             }
@@ -878,23 +1083,30 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
             tape_get_state(tp.seg, static_cast<uint64_t>(j - j0 - 1u) * 7u, N, col, sp, sd, sq);
             const double hs = tp.h[(2ull * (j - 1u) + 0u) * N + col];
             const double ha = tp.h[(2ull * (j - 1u) + 1u) * N + col];
+            // a step sized by the navigator: hbar from the adjoint of the
+            // state it ended in (complete once the events after it are done)
+            const uint32_t tgt = exact ? tp.step_land[static_cast<uint64_t>(j - 1u) * N + col] : kInvalid;
+            double hbar = 0.0;
+            if (tgt != kInvalid) {
+                V3 td, tk;
+                step_tangent<S>(field, sp, sd, sq, hs, ha, td, tk);
+                hbar = dot(td, pb) + dot(tk, db);
+            }
             rk_vjp<S>(field, sp, sd, sq, hs, ha, exact, pb, db, qb);
-            if (ld.active) {
+            if (tgt != kInvalid) {
                 // the state the step length was computed from: the step's
                 // start, or a state of the events after the previous step
-                ld.active = false;
-                const NavSurface& tr = det.sf[ld.sf];
-                const V3 n = rec_col(tr, 2);
+                const NavSurface& tr = det.sf[tgt];
+                V3 gp, gd;
+                // whether (p, d) gives the step length; its gradient in gp, gd
                 auto same = [&](V3 p, V3 d) {
-                    const double s = plane_path(tr, p, d);
-                    return fabs(s - ha) <= 1e-9 * fabs(ha) + 1e-15;
+                    double s;
+                    return target_path(tr, p, d, ha, s, gp, gd) && fabs(s - ha) <= 1e-9 * fabs(ha) + 1e-15;
                 };
                 bool found = same(sp, sd);
                 if (found) {
-                    V3 cp, cd;
-                    landing_contrib(n, ld.hbar, ha, sd, cp, cd);
-                    pb = add(pb, cp);
-                    db = add(db, cd);
+                    pb = add(pb, mul(hbar, gp));
+                    db = add(db, mul(hbar, gd));
                 }
                 for (int32_t e2 = e; !found && e2 >= 0 && tp.ev_step[static_cast<uint64_t>(e2) * N + col] == j - 1u;
                      --e2) {
@@ -912,28 +1124,32 @@ DTR_AI void adjoint_backward(const FastDetector& det, const DevField& field, con
                             found = true;
                             pend.ev = e2;
                             pend.post = (w == 0);
-                            landing_contrib(n, ld.hbar, ha, d, pend.pc, pend.dc);
+                            pend.pc = mul(hbar, gp);
+                            pend.dc = mul(hbar, gd);
                         }
                     }
                 }
                 if (!found) {
                     // not identified: taken from the step's start
-                    V3 cp, cd;
-                    landing_contrib(n, ld.hbar, ha, sd, cp, cd);
-                    pb = add(pb, cp);
-                    db = add(db, cd);
+                    double s;
+                    if (target_path(tr, sp, sd, ha, s, gp, gd)) {
+                        pb = add(pb, mul(hbar, gp));
+                        db = add(db, mul(hbar, gd));
+                    }
                 }
             }
         }
     }
     while (e >= 0 && open) {
-        open = event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, ld, pend,
-                                  b0);
+        open = event_step_adjoint(det, field, cfg, io, exact, gid, col, e, mass, charge, pb, db, qb, grad, pbb,
+                                  pend, b0);
         e--;
     }
     if (open) {
-        // no event on the start surface: the initial state is B2F(b0)
+        // no event on the start surface: the initial state is B2F(b0), and
+        // events that reused the start parameters add their adjoint
         b2f_transpose(det.sf[io.start_sf[gid]], b0, pb, db, qb, grad);
+        for (int k = 0; k < 6; ++k) grad[k] += pbb[k];
     }
     for (int k = 0; k < 6; ++k) grad_out[k * N + gid] = grad[k];
 }

@@ -588,14 +588,14 @@ enum dtr_tape_status {
     DTR_TAPE_OK = 0,
     DTR_TAPE_STEP_OVERFLOW = 1,   /* more than max_steps steps */
     DTR_TAPE_EVENT_OVERFLOW = 2,  /* more than max_events surfaces */
-    /* not differentiable here: the track stopped in material, or an event
-     * reused the bound parameters of an earlier one */
+    /* not differentiable here: the track stopped in material, or it
+     * started free */
     DTR_TAPE_UNSUPPORTED = 3
 };
 
 /* Defaults of dtr_adjoint_config. */
 DTR_API void dtr_default_adjoint(dtr_adjoint_config* cfg);
-/* Bytes of tape for n tracks (about 24 KB per track at the defaults). */
+/* Bytes of tape for n tracks (about 26 KB per track at the defaults). */
 DTR_API size_t dtr_adjoint_tape_size(const dtr_adjoint_config* cfg, uint32_t n);
 
 /* Forward pass: propagator::propagate of every track from its bound start

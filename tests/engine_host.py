@@ -120,3 +120,35 @@ def propagate_direct(detector, field, config, free_params, sequence, n_sequence,
                                                     C.c_int(1 if mixed else 0)),
                "dtr_test_engine_propagate_direct")
     return out
+
+
+def adjoint(detector, field, config, adjoint_config, bound_params, start_surface, hit_surface=None, hit_loc=None,
+            n_hits=None, transport_gradient: bool = False, backward: bool = True) -> dict:
+    """Both passes of the adjoint (adjoint_record, adjoint_backward) over
+    host arrays: chi2 [n], status [n], grad [6, n] (backward) and the
+    Jacobian-chain gradient grad_transport [6, n] (transport_gradient)."""
+    lib = load()
+    bp = np.ascontiguousarray(bound_params, dtype=np.float64)
+    ss = np.ascontiguousarray(start_surface, dtype=np.uint32)
+    n = bp.shape[1]
+    out = {"chi2": np.zeros(n), "status": np.zeros(n, dtype=np.uint32),
+           "grad": np.zeros((6, n)) if backward else None,
+           "grad_transport": np.zeros((6, n)) if transport_gradient else None}
+    hits = None
+    if hit_surface is not None:
+        hs = np.ascontiguousarray(hit_surface, dtype=np.uint32)
+        hl = np.ascontiguousarray(hit_loc, dtype=np.float64)
+        hn = np.ascontiguousarray(n_hits, dtype=np.uint32)
+        out["_keep"] = (hs, hl, hn)
+        hits = capi.Hits(int(hs.shape[0]), 0, hs.ctypes.data, hl.ctypes.data, hn.ctypes.data)
+    cfg = config.to_c()
+    acfg = adjoint_config.to_c()
+    ptr = lambda a: None if a is None else C.c_void_p(a.ctypes.data)  # noqa: E731
+    lib.dtr_test_adjoint.restype = C.c_int
+    capi.check(lib.dtr_test_adjoint(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(acfg),
+                                    C.c_uint32(n), ptr(bp), ptr(ss), C.byref(hits) if hits is not None else None,
+                                    ptr(out["chi2"]), ptr(out["grad_transport"]), ptr(out["grad"]),
+                                    ptr(out["status"])),
+               "dtr_test_adjoint")
+    out.pop("_keep", None)
+    return out

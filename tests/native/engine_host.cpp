@@ -10,10 +10,13 @@
 //    and results as dtr_propagate_ex, to compare with the oracle on the CPU.
 //  * dtr_test_engine_propagate_direct: the direct navigator's loop
 //    (propagate_direct) the same way, forward or backward.
+//  * dtr_test_adjoint: the adjoint's recording pass and backward pass
+//    (adjoint_record, adjoint_backward) track by track on the host.
 #include <cstdint>
 #include <cstring>
 #include <vector>
 
+#include "device/adjoint.hpp"
 #include "device/direct_nav.hpp"
 #include "device/engine.hpp"
 
@@ -140,6 +143,20 @@ void propagate_direct_all(const HostDetector& hd, const DevField& field, const D
         propagate_direct<P>(hd.fast, field, cfg, dd, in, out, sq, static_cast<uint32_t>(t), c);
 }
 
+template <class P>
+void adjoint_all(const HostDetector& hd, const DevField& field, const DevConfig& cfg, const AdjointIO& io,
+                 bool exact, double* grad) {
+    double path[kCache];
+    uint32_t sfs[kCache], ls[kCache];
+    const CacheView c{path, sfs, ls, 1};
+    for (uint64_t t = 0; t < io.n; ++t)
+        adjoint_record<P>(hd.fast, field, cfg, io, static_cast<uint32_t>(t), static_cast<uint32_t>(t), c);
+    if (grad)
+        for (uint64_t t = 0; t < io.n; ++t)
+            adjoint_backward<P>(hd.fast, field, cfg, io, exact, static_cast<uint32_t>(t), static_cast<uint32_t>(t),
+                                grad);
+}
+
 DevField host_field(const dtr_field_desc* f) {
     DevField field{};
     field.kind = f->kind;
@@ -238,6 +255,48 @@ __attribute__((visibility("default"))) int dtr_test_engine_propagate_direct(
         if (vol) propagate_direct_all<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(hd, field, dc, ti, r, sq);
         else if (ext) propagate_direct_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r, sq);
         else propagate_direct_all<PolicyMixed>(hd, field, dc, ti, r, sq);
+    }
+    return DTR_SUCCESS;
+}
+
+// Both passes of the adjoint over host arrays (the tape is allocated here):
+// chi2 [n], grad_transport [6][n] (or null: no Jacobian chain), grad [6][n]
+// (or null: forward pass only), status [n].  The detector may hold the
+// extended geometry but no volume material.
+__attribute__((visibility("default"))) int dtr_test_adjoint(const dtr_detector_desc* d, const dtr_field_desc* f,
+                                                            const dtr_prop_config* cfg,
+                                                            const dtr_adjoint_config* a, uint32_t n,
+                                                            const double* bound, const uint32_t* start,
+                                                            const dtr_hits* hits, double* chi2,
+                                                            double* grad_transport, double* grad, uint32_t* status) {
+    if (!d || !f || !cfg || !a || !bound || !start || !chi2 || !status) return DTR_ERROR_INVALID_ARGUMENT;
+    if (has_volume_material(*d) || cfg->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (a->max_steps == 0u || a->max_events == 0u || a->checkpoint_every == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    HostDetector hd(*d);
+    const DevField field = host_field(f);
+    DevConfig dc = make_dev_config(cfg);
+    dc.max_surfaces = 0u;
+    if (!grad_transport && !cfg->estimate_scattering_noise) dc.do_cov = 0u;
+    std::vector<unsigned char> tape(tape_bytes(n, a->max_steps, a->max_events, a->checkpoint_every));
+    AdjointIO io{};
+    io.bound0 = bound;
+    io.start_sf = start;
+    io.n = n;
+    if (hits) io.hits = HitsView{hits->max_hits, hits->surface, hits->loc, hits->n_hits};
+    io.iv0 = 1.0 / (a->sigma_loc0 * a->sigma_loc0);
+    io.iv1 = 1.0 / (a->sigma_loc1 * a->sigma_loc1);
+    io.tape = tape_carve(tape.data(), n, a->max_steps, a->max_events, a->checkpoint_every);
+    io.chi2 = chi2;
+    io.grad_transport = grad_transport;
+    io.status = status;
+    const bool exact = a->model == DTR_ADJOINT_EXACT;
+    const bool ext = has_ext(*d);
+    if (a->precision == DTR_PRECISION_FP64) {
+        if (ext) adjoint_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, io, exact, grad);
+        else adjoint_all<PolicyFP64>(hd, field, dc, io, exact, grad);
+    } else {
+        if (ext) adjoint_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, io, exact, grad);
+        else adjoint_all<PolicyMixed>(hd, field, dc, io, exact, grad);
     }
     return DTR_SUCCESS;
 }

@@ -34,7 +34,7 @@ def test_tape_size(ms, me, k):
     a = da.AdjointConfig(max_steps=ms, max_events=me, checkpoint_every=k).to_c()
     n = 1000
     doubles = 2 * ms + 7 * (ms // k + 1) + (10 + 6 + 7) * me + 7 * k + 36
-    words = 5 * me + 3
+    words = 4 * me + ms + 3  # per event: step, surface, hit, flags; per step: its target surface
     assert capi.load().dtr_adjoint_tape_size(C.byref(a), n) == n * (8 * doubles + 4 * words)
     bad = da.AdjointConfig(checkpoint_every=0).to_c()
     assert capi.load().dtr_adjoint_tape_size(C.byref(bad), n) == 0

@@ -1,0 +1,91 @@
+"""The adjoint kernels on the extended geometry (wire chamber: line frames,
+stereo wires, tungsten rods; telescopes of annuli and of placed cylinders
+crossed twice), on the device, against the oracle: the workloads and
+contracts of tests/test_adjoint_ext_host.py (which checks the host build of
+the same code) in the solenoid map and in a constant field, for the FP64
+and mixed policies:
+* exact adjoint vs central finite differences of the oracle's FP64 chi2:
+  max <= 1e-3, median <= 1e-5 (unstable difference quotients excluded, at
+  most 3 %);
+* transport adjoint vs the Jacobian-chain gradient of the forward pass:
+  1e-6 (FP64), 1e-4 (mixed);
+* forward chi2 vs the oracle's: 1e-9 (FP64);
+* checkpoint interval, track order and register budget leave the result
+  bit-identical."""
+
+import numpy as np
+import pytest
+
+import detray_amd as da
+from detray_amd import adjoint as adj
+from tests import oracle_host
+from tests.test_adjoint_ext_host import CONFIG, FIELDS, MAX_HITS, hits_for, oracle_fd, scaled_error, workload
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=["chamber", "annulus", "cylinder"])
+def case(request, solenoid):
+    kind = request.param
+    det, b0, start, sig = workload(kind, 256)
+    out = {}
+    for fname, field in (("const", FIELDS[kind]()), ("solenoid", solenoid)):
+        hits = hits_for(det, field, b0, start, sig)
+        dev = da.Hits(torch.from_numpy(hits[0].view(np.int32)).cuda(), torch.from_numpy(hits[1]).cuda(),
+                      torch.from_numpy(hits[2].view(np.int32)).cuda())
+        out[fname] = dict(field=field, hits=hits, dev=dev, fd=oracle_fd(det, field, b0, start, sig, hits))
+    return dict(kind=kind, det=det, b0=b0, start=start, sig=sig, fields=out)
+
+
+def run(c, fname, precision, model="exact", transport=False, every=32, sort="eta_phi", waves=2):
+    f = c["fields"][fname]
+    prop = da.Chi2Propagator(CONFIG, da.AdjointConfig(precision=precision, model=model, checkpoint_every=every,
+                                                      sort=sort, waves_per_simd=waves))
+    bound = torch.from_numpy(c["b0"]).cuda()
+    start = torch.from_numpy(c["start"]).cuda()
+    fwd = prop.forward(c["det"], f["field"], bound, start, f["dev"], transport_gradient=transport)
+    g = prop.backward(c["det"], f["field"], bound, start, f["dev"], fwd)
+    torch.cuda.synchronize()
+    return dict(chi2=fwd.chi2.cpu().numpy(), status=fwd.status.cpu().numpy(), grad=g.cpu().numpy(),
+                gt=fwd.grad_transport.cpu().numpy() if transport else None)
+
+
+@pytest.mark.parametrize("fname", ["const", "solenoid"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_exact_adjoint_matches_oracle_finite_differences(case, fname, precision):
+    fd, stable = case["fields"][fname]["fd"]
+    r = run(case, fname, precision)
+    assert (r["status"] == 0).all()
+    assert stable.mean() >= 0.97
+    e = scaled_error(r["grad"], fd, case["sig"])[stable]
+    print(f"[{case['kind']} {fname} {precision}] exact adjoint vs oracle FD: max {e.max():.2e} "
+          f"median {np.median(e):.2e}")
+    assert e.max() <= 1e-3
+    assert np.median(e) <= 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("mixed", 1e-4)])
+def test_transport_adjoint_matches_jacobian_chain(case, precision, tol):
+    r = run(case, "solenoid", precision, model="transport", transport=True)
+    assert (r["status"] == 0).all()
+    e = scaled_error(r["grad"], r["gt"], case["sig"])
+    print(f"[{case['kind']} {precision}] transport adjoint vs Jacobian chain: max {e.max():.2e}")
+    assert e.max() <= tol
+
+
+def test_forward_chi2_matches_oracle(case):
+    f = case["fields"]["solenoid"]
+    r = run(case, "solenoid", "fp64")
+    ref = oracle_host.chi2(oracle_host.propagate_hits(case["det"], f["field"], CONFIG, case["b0"], case["start"],
+                                                      MAX_HITS), *f["hits"], adj.SIGMA_LOC0, adj.SIGMA_LOC1)
+    np.testing.assert_allclose(r["chi2"], ref, rtol=1e-9, atol=1e-12)
+
+
+def test_schedule_does_not_change_the_gradient(case):
+    ref = run(case, "solenoid", "mixed")
+    for kw in (dict(every=1), dict(every=7), dict(sort="none"), dict(waves=1)):
+        r = run(case, "solenoid", "mixed", **kw)
+        np.testing.assert_array_equal(r["status"], ref["status"])
+        np.testing.assert_array_equal(r["grad"], ref["grad"])
