@@ -15,6 +15,7 @@ from .propagator import (  # noqa: F401
     MagneticField,
     PropagationConfig,
     PropagationResult,
+    PropagationState,
     Propagator,
     StepTrace,
     generate_tracks,
@@ -23,5 +24,5 @@ from .propagator import (  # noqa: F401
 
 __all__ = [
     "AdjointConfig", "Chi2Propagator", "Hits", "chi2", "DetrayError", "load", "Detector", "LaunchConfig", "MagneticField", "PropagationConfig",
-    "PropagationResult", "Propagator", "StepTrace", "Actors", "generate_tracks", "material", "MM", "UM", "GEV", "MEV", "TESLA",
+    "PropagationResult", "PropagationState", "Propagator", "StepTrace", "Actors", "generate_tracks", "material", "MM", "UM", "GEV", "MEV", "TESLA",
 ]

@@ -27,6 +27,7 @@ STATUS_NAMES = {
 
 # dtr_track_status
 TRACK_EXIT, TRACK_ABORT, TRACK_PATH_LIMIT, TRACK_OVERFLOW, TRACK_MOMENTUM_LIMIT, TRACK_TARGET = 0, 1, 2, 3, 4, 5
+TRACK_PAUSED = 6
 # dtr_mask_shape
 RECTANGLE2D, TRAPEZOID2D, CONCENTRIC_CYLINDER2D, RING2D, LINE_SQUARE, LINE_CIRCULAR = 0, 1, 2, 3, 4, 5
 ANNULUS2D, CYLINDER2D = 6, 7
@@ -278,6 +279,14 @@ class SurfaceSequence(C.Structure):
     ]
 
 
+class PropagationState(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("max_steps", C.c_uint32),
+        ("resume", C.c_uint32),
+    ]
+
+
 class WireChamberConfig(C.Structure):
     _fields_ = [
         ("n_layers", C.c_uint32),
@@ -345,6 +354,10 @@ _SIGNATURES = {
     "dtr_propagate_direct": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(TrackBatch),
                                        C.POINTER(SurfaceSequence), C.POINTER(ResultBatch),
                                        C.POINTER(LaunchConfig), C.c_void_p]),
+    "dtr_propagation_state_size": (C.c_size_t, [C.c_uint32]),
+    "dtr_propagate_resumable": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(TrackBatch),
+                                          C.POINTER(ResultBatch), C.POINTER(LaunchConfig),
+                                          C.POINTER(PropagationState), C.c_void_p]),
     "dtr_default_adjoint": (None, [C.POINTER(AdjointConfig)]),
     "dtr_adjoint_tape_size": (C.c_size_t, [C.POINTER(AdjointConfig), C.c_uint32]),
     "dtr_adjoint_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(PropConfig), C.POINTER(AdjointConfig),

@@ -668,6 +668,63 @@ int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field, const 
     return dtr::launch_direct<dtr::PolicyMixed>(det, field, dc, dd, in, r, sq, block, L.order, st);
 }
 
+size_t dtr_propagation_state_size(uint32_t n) {
+    return static_cast<size_t>(n) * (dtr::kPauseD * sizeof(double) + dtr::kPauseW * sizeof(uint32_t));
+}
+
+int dtr_propagate_resumable(const dtr_detector* det, const dtr_field* field, const dtr_prop_config* c,
+                            const dtr_track_batch* in, dtr_result_batch* out, const dtr_launch_config* launch,
+                            const dtr_propagation_state* state, void* stream) {
+    const int rc = check_propagate_args(det, field, c, in, out);
+    if (rc != DTR_SUCCESS) return rc;
+    if (!launch || !state) return DTR_ERROR_INVALID_ARGUMENT;
+    dtr_launch_config L = *launch;
+    if (L.block_size == 0u) L.block_size = 256u;
+    if (L.block_size % 64u != 0u || L.block_size > 256u || L.xcd_chunk > 65536u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.waves_per_simd > 2u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.precision != DTR_PRECISION_FP64 && L.precision != DTR_PRECISION_MIXED) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.navigation > DTR_NAVIGATION_GATHER || L.schedule > DTR_SCHEDULE_PERSISTENT)
+        return DTR_ERROR_INVALID_ARGUMENT;
+    if (state->max_steps == 0u || state->resume > 1u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (L.schedule != DTR_SCHEDULE_GRID || L.navigation == DTR_NAVIGATION_WAVE) return DTR_ERROR_UNSUPPORTED;
+    if (c->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+    if (in->n == 0u) return DTR_SUCCESS;
+    if (!state->data) return DTR_ERROR_INVALID_ARGUMENT;
+    const dtr::DevConfig dc = dtr::make_dev_config(c);
+    dtr::DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                      c->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                      out->n_trials, out->status};
+    dtr::PauseView pv;
+    pv.n = in->n;
+    pv.d = static_cast<double*>(state->data);
+    pv.w = reinterpret_cast<uint32_t*>(pv.d + static_cast<uint64_t>(dtr::kPauseD) * in->n);
+    pv.max_steps = state->max_steps;
+    pv.resume = state->resume;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int gather = L.navigation == DTR_NAVIGATION_GATHER ? 1 : 0;
+    const bool fp64 = L.precision == DTR_PRECISION_FP64;
+    using namespace dtr;
+    if (det->volume_material) {
+        if (fp64)
+            return launch_resumable<WithVolumeMaterial<WithExtendedGeometry<PolicyFP64>>>(
+                det, field, dc, in, r, L.block_size, L.order, gather, L.xcd_chunk, pv, st);
+        return launch_resumable<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(
+            det, field, dc, in, r, L.block_size, L.order, gather, L.xcd_chunk, pv, st);
+    }
+    if (det->wires) {
+        if (fp64)
+            return launch_resumable<WithExtendedGeometry<PolicyFP64>>(det, field, dc, in, r, L.block_size, L.order,
+                                                                      gather, L.xcd_chunk, pv, st);
+        return launch_resumable<WithExtendedGeometry<PolicyMixed>>(det, field, dc, in, r, L.block_size, L.order,
+                                                                   gather, L.xcd_chunk, pv, st);
+    }
+    if (fp64)
+        return launch_resumable<PolicyFP64>(det, field, dc, in, r, L.block_size, L.order, gather, L.xcd_chunk, pv,
+                                            st);
+    return launch_resumable<PolicyMixed>(det, field, dc, in, r, L.block_size, L.order, gather, L.xcd_chunk, pv,
+                                         st);
+}
+
 void dtr_default_adjoint(dtr_adjoint_config* a) {
     if (!a) return;
     std::memset(a, 0, sizeof(*a));

@@ -2077,6 +2077,36 @@ struct TrackInputs {
     uint64_t n;
 };
 
+// A propagation that pauses and resumes (propagator.hpp:238-250, 274-282):
+// the reference pauses a track when an actor stops its heartbeat while the
+// navigation is alive, and a later propagate() call continues it without
+// initialising the navigation.  Here the pause comes from a budget of
+// Runge-Kutta steps per call, checked where the actor half of the loop
+// begins (where such an actor, first in the chain, would run), and the
+// lane's whole state goes to a device buffer, structure of arrays over the
+// tracks of the batch: kPauseD doubles and kPauseW words per track.
+enum PauseSlotD : int {
+    kPdPos = 0, kPdDir = 3, kPdTime = 6, kPdQop = 7, kPdStep = 8, kPdNextStep = 9, kPdPath = 10,
+    kPdAbsPath = 11, kPdConstraint = 12, kPdDtds = 13, kPdDqopds = 16, kPdHStage = 17, kPdExtTol = 18,
+    kPdNavDir = 19, kPdBound = 20, kPdJ = 26, kPdCov = 51, kPdJx = 72, kPdCache = 90, kPauseD = 98
+};
+enum PauseSlotW : int {
+    kPwTrials = 0, kPwSteps = 1, kPwUpSf = 2, kPwSeq = 3, kPwTrace = 4, kPwNav = 5, kPwVolFlags = 6,
+    kPwStall = 7, kPwState = 8, kPwCacheSf = 9, kPwCacheLs = 17, kPauseW = 25
+};
+// kPwState: the first call writes kPaused or kDone for every track; a
+// resuming call runs the kPaused ones
+enum : uint32_t { kLanePaused = 1u, kLaneDone = 2u };
+struct PauseView {
+    double* d;    // [kPauseD][n]
+    uint32_t* w;  // [kPauseW][n]
+    uint64_t n;
+    uint32_t max_steps;  // steps a track takes in one call before it pauses
+    uint32_t resume;     // 0: start the tracks from the inputs; 1: continue the paused ones
+    DTR_AI double& dv(int slot, uint32_t t) const { return d[static_cast<uint64_t>(slot) * n + t]; }
+    DTR_AI uint32_t& wv(int slot, uint32_t t) const { return w[static_cast<uint64_t>(slot) * n + t]; }
+};
+
 // propagator::propagate (propagator.hpp:262-400) for one track, split into
 // phases so that the lanes of a wave can meet for the local navigations
 // (each lane's navigation cascade stops where it needs one) and a persistent
@@ -2288,6 +2318,125 @@ struct Lane {
         if (out.n_trials) out.n_trials[gid] = tk.n_trials;
     }
 
+    // The lane's state at the start of an actor half, to the pause buffer
+    // and back; tk.bound, the FP32 Jacobian and covariance of the stage
+    // precision and the candidate cache included.
+    DTR_AI void save(const PauseView& s, CacheView c) const {
+        const uint32_t t = gid;
+        s.dv(kPdPos + 0, t) = tk.pos.x;
+        s.dv(kPdPos + 1, t) = tk.pos.y;
+        s.dv(kPdPos + 2, t) = tk.pos.z;
+        s.dv(kPdDir + 0, t) = tk.dir.x;
+        s.dv(kPdDir + 1, t) = tk.dir.y;
+        s.dv(kPdDir + 2, t) = tk.dir.z;
+        s.dv(kPdTime, t) = tk.time;
+        s.dv(kPdQop, t) = tk.qop;
+        s.dv(kPdStep, t) = tk.step_size;
+        s.dv(kPdNextStep, t) = tk.next_step_size;
+        s.dv(kPdPath, t) = tk.path_length;
+        s.dv(kPdAbsPath, t) = tk.abs_path_length;
+        s.dv(kPdConstraint, t) = tk.constraint_abort;
+        s.dv(kPdDtds + 0, t) = static_cast<double>(tk.dtds3.x);
+        s.dv(kPdDtds + 1, t) = static_cast<double>(tk.dtds3.y);
+        s.dv(kPdDtds + 2, t) = static_cast<double>(tk.dtds3.z);
+        s.dv(kPdDqopds, t) = tk.dqopds3;
+        s.dv(kPdHStage, t) = tk.h_stage;
+        s.dv(kPdExtTol, t) = tk.ext_tol;
+        s.dv(kPdNavDir, t) = tk.nav_dir;
+        for (int i = 0; i < 6; ++i) s.dv(kPdBound + i, t) = tk.bound[i];
+        for (int i = 0; i < 25; ++i) s.dv(kPdJ + i, t) = static_cast<double>(tk.J[i]);
+        for (int i = 0; i < 21; ++i) s.dv(kPdCov + i, t) = static_cast<double>(tk.cov[i]);
+        if constexpr (P::kGrad)
+            for (int i = 0; i < 18; ++i) s.dv(kPdJx + i, t) = static_cast<double>(tk.Jx.v[i]);
+        for (int i = 0; i < kCache; ++i) {
+            s.dv(kPdCache + i, t) = c.p(i);
+            s.wv(kPwCacheSf + i, t) = c.s(i);
+            s.wv(kPwCacheLs + i, t) = c.l(i);
+        }
+        s.wv(kPwTrials, t) = tk.n_trials;
+        s.wv(kPwSteps, t) = tk.n_steps;
+        s.wv(kPwUpSf, t) = tk.up_sf;
+        s.wv(kPwSeq, t) = tk.n_seq;
+        s.wv(kPwTrace, t) = tk.n_trace;
+        s.wv(kPwNav, t) = static_cast<uint32_t>(static_cast<uint8_t>(tk.next)) |
+                          (static_cast<uint32_t>(static_cast<uint8_t>(tk.last)) << 8) |
+                          (static_cast<uint32_t>(static_cast<uint8_t>(tk.status)) << 16) |
+                          (static_cast<uint32_t>(tk.trust) << 24);
+        s.wv(kPwVolFlags, t) = static_cast<uint32_t>(tk.volume) | (tk.path_limit_hit ? 1u << 16 : 0u) |
+                               (tk.overflow ? 1u << 17 : 0u) | (tk.stopped ? 1u << 18 : 0u) |
+                               (tk.heartbeat ? 1u << 19 : 0u) | (tk.momentum_hit ? 1u << 20 : 0u) |
+                               (tk.target_hit ? 1u << 21 : 0u) | (is_init ? 1u << 22 : 0u) |
+                               (use_init_cov ? 1u << 23 : 0u);
+        s.wv(kPwStall, t) = stall;
+    }
+    DTR_AI void load(const PauseView& s, CacheView c, uint32_t track) {
+        gid = track;
+        const uint32_t t = track;
+        if constexpr (P::kLdsState) {
+            tk.J.p = c.state;
+            tk.cov.p = c.state + 25;
+        }
+        tk.pos = {s.dv(kPdPos + 0, t), s.dv(kPdPos + 1, t), s.dv(kPdPos + 2, t)};
+        tk.dir = {s.dv(kPdDir + 0, t), s.dv(kPdDir + 1, t), s.dv(kPdDir + 2, t)};
+        tk.time = s.dv(kPdTime, t);
+        tk.qop = s.dv(kPdQop, t);
+        tk.step_size = s.dv(kPdStep, t);
+        tk.next_step_size = s.dv(kPdNextStep, t);
+        tk.path_length = s.dv(kPdPath, t);
+        tk.abs_path_length = s.dv(kPdAbsPath, t);
+        tk.constraint_abort = s.dv(kPdConstraint, t);
+        tk.dtds3 = {static_cast<S>(s.dv(kPdDtds + 0, t)), static_cast<S>(s.dv(kPdDtds + 1, t)),
+                    static_cast<S>(s.dv(kPdDtds + 2, t))};
+        tk.dqopds3 = s.dv(kPdDqopds, t);
+        tk.h_stage = s.dv(kPdHStage, t);
+        tk.ext_tol = s.dv(kPdExtTol, t);
+        tk.nav_dir = s.dv(kPdNavDir, t);
+        for (int i = 0; i < 6; ++i) tk.bound[i] = s.dv(kPdBound + i, t);
+        for (int i = 0; i < 25; ++i) tk.J[i] = static_cast<S>(s.dv(kPdJ + i, t));
+        for (int i = 0; i < 21; ++i) tk.cov[i] = static_cast<S>(s.dv(kPdCov + i, t));
+        if constexpr (P::kGrad)
+            for (int i = 0; i < 18; ++i) tk.Jx.v[i] = static_cast<S>(s.dv(kPdJx + i, t));
+        for (int i = 0; i < kCache; ++i) {
+            c.p(i) = s.dv(kPdCache + i, t);
+            c.s(i) = s.wv(kPwCacheSf + i, t);
+            c.l(i) = s.wv(kPwCacheLs + i, t);
+        }
+        tk.n_trials = s.wv(kPwTrials, t);
+        tk.n_steps = s.wv(kPwSteps, t);
+        tk.up_sf = s.wv(kPwUpSf, t);
+        tk.n_seq = s.wv(kPwSeq, t);
+        tk.n_trace = s.wv(kPwTrace, t);
+        const uint32_t nav = s.wv(kPwNav, t);
+        tk.next = static_cast<int8_t>(nav & 0xffu);
+        tk.last = static_cast<int8_t>((nav >> 8) & 0xffu);
+        tk.status = static_cast<int8_t>((nav >> 16) & 0xffu);
+        tk.trust = static_cast<uint8_t>(nav >> 24);
+        const uint32_t vf = s.wv(kPwVolFlags, t);
+        tk.volume = static_cast<uint16_t>(vf & 0xffffu);
+        tk.path_limit_hit = (vf >> 16) & 1u;
+        tk.overflow = (vf >> 17) & 1u;
+        tk.stopped = (vf >> 18) & 1u;
+        tk.heartbeat = (vf >> 19) & 1u;
+        tk.momentum_hit = (vf >> 20) & 1u;
+        tk.target_hit = (vf >> 21) & 1u;
+        is_init = (vf >> 22) & 1u;
+        use_init_cov = (vf >> 23) & 1u;
+        stall = s.wv(kPwStall, t);
+        last_path = tk.path_length;
+        nav_ret = false;
+        nav_stage = 0;
+    }
+    // pause here: the state to the buffer, status DTR_TRACK_PAUSED and the
+    // counts so far to the results (the surface sequence is already there)
+    DTR_AI void pause(const PauseView& s, CacheView c, const DevResults& out) const {
+        save(s, c);
+        s.wv(kPwState, gid) = kLanePaused;
+        if (out.status) out.status[gid] = DTR_TRACK_PAUSED;
+        if (out.n_surfaces) out.n_surfaces[gid] = tk.n_seq;
+        if (out.n_steps) out.n_steps[gid] = tk.n_steps;
+        if (out.n_trials) out.n_trials[gid] = tk.n_trials;
+    }
+
     // one lane on its own, with out-of-line local navigations
     DTR_AI void navigate_alone(const FastDetector& det, const DevConfig& cfg, CacheView c) {
         float os;
@@ -2329,6 +2478,32 @@ DTR_AI void propagate_fast(const FastDetector& det, const DevField& field, const
     while (!lane.advance(det, field, cfg, in, out, c)) {
     }
     lane.finish(out, in.n, cfg);
+}
+
+// propagate_fast with a step budget per call: the track starts from the
+// inputs (or, resuming, from its paused state), pauses once it has taken
+// max_steps steps in this call, and finishes as propagate_fast would.
+template <class P>
+DTR_AI void propagate_resumable(const FastDetector& det, const DevField& field, const DevConfig& cfg,
+                                const TrackInputs& in, const DevResults& out, uint32_t track, CacheView c,
+                                const PauseView& pv) {
+    Lane<P> lane;
+    if (pv.resume) {
+        if (pv.wv(kPwState, track) != kLanePaused) return;
+        lane.load(pv, c, track);
+    } else {
+        lane.start(det, field, cfg, in, c, track);
+    }
+    const uint32_t s0 = lane.tk.n_steps;
+    for (;;) {
+        if (lane.tk.n_steps - s0 >= pv.max_steps) {
+            lane.pause(pv, c, out);
+            return;
+        }
+        if (lane.advance(det, field, cfg, in, out, c)) break;
+    }
+    lane.finish(out, in.n, cfg);
+    pv.wv(kPwState, track) = kLaneDone;
 }
 
 }  // namespace dtr

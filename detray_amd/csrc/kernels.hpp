@@ -113,20 +113,41 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t G) {
 // run with the whole wave active.  Each lane runs exactly the operations of
 // propagate() in its own order; only the interleaving across lanes changes,
 // so the results are those of the per-lane kernel.
-template <class P>
+//
+// kPause: a step budget per call (propagate_resumable): a lane pauses where
+// its actor half would begin once it has taken pv.max_steps steps in this
+// call, and a resuming call starts the paused lanes there.
+template <class P, bool kPause = false>
 __device__ __attribute__((always_inline)) inline void propagate_gathered(
     const FastDetector& det, const DevField& field, const DevConfig& cfg, const TrackInputs& in,
-    const DevResults& out, bool has_track, uint32_t track, CacheView c) {
+    const DevResults& out, bool has_track, uint32_t track, CacheView c, const PauseView& pv = PauseView{}) {
     enum : uint8_t { kStart, kActor, kNav1, kNav2, kDone };
     Lane<P> lane;
     float os = 0.f;
     uint8_t phase = kDone;
     bool waiting = false;
+    uint32_t s0 = 0u;
+    if constexpr (kPause) {
+        if (has_track && pv.resume) {
+            if (pv.wv(kPwState, track) == kLanePaused) {
+                lane.load(pv, c, track);
+                s0 = lane.tk.n_steps;
+                phase = kActor;
+            }
+            has_track = false;
+        }
+    }
     if (has_track) {
         lane.start_begin(det, field, cfg, in, c, track, os);
         phase = kStart;
         waiting = true;  // the navigator's initialisation is a volume search
     }
+    // a finished track: its results, and its state as done
+    auto done = [&]() {
+        lane.finish(out, in.n, cfg);
+        if constexpr (kPause) pv.wv(kPwState, lane.gid) = kLaneDone;
+        phase = kDone;
+    };
     for (;;) {
         const uint64_t running = __ballot(phase != kDone);
         if (running == 0ull) break;
@@ -153,8 +174,7 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
                 continue;
             }
             if (end) {
-                lane.finish(out, in.n, cfg);
-                phase = kDone;
+                done();
                 continue;
             }
             lane.step_half(det, field, cfg, c);
@@ -167,19 +187,24 @@ __device__ __attribute__((always_inline)) inline void propagate_gathered(
             continue;
         }
         if (phase == kActor && !waiting) {
+            if constexpr (kPause) {
+                if (lane.tk.n_steps - s0 >= pv.max_steps) {
+                    lane.pause(pv, c, out);
+                    phase = kDone;
+                    continue;
+                }
+            }
             // actor half, navigation update, step half, navigation update
             // (Lane::advance), up to the first request for a search
             if (lane.actor_half(det, field, cfg, in, out, c)) {
-                lane.finish(out, in.n, cfg);
-                phase = kDone;
+                done();
                 continue;
             }
             phase = kNav1;
             waiting = lane.nav_begin(det, cfg, c, os);
             if (waiting) continue;
             if (lane.nav_end()) {
-                lane.finish(out, in.n, cfg);
-                phase = kDone;
+                done();
                 continue;
             }
             lane.step_half(det, field, cfg, c);
@@ -213,6 +238,25 @@ DTR_BATCHED_KERNEL propagate_gather_kernel(FastDetector det, DevField field, Dev
     const bool has = k < in.n;
     const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
     propagate_gathered<P>(det, field, cfg, in, out, has, track, lds_cache<P::kLdsState>(smem));
+}
+
+// The per-lane (G = 0) and gathered (G = 1) kernels with a step budget per
+// call (dtr_propagate_resumable).
+template <class P, int W, int G>
+DTR_BATCHED_KERNEL propagate_resumable_kernel(FastDetector det, DevField field, DevConfig cfg, TrackInputs in,
+                                              DevResults out, const uint32_t* __restrict__ order, uint32_t xcd,
+                                              PauseView pv) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t k = static_cast<uint64_t>(xcd_block(blockIdx.x, gridDim.x, xcd)) * blockDim.x + threadIdx.x;
+    if constexpr (G == 0) {
+        if (k >= in.n) return;
+        const uint32_t track = order ? order[k] : static_cast<uint32_t>(k);
+        propagate_resumable<P>(det, field, cfg, in, out, track, lds_cache<P::kLdsState>(smem), pv);
+    } else {
+        const bool has = k < in.n;
+        const uint32_t track = has ? (order ? order[k] : static_cast<uint32_t>(k)) : 0u;
+        propagate_gathered<P, true>(det, field, cfg, in, out, has, track, lds_cache<P::kLdsState>(smem), pv);
+    }
 }
 
 // One track per lane, navigation searches pooled across the wave.  Every
@@ -344,6 +388,20 @@ int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig
         hipLaunchKernelGGL((propagate_grid_kernel<LanePolicy<P>, 2>), dim3(grid_of(in->n, block)), dim3(block),
                            lane_lds_bytes<LanePolicy<P>>(block), stream, det->fast, field->dev, dc, track_inputs(in),
                            r, order, 0u);
+    return to_status(hipGetLastError());
+}
+
+template <class P>
+int launch_resumable(const dtr_detector* det, const dtr_field* field, const DevConfig& dc,
+                     const dtr_track_batch* in, const DevResults& r, uint32_t block, const uint32_t* order,
+                     int gather, uint32_t xcd, const PauseView& pv, hipStream_t stream) {
+    const size_t lds = lane_lds_bytes<LanePolicy<P>>(block);
+    if (gather)
+        hipLaunchKernelGGL((propagate_resumable_kernel<LanePolicy<P>, 2, 1>), dim3(grid_of(in->n, block)),
+                           dim3(block), lds, stream, det->fast, field->dev, dc, track_inputs(in), r, order, xcd, pv);
+    else
+        hipLaunchKernelGGL((propagate_resumable_kernel<LanePolicy<P>, 2, 0>), dim3(grid_of(in->n, block)),
+                           dim3(block), lds, stream, det->fast, field->dev, dc, track_inputs(in), r, order, xcd, pv);
     return to_status(hipGetLastError());
 }
 

@@ -70,6 +70,14 @@ template <class P>
 int launch_lane(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const dtr_track_batch* in,
                 const DevResults& r, uint32_t block, const uint32_t* order, int gather, hipStream_t stream);
 
+// The per-lane (gather 0) or gathered (1) kernels with a step budget per
+// call and the lanes' state in pv (dtr_propagate_resumable), two waves per
+// SIMD.
+template <class P>
+int launch_resumable(const dtr_detector* det, const dtr_field* field, const DevConfig& dc,
+                     const dtr_track_batch* in, const DevResults& r, uint32_t block, const uint32_t* order,
+                     int gather, uint32_t xcd, const PauseView& pv, hipStream_t stream);
+
 // The direct navigator (dtr_propagate_direct).
 template <class P>
 int launch_direct(const dtr_detector* det, const dtr_field* field, const DevConfig& dc, const DevConfig& dd,

@@ -519,6 +519,20 @@ class Actors:
         return a
 
 
+class PropagationState:
+    """The paused state of every track of a batch of ``n`` between calls of
+    :meth:`Propagator.propagate_resumable` (``dtr_propagation_state``: a
+    device buffer of about 0.9 KB per track, indexed by track)."""
+
+    def __init__(self, n: int, device="cuda"):
+        import torch
+
+        self.n = int(n)
+        nbytes = int(capi.load().dtr_propagation_state_size(self.n))
+        self.data = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=device)
+        self.started = False
+
+
 class Propagator:
     """``detray::propagator`` over a batch of tracks on one GPU."""
 
@@ -608,6 +622,44 @@ class Propagator:
         capi.check(lib.dtr_propagate_ex(detector.device(), field.device(), C.byref(cfg),
                                         C.byref(tb), C.byref(rb), C.byref(lc), C.c_void_p(stream)),
                    "dtr_propagate_ex")
+        return out
+
+    def propagate_resumable(self, detector: Detector, field: MagneticField, state: PropagationState,
+                            max_steps: int, tracks=None, initial_cov=None, out: Optional[PropagationResult] = None,
+                            stream=None, bound_params=None, start_surface=None,
+                            actors: Optional[Actors] = None) -> PropagationResult:
+        """One call of a propagation that pauses and resumes
+        (``dtr_propagate_resumable``, propagator.hpp:238-250): the first call
+        on ``state`` starts every track, each later call continues the tracks
+        the previous one paused; a track pauses (status ``TRACK_PAUSED``)
+        once it has taken ``max_steps`` steps in a call.  Pass the same
+        inputs and the same ``out`` to every call; the results of a
+        propagation done in several calls are those of one :meth:`propagate`
+        call.  Needs a launch with the grid schedule and "lane" or "gather"
+        navigation."""
+        import torch
+
+        if self.launch is None or self.launch.schedule != "grid" or self.launch.navigation == "wave":
+            raise ValueError("propagate_resumable needs a grid launch with lane or gather navigation")
+        tb, rb, out, n, device, _keep = self._batches(detector, tracks, initial_cov, out, bound_params,
+                                                      start_surface)
+        if state.n != n or state.data.device != device:
+            raise ValueError(f"state holds {state.n} tracks on {state.data.device}, the batch {n} on {device}")
+        if int(max_steps) < 1:
+            raise ValueError("max_steps must be at least 1")
+        cfg, _states = self._config_with(actors, n, device, out)
+        order = self.launch.order
+        if order is not None and (not order.is_cuda or order.dtype != torch.int32 or order.shape != (n,)
+                                  or not order.is_contiguous() or order.device != device):
+            raise ValueError("launch.order must be a contiguous int32 CUDA tensor [n] on the tracks' device")
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        lc = self.launch.to_c(None)
+        st = capi.PropagationState(state.data.data_ptr(), int(max_steps), 1 if state.started else 0)
+        capi.check(capi.load().dtr_propagate_resumable(detector.device(), field.device(), C.byref(cfg),
+                                                       C.byref(tb), C.byref(rb), C.byref(lc), C.byref(st),
+                                                       C.c_void_p(stream)), "dtr_propagate_resumable")
+        state.started = True
         return out
 
     def propagate_direct(self, detector: Detector, field: MagneticField, sequence, n_sequence, tracks=None,

@@ -58,6 +58,8 @@ enum dtr_track_status {
     DTR_TRACK_OVERFLOW = 3,    /* surface_sequencer capacity reached */
     DTR_TRACK_MOMENTUM_LIMIT = 4, /* momentum_aborter fired (p or pT at or below its limit) */
     DTR_TRACK_TARGET = 5,      /* target_aborter fired (the target surface was reached) */
+    DTR_TRACK_PAUSED = 6,      /* paused (dtr_propagate_resumable): propagate() returned with the
+                                  navigation alive, is_paused() (propagator.hpp:238-243) */
     DTR_TRACK_NOT_RUN = 255
 };
 
@@ -520,6 +522,39 @@ DTR_API int dtr_propagate_direct(const dtr_detector* det, const dtr_field* field
                                  const dtr_prop_config* cfg, const dtr_track_batch* in,
                                  const dtr_surface_sequence* seq, dtr_result_batch* out,
                                  const dtr_launch_config* launch, void* stream);
+
+/* Pause and resume (propagator.hpp:238-250, 274-282: is_paused, resume, and
+ * propagate() continuing a paused state without initialising the
+ * navigation).  The reference pauses a track when an actor stops its
+ * heartbeat while the navigation is alive; here a step budget per call
+ * gives the pause: a track that has taken max_steps Runge-Kutta steps in the
+ * call stops where the actor half of propagate()'s loop would begin, with
+ * status DTR_TRACK_PAUSED and its step, trial and surface counts so far;
+ * the surface sequence is written as it grows.  The next call with resume
+ * = 1 continues every paused track from that point (tracks that finished
+ * are left alone, their results stay as written).  A propagation done in
+ * several calls gives the results of one dtr_propagate_ex call bit for bit.
+ * Between calls the caller may copy, reorder or drop nothing in the state:
+ * it is indexed by track.  data: DEVICE buffer of
+ * dtr_propagation_state_size(n) bytes (about 0.9 KB per track). */
+typedef struct dtr_propagation_state {
+    void* data;
+    uint32_t max_steps; /* steps a track takes in one call before it pauses, >= 1 */
+    uint32_t resume;    /* 0: start every track from the batch's inputs; 1: continue the paused ones */
+} dtr_propagation_state;
+
+DTR_API size_t dtr_propagation_state_size(uint32_t n);
+
+/* dtr_propagate_ex with pause and resume: grid schedule, per-lane or
+ * gathered navigation (DTR_ERROR_UNSUPPORTED otherwise, and with
+ * use_field_gradient), every call with the same detector, field, config,
+ * track batch (its inputs are read by the first call; the initial
+ * covariance of a free start also when a later call reaches the first
+ * surface) and result batch.  No allocation, no synchronisation. */
+DTR_API int dtr_propagate_resumable(const dtr_detector* det, const dtr_field* field,
+                                    const dtr_prop_config* cfg, const dtr_track_batch* in,
+                                    dtr_result_batch* out, const dtr_launch_config* launch,
+                                    const dtr_propagation_state* state, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiation: d(chi2 of hit residuals)/d(initial bound parameters)    */

@@ -361,6 +361,31 @@ class propagator {
         }
     }
 
+    /* caching navigator with pause and resume (propagator.hpp:238-250,
+     * 274-282): every track takes at most pause.max_steps steps per call;
+     * the first call (pause.resume == 0) starts the tracks, later ones
+     * (pause.resume == 1) continue those left paused (status
+     * DTR_TRACK_PAUSED, is_paused()).  pause.data: caller-owned device
+     * buffer of dtr_propagation_state_size(n) bytes.  Needs a launch with
+     * the grid schedule and lane or gathered navigation. */
+    template <class... states_t>
+    void propagate(const device_detector& det, const device_field& field, const dtr_track_batch& tracks,
+                   dtr_result_batch& results, std::tuple<states_t&...> states, const dtr_propagation_state& pause,
+                   void* stream = nullptr) const
+        requires std::is_same_v<navigator_t, caching_navigator>
+    {
+        static_assert(std::is_same_v<std::tuple<states_t&...>, typename actor_chain_t::state_tuple>,
+                      "one state per actor of the chain, in the chain's order");
+        if (!launch_) throw error(DTR_ERROR_INVALID_ARGUMENT, "pause and resume need a launch configuration");
+        dtr_actor_states as;
+        const dtr_prop_config c = chain_config(as, states);
+        check(dtr_propagate_resumable(det.get(), field.get(), &c, &tracks, &results, &*launch_, &pause, stream),
+              "dtr_propagate_resumable");
+    }
+
+    /* propagator::is_paused (propagator.hpp:238-243) of a track's status */
+    static constexpr bool is_paused(uint8_t status) noexcept { return status == DTR_TRACK_PAUSED; }
+
     /* direct navigator: replay a surface sequence per track */
     template <class... states_t>
     void propagate(const device_detector& det, const device_field& field, const dtr_track_batch& tracks,

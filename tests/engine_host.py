@@ -152,3 +152,46 @@ def adjoint(detector, field, config, adjoint_config, bound_params, start_surface
                "dtr_test_adjoint")
     out.pop("_keep", None)
     return out
+
+
+def propagate_resumable(detector, field, config, free_params, max_steps: int, mixed: bool = False,
+                        max_calls: int = 100000) -> tuple:
+    """The engine's resumable lane loop (propagate_resumable) over host
+    arrays, called with a budget of ``max_steps`` steps until no track is
+    paused; returns (results as propagate(), number of calls, number of
+    tracks paused by the first call)."""
+    lib = load()
+    fp = np.ascontiguousarray(free_params, dtype=np.float64)
+    n = fp.shape[1]
+    ms = max(int(config.max_surfaces), 1)
+    out = {
+        "bound_params": np.zeros((6, n)),
+        "bound_cov": np.zeros((21, n)),
+        "free_params": np.zeros((8, n)),
+        "path_length": np.zeros(n),
+        "surfaces": np.full((ms, n), 0xFFFFFFFF, dtype=np.uint32),
+        "n_surfaces": np.zeros(n, dtype=np.uint32),
+        "n_steps": np.zeros(n, dtype=np.uint32),
+        "n_trials": np.zeros(n, dtype=np.uint32),
+        "status": np.zeros(n, dtype=np.uint8),
+    }
+    state = np.zeros(int(capi.load().dtr_propagation_state_size(n)) // 8 + 1, dtype=np.float64)
+    tb = capi.TrackBatch(n, 0, fp.ctypes.data, None, None, None)
+    rb = capi.ResultBatch(*[out[k].ctypes.data for k in (
+        "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
+        "n_steps", "n_trials", "status")])
+    cfg = config.to_c()
+    lib.dtr_test_engine_propagate_resumable.restype = C.c_int
+    calls, first_paused = 0, 0
+    while calls < max_calls:
+        capi.check(lib.dtr_test_engine_propagate_resumable(
+            C.byref(detector.desc), C.byref(field.desc), C.byref(cfg), C.byref(tb), C.byref(rb),
+            C.c_void_p(state.ctypes.data), C.c_uint32(max_steps), C.c_uint32(1 if calls else 0),
+            C.c_int(1 if mixed else 0)), "dtr_test_engine_propagate_resumable")
+        paused = int((out["status"] == capi.TRACK_PAUSED).sum())
+        if calls == 0:
+            first_paused = paused
+        calls += 1
+        if paused == 0:
+            break
+    return out, calls, first_paused

@@ -544,6 +544,39 @@ int run_gpu() {
         expect(e < 1e-4, "propagator (batched mixed kernel): bound parameters within 1e-4 (" + std::to_string(e) + ")");
     }
     {
+        // pause and resume: budgets of 4 steps until no track is paused
+        // equal one call of the same launch bit for bit
+        dtr_launch_config l = default_launch();
+        l.precision = DTR_PRECISION_MIXED;
+        l.schedule = DTR_SCHEDULE_GRID;
+        l.navigation = DTR_NAVIGATION_GATHER;
+        l.block_size = 64u;
+        propagator<rk_stepper, caching_navigator, chain> p(cfg, l);
+        DeviceResults one(n, ms), part(n, ms);
+        dtr_result_batch ob = one.batch(), pb = part.batch();
+        p.propagate(det, field, in, ob, std::tie(pl, mi, sq));
+        DeviceBuffer state(dtr_propagation_state_size(n));
+        dtr_propagation_state ps{state.p, 4u, 0u};
+        uint32_t calls = 0, first_paused = 0;
+        for (; calls < 1000u; ++calls) {
+            p.propagate(det, field, in, pb, std::tie(pl, mi, sq), ps);
+            ps.resume = 1u;
+            (void)hipDeviceSynchronize();
+            std::vector<uint8_t> st(n);
+            (void)hipMemcpy(st.data(), part.status.p, n, hipMemcpyDeviceToHost);
+            uint32_t paused = 0;
+            for (uint8_t v : st) paused += decltype(p)::is_paused(v) ? 1u : 0u;
+            if (calls == 0u) first_paused = paused;
+            if (paused == 0u) break;
+        }
+        (void)hipDeviceSynchronize();
+        const Results a = one.fetch(), b = part.fetch();
+        expect(first_paused > n / 2u, "pause: most tracks take more than 4 steps (" + std::to_string(first_paused) + ")");
+        expect(calls >= 3u && a.bound == b.bound && a.cov == b.cov && a.free_p == b.free_p && a.path == b.path &&
+                   a.steps == b.steps && a.trials == b.trials && same_sequences(a, b, n),
+               "pause and resume: " + std::to_string(calls + 1u) + " calls equal one call bit for bit");
+    }
+    {
         // momentum aborter at 3 GeV: the oracle with the same actor states
         using mchain = actor_chain<actor::pathlimit_aborter, actor::parameter_updater<actor::pointwise_material_interactor>,
                                    actor::surface_sequencer, actor::momentum_aborter>;

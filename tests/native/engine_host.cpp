@@ -12,6 +12,8 @@
 //    (propagate_direct) the same way, forward or backward.
 //  * dtr_test_adjoint: the adjoint's recording pass and backward pass
 //    (adjoint_record, adjoint_backward) track by track on the host.
+//  * dtr_test_engine_propagate_resumable: the lane loop with a step budget
+//    per call (propagate_resumable), pausing and resuming.
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -157,6 +159,16 @@ void adjoint_all(const HostDetector& hd, const DevField& field, const DevConfig&
                                 grad);
 }
 
+template <class P>
+void propagate_resumable_all(const HostDetector& hd, const DevField& field, const DevConfig& cfg,
+                             const TrackInputs& in, const DevResults& out, const PauseView& pv) {
+    double path[kCache];
+    uint32_t sfs[kCache], ls[kCache];
+    const CacheView c{path, sfs, ls, 1};
+    for (uint64_t t = 0; t < in.n; ++t)
+        propagate_resumable<P>(hd.fast, field, cfg, in, out, static_cast<uint32_t>(t), c, pv);
+}
+
 DevField host_field(const dtr_field_desc* f) {
     DevField field{};
     field.kind = f->kind;
@@ -297,6 +309,40 @@ __attribute__((visibility("default"))) int dtr_test_adjoint(const dtr_detector_d
     } else {
         if (ext) adjoint_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, io, exact, grad);
         else adjoint_all<PolicyMixed>(hd, field, dc, io, exact, grad);
+    }
+    return DTR_SUCCESS;
+}
+
+// One call of the resumable propagation over host arrays: state is the
+// buffer of dtr_propagation_state_size(n) bytes (doubles, then words).
+__attribute__((visibility("default"))) int dtr_test_engine_propagate_resumable(
+    const dtr_detector_desc* d, const dtr_field_desc* f, const dtr_prop_config* cfg, const dtr_track_batch* in,
+    dtr_result_batch* out, void* state, uint32_t max_steps, uint32_t resume, int precision) {
+    if (!d || !f || !cfg || !in || !out || !state || max_steps == 0u) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cfg->use_field_gradient) return DTR_ERROR_UNSUPPORTED;
+    HostDetector hd(*d);
+    const DevField field = host_field(f);
+    const DevConfig dc = make_dev_config(cfg);
+    const TrackInputs ti{in->bound_params ? nullptr : in->free_params, in->initial_cov, in->bound_params,
+                         in->start_surface, in->n};
+    const DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                       cfg->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                       out->n_trials, out->status};
+    PauseView pv;
+    pv.n = in->n;
+    pv.d = static_cast<double*>(state);
+    pv.w = reinterpret_cast<uint32_t*>(pv.d + static_cast<uint64_t>(kPauseD) * in->n);
+    pv.max_steps = max_steps;
+    pv.resume = resume;
+    const bool ext = has_ext(*d), vol = has_volume_material(*d);
+    if (precision == 0) {
+        if (vol) propagate_resumable_all<WithVolumeMaterial<WithExtendedGeometry<PolicyFP64>>>(hd, field, dc, ti, r, pv);
+        else if (ext) propagate_resumable_all<WithExtendedGeometry<PolicyFP64>>(hd, field, dc, ti, r, pv);
+        else propagate_resumable_all<PolicyFP64>(hd, field, dc, ti, r, pv);
+    } else {
+        if (vol) propagate_resumable_all<WithVolumeMaterial<WithExtendedGeometry<PolicyMixed>>>(hd, field, dc, ti, r, pv);
+        else if (ext) propagate_resumable_all<WithExtendedGeometry<PolicyMixed>>(hd, field, dc, ti, r, pv);
+        else propagate_resumable_all<PolicyMixed>(hd, field, dc, ti, r, pv);
     }
     return DTR_SUCCESS;
 }

@@ -97,6 +97,32 @@ def test_propagate_ex_rejects_bad_launch_before_touching_the_device():
         assert lib.dtr_propagate_ex(fake, fake, C.byref(cfg), C.byref(tb), C.byref(rb), C.byref(lc), None) == rc, w
 
 
+def test_propagate_resumable_checks_before_touching_the_device():
+    """State size (98 doubles and 25 words per track) and the argument
+    checks of dtr_propagate_resumable, all before any device work."""
+    lib = capi.load()
+    assert lib.dtr_propagation_state_size(1000) == 1000 * (98 * 8 + 25 * 4)
+    cfg = capi.PropConfig()
+    lib.dtr_default_config(C.byref(cfg))
+    tb = capi.TrackBatch(0, 0, None, None, None, None)
+    rb = capi.ResultBatch()
+    lc = capi.LaunchConfig()
+    lib.dtr_default_launch(C.byref(lc))
+    lc.schedule, lc.navigation = capi.SCHEDULE_GRID, capi.NAVIGATION_GATHER
+    fake = C.c_void_p(8)  # never dereferenced: an empty batch returns after the checks
+    st = capi.PropagationState(None, 4, 0)
+    call = lambda s: lib.dtr_propagate_resumable(fake, fake, C.byref(cfg), C.byref(tb), C.byref(rb), C.byref(lc),
+                                                  s, None)
+    assert call(C.byref(st)) == 0
+    assert call(None) == 1
+    assert call(C.byref(capi.PropagationState(None, 0, 0))) == 1  # a budget of no steps
+    assert call(C.byref(capi.PropagationState(None, 4, 2))) == 1
+    lc.navigation = capi.NAVIGATION_WAVE
+    assert call(C.byref(st)) == 5  # DTR_ERROR_UNSUPPORTED: pooled navigation
+    lc.navigation, lc.schedule = capi.NAVIGATION_LANE, capi.SCHEDULE_PERSISTENT
+    assert call(C.byref(st)) == 5
+
+
 def _copy_tables(det):
     """Python-owned copies of a detector's tables (for corruption tests)."""
     return {name: [type(x).from_buffer_copy(x) if hasattr(x, "_fields_") else int(x) for x in det.table(name)]
