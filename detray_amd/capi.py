@@ -96,6 +96,19 @@ class Volume(C.Structure):
     ]
 
 
+MATERIAL_MAP = 0x80000000
+
+
+class MaterialMap(C.Structure):
+    _fields_ = [
+        ("n0", C.c_uint32), ("n1", C.c_uint32),
+        ("slabs", C.c_uint32),
+        ("circular0", C.c_uint8), ("circular1", C.c_uint8), ("pad", C.c_uint8 * 2),
+        ("min0", C.c_double), ("max0", C.c_double),
+        ("min1", C.c_double), ("max1", C.c_double),
+    ]
+
+
 class DetectorDesc(C.Structure):
     _fields_ = [
         ("n_volumes", C.c_uint32), ("n_surfaces", C.c_uint32),
@@ -110,6 +123,8 @@ class DetectorDesc(C.Structure):
         ("grids", C.POINTER(Grid)),
         ("bf_surfaces", C.POINTER(C.c_uint32)),
         ("grid_bins", C.POINTER(C.c_uint32)),
+        ("n_material_maps", C.c_uint32), ("pad", C.c_uint32),
+        ("material_maps", C.POINTER(MaterialMap)),
     ]
 
 
@@ -370,12 +385,14 @@ _SIGNATURES = {
     "dtr_build_telescope_detector": (C.c_int, [C.POINTER(TelescopeConfig), C.POINTER(DetectorDesc)]),
     "dtr_predefined_material": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(MaterialSlab)]),
     "dtr_host_detector_free": (None, [C.POINTER(DetectorDesc)]),
-    "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(DetectorDesc),
-                                         C.c_char_p, C.c_size_t]),
+    "dtr_check_detector": (C.c_int, [C.POINTER(DetectorDesc), C.c_char_p, C.c_size_t]),
+    "dtr_read_detector_json": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                         C.POINTER(DetectorDesc), C.c_char_p, C.c_size_t]),
     # the view records are filled by the C++ adapter (include/detray_amd.hpp)
     "dtr_flatten_detector": (C.c_int, [C.c_void_p, C.POINTER(DetectorDesc), C.c_char_p, C.c_size_t]),
     "dtr_write_detector_json": (C.c_int, [C.POINTER(DetectorDesc), C.c_char_p, C.POINTER(C.c_char_p),
-                                          C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+                                          C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                          C.c_size_t]),
     "dtr_sample_solenoid": (C.c_int, [C.c_double, C.c_double, C.c_double,
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_float)]),

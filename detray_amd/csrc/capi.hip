@@ -363,6 +363,7 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     if (!rc) rc = upload(d->grids, d->n_grids, &h->blocks[5]);
     if (!rc) rc = upload(d->bf_surfaces, d->n_bf, &h->blocks[6]);
     if (!rc) rc = upload(d->grid_bins, d->n_bins, &h->blocks[7]);
+    if (!rc) rc = upload(d->material_maps, d->n_material_maps, &h->blocks[12]);
     if (!rc) {
         for (uint32_t k = 0; k < d->n_surfaces; ++k) {
             const dtr_surface& sf = d->surfaces[k];
@@ -415,9 +416,11 @@ int dtr_detector_create(const dtr_detector_desc* d, dtr_detector** out) {
     h->dev.grids = static_cast<const dtr_grid*>(h->blocks[5]);
     h->dev.bf = static_cast<const uint32_t*>(h->blocks[6]);
     h->dev.bins = static_cast<const uint32_t*>(h->blocks[7]);
+    h->dev.mat_maps = static_cast<const dtr_material_map*>(h->blocks[12]);
     h->fast.sf = static_cast<const dtr::NavSurface*>(h->blocks[8]);
     h->fast.masks = h->dev.masks;
     h->fast.materials = h->dev.materials;
+    h->fast.mat_maps = h->dev.mat_maps;
     h->fast.volumes = h->dev.volumes;
     h->fast.grids = h->dev.grids;
     h->fast.transforms = h->dev.transforms;

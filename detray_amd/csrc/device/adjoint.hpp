@@ -944,7 +944,9 @@ DTR_AI bool event_adjoint(const FastDetector& det, const DevField& field, const 
         }
     }
     if (mat && exact && cfg.do_eloss) {
-        const dtr_material_slab& m = det.materials[rec.material];
+        // the slab of the bin the bound position is in (a map's bins are
+        // piecewise constant in loc0, loc1)
+        const dtr_material_slab& m = surface_material(det.materials, det.mat_maps, rec.material, b[0], b[1]);
         if (m.thickness > 2.220446049250313e-16 && b[4] != 0.0) {
             const D4 q2 = eloss_qop(rec, m, b, mass, charge);
             const double g4 = bb[4];

@@ -95,6 +95,7 @@ struct FastDetector {
     const NavSurface* sf;
     const dtr_mask* masks;
     const dtr_material_slab* materials;
+    const dtr_material_map* mat_maps;
     const dtr_volume* volumes;
     const dtr_grid* grids;
     const dtr_transform* transforms;
@@ -1812,7 +1813,8 @@ struct FastEngine {
     // evaluated in S.
     DTR_AI void material_interaction(uint32_t dest) {
         const NavSurface& rec = det.sf[dest];
-        const dtr_material_slab& m = det.materials[rec.material];
+        const dtr_material_slab& m =
+            surface_material(det.materials, det.mat_maps, rec.material, tk.bound[0], tk.bound[1]);
         double cos_inc;
         if constexpr (P::kMixed) {
             // incidence from the free direction and position (the bound

@@ -39,6 +39,7 @@ struct DevDetector {
     const dtr_grid* grids;
     const uint32_t* bf;
     const uint32_t* bins;
+    const dtr_material_map* mat_maps;
 };
 
 struct DevField {
@@ -449,6 +450,29 @@ DTR_HD double eloss_d2qopdsdqop(const dtr_material_slab* vm, double qop, double 
     const double dq = eloss_dqopds(vm, qop, mass, q);
     return dq * (1.0 / qop * (3.0 - p2 / E2) + 1.0 / g * dgdqop);
 }
+// The slab a material surface applies at the bound local position (l0, l1)
+// (material_accessor.hpp:30-48): its own slab, or the bin of its material
+// map (material_map.hpp) — regular axes, bin int((x - min) / width + 1) - 1
+// (axis_binning.hpp:80-82), a circular axis wrapping, a closed one clamping
+// (axis_bounds.hpp:107-206)
+DTR_HD int material_map_bin(double x, double mn, double mx, uint32_t n, uint8_t circular) {
+    const int nb = static_cast<int>(n);
+    const int b = static_cast<int>((x - mn) / ((mx - mn) / static_cast<double>(n)) + 1.0) - 1;
+    if (circular) {
+        if (b >= 0 && b < nb) return b;
+        return (nb + (b % nb)) % nb;
+    }
+    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+DTR_HD const dtr_material_slab& surface_material(const dtr_material_slab* mats, const dtr_material_map* maps,
+                                                 uint32_t link, double l0, double l1) {
+    if (!(link & DTR_MATERIAL_MAP)) return mats[link];
+    const dtr_material_map& m = maps[link & ~DTR_MATERIAL_MAP];
+    const int i0 = material_map_bin(l0, m.min0, m.max0, m.n0, m.circular0);
+    const int i1 = material_map_bin(l1, m.min1, m.max1, m.n1, m.circular1);
+    return mats[m.slabs + static_cast<uint32_t>(i0) * m.n1 + static_cast<uint32_t>(i1)];
+}
+
 // The homogeneous material of volume v (0xffff: none), propagator.hpp:320-325
 DTR_HD const dtr_material_slab* volume_material(const dtr_volume* vols, const dtr_material_slab* mats,
                                                 uint16_t v) {
@@ -1412,7 +1436,8 @@ struct EngineT {
 
     // relativistic_quantities.hpp + interaction.hpp
     DTR_HD void material_interaction(uint32_t dest) {
-        const dtr_material_slab& m = det.materials[det.surfaces[dest].material];
+        const dtr_material_slab& m =
+            surface_material(det.materials, det.mat_maps, det.surfaces[dest].material, tk.bound[0], tk.bound[1]);
         V3 nrm;
         const dtr_surface& msf = det.surfaces[dest];
         if (msf.shape == DTR_CONCENTRIC_CYLINDER2D) {

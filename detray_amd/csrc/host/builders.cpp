@@ -145,6 +145,7 @@ struct SurfaceDef {
     std::vector<MaskDef> masks;  // >1 for multi-mask portals
     uint8_t sf_id;
     int material = -1;           // index into the material list of the emitter
+    int material_map = -1;       // index into the material maps of the emitter
     int mask_share = -1;         // share the masks of an earlier surface
 };
 
@@ -161,6 +162,7 @@ struct VolumeDef {
     std::vector<SurfaceDef> surfaces;
     GridDef grid;
     int material = -1;  // homogeneous volume material, index into the material table
+    bool endcap_layer = false;  // thin cylinder portal maps (build_toy_detector.hpp:919-922)
 };
 
 // Axis-aligned bounding box, utils/bounding_volume.hpp
@@ -308,8 +310,10 @@ struct Extent {
 // (volume_builder.hpp build/add_to_detector).
 // ---------------------------------------------------------------------------
 void assemble(const std::vector<VolumeDef>& vols,
-              const std::vector<dtr_material_slab>& mats, Tables& T) {
+              const std::vector<dtr_material_slab>& mats, Tables& T,
+              const std::vector<dtr_material_map>& maps = {}) {
     T.materials = mats;
+    T.maps = maps;
     for (uint32_t vi = 0; vi < vols.size(); ++vi) {
         const VolumeDef& V = vols[vi];
         dtr_volume dv{};
@@ -341,7 +345,9 @@ void assemble(const std::vector<VolumeDef>& vols,
                 }
             }
             first_mask_of[k] = ds.mask;
-            ds.material = (S.material >= 0) ? static_cast<uint32_t>(S.material) : kInvalid;
+            ds.material = (S.material_map >= 0) ? (DTR_MATERIAL_MAP | static_cast<uint32_t>(S.material_map))
+                          : (S.material >= 0)   ? static_cast<uint32_t>(S.material)
+                                                : kInvalid;
             ds.volume = static_cast<uint16_t>(vi);
             ds.sf_id = S.sf_id;
             ds.barcode = make_barcode(vi, S.sf_id, idx, ds.transform);
@@ -437,6 +443,7 @@ void assemble(const std::vector<VolumeDef>& vols,
 struct ToyCfg {
     unsigned n_brl = 4, n_edc = 3;
     bool material = true;
+    bool maps = false;  // toy_det_config::use_material_maps
     double outer_radius = F(180.0);
     double beampipe_vol_radius = F(25.0);
     double envelope = F(2.0);
@@ -599,6 +606,7 @@ std::vector<Extent> add_endcap(std::vector<VolumeDef>& vols, ToyCfg& c, int side
             }
             const double center_z = sign * c.edc_positions.at(j);
             vol.placement = make_translation({0.0, 0.0, center_z});
+            vol.endcap_layer = true;
             endcap_modules(vol, c, std::fabs(center_z), inner_radius,
                            outer_radius - c.envelope, side, si_mat);
             vb = add_cylinder_portal_box(vol, c.envelope, 0.0, inner_radius, outer_radius,
@@ -718,6 +726,84 @@ void add_connector(std::vector<VolumeDef>& vols, const ToyCfg& c, uint32_t beamp
     vol.surfaces.push_back(cyl_portal(c.outer_radius, min_z, max_z, kEndOfWorld));
 }
 
+// predefined_materials.hpp:93-97
+Material aluminium() {
+    return {F(88.97) * kMM, F(397.0) * kMM, F(26.98), F(13.0), 2.699 * kGram / kCm3};
+}
+
+// mixture.hpp:33-80: fraction-weighted A, Z and mass density, X0 and L0
+// from the fraction-weighted inverse lengths
+Material mixture(const Material& a, double fa, const Material& b, double fb) {
+    return {1.0 / (fa / a.X0 + fb / b.X0), 1.0 / (fa / a.L0 + fb / b.L0), a.A * fa + b.A * fb,
+            a.Z * fa + b.Z * fb, a.rho * fa + b.rho * fb};
+}
+
+// Material maps of the toy detector (toy_det_config::use_material_maps,
+// build_toy_detector.hpp:84-110, 371-409; material_map_generator.hpp:
+// 34-83, 177-253): every portal and the beampipe carry a map instead of the
+// module slabs.  Cylinder portals: 20 phi x 20 z bins of Si(90 %)/Al(10 %),
+// thickness 5 mm (0.15 mm on the portals of endcap layer volumes) + 1e-8 z^2
+// per z bin; disc portals: 5 r x 20 phi bins of the same mixture, 1 mm +
+// 1e-4 r per r bin; the beampipe: 20 x 20 bins of 0.8 mm beryllium.  The
+// bin positions z (r) run from the lowest to the highest edge over all masks
+// of the surface in n - 1 accumulated steps.  The map axes are the merged
+// mask extents: (phi in [-pi, pi] circular, z closed) on cylinders, (r
+// closed, phi circular) on discs (grid_factory.hpp:243-280, 356-400).
+void toy_material_maps(std::vector<VolumeDef>& vols, std::vector<dtr_material_slab>& mats,
+                       std::vector<dtr_material_map>& maps) {
+    const Material mix = mixture(silicon_tml(), 0.9, aluminium(), 0.1);
+    for (VolumeDef& vol : vols) {
+        for (SurfaceDef& S : vol.surfaces) {
+            const std::vector<MaskDef>& ms = S.mask_share >= 0 ? vol.surfaces[S.mask_share].masks : S.masks;
+            const uint8_t shape = ms[0].shape;
+            const bool cyl = shape == DTR_CONCENTRIC_CYLINDER2D;
+            if (!(cyl || shape == DTR_RING2D)) continue;
+            if (S.sf_id != DTR_PORTAL && !(cyl && S.sf_id == DTR_PASSIVE)) continue;
+            const bool beampipe = S.sf_id == DTR_PASSIVE;
+            const Material mat = beampipe ? beryllium_tml() : mix;
+            const double t = beampipe ? F(0.8) : cyl ? (vol.endcap_layer ? F(0.15) : F(5.0)) : F(1.0);
+            const double scalor = beampipe ? 0.0 : cyl ? static_cast<double>(1e-8f) : static_cast<double>(1e-4f);
+            dtr_material_map m{};
+            double lo = std::numeric_limits<double>::max(), hi = cyl ? -lo : 0.0;
+            for (const MaskDef& mk : ms) {
+                lo = std::min(lo, cyl ? mk.v[1] : mk.v[0]);
+                hi = std::max(hi, cyl ? mk.v[2] : mk.v[1]);
+            }
+            const uint32_t nvar = cyl ? 20u : 5u;
+            std::vector<double> thick;
+            double x = lo;
+            const double dx = (cyl ? std::fabs(hi - lo) : (hi - lo)) / static_cast<double>(nvar - 1u);
+            for (uint32_t k = 0; k < nvar; ++k) {
+                thick.push_back(cyl ? scalor * x * x + t : scalor * x + t);
+                x += dx;
+            }
+            if (cyl) {
+                m.n0 = 20u;
+                m.n1 = 20u;
+                m.min0 = -kPi;
+                m.max0 = kPi;
+                m.circular0 = 1u;
+                m.min1 = lo;
+                m.max1 = hi;
+            } else {
+                m.n0 = 5u;
+                m.n1 = 20u;
+                m.min0 = lo;
+                m.max0 = hi;
+                m.min1 = -kPi;
+                m.max1 = kPi;
+                m.circular1 = 1u;
+            }
+            m.slabs = static_cast<uint32_t>(mats.size());
+            for (uint32_t b0 = 0; b0 < m.n0; ++b0)
+                for (uint32_t b1 = 0; b1 < m.n1; ++b1) mats.push_back(make_slab(mat, thick[cyl ? b1 : b0]));
+            S.material = -1;
+            S.material_map = static_cast<int>(maps.size());
+            maps.push_back(m);
+        }
+    }
+}
+
 void build_toy(ToyCfg c, Tables& T) {
     std::vector<VolumeDef> vols;
     std::vector<dtr_material_slab> mats;
@@ -764,7 +850,9 @@ void build_toy(ToyCfg c, Tables& T) {
         add_connector(vols, c, bp_idx, neg, brl);
         add_connector(vols, c, bp_idx, pos, brl);
     }
-    assemble(vols, mats, T);
+    std::vector<dtr_material_map> maps;
+    if (c.maps) toy_material_maps(vols, mats, maps);
+    assemble(vols, mats, T, maps);
 }
 
 // ---------------------------------------------------------------------------
@@ -1025,7 +1113,9 @@ int dtr_build_toy_detector(const dtr_toy_config* cfg, dtr_detector_desc* out) {
         ToyCfg c;
         c.n_brl = cfg->n_brl_layers;
         c.n_edc = cfg->n_edc_layers;
-        c.material = cfg->use_material != 0;
+        if (cfg->use_material > 2u) return DTR_ERROR_INVALID_ARGUMENT;
+        c.material = cfg->use_material == 1u;
+        c.maps = cfg->use_material == 2u;
         Tables T;
         build_toy(c, T);
         export_tables(T, out);
@@ -1085,6 +1175,16 @@ int dtr_predefined_material(const char* name, double thickness, dtr_material_sla
     return DTR_ERROR_INVALID_ARGUMENT;
 }
 
+int dtr_check_detector(const dtr_detector_desc* d, char* msg, size_t msg_size) {
+    const std::string why = d ? validate_desc(*d) : std::string("no detector tables");
+    if (msg && msg_size) {
+        const size_t k = std::min(msg_size - 1, why.size());
+        std::memcpy(msg, why.data(), k);
+        msg[k] = '\0';
+    }
+    return why.empty() ? DTR_SUCCESS : DTR_ERROR_INVALID_ARGUMENT;
+}
+
 void dtr_host_detector_free(dtr_detector_desc* d) {
     if (!d) return;
     std::free(const_cast<dtr_volume*>(d->volumes));
@@ -1095,6 +1195,7 @@ void dtr_host_detector_free(dtr_detector_desc* d) {
     std::free(const_cast<dtr_grid*>(d->grids));
     std::free(const_cast<uint32_t*>(d->bf_surfaces));
     std::free(const_cast<uint32_t*>(d->grid_bins));
+    std::free(const_cast<dtr_material_map*>(d->material_maps));
     std::memset(d, 0, sizeof(*d));
 }
 

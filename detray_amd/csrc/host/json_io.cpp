@@ -76,6 +76,8 @@ const char* const kIoShapeNames[kIoShapes] = {
     "trapezoid2", "drift_cell", "straw_tube", "single1", "single2", "single3"};
 // io::material_id
 constexpr unsigned kIoSlab = 6u, kIoRod = 7u;
+// io::material_id of the 2D material maps (definitions.hpp:56-65)
+constexpr unsigned kIoPolar2Map = 0u, kIoCart2Map = 1u, kIoConcCyl2Map = 3u, kIoCyl2Map = 4u;
 // io::accel_id
 constexpr unsigned kIoPolar2Grid = 3u, kIoConcCyl2Grid = 4u, kIoAccels = 7u;
 // axis::bounds, axis::binning, axis::label (core/include/detray/definitions/grid_axis.hpp)
@@ -221,10 +223,19 @@ struct MaterialVolumeP {
     uint64_t volume = 0;
     std::vector<MaterialP> entries;
 };
+// detector_grids_payload<surface_material_payload, material_id>
+// (payloads.hpp, json_grids_io.hpp:120-157): one slab per bin
+struct MapP {
+    uint64_t volume = 0, owner = 0, type = 0;
+    std::vector<AxisP> axes;
+    std::vector<std::vector<uint64_t>> loc_index;
+    std::vector<std::vector<MaterialP>> content;
+};
 struct DetectorP {
     std::vector<VolumeP> volumes;
     std::vector<GridP> grids;
     std::vector<MaterialVolumeP> materials;
+    std::vector<MapP> maps;
 };
 
 // --- files -> payloads --------------------------------------------------------
@@ -306,6 +317,64 @@ void parse_grids(const Value& root, DetectorP& P) {
     }
 }
 
+// material_slab_payload / surface_material_payload (json_material_io.hpp)
+MaterialP parse_material_entry(const Value& e, const std::string& ew, bool with_surface) {
+    MaterialP m;
+    m.type = as_index(need(e, "type", ew), ew + ".type");
+    numbers(need(need(e, "material", ew), "params", ew), 7, m.params, ew + ".material.params");
+    m.thickness = as_number(need(e, "thickness", ew), ew + ".thickness");
+    if (with_surface) m.surface = as_index(need(e, "surface_idx", ew), ew + ".surface_idx");
+    if (const Value* ic = e.find("index_in_coll")) {
+        m.has_index = true;
+        m.index = as_index(*ic, ew + ".index_in_coll");
+    }
+    return m;
+}
+
+// material_map_reader.hpp:51-185 over json_grids_io.hpp:120-157
+void parse_material_maps(const Value& root, DetectorP& P) {
+    const Value* data = data_of(root, "material maps");
+    const Value* jg = data ? data->find("grids") : nullptr;
+    if (!jg) return;
+    for (const Value& entry : as_array(*jg, "material maps")) {
+        const uint64_t vol = as_index(need(entry, "volume_link", "material maps entry"), "material maps volume_link");
+        const auto& list = as_array(need(entry, "grid_data", "material maps entry"), "grid_data");
+        for (size_t q = 0; q < list.size(); ++q) {
+            const Value& g = list[q];
+            MapP M;
+            M.volume = vol;
+            const std::string where = "material map " + std::to_string(q) + " of volume " + std::to_string(vol);
+            M.owner = as_index(need(g, "owner_link", where), where + ".owner_link");
+            M.type = as_index(need(need(g, "grid_link", where), "type", where + ".grid_link"), where + ".grid_link.type");
+            const auto& axes = as_array(need(g, "axes", where), where + ".axes");
+            for (size_t k = 0; k < axes.size(); ++k) {
+                const std::string aw = where + ".axes[" + std::to_string(k) + "]";
+                AxisP A;
+                A.binning = as_index(need(axes[k], "binning", aw), aw + ".binning");
+                A.bounds = as_index(need(axes[k], "bounds", aw), aw + ".bounds");
+                A.bins = as_index(need(axes[k], "bins", aw), aw + ".bins");
+                for (const Value& e : as_array(need(axes[k], "edges", aw), aw + ".edges"))
+                    A.edges.push_back(as_number(e, aw + ".edges"));
+                M.axes.push_back(std::move(A));
+            }
+            const auto& bins = as_array(need(g, "bins", where), where + ".bins");
+            for (size_t b = 0; b < bins.size(); ++b) {
+                const std::string bw = where + ".bins[" + std::to_string(b) + "]";
+                std::vector<uint64_t> li;
+                for (const Value& i : as_array(need(bins[b], "loc_index", bw), bw + ".loc_index"))
+                    li.push_back(as_index(i, bw + ".loc_index"));
+                std::vector<MaterialP> content;
+                const auto& cs = as_array(need(bins[b], "content", bw), bw + ".content");
+                for (size_t k = 0; k < cs.size(); ++k)
+                    content.push_back(parse_material_entry(cs[k], bw + ".content[" + std::to_string(k) + "]", false));
+                M.loc_index.push_back(std::move(li));
+                M.content.push_back(std::move(content));
+            }
+            P.maps.push_back(std::move(M));
+        }
+    }
+}
+
 void parse_material(const Value& root, DetectorP& P) {
     const Value* data = data_of(root, "material");
     const Value* jv = data ? data->find("volumes") : nullptr;
@@ -318,20 +387,8 @@ void parse_material(const Value& root, DetectorP& P) {
         if (!list) list = mv.find("material_slabs");  // older files
         if (list) {
             const auto& entries = as_array(*list, where);
-            for (size_t k = 0; k < entries.size(); ++k) {
-                const std::string ew = where + " entry " + std::to_string(k);
-                const Value& e = entries[k];
-                MaterialP m;
-                m.type = as_index(need(e, "type", ew), ew + ".type");
-                numbers(need(need(e, "material", ew), "params", ew), 7, m.params, ew + ".material.params");
-                m.thickness = as_number(need(e, "thickness", ew), ew + ".thickness");
-                m.surface = as_index(need(e, "surface_idx", ew), ew + ".surface_idx");
-                if (const Value* ic = e.find("index_in_coll")) {
-                    m.has_index = true;
-                    m.index = as_index(*ic, ew + ".index_in_coll");
-                }
-                M.entries.push_back(m);
-            }
+            for (size_t k = 0; k < entries.size(); ++k)
+                M.entries.push_back(parse_material_entry(entries[k], where + " entry " + std::to_string(k), true));
         }
         P.materials.push_back(std::move(M));
     }
@@ -450,6 +507,9 @@ struct VolumeIn {
     std::vector<std::pair<int, size_t>> mat_of;
     std::vector<dtr_material_slab> mats[2];  // slabs, rods (separate index_in_coll spaces)
     std::vector<char> mat_set[2];
+    // per surface: index into maps, -1 none
+    std::vector<int> map_of;
+    std::vector<std::pair<dtr_material_map, std::vector<dtr_material_slab>>> maps;
 };
 
 SurfaceIn ingest_surface(const SurfaceP& s, const std::string& where) {
@@ -709,6 +769,70 @@ void ingest_material(const DetectorP& P, std::vector<VolumeIn>& vols) {
                         " at index_in_coll " + std::to_string(k));
 }
 
+// material_map_reader.hpp:95-185, material_map_builder.hpp:280-330: a
+// regular closed or circular axis per bound local coordinate of the
+// surface's frame, one slab per bin (unlisted bins hold no material)
+void ingest_material_maps(const DetectorP& P, std::vector<VolumeIn>& vols) {
+    for (const MapP& M : P.maps) {
+        const std::string where = "material map of volume " + std::to_string(M.volume) + " surface " +
+                                  std::to_string(M.owner);
+        if (M.volume >= vols.size()) bad(where + ": the volume is not in the geometry");
+        VolumeIn& V = vols[M.volume];
+        if (V.map_of.size() != V.surfaces.size()) V.map_of.assign(V.surfaces.size(), -1);
+        if (M.owner >= V.surfaces.size()) bad(where + ": owner_link beyond the volume's surfaces");
+        if (V.map_of[M.owner] >= 0) bad(where + ": two material maps on one surface");
+        if (V.mat_of[M.owner].first >= 0) bad(where + ": the surface also has homogeneous material");
+        const uint8_t shape = V.surfaces[M.owner].shape;
+        unsigned want;
+        switch (shape) {
+            case DTR_RING2D:
+            case DTR_ANNULUS2D: want = kIoPolar2Map; break;
+            case DTR_RECTANGLE2D:
+            case DTR_TRAPEZOID2D: want = kIoCart2Map; break;
+            case DTR_CONCENTRIC_CYLINDER2D: want = kIoConcCyl2Map; break;
+            case DTR_CYLINDER2D: want = kIoCyl2Map; break;
+            default: unsupported(where + ": material maps on line surfaces");
+        }
+        if (M.type != want)
+            bad(where + ": grid_link.type " + std::to_string(M.type) + " is not the map of the surface's frame (" +
+                std::to_string(want) + ")");
+        if (M.axes.size() != 2u) unsupported(where + ": only 2D material maps");
+        dtr_material_map m{};
+        for (int k = 0; k < 2; ++k) {
+            const AxisP& a = M.axes[k];
+            const std::string aw = where + " axis " + std::to_string(k);
+            if (a.binning != kRegular) unsupported(aw + ": irregular binning");
+            if (a.bounds != kClosed && a.bounds != kCircular) unsupported(aw + ": open bounds");
+            if (a.bins == 0u || a.bins > (1u << 20)) bad(aw + ": bin count");
+            if (a.edges.size() != 2u || !(a.edges[1] > a.edges[0])) bad(aw + ": edges are not [min, max]");
+            (k ? m.n1 : m.n0) = static_cast<uint32_t>(a.bins);
+            (k ? m.circular1 : m.circular0) = a.bounds == kCircular ? 1u : 0u;
+            (k ? m.min1 : m.min0) = a.edges[0];
+            (k ? m.max1 : m.max0) = a.edges[1];
+        }
+        const size_t nb = static_cast<size_t>(m.n0) * m.n1;
+        if (nb > (1u << 24)) unsupported(where + ": more than 2^24 bins");
+        std::vector<dtr_material_slab> slabs(nb, dtr_material_slab{});
+        std::vector<char> set(nb, 0);
+        for (size_t b = 0; b < M.loc_index.size(); ++b) {
+            const std::string bw = where + " bin " + std::to_string(b);
+            const auto& li = M.loc_index[b];
+            if (li.size() != 2u || li[0] >= m.n0 || li[1] >= m.n1) bad(bw + ": loc_index out of range");
+            const size_t at = static_cast<size_t>(li[0]) * m.n1 + li[1];
+            if (set[at]) bad(bw + ": bin given twice");
+            set[at] = 1;
+            const auto& c = M.content[b];
+            if (c.size() > 1u) bad(bw + ": more than one slab per bin");
+            if (c.empty()) continue;
+            if (c[0].type != kIoSlab) bad(bw + ": bin content is not a material slab");
+            const double* p = c[0].params;
+            slabs[at] = make_slab(p[0], p[1], p[2], p[3], p[4], c[0].thickness);
+        }
+        V.map_of[M.owner] = static_cast<int>(V.maps.size());
+        V.maps.emplace_back(m, std::move(slabs));
+    }
+}
+
 void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
     if (vols.size() > 4096u) unsupported("more than 4096 volumes");
     for (uint32_t vi = 0; vi < vols.size(); ++vi) {
@@ -752,6 +876,8 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
             ds.material = mo.first < 0 ? kInvalid
                                        : mat_offset + static_cast<uint32_t>(mo.first ? V.mats[0].size() : 0u) +
                                              static_cast<uint32_t>(mo.second);
+            if (!V.map_of.empty() && V.map_of[k] >= 0)
+                ds.material = DTR_MATERIAL_MAP | static_cast<uint32_t>(T.maps.size() + V.map_of[k]);
             ds.volume = static_cast<uint16_t>(vi);
             ds.sf_id = S.sf_id;
             ds.barcode = make_barcode(vi, S.sf_id, idx, ds.transform);
@@ -761,6 +887,13 @@ void assemble(const std::vector<VolumeIn>& vols, Tables& T) {
             unsupported("more than 2^21 surfaces or transforms");
         T.materials.insert(T.materials.end(), V.mats[0].begin(), V.mats[0].end());
         T.materials.insert(T.materials.end(), V.mats[1].begin(), V.mats[1].end());
+        // then the slabs of its material maps
+        for (const auto& mp : V.maps) {
+            dtr_material_map m = mp.first;
+            m.slabs = static_cast<uint32_t>(T.materials.size());
+            T.materials.insert(T.materials.end(), mp.second.begin(), mp.second.end());
+            T.maps.push_back(m);
+        }
         dv.sf_end = static_cast<uint32_t>(T.surfaces.size());
         dv.bf_begin = static_cast<uint32_t>(T.bf.size());
         for (uint32_t s = dv.sf_begin; s < dv.sf_end; ++s)
@@ -815,7 +948,8 @@ void check_desc(const dtr_detector_desc& d) {
             if (ds.transform >= d.n_transforms || ds.n_masks == 0u ||
                 static_cast<uint64_t>(ds.mask) + ds.n_masks > d.n_masks)
                 bad("surface " + std::to_string(s) + ": transform or mask out of range");
-            if (ds.material != kInvalid && (ds.material >= d.n_materials || !d.materials))
+            if (ds.material != kInvalid && !(ds.material & DTR_MATERIAL_MAP) &&
+                (ds.material >= d.n_materials || !d.materials))
                 bad("surface " + std::to_string(s) + ": material out of range");
         }
         if (v.grid >= 0) {
@@ -885,7 +1019,33 @@ std::vector<uint32_t> grid_ordinals(const dtr_detector_desc& d) {
 
 unsigned io_grid(uint8_t type) { return type == DTR_GRID_RING ? kIoPolar2Grid : kIoConcCyl2Grid; }
 
+// io::material_id of the map on a surface of this shape
+unsigned io_map(uint8_t shape) {
+    switch (shape) {
+        case DTR_RING2D:
+        case DTR_ANNULUS2D: return kIoPolar2Map;
+        case DTR_RECTANGLE2D:
+        case DTR_TRAPEZOID2D: return kIoCart2Map;
+        case DTR_CONCENTRIC_CYLINDER2D: return kIoConcCyl2Map;
+        default: return kIoCyl2Map;
+    }
+}
+
+// Per-type ordinal of each material map (its index in the map collection of
+// its type), in surface order
+std::vector<uint32_t> map_ordinals(const dtr_detector_desc& d) {
+    std::vector<uint32_t> ord(d.n_material_maps, 0);
+    uint32_t count[8] = {};
+    for (uint32_t s = 0; s < d.n_surfaces; ++s) {
+        const uint32_t m = d.surfaces[s].material;
+        if (m == kInvalid || !(m & DTR_MATERIAL_MAP)) continue;
+        ord[m & ~DTR_MATERIAL_MAP] = count[io_map(d.surfaces[s].shape)]++;
+    }
+    return ord;
+}
+
 Value write_geometry(const dtr_detector_desc& d, const char* name, const char* const* names) {
+    const std::vector<uint32_t> map_ord = map_ordinals(d);
     Value root = Value::object();
     Value& h = root.set("header", Value::object());
     h.set("common", header_common(name, "geometry"));
@@ -925,7 +1085,13 @@ Value write_geometry(const dtr_detector_desc& d, const char* name, const char* c
                 for (size_t q = 0; q < n_bounds(ds.shape); ++q) b.push(Value::number(dm.v[q]));
                 masks.push(std::move(jm));
             }
-            if (ds.material != kInvalid) {
+            if (ds.material != kInvalid && (ds.material & DTR_MATERIAL_MAP)) {
+                const uint32_t mi = ds.material & ~DTR_MATERIAL_MAP;
+                Value ml = Value::object();
+                ml.set("type", Value::uint(io_map(ds.shape)));
+                ml.set("index", Value::uint(map_ord[mi]));
+                sf.set("material", std::move(ml));
+            } else if (ds.material != kInvalid) {
                 const bool rod = d.materials[ds.material].kind == DTR_MATERIAL_ROD;
                 Value ml = Value::object();
                 ml.set("type", Value::uint(rod ? kIoRod : kIoSlab));
@@ -1025,7 +1191,7 @@ Value write_grids(const dtr_detector_desc& d, const char* name) {
 Value write_material(const dtr_detector_desc& d, const char* name) {
     uint64_t n_slabs = 0, n_rods = 0;
     for (uint32_t s = 0; s < d.n_surfaces; ++s) {
-        if (d.surfaces[s].material == kInvalid) continue;
+        if (d.surfaces[s].material == kInvalid || (d.surfaces[s].material & DTR_MATERIAL_MAP)) continue;
         if (d.materials[d.surfaces[s].material].kind == DTR_MATERIAL_ROD) ++n_rods;
         else ++n_slabs;
     }
@@ -1046,7 +1212,7 @@ Value write_material(const dtr_detector_desc& d, const char* name) {
         uint64_t slab = 0, rods = 0;
         for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
             const dtr_surface& ds = d.surfaces[s];
-            if (ds.material == kInvalid) continue;
+            if (ds.material == kInvalid || (ds.material & DTR_MATERIAL_MAP)) continue;
             const dtr_material_slab& m = d.materials[ds.material];
             const bool rod = m.kind == DTR_MATERIAL_ROD;
             Value e = Value::object();
@@ -1067,12 +1233,86 @@ Value write_material(const dtr_detector_desc& d, const char* name) {
     return root;
 }
 
+Value slab_entry(const dtr_material_slab& m, uint32_t surface_idx) {
+    Value e = Value::object();
+    e.set("type", Value::uint(kIoSlab));
+    e.set("surface_idx", Value::uint(surface_idx));
+    e.set("thickness", Value::number(m.thickness));
+    Value mat = Value::object();
+    Value p = Value::array();
+    for (double x : {m.X0, m.L0, m.A, m.Z, m.mass_rho, m.molar_rho, 0.0}) p.push(Value::number(x));
+    mat.set("params", std::move(p));
+    e.set("material", std::move(mat));
+    return e;
+}
+
+// material_map_writer.hpp:45-110 over grid_writer.hpp:56-160: per volume,
+// one grid payload per mapped surface, owner_link its index in the volume,
+// the bins in the serial order of the grid (axis 0 fastest), one slab each
+Value write_material_maps(const dtr_detector_desc& d, const char* name) {
+    const std::vector<uint32_t> ord = map_ordinals(d);
+    Value root = Value::object();
+    Value& h = root.set("header", Value::object());
+    h.set("common", header_common(name, "material_maps"));
+    h.set("grid_count", Value::uint(d.n_material_maps));
+    Value grids = Value::array();
+    for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
+        const dtr_volume& v = d.volumes[vi];
+        Value list = Value::array();
+        for (uint32_t s = v.sf_begin; s < v.sf_end; ++s) {
+            const dtr_surface& ds = d.surfaces[s];
+            if (ds.material == kInvalid || !(ds.material & DTR_MATERIAL_MAP)) continue;
+            const uint32_t mi = ds.material & ~DTR_MATERIAL_MAP;
+            const dtr_material_map& m = d.material_maps[mi];
+            Value gd = Value::object();
+            gd.set("owner_link", Value::uint(s - v.sf_begin));
+            Value link = Value::object();
+            link.set("type", Value::uint(io_map(ds.shape)));
+            link.set("index", Value::uint(ord[mi]));
+            gd.set("grid_link", std::move(link));
+            Value axes = Value::array();
+            axes.push(write_axis(0u, m.circular0 ? kCircular : kClosed, m.n0, m.min0, m.max0));
+            axes.push(write_axis(1u, m.circular1 ? kCircular : kClosed, m.n1, m.min1, m.max1));
+            gd.set("axes", std::move(axes));
+            Value bins = Value::array();
+            for (uint32_t i1 = 0; i1 < m.n1; ++i1) {
+                for (uint32_t i0 = 0; i0 < m.n0; ++i0) {
+                    Value b = Value::object();
+                    Value li = Value::array();
+                    li.push(Value::uint(i0));
+                    li.push(Value::uint(i1));
+                    b.set("loc_index", std::move(li));
+                    Value content = Value::array();
+                    content.push(slab_entry(d.materials[m.slabs + static_cast<size_t>(i0) * m.n1 + i1], s - v.sf_begin));
+                    b.set("content", std::move(content));
+                    bins.push(std::move(b));
+                }
+            }
+            gd.set("bins", std::move(bins));
+            list.push(std::move(gd));
+        }
+        if (list.a.empty()) continue;
+        Value entry = Value::object();
+        entry.set("volume_link", Value::uint(vi));
+        entry.set("grid_data", std::move(list));
+        grids.push(std::move(entry));
+    }
+    if (grids.a.empty()) {
+        root.set("data", Value::null());
+    } else {
+        Value& data = root.set("data", Value::object());
+        data.set("grids", std::move(grids));
+    }
+    return root;
+}
+
 // payloads -> validated tables, exported into `out`
 void build_tables(const DetectorP& P, dtr_detector_desc* out) {
     std::vector<VolumeIn> vols;
     ingest_geometry(P, vols);
     ingest_grids(P, vols);
     ingest_material(P, vols);
+    ingest_material_maps(P, vols);
     Tables T;
     assemble(vols, T);
     dtr_detector_desc view{};
@@ -1092,6 +1332,8 @@ void build_tables(const DetectorP& P, dtr_detector_desc* out) {
     view.grids = T.grids.data();
     view.bf_surfaces = T.bf.data();
     view.grid_bins = T.bins.data();
+    view.n_material_maps = static_cast<uint32_t>(T.maps.size());
+    view.material_maps = T.maps.data();
     const std::string why = validate_desc(view);
     if (!why.empty()) bad("detector tables read from the files: " + why);
     export_tables(T, out);
@@ -1109,7 +1351,7 @@ void message(char* out, size_t n, const std::string& m) {
 extern "C" {
 
 int dtr_read_detector_json(const char* geometry_file, const char* grids_file, const char* material_file,
-                           dtr_detector_desc* out, char* msg, size_t msg_size) {
+                           const char* material_maps_file, dtr_detector_desc* out, char* msg, size_t msg_size) {
     message(msg, msg_size, "");
     if (!geometry_file || !out) {
         message(msg, msg_size, "geometry file and output tables are required");
@@ -1120,6 +1362,7 @@ int dtr_read_detector_json(const char* geometry_file, const char* grids_file, co
         parse_geometry(load(geometry_file, "geometry"), P);
         if (grids_file) parse_grids(load(grids_file, "surface grids"), P);
         if (material_file) parse_material(load(material_file, "homogeneous material"), P);
+        if (material_maps_file) parse_material_maps(load(material_maps_file, "material maps"), P);
         build_tables(P, out);
         return DTR_SUCCESS;
     } catch (const IoError& e) {
@@ -1159,8 +1402,8 @@ int dtr_flatten_detector(const dtr_detector_view* view, dtr_detector_desc* out, 
 
 int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_name,
                             const char* const* volume_names, const char* geometry_file,
-                            const char* grids_file, const char* material_file, char* msg,
-                            size_t msg_size) {
+                            const char* grids_file, const char* material_file,
+                            const char* material_maps_file, char* msg, size_t msg_size) {
     message(msg, msg_size, "");
     if (!det) {
         message(msg, msg_size, "detector tables are required");
@@ -1173,6 +1416,7 @@ int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_n
         if (geometry_file) save(geometry_file, write_geometry(*det, detector_name, volume_names), "geometry");
         if (grids_file) save(grids_file, write_grids(*det, detector_name), "surface grids");
         if (material_file) save(material_file, write_material(*det, detector_name), "homogeneous material");
+        if (material_maps_file) save(material_maps_file, write_material_maps(*det, detector_name), "material maps");
         return DTR_SUCCESS;
     } catch (const IoError& e) {
         message(msg, msg_size, e.msg);

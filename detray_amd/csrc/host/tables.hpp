@@ -34,6 +34,7 @@ struct Tables {
     std::vector<dtr_grid> grids;
     std::vector<uint32_t> bf;
     std::vector<uint32_t> bins;
+    std::vector<dtr_material_map> maps;
 };
 
 // geometry::identifier bit fields (core/include/detray/geometry/identifier.hpp:
@@ -83,7 +84,7 @@ inline std::string validate_desc(const dtr_detector_desc& d) {
     if (d.n_volumes > kEndOfWorld) return "more volumes than a 16-bit volume link can name";
     if (!d.volumes || !d.surfaces || (d.n_transforms && !d.transforms) || (d.n_masks && !d.masks) ||
         (d.n_materials && !d.materials) || (d.n_grids && !d.grids) || (d.n_bf && !d.bf_surfaces) ||
-        (d.n_bins && !d.grid_bins))
+        (d.n_bins && !d.grid_bins) || (d.n_material_maps && !d.material_maps))
         return "a table with a non-zero count has no data";
     for (uint32_t vi = 0; vi < d.n_volumes; ++vi) {
         const dtr_volume& v = d.volumes[vi];
@@ -103,7 +104,14 @@ inline std::string validate_desc(const dtr_detector_desc& d) {
             return at("surface", s) + "mask range out of range";
         if (sf.shape > DTR_CYLINDER2D) return at("surface", s) + "unknown shape";
         if (sf.sf_id > DTR_PASSIVE) return at("surface", s) + "unknown surface id";
-        if (sf.material != kInvalid && sf.material >= d.n_materials) return at("surface", s) + "material out of range";
+        if (sf.material != kInvalid) {
+            if (sf.material & DTR_MATERIAL_MAP) {
+                if ((sf.material & ~DTR_MATERIAL_MAP) >= d.n_material_maps)
+                    return at("surface", s) + "material map out of range";
+            } else if (sf.material >= d.n_materials) {
+                return at("surface", s) + "material out of range";
+            }
+        }
         if (sf.volume >= d.n_volumes) return at("surface", s) + "volume out of range";
     }
     for (uint32_t m = 0; m < d.n_masks; ++m) {
@@ -114,6 +122,16 @@ inline std::string validate_desc(const dtr_detector_desc& d) {
     for (uint32_t k = 0; k < d.n_materials; ++k) {
         if (d.materials[k].kind > DTR_MATERIAL_ROD) return at("material", k) + "unknown kind";
         if (d.materials[k].has_density > 1u) return at("material", k) + "has_density is not 0 or 1";
+    }
+    for (uint32_t k = 0; k < d.n_material_maps; ++k) {
+        const dtr_material_map& m = d.material_maps[k];
+        if (m.n0 == 0u || m.n1 == 0u || !(m.max0 > m.min0) || !(m.max1 > m.min1) || m.circular0 > 1u ||
+            m.circular1 > 1u)
+            return at("material map", k) + "empty or inverted axis";
+        if (static_cast<uint64_t>(m.slabs) + static_cast<uint64_t>(m.n0) * m.n1 > d.n_materials)
+            return at("material map", k) + "slabs out of range";
+        for (uint64_t b = 0; b < static_cast<uint64_t>(m.n0) * m.n1; ++b)
+            if (d.materials[m.slabs + b].kind != DTR_MATERIAL_SLAB) return at("material map", k) + "holds a rod";
     }
     for (uint32_t gi = 0; gi < d.n_grids; ++gi) {
         const dtr_grid& g = d.grids[gi];
@@ -147,6 +165,8 @@ inline void export_tables(const Tables& T, dtr_detector_desc* out) {
     out->grids = dup(T.grids);
     out->bf_surfaces = dup(T.bf);
     out->grid_bins = dup(T.bins);
+    out->n_material_maps = static_cast<uint32_t>(T.maps.size());
+    out->material_maps = dup(T.maps);
 }
 
 }  // namespace dtr_host

@@ -17,7 +17,8 @@
 struct dtr_detector {
     dtr::DevDetector dev;
     dtr::FastDetector fast;
-    void* blocks[12];  // the eight tables, surface records, bin records, brute-force records, grid tolerances
+    void* blocks[13];  // the eight tables, surface records, bin records, brute-force records, grid tolerances,
+                       // material maps
     uint32_t n_volumes;
     uint32_t n_surfaces;
     int device;

@@ -117,11 +117,16 @@ class Detector:
 
     # -- builders ----------------------------------------------------------
     @classmethod
-    def toy(cls, n_brl_layers: int = 4, n_edc_layers: int = 3, use_material: bool = True) -> "Detector":
-        """detray's toy detector (build_toy_detector.hpp)."""
+    def toy(cls, n_brl_layers: int = 4, n_edc_layers: int = 3, use_material: "bool | str" = True) -> "Detector":
+        """detray's toy detector (build_toy_detector.hpp): ``use_material``
+        True for silicon slabs on the modules and a beryllium beampipe,
+        ``"maps"`` for material maps on every portal and the beampipe
+        (toy_det_config::use_material_maps), False for none."""
         lib = capi.load()
         d = capi.DetectorDesc()
-        cfg = capi.ToyConfig(n_brl_layers, n_edc_layers, int(use_material))
+        if use_material not in (True, False, "maps"):
+            raise ValueError("use_material is True, False or 'maps'")
+        cfg = capi.ToyConfig(n_brl_layers, n_edc_layers, 2 if use_material == "maps" else int(use_material))
         capi.check(lib.dtr_build_toy_detector(C.byref(cfg), C.byref(d)), "dtr_build_toy_detector")
         return cls(d, f"toy_{n_brl_layers}brl_{n_edc_layers}edc")
 
@@ -161,9 +166,10 @@ class Detector:
 
     @classmethod
     def from_tables(cls, volumes, surfaces, transforms, masks, materials=(), grids=(), bf_surfaces=(),
-                    grid_bins=(), name: str = "tables") -> "Detector":
+                    grid_bins=(), name: str = "tables", material_maps=()) -> "Detector":
         """Detector from explicit tables (sequences of the capi structures
-        and of surface indices), e.g. for geometries no emitter builds."""
+        and of surface indices), e.g. for geometries no emitter builds; a
+        surface links a map with material = capi.MATERIAL_MAP | index."""
         def arr(ctype, items):
             items = list(items)
             return (ctype * max(len(items), 1))(*items), len(items)
@@ -172,28 +178,32 @@ class Detector:
         for field, ctype, items in (("volumes", capi.Volume, volumes), ("surfaces", capi.Surface, surfaces),
                                     ("transforms", capi.Transform, transforms), ("masks", capi.Mask, masks),
                                     ("materials", capi.MaterialSlab, materials), ("grids", capi.Grid, grids),
-                                    ("bf_surfaces", C.c_uint32, bf_surfaces), ("grid_bins", C.c_uint32, grid_bins)):
+                                    ("bf_surfaces", C.c_uint32, bf_surfaces), ("grid_bins", C.c_uint32, grid_bins),
+                                    ("material_maps", capi.MaterialMap, material_maps)):
             a, n = arr(ctype, items)
             keep.append(a)
             setattr(d, field, C.cast(a, C.POINTER(ctype)))
             count = {"volumes": "n_volumes", "surfaces": "n_surfaces", "transforms": "n_transforms",
                      "masks": "n_masks", "materials": "n_materials", "grids": "n_grids", "bf_surfaces": "n_bf",
-                     "grid_bins": "n_bins"}[field]
+                     "grid_bins": "n_bins", "material_maps": "n_material_maps"}[field]
             setattr(d, count, n)
         return cls(d, name, keep=keep)
 
     @classmethod
     def from_json(cls, geometry: "os.PathLike | str", grids: "os.PathLike | str | None" = None,
-                  material: "os.PathLike | str | None" = None, name: Optional[str] = None) -> "Detector":
+                  material: "os.PathLike | str | None" = None, name: Optional[str] = None,
+                  material_maps: "os.PathLike | str | None" = None) -> "Detector":
         """Read detray's JSON detector files (``<name>_geometry.json`` and,
-        optionally, ``<name>_surface_grids.json`` and
-        ``<name>_homogeneous_material.json``) as the reference's
-        io::read_detector builds them (dtr_read_detector_json)."""
+        optionally, ``<name>_surface_grids.json``,
+        ``<name>_homogeneous_material.json`` and
+        ``<name>_material_maps.json``) as the reference's io::read_detector
+        builds them (dtr_read_detector_json)."""
         lib = capi.load()
         d = capi.DetectorDesc()
         msg = C.create_string_buffer(512)
         enc = lambda p: None if p is None else os.fsencode(p)
-        code = lib.dtr_read_detector_json(enc(geometry), enc(grids), enc(material), C.byref(d), msg, len(msg))
+        code = lib.dtr_read_detector_json(enc(geometry), enc(grids), enc(material), enc(material_maps), C.byref(d),
+                                          msg, len(msg))
         if code != capi.DTR_SUCCESS:
             raise capi.DetrayError(f"dtr_read_detector_json failed: {capi.STATUS_NAMES.get(code, code)}: "
                                    f"{msg.value.decode(errors='replace')}")
@@ -206,7 +216,8 @@ class Detector:
                    grids: bool = True, material: bool = True,
                    volume_names: Optional[Sequence[str]] = None) -> dict:
         """Write the tables as detray JSON files into ``directory``
-        (io::write_detector file names); returns {"geometry": path, ...}."""
+        (io::write_detector file names, the material-map file when a surface
+        carries a map); returns {"geometry": path, ...}."""
         lib = capi.load()
         name = name or self.name
         out = Path(directory)
@@ -215,6 +226,8 @@ class Detector:
             files["surface_grids"] = out / f"{name}_surface_grids.json"
         if material:
             files["homogeneous_material"] = out / f"{name}_homogeneous_material.json"
+            if self.desc.n_material_maps:
+                files["material_maps"] = out / f"{name}_material_maps.json"
         names = None
         if volume_names is not None:
             if len(volume_names) != self.desc.n_volumes:
@@ -223,11 +236,19 @@ class Detector:
         msg = C.create_string_buffer(512)
         enc = lambda k: os.fsencode(files[k]) if k in files else None
         code = lib.dtr_write_detector_json(C.byref(self.desc), name.encode(), names, enc("geometry"),
-                                           enc("surface_grids"), enc("homogeneous_material"), msg, len(msg))
+                                           enc("surface_grids"), enc("homogeneous_material"), enc("material_maps"),
+                                           msg, len(msg))
         if code != capi.DTR_SUCCESS:
             raise capi.DetrayError(f"dtr_write_detector_json failed: {capi.STATUS_NAMES.get(code, code)}: "
                                    f"{msg.value.decode(errors='replace')}")
         return {k: str(v) for k, v in files.items()}
+
+    def check(self) -> None:
+        """The table checks dtr_detector_create applies, on the host
+        (dtr_check_detector); raises DetrayError naming the first problem."""
+        msg = C.create_string_buffer(512)
+        if capi.load().dtr_check_detector(C.byref(self.desc), msg, len(msg)) != capi.DTR_SUCCESS:
+            raise capi.DetrayError(f"{self.name}: {msg.value.decode(errors='replace')}")
 
     # -- table views -------------------------------------------------------
     def table(self, name: str):
@@ -236,6 +257,7 @@ class Detector:
             "transforms": self.desc.n_transforms, "masks": self.desc.n_masks,
             "materials": self.desc.n_materials, "grids": self.desc.n_grids,
             "bf_surfaces": self.desc.n_bf, "grid_bins": self.desc.n_bins,
+            "material_maps": self.desc.n_material_maps,
         }[name]
         ptr = getattr(self.desc, name)
         return [ptr[i] for i in range(n)]

@@ -127,11 +127,31 @@ typedef struct dtr_material_slab {
     double density[5];    /* a, k (the exponent m), x0, x1, C-bar */
 } dtr_material_slab;
 
+/* Material map (material_map.hpp: grid<axes<shape>, bins::single<
+ * material_slab>>): one slab per bin over the surface's bound local
+ * coordinates (loc0, loc1) — (phi, z) on a concentric cylinder, (r, phi) on
+ * a ring or annulus, (x, y) on a rectangle or trapezoid, (r phi, z) on a
+ * placed cylinder — regular axes, a circular axis wrapping (phi over
+ * [-pi, pi]), a closed one clamping to its edge bins (axis_bounds.hpp:
+ * 107-206, axis_binning.hpp:80-115).  Bin (i0, i1) holds
+ * materials[slabs + i0 * n1 + i1]; a slab of zero thickness is no material
+ * (pointwise_material_interactor.hpp:84-88).  A surface uses a map with
+ * material = DTR_MATERIAL_MAP | map index. */
+#define DTR_MATERIAL_MAP 0x80000000u
+typedef struct dtr_material_map {
+    uint32_t n0, n1;      /* bins per axis */
+    uint32_t slabs;       /* first of the n0 * n1 slabs in materials[] */
+    uint8_t circular0, circular1;
+    uint8_t pad[2];
+    double min0, max0, min1, max1;
+} dtr_material_map;
+
 typedef struct dtr_surface {
     uint64_t barcode;   /* detray::geometry::identifier (identifier.hpp:153-157) */
     uint32_t transform; /* into transforms[] */
     uint32_t mask;      /* first mask in masks[] */
-    uint32_t material;  /* into materials[], 0xffffffff = none */
+    uint32_t material;  /* into materials[], DTR_MATERIAL_MAP | index into
+                           material_maps[], 0xffffffff = none */
     uint16_t volume;
     uint8_t n_masks;
     uint8_t sf_id;      /* dtr_surface_id */
@@ -181,6 +201,9 @@ typedef struct dtr_detector_desc {
     const dtr_grid* grids;
     const uint32_t* bf_surfaces;
     const uint32_t* grid_bins; /* surface index or 0xffffffff */
+    uint32_t n_material_maps;
+    uint32_t pad;
+    const dtr_material_map* material_maps;
 } dtr_detector_desc;
 
 /* ------------------------------------------------------------------------ */
@@ -664,7 +687,10 @@ DTR_API int dtr_adjoint_backward(const dtr_detector* det, const dtr_field* field
 typedef struct dtr_toy_config {
     uint32_t n_brl_layers;    /* 4 (build_toy_detector.hpp:117) */
     uint32_t n_edc_layers;    /* 3 (build_toy_detector.hpp:119) */
-    uint8_t use_material;     /* 1: Si slabs on modules, Be beampipe */
+    uint8_t use_material;     /* 0: none; 1: Si slabs on modules, Be beampipe;
+                                 2: material maps on every portal and the
+                                 beampipe, none on modules
+                                 (toy_det_config::use_material_maps) */
     uint8_t pad[3];
 } dtr_toy_config;
 
@@ -701,6 +727,15 @@ DTR_API int dtr_build_telescope_detector(const dtr_telescope_config* cfg,
                                  dtr_detector_desc* out);
 DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);
 
+/* The table checks dtr_detector_create applies, on the host: every index the
+ * navigation follows in range (volumes, surfaces, masks, materials and
+ * material maps, grid bins), known shapes and ids, maps whose slabs exist
+ * and are slabs.  The part of detail::check_consistency
+ * (utils/consistency_checker.hpp:314-380) that flat tables can break.
+ * DTR_SUCCESS, or DTR_ERROR_INVALID_ARGUMENT with the first problem in
+ * `message` (may be NULL). */
+DTR_API int dtr_check_detector(const dtr_detector_desc* desc, char* message, size_t message_size);
+
 /* A material of predefined_materials.hpp by its name there ("silicon",
  * "silicon_with_ded", "silicon_tml", "beryllium_tml", "tungsten",
  * "isobutane", "argon_liquid", "iron_with_ded", "copper_with_ded",
@@ -710,22 +745,27 @@ DTR_API void dtr_host_detector_free(dtr_detector_desc* desc);
 DTR_API int dtr_predefined_material(const char* name, double thickness, dtr_material_slab* out);
 
 /* Detector tables from detray's JSON detector files: the geometry file and,
- * optionally (NULL to skip), the surface-grid and homogeneous-material
- * files.  Replaces io::read_detector with the json geometry_reader,
- * surface_grid_reader and homogeneous_material_reader
- * (io/include/detray/io/frontend/detector_reader.hpp,
+ * optionally (NULL to skip), the surface-grid, homogeneous-material and
+ * material-map files.  Replaces io::read_detector with the json
+ * geometry_reader, surface_grid_reader, homogeneous_material_reader and
+ * material_map_reader (io/include/detray/io/frontend/detector_reader.hpp,
  * io/include/detray/io/backend/geometry_reader.hpp:51-170,
- * detail/grid_reader.hpp:55-420, homogeneous_material_reader.hpp:40-120):
- * the tables are those the reference's detector builder makes of the files
- * (volume order, surface order by index_in_coll or by portal/shape
- * factories, cylinder z shifts folded into the masks, one slab per material
- * surface).  Shapes, grids and materials the tables cannot hold return
+ * detail/grid_reader.hpp:55-420, homogeneous_material_reader.hpp:40-120,
+ * material_map_reader.hpp:51-185): the tables are those the reference's
+ * detector builder makes of the files (volume order, surface order by
+ * index_in_coll or by portal/shape factories, cylinder z shifts folded into
+ * the masks, one slab per material surface, one dtr_material_map per mapped
+ * surface with its slabs after the volume's homogeneous ones, bins the file
+ * leaves out holding no material).  Material maps need regular closed or
+ * circular axes and the map type of the surface's frame (a surface with
+ * both a slab and a map is an error).  Shapes, grids and materials the
+ * tables cannot hold return
  * DTR_ERROR_UNSUPPORTED; malformed files DTR_ERROR_IO or
  * DTR_ERROR_INVALID_ARGUMENT, with a description in `message` (may be NULL).
  * Free the tables with dtr_host_detector_free. */
 DTR_API int dtr_read_detector_json(const char* geometry_file, const char* grids_file,
-                                   const char* material_file, dtr_detector_desc* out,
-                                   char* message, size_t message_size);
+                                   const char* material_file, const char* material_maps_file,
+                                   dtr_detector_desc* out, char* message, size_t message_size);
 
 /* In-memory detector view: the reference's IO payloads
  * (io/include/detray/io/frontend/payloads.hpp:86-330) as plain arrays, so a
@@ -805,17 +845,19 @@ typedef struct dtr_detector_view {
 DTR_API int dtr_flatten_detector(const dtr_detector_view* view, dtr_detector_desc* out, char* message,
                                  size_t message_size);
 
-/* Write detector tables as detray JSON files (geometry; surface grids and
- * homogeneous material, NULL to skip), as io::write_detector with the json
- * geometry_writer, surface_grid_writer and homogeneous_material_writer
+/* Write detector tables as detray JSON files (geometry; surface grids,
+ * homogeneous material and material maps, NULL to skip), as
+ * io::write_detector with the json geometry_writer, surface_grid_writer,
+ * homogeneous_material_writer and material_map_writer
  * (io/include/detray/io/frontend/detector_writer.hpp,
  * io/include/detray/io/backend/geometry_writer.hpp:40-150,
- * surface_grid_writer.hpp:40-85, homogeneous_material_writer.hpp:40-190)
- * write them.  volume_names may be NULL (empty names). */
+ * surface_grid_writer.hpp:40-85, homogeneous_material_writer.hpp:40-190,
+ * material_map_writer.hpp:45-110) write them.  volume_names may be NULL
+ * (empty names). */
 DTR_API int dtr_write_detector_json(const dtr_detector_desc* det, const char* detector_name,
                                     const char* const* volume_names, const char* geometry_file,
                                     const char* grids_file, const char* material_file,
-                                    char* message, size_t message_size);
+                                    const char* material_maps_file, char* message, size_t message_size);
 
 /* Sample the analytic finite solenoid (on-axis 2 T, radius 1.2 m, half
  * length 3 m) on a regular grid.  `values` must hold nx*ny*nz*3 floats. */

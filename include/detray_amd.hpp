@@ -121,11 +121,14 @@ class host_detector {
         check(dtr_build_wire_chamber(&c, &h.d_), "dtr_build_wire_chamber");
         return h;
     }
-    /* io::read_detector with the JSON readers (grids and material may be null) */
-    static host_detector from_json(const char* geometry, const char* grids = nullptr, const char* material = nullptr) {
+    /* io::read_detector with the JSON readers (grids, material and material
+     * maps may be null) */
+    static host_detector from_json(const char* geometry, const char* grids = nullptr, const char* material = nullptr,
+                                   const char* material_maps = nullptr) {
         host_detector h;
         char msg[512] = {0};
-        check(dtr_read_detector_json(geometry, grids, material, &h.d_, msg, sizeof msg), "dtr_read_detector_json", msg);
+        check(dtr_read_detector_json(geometry, grids, material, material_maps, &h.d_, msg, sizeof msg),
+              "dtr_read_detector_json", msg);
         return h;
     }
     /* tables of an in-memory detector view (see flatten()) */

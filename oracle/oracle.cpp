@@ -1723,10 +1723,33 @@ struct Engine {
         n_hit++;
     }
 
+    // material_accessor::get (material/detail/material_accessor.hpp:30-48):
+    // a homogeneous slab, or the slab of the material map's bin at the bound
+    // local position, grid::bin of a regular 2D grid whose axes clamp
+    // (closed) or wrap (circular): axis_binning.hpp:80-82 (bin = int((v -
+    // min) / width + 1) - 1), axis_bounds.hpp:107-206
+    const dtr_material_slab& surface_slab(uint32_t link, double l0, double l1) const {
+        if (!(link & DTR_MATERIAL_MAP)) return det.d->materials[link];
+        const dtr_material_map& mp = det.d->material_maps[link & ~DTR_MATERIAL_MAP];
+        auto bin = [](double v, double lo, double hi, uint32_t n, bool circular) {
+            const double width = (hi - lo) / static_cast<double>(n);
+            int b = static_cast<int>((v - lo) / width + 1.0) - 1;
+            const int nb = static_cast<int>(n);
+            if (circular) {
+                b %= nb;
+                return b < 0 ? b + nb : b;
+            }
+            return std::min(std::max(b, 0), nb - 1);
+        };
+        const int i0 = bin(l0, mp.min0, mp.max0, mp.n0, mp.circular0 != 0u);
+        const int i1 = bin(l1, mp.min1, mp.max1, mp.n1, mp.circular1 != 0u);
+        return det.d->materials[mp.slabs + static_cast<uint32_t>(i0) * mp.n1 + static_cast<uint32_t>(i1)];
+    }
+
     void material_and_setter(uint32_t dest, const double fp[8]) {
         bool success = false;
         if (encountered_material()) {
-            const dtr_material_slab& m = det.d->materials[det.sf(dest).material];
+            const dtr_material_slab& m = surface_slab(det.sf(dest).material, up.bound[0], up.bound[1]);
             const V3 n = surface_normal(dest, up.bound);
             const double cos_inc = std::fabs(dot(bound_dir(up.bound), n));
             double e_loss = 0.0, sigma_qop = 0.0, theta0 = 0.0;

@@ -378,7 +378,8 @@ int run_cpu(const std::string& dir) {
         const std::string g = dir + "/toy_geometry.json", s = dir + "/toy_surface_grids.json",
                           m = dir + "/toy_homogeneous_material.json";
         char msg[256];
-        check(dtr_write_detector_json(&toy.desc(), "toy", nullptr, g.c_str(), s.c_str(), m.c_str(), msg, sizeof msg),
+        check(dtr_write_detector_json(&toy.desc(), "toy", nullptr, g.c_str(), s.c_str(), m.c_str(), nullptr, msg,
+                                      sizeof msg),
               "dtr_write_detector_json", msg);
         const auto geo = geometry_of(load(g));
         const auto grids = grids_of(load(s));

@@ -41,6 +41,7 @@ struct HostDetector {
         fast.sf = sf.data();
         fast.masks = d.masks;
         fast.materials = d.materials;
+        fast.mat_maps = d.material_maps;
         fast.volumes = d.volumes;
         fast.grids = d.grids;
         fast.transforms = d.transforms;

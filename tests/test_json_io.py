@@ -39,12 +39,13 @@ IO_PORTAL_CYLINDER2, IO_RECTANGLE2, IO_RING2, IO_TRAPEZOID2 = 4, 5, 6, 7
 INVALID_ARGUMENT, UNSUPPORTED, IO = 1, 5, 6
 
 
-def read_error(geometry, grids=None, material=None):
+def read_error(geometry, grids=None, material=None, material_maps=None):
     lib = capi.load()
     d = capi.DetectorDesc()
     msg = C.create_string_buffer(512)
     enc = lambda p: None if p is None else str(p).encode()
-    code = lib.dtr_read_detector_json(enc(geometry), enc(grids), enc(material), C.byref(d), msg, len(msg))
+    code = lib.dtr_read_detector_json(enc(geometry), enc(grids), enc(material), enc(material_maps), C.byref(d), msg,
+                                      len(msg))
     if code == 0:
         lib.dtr_host_detector_free(C.byref(d))
     return code, msg.value.decode()
@@ -369,9 +370,9 @@ def test_writer_rejects_inconsistent_tables():
     lib = capi.load()
     msg = C.create_string_buffer(256)
     empty = capi.DetectorDesc()
-    assert lib.dtr_write_detector_json(C.byref(empty), b"x", None, b"/dev/null", None, None, msg, 256) == \
+    assert lib.dtr_write_detector_json(C.byref(empty), b"x", None, b"/dev/null", None, None, None, msg, 256) == \
         INVALID_ARGUMENT
-    assert lib.dtr_write_detector_json(None, b"x", None, b"/dev/null", None, None, msg, 256) == INVALID_ARGUMENT
+    assert lib.dtr_write_detector_json(None, b"x", None, b"/dev/null", None, None, None, msg, 256) == INVALID_ARGUMENT
 
 
 def test_volume_names_and_header(tmp_path):
