@@ -48,9 +48,10 @@ def prefilter(detector, config, rays, ext_tol: float = 0.0):
 
 
 def propagate(detector, field, config, free_params, mixed: bool = False, initial_cov=None, bound_params=None,
-              start_surface=None) -> dict:
+              start_surface=None, plain: bool = False) -> dict:
     """The engine's lane loop (propagate_fast) over host arrays, FP64 or
-    mixed policy; the result arrays of dtr_propagate_ex."""
+    mixed policy; with plain=True the plain FP64 kernel's per-track loop
+    (propagate_track); the result arrays of dtr_propagate_ex."""
     lib = load()
     fp = None if free_params is None else np.ascontiguousarray(free_params, dtype=np.float64)
     bp = None if bound_params is None else np.ascontiguousarray(bound_params, dtype=np.float64)
@@ -76,6 +77,10 @@ def propagate(detector, field, config, free_params, mixed: bool = False, initial
         "bound_params", "bound_cov", "free_params", "path_length", "surfaces", "n_surfaces",
         "n_steps", "n_trials", "status")])
     cfg = config.to_c()
+    if plain:
+        capi.check(lib.dtr_test_plain_propagate(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
+                                                C.byref(tb), C.byref(rb)), "dtr_test_plain_propagate")
+        return out
     capi.check(lib.dtr_test_engine_propagate(C.byref(detector.desc), C.byref(field.desc), C.byref(cfg),
                                              C.byref(tb), C.byref(rb), C.c_int(1 if mixed else 0)),
                "dtr_test_engine_propagate")

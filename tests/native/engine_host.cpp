@@ -242,6 +242,35 @@ __attribute__((visibility("default"))) int dtr_test_engine_propagate(const dtr_d
     return DTR_SUCCESS;
 }
 
+// The plain kernel's per-track propagation (propagate_track, the FP64
+// kernel of dtr_propagate), extended geometry when the detector needs it as
+// dtr_detector_create decides.  Host memory throughout.
+__attribute__((visibility("default"))) int dtr_test_plain_propagate(const dtr_detector_desc* d,
+                                                                    const dtr_field_desc* f,
+                                                                    const dtr_prop_config* cfg,
+                                                                    const dtr_track_batch* in,
+                                                                    dtr_result_batch* out) {
+    if (!d || !f || !cfg || !in || !out) return DTR_ERROR_INVALID_ARGUMENT;
+    if (cfg->use_field_gradient || cfg->actors) return DTR_ERROR_UNSUPPORTED;
+    const DevDetector det{d->volumes, d->surfaces, d->transforms, d->masks, d->materials,
+                          d->grids, d->bf_surfaces, d->grid_bins, d->material_maps};
+    const DevField field = host_field(f);
+    const DevConfig dc = make_dev_config(cfg);
+    const DevResults r{out->bound_params, out->bound_cov, out->free_params, out->path_length,
+                       cfg->max_surfaces ? out->surfaces : nullptr, out->n_surfaces, out->n_steps,
+                       out->n_trials, out->status};
+    const bool ext = has_ext(*d);
+    for (uint32_t gid = 0; gid < in->n; ++gid) {
+        if (ext)
+            propagate_track<true>(det, field, dc, in->bound_params ? nullptr : in->free_params, in->initial_cov,
+                                  in->bound_params, in->start_surface, r, in->n, gid);
+        else
+            propagate_track<false>(det, field, dc, in->bound_params ? nullptr : in->free_params, in->initial_cov,
+                                   in->bound_params, in->start_surface, r, in->n, gid);
+    }
+    return DTR_SUCCESS;
+}
+
 // The direct navigator: sequence [max_sequence][n] surfaces, n_sequence [n],
 // direction DTR_DIRECTION_FORWARD | BACKWARD.  Host memory throughout.
 __attribute__((visibility("default"))) int dtr_test_engine_propagate_direct(

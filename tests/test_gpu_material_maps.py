@@ -25,7 +25,8 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 LAUNCHES = {
-    "plain": da.LaunchConfig(),
+    "plain": None,
+    "mixed_persistent": da.LaunchConfig(),
     "fp64_lane": da.LaunchConfig("fp64", "grid", 256, 1, 64, None, 1, "lane"),
     "mixed_gather": da.LaunchConfig("mixed", "grid", 64, 2, 64, None, 1, "gather", 64),
     "fp64_gather": da.LaunchConfig("fp64", "grid", 64, 2, 64, None, 1, "gather", 64),

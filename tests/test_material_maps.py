@@ -112,12 +112,12 @@ def _assert_same(a, b, cols, label):
 
 
 @pytest.mark.parametrize("circular", [False, True])
-@pytest.mark.parametrize("run", ["oracle", "engine_fp64", "engine_mixed"])
+@pytest.mark.parametrize("run", ["oracle", "plain", "engine_fp64", "engine_mixed"])
 def test_map_bins_equal_homogeneous_slabs(circular, run):
     def go(det, fp):
         if run == "oracle":
             return oracle_host.propagate(det, FIELD, CFG, fp)
-        return engine_host.propagate(det, FIELD, CFG, fp, mixed=run == "engine_mixed")
+        return engine_host.propagate(det, FIELD, CFG, fp, mixed=run == "engine_mixed", plain=run == "plain")
     fp = beam_tracks(XS)
     got = go(map_telescope(circular), fp)
     bins = np.array([expected_bin(x, circular) for x in XS])
@@ -233,16 +233,16 @@ def test_toy_maps_follow_the_generator():
     assert seen_thin > 0 and seen_thick > 0
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_toy_with_maps_engine_matches_oracle(solenoid, mixed):
-    """The batched engine (host build) against the oracle on the toy with
-    maps in the solenoid map: identical surface sequences and the batch
-    contracts on the parameters."""
+@pytest.mark.parametrize("mixed,plain", [(False, False), (True, False), (False, True)])
+def test_toy_with_maps_engine_matches_oracle(solenoid, mixed, plain):
+    """The batched engine and the plain kernel (host builds) against the
+    oracle on the toy with maps in the solenoid map: identical surface
+    sequences and the FP64 / mixed contracts on the parameters."""
     from tests.test_engine_host import _rel, _same_sequences
     toy = da.Detector.toy(4, 7, use_material="maps")
     fp = da.generate_tracks("toy", 300, 47)
     ref = oracle_host.propagate(toy, solenoid, CFG, fp)
-    got = engine_host.propagate(toy, solenoid, CFG, fp, mixed=mixed)
+    got = engine_host.propagate(toy, solenoid, CFG, fp, mixed=mixed, plain=plain)
     _same_sequences(got, ref)
     assert _rel(got["bound_params"], ref["bound_params"]) < (1e-4 if mixed else 1e-9)
     # the maps are crossed: the final momenta differ from the toy without material
