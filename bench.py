@@ -233,7 +233,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--tracks", type=int, default=None)
-    ap.add_argument("--workload", choices=("toy_solenoid", "toy_const", "loopers", "adjoint", "wire_chamber"),
+    ap.add_argument("--workload", choices=("toy_solenoid", "toy_const", "loopers", "adjoint", "wire_chamber",
+                                               "toy_maps"),
                     default="toy_solenoid")
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="tracks of the CPU oracle leg (default: the whole batch)")
@@ -274,7 +275,8 @@ def main() -> int:
     from detray_amd.schedule import CELL_KEYS, autotune, sort_order
 
     dev = torch.device("cuda:0")
-    det = da.Detector.toy(4, 7)
+    maps = args.workload == "toy_maps"
+    det = da.Detector.toy(4, 7, use_material="maps" if maps else True)
     loopers = args.workload == "loopers"
     const = args.workload == "toy_const"
     field = da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA) if const else da.MagneticField.solenoid()
@@ -425,7 +427,7 @@ def main() -> int:
         tf = flops_launch / measured_s / 1e12
         tf64 = flops_launch / (kern64_ms * 1e-3) / 1e12
         tf_mixed = flops_launch / kernel_s_per_launch / 1e12
-        meas = measured_counters("propagate_gather_kernel<mixed,2>") if not const and not loopers else None
+        meas = measured_counters("propagate_gather_kernel<mixed,2>") if not (const or loopers or maps) else None
         valu = {"bound": "valu", "achieved": tf, "peak": VALU_PEAK_FP64, "unit": "TFLOP/s (fp64-equivalent)",
                 "frac": tf / VALU_PEAK_FP64, "frac_of_fp32_peak": tf / VALU_PEAK_FP32,
                 "flops_per_track": flops_track,
@@ -442,6 +444,9 @@ def main() -> int:
                     "log-uniform, batches of 262144 replayed from HIP graphs")
     elif const:
         workload = "toy detector 4brl+7edc, constant Bz 2 T, 100k muons |eta|<2.5 pT 1-100 GeV log-uniform"
+    elif maps:
+        workload = ("toy detector 4brl+7edc with material maps on every portal and the beampipe "
+                    "(use_material_maps), solenoid 5cm fp32 grid, 1M muons |eta|<2.5 pT 1-100 GeV log-uniform")
     else:
         workload = ("toy detector 4brl+7edc, solenoid 5cm fp32 grid, 1M muons |eta|<2.5 "
                     "pT 1-100 GeV log-uniform")
@@ -449,7 +454,7 @@ def main() -> int:
              "lane": "propagate_grid_kernel", "gather": "propagate_gather_kernel"}
     kernel = kname["persistent"] if launch.schedule == "persistent" else kname[launch.navigation]
     traffic = None
-    if not const and not loopers:
+    if not (const or loopers or maps):
         meas = measured_counters("propagate_gather_kernel<mixed,2>")
         if meas and launch.navigation == "gather" and launch.waves_per_simd == 2:
             traffic = meas.get("hbm_bytes_per_launch")
