@@ -84,9 +84,23 @@ def telescope_workload(kind: str, n: int, seed: int = 7):
     return det, b0, np.full(n, 7, dtype=np.int32), sig
 
 
+def maps_workload(n: int, seed: int = 11):
+    """The toy with material maps on every portal and the beampipe
+    (material on ring portals takes the polar frame's event adjoint):
+    bound starts on the beampipe."""
+    det = da.Detector.toy(4, 7, use_material="maps")
+    b0, start = adj.beampipe_starts(n, seed=seed)
+    r = det.table("masks")[det.table("surfaces")[adj.BEAMPIPE_SURFACE].mask].v[0]
+    sig = np.stack([np.full(n, 0.010 / r), np.full(n, 0.050), np.full(n, 1e-3), np.full(n, 1e-3),
+                    0.01 * np.abs(b0[4]), np.zeros(n)])
+    return det, b0, start.astype(np.int32), sig
+
+
 def workload(kind: str, n: int):
     if kind == "chamber":
         return chamber_workload(n)
+    if kind == "maps":
+        return maps_workload(n)
     return telescope_workload(kind, n)
 
 
@@ -107,10 +121,11 @@ FIELDS = {
     "chamber": lambda: da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA),
     "annulus": lambda: da.MagneticField.constant(0.0, 1.0 * da.TESLA, 0.0),
     "cylinder": lambda: da.MagneticField.constant(0.0, 1.0 * da.TESLA, 0.0),
+    "maps": lambda: da.MagneticField.constant(0.0, 0.0, 2.0 * da.TESLA),
 }
 
 
-@pytest.fixture(scope="module", params=["chamber", "annulus", "cylinder"])
+@pytest.fixture(scope="module", params=["chamber", "annulus", "cylinder", "maps"])
 def case(request, solenoid):
     kind = request.param
     det, b0, start, sig = workload(kind, 96)

@@ -26,7 +26,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=["chamber", "annulus", "cylinder"])
+@pytest.fixture(scope="module", params=["chamber", "annulus", "cylinder", "maps"])
 def case(request, solenoid):
     kind = request.param
     det, b0, start, sig = workload(kind, 256)
