@@ -337,3 +337,29 @@ def test_json_map_errors(tmp_path, case, code):
     rc, msg = read_error(files["geometry"], files["surface_grids"], material, bad)
     assert rc == code, (rc, msg)
     assert "material map" in msg or "axis" in msg
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_direct_navigator_replays_the_toy_with_maps(solenoid, mixed):
+    """The direct navigator (host build of the batched engine) replays the
+    caching navigator's sequences on the toy with maps, applying the same
+    map bins: identical sequences and momenta at the last surface within
+    the reference's direct-navigator tolerance (2e-3,
+    propagation_direct_navigator.cpp:180-185); the oracle's own replay
+    agrees with it."""
+    toy = da.Detector.toy(4, 7, use_material="maps")
+    fp = da.generate_tracks("toy", 200, 49)
+    ref = oracle_host.propagate(toy, solenoid, CFG, fp)
+    got = engine_host.propagate_direct(toy, solenoid, CFG, fp, ref["surfaces"], ref["n_surfaces"], mixed=mixed)
+    odir = oracle_host.propagate_direct(toy, solenoid, CFG, fp, ref["surfaces"], ref["n_surfaces"])
+    np.testing.assert_array_equal(got["n_surfaces"], ref["n_surfaces"])
+    has = ref["n_surfaces"] > 0
+    for i in np.nonzero(has)[0]:
+        k = ref["n_surfaces"][i]
+        np.testing.assert_array_equal(got["surfaces"][:k, i], ref["surfaces"][:k, i])
+    p = lambda b: np.abs(1.0 / b[4])
+    assert np.max(np.abs(p(got["bound_params"][:, has]) - p(ref["bound_params"][:, has])) /
+                  p(ref["bound_params"][:, has])) < 2e-3
+    tol = 1e-4 if mixed else 1e-9
+    assert np.max(np.abs(p(got["bound_params"][:, has]) - p(odir["bound_params"][:, has])) /
+                  p(odir["bound_params"][:, has])) < tol
