@@ -58,3 +58,32 @@ def test_toy_with_maps_matches_the_oracle(toy_maps, solenoid, launch):
     ref = oracle_host.propagate(toy_maps, solenoid, CFG, fp)
     got = run_launch(toy_maps, solenoid, CFG, fp, LAUNCHES[launch])
     check(got, ref, "mixed" if launch.startswith("mixed") else "fp64")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_direct_navigator_on_the_toy_with_maps(toy_maps, solenoid, precision):
+    """The device direct navigator replays the oracle's sequences on the toy
+    with maps and matches the oracle's replay at the batch contracts."""
+    from tests.test_gpu_batched import bound_error_scale
+    from tests.test_gpu_direct import run_direct
+    fp = da.generate_tracks("toy", 4096, 50)
+    rec = oracle_host.propagate(toy_maps, solenoid, CFG, fp)
+    ref = oracle_host.propagate_direct(toy_maps, solenoid, CFG, fp, rec["surfaces"], rec["n_surfaces"])
+    got = run_direct(toy_maps, solenoid, CFG, fp, rec["surfaces"], rec["n_surfaces"], precision)
+    np.testing.assert_array_equal(got["status"], ref["status"])
+    np.testing.assert_array_equal(got["surfaces"], ref["surfaces"])
+    err = np.abs(got["bound_params"] - ref["bound_params"]) / bound_error_scale(ref)
+    assert err.max() <= (1e-4 if precision == "mixed" else 1e-9), err.max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pause_and_resume_on_the_toy_with_maps(toy_maps, solenoid, precision):
+    """Several calls with a step budget of 4 equal one call bit for bit."""
+    from tests.test_gpu_resumable import KEYS, host, resumable
+    tracks = torch.from_numpy(da.generate_tracks("toy", 4096, 51)).cuda()
+    launch = da.LaunchConfig(precision, "grid", 64, 2, 64, None, 1, "gather", 64)
+    ref = host(da.Propagator(CFG, launch).propagate(toy_maps, solenoid, tracks))
+    got, _, first = resumable(toy_maps, solenoid, CFG, tracks, launch, 4)
+    assert first > 0
+    for k in KEYS:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
